@@ -1,0 +1,219 @@
+/* paces_hip.h -- C ABI of libpaces_hip.so, the MI355X (gfx950) engine behind paces_amd.
+ *
+ * Every entry point:
+ *   - returns an int status (PACES_OK == 0, negative PACES_E_* on error; paces_strerror()),
+ *     and never throws across the ABI;
+ *   - takes device pointers owned by the caller, element counts as int64 and a hipStream_t
+ *     passed as void* (0 = the null stream);
+ *   - enqueues stream-ordered work and does not synchronise the host;
+ *   - allocates no device memory: temporaries live in a caller workspace whose size the
+ *     matching *_workspace_bytes() function returns.
+ * Rows are packed Fock states: W uint64 words per row, row-major, field 0 (exciton position) in
+ * the most significant bits of word 0. Build word k equals the reference's
+ * (w32[2k] << 32) | w32[2k+1], so lexicographic row order is the reference's.
+ * Complex128 vectors are interleaved (re, im) doubles.
+ */
+#ifndef PACES_HIP_H
+#define PACES_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PACES_OK 0
+#define PACES_E_INVALID -1     /* bad argument (size, null pointer, out-of-range value) */
+#define PACES_E_UNSUPPORTED -2 /* row width or term count beyond the compiled limits */
+#define PACES_E_HIP -3         /* a HIP launch or runtime call failed */
+#define PACES_E_WORKSPACE -4   /* workspace smaller than *_workspace_bytes() */
+
+#define PACES_MAX_ROW_WORDS 16
+#define PACES_MAX_RUNS 32
+#define PACES_MAX_TERMS 4
+#define PACES_MAX_SITES 512
+
+#define PACES_TERM_HOP 0
+#define PACES_TERM_VIB 1
+
+const char* paces_strerror(int code);
+int paces_version(void);
+
+/* Packed-row layout: per-field bit offset from the MSB of word 0, width and dimension
+ * (replaces HamiltonianFramework's bitwidths/totwordwidth, paces/core/hamiltonian.py:134-160). */
+typedef struct {
+  int32_t W;               /* uint64 words per row, 1..PACES_MAX_ROW_WORDS */
+  int32_t nfields;         /* 1 + number of sites */
+  const int32_t* d_off;    /* device [nfields] */
+  const int32_t* d_width;  /* device [nfields] */
+  const int32_t* d_maxdim; /* device [nfields] */
+} paces_layout_t;
+
+/* One off-diagonal Holstein term, in use_terms order (paces/models/holstein.py:151-277). */
+typedef struct {
+  int32_t kind;  /* PACES_TERM_HOP or PACES_TERM_VIB */
+  int32_t order; /* hopping base order (1: hopping, 2: hopping2) */
+  double value;  /* J (hopping) or g (vib_coupling); real (paces/core/time_evolution.py:190-196) */
+} paces_term_t;
+
+/* Holstein lattice and terms (paces/models/holstein.py:24-66, 280-287). */
+typedef struct {
+  int32_t nchain, lattice_l, geo_dim, periodic;
+  int32_t nterms;
+  paces_term_t terms[PACES_MAX_TERMS];
+  double eps_sys, hbar_omega, delta_eps;
+  const double* d_sqrt_table; /* device: sqrt(m) for m = 0..sqrt_table_len-1, correctly rounded */
+  int32_t sqrt_table_len;
+} paces_holstein_t;
+
+/* ---- scans (prefix sums behind every mask compaction) ----------------------------------------- */
+size_t paces_scan_workspace_bytes(int64_t n);
+/* out[0..n] = exclusive prefix sums of in[0..n-1]; out[n] = total. */
+int paces_exclusive_scan_u8(const uint8_t* in, int64_t n, int32_t* out, void* ws, size_t ws_bytes,
+                            void* stream);
+int paces_exclusive_scan_i32(const int32_t* in, int64_t n, int32_t* out, void* ws,
+                             size_t ws_bytes, void* stream);
+
+/* ---- A2-A5: pack / field read / field add --------------------------------------------------------
+ * paces_pack_rows replaces compress_ind_arr (paces/core/hamiltonian.py:218-248);
+ * paces_get_field replaces get_n_at_site (hamiltonian.py:291-318) and, with field == -1,
+ * get_phonon_at_exc (paces/models/_kernels/_holstein_kernels.py:153-179);
+ * paces_add_field replaces add_n_at_site (hamiltonian.py:321-360) and, with field == -1,
+ * add/rem_phonon_at_exc (_holstein_kernels.py:96-149). */
+int paces_pack_rows(const int32_t* raw, int64_t n, const paces_layout_t* lay, uint64_t* rows,
+                    void* stream);
+int paces_get_field(const uint64_t* rows, int64_t n, const paces_layout_t* lay, int32_t field,
+                    int32_t* out, void* stream);
+int paces_add_field(uint64_t* rows, int64_t n, const paces_layout_t* lay, int32_t field,
+                    int32_t delta, void* stream);
+
+/* ---- B1: lexicographic lower_bound (searchsorted_full_binary, paces/aux/cupy_search.py:78-253)
+ * out_idx[i] in [0, M]; out_hit (optional) = row equality at out_idx; miss_count (optional,
+ * device int32) is incremented once per query not found -- the host-visible replacement of the
+ * device assert of full_binary_noesc (cupy_search.py:205-228). q_sorted != 0 promises ascending
+ * queries (every call on the step), which lets a wavefront narrow its search range. */
+int paces_lower_bound(const uint64_t* book, int64_t M, const uint64_t* q, int64_t Q, int32_t W,
+                      int32_t q_sorted, int32_t* out_idx, uint8_t* out_hit, int32_t* miss_count,
+                      void* stream);
+
+/* ---- A7: union of K sorted unique runs (cupy_unique, paces/aux/helpers.py:63-75, applied to
+ * the concatenation built in hamiltonian.py:434-452 and 563-577). out_rows needs capacity for
+ * sum(len) rows; *out_count (device) receives the number of unique rows; out_from_run0[p] = 1 when
+ * output row p came from run 0; cand_pos (optional, sum(len) entries) receives the output index
+ * of every input row, duplicates included. */
+typedef struct {
+  const uint64_t* ptr[PACES_MAX_RUNS];
+  int64_t len[PACES_MAX_RUNS];
+  int32_t K;
+} paces_runs_t;
+size_t paces_merge_unique_workspace_bytes(int64_t total, int32_t K);
+int paces_merge_unique(const paces_runs_t* runs, int32_t W, uint64_t* out_rows,
+                       int32_t* out_count, uint8_t* out_from_run0, int32_t* cand_pos, void* ws,
+                       size_t ws_bytes, void* stream);
+
+/* ---- F1: top-k select (_determine_select_whoami, paces/core/time_evolution.py:833-910,
+ * shuffle_seed=None). Keeps the k rows of largest |psi|^2 = re*re + im*im; ties at the cut keep
+ * the highest indices (stable ascending argsort, last k). Writes min(k, R) rows in index order,
+ * which for a sorted basis is lexicographic order; out_index (optional) gets their indices. */
+size_t paces_select_workspace_bytes(int64_t R);
+int paces_select_topk(const double* psi, const uint64_t* rows, int64_t R, int32_t W, int64_t k,
+                      uint64_t* out_rows, int32_t* out_index, void* ws, size_t ws_bytes,
+                      void* stream);
+
+/* ---- C1/C2/A8: Holstein candidate images (enlarge_basis_set with raw_map_to=True,
+ * paces/core/hamiltonian.py:550-577; paces/models/holstein.py:151-277). Each image run of a
+ * sorted unique basis is itself sorted and unique; the runs are written back to back.
+ * Two phases: _count fills the workspace and run_offsets (device, nruns+1, [nruns] = total);
+ * the caller reads the total, sizes out_images, then calls _emit with the same workspace. */
+int paces_holstein_num_runs(const paces_holstein_t* h);
+size_t paces_holstein_images_workspace_bytes(int64_t n, const paces_holstein_t* h);
+int paces_holstein_images_count(const uint64_t* basis, int64_t n, const paces_layout_t* lay,
+                                const paces_holstein_t* h, int32_t* run_offsets, void* ws,
+                                size_t ws_bytes, void* stream);
+int paces_holstein_images_emit(const uint64_t* basis, int64_t n, const paces_layout_t* lay,
+                               const paces_holstein_t* h, uint64_t* out_images, void* ws,
+                               size_t ws_bytes, void* stream);
+
+/* ---- D1/D2: per-term canonical CSR of the Holstein H on the new basis (generate_mel +
+ * _add_hc_and_generate_coo + coo_matrix().tocsr(): hamiltonian.py:383-547,
+ * time_evolution.py:365-372). An entry (i, j) of term t exists when rows i, j are linked by t and
+ * at least one of them is in the pre-expansion basis (in_basis[i] != 0). Two phases: _count
+ * writes nnz per term to term_nnz (device, nterms entries); _fill writes indptr (R+1 each),
+ * indices and data of every term. Columns ascend within a row. */
+size_t paces_holstein_csr_workspace_bytes(int64_t R, const paces_holstein_t* h);
+int paces_holstein_csr_count(const uint64_t* rows, int64_t R, const uint8_t* in_basis,
+                             const paces_layout_t* lay, const paces_holstein_t* h,
+                             int32_t* term_nnz, void* ws, size_t ws_bytes, void* stream);
+int paces_holstein_csr_fill(const uint64_t* rows, int64_t R, const paces_layout_t* lay,
+                            const paces_holstein_t* h, int32_t* const* indptr,
+                            int32_t* const* indices, double* const* data, void* ws,
+                            size_t ws_bytes, void* stream);
+
+/* ---- A6: diagonal eps + hbar_omega*sum(n) (+ delta_eps*pos) (generate_mel_diag,
+ * paces/models/holstein.py:280-287; sum_occ_pte_kernel, _holstein_kernels.py:183-210). */
+int paces_holstein_diag(const uint64_t* rows, int64_t R, const paces_layout_t* lay,
+                        const paces_holstein_t* h, double* diag, void* stream);
+
+/* ---- H1: observables n_exc[N], n_pho[N], sum |psi|^2 (calculate_n_exc / calculate_n_pho,
+ * paces/models/holstein.py:311-322; calc_bath_n_b_kernel, _holstein_kernels.py:215-255).
+ * out has 2N+1 doubles. Deterministic (fixed reduction order). */
+size_t paces_holstein_observables_workspace_bytes(int64_t R, const paces_holstein_t* h);
+int paces_holstein_observables(const uint64_t* rows, const double* psi, int64_t R,
+                               const paces_layout_t* lay, const paces_holstein_t* h, double* out,
+                               void* ws, size_t ws_bytes, void* stream);
+
+/* ---- G1: reassignment (paces/core/time_evolution.py:781-797): new_psi[i] = old_psi[j] where
+ * old_rows[j] == new_rows[i], else 0. */
+int paces_reassign(const uint64_t* old_rows, int64_t M, const double* old_psi,
+                   const uint64_t* new_rows, int64_t R, int32_t W, double* new_psi, void* stream);
+
+/* ---- E1/E2: H.x and the Taylor series (total_ham / _simple_taylor,
+ * paces/core/time_evolution.py:382-400, 482-512). FP64 parity order: per row
+ * y = d*x, then each term's entries summed left to right from zero and added, no FMA. */
+typedef struct {
+  int32_t nterms;
+  const int32_t* indptr[PACES_MAX_TERMS];
+  const int32_t* indices[PACES_MAX_TERMS];
+  const double* data[PACES_MAX_TERMS];
+} paces_csr_set_t;
+
+int paces_spmv(int64_t R, const double* diag, const paces_csr_set_t* H, const double* x,
+               double* y, void* stream);
+
+/* Taylor state (device, 64 bytes): see paces_taylor_state_t. */
+typedef struct {
+  unsigned long long max_term2; /* bit pattern of max |term|^2 of the current term */
+  unsigned long long max_res2;  /* bit pattern of max |res|^2 after adding it */
+  double c1;                    /* inf-norm of the previous term */
+  double c1_plus_c2;
+  double ratio;
+  int32_t done;
+  int32_t j_ind;
+  int32_t pad[4];
+} paces_taylor_state_t;
+
+/* res = v; state.c1 = ||v||_inf; state.done = 0. */
+int paces_taylor_begin(int64_t R, const double* v, double* res, paces_taylor_state_t* state,
+                       void* stream);
+/* One series term j: x_out = (-i*c) * (H x_in); res += x_out; then the convergence test
+ * (c1 + c2) / ||res||_inf <= tol. No-op once state.done != 0. */
+int paces_taylor_term(int64_t R, const double* diag, const paces_csr_set_t* H, const double* x_in,
+                      double* x_out, double* res, double c, int32_t j, double tol,
+                      paces_taylor_state_t* state, void* stream);
+
+/* ---- deterministic vector reductions (norms, vdot) and Lanczos helpers (E3,
+ * paces/core/time_evolution.py:515-549). out is a device double[2] (re, im) or double[1]. */
+size_t paces_reduce_workspace_bytes(int64_t R);
+int paces_norm2(const double* x, int64_t R, double* out, void* ws, size_t ws_bytes, void* stream);
+int paces_vdot(const double* x, const double* y, int64_t R, double* out, void* ws,
+               size_t ws_bytes, void* stream);
+/* y = a*x + b*y with complex scalars given on the host; with b == 0, y is only written */
+int paces_axpby(int64_t R, double ar, double ai, const double* x, double br, double bi, double* y,
+                void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PACES_HIP_H */

@@ -1,0 +1,164 @@
+"""ctypes boundary to libpaces_hip.so (declared in include/paces_hip.h).
+
+Device buffers are torch tensors on the current ROCm device; their data pointers and the current
+stream handle cross the boundary, nothing else. The library must be built (``python -m
+paces_amd.build``); there is no fallback: a missing library raises at import of the engine.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpaces_hip.so")
+
+MAX_RUNS = 32
+MAX_TERMS = 4
+MAX_ROW_WORDS = 16
+MAX_SITES = 512
+TERM_HOP = 0
+TERM_VIB = 1
+
+_lib = None
+
+
+class PacesError(RuntimeError):
+    pass
+
+
+class Layout(C.Structure):
+    _fields_ = [("W", C.c_int32), ("nfields", C.c_int32), ("d_off", C.c_void_p),
+                ("d_width", C.c_void_p), ("d_maxdim", C.c_void_p)]
+
+
+class Term(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("order", C.c_int32), ("value", C.c_double)]
+
+
+class Holstein(C.Structure):
+    _fields_ = [("nchain", C.c_int32), ("lattice_l", C.c_int32), ("geo_dim", C.c_int32),
+                ("periodic", C.c_int32), ("nterms", C.c_int32), ("terms", Term * MAX_TERMS),
+                ("eps_sys", C.c_double), ("hbar_omega", C.c_double), ("delta_eps", C.c_double),
+                ("d_sqrt_table", C.c_void_p), ("sqrt_table_len", C.c_int32)]
+
+
+class Runs(C.Structure):
+    _fields_ = [("ptr", C.c_void_p * MAX_RUNS), ("len", C.c_int64 * MAX_RUNS), ("K", C.c_int32)]
+
+
+class CsrSet(C.Structure):
+    _fields_ = [("nterms", C.c_int32), ("indptr", C.c_void_p * MAX_TERMS),
+                ("indices", C.c_void_p * MAX_TERMS), ("data", C.c_void_p * MAX_TERMS)]
+
+
+class TaylorState(C.Structure):
+    _fields_ = [("max_term2", C.c_uint64), ("max_res2", C.c_uint64), ("c1", C.c_double),
+                ("c1_plus_c2", C.c_double), ("ratio", C.c_double), ("done", C.c_int32),
+                ("j_ind", C.c_int32), ("pad", C.c_int32 * 4)]
+
+
+_P, _I64, _I32, _D, _SZ = C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_size_t
+
+_SIGNATURES = {
+    "paces_strerror": (C.c_char_p, [C.c_int]),
+    "paces_version": (C.c_int, []),
+    "paces_scan_workspace_bytes": (_SZ, [_I64]),
+    "paces_exclusive_scan_u8": (C.c_int, [_P, _I64, _P, _P, _SZ, _P]),
+    "paces_exclusive_scan_i32": (C.c_int, [_P, _I64, _P, _P, _SZ, _P]),
+    "paces_pack_rows": (C.c_int, [_P, _I64, C.POINTER(Layout), _P, _P]),
+    "paces_get_field": (C.c_int, [_P, _I64, C.POINTER(Layout), _I32, _P, _P]),
+    "paces_add_field": (C.c_int, [_P, _I64, C.POINTER(Layout), _I32, _I32, _P]),
+    "paces_lower_bound": (C.c_int, [_P, _I64, _P, _I64, _I32, _I32, _P, _P, _P, _P]),
+    "paces_merge_unique_workspace_bytes": (_SZ, [_I64, _I32]),
+    "paces_merge_unique": (C.c_int, [C.POINTER(Runs), _I32, _P, _P, _P, _P, _P, _SZ, _P]),
+    "paces_select_workspace_bytes": (_SZ, [_I64]),
+    "paces_select_topk": (C.c_int, [_P, _P, _I64, _I32, _I64, _P, _P, _P, _SZ, _P]),
+    "paces_holstein_num_runs": (C.c_int, [C.POINTER(Holstein)]),
+    "paces_holstein_images_workspace_bytes": (_SZ, [_I64, C.POINTER(Holstein)]),
+    "paces_holstein_images_count": (C.c_int, [_P, _I64, C.POINTER(Layout), C.POINTER(Holstein),
+                                              _P, _P, _SZ, _P]),
+    "paces_holstein_images_emit": (C.c_int, [_P, _I64, C.POINTER(Layout), C.POINTER(Holstein),
+                                             _P, _P, _SZ, _P]),
+    "paces_holstein_csr_workspace_bytes": (_SZ, [_I64, C.POINTER(Holstein)]),
+    "paces_holstein_csr_count": (C.c_int, [_P, _I64, _P, C.POINTER(Layout), C.POINTER(Holstein),
+                                           _P, _P, _SZ, _P]),
+    "paces_holstein_csr_fill": (C.c_int, [_P, _I64, C.POINTER(Layout), C.POINTER(Holstein),
+                                          C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), _P, _SZ,
+                                          _P]),
+    "paces_holstein_diag": (C.c_int, [_P, _I64, C.POINTER(Layout), C.POINTER(Holstein), _P, _P]),
+    "paces_holstein_observables_workspace_bytes": (_SZ, [_I64, C.POINTER(Holstein)]),
+    "paces_holstein_observables": (C.c_int, [_P, _P, _I64, C.POINTER(Layout),
+                                             C.POINTER(Holstein), _P, _P, _SZ, _P]),
+    "paces_reassign": (C.c_int, [_P, _I64, _P, _P, _I64, _I32, _P, _P]),
+    "paces_spmv": (C.c_int, [_I64, _P, C.POINTER(CsrSet), _P, _P, _P]),
+    "paces_taylor_begin": (C.c_int, [_I64, _P, _P, _P, _P]),
+    "paces_taylor_term": (C.c_int, [_I64, _P, C.POINTER(CsrSet), _P, _P, _P, _D, _I32, _D, _P,
+                                    _P]),
+    "paces_reduce_workspace_bytes": (_SZ, [_I64]),
+    "paces_norm2": (C.c_int, [_P, _I64, _P, _P, _SZ, _P]),
+    "paces_vdot": (C.c_int, [_P, _P, _I64, _P, _P, _SZ, _P]),
+    "paces_axpby": (C.c_int, [_I64, _D, _D, _P, _D, _D, _P, _P]),
+}
+
+
+def declared_symbols():
+    return sorted(_SIGNATURES)
+
+
+def load(path: str = LIB_PATH):
+    """Load the library (once) and bind every declared entry point."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise PacesError(f"{path} is missing: build it with `python -m paces_amd.build`")
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = load().paces_strerror(rc).decode()
+        raise PacesError(f"{what}: {msg} (status {rc})")
+
+
+def ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Workspace:
+    """One growing device byte buffer reused by consecutive (stream-ordered) library calls."""
+
+    def __init__(self, device):
+        self.device = device
+        self.buf = torch.empty(0, dtype=torch.uint8, device=device)
+
+    def get(self, nbytes: int):
+        nbytes = max(int(nbytes), 256)
+        if self.buf.numel() < nbytes:
+            self.buf = torch.empty(int(nbytes * 1.25) + 4096, dtype=torch.uint8,
+                                   device=self.device)
+        return C.c_void_p(self.buf.data_ptr()), C.c_size_t(self.buf.numel())
+
+
+def rows_to_numpy(t):
+    """Device int64 rows (bit patterns of uint64 words) -> host uint64 ndarray."""
+    return t.detach().cpu().numpy().view(np.uint64)
+
+
+def rows_from_numpy(a, device):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.uint64))
+    return torch.from_numpy(a.view(np.int64).copy()).to(device)

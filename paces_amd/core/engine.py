@@ -1,0 +1,199 @@
+"""Device primitives of the step, each a thin call into libpaces_hip.so.
+
+Rows are (n, W) torch.int64 tensors holding uint64 bit patterns; complex vectors are
+torch.complex128 tensors. Every function runs on the current device and stream.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import torch
+
+from .. import _hip
+from .._hip import check, ptr, stream
+
+lib = _hip.load()
+
+
+class CsrMatrix:
+    """Canonical CSR of one Hamiltonian term (int32 indptr/indices, float64 data)."""
+
+    def __init__(self, indptr, indices, data, shape):
+        self.indptr = indptr
+        self.indices = indices
+        self.data = data
+        self.shape = shape
+
+    @property
+    def nnz(self):
+        return int(self.indices.numel())
+
+    def dot(self, x):
+        """M.x for a complex128 vector (the cuSPARSE product the reference calls)."""
+        zero = torch.zeros(self.shape[0], dtype=torch.float64, device=x.device)
+        return spmv(zero, [self], x)
+
+    def to_scipy(self):
+        import scipy.sparse
+        return scipy.sparse.csr_matrix((self.data.cpu().numpy(), self.indices.cpu().numpy(),
+                                        self.indptr.cpu().numpy()), shape=self.shape)
+
+
+def csr_set(mats):
+    s = _hip.CsrSet()
+    s.nterms = len(mats)
+    if len(mats) > _hip.MAX_TERMS:
+        raise ValueError("too many Hamiltonian terms")
+    for t, m in enumerate(mats):
+        s.indptr[t] = m.indptr.data_ptr()
+        s.indices[t] = m.indices.data_ptr() if m.nnz else m.indptr.data_ptr()
+        s.data[t] = m.data.data_ptr() if m.nnz else m.indptr.data_ptr()
+    return s
+
+
+def lower_bound(book, q, want_hit=True, count_misses=False):
+    """Lexicographic lower_bound of q's rows in the sorted book (int32 indices, uint8 hits)."""
+    M, Q = book.shape[0], q.shape[0]
+    W = q.shape[1]
+    if book.shape[1] != W:
+        raise ValueError("Incompatible array shapes.")
+    idx = torch.empty(Q, dtype=torch.int32, device=q.device)
+    hit = torch.empty(Q, dtype=torch.uint8, device=q.device) if want_hit else None
+    miss = torch.zeros(1, dtype=torch.int32, device=q.device) if count_misses else None
+    check(lib.paces_lower_bound(ptr(book) if M else None, M, ptr(q) if Q else None, Q, W, 1,
+                                ptr(idx) if Q else None, ptr(hit) if (hit is not None and Q) else None,
+                                ptr(miss), stream()), "lower_bound")
+    return idx, hit, miss
+
+
+def merge_unique(runs, W, ws, want_from0=False):
+    """Sorted union of sorted unique row runs; returns (rows, from_run0 flags or None)."""
+    if len(runs) > _hip.MAX_RUNS:
+        raise ValueError("too many runs")
+    R = _hip.Runs()
+    R.K = len(runs)
+    total = 0
+    dev = None
+    for k, (t, off, n) in enumerate(runs):
+        dev = t.device
+        R.ptr[k] = t.data_ptr() + off * 8 * W if n else 0
+        R.len[k] = n
+        total += n
+    out = torch.empty((max(total, 1), W), dtype=torch.int64, device=dev)
+    cnt = torch.empty(1, dtype=torch.int32, device=dev)
+    from0 = torch.empty(max(total, 1), dtype=torch.uint8, device=dev) if want_from0 else None
+    wsp, wsn = ws.get(lib.paces_merge_unique_workspace_bytes(total, R.K))
+    check(lib.paces_merge_unique(C.byref(R), W, ptr(out), ptr(cnt), ptr(from0), None, wsp, wsn,
+                                 stream()), "merge_unique")
+    n = int(cnt.item())
+    return out[:n], (from0[:n] if want_from0 else None)
+
+
+def select_topk(psi, rows, k, ws):
+    R, W = rows.shape
+    k = int(min(k, R))
+    out = torch.empty((k, W), dtype=torch.int64, device=rows.device)
+    if k == 0:
+        return out
+    wsp, wsn = ws.get(lib.paces_select_workspace_bytes(R))
+    check(lib.paces_select_topk(ptr(torch.view_as_real(psi)), ptr(rows), R, W, k, ptr(out), None,
+                                wsp, wsn, stream()), "select_topk")
+    return out
+
+
+def reassign(old_rows, old_psi, new_rows):
+    R, W = new_rows.shape
+    M = old_rows.shape[0]
+    out = torch.empty(R, dtype=torch.complex128, device=new_rows.device)
+    if R:
+        check(lib.paces_reassign(ptr(old_rows) if M else None, M,
+                                 ptr(torch.view_as_real(old_psi)) if M else None, ptr(new_rows), R,
+                                 W, ptr(torch.view_as_real(out)), stream()), "reassign")
+    return out
+
+
+def spmv(diag, mats, x, out=None):
+    R = x.shape[0]
+    y = torch.empty_like(x) if out is None else out
+    s = csr_set(mats)
+    if R:
+        check(lib.paces_spmv(R, ptr(diag), C.byref(s), ptr(torch.view_as_real(x)),
+                             ptr(torch.view_as_real(y)), stream()), "spmv")
+    return y
+
+
+class Reducer:
+    def __init__(self, device):
+        self.ws = _hip.Workspace(device)
+        self.out = torch.empty(2, dtype=torch.float64, device=device)
+
+    def norm2(self, x):
+        """sum |x_i|^2 (device, deterministic order) -> float."""
+        R = x.shape[0]
+        wsp, wsn = self.ws.get(lib.paces_reduce_workspace_bytes(R))
+        check(lib.paces_norm2(ptr(torch.view_as_real(x)) if R else None, R, ptr(self.out), wsp,
+                              wsn, stream()), "norm2")
+        return float(self.out[0].item())
+
+    def norm(self, x):
+        return math.sqrt(self.norm2(x))
+
+    def vdot(self, x, y):
+        R = x.shape[0]
+        wsp, wsn = self.ws.get(lib.paces_reduce_workspace_bytes(R))
+        check(lib.paces_vdot(ptr(torch.view_as_real(x)) if R else None,
+                             ptr(torch.view_as_real(y)) if R else None, R, ptr(self.out), wsp,
+                             wsn, stream()), "vdot")
+        h = self.out.tolist()
+        return complex(h[0], h[1])
+
+
+def axpby(a, x, b, y):
+    """y <- a*x + b*y (complex host scalars)."""
+    R = x.shape[0]
+    a, b = complex(a), complex(b)
+    if R:
+        check(lib.paces_axpby(R, a.real, a.imag, ptr(torch.view_as_real(x)), b.real, b.imag,
+                              ptr(torch.view_as_real(y)), stream()), "axpby")
+    return y
+
+
+def taylor_coeff(delta_t, j):
+    """Imaginary magnitude c of -1j*dt/(j+1) computed as the reference does in Python."""
+    return -((-1j * delta_t) / float(j + 1)).imag
+
+
+TAYLOR_TOL = 1.1102230246251565e-16  # 2**-53 (paces/core/time_evolution.py:484)
+
+
+def simple_taylor(diag, mats, v, delta_t, m_star=100, tol=TAYLOR_TOL, chunk=6):
+    """res = sum_j (-i dt H)^j v / j! until (|t_j| + |t_{j+1}|)_inf / |res|_inf <= tol
+    (paces/core/time_evolution.py:482-512). Returns (res, [converged, j_ind, c1+c2, ratio])."""
+    R = v.shape[0]
+    res = torch.empty_like(v)
+    bufs = (torch.empty_like(v), torch.empty_like(v))
+    state = torch.zeros(C.sizeof(_hip.TaylorState), dtype=torch.uint8, device=v.device)
+    s = csr_set(mats)
+    st = stream()
+    check(lib.paces_taylor_begin(R, ptr(torch.view_as_real(v)), ptr(torch.view_as_real(res)),
+                                 ptr(state), st), "taylor_begin")
+    x_in = v
+    j = 0
+    host = None
+    while j < m_star:
+        for _ in range(chunk):
+            if j >= m_star:
+                break
+            x_out = bufs[j % 2]
+            check(lib.paces_taylor_term(R, ptr(diag), C.byref(s), ptr(torch.view_as_real(x_in)),
+                                        ptr(torch.view_as_real(x_out)),
+                                        ptr(torch.view_as_real(res)), taylor_coeff(delta_t, j), j,
+                                        tol, ptr(state), st), "taylor_term")
+            x_in = x_out
+            j += 1
+        host = _hip.TaylorState.from_buffer_copy(bytes(state.cpu().numpy()))
+        if host.done:
+            break
+    return res, [bool(host.done), int(host.j_ind), float(host.c1_plus_c2), float(host.ratio)]

@@ -1,0 +1,184 @@
+// Shared device helpers for the paces HIP library (gfx950, wave64).
+//
+// Rows are packed Fock states: W uint64 words per row, row-major, field 0 (exciton position)
+// in the most significant bits of word 0 (big-endian bit concatenation, see
+// paces/core/hamiltonian.py:218-248 for the reference's uint32 form).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "paces_hip.h"
+
+typedef unsigned long long u64;
+typedef unsigned __int128 u128;
+
+#define PACES_MAX_W 16
+
+namespace paces {
+
+constexpr int kBlock = 256;
+
+// ---- status plumbing --------------------------------------------------------------------------
+inline int launch_status() {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? PACES_OK : PACES_E_HIP;
+}
+
+inline unsigned grid_for(int64_t n, int per_block) {
+  int64_t g = (n + per_block - 1) / per_block;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
+// ---- rows -------------------------------------------------------------------------------------
+template <int W>
+struct Row {
+  u64 w[W];
+};
+
+template <int W>
+__device__ __forceinline__ Row<W> load_row(const u64* __restrict__ base, int64_t i) {
+  Row<W> r;
+  const u64* p = base + i * W;
+#pragma unroll
+  for (int k = 0; k < W; ++k) r.w[k] = p[k];
+  return r;
+}
+
+template <int W>
+__device__ __forceinline__ void store_row(u64* __restrict__ base, int64_t i, const Row<W>& r) {
+  u64* p = base + i * W;
+#pragma unroll
+  for (int k = 0; k < W; ++k) p[k] = r.w[k];
+}
+
+// -1 if a < b, 0 if equal, +1 if a > b (lexicographic over words)
+template <int W>
+__device__ __forceinline__ int cmp_row(const Row<W>& a, const u64* __restrict__ b) {
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    u64 bk = b[k];
+    if (a.w[k] != bk) return a.w[k] < bk ? -1 : 1;
+  }
+  return 0;
+}
+
+template <int W>
+__device__ __forceinline__ bool eq_row(const Row<W>& a, const u64* __restrict__ b) {
+  bool e = true;
+#pragma unroll
+  for (int k = 0; k < W; ++k) e &= (a.w[k] == b[k]);
+  return e;
+}
+
+// lexicographic lower_bound of row q in sorted book[0, M): first index m with book[m] >= q
+template <int W>
+__device__ __forceinline__ int64_t lower_bound_row(const u64* __restrict__ book, int64_t lo,
+                                                   int64_t hi, const Row<W>& q) {
+  while (lo < hi) {
+    int64_t m = (lo + hi) >> 1;
+    if (cmp_row<W>(q, book + m * W) > 0)
+      lo = m + 1;
+    else
+      hi = m;
+  }
+  return lo;
+}
+
+// ---- bit fields ---------------------------------------------------------------------------------
+// A field at bit offset `off` (from the MSB of word 0) and width `b` (1..64).
+template <int W>
+__device__ __forceinline__ u64 get_field(const Row<W>& r, int off, int b) {
+  if (b == 0) return 0;
+  int k = off >> 6;
+  int o = off & 63;
+  u64 hi = 0, lo = 0;
+#pragma unroll
+  for (int kk = 0; kk < W; ++kk) {  // select without dynamic register indexing
+    hi = (kk == k) ? r.w[kk] : hi;
+    lo = (kk == k + 1) ? r.w[kk] : lo;
+  }
+  u128 c = ((u128)hi << 64) | (u128)lo;
+  c >>= (128 - o - b);
+  u64 mask = (b == 64) ? ~0ull : ((1ull << b) - 1ull);
+  return (u64)c & mask;
+}
+
+// add a signed delta to a field; the caller guarantees the result stays inside the field, so the
+// carry or borrow never leaves the two words the field touches.
+template <int W>
+__device__ __forceinline__ void add_field(Row<W>& r, int off, int b, int64_t delta) {
+  if (b == 0 || delta == 0) return;
+  int k = off >> 6;
+  int o = off & 63;
+  u64 hi = 0, lo = 0;
+#pragma unroll
+  for (int kk = 0; kk < W; ++kk) {
+    hi = (kk == k) ? r.w[kk] : hi;
+    lo = (kk == k + 1) ? r.w[kk] : lo;
+  }
+  u128 c = ((u128)hi << 64) | (u128)lo;
+  u128 unit = (u128)1 << (128 - o - b);
+  if (delta > 0)
+    c += unit * (u128)(u64)delta;
+  else
+    c -= unit * (u128)(u64)(-delta);
+#pragma unroll
+  for (int kk = 0; kk < W; ++kk) {
+    if (kk == k) r.w[kk] = (u64)(c >> 64);
+    if (kk == k + 1) r.w[kk] = (u64)c;
+  }
+}
+
+template <int W>
+__device__ __forceinline__ void set_field(Row<W>& r, int off, int b, u64 v) {
+  if (b == 0) return;
+  u64 cur = get_field<W>(r, off, b);
+  add_field<W>(r, off, b, (int64_t)v - (int64_t)cur);
+}
+
+// ---- wave / block reductions ----------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) v += __shfl_xor(v, s, 64);
+  return v;
+}
+
+__device__ __forceinline__ u64 wave_max_u64(u64 v) {
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) {
+    u64 o = __shfl_xor(v, s, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) v += __shfl_xor(v, s, 64);
+  return v;
+}
+
+}  // namespace paces
+
+// Dispatch a templated launcher over the supported row widths.
+#define PACES_DISPATCH_W(Wv, FN, ...)                 \
+  switch (Wv) {                                       \
+    case 1: return FN<1>(__VA_ARGS__);                \
+    case 2: return FN<2>(__VA_ARGS__);                \
+    case 3: return FN<3>(__VA_ARGS__);                \
+    case 4: return FN<4>(__VA_ARGS__);                \
+    case 5: return FN<5>(__VA_ARGS__);                \
+    case 6: return FN<6>(__VA_ARGS__);                \
+    case 7: return FN<7>(__VA_ARGS__);                \
+    case 8: return FN<8>(__VA_ARGS__);                \
+    case 9: return FN<9>(__VA_ARGS__);                \
+    case 10: return FN<10>(__VA_ARGS__);              \
+    case 11: return FN<11>(__VA_ARGS__);              \
+    case 12: return FN<12>(__VA_ARGS__);              \
+    case 13: return FN<13>(__VA_ARGS__);              \
+    case 14: return FN<14>(__VA_ARGS__);              \
+    case 15: return FN<15>(__VA_ARGS__);              \
+    case 16: return FN<16>(__VA_ARGS__);              \
+    default: return PACES_E_UNSUPPORTED;              \
+  }

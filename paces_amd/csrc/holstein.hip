@@ -1,0 +1,673 @@
+// Holstein-model kernels: candidate images (C1/C2/A8), per-term CSR (D1/D2), diagonal (A6) and
+// observables (H1). References: paces/models/holstein.py:151-322,
+// paces/models/_kernels/_holstein_kernels.py:96-255, paces/core/hamiltonian.py:363-547.
+#include "common.hpp"
+
+namespace paces {
+
+int scan_u8(const uint8_t* in, int64_t n, int32_t* out, void* ws, size_t ws_bytes, hipStream_t st);
+int scan_i32(const int32_t* in, int64_t n, int32_t* out, void* ws, size_t ws_bytes, hipStream_t st);
+size_t scan_ws_bytes(int64_t n);
+
+static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// ------------------------------------------------------------------------------------------------
+// Operation tables built on the host from paces_holstein_t
+// ------------------------------------------------------------------------------------------------
+
+constexpr int kMaxOps = 32;
+
+// An image op maps a row to one neighbour. Hop ops shift the exciton position inside one lattice
+// dimension: with mod = pos % span, target = pos + dir*oe when the move stays inside the span
+// (wrap == 0), or pos + dir*oe - dir*span when it crosses the periodic boundary (wrap == 1).
+// Vib ops add dir to the phonon count at the exciton site.
+struct OpTable {
+  int n;
+  int8_t kind[kMaxOps];
+  int8_t dir[kMaxOps];
+  int8_t wrap[kMaxOps];
+  int8_t term[kMaxOps];
+  int32_t oe[kMaxOps];
+  int32_t span[kMaxOps];
+  double value[kMaxOps];
+};
+
+// Image runs (enlarge_basis_set / generate_mel_*(raw_map_to=True)): OBC keeps the in-span moves
+// (holstein.py:185-199); periodic adds the wrapped moves as separate runs so every run stays
+// sorted.
+static int build_image_ops(const paces_holstein_t* h, OpTable* T) {
+  T->n = 0;
+  for (int t = 0; t < h->nterms; ++t) {
+    const paces_term_t& tm = h->terms[t];
+    if (tm.kind == PACES_TERM_HOP) {
+      long long Ld = 1;
+      for (int d = 0; d < h->geo_dim; ++d) {
+        long long oe = (long long)tm.order * Ld, span = Ld * h->lattice_l;
+        for (int wrap = 0; wrap <= (h->periodic ? 1 : 0); ++wrap) {
+          for (int dir = +1; dir >= -1; dir -= 2) {
+            if (T->n >= kMaxOps) return PACES_E_UNSUPPORTED;
+            int k = T->n++;
+            T->kind[k] = PACES_TERM_HOP;
+            T->dir[k] = (int8_t)dir;
+            T->wrap[k] = (int8_t)wrap;
+            T->term[k] = (int8_t)t;
+            T->oe[k] = (int32_t)oe;
+            T->span[k] = (int32_t)span;
+            T->value[k] = tm.value;
+          }
+        }
+        Ld *= h->lattice_l;
+      }
+    } else if (tm.kind == PACES_TERM_VIB) {
+      for (int dir = +1; dir >= -1; dir -= 2) {
+        if (T->n >= kMaxOps) return PACES_E_UNSUPPORTED;
+        int k = T->n++;
+        T->kind[k] = PACES_TERM_VIB;
+        T->dir[k] = (int8_t)dir;
+        T->wrap[k] = 0;
+        T->term[k] = (int8_t)t;
+        T->oe[k] = 0;
+        T->span[k] = 0;
+        T->value[k] = tm.value;
+      }
+    } else {
+      return PACES_E_INVALID;
+    }
+  }
+  return PACES_OK;
+}
+
+// Matrix-element neighbours (one per direction; the periodic wrap is folded in), grouped by term.
+static int build_nb_ops(const paces_holstein_t* h, OpTable* T) {
+  T->n = 0;
+  for (int t = 0; t < h->nterms; ++t) {
+    const paces_term_t& tm = h->terms[t];
+    if (tm.kind == PACES_TERM_HOP) {
+      long long Ld = 1;
+      for (int d = 0; d < h->geo_dim; ++d) {
+        for (int dir = +1; dir >= -1; dir -= 2) {
+          if (T->n >= kMaxOps) return PACES_E_UNSUPPORTED;
+          int k = T->n++;
+          T->kind[k] = PACES_TERM_HOP;
+          T->dir[k] = (int8_t)dir;
+          T->wrap[k] = (int8_t)(h->periodic ? 2 : 0);  // 2: either side, wrap when needed
+          T->term[k] = (int8_t)t;
+          T->oe[k] = (int32_t)(tm.order * Ld);
+          T->span[k] = (int32_t)(Ld * h->lattice_l);
+          T->value[k] = tm.value;
+        }
+        Ld *= h->lattice_l;
+      }
+    } else {
+      for (int dir = +1; dir >= -1; dir -= 2) {
+        if (T->n >= kMaxOps) return PACES_E_UNSUPPORTED;
+        int k = T->n++;
+        T->kind[k] = PACES_TERM_VIB;
+        T->dir[k] = (int8_t)dir;
+        T->wrap[k] = 0;
+        T->term[k] = (int8_t)t;
+        T->oe[k] = 0;
+        T->span[k] = 0;
+        T->value[k] = tm.value;
+      }
+    }
+  }
+  return PACES_OK;
+}
+
+struct LayoutArg {
+  const int32_t* off;
+  const int32_t* width;
+  const int32_t* maxdim;
+};
+
+// Apply op k to row r. Returns false when the image does not exist (boundary or ceiling).
+// `nup` receives the upper phonon count of a vib pair (for the matrix element).
+template <int W>
+__device__ __forceinline__ bool apply_op(const OpTable& T, int k, const LayoutArg& L,
+                                         Row<W>& r, int& nup) {
+  const int pb = L.width[0];
+  const int pos = (int)get_field<W>(r, 0, pb);
+  if (T.kind[k] == PACES_TERM_HOP) {
+    const int oe = T.oe[k], span = T.span[k], dir = T.dir[k];
+    const int mod = pos % span;
+    int npos;
+    bool inside = (dir > 0) ? (mod < span - oe) : (mod >= oe);
+    if (T.wrap[k] == 0) {
+      if (!inside) return false;
+      npos = pos + dir * oe;
+    } else if (T.wrap[k] == 1) {
+      if (inside) return false;
+      npos = pos + dir * oe - dir * span;
+    } else {
+      npos = inside ? pos + dir * oe : pos + dir * oe - dir * span;
+    }
+    add_field<W>(r, 0, pb, (int64_t)(npos - pos));
+    nup = 0;
+    return true;
+  } else {
+    const int f = pos + 1;
+    const int off = L.off[f], b = L.width[f];
+    const int n = (int)get_field<W>(r, off, b);
+    if (T.dir[k] > 0) {
+      if (n + 1 >= L.maxdim[f]) return false;
+      add_field<W>(r, off, b, 1);
+      nup = n + 1;
+    } else {
+      if (n <= 0) return false;
+      add_field<W>(r, off, b, -1);
+      nup = n;
+    }
+    return true;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Image runs: flags -> scan -> emit
+// ------------------------------------------------------------------------------------------------
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_img_flags(const u64* __restrict__ basis, int64_t n,
+                                                      OpTable T, LayoutArg L,
+                                                      uint8_t* __restrict__ flags) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Row<W> r0 = load_row<W>(basis, i);
+  for (int k = 0; k < T.n; ++k) {
+    Row<W> r = r0;
+    int nup;
+    flags[(int64_t)k * n + i] = apply_op<W>(T, k, L, r, nup) ? 1 : 0;
+  }
+}
+
+__global__ void k_img_offsets(const int32_t* __restrict__ G, int64_t n, int nops,
+                              int32_t* __restrict__ run_offsets) {
+  int k = threadIdx.x;
+  if (k <= nops) run_offsets[k] = G[(int64_t)k * n];
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_img_emit(const u64* __restrict__ basis, int64_t n,
+                                                     OpTable T, LayoutArg L,
+                                                     const uint8_t* __restrict__ flags,
+                                                     const int32_t* __restrict__ G,
+                                                     u64* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Row<W> r0 = load_row<W>(basis, i);
+  for (int k = 0; k < T.n; ++k) {
+    int64_t f = (int64_t)k * n + i;
+    if (!flags[f]) continue;
+    Row<W> r = r0;
+    int nup;
+    apply_op<W>(T, k, L, r, nup);
+    store_row<W>(out, G[f], r);
+  }
+}
+
+static size_t images_ws(int64_t n, int nops) {
+  int64_t m = (int64_t)nops * n;
+  return a256((size_t)m) + a256((size_t)(m + 1) * 4) + a256(scan_ws_bytes(m));
+}
+
+template <int W>
+static int run_img_count(const uint64_t* basis, int64_t n, const LayoutArg& L, const OpTable& T,
+                         int32_t* run_offsets, void* ws, size_t ws_bytes, hipStream_t st) {
+  int64_t m = (int64_t)T.n * n;
+  char* p = (char*)ws;
+  uint8_t* flags = (uint8_t*)p;
+  p += a256((size_t)m);
+  int32_t* G = (int32_t*)p;
+  p += a256((size_t)(m + 1) * 4);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_img_flags<W>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
+                       (const u64*)basis, n, T, L, flags);
+    int rc = launch_status();
+    if (rc) return rc;
+  }
+  int rc = scan_u8(flags, m, G, p, scan_ws_bytes(m), st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_img_offsets, dim3(1), dim3(64), 0, st, (const int32_t*)G, n, T.n,
+                     run_offsets);
+  return launch_status();
+}
+
+template <int W>
+static int run_img_emit(const uint64_t* basis, int64_t n, const LayoutArg& L, const OpTable& T,
+                        uint64_t* out, void* ws, hipStream_t st) {
+  if (n == 0) return PACES_OK;
+  int64_t m = (int64_t)T.n * n;
+  char* p = (char*)ws;
+  const uint8_t* flags = (const uint8_t*)p;
+  p += a256((size_t)m);
+  const int32_t* G = (const int32_t*)p;
+  hipLaunchKernelGGL(k_img_emit<W>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
+                     (const u64*)basis, n, T, L, flags, G, (u64*)out);
+  return launch_status();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Per-term CSR, built row by row: row i of term t holds every neighbour j of row i under t with
+// j in the new basis and i or j in the pre-expansion basis. This is the entry set the reference's
+// MelTriples + o_star masks + H.c. produce (hamiltonian.py:363-374, 495-547; holstein.py:201-206,
+// 266-277), each unordered pair once per direction.
+// ------------------------------------------------------------------------------------------------
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_csr_count(const u64* __restrict__ rows, int64_t R,
+                                                      const uint8_t* __restrict__ inB, OpTable T,
+                                                      LayoutArg L, int nterms,
+                                                      int32_t* __restrict__ nb,
+                                                      int32_t* __restrict__ cnt) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const Row<W> r0 = load_row<W>(rows, i);
+  const bool bi = inB[i] != 0;
+  int c[PACES_MAX_TERMS] = {0, 0, 0, 0};
+  for (int k = 0; k < T.n; ++k) {
+    Row<W> r = r0;
+    int nup;
+    int32_t j = -1;
+    if (apply_op<W>(T, k, L, r, nup)) {
+      int64_t lb = lower_bound_row<W>(rows, 0, R, r);
+      if (lb < R && eq_row<W>(r, rows + lb * W) && (bi || inB[lb])) j = (int32_t)lb;
+    }
+    nb[(int64_t)k * R + i] = j;
+    if (j >= 0) {
+#pragma unroll
+      for (int t = 0; t < PACES_MAX_TERMS; ++t) c[t] += (T.term[k] == t) ? 1 : 0;
+    }
+  }
+  for (int t = 0; t < nterms; ++t) cnt[(int64_t)t * R + i] = c[t];
+}
+
+__global__ void k_csr_nnz(const int32_t* __restrict__ G, int64_t R, int nterms,
+                          int32_t* __restrict__ nnz) {
+  int t = threadIdx.x;
+  if (t < nterms) nnz[t] = G[(int64_t)(t + 1) * R] - G[(int64_t)t * R];
+}
+
+struct CsrOut {
+  int32_t* indptr[PACES_MAX_TERMS];
+  int32_t* indices[PACES_MAX_TERMS];
+  double* data[PACES_MAX_TERMS];
+};
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_csr_fill(const u64* __restrict__ rows, int64_t R,
+                                                     OpTable T, LayoutArg L, int nterms,
+                                                     const double* __restrict__ sqrt_tab,
+                                                     const int32_t* __restrict__ nb,
+                                                     const int32_t* __restrict__ G, CsrOut out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const Row<W> r0 = load_row<W>(rows, i);
+  for (int t = 0; t < nterms; ++t) {
+    const int32_t base0 = G[(int64_t)t * R];
+    const int32_t base = G[(int64_t)t * R + i] - base0;
+    out.indptr[t][i] = base;
+    if (i == R - 1) out.indptr[t][R] = G[(int64_t)(t + 1) * R] - base0;
+    int32_t cj[8];
+    double cv[8];
+    int m = 0;
+    for (int k = 0; k < T.n; ++k) {
+      if (T.term[k] != t) continue;
+      int32_t j = nb[(int64_t)k * R + i];
+      if (j < 0) continue;
+      double v;
+      if (T.kind[k] == PACES_TERM_HOP) {
+        v = T.value[k];
+      } else {
+        Row<W> r = r0;
+        int nup = 0;
+        apply_op<W>(T, k, L, r, nup);
+        v = sqrt_tab[nup] * T.value[k];
+      }
+      // insertion into the ascending (column) list; at most 8 entries per term and row
+      int p = m;
+      while (p > 0 && cj[p - 1] > j) {
+        cj[p] = cj[p - 1];
+        cv[p] = cv[p - 1];
+        --p;
+      }
+      cj[p] = j;
+      cv[p] = v;
+      ++m;
+    }
+    for (int e = 0; e < m; ++e) {
+      out.indices[t][base + e] = cj[e];
+      out.data[t][base + e] = cv[e];
+    }
+  }
+}
+
+__global__ void k_csr_empty(CsrOut out, int nterms) {
+  int t = threadIdx.x;
+  if (t < nterms) out.indptr[t][0] = 0;
+}
+
+static size_t csr_ws(int64_t R, int nops, int nterms) {
+  return a256((size_t)nops * R * 4) + a256((size_t)nterms * R * 4) +
+         a256((size_t)(nterms * R + 1) * 4) + a256(scan_ws_bytes((int64_t)nterms * R));
+}
+
+template <int W>
+static int run_csr_count(const uint64_t* rows, int64_t R, const uint8_t* inB, const LayoutArg& L,
+                         const OpTable& T, int nterms, int32_t* nnz, void* ws, hipStream_t st) {
+  char* p = (char*)ws;
+  int32_t* nb = (int32_t*)p;
+  p += a256((size_t)T.n * R * 4);
+  int32_t* cnt = (int32_t*)p;
+  p += a256((size_t)nterms * R * 4);
+  int32_t* G = (int32_t*)p;
+  p += a256((size_t)(nterms * R + 1) * 4);
+  if (R > 0) {
+    hipLaunchKernelGGL(k_csr_count<W>, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
+                       (const u64*)rows, R, inB, T, L, nterms, nb, cnt);
+    int rc = launch_status();
+    if (rc) return rc;
+  }
+  int rc = scan_i32(cnt, (int64_t)nterms * R, G, p, scan_ws_bytes((int64_t)nterms * R), st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_csr_nnz, dim3(1), dim3(64), 0, st, (const int32_t*)G, R, nterms, nnz);
+  return launch_status();
+}
+
+template <int W>
+static int run_csr_fill(const uint64_t* rows, int64_t R, const LayoutArg& L, const OpTable& T,
+                        int nterms, const double* sqrt_tab, const CsrOut& out, void* ws,
+                        hipStream_t st) {
+  if (R == 0) {
+    hipLaunchKernelGGL(k_csr_empty, dim3(1), dim3(64), 0, st, out, nterms);
+    return launch_status();
+  }
+  char* p = (char*)ws;
+  const int32_t* nb = (const int32_t*)p;
+  p += a256((size_t)T.n * R * 4);
+  p += a256((size_t)nterms * R * 4);
+  const int32_t* G = (const int32_t*)p;
+  hipLaunchKernelGGL(k_csr_fill<W>, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
+                     (const u64*)rows, R, T, L, nterms, sqrt_tab, nb, G, out);
+  return launch_status();
+}
+
+// ------------------------------------------------------------------------------------------------
+// A6 diagonal: eps + hbar_omega * sum_i n_i (+ delta_eps * pos), holstein.py:280-287. The integer
+// sum is exact in FP64, so the result is the reference's rounding of the same expression.
+// ------------------------------------------------------------------------------------------------
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_diag(const u64* __restrict__ rows, int64_t R,
+                                                 LayoutArg L, int nsites, double eps, double hw,
+                                                 double de, double* __restrict__ diag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const Row<W> r = load_row<W>(rows, i);
+  long long s = 0;
+  for (int f = 1; f <= nsites; ++f) s += (long long)get_field<W>(r, L.off[f], L.width[f]);
+  double d = eps + hw * (double)s;
+  if (de != 0.0) d = d + de * (double)get_field<W>(r, 0, L.width[0]);
+  diag[i] = d;
+}
+
+// ------------------------------------------------------------------------------------------------
+// H1 observables: per-workgroup partial sums (fixed order) then a fixed-order reduction.
+// out[0..N) = n_exc, out[N..2N) = n_pho, out[2N] = sum |psi|^2.
+// ------------------------------------------------------------------------------------------------
+
+constexpr int kObsRows = 4;  // rows per thread
+constexpr int kObsTile = kBlock * kObsRows;
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_obs_partial(const u64* __restrict__ rows,
+                                                        const double2* __restrict__ psi, int64_t R,
+                                                        LayoutArg L, int N,
+                                                        double* __restrict__ part) {
+  extern __shared__ double lds[];  // [4 waves][2N+1]
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int S = 2 * N + 1;
+  const int64_t base = (int64_t)blockIdx.x * kObsTile;
+  Row<W> rr[kObsRows];
+  double w[kObsRows];
+  int pos[kObsRows];
+#pragma unroll
+  for (int k = 0; k < kObsRows; ++k) {
+    int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
+    if (i < R) {
+      rr[k] = load_row<W>(rows, i);
+      double2 v = psi[i];
+      w[k] = v.x * v.x + v.y * v.y;
+      pos[k] = (int)get_field<W>(rr[k], 0, L.width[0]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < W; ++q) rr[k].w[q] = 0;
+      w[k] = 0.0;
+      pos[k] = -1;
+    }
+  }
+  for (int s = 0; s < N; ++s) {
+    double ve = 0.0, vp = 0.0;
+    const int off = L.off[s + 1], b = L.width[s + 1];
+#pragma unroll
+    for (int k = 0; k < kObsRows; ++k) {
+      ve += (pos[k] == s) ? w[k] : 0.0;
+      vp += w[k] * (double)get_field<W>(rr[k], off, b);
+    }
+    ve = wave_sum(ve);
+    vp = wave_sum(vp);
+    if (lane == 0) {
+      lds[wid * S + s] = ve;
+      lds[wid * S + N + s] = vp;
+    }
+  }
+  double vn = 0.0;
+#pragma unroll
+  for (int k = 0; k < kObsRows; ++k) vn += w[k];
+  vn = wave_sum(vn);
+  if (lane == 0) lds[wid * S + 2 * N] = vn;
+  __syncthreads();
+  for (int s = threadIdx.x; s < S; s += blockDim.x) {
+    double a = 0.0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) a += lds[q * S + s];
+    part[(int64_t)blockIdx.x * S + s] = a;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_obs_final(const double* __restrict__ part, int nblk,
+                                                      int S, double* __restrict__ out) {
+  __shared__ double red[kBlock / 64];
+  const int s = blockIdx.x;
+  double a = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) a += part[(int64_t)b * S + s];
+  a = wave_sum(a);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < kBlock / 64; ++q) t += red[q];
+    out[s] = t;
+  }
+}
+
+static size_t obs_ws(int64_t R, int N) {
+  int64_t nblk = (R + kObsTile - 1) / kObsTile;
+  if (nblk < 1) nblk = 1;
+  return a256((size_t)nblk * (2 * N + 1) * 8);
+}
+
+template <int W>
+static int run_obs(const uint64_t* rows, const double* psi, int64_t R, const LayoutArg& L, int N,
+                   double* out, void* ws, hipStream_t st) {
+  int64_t nblk = (R + kObsTile - 1) / kObsTile;
+  const int S = 2 * N + 1;
+  double* part = (double*)ws;
+  if (nblk > 0) {
+    size_t lds = (size_t)(kBlock / 64) * S * sizeof(double);
+    hipLaunchKernelGGL(k_obs_partial<W>, dim3((unsigned)nblk), dim3(kBlock), lds, st,
+                       (const u64*)rows, (const double2*)psi, R, L, N, part);
+    int rc = launch_status();
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_obs_final, dim3(S), dim3(kBlock), 0, st, (const double*)part, (int)nblk, S,
+                     out);
+  return launch_status();
+}
+
+template <int W>
+static int run_diag(const uint64_t* rows, int64_t R, const LayoutArg& L, int N, double eps,
+                    double hw, double de, double* diag, hipStream_t st) {
+  if (R == 0) return PACES_OK;
+  hipLaunchKernelGGL(k_diag<W>, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st, (const u64*)rows,
+                     R, L, N, eps, hw, de, diag);
+  return launch_status();
+}
+
+static int check_h(const paces_layout_t* lay, const paces_holstein_t* h) {
+  if (!lay || !h || !lay->d_off || !lay->d_width || !lay->d_maxdim) return PACES_E_INVALID;
+  if (lay->W < 1 || lay->W > PACES_MAX_ROW_WORDS) return PACES_E_UNSUPPORTED;
+  if (h->nchain < 1 || h->nchain > PACES_MAX_SITES || lay->nfields != h->nchain + 1)
+    return PACES_E_INVALID;
+  if (h->nterms < 0 || h->nterms > PACES_MAX_TERMS) return PACES_E_UNSUPPORTED;
+  if (h->geo_dim < 1 || h->lattice_l < 1) return PACES_E_INVALID;
+  if (h->periodic && h->lattice_l < 3) return PACES_E_INVALID;
+  for (int t = 0; t < h->nterms; ++t) {
+    if (h->terms[t].kind == PACES_TERM_HOP &&
+        (h->terms[t].order < 1 || h->terms[t].order > h->lattice_l - 1))
+      return PACES_E_INVALID;
+  }
+  return PACES_OK;
+}
+
+static LayoutArg larg(const paces_layout_t* lay) {
+  LayoutArg L;
+  L.off = lay->d_off;
+  L.width = lay->d_width;
+  L.maxdim = lay->d_maxdim;
+  return L;
+}
+
+}  // namespace paces
+
+using namespace paces;
+
+extern "C" {
+
+int paces_holstein_num_runs(const paces_holstein_t* h) {
+  OpTable T;
+  if (!h) return PACES_E_INVALID;
+  int rc = build_image_ops(h, &T);
+  return rc ? rc : T.n;
+}
+
+size_t paces_holstein_images_workspace_bytes(int64_t n, const paces_holstein_t* h) {
+  OpTable T;
+  if (!h || build_image_ops(h, &T)) return 0;
+  return images_ws(n, T.n);
+}
+
+int paces_holstein_images_count(const uint64_t* basis, int64_t n, const paces_layout_t* lay,
+                                const paces_holstein_t* h, int32_t* run_offsets, void* ws,
+                                size_t ws_bytes, void* stream) {
+  int rc = check_h(lay, h);
+  if (rc) return rc;
+  OpTable T;
+  rc = build_image_ops(h, &T);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && !basis) || !run_offsets || !ws) return PACES_E_INVALID;
+  if ((int64_t)T.n * n >= 2147483647LL) return PACES_E_INVALID;
+  if (ws_bytes < images_ws(n, T.n)) return PACES_E_WORKSPACE;
+  PACES_DISPATCH_W(lay->W, run_img_count, basis, n, larg(lay), T, run_offsets, ws, ws_bytes,
+                   (hipStream_t)stream);
+}
+
+int paces_holstein_images_emit(const uint64_t* basis, int64_t n, const paces_layout_t* lay,
+                               const paces_holstein_t* h, uint64_t* out_images, void* ws,
+                               size_t ws_bytes, void* stream) {
+  int rc = check_h(lay, h);
+  if (rc) return rc;
+  OpTable T;
+  rc = build_image_ops(h, &T);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!basis || !out_images)) || !ws) return PACES_E_INVALID;
+  if (ws_bytes < images_ws(n, T.n)) return PACES_E_WORKSPACE;
+  PACES_DISPATCH_W(lay->W, run_img_emit, basis, n, larg(lay), T, out_images, ws,
+                   (hipStream_t)stream);
+}
+
+size_t paces_holstein_csr_workspace_bytes(int64_t R, const paces_holstein_t* h) {
+  OpTable T;
+  if (!h || build_nb_ops(h, &T)) return 0;
+  return csr_ws(R, T.n, h->nterms);
+}
+
+int paces_holstein_csr_count(const uint64_t* rows, int64_t R, const uint8_t* in_basis,
+                             const paces_layout_t* lay, const paces_holstein_t* h,
+                             int32_t* term_nnz, void* ws, size_t ws_bytes, void* stream) {
+  int rc = check_h(lay, h);
+  if (rc) return rc;
+  OpTable T;
+  rc = build_nb_ops(h, &T);
+  if (rc) return rc;
+  for (int t = 0; t < h->nterms; ++t) {
+    int m = 0;
+    for (int k = 0; k < T.n; ++k) m += (T.term[k] == t);
+    if (m > 8) return PACES_E_UNSUPPORTED;
+  }
+  if (R < 0 || (R > 0 && (!rows || !in_basis)) || !term_nnz || !ws) return PACES_E_INVALID;
+  if ((int64_t)h->nterms * R >= 2147483647LL || (int64_t)T.n * R >= 2147483647LL)
+    return PACES_E_INVALID;
+  if (ws_bytes < csr_ws(R, T.n, h->nterms)) return PACES_E_WORKSPACE;
+  PACES_DISPATCH_W(lay->W, run_csr_count, rows, R, in_basis, larg(lay), T, h->nterms, term_nnz, ws,
+                   (hipStream_t)stream);
+}
+
+int paces_holstein_csr_fill(const uint64_t* rows, int64_t R, const paces_layout_t* lay,
+                            const paces_holstein_t* h, int32_t* const* indptr,
+                            int32_t* const* indices, double* const* data, void* ws,
+                            size_t ws_bytes, void* stream) {
+  int rc = check_h(lay, h);
+  if (rc) return rc;
+  OpTable T;
+  rc = build_nb_ops(h, &T);
+  if (rc) return rc;
+  if (R < 0 || (R > 0 && !rows) || !indptr || !indices || !data || !ws) return PACES_E_INVALID;
+  if (ws_bytes < csr_ws(R, T.n, h->nterms)) return PACES_E_WORKSPACE;
+  if (!h->d_sqrt_table) return PACES_E_INVALID;
+  CsrOut out;
+  for (int t = 0; t < PACES_MAX_TERMS; ++t) {
+    out.indptr[t] = t < h->nterms ? indptr[t] : nullptr;
+    out.indices[t] = t < h->nterms ? indices[t] : nullptr;
+    out.data[t] = t < h->nterms ? data[t] : nullptr;
+    if (t < h->nterms && !out.indptr[t]) return PACES_E_INVALID;
+  }
+  PACES_DISPATCH_W(lay->W, run_csr_fill, rows, R, larg(lay), T, h->nterms, h->d_sqrt_table, out,
+                   ws, (hipStream_t)stream);
+}
+
+int paces_holstein_diag(const uint64_t* rows, int64_t R, const paces_layout_t* lay,
+                        const paces_holstein_t* h, double* diag, void* stream) {
+  int rc = check_h(lay, h);
+  if (rc) return rc;
+  if (R < 0 || (R > 0 && (!rows || !diag))) return PACES_E_INVALID;
+  PACES_DISPATCH_W(lay->W, run_diag, rows, R, larg(lay), h->nchain, h->eps_sys, h->hbar_omega,
+                   h->delta_eps, diag, (hipStream_t)stream);
+}
+
+size_t paces_holstein_observables_workspace_bytes(int64_t R, const paces_holstein_t* h) {
+  if (!h) return 0;
+  return obs_ws(R, h->nchain);
+}
+
+int paces_holstein_observables(const uint64_t* rows, const double* psi, int64_t R,
+                               const paces_layout_t* lay, const paces_holstein_t* h, double* out,
+                               void* ws, size_t ws_bytes, void* stream) {
+  int rc = check_h(lay, h);
+  if (rc) return rc;
+  if (R < 0 || (R > 0 && (!rows || !psi)) || !out || !ws) return PACES_E_INVALID;
+  if (ws_bytes < obs_ws(R, h->nchain)) return PACES_E_WORKSPACE;
+  PACES_DISPATCH_W(lay->W, run_obs, rows, psi, R, larg(lay), h->nchain, out, ws,
+                   (hipStream_t)stream);
+}
+
+}  // extern "C"

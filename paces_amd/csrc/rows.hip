@@ -1,0 +1,324 @@
+// Packed-row primitives: pack / field read / field add (A2-A5), lexicographic lower_bound (B1),
+// union of sorted unique runs (A7) and amplitude reassignment (G1).
+#include "common.hpp"
+
+namespace paces {
+
+int scan_u8(const uint8_t* in, int64_t n, int32_t* out, void* ws, size_t ws_bytes, hipStream_t st);
+size_t scan_ws_bytes(int64_t n);
+
+// ------------------------------------------------------------------------------------------------
+// A2-A5 (paces/core/hamiltonian.py:218-360, paces/models/_kernels/_holstein_kernels.py:96-179)
+// ------------------------------------------------------------------------------------------------
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_pack(const int32_t* __restrict__ raw, int64_t n,
+                                                 int nfields, const int32_t* __restrict__ off,
+                                                 const int32_t* __restrict__ width,
+                                                 u64* __restrict__ rows) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Row<W> r;
+#pragma unroll
+  for (int k = 0; k < W; ++k) r.w[k] = 0;
+  for (int f = 0; f < nfields; ++f) {
+    int b = width[f];
+    if (b) add_field<W>(r, off[f], b, (int64_t)(uint32_t)raw[i * nfields + f]);
+  }
+  store_row<W>(rows, i, r);
+}
+
+// field >= 0: that field; field == -1: the phonon field at the exciton (field pos+1)
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_get_field(const u64* __restrict__ rows, int64_t n,
+                                                      int field, const int32_t* __restrict__ off,
+                                                      const int32_t* __restrict__ width,
+                                                      int32_t* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Row<W> r = load_row<W>(rows, i);
+  int f = field;
+  if (f < 0) f = (int)get_field<W>(r, off[0], width[0]) + 1;
+  out[i] = (int32_t)get_field<W>(r, off[f], width[f]);
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_add_field(u64* __restrict__ rows, int64_t n, int field,
+                                                      int delta, const int32_t* __restrict__ off,
+                                                      const int32_t* __restrict__ width) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Row<W> r = load_row<W>(rows, i);
+  int f = field;
+  if (f < 0) f = (int)get_field<W>(r, off[0], width[0]) + 1;
+  add_field<W>(r, off[f], width[f], delta);
+  store_row<W>(rows, i, r);
+}
+
+template <int W>
+static int run_pack(const int32_t* raw, int64_t n, const paces_layout_t* L, uint64_t* rows,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(k_pack<W>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, raw, n,
+                     L->nfields, L->d_off, L->d_width, (u64*)rows);
+  return launch_status();
+}
+
+template <int W>
+static int run_get(const uint64_t* rows, int64_t n, const paces_layout_t* L, int field,
+                   int32_t* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_get_field<W>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
+                     (const u64*)rows, n, field, L->d_off, L->d_width, out);
+  return launch_status();
+}
+
+template <int W>
+static int run_add(uint64_t* rows, int64_t n, const paces_layout_t* L, int field, int delta,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(k_add_field<W>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, (u64*)rows,
+                     n, field, delta, L->d_off, L->d_width);
+  return launch_status();
+}
+
+static bool layout_ok(const paces_layout_t* L) {
+  return L && L->W >= 1 && L->W <= PACES_MAX_ROW_WORDS && L->nfields >= 1 && L->d_off &&
+         L->d_width;
+}
+
+// ------------------------------------------------------------------------------------------------
+// B1 lower_bound (paces/aux/cupy_search.py:149-228)
+// ------------------------------------------------------------------------------------------------
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_lower_bound(const u64* __restrict__ book, int64_t M,
+                                                        const u64* __restrict__ q, int64_t Q,
+                                                        int32_t* __restrict__ out_idx,
+                                                        uint8_t* __restrict__ out_hit,
+                                                        int32_t* __restrict__ miss) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Q) return;
+  Row<W> r = load_row<W>(q, i);
+  int64_t lb = lower_bound_row<W>(book, 0, M, r);
+  bool hit = (lb < M) && eq_row<W>(r, book + lb * W);
+  if (out_idx) out_idx[i] = (int32_t)lb;
+  if (out_hit) out_hit[i] = hit ? 1 : 0;
+  if (miss && !hit) atomicAdd(miss, 1);
+}
+
+template <int W>
+static int run_lb(const uint64_t* book, int64_t M, const uint64_t* q, int64_t Q, int32_t* idx,
+                  uint8_t* hit, int32_t* miss, hipStream_t st) {
+  if (Q == 0) return PACES_OK;
+  hipLaunchKernelGGL(k_lower_bound<W>, dim3(grid_for(Q, kBlock)), dim3(kBlock), 0, st,
+                     (const u64*)book, M, (const u64*)q, Q, idx, hit, miss);
+  return launch_status();
+}
+
+// ------------------------------------------------------------------------------------------------
+// A7 union of K sorted unique runs (paces/aux/helpers.py:63-75 on the concatenation of
+// paces/core/hamiltonian.py:434-452). Element x of run r goes to output index
+//   pos(x) = sum_s keptprefix_s[lb_s(x)]
+// where lb_s is its lower_bound in run s (its own index for s == r) and keptprefix_s counts the
+// rows of run s that are not duplicates of an earlier run. Rows of run r equal to a row of an
+// earlier run are dropped; the formula gives them the index of the kept copy.
+// ------------------------------------------------------------------------------------------------
+
+struct RunsArg {
+  const u64* ptr[PACES_MAX_RUNS];
+  int64_t off[PACES_MAX_RUNS + 1];
+  int K;
+};
+
+__device__ __forceinline__ int run_of(const RunsArg& R, int64_t g) {
+  int r = 0;
+  for (int s = 1; s < R.K; ++s) r += (g >= R.off[s]) ? 1 : 0;
+  return r;
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, int32_t* __restrict__ lbs,
+                                                    uint8_t* __restrict__ kept) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t C = R.off[R.K];
+  if (g >= C) return;
+  int r = run_of(R, g);
+  int64_t own = g - R.off[r];
+  Row<W> x = load_row<W>(R.ptr[r], own);
+  bool keep = true;
+  for (int s = 0; s < R.K; ++s) {
+    int64_t len = R.off[s + 1] - R.off[s];
+    int64_t lb;
+    if (s == r) {
+      lb = own;
+    } else {
+      lb = lower_bound_row<W>(R.ptr[s], 0, len, x);
+      if (s < r && lb < len && eq_row<W>(x, R.ptr[s] + lb * W)) keep = false;
+    }
+    lbs[g * R.K + s] = (int32_t)lb;
+  }
+  kept[g] = keep ? 1 : 0;
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_mu_scatter(RunsArg R, const int32_t* __restrict__ lbs,
+                                                       const uint8_t* __restrict__ kept,
+                                                       const int32_t* __restrict__ G,
+                                                       u64* __restrict__ out,
+                                                       uint8_t* __restrict__ from0,
+                                                       int32_t* __restrict__ cand_pos,
+                                                       int32_t* __restrict__ out_count) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t C = R.off[R.K];
+  if (g == 0) *out_count = G[C];
+  if (g >= C) return;
+  int64_t pos = 0;
+  for (int s = 0; s < R.K; ++s) {
+    int64_t o = R.off[s];
+    pos += (int64_t)G[o + lbs[g * R.K + s]] - (int64_t)G[o];
+  }
+  if (cand_pos) cand_pos[g] = (int32_t)pos;
+  if (kept[g]) {
+    int r = run_of(R, g);
+    Row<W> x = load_row<W>(R.ptr[r], g - R.off[r]);
+    store_row<W>(out, pos, x);
+    if (from0) from0[pos] = (r == 0) ? 1 : 0;
+  }
+}
+
+__global__ void k_write_zero(int32_t* p) { *p = 0; }
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t mu_ws_bytes(int64_t C, int K) {
+  return align256((size_t)C * K * 4) + align256((size_t)C) + align256((size_t)(C + 1) * 4) +
+         align256(scan_ws_bytes(C));
+}
+
+template <int W>
+static int run_mu(const RunsArg& R, uint64_t* out, int32_t* out_count, uint8_t* from0,
+                  int32_t* cand_pos, void* ws, size_t ws_bytes, hipStream_t st) {
+  int64_t C = R.off[R.K];
+  if (ws_bytes < mu_ws_bytes(C, R.K)) return PACES_E_WORKSPACE;
+  if (C == 0) {
+    hipLaunchKernelGGL(k_write_zero, dim3(1), dim3(1), 0, st, out_count);
+    return launch_status();
+  }
+  char* p = (char*)ws;
+  int32_t* lbs = (int32_t*)p;
+  p += align256((size_t)C * R.K * 4);
+  uint8_t* kept = (uint8_t*)p;
+  p += align256((size_t)C);
+  int32_t* G = (int32_t*)p;
+  p += align256((size_t)(C + 1) * 4);
+  hipLaunchKernelGGL(k_mu_rank<W>, dim3(grid_for(C, kBlock)), dim3(kBlock), 0, st, R, lbs, kept);
+  int rc = launch_status();
+  if (rc) return rc;
+  rc = scan_u8(kept, C, G, p, scan_ws_bytes(C), st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_mu_scatter<W>, dim3(grid_for(C, kBlock)), dim3(kBlock), 0, st, R,
+                     (const int32_t*)lbs, (const uint8_t*)kept, (const int32_t*)G, (u64*)out,
+                     from0, cand_pos, out_count);
+  return launch_status();
+}
+
+// ------------------------------------------------------------------------------------------------
+// G1 reassignment (paces/core/time_evolution.py:781-797)
+// ------------------------------------------------------------------------------------------------
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_reassign(const u64* __restrict__ old_rows, int64_t M,
+                                                     const double2* __restrict__ old_psi,
+                                                     const u64* __restrict__ new_rows, int64_t R,
+                                                     double2* __restrict__ new_psi) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  Row<W> x = load_row<W>(new_rows, i);
+  int64_t lb = lower_bound_row<W>(old_rows, 0, M, x);
+  double2 v = make_double2(0.0, 0.0);
+  if (lb < M && eq_row<W>(x, old_rows + lb * W)) v = old_psi[lb];
+  new_psi[i] = v;
+}
+
+template <int W>
+static int run_reassign(const uint64_t* old_rows, int64_t M, const double* old_psi,
+                        const uint64_t* new_rows, int64_t R, double* new_psi, hipStream_t st) {
+  if (R == 0) return PACES_OK;
+  hipLaunchKernelGGL(k_reassign<W>, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
+                     (const u64*)old_rows, M, (const double2*)old_psi, (const u64*)new_rows, R,
+                     (double2*)new_psi);
+  return launch_status();
+}
+
+}  // namespace paces
+
+using namespace paces;
+
+extern "C" {
+
+int paces_pack_rows(const int32_t* raw, int64_t n, const paces_layout_t* lay, uint64_t* rows,
+                    void* stream) {
+  if (!layout_ok(lay) || n < 0 || (n > 0 && (!raw || !rows))) return PACES_E_INVALID;
+  if (n == 0) return PACES_OK;
+  PACES_DISPATCH_W(lay->W, run_pack, raw, n, lay, rows, (hipStream_t)stream);
+}
+
+int paces_get_field(const uint64_t* rows, int64_t n, const paces_layout_t* lay, int32_t field,
+                    int32_t* out, void* stream) {
+  if (!layout_ok(lay) || n < 0 || field < -1 || field >= lay->nfields) return PACES_E_INVALID;
+  if (n == 0) return PACES_OK;
+  if (!rows || !out) return PACES_E_INVALID;
+  PACES_DISPATCH_W(lay->W, run_get, rows, n, lay, field, out, (hipStream_t)stream);
+}
+
+int paces_add_field(uint64_t* rows, int64_t n, const paces_layout_t* lay, int32_t field,
+                    int32_t delta, void* stream) {
+  if (!layout_ok(lay) || n < 0 || field < -1 || field >= lay->nfields) return PACES_E_INVALID;
+  if (n == 0 || delta == 0) return PACES_OK;
+  if (!rows) return PACES_E_INVALID;
+  PACES_DISPATCH_W(lay->W, run_add, rows, n, lay, field, delta, (hipStream_t)stream);
+}
+
+int paces_lower_bound(const uint64_t* book, int64_t M, const uint64_t* q, int64_t Q, int32_t W,
+                      int32_t q_sorted, int32_t* out_idx, uint8_t* out_hit, int32_t* miss_count,
+                      void* stream) {
+  (void)q_sorted;
+  if (M < 0 || Q < 0 || M >= 2147483647LL || W < 1) return PACES_E_INVALID;
+  if (Q > 0 && (!q || (M > 0 && !book))) return PACES_E_INVALID;
+  PACES_DISPATCH_W(W, run_lb, book, M, q, Q, out_idx, out_hit, miss_count, (hipStream_t)stream);
+}
+
+size_t paces_merge_unique_workspace_bytes(int64_t total, int32_t K) {
+  return mu_ws_bytes(total, K);
+}
+
+int paces_merge_unique(const paces_runs_t* runs, int32_t W, uint64_t* out_rows,
+                       int32_t* out_count, uint8_t* out_from_run0, int32_t* cand_pos, void* ws,
+                       size_t ws_bytes, void* stream) {
+  if (!runs || runs->K < 1 || runs->K > PACES_MAX_RUNS || !out_count) return PACES_E_INVALID;
+  RunsArg R;
+  R.K = runs->K;
+  R.off[0] = 0;
+  for (int s = 0; s < R.K; ++s) {
+    if (runs->len[s] < 0 || (runs->len[s] > 0 && !runs->ptr[s])) return PACES_E_INVALID;
+    R.ptr[s] = (const u64*)runs->ptr[s];
+    R.off[s + 1] = R.off[s] + runs->len[s];
+  }
+  for (int s = R.K; s < PACES_MAX_RUNS; ++s) {
+    R.ptr[s] = nullptr;
+    R.off[s + 1] = R.off[R.K];
+  }
+  if (R.off[R.K] >= 2147483647LL) return PACES_E_INVALID;
+  if (R.off[R.K] > 0 && !out_rows) return PACES_E_INVALID;
+  PACES_DISPATCH_W(W, run_mu, R, out_rows, out_count, out_from_run0, cand_pos, ws, ws_bytes,
+                   (hipStream_t)stream);
+}
+
+int paces_reassign(const uint64_t* old_rows, int64_t M, const double* old_psi,
+                   const uint64_t* new_rows, int64_t R, int32_t W, double* new_psi, void* stream) {
+  if (M < 0 || R < 0 || W < 1) return PACES_E_INVALID;
+  if (R > 0 && (!new_rows || !new_psi || (M > 0 && (!old_rows || !old_psi))))
+    return PACES_E_INVALID;
+  PACES_DISPATCH_W(W, run_reassign, old_rows, M, old_psi, new_rows, R, new_psi,
+                   (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,156 @@
+// Device-wide exclusive prefix sums: the compaction primitive behind every boolean-mask index
+// of the reference step (a[mask] in paces/models/holstein.py:187, 255 and the position
+// partitions of paces/aux/cupy_search.py:16-51).
+//
+// Reduce-then-scan: tiles of kTile elements are reduced, the tile sums are scanned recursively,
+// and each tile is rescanned with its offset. Inputs are read twice, outputs written once.
+#include "common.hpp"
+
+namespace paces {
+
+constexpr int kItems = 8;
+constexpr int kTile = kBlock * kItems;  // 2048 elements per workgroup
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int s = 1; s < 64; s <<= 1) {
+    int o = __shfl_up(v, s, 64);
+    if (lane >= s) v += o;
+  }
+  return v;
+}
+
+// exclusive scan of one value per thread across the workgroup; returns the block total in *tot
+__device__ __forceinline__ int block_excl_scan(int v, int* tot) {
+  __shared__ int wsum[kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int inc = wave_incl_scan(v);
+  if (lane == 63) wsum[wid] = inc;
+  __syncthreads();
+  int off = 0, total = 0;
+#pragma unroll
+  for (int k = 0; k < kBlock / 64; ++k) {
+    int s = wsum[k];
+    off += (k < wid) ? s : 0;
+    total += s;
+  }
+  __syncthreads();
+  *tot = total;
+  return off + inc - v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_tile_reduce(const T* __restrict__ in, int64_t n,
+                                                        int* __restrict__ sums) {
+  int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kItems;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    int64_t i = base + k;
+    s += (i < n) ? (int)in[i] : 0;
+  }
+  int tot;
+  block_excl_scan(s, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// rescan every tile; tile_off == nullptr means a single tile with offset 0.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_tile_scan(const T* __restrict__ in, int64_t n,
+                                                      const int* __restrict__ tile_off,
+                                                      int* __restrict__ out) {
+  int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kItems;
+  int v[kItems];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    int64_t i = base + k;
+    v[k] = (i < n) ? (int)in[i] : 0;
+    s += v[k];
+  }
+  int tot;
+  int run = block_excl_scan(s, &tot) + (tile_off ? tile_off[blockIdx.x] : 0);
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    int64_t i = base + k;
+    if (i < n) out[i] = run;
+    run += v[k];
+  }
+  // the thread owning element n-1 (or thread 0 of an empty range) writes the total
+  if (n == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
+  } else if (base <= n - 1 && n - 1 < base + kItems) {
+    out[n] = run;
+  }
+}
+
+static int64_t n_tiles(int64_t n) { return n <= 0 ? 1 : (n + kTile - 1) / kTile; }
+
+static size_t scan_ws_ints(int64_t n) {
+  // tile sums and their exclusive scan (nt + 1) at every recursion level
+  size_t total = 0;
+  int64_t m = n;
+  while (m > kTile) {
+    int64_t nt = n_tiles(m);
+    total += (size_t)nt + (size_t)(nt + 1);
+    m = nt;
+  }
+  return total + 64;
+}
+
+template <typename T>
+static int scan_rec(const T* in, int64_t n, int* out, int* ws, hipStream_t st) {
+  if (n <= kTile) {
+    hipLaunchKernelGGL(k_tile_scan<T>, dim3(1), dim3(kBlock), 0, st, in, n, (const int*)nullptr,
+                       out);
+    return launch_status();
+  }
+  int64_t nt = n_tiles(n);
+  int* sums = ws;
+  int* offs = ws + nt;
+  hipLaunchKernelGGL(k_tile_reduce<T>, dim3((unsigned)nt), dim3(kBlock), 0, st, in, n, sums);
+  int rc = launch_status();
+  if (rc) return rc;
+  rc = scan_rec<int>(sums, nt, offs, ws + nt + (nt + 1), st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_tile_scan<T>, dim3((unsigned)nt), dim3(kBlock), 0, st, in, n,
+                     (const int*)offs, out);
+  return launch_status();
+}
+
+int scan_u8(const uint8_t* in, int64_t n, int32_t* out, void* ws, size_t ws_bytes,
+            hipStream_t st) {
+  if (n < 0 || !out || (n > 0 && !in)) return PACES_E_INVALID;
+  if (n >= 2147483647LL) return PACES_E_INVALID;
+  if (ws_bytes < scan_ws_ints(n) * sizeof(int)) return PACES_E_WORKSPACE;
+  return scan_rec<uint8_t>(in, n, out, (int*)ws, st);
+}
+
+int scan_i32(const int32_t* in, int64_t n, int32_t* out, void* ws, size_t ws_bytes,
+             hipStream_t st) {
+  if (n < 0 || !out || (n > 0 && !in)) return PACES_E_INVALID;
+  if (n >= 2147483647LL) return PACES_E_INVALID;
+  if (ws_bytes < scan_ws_ints(n) * sizeof(int)) return PACES_E_WORKSPACE;
+  return scan_rec<int>(in, n, out, (int*)ws, st);
+}
+
+size_t scan_ws_bytes(int64_t n) { return scan_ws_ints(n) * sizeof(int); }
+
+}  // namespace paces
+
+extern "C" {
+
+size_t paces_scan_workspace_bytes(int64_t n) { return paces::scan_ws_bytes(n); }
+
+int paces_exclusive_scan_u8(const uint8_t* in, int64_t n, int32_t* out, void* ws, size_t ws_bytes,
+                            void* stream) {
+  return paces::scan_u8(in, n, out, ws, ws_bytes, (hipStream_t)stream);
+}
+
+int paces_exclusive_scan_i32(const int32_t* in, int64_t n, int32_t* out, void* ws,
+                             size_t ws_bytes, void* stream) {
+  return paces::scan_i32(in, n, out, ws, ws_bytes, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,316 @@
+// E1/E2/E3: H.x, the Taylor series for exp(-i dt H) v, and the deterministic reductions behind
+// norms, vdot and the Lanczos recurrence (paces/core/time_evolution.py:382-400, 482-549).
+//
+// FP64 parity order (compiled with -ffp-contract=off): per row
+//   y = d*x;  for each term: t = 0; t = t + a_k * x_{j_k} (ascending columns); y = y + t
+// which is the reference's res = diag*x; res += M_term.dot(x) in use_terms order, with each
+// row of a term summed left to right.
+#include "common.hpp"
+
+namespace paces {
+
+static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct CsrSet {
+  int nterms;
+  const int32_t* indptr[PACES_MAX_TERMS];
+  const int32_t* indices[PACES_MAX_TERMS];
+  const double* data[PACES_MAX_TERMS];
+};
+
+__device__ __forceinline__ double2 hx_row(int64_t i, const double* __restrict__ diag,
+                                          const CsrSet& H, const double2* __restrict__ x) {
+  const double2 xi = x[i];
+  const double d = diag[i];
+  double yr = d * xi.x, yi = d * xi.y;
+  for (int t = 0; t < H.nterms; ++t) {
+    const int32_t b = H.indptr[t][i], e = H.indptr[t][i + 1];
+    double tr = 0.0, ti = 0.0;
+    for (int32_t k = b; k < e; ++k) {
+      const double a = H.data[t][k];
+      const double2 xj = x[H.indices[t][k]];
+      tr = tr + a * xj.x;
+      ti = ti + a * xj.y;
+    }
+    yr = yr + tr;
+    yi = yi + ti;
+  }
+  return make_double2(yr, yi);
+}
+
+__global__ void __launch_bounds__(kBlock) k_spmv(int64_t R, const double* __restrict__ diag,
+                                                 CsrSet H, const double2* __restrict__ x,
+                                                 double2* __restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  y[i] = hx_row(i, diag, H, x);
+}
+
+// ---- Taylor ---------------------------------------------------------------------------------------
+
+__global__ void k_tay_zero(paces_taylor_state_t* s) {
+  s->max_term2 = 0;
+  s->max_res2 = 0;
+  s->c1 = 0.0;
+  s->c1_plus_c2 = 0.0;
+  s->ratio = 0.0;
+  s->done = 0;
+  s->j_ind = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_tay_begin(int64_t R, const double2* __restrict__ v,
+                                                      double2* __restrict__ res,
+                                                      paces_taylor_state_t* s) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  u64 m = 0;
+  if (i < R) {
+    double2 a = v[i];
+    res[i] = a;
+    double q = a.x * a.x + a.y * a.y;
+    m = (u64)__double_as_longlong(q);
+  }
+  m = wave_max_u64(m);
+  if ((threadIdx.x & 63) == 0 && m) atomicMax(&s->max_term2, m);
+}
+
+__global__ void k_tay_begin_fin(paces_taylor_state_t* s) {
+  s->c1 = sqrt(__longlong_as_double((long long)s->max_term2));
+  s->max_term2 = 0;
+  s->max_res2 = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_tay_term(int64_t R, const double* __restrict__ diag,
+                                                     CsrSet H, const double2* __restrict__ xin,
+                                                     double2* __restrict__ xout,
+                                                     double2* __restrict__ res, double c,
+                                                     paces_taylor_state_t* s) {
+  if (s->done) return;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  u64 mt = 0, mr = 0;
+  if (i < R) {
+    const double2 h = hx_row(i, diag, H, xin);
+    // (a + ib) * (-0 - ic) = (b*c, -(a*c)) exactly
+    const double2 t = make_double2(h.y * c, -(h.x * c));
+    xout[i] = t;
+    double2 r = res[i];
+    r.x = r.x + t.x;
+    r.y = r.y + t.y;
+    res[i] = r;
+    mt = (u64)__double_as_longlong(t.x * t.x + t.y * t.y);
+    mr = (u64)__double_as_longlong(r.x * r.x + r.y * r.y);
+  }
+  mt = wave_max_u64(mt);
+  mr = wave_max_u64(mr);
+  if ((threadIdx.x & 63) == 0) {
+    if (mt) atomicMax(&s->max_term2, mt);
+    if (mr) atomicMax(&s->max_res2, mr);
+  }
+}
+
+__global__ void k_tay_check(paces_taylor_state_t* s, int j, double tol) {
+  if (s->done) return;
+  const double c2 = sqrt(__longlong_as_double((long long)s->max_term2));
+  const double rn = sqrt(__longlong_as_double((long long)s->max_res2));
+  const double c1pc2 = s->c1 + c2;
+  const double ratio = c1pc2 / rn;
+  s->c1_plus_c2 = c1pc2;
+  s->ratio = ratio;
+  s->j_ind = j;
+  if (ratio <= tol)
+    s->done = 1;
+  else
+    s->c1 = c2;
+  s->max_term2 = 0;
+  s->max_res2 = 0;
+}
+
+// ---- deterministic reductions ------------------------------------------------------------------
+
+constexpr int kRedTile = kBlock * 4;
+
+// mode 0: sum |x|^2 -> part[b]; mode 1: sum conj(x) y -> part[2b], part[2b+1]
+__global__ void __launch_bounds__(kBlock) k_red_partial(const double2* __restrict__ x,
+                                                        const double2* __restrict__ y, int64_t R,
+                                                        int mode, double* __restrict__ part) {
+  __shared__ double red[2][kBlock / 64];
+  const int64_t base = (int64_t)blockIdx.x * kRedTile;
+  double a = 0.0, b = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
+    if (i < R) {
+      double2 u = x[i];
+      if (mode == 0) {
+        a += u.x * u.x + u.y * u.y;
+      } else {
+        double2 v = y[i];
+        a += u.x * v.x + u.y * v.y;
+        b += u.x * v.y - u.y * v.x;
+      }
+    }
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = a;
+    red[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int q = 0; q < kBlock / 64; ++q) {
+      s0 += red[0][q];
+      s1 += red[1][q];
+    }
+    if (mode == 0) {
+      part[blockIdx.x] = s0;
+    } else {
+      part[2 * blockIdx.x] = s0;
+      part[2 * blockIdx.x + 1] = s1;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_red_final(const double* __restrict__ part, int nblk,
+                                                      int width, double* __restrict__ out) {
+  __shared__ double red[kBlock / 64];
+  const int c = blockIdx.x;
+  double a = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) a += part[(int64_t)b * width + c];
+  a = wave_sum(a);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < kBlock / 64; ++q) t += red[q];
+    out[c] = t;
+  }
+}
+
+static int64_t red_blocks(int64_t R) {
+  int64_t n = (R + kRedTile - 1) / kRedTile;
+  return n < 1 ? 1 : n;
+}
+
+__global__ void __launch_bounds__(kBlock) k_axpby(int64_t R, double ar, double ai,
+                                                  const double2* __restrict__ x, double br,
+                                                  double bi, double2* __restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  double2 u = x[i];
+  double2 o;
+  o.x = ar * u.x - ai * u.y;
+  o.y = ar * u.y + ai * u.x;
+  if (br != 0.0 || bi != 0.0) {  // b == 0: y is write-only (may be uninitialised)
+    double2 v = y[i];
+    o.x = o.x + (br * v.x - bi * v.y);
+    o.y = o.y + (br * v.y + bi * v.x);
+  }
+  y[i] = o;
+}
+
+static bool csr_set(const paces_csr_set_t* h, CsrSet* H) {
+  if (!h || h->nterms < 0 || h->nterms > PACES_MAX_TERMS) return false;
+  H->nterms = h->nterms;
+  for (int t = 0; t < PACES_MAX_TERMS; ++t) {
+    H->indptr[t] = t < h->nterms ? h->indptr[t] : nullptr;
+    H->indices[t] = t < h->nterms ? h->indices[t] : nullptr;
+    H->data[t] = t < h->nterms ? h->data[t] : nullptr;
+    if (t < h->nterms && !H->indptr[t]) return false;
+  }
+  return true;
+}
+
+}  // namespace paces
+
+using namespace paces;
+
+extern "C" {
+
+int paces_spmv(int64_t R, const double* diag, const paces_csr_set_t* h, const double* x,
+               double* y, void* stream) {
+  CsrSet H;
+  if (R < 0 || !csr_set(h, &H)) return PACES_E_INVALID;
+  if (R == 0) return PACES_OK;
+  if (!diag || !x || !y) return PACES_E_INVALID;
+  hipLaunchKernelGGL(k_spmv, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, R,
+                     diag, H, (const double2*)x, (double2*)y);
+  return launch_status();
+}
+
+int paces_taylor_begin(int64_t R, const double* v, double* res, paces_taylor_state_t* state,
+                       void* stream) {
+  if (R < 0 || !state || (R > 0 && (!v || !res))) return PACES_E_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_tay_zero, dim3(1), dim3(1), 0, st, state);
+  if (R > 0)
+    hipLaunchKernelGGL(k_tay_begin, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st, R,
+                       (const double2*)v, (double2*)res, state);
+  hipLaunchKernelGGL(k_tay_begin_fin, dim3(1), dim3(1), 0, st, state);
+  return launch_status();
+}
+
+int paces_taylor_term(int64_t R, const double* diag, const paces_csr_set_t* h, const double* x_in,
+                      double* x_out, double* res, double c, int32_t j, double tol,
+                      paces_taylor_state_t* state, void* stream) {
+  CsrSet H;
+  if (R < 0 || !state || !csr_set(h, &H)) return PACES_E_INVALID;
+  if (R > 0 && (!diag || !x_in || !x_out || !res)) return PACES_E_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  if (R > 0)
+    hipLaunchKernelGGL(k_tay_term, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st, R, diag, H,
+                       (const double2*)x_in, (double2*)x_out, (double2*)res, c, state);
+  hipLaunchKernelGGL(k_tay_check, dim3(1), dim3(1), 0, st, state, (int)j, tol);
+  return launch_status();
+}
+
+size_t paces_reduce_workspace_bytes(int64_t R) { return a256((size_t)red_blocks(R) * 2 * 8); }
+
+int paces_norm2(const double* x, int64_t R, double* out, void* ws, size_t ws_bytes, void* stream) {
+  if (R < 0 || !out || !ws || (R > 0 && !x)) return PACES_E_INVALID;
+  if (ws_bytes < paces_reduce_workspace_bytes(R)) return PACES_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  int64_t nb = red_blocks(R);
+  hipLaunchKernelGGL(k_red_partial, dim3((unsigned)nb), dim3(kBlock), 0, st, (const double2*)x,
+                     (const double2*)nullptr, R, 0, (double*)ws);
+  hipLaunchKernelGGL(k_red_final, dim3(1), dim3(kBlock), 0, st, (const double*)ws, (int)nb, 1,
+                     out);
+  return launch_status();
+}
+
+int paces_vdot(const double* x, const double* y, int64_t R, double* out, void* ws,
+               size_t ws_bytes, void* stream) {
+  if (R < 0 || !out || !ws || (R > 0 && (!x || !y))) return PACES_E_INVALID;
+  if (ws_bytes < paces_reduce_workspace_bytes(R)) return PACES_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  int64_t nb = red_blocks(R);
+  hipLaunchKernelGGL(k_red_partial, dim3((unsigned)nb), dim3(kBlock), 0, st, (const double2*)x,
+                     (const double2*)y, R, 1, (double*)ws);
+  hipLaunchKernelGGL(k_red_final, dim3(2), dim3(kBlock), 0, st, (const double*)ws, (int)nb, 2,
+                     out);
+  return launch_status();
+}
+
+int paces_axpby(int64_t R, double ar, double ai, const double* x, double br, double bi, double* y,
+                void* stream) {
+  if (R < 0 || (R > 0 && (!x || !y))) return PACES_E_INVALID;
+  if (R == 0) return PACES_OK;
+  hipLaunchKernelGGL(k_axpby, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, R,
+                     ar, ai, (const double2*)x, br, bi, (double2*)y);
+  return launch_status();
+}
+
+const char* paces_strerror(int code) {
+  switch (code) {
+    case PACES_OK: return "ok";
+    case PACES_E_INVALID: return "invalid argument";
+    case PACES_E_UNSUPPORTED: return "unsupported size (row width, term or run count)";
+    case PACES_E_HIP: return "HIP launch or runtime error";
+    case PACES_E_WORKSPACE: return "workspace too small";
+    default: return "unknown paces status";
+  }
+}
+
+int paces_version(void) { return 10000; }
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+"""Shared set-up for parity tests: the same Holstein run on the oracle and on the GPU engine."""
+
+import numpy as np
+
+from oracle import paces_oracle as po
+
+
+def terms(J=1.0, g=1.0, w0=1.0, eps=0.0, de=0.0, J2=None):
+    """Config Hamiltonian -J sum(a+a + h.c.) + w0 sum b+b + g w0 sum n(b+ + b) mapped onto the
+    reference's use_terms (hopping element +J_ref, vib element g_ref sqrt(n+1))."""
+    t = {"diag": {"eps_sys": eps, "hbar_omega": w0, "delta_eps": de}, "hopping": {"J": -J}}
+    if J2 is not None:
+        t["hopping2"] = {"J2": -J2}
+    t["vib_coupling"] = {"g": g * w0}
+    return t
+
+
+CASES = {
+    # config 1: 1D OBC chain N=16, cap 7, J=1, g=1, exciton at 8
+    "cfg1": dict(N=16, dim=8, geo=1, periodic=False, terms=terms(1.0, 1.0), pos0=8,
+                 ms=20000, dt=0.05, steps=12, enlarge=1, fillfac=0.1),
+    # config 2 layout (64-site ring, cap 15, g=1.5, five uint64 words) at a reduced maxstates
+    "cfg2_small": dict(N=64, dim=16, geo=1, periodic=True, terms=terms(1.0, 1.5), pos0=32,
+                       ms=4000, dt=0.02, steps=4, enlarge=1, fillfac=0.1),
+    # config 5 physics on a 6x6 torus, cap 7, g=1.2 (multi-word 2D rows)
+    "torus6": dict(N=36, dim=8, geo=2, periodic=True, terms=terms(1.0, 1.2), pos0=21,
+                   ms=3000, dt=0.02, steps=3, enlarge=0, fillfac=0.1),
+    # OBC 2D lattice with next-nearest hopping and a site bias
+    "square4_hop2": dict(N=16, dim=4, geo=2, periodic=False,
+                         terms=terms(0.8, 0.9, de=0.05, J2=0.3), pos0=5,
+                         ms=1500, dt=0.05, steps=4, enlarge=1, fillfac=0.2),
+}
+
+
+def oracle_run(case):
+    c = CASES[case] if isinstance(case, str) else case
+    ham = po.HolsteinOracle(c["N"], [c["dim"]] * c["N"], geo_dim=c["geo"], use_terms=c["terms"],
+                            periodic=c["periodic"])
+    te = po.TimeEvolutionOracle(ham, c["ms"], c["enlarge"])
+    te.create_single_state_basis(c["pos0"])
+    te.grow_optimal_basis(fillfac=c["fillfac"])
+    raw = np.zeros((1, c["N"] + 1), dtype=np.int64)
+    raw[0, 0] = c["pos0"]
+    te.create_initial_vector([1.0], raw)
+    t = c["dt"] * np.arange(c["steps"] + 1)
+    recs = []
+    te.generate_timeline(t, recs.append)
+    return ham, recs, t
+
+
+def gpu_setup(case, lanczos_order=0):
+    from paces_amd.config import devices
+    from paces_amd.core.time_evolution import ExpmParams, TimeEvoParams
+    from paces_amd.models import holstein
+    c = CASES[case] if isinstance(case, str) else case
+    if not devices.configure_called:
+        devices.configure()
+    ham = holstein.Hamiltonian(c["N"], [c["dim"]] * c["N"], geo_dim=c["geo"],
+                               periodic=c["periodic"], use_terms=c["terms"])
+    te = holstein.TimeEvolution(ham, holstein.Observables, ["n_exc", "n_pho", "total_energy"],
+                                None, TimeEvoParams(maxstates=c["ms"], enlarge_steps=c["enlarge"],
+                                                    shuffle_seed=None),
+                                ExpmParams(lanczos_order=lanczos_order))
+    te.create_nonuniform_basis([1] * c["N"], minpos=c["pos0"], maxpos=c["pos0"] + 1)
+    te.grow_optimal_basis(fillfac=c["fillfac"])
+    raw = np.zeros((1, c["N"] + 1), dtype=np.int64)
+    raw[0, 0] = c["pos0"]
+    te.create_initial_vector([1.0], raw)
+    return ham, te
+
+
+def gpu_step(te, i, t):
+    if i == 0:
+        te._perform_first_step(t[0])
+    else:
+        te._evolve_single_step(i, t[i], t[i] - t[i - 1])

@@ -1,0 +1,224 @@
+"""GPU parity of each device primitive against the oracle, through the C ABI.
+
+Integer/row results must be bit-exact; FP64 products in the parity order must be bit-exact;
+reductions (norms, observables) must agree to 1e-12 relative.
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import paces_oracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+    from paces_amd.config import devices
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    if not devices.configure_called:
+        devices.configure()
+    from paces_amd import _hip
+    from paces_amd.core import engine
+    return dict(torch=torch, dev=devices.vector_dev, hip=_hip, engine=engine, lib=_hip.load())
+
+
+def _rows(env, a):
+    return env["hip"].rows_from_numpy(a, env["dev"])
+
+
+def _np(env, t):
+    return env["hip"].rows_to_numpy(t)
+
+
+def _rand_unique_rows(rng, n, W, hi=None):
+    if hi is None:
+        a = rng.integers(0, 2 ** 63, size=(n, W), dtype=np.int64).astype(np.uint64)
+        a[:, 0] >>= np.uint64(rng.integers(0, 8))
+    else:
+        a = rng.integers(0, hi, size=(n, W)).astype(np.uint64)
+    return po.unique_rows(a)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2047, 2048, 2049, 70000, 5_000_003])
+def test_scan_u8_and_i32(env, n):
+    torch, lib, hip = env["torch"], env["lib"], env["hip"]
+    rng = np.random.default_rng(n)
+    a8 = rng.integers(0, 2, n).astype(np.uint8)
+    a32 = rng.integers(0, 9, n).astype(np.int32)
+    ws = hip.Workspace(env["dev"])
+    for arr, fn in ((a8, lib.paces_exclusive_scan_u8), (a32, lib.paces_exclusive_scan_i32)):
+        d_in = torch.from_numpy(arr).to(env["dev"])
+        d_out = torch.empty(n + 1, dtype=torch.int32, device=env["dev"])
+        wsp, wsn = ws.get(lib.paces_scan_workspace_bytes(n))
+        hip.check(fn(hip.ptr(d_in) if n else None, n, hip.ptr(d_out), wsp, wsn, hip.stream()))
+        exp = np.concatenate([[0], np.cumsum(arr.astype(np.int64))])
+        np.testing.assert_array_equal(d_out.cpu().numpy(), exp)
+
+
+@pytest.mark.parametrize("W", [1, 2, 5, 13, 16])
+def test_lower_bound_bit_exact(env, W):
+    torch, eng = env["torch"], env["engine"]
+    rng = np.random.default_rng(W)
+    book = _rand_unique_rows(rng, 20000, W, hi=7 if W > 1 else 50000)
+    present = book[rng.choice(book.shape[0], 3000, replace=False)]
+    absent = rng.integers(0, 7 if W > 1 else 50000, size=(3000, W)).astype(np.uint64)
+    q = np.concatenate([present, absent, book[:1], book[-1:], np.full((1, W), 2 ** 64 - 1,
+                                                                      dtype=np.uint64)])
+    for qq in (q, po.unique_rows(q)):
+        idx, hit, miss = eng.lower_bound(_rows(env, book), _rows(env, qq), True, True)
+        exp_idx, exp_hit = po.lookup(book, qq)
+        np.testing.assert_array_equal(idx.cpu().numpy(), exp_idx)
+        np.testing.assert_array_equal(hit.cpu().numpy().astype(bool), exp_hit)
+        assert int(miss.item()) == int((~exp_hit).sum())
+    # empty phonebook and single-row phonebook
+    idx, hit, _ = eng.lower_bound(_rows(env, book[:0]), _rows(env, q[:5]), True)
+    np.testing.assert_array_equal(idx.cpu().numpy(), 0)
+    idx, hit, _ = eng.lower_bound(_rows(env, book[3:4]), _rows(env, q), True)
+    np.testing.assert_array_equal(idx.cpu().numpy(), po.lower_bound(book[3:4], q))
+
+
+@pytest.mark.parametrize("dims", [[16] + [8] * 16, [64] + [16] * 64, [256] + [8] * 256,
+                                  [5, 3, 7, 1, 2, 9]])
+def test_pack_get_add_fields(env, dims):
+    import paces_amd._hip as hip
+    torch, lib = env["torch"], env["lib"]
+    lay = po.Layout(dims)
+    rng = np.random.default_rng(len(dims))
+    raw = np.stack([rng.integers(0, d, 500) for d in lay.max_dims], axis=1)
+    dev = env["dev"]
+    d_off = torch.tensor(lay.offsets, dtype=torch.int32, device=dev)
+    d_w = torch.tensor(lay.bitwidths, dtype=torch.int32, device=dev)
+    d_m = torch.tensor(lay.max_dims, dtype=torch.int32, device=dev)
+    L = hip.Layout(lay.W, lay.nfields, d_off.data_ptr(), d_w.data_ptr(), d_m.data_ptr())
+    d_raw = torch.tensor(raw, dtype=torch.int32, device=dev)
+    rows = torch.empty((500, lay.W), dtype=torch.int64, device=dev)
+    hip.check(lib.paces_pack_rows(hip.ptr(d_raw), 500, C.byref(L), hip.ptr(rows), hip.stream()))
+    np.testing.assert_array_equal(_np(env, rows), lay.pack(raw))
+    out = torch.empty(500, dtype=torch.int32, device=dev)
+    for f in range(lay.nfields):
+        hip.check(lib.paces_get_field(hip.ptr(rows), 500, C.byref(L), f, hip.ptr(out),
+                                      hip.stream()))
+        np.testing.assert_array_equal(out.cpu().numpy(), raw[:, f])
+    for f in range(lay.nfields):
+        if lay.bitwidths[f] == 0:
+            continue
+        for delta, sel in ((+1, raw[:, f] + 1 < lay.max_dims[f]), (-1, raw[:, f] > 0)):
+            sub = raw[sel]
+            d = _rows(env, lay.pack(sub))
+            hip.check(lib.paces_add_field(hip.ptr(d), sub.shape[0], C.byref(L), f, delta,
+                                          hip.stream()))
+            exp = sub.copy()
+            exp[:, f] += delta
+            np.testing.assert_array_equal(_np(env, d), lay.pack(exp))
+
+
+@pytest.mark.parametrize("W,K", [(1, 5), (3, 7), (5, 11), (13, 3)])
+def test_merge_unique_matches_numpy_unique(env, W, K):
+    eng, hip = env["engine"], env["hip"]
+    rng = np.random.default_rng(W * 100 + K)
+    pool = _rand_unique_rows(rng, 30000, W, hi=9)
+    runs = []
+    for k in range(K):
+        n = int(rng.integers(0, 6000)) if k else 5000
+        if k == 2:
+            n = 0
+        runs.append(pool[np.sort(rng.choice(pool.shape[0], min(n, pool.shape[0]),
+                                            replace=False))])
+    d_runs = [(_rows(env, r) if r.shape[0] else _rows(env, pool[:1]), 0, r.shape[0])
+              for r in runs]
+    ws = hip.Workspace(env["dev"])
+    out, from0 = eng.merge_unique(d_runs, W, ws, want_from0=True)
+    exp = po.unique_rows(np.concatenate(runs))
+    np.testing.assert_array_equal(_np(env, out), exp)
+    _, in0 = po.lookup(runs[0], exp)
+    np.testing.assert_array_equal(from0.cpu().numpy().astype(bool), in0)
+    # all runs empty
+    out, _ = eng.merge_unique([(d_runs[0][0], 0, 0)], W, ws)
+    assert out.shape[0] == 0
+
+
+@pytest.mark.parametrize("R,k", [(1, 1), (10, 3), (5000, 1), (5000, 4999), (5000, 5000),
+                                 (200000, 20000), (200000, 300000)])
+def test_select_topk_stable_ties(env, R, k):
+    torch, eng, hip = env["torch"], env["engine"], env["hip"]
+    rng = np.random.default_rng(R + k)
+    vec = (rng.integers(0, 5, R) * 0.25 + 1j * rng.integers(0, 3, R) * 0.5).astype(np.complex128)
+    vec[rng.integers(0, R, max(1, R // 10))] *= 1e-300  # subnormal-ish keys
+    rows = _rand_unique_rows(rng, R * 2, 2, hi=2 ** 40)[:R]
+    assert rows.shape[0] == R
+    out = eng.select_topk(torch.from_numpy(vec).to(env["dev"]), _rows(env, rows), k,
+                          hip.Workspace(env["dev"]))
+    np.testing.assert_array_equal(_np(env, out), rows[po.select_indices(vec, k)])
+
+
+def test_reassign_and_reductions(env):
+    torch, eng = env["torch"], env["engine"]
+    rng = np.random.default_rng(5)
+    old = _rand_unique_rows(rng, 50000, 3, hi=6)
+    new = _rand_unique_rows(rng, 60000, 3, hi=6)
+    vec = rng.standard_normal(old.shape[0]) + 1j * rng.standard_normal(old.shape[0])
+    d_old_vec = torch.from_numpy(vec).to(env["dev"])
+    out = eng.reassign(_rows(env, old), d_old_vec, _rows(env, new))
+    np.testing.assert_array_equal(out.cpu().numpy(), po.reassign(old, vec, new))
+    red = eng.Reducer(env["dev"])
+    assert abs(red.norm2(d_old_vec) - np.sum(np.abs(vec) ** 2)) <= 1e-12 * np.sum(np.abs(vec) ** 2)
+    y = rng.standard_normal(old.shape[0]) + 1j * rng.standard_normal(old.shape[0])
+    v = red.vdot(d_old_vec, torch.from_numpy(y).to(env["dev"]))
+    assert abs(v - np.vdot(vec, y)) <= 1e-12 * np.sum(np.abs(vec) * np.abs(y))
+    z = torch.from_numpy(y).to(env["dev"])
+    eng.axpby(0.5 - 2j, d_old_vec, 1.5 + 0.25j, z)
+    np.testing.assert_allclose(z.cpu().numpy(), (0.5 - 2j) * vec + (1.5 + 0.25j) * y,
+                               rtol=0, atol=1e-13)
+
+
+GEN_CASES = ["cfg1", "cfg2_small", "torus6", "square4_hop2"]
+
+
+@pytest.mark.parametrize("case", GEN_CASES)
+def test_expand_csr_diag_taylor_observables(env, case):
+    """generate_mel on the oracle's own step-1 basis: rows, per-term CSR, diagonal bit-exact;
+    H.x and the Taylor series bit-exact; observables to 1e-12."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from holstein_cases import CASES, oracle_run
+    torch, eng, hip = env["torch"], env["engine"], env["hip"]
+    c = dict(CASES[case])
+    c["steps"] = min(c["steps"], 2)
+    oham, recs, t = oracle_run(c)
+    from paces_amd.models import holstein
+    ham = holstein.Hamiltonian(c["N"], [c["dim"]] * c["N"], geo_dim=c["geo"],
+                               periodic=c["periodic"], use_terms=c["terms"])
+    rec = recs[-1]
+    sel = _rows(env, rec.select)
+    (diag, new_inds), mats, dbg = ham.generate_mel(sel, enlarge_steps=c["enlarge"])
+    np.testing.assert_array_equal(_np(env, ham.last_basis_e), rec.basis_e)
+    np.testing.assert_array_equal(_np(env, new_inds), rec.whoami)
+    np.testing.assert_array_equal(diag.cpu().numpy(), rec.diag)
+    for name in oham.offdiag_terms:
+        ip, ix, dv = rec.csrs[name]
+        m = mats[name]
+        np.testing.assert_array_equal(m.indptr.cpu().numpy(), ip)
+        np.testing.assert_array_equal(m.indices.cpu().numpy(), ix)
+        np.testing.assert_array_equal(m.data.cpu().numpy(), dv)
+    if "vib_coupling" in oham.offdiag_terms:
+        assert dbg["vib_coupling"] == rec.dbg["vib_coupling"]
+    csr_list = [rec.csrs[n] for n in oham.offdiag_terms]
+    mat_list = [mats[n] for n in oham.offdiag_terms]
+    v = rec.vec_adapted
+    d_v = torch.from_numpy(v).to(env["dev"])
+    hx = eng.spmv(diag, mat_list, d_v).cpu().numpy()
+    np.testing.assert_array_equal(hx, po.total_ham(rec.diag, csr_list, v))
+    res, dbg_g = eng.simple_taylor(diag, mat_list, d_v, float(t[-1] - t[-2]))
+    exp, dbg_o = po.simple_taylor(rec.diag, csr_list, v, float(t[-1] - t[-2]))
+    np.testing.assert_array_equal(res.cpu().numpy(), exp)
+    assert dbg_g[:2] == dbg_o[:2] and dbg_g[2] == dbg_o[2] and dbg_g[3] == dbg_o[3]
+    n_exc, n_pho, nrm2 = ham.site_observables(new_inds, torch.from_numpy(rec.vec).to(env["dev"]))
+    np.testing.assert_allclose(n_exc, rec.obs["n_exc"], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(n_pho, rec.obs["n_pho"], rtol=0, atol=1e-12)
+    assert abs(np.sqrt(nrm2) - rec.obs["norm"]) < 1e-12
