@@ -63,9 +63,11 @@ def measure_taylor_kernel(te, reps=20):
     bufs = (torch.empty_like(v), torch.empty_like(v))
     state = torch.zeros(C.sizeof(_hip.TaylorState), dtype=torch.uint8, device=v.device)
     st = torch.cuda.current_stream()
+    ws = _hip.Workspace(v.device)
+    wsp, wsn = ws.get(lib.paces_taylor_workspace_bytes(R))
     _hip.check(lib.paces_taylor_begin(R, _hip.ptr(torch.view_as_real(v)),
-                                      _hip.ptr(torch.view_as_real(res)), _hip.ptr(state),
-                                      C.c_void_p(st.cuda_stream)))
+                                      _hip.ptr(torch.view_as_real(res)), _hip.ptr(state), wsp,
+                                      wsn, C.c_void_p(st.cuda_stream)))
     x_in = v
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     # tol < 0 never converges, so every launch does the full term
@@ -77,7 +79,7 @@ def measure_taylor_kernel(te, reps=20):
                                          _hip.ptr(torch.view_as_real(x_in)),
                                          _hip.ptr(torch.view_as_real(x_out)),
                                          _hip.ptr(torch.view_as_real(res)), 1e-3, j, -1.0,
-                                         _hip.ptr(state), C.c_void_p(st.cuda_stream)))
+                                         _hip.ptr(state), wsp, wsn, C.c_void_p(st.cuda_stream)))
         x_in = x_out
     e1.record(st)
     e1.synchronize()

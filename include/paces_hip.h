@@ -193,14 +193,16 @@ typedef struct {
   int32_t pad[4];
 } paces_taylor_state_t;
 
+/* Workspace of per-workgroup partial maxima shared by _begin and _term. */
+size_t paces_taylor_workspace_bytes(int64_t R);
 /* res = v; state.c1 = ||v||_inf; state.done = 0. */
 int paces_taylor_begin(int64_t R, const double* v, double* res, paces_taylor_state_t* state,
-                       void* stream);
+                       void* ws, size_t ws_bytes, void* stream);
 /* One series term j: x_out = (-i*c) * (H x_in); res += x_out; then the convergence test
  * (c1 + c2) / ||res||_inf <= tol. No-op once state.done != 0. */
 int paces_taylor_term(int64_t R, const double* diag, const paces_csr_set_t* H, const double* x_in,
                       double* x_out, double* res, double c, int32_t j, double tol,
-                      paces_taylor_state_t* state, void* stream);
+                      paces_taylor_state_t* state, void* ws, size_t ws_bytes, void* stream);
 
 /* ---- deterministic vector reductions (norms, vdot) and Lanczos helpers (E3,
  * paces/core/time_evolution.py:515-549). out is a device double[2] (re, im) or double[1]. */

@@ -95,9 +95,10 @@ _SIGNATURES = {
                                              C.POINTER(Holstein), _P, _P, _SZ, _P]),
     "paces_reassign": (C.c_int, [_P, _I64, _P, _P, _I64, _I32, _P, _P]),
     "paces_spmv": (C.c_int, [_I64, _P, C.POINTER(CsrSet), _P, _P, _P]),
-    "paces_taylor_begin": (C.c_int, [_I64, _P, _P, _P, _P]),
+    "paces_taylor_workspace_bytes": (_SZ, [_I64]),
+    "paces_taylor_begin": (C.c_int, [_I64, _P, _P, _P, _P, _SZ, _P]),
     "paces_taylor_term": (C.c_int, [_I64, _P, C.POINTER(CsrSet), _P, _P, _P, _D, _I32, _D, _P,
-                                    _P]),
+                                    _P, _SZ, _P]),
     "paces_reduce_workspace_bytes": (_SZ, [_I64]),
     "paces_norm2": (C.c_int, [_P, _I64, _P, _P, _SZ, _P]),
     "paces_vdot": (C.c_int, [_P, _P, _I64, _P, _P, _SZ, _P]),

@@ -168,7 +168,7 @@ def taylor_coeff(delta_t, j):
 TAYLOR_TOL = 1.1102230246251565e-16  # 2**-53 (paces/core/time_evolution.py:484)
 
 
-def simple_taylor(diag, mats, v, delta_t, m_star=100, tol=TAYLOR_TOL, chunk=6):
+def simple_taylor(diag, mats, v, delta_t, m_star=100, tol=TAYLOR_TOL, chunk=6, ws=None):
     """res = sum_j (-i dt H)^j v / j! until (|t_j| + |t_{j+1}|)_inf / |res|_inf <= tol
     (paces/core/time_evolution.py:482-512). Returns (res, [converged, j_ind, c1+c2, ratio])."""
     R = v.shape[0]
@@ -177,8 +177,10 @@ def simple_taylor(diag, mats, v, delta_t, m_star=100, tol=TAYLOR_TOL, chunk=6):
     state = torch.zeros(C.sizeof(_hip.TaylorState), dtype=torch.uint8, device=v.device)
     s = csr_set(mats)
     st = stream()
+    ws = ws if ws is not None else _hip.Workspace(v.device)
+    wsp, wsn = ws.get(lib.paces_taylor_workspace_bytes(R))
     check(lib.paces_taylor_begin(R, ptr(torch.view_as_real(v)), ptr(torch.view_as_real(res)),
-                                 ptr(state), st), "taylor_begin")
+                                 ptr(state), wsp, wsn, st), "taylor_begin")
     x_in = v
     j = 0
     host = None
@@ -190,7 +192,7 @@ def simple_taylor(diag, mats, v, delta_t, m_star=100, tol=TAYLOR_TOL, chunk=6):
             check(lib.paces_taylor_term(R, ptr(diag), C.byref(s), ptr(torch.view_as_real(x_in)),
                                         ptr(torch.view_as_real(x_out)),
                                         ptr(torch.view_as_real(res)), taylor_coeff(delta_t, j), j,
-                                        tol, ptr(state), st), "taylor_term")
+                                        tol, ptr(state), wsp, wsn, st), "taylor_term")
             x_in = x_out
             j += 1
         host = _hip.TaylorState.from_buffer_copy(bytes(state.cpu().numpy()))

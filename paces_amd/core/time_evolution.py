@@ -22,7 +22,7 @@ from typing import Optional
 import numpy
 import torch
 
-from .. import __version__
+from .. import __version__, _hip
 from ..aux.helpers import flatten_dbg_dict
 from ..config import (BASE_STEP_LEVEL, EXPM_EVO_LEVEL, HILBERT_SPACE_LEVEL,
                       INIT_VERBOSITY_LEVEL, MEM_INFO_LEVEL, devices)
@@ -119,6 +119,7 @@ class TimeEvolutionFramework:
         self.last_expm_dbg = None
         self.last_dbg_dict = {}
         self._reducer = engine.Reducer(self.device)
+        self._taylor_ws = _hip.Workspace(self.device)
         self._ws = hamobj._ws
         self._step_index = 0
 
@@ -251,7 +252,7 @@ class TimeEvolutionFramework:
     # ---- propagators (482-549) ------------------------------------------------------------------------
     def _simple_taylor(self, delta_t):
         res, dbg = engine.simple_taylor(self.diag_vals, self._mats(), self.vector, delta_t,
-                                        self.expm_params.m_star)
+                                        self.expm_params.m_star, ws=self._taylor_ws)
         if self.debug_verb > EXPM_EVO_LEVEL:
             print(f"         Taylor series: converged={dbg[0]} after j={dbg[1]}.")
         return res, dbg

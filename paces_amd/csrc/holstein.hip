@@ -258,6 +258,7 @@ __global__ void __launch_bounds__(kBlock) k_csr_count(const u64* __restrict__ ro
                                                       const uint8_t* __restrict__ inB, OpTable T,
                                                       LayoutArg L, int nterms,
                                                       int32_t* __restrict__ nb,
+                                                      uint8_t* __restrict__ nbup,
                                                       int32_t* __restrict__ cnt) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= R) return;
@@ -266,13 +267,14 @@ __global__ void __launch_bounds__(kBlock) k_csr_count(const u64* __restrict__ ro
   int c[PACES_MAX_TERMS] = {0, 0, 0, 0};
   for (int k = 0; k < T.n; ++k) {
     Row<W> r = r0;
-    int nup;
+    int nup = 0;
     int32_t j = -1;
     if (apply_op<W>(T, k, L, r, nup)) {
       int64_t lb = lower_bound_row<W>(rows, 0, R, r);
       if (lb < R && eq_row<W>(r, rows + lb * W) && (bi || inB[lb])) j = (int32_t)lb;
     }
     nb[(int64_t)k * R + i] = j;
+    nbup[(int64_t)k * R + i] = (uint8_t)nup;
     if (j >= 0) {
 #pragma unroll
       for (int t = 0; t < PACES_MAX_TERMS; ++t) c[t] += (T.term[k] == t) ? 1 : 0;
@@ -293,50 +295,38 @@ struct CsrOut {
   double* data[PACES_MAX_TERMS];
 };
 
-template <int W>
-__global__ void __launch_bounds__(kBlock) k_csr_fill(const u64* __restrict__ rows, int64_t R,
-                                                     OpTable T, LayoutArg L, int nterms,
+// Emits each row's entries of every term in ascending column order by repeated minimum
+// selection over the (at most 8) neighbour slots of the term; the slots were filled by
+// k_csr_count with the neighbour index (or -1) and the upper phonon count of a vib pair.
+__global__ void __launch_bounds__(kBlock) k_csr_fill(int64_t R, OpTable T, int nterms,
                                                      const double* __restrict__ sqrt_tab,
                                                      const int32_t* __restrict__ nb,
+                                                     const uint8_t* __restrict__ nbup,
                                                      const int32_t* __restrict__ G, CsrOut out) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= R) return;
-  const Row<W> r0 = load_row<W>(rows, i);
   for (int t = 0; t < nterms; ++t) {
     const int32_t base0 = G[(int64_t)t * R];
     const int32_t base = G[(int64_t)t * R + i] - base0;
     out.indptr[t][i] = base;
     if (i == R - 1) out.indptr[t][R] = G[(int64_t)(t + 1) * R] - base0;
-    int32_t cj[8];
-    double cv[8];
-    int m = 0;
-    for (int k = 0; k < T.n; ++k) {
-      if (T.term[k] != t) continue;
-      int32_t j = nb[(int64_t)k * R + i];
-      if (j < 0) continue;
-      double v;
-      if (T.kind[k] == PACES_TERM_HOP) {
-        v = T.value[k];
-      } else {
-        Row<W> r = r0;
-        int nup = 0;
-        apply_op<W>(T, k, L, r, nup);
-        v = sqrt_tab[nup] * T.value[k];
+    int32_t last = -1;
+    for (int e = 0; e < 8; ++e) {
+      int32_t best = 0x7fffffff;
+      double bv = 0.0;
+      for (int k = 0; k < T.n; ++k) {
+        if (T.term[k] != t) continue;
+        const int32_t j = nb[(int64_t)k * R + i];
+        if (j > last && j < best) {
+          best = j;
+          bv = (T.kind[k] == PACES_TERM_HOP) ? T.value[k]
+                                             : sqrt_tab[nbup[(int64_t)k * R + i]] * T.value[k];
+        }
       }
-      // insertion into the ascending (column) list; at most 8 entries per term and row
-      int p = m;
-      while (p > 0 && cj[p - 1] > j) {
-        cj[p] = cj[p - 1];
-        cv[p] = cv[p - 1];
-        --p;
-      }
-      cj[p] = j;
-      cv[p] = v;
-      ++m;
-    }
-    for (int e = 0; e < m; ++e) {
-      out.indices[t][base + e] = cj[e];
-      out.data[t][base + e] = cv[e];
+      if (best == 0x7fffffff) break;
+      out.indices[t][base + e] = best;
+      out.data[t][base + e] = bv;
+      last = best;
     }
   }
 }
@@ -347,7 +337,7 @@ __global__ void k_csr_empty(CsrOut out, int nterms) {
 }
 
 static size_t csr_ws(int64_t R, int nops, int nterms) {
-  return a256((size_t)nops * R * 4) + a256((size_t)nterms * R * 4) +
+  return a256((size_t)nops * R * 4) + a256((size_t)nops * R) + a256((size_t)nterms * R * 4) +
          a256((size_t)(nterms * R + 1) * 4) + a256(scan_ws_bytes((int64_t)nterms * R));
 }
 
@@ -357,13 +347,15 @@ static int run_csr_count(const uint64_t* rows, int64_t R, const uint8_t* inB, co
   char* p = (char*)ws;
   int32_t* nb = (int32_t*)p;
   p += a256((size_t)T.n * R * 4);
+  uint8_t* nbup = (uint8_t*)p;
+  p += a256((size_t)T.n * R);
   int32_t* cnt = (int32_t*)p;
   p += a256((size_t)nterms * R * 4);
   int32_t* G = (int32_t*)p;
   p += a256((size_t)(nterms * R + 1) * 4);
   if (R > 0) {
     hipLaunchKernelGGL(k_csr_count<W>, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
-                       (const u64*)rows, R, inB, T, L, nterms, nb, cnt);
+                       (const u64*)rows, R, inB, T, L, nterms, nb, nbup, cnt);
     int rc = launch_status();
     if (rc) return rc;
   }
@@ -373,10 +365,8 @@ static int run_csr_count(const uint64_t* rows, int64_t R, const uint8_t* inB, co
   return launch_status();
 }
 
-template <int W>
-static int run_csr_fill(const uint64_t* rows, int64_t R, const LayoutArg& L, const OpTable& T,
-                        int nterms, const double* sqrt_tab, const CsrOut& out, void* ws,
-                        hipStream_t st) {
+static int run_csr_fill(int64_t R, const OpTable& T, int nterms, const double* sqrt_tab,
+                        const CsrOut& out, void* ws, hipStream_t st) {
   if (R == 0) {
     hipLaunchKernelGGL(k_csr_empty, dim3(1), dim3(64), 0, st, out, nterms);
     return launch_status();
@@ -384,10 +374,12 @@ static int run_csr_fill(const uint64_t* rows, int64_t R, const LayoutArg& L, con
   char* p = (char*)ws;
   const int32_t* nb = (const int32_t*)p;
   p += a256((size_t)T.n * R * 4);
+  const uint8_t* nbup = (const uint8_t*)p;
+  p += a256((size_t)T.n * R);
   p += a256((size_t)nterms * R * 4);
   const int32_t* G = (const int32_t*)p;
-  hipLaunchKernelGGL(k_csr_fill<W>, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
-                     (const u64*)rows, R, T, L, nterms, sqrt_tab, nb, G, out);
+  hipLaunchKernelGGL(k_csr_fill, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st, R, T, nterms,
+                     sqrt_tab, nb, nbup, G, out);
   return launch_status();
 }
 
@@ -641,8 +633,7 @@ int paces_holstein_csr_fill(const uint64_t* rows, int64_t R, const paces_layout_
     out.data[t] = t < h->nterms ? data[t] : nullptr;
     if (t < h->nterms && !out.indptr[t]) return PACES_E_INVALID;
   }
-  PACES_DISPATCH_W(lay->W, run_csr_fill, rows, R, larg(lay), T, h->nterms, h->d_sqrt_table, out,
-                   ws, (hipStream_t)stream);
+  return run_csr_fill(R, T, h->nterms, h->d_sqrt_table, out, ws, (hipStream_t)stream);
 }
 
 int paces_holstein_diag(const uint64_t* rows, int64_t R, const paces_layout_t* lay,

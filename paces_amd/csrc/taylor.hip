@@ -58,9 +58,60 @@ __global__ void k_tay_zero(paces_taylor_state_t* s) {
   s->j_ind = 0;
 }
 
+// Block-level max of two non-negative doubles given as bit patterns; thread 0 writes the pair
+// to part[2*blockIdx.x]. Non-negative doubles order like their bit patterns.
+__device__ __forceinline__ void block_max2_store(u64 a, u64 b, u64* __restrict__ part) {
+  __shared__ u64 sa[kBlock / 64], sb[kBlock / 64];
+  a = wave_max_u64(a);
+  b = wave_max_u64(b);
+  if ((threadIdx.x & 63) == 0) {
+    sa[threadIdx.x >> 6] = a;
+    sb[threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u64 ma = 0, mb = 0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) {
+      ma = sa[q] > ma ? sa[q] : ma;
+      mb = sb[q] > mb ? sb[q] : mb;
+    }
+    part[2 * blockIdx.x] = ma;
+    part[2 * blockIdx.x + 1] = mb;
+  }
+}
+
+// one workgroup: max over nblk partial pairs -> (m0, m1)
+__device__ __forceinline__ void reduce_partials(const u64* __restrict__ part, int nblk, u64* m0,
+                                                u64* m1) {
+  __shared__ u64 sa[1024 / 64], sb[1024 / 64];
+  u64 a = 0, b = 0;
+  for (int k = threadIdx.x; k < nblk; k += blockDim.x) {
+    u64 x = part[2 * k], y = part[2 * k + 1];
+    a = x > a ? x : a;
+    b = y > b ? y : b;
+  }
+  a = wave_max_u64(a);
+  b = wave_max_u64(b);
+  if ((threadIdx.x & 63) == 0) {
+    sa[threadIdx.x >> 6] = a;
+    sb[threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u64 ma = 0, mb = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+      ma = sa[q] > ma ? sa[q] : ma;
+      mb = sb[q] > mb ? sb[q] : mb;
+    }
+    *m0 = ma;
+    *m1 = mb;
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_tay_begin(int64_t R, const double2* __restrict__ v,
                                                       double2* __restrict__ res,
-                                                      paces_taylor_state_t* s) {
+                                                      u64* __restrict__ part) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   u64 m = 0;
   if (i < R) {
@@ -69,21 +120,22 @@ __global__ void __launch_bounds__(kBlock) k_tay_begin(int64_t R, const double2* 
     double q = a.x * a.x + a.y * a.y;
     m = (u64)__double_as_longlong(q);
   }
-  m = wave_max_u64(m);
-  if ((threadIdx.x & 63) == 0 && m) atomicMax(&s->max_term2, m);
+  block_max2_store(m, 0, part);
 }
 
-__global__ void k_tay_begin_fin(paces_taylor_state_t* s) {
-  s->c1 = sqrt(__longlong_as_double((long long)s->max_term2));
-  s->max_term2 = 0;
-  s->max_res2 = 0;
+__global__ void __launch_bounds__(1024) k_tay_begin_fin(paces_taylor_state_t* s,
+                                                        const u64* __restrict__ part, int nblk) {
+  u64 m0 = 0, m1 = 0;
+  reduce_partials(part, nblk, &m0, &m1);
+  if (threadIdx.x == 0) s->c1 = sqrt(__longlong_as_double((long long)m0));
 }
 
 __global__ void __launch_bounds__(kBlock) k_tay_term(int64_t R, const double* __restrict__ diag,
                                                      CsrSet H, const double2* __restrict__ xin,
                                                      double2* __restrict__ xout,
                                                      double2* __restrict__ res, double c,
-                                                     paces_taylor_state_t* s) {
+                                                     const paces_taylor_state_t* __restrict__ s,
+                                                     u64* __restrict__ part) {
   if (s->done) return;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   u64 mt = 0, mr = 0;
@@ -99,18 +151,18 @@ __global__ void __launch_bounds__(kBlock) k_tay_term(int64_t R, const double* __
     mt = (u64)__double_as_longlong(t.x * t.x + t.y * t.y);
     mr = (u64)__double_as_longlong(r.x * r.x + r.y * r.y);
   }
-  mt = wave_max_u64(mt);
-  mr = wave_max_u64(mr);
-  if ((threadIdx.x & 63) == 0) {
-    if (mt) atomicMax(&s->max_term2, mt);
-    if (mr) atomicMax(&s->max_res2, mr);
-  }
+  block_max2_store(mt, mr, part);
 }
 
-__global__ void k_tay_check(paces_taylor_state_t* s, int j, double tol) {
+__global__ void __launch_bounds__(1024) k_tay_check(paces_taylor_state_t* s,
+                                                    const u64* __restrict__ part, int nblk, int j,
+                                                    double tol) {
   if (s->done) return;
-  const double c2 = sqrt(__longlong_as_double((long long)s->max_term2));
-  const double rn = sqrt(__longlong_as_double((long long)s->max_res2));
+  u64 m0 = 0, m1 = 0;
+  reduce_partials(part, nblk, &m0, &m1);
+  if (threadIdx.x != 0) return;
+  const double c2 = sqrt(__longlong_as_double((long long)m0));
+  const double rn = sqrt(__longlong_as_double((long long)m1));
   const double c1pc2 = s->c1 + c2;
   const double ratio = c1pc2 / rn;
   s->c1_plus_c2 = c1pc2;
@@ -120,8 +172,6 @@ __global__ void k_tay_check(paces_taylor_state_t* s, int j, double tol) {
     s->done = 1;
   else
     s->c1 = c2;
-  s->max_term2 = 0;
-  s->max_res2 = 0;
 }
 
 // ---- deterministic reductions ------------------------------------------------------------------
@@ -238,29 +288,44 @@ int paces_spmv(int64_t R, const double* diag, const paces_csr_set_t* h, const do
   return launch_status();
 }
 
+size_t paces_taylor_workspace_bytes(int64_t R) {
+  int64_t nb = (R + kBlock - 1) / kBlock;
+  return a256((size_t)(nb < 1 ? 1 : nb) * 2 * sizeof(u64));
+}
+
 int paces_taylor_begin(int64_t R, const double* v, double* res, paces_taylor_state_t* state,
-                       void* stream) {
-  if (R < 0 || !state || (R > 0 && (!v || !res))) return PACES_E_INVALID;
+                       void* ws, size_t ws_bytes, void* stream) {
+  if (R < 0 || !state || !ws || (R > 0 && (!v || !res))) return PACES_E_INVALID;
+  if (ws_bytes < paces_taylor_workspace_bytes(R)) return PACES_E_WORKSPACE;
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_tay_zero, dim3(1), dim3(1), 0, st, state);
-  if (R > 0)
-    hipLaunchKernelGGL(k_tay_begin, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st, R,
-                       (const double2*)v, (double2*)res, state);
-  hipLaunchKernelGGL(k_tay_begin_fin, dim3(1), dim3(1), 0, st, state);
+  int nblk = 0;
+  if (R > 0) {
+    nblk = (int)grid_for(R, kBlock);
+    hipLaunchKernelGGL(k_tay_begin, dim3(nblk), dim3(kBlock), 0, st, R, (const double2*)v,
+                       (double2*)res, (u64*)ws);
+  }
+  hipLaunchKernelGGL(k_tay_begin_fin, dim3(1), dim3(1024), 0, st, state, (const u64*)ws, nblk);
   return launch_status();
 }
 
 int paces_taylor_term(int64_t R, const double* diag, const paces_csr_set_t* h, const double* x_in,
                       double* x_out, double* res, double c, int32_t j, double tol,
-                      paces_taylor_state_t* state, void* stream) {
+                      paces_taylor_state_t* state, void* ws, size_t ws_bytes, void* stream) {
   CsrSet H;
-  if (R < 0 || !state || !csr_set(h, &H)) return PACES_E_INVALID;
+  if (R < 0 || !state || !ws || !csr_set(h, &H)) return PACES_E_INVALID;
   if (R > 0 && (!diag || !x_in || !x_out || !res)) return PACES_E_INVALID;
+  if (ws_bytes < paces_taylor_workspace_bytes(R)) return PACES_E_WORKSPACE;
   hipStream_t st = (hipStream_t)stream;
-  if (R > 0)
-    hipLaunchKernelGGL(k_tay_term, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st, R, diag, H,
-                       (const double2*)x_in, (double2*)x_out, (double2*)res, c, state);
-  hipLaunchKernelGGL(k_tay_check, dim3(1), dim3(1), 0, st, state, (int)j, tol);
+  int nblk = 0;
+  if (R > 0) {
+    nblk = (int)grid_for(R, kBlock);
+    hipLaunchKernelGGL(k_tay_term, dim3(nblk), dim3(kBlock), 0, st, R, diag, H,
+                       (const double2*)x_in, (double2*)x_out, (double2*)res, c,
+                       (const paces_taylor_state_t*)state, (u64*)ws);
+  }
+  hipLaunchKernelGGL(k_tay_check, dim3(1), dim3(1024), 0, st, state, (const u64*)ws, nblk,
+                     (int)j, tol);
   return launch_status();
 }
 

@@ -83,11 +83,13 @@ class Hamiltonian(HamiltonianFramework):
         h.eps_sys = float(d["eps_sys"])
         h.hbar_omega = float(d["hbar_omega"])
         h.delta_eps = float(d.get("delta_eps", 0.0))
-        # sqrt(m) for the vibronic elements; NumPy's sqrt is correctly rounded
-        tab = numpy.sqrt(numpy.arange(int(self.max_ho_dims.max()) + 1, dtype=numpy.float64))
-        self._d_sqrt = torch.tensor(tab, dtype=torch.float64, device=self.device)
+        # sqrt(m) for the vibronic elements; NumPy's sqrt is correctly rounded. One table per
+        # Hamiltonian: every struct built from it points at the same device buffer.
+        if getattr(self, "_d_sqrt", None) is None:
+            tab = numpy.sqrt(numpy.arange(int(self.max_ho_dims.max()) + 1, dtype=numpy.float64))
+            self._d_sqrt = torch.tensor(tab, dtype=torch.float64, device=self.device)
         h.d_sqrt_table = self._d_sqrt.data_ptr()
-        h.sqrt_table_len = len(tab)
+        h.sqrt_table_len = int(self._d_sqrt.numel())
         return h
 
     # ---- field helpers (holstein.py:68-147) --------------------------------------------------------
