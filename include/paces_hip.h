@@ -48,6 +48,11 @@ typedef struct {
   const int32_t* d_off;    /* device [nfields] */
   const int32_t* d_width;  /* device [nfields] */
   const int32_t* d_maxdim; /* device [nfields] */
+  /* device [W * sum_bits]: mask of the bits of word w that carry significance 2^k inside a
+   * field 1..nfields-1 (all fields but the first); sum_i n_i = sum_{w,k} popcount(row[w] &
+   * mask[w*sum_bits+k]) << k. May be null for calls that do not sum fields. */
+  const uint64_t* d_sum_mask;
+  int32_t sum_bits;
 } paces_layout_t;
 
 /* One off-diagonal Holstein term, in use_terms order (paces/models/holstein.py:151-277). */
@@ -92,10 +97,13 @@ int paces_add_field(uint64_t* rows, int64_t n, const paces_layout_t* lay, int32_
  * out_idx[i] in [0, M]; out_hit (optional) = row equality at out_idx; miss_count (optional,
  * device int32) is incremented once per query not found -- the host-visible replacement of the
  * device assert of full_binary_noesc (cupy_search.py:205-228). q_sorted != 0 promises ascending
- * queries (every call on the step), which lets a wavefront narrow its search range. */
+ * queries (every call on the step): a partition pass then bounds each workgroup's range, which
+ * is staged in LDS; it needs a workspace of paces_lower_bound_workspace_bytes(Q) (ws may be null
+ * for unsorted queries). */
+size_t paces_lower_bound_workspace_bytes(int64_t Q);
 int paces_lower_bound(const uint64_t* book, int64_t M, const uint64_t* q, int64_t Q, int32_t W,
                       int32_t q_sorted, int32_t* out_idx, uint8_t* out_hit, int32_t* miss_count,
-                      void* stream);
+                      void* ws, size_t ws_bytes, void* stream);
 
 /* ---- A7: union of K sorted unique runs (cupy_unique, paces/aux/helpers.py:63-75, applied to
  * the concatenation built in hamiltonian.py:434-452 and 563-577). out_rows needs capacity for
@@ -164,9 +172,11 @@ int paces_holstein_observables(const uint64_t* rows, const double* psi, int64_t 
                                void* ws, size_t ws_bytes, void* stream);
 
 /* ---- G1: reassignment (paces/core/time_evolution.py:781-797): new_psi[i] = old_psi[j] where
- * old_rows[j] == new_rows[i], else 0. */
+ * old_rows[j] == new_rows[i], else 0. Both row sets sorted. */
+size_t paces_reassign_workspace_bytes(int64_t R);
 int paces_reassign(const uint64_t* old_rows, int64_t M, const double* old_psi,
-                   const uint64_t* new_rows, int64_t R, int32_t W, double* new_psi, void* stream);
+                   const uint64_t* new_rows, int64_t R, int32_t W, double* new_psi, void* ws,
+                   size_t ws_bytes, void* stream);
 
 /* ---- E1/E2: H.x and the Taylor series (total_ham / _simple_taylor,
  * paces/core/time_evolution.py:382-400, 482-512). FP64 parity order: per row

@@ -32,7 +32,20 @@ class PacesError(RuntimeError):
 
 class Layout(C.Structure):
     _fields_ = [("W", C.c_int32), ("nfields", C.c_int32), ("d_off", C.c_void_p),
-                ("d_width", C.c_void_p), ("d_maxdim", C.c_void_p)]
+                ("d_width", C.c_void_p), ("d_maxdim", C.c_void_p), ("d_sum_mask", C.c_void_p),
+                ("sum_bits", C.c_int32)]
+
+
+def sum_masks(offsets, widths, W):
+    """Per (word, in-field significance) masks over fields 1.. (see paces_layout_t)."""
+    kb = max([int(b) for b in widths[1:]] + [1])
+    m = np.zeros((W, kb), dtype=np.uint64)
+    for f in range(1, len(widths)):
+        b = int(widths[f])
+        for k in range(b):  # significance 2^k is bit (off + b - 1 - k) from the MSB
+            pos = int(offsets[f]) + b - 1 - k
+            m[pos // 64, k] |= np.uint64(1) << np.uint64(63 - pos % 64)
+    return m, kb
 
 
 class Term(C.Structure):
@@ -72,7 +85,8 @@ _SIGNATURES = {
     "paces_pack_rows": (C.c_int, [_P, _I64, C.POINTER(Layout), _P, _P]),
     "paces_get_field": (C.c_int, [_P, _I64, C.POINTER(Layout), _I32, _P, _P]),
     "paces_add_field": (C.c_int, [_P, _I64, C.POINTER(Layout), _I32, _I32, _P]),
-    "paces_lower_bound": (C.c_int, [_P, _I64, _P, _I64, _I32, _I32, _P, _P, _P, _P]),
+    "paces_lower_bound_workspace_bytes": (_SZ, [_I64]),
+    "paces_lower_bound": (C.c_int, [_P, _I64, _P, _I64, _I32, _I32, _P, _P, _P, _P, _SZ, _P]),
     "paces_merge_unique_workspace_bytes": (_SZ, [_I64, _I32]),
     "paces_merge_unique": (C.c_int, [C.POINTER(Runs), _I32, _P, _P, _P, _P, _P, _SZ, _P]),
     "paces_select_workspace_bytes": (_SZ, [_I64]),
@@ -93,7 +107,8 @@ _SIGNATURES = {
     "paces_holstein_observables_workspace_bytes": (_SZ, [_I64, C.POINTER(Holstein)]),
     "paces_holstein_observables": (C.c_int, [_P, _P, _I64, C.POINTER(Layout),
                                              C.POINTER(Holstein), _P, _P, _SZ, _P]),
-    "paces_reassign": (C.c_int, [_P, _I64, _P, _P, _I64, _I32, _P, _P]),
+    "paces_reassign_workspace_bytes": (_SZ, [_I64]),
+    "paces_reassign": (C.c_int, [_P, _I64, _P, _P, _I64, _I32, _P, _P, _SZ, _P]),
     "paces_spmv": (C.c_int, [_I64, _P, C.POINTER(CsrSet), _P, _P, _P]),
     "paces_taylor_workspace_bytes": (_SZ, [_I64]),
     "paces_taylor_begin": (C.c_int, [_I64, _P, _P, _P, _P, _SZ, _P]),

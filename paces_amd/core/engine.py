@@ -53,8 +53,9 @@ def csr_set(mats):
     return s
 
 
-def lower_bound(book, q, want_hit=True, count_misses=False):
-    """Lexicographic lower_bound of q's rows in the sorted book (int32 indices, uint8 hits)."""
+def lower_bound(book, q, want_hit=True, count_misses=False, q_sorted=False, ws=None):
+    """Lexicographic lower_bound of q's rows in the sorted book (int32 indices, uint8 hits).
+    q_sorted=True promises ascending queries and uses the partitioned LDS-staged kernel."""
     M, Q = book.shape[0], q.shape[0]
     W = q.shape[1]
     if book.shape[1] != W:
@@ -62,9 +63,14 @@ def lower_bound(book, q, want_hit=True, count_misses=False):
     idx = torch.empty(Q, dtype=torch.int32, device=q.device)
     hit = torch.empty(Q, dtype=torch.uint8, device=q.device) if want_hit else None
     miss = torch.zeros(1, dtype=torch.int32, device=q.device) if count_misses else None
-    check(lib.paces_lower_bound(ptr(book) if M else None, M, ptr(q) if Q else None, Q, W, 1,
-                                ptr(idx) if Q else None, ptr(hit) if (hit is not None and Q) else None,
-                                ptr(miss), stream()), "lower_bound")
+    wsp, wsn = None, 0
+    if q_sorted:
+        ws = ws if ws is not None else _hip.Workspace(q.device)
+        wsp, wsn = ws.get(lib.paces_lower_bound_workspace_bytes(Q))
+    check(lib.paces_lower_bound(ptr(book) if M else None, M, ptr(q) if Q else None, Q, W,
+                                1 if q_sorted else 0, ptr(idx) if Q else None,
+                                ptr(hit) if (hit is not None and Q) else None, ptr(miss), wsp, wsn,
+                                stream()), "lower_bound")
     return idx, hit, miss
 
 
@@ -103,14 +109,15 @@ def select_topk(psi, rows, k, ws):
     return out
 
 
-def reassign(old_rows, old_psi, new_rows):
+def reassign(old_rows, old_psi, new_rows, ws):
     R, W = new_rows.shape
     M = old_rows.shape[0]
     out = torch.empty(R, dtype=torch.complex128, device=new_rows.device)
     if R:
+        wsp, wsn = ws.get(lib.paces_reassign_workspace_bytes(R))
         check(lib.paces_reassign(ptr(old_rows) if M else None, M,
                                  ptr(torch.view_as_real(old_psi)) if M else None, ptr(new_rows), R,
-                                 W, ptr(torch.view_as_real(out)), stream()), "reassign")
+                                 W, ptr(torch.view_as_real(out)), wsp, wsn, stream()), "reassign")
     return out
 
 

@@ -82,8 +82,11 @@ class HamiltonianFramework:
         self._d_off = torch.tensor(self.bit_offsets, dtype=torch.int32, device=dev)
         self._d_width = torch.tensor(self.bitwidths, dtype=torch.int32, device=dev)
         self._d_maxdim = torch.tensor(self.max_dims, dtype=torch.int32, device=dev)
+        masks, kb = _hip.sum_masks(self.bit_offsets, self.bitwidths, self.totwordwidth)
+        self._d_sum_mask = torch.from_numpy(masks.view(numpy.int64).copy()).to(dev)
         self._layout = _hip.Layout(self.totwordwidth, self.n_sites, self._d_off.data_ptr(),
-                                   self._d_width.data_ptr(), self._d_maxdim.data_ptr())
+                                   self._d_width.data_ptr(), self._d_maxdim.data_ptr(),
+                                   self._d_sum_mask.data_ptr(), kb)
         self._ws = _hip.Workspace(dev)
 
         self.use_terms = use_terms

@@ -368,7 +368,7 @@ class TimeEvolutionFramework:
                                                                     enlarge_steps=enlarge_steps)
         self.numstates = new_inds.shape[0]
         self.create_matrices(diag_vals, mats)
-        self.vector = engine.reassign(self.whoami, self.vector, new_inds)
+        self.vector = engine.reassign(self.whoami, self.vector, new_inds, self._ws)
         self.whoami = new_inds
         return dbg, self._post_assignment_diagnostics()
 

@@ -85,6 +85,161 @@ __device__ __forceinline__ int64_t lower_bound_row(const u64* __restrict__ book,
   return lo;
 }
 
+// ---- block-cooperative lookup ---------------------------------------------------------------------
+// Every thread of a workgroup holds at most one query row (valid = false for none). The workgroup
+// reduces the lexicographic min and max of its valid queries, finds their lower bounds lo <= hi in
+// the sorted book with two searches, and stages book[lo, hi) in LDS when it fits; otherwise it
+// stages evenly spaced fence rows of that range, so each query finishes with an LDS search plus a
+// short search inside one fence window. Queries that are sorted or clustered (every call on the
+// step) give a narrow range, so the book is read once, coalesced. Must be called by every thread
+// of the workgroup (it synchronises).
+
+constexpr int kLkWords = 3072;  // 24 KiB of staged rows per workgroup
+
+template <int W>
+struct LookupSmem {
+  u64 rows[kLkWords];
+  u64 wmin[kBlock / 64][W];
+  u64 wmax[kBlock / 64][W];
+  int wany[kBlock / 64];
+  long long lo, hi, stride, nst;
+  int any;
+};
+
+template <int W>
+__device__ __forceinline__ bool row_less(const Row<W>& a, const Row<W>& b) {
+#pragma unroll
+  for (int k = 0; k < W; ++k)
+    if (a.w[k] != b.w[k]) return a.w[k] < b.w[k];
+  return false;
+}
+
+template <int W>
+__device__ __forceinline__ int64_t lower_bound_lds(const u64* rows, int64_t n, const Row<W>& q) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t m = (lo + hi) >> 1;
+    if (cmp_row<W>(q, rows + m * W) > 0)
+      lo = m + 1;
+    else
+      hi = m;
+  }
+  return lo;
+}
+
+template <int W>
+__device__ int64_t block_lower_bound(const Row<W>& q, bool valid, const u64* __restrict__ book,
+                                     int64_t M, LookupSmem<W>& sm) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  Row<W> mn = q, mx = q;
+  int v = valid ? 1 : 0;
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) {
+    Row<W> omn, omx;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      omn.w[k] = __shfl_xor(mn.w[k], s, 64);
+      omx.w[k] = __shfl_xor(mx.w[k], s, 64);
+    }
+    int ov = __shfl_xor(v, s, 64);
+    if (ov && (!v || row_less<W>(omn, mn))) mn = omn;
+    if (ov && (!v || row_less<W>(mx, omx))) mx = omx;
+    v |= ov;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      sm.wmin[wid][k] = mn.w[k];
+      sm.wmax[wid][k] = mx.w[k];
+    }
+    sm.wany[wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    // thread 0 searches the block minimum, thread 1 the block maximum
+    Row<W> best;
+    int have = 0;
+    for (int q2 = 0; q2 < kBlock / 64; ++q2) {
+      if (!sm.wany[q2]) continue;
+      Row<W> c;
+#pragma unroll
+      for (int k = 0; k < W; ++k) c.w[k] = threadIdx.x == 0 ? sm.wmin[q2][k] : sm.wmax[q2][k];
+      if (!have || (threadIdx.x == 0 ? row_less<W>(c, best) : row_less<W>(best, c))) best = c;
+      have = 1;
+    }
+    if (have) {
+      int64_t b = lower_bound_row<W>(book, 0, M, best);
+      if (threadIdx.x == 0)
+        sm.lo = b;
+      else
+        sm.hi = b;
+    }
+    if (threadIdx.x == 0) sm.any = have;
+  }
+  __syncthreads();
+  if (!sm.any) return 0;
+  const int64_t lo = sm.lo, hi = sm.hi;
+  const int64_t span = hi - lo;
+  constexpr int cap = kLkWords / W;
+  int64_t stride = 1, nst = span;
+  if (span > cap) {
+    stride = (span + cap - 1) / cap;
+    nst = (span + stride - 1) / stride;
+  }
+  // stage rows lo + f*stride, f < nst
+  for (int64_t e = threadIdx.x; e < nst * W; e += blockDim.x) {
+    int64_t f = e / W, k = e - f * W;
+    sm.rows[e] = book[(lo + f * stride) * W + k];
+  }
+  __syncthreads();
+  int64_t res = 0;
+  if (valid) {
+    int64_t fi = lower_bound_lds<W>(sm.rows, nst, q);
+    if (stride == 1) {
+      res = lo + fi;
+    } else {
+      int64_t a = (fi == 0) ? lo : lo + (fi - 1) * stride + 1;
+      int64_t b = (fi == nst) ? hi : lo + fi * stride;
+      res = lower_bound_row<W>(book, a, b, q);
+    }
+  }
+  __syncthreads();  // the staging area is reused by the next call
+  return res;
+}
+
+// Same as block_lower_bound for queries whose lower bounds are known to lie in [lo, hi] (the
+// range comes from a partition pass, so no thread searches the whole book here).
+template <int W>
+__device__ int64_t block_lower_bound_range(const Row<W>& q, bool valid,
+                                           const u64* __restrict__ book, int64_t lo, int64_t hi,
+                                           LookupSmem<W>& sm) {
+  const int64_t span = hi - lo;
+  constexpr int cap = kLkWords / W;
+  int64_t stride = 1, nst = span;
+  if (span > cap) {
+    stride = (span + cap - 1) / cap;
+    nst = (span + stride - 1) / stride;
+  }
+  for (int64_t e = threadIdx.x; e < nst * W; e += blockDim.x) {
+    int64_t f = e / W, k = e - f * W;
+    sm.rows[e] = book[(lo + f * stride) * W + k];
+  }
+  __syncthreads();
+  int64_t res = 0;
+  if (valid) {
+    int64_t fi = lower_bound_lds<W>(sm.rows, nst, q);
+    if (stride == 1) {
+      res = lo + fi;
+    } else {
+      int64_t a = (fi == 0) ? lo : lo + (fi - 1) * stride + 1;
+      int64_t b = (fi == nst) ? hi : lo + fi * stride;
+      res = lower_bound_row<W>(book, a, b, q);
+    }
+  }
+  __syncthreads();
+  return res;
+}
+
 // ---- bit fields ---------------------------------------------------------------------------------
 // A field at bit offset `off` (from the MSB of word 0) and width `b` (1..64).
 template <int W>

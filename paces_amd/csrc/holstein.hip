@@ -119,6 +119,8 @@ struct LayoutArg {
   const int32_t* off;
   const int32_t* width;
   const int32_t* maxdim;
+  const u64* sum_mask;
+  int sum_bits;
 };
 
 // Apply op k to row r. Returns false when the image does not exist (boundary or ceiling).
@@ -260,27 +262,34 @@ __global__ void __launch_bounds__(kBlock) k_csr_count(const u64* __restrict__ ro
                                                       int32_t* __restrict__ nb,
                                                       uint8_t* __restrict__ nbup,
                                                       int32_t* __restrict__ cnt) {
+  __shared__ LookupSmem<W> sm;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= R) return;
-  const Row<W> r0 = load_row<W>(rows, i);
-  const bool bi = inB[i] != 0;
+  const bool v = i < R;
+  Row<W> r0;
+#pragma unroll
+  for (int k = 0; k < W; ++k) r0.w[k] = 0;
+  if (v) r0 = load_row<W>(rows, i);
+  const bool bi = v && inB[i] != 0;
   int c[PACES_MAX_TERMS] = {0, 0, 0, 0};
   for (int k = 0; k < T.n; ++k) {
     Row<W> r = r0;
     int nup = 0;
+    const bool has = v && apply_op<W>(T, k, L, r, nup);
+    // the images of consecutive rows under one op are clustered, so the lookup stays local
+    const int64_t lb = block_lower_bound<W>(r, has, rows, R, sm);
     int32_t j = -1;
-    if (apply_op<W>(T, k, L, r, nup)) {
-      int64_t lb = lower_bound_row<W>(rows, 0, R, r);
-      if (lb < R && eq_row<W>(r, rows + lb * W) && (bi || inB[lb])) j = (int32_t)lb;
+    if (has && lb < R && eq_row<W>(r, rows + lb * W) && (bi || inB[lb])) j = (int32_t)lb;
+    if (v) {
+      nb[(int64_t)k * R + i] = j;
+      nbup[(int64_t)k * R + i] = (uint8_t)nup;
     }
-    nb[(int64_t)k * R + i] = j;
-    nbup[(int64_t)k * R + i] = (uint8_t)nup;
     if (j >= 0) {
 #pragma unroll
       for (int t = 0; t < PACES_MAX_TERMS; ++t) c[t] += (T.term[k] == t) ? 1 : 0;
     }
   }
-  for (int t = 0; t < nterms; ++t) cnt[(int64_t)t * R + i] = c[t];
+  if (v)
+    for (int t = 0; t < nterms; ++t) cnt[(int64_t)t * R + i] = c[t];
 }
 
 __global__ void k_csr_nnz(const int32_t* __restrict__ G, int64_t R, int nterms,
@@ -390,79 +399,105 @@ static int run_csr_fill(int64_t R, const OpTable& T, int nterms, const double* s
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_diag(const u64* __restrict__ rows, int64_t R,
-                                                 LayoutArg L, int nsites, double eps, double hw,
-                                                 double de, double* __restrict__ diag) {
+                                                 LayoutArg L, double eps, double hw, double de,
+                                                 double* __restrict__ diag) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= R) return;
   const Row<W> r = load_row<W>(rows, i);
+  // sum over all phonon fields as a weighted popcount of the bits of each significance
   long long s = 0;
-  for (int f = 1; f <= nsites; ++f) s += (long long)get_field<W>(r, L.off[f], L.width[f]);
+#pragma unroll
+  for (int w = 0; w < W; ++w)
+    for (int k = 0; k < L.sum_bits; ++k)
+      s += (long long)__popcll(r.w[w] & L.sum_mask[w * L.sum_bits + k]) << k;
   double d = eps + hw * (double)s;
   if (de != 0.0) d = d + de * (double)get_field<W>(r, 0, L.width[0]);
   diag[i] = d;
 }
 
 // ------------------------------------------------------------------------------------------------
-// H1 observables: per-workgroup partial sums (fixed order) then a fixed-order reduction.
+// H1 observables: a workgroup stages a tile of rows and |psi|^2 in LDS; each thread then owns one
+// or more sites and sums over the tile's rows (fixed order), so every partial is deterministic.
 // out[0..N) = n_exc, out[N..2N) = n_pho, out[2N] = sum |psi|^2.
 // ------------------------------------------------------------------------------------------------
 
-constexpr int kObsRows = 4;  // rows per thread
-constexpr int kObsTile = kBlock * kObsRows;
+constexpr int kObsLdsBytes = 32768;
+
+template <int W>
+struct ObsTile {
+  static constexpr int rows = (kObsLdsBytes / (8 * W + 12)) >= 1024 ? 1024
+                              : (kObsLdsBytes / (8 * W + 12)) >= 512 ? 512 : 256;
+};
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_obs_partial(const u64* __restrict__ rows,
                                                         const double2* __restrict__ psi, int64_t R,
                                                         LayoutArg L, int N,
                                                         double* __restrict__ part) {
-  extern __shared__ double lds[];  // [4 waves][2N+1]
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  constexpr int TR = ObsTile<W>::rows;
+  __shared__ u64 srow[TR * W];
+  __shared__ double sw[TR];
+  __shared__ int spos[TR];
+  __shared__ double sgrp[4 * 2 * PACES_MAX_SITES / 4 + 8];
+  __shared__ double swn[kBlock / 64];
   const int S = 2 * N + 1;
-  const int64_t base = (int64_t)blockIdx.x * kObsTile;
-  Row<W> rr[kObsRows];
-  double w[kObsRows];
-  int pos[kObsRows];
-#pragma unroll
-  for (int k = 0; k < kObsRows; ++k) {
-    int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
-    if (i < R) {
-      rr[k] = load_row<W>(rows, i);
-      double2 v = psi[i];
-      w[k] = v.x * v.x + v.y * v.y;
-      pos[k] = (int)get_field<W>(rr[k], 0, L.width[0]);
-    } else {
-#pragma unroll
-      for (int q = 0; q < W; ++q) rr[k].w[q] = 0;
-      w[k] = 0.0;
-      pos[k] = -1;
-    }
-  }
-  for (int s = 0; s < N; ++s) {
-    double ve = 0.0, vp = 0.0;
-    const int off = L.off[s + 1], b = L.width[s + 1];
-#pragma unroll
-    for (int k = 0; k < kObsRows; ++k) {
-      ve += (pos[k] == s) ? w[k] : 0.0;
-      vp += w[k] * (double)get_field<W>(rr[k], off, b);
-    }
-    ve = wave_sum(ve);
-    vp = wave_sum(vp);
-    if (lane == 0) {
-      lds[wid * S + s] = ve;
-      lds[wid * S + N + s] = vp;
-    }
-  }
-  double vn = 0.0;
-#pragma unroll
-  for (int k = 0; k < kObsRows; ++k) vn += w[k];
-  vn = wave_sum(vn);
-  if (lane == 0) lds[wid * S + 2 * N] = vn;
+  const int64_t base = (int64_t)blockIdx.x * TR;
+  const int nr = (int)((R - base) < TR ? (R - base) : TR);
+  for (int e = threadIdx.x; e < nr * W; e += blockDim.x) srow[e] = rows[base * W + e];
   __syncthreads();
-  for (int s = threadIdx.x; s < S; s += blockDim.x) {
+  double wsum = 0.0;
+  for (int r = threadIdx.x; r < nr; r += blockDim.x) {
+    double2 v = psi[base + r];
+    double w = v.x * v.x + v.y * v.y;
+    sw[r] = w;
+    wsum += w;
+    Row<W> rr;
+#pragma unroll
+    for (int k = 0; k < W; ++k) rr.w[k] = srow[r * W + k];
+    spos[r] = (int)get_field<W>(rr, 0, L.width[0]);
+  }
+  wsum = wave_sum(wsum);
+  if ((threadIdx.x & 63) == 0) swn[threadIdx.x >> 6] = wsum;
+  __syncthreads();
+  // groups of gsz threads split the tile's rows; each thread owns sites gt, gt + gsz, ...
+  const int gsz = N <= 64 ? 64 : (N <= 128 ? 128 : 256);
+  const int ngrp = kBlock / gsz;
+  const int grp = threadIdx.x / gsz, gt = threadIdx.x % gsz;
+  const int r0 = (nr * grp) / ngrp, r1 = (nr * (grp + 1)) / ngrp;
+  for (int s = gt; s < N; s += gsz) {
+    const int off = L.off[s + 1], b = L.width[s + 1];
+    const int k = off >> 6, o = off & 63;
+    const bool straddle = o + b > 64;
+    const u64 mask = (b >= 64) ? ~0ull : ((1ull << b) - 1ull);
+    double ae = 0.0, ap = 0.0;
+    for (int r = r0; r < r1; ++r) {
+      const double w = sw[r];
+      ae += (spos[r] == s) ? w : 0.0;
+      u64 n;
+      if (b == 0) {
+        n = 0;
+      } else if (straddle) {
+        const u64 hi = srow[r * W + k], lo = srow[r * W + k + 1];
+        n = ((hi << (o + b - 64)) | (lo >> (128 - o - b))) & mask;
+      } else {
+        n = (srow[r * W + k] >> (64 - o - b)) & mask;
+      }
+      ap += w * (double)n;
+    }
+    sgrp[grp * 2 * N + s] = ae;
+    sgrp[grp * 2 * N + N + s] = ap;
+  }
+  __syncthreads();
+  for (int s = threadIdx.x; s < 2 * N; s += blockDim.x) {
+    double a = 0.0;
+    for (int q = 0; q < ngrp; ++q) a += sgrp[q * 2 * N + s];
+    part[(int64_t)blockIdx.x * S + s] = a;
+  }
+  if (threadIdx.x == 0) {
     double a = 0.0;
 #pragma unroll
-    for (int q = 0; q < kBlock / 64; ++q) a += lds[q * S + s];
-    part[(int64_t)blockIdx.x * S + s] = a;
+    for (int q = 0; q < kBlock / 64; ++q) a += swn[q];
+    part[(int64_t)blockIdx.x * S + 2 * N] = a;
   }
 }
 
@@ -482,8 +517,14 @@ __global__ void __launch_bounds__(kBlock) k_obs_final(const double* __restrict__
   }
 }
 
+template <int W>
+static int64_t obs_blocks(int64_t R) {
+  return (R + ObsTile<W>::rows - 1) / ObsTile<W>::rows;
+}
+
 static size_t obs_ws(int64_t R, int N) {
-  int64_t nblk = (R + kObsTile - 1) / kObsTile;
+  // sized for the smallest tile (256 rows), valid for every W
+  int64_t nblk = (R + 255) / 256;
   if (nblk < 1) nblk = 1;
   return a256((size_t)nblk * (2 * N + 1) * 8);
 }
@@ -491,12 +532,11 @@ static size_t obs_ws(int64_t R, int N) {
 template <int W>
 static int run_obs(const uint64_t* rows, const double* psi, int64_t R, const LayoutArg& L, int N,
                    double* out, void* ws, hipStream_t st) {
-  int64_t nblk = (R + kObsTile - 1) / kObsTile;
+  int64_t nblk = obs_blocks<W>(R);
   const int S = 2 * N + 1;
   double* part = (double*)ws;
   if (nblk > 0) {
-    size_t lds = (size_t)(kBlock / 64) * S * sizeof(double);
-    hipLaunchKernelGGL(k_obs_partial<W>, dim3((unsigned)nblk), dim3(kBlock), lds, st,
+    hipLaunchKernelGGL(k_obs_partial<W>, dim3((unsigned)nblk), dim3(kBlock), 0, st,
                        (const u64*)rows, (const double2*)psi, R, L, N, part);
     int rc = launch_status();
     if (rc) return rc;
@@ -507,11 +547,11 @@ static int run_obs(const uint64_t* rows, const double* psi, int64_t R, const Lay
 }
 
 template <int W>
-static int run_diag(const uint64_t* rows, int64_t R, const LayoutArg& L, int N, double eps,
-                    double hw, double de, double* diag, hipStream_t st) {
+static int run_diag(const uint64_t* rows, int64_t R, const LayoutArg& L, double eps, double hw,
+                    double de, double* diag, hipStream_t st) {
   if (R == 0) return PACES_OK;
   hipLaunchKernelGGL(k_diag<W>, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st, (const u64*)rows,
-                     R, L, N, eps, hw, de, diag);
+                     R, L, eps, hw, de, diag);
   return launch_status();
 }
 
@@ -536,6 +576,8 @@ static LayoutArg larg(const paces_layout_t* lay) {
   L.off = lay->d_off;
   L.width = lay->d_width;
   L.maxdim = lay->d_maxdim;
+  L.sum_mask = (const u64*)lay->d_sum_mask;
+  L.sum_bits = lay->sum_bits;
   return L;
 }
 
@@ -641,8 +683,9 @@ int paces_holstein_diag(const uint64_t* rows, int64_t R, const paces_layout_t* l
   int rc = check_h(lay, h);
   if (rc) return rc;
   if (R < 0 || (R > 0 && (!rows || !diag))) return PACES_E_INVALID;
-  PACES_DISPATCH_W(lay->W, run_diag, rows, R, larg(lay), h->nchain, h->eps_sys, h->hbar_omega,
-                   h->delta_eps, diag, (hipStream_t)stream);
+  if (!lay->d_sum_mask || lay->sum_bits < 1 || lay->sum_bits > 32) return PACES_E_INVALID;
+  PACES_DISPATCH_W(lay->W, run_diag, rows, R, larg(lay), h->eps_sys, h->hbar_omega, h->delta_eps,
+                   diag, (hipStream_t)stream);
 }
 
 size_t paces_holstein_observables_workspace_bytes(int64_t R, const paces_holstein_t* h) {

@@ -94,20 +94,75 @@ __global__ void __launch_bounds__(kBlock) k_lower_bound(const u64* __restrict__ 
                                                         int32_t* __restrict__ out_idx,
                                                         uint8_t* __restrict__ out_hit,
                                                         int32_t* __restrict__ miss) {
+  __shared__ LookupSmem<W> sm;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= Q) return;
-  Row<W> r = load_row<W>(q, i);
-  int64_t lb = lower_bound_row<W>(book, 0, M, r);
+  const bool v = i < Q;
+  Row<W> r;
+#pragma unroll
+  for (int k = 0; k < W; ++k) r.w[k] = 0;
+  if (v) r = load_row<W>(q, i);
+  int64_t lb = block_lower_bound<W>(r, v, book, M, sm);
+  if (!v) return;
   bool hit = (lb < M) && eq_row<W>(r, book + lb * W);
   if (out_idx) out_idx[i] = (int32_t)lb;
   if (out_hit) out_hit[i] = hit ? 1 : 0;
   if (miss && !hit) atomicAdd(miss, 1);
 }
 
+// Partition pass for sorted queries: the lower bounds of the first and last query of every tile
+// of kBlock queries, one thread each, so all the searches run concurrently.
 template <int W>
-static int run_lb(const uint64_t* book, int64_t M, const uint64_t* q, int64_t Q, int32_t* idx,
-                  uint8_t* hit, int32_t* miss, hipStream_t st) {
+__global__ void __launch_bounds__(kBlock) k_tile_bounds(const u64* __restrict__ q, int64_t Q,
+                                                        const u64* __restrict__ book, int64_t M,
+                                                        int64_t ntiles,
+                                                        int64_t* __restrict__ lohi) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 2 * ntiles) return;
+  int64_t t = e >> 1;
+  int64_t qi = (e & 1) ? ((t + 1) * kBlock - 1 < Q - 1 ? (t + 1) * kBlock - 1 : Q - 1)
+                       : t * kBlock;
+  Row<W> r = load_row<W>(q, qi);
+  lohi[e] = lower_bound_row<W>(book, 0, M, r);
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_lower_bound_sorted(
+    const u64* __restrict__ book, int64_t M, const u64* __restrict__ q, int64_t Q,
+    const int64_t* __restrict__ lohi, int32_t* __restrict__ out_idx, uint8_t* __restrict__ out_hit,
+    int32_t* __restrict__ miss) {
+  __shared__ LookupSmem<W> sm;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool v = i < Q;
+  Row<W> r;
+#pragma unroll
+  for (int k = 0; k < W; ++k) r.w[k] = 0;
+  if (v) r = load_row<W>(q, i);
+  int64_t lb = block_lower_bound_range<W>(r, v, book, lohi[2 * blockIdx.x],
+                                          lohi[2 * blockIdx.x + 1], sm);
+  if (!v) return;
+  bool hit = (lb < M) && eq_row<W>(r, book + lb * W);
+  if (out_idx) out_idx[i] = (int32_t)lb;
+  if (out_hit) out_hit[i] = hit ? 1 : 0;
+  if (miss && !hit) atomicAdd(miss, 1);
+}
+
+size_t lb_ws_bytes(int64_t Q) { return (size_t)(2 * ((Q + kBlock - 1) / kBlock) + 2) * 8; }
+
+template <int W>
+static int run_lb(const uint64_t* book, int64_t M, const uint64_t* q, int64_t Q, int q_sorted,
+                  int32_t* idx, uint8_t* hit, int32_t* miss, void* ws, size_t ws_bytes,
+                  hipStream_t st) {
   if (Q == 0) return PACES_OK;
+  if (q_sorted && ws && ws_bytes >= lb_ws_bytes(Q)) {
+    int64_t nt = (Q + kBlock - 1) / kBlock;
+    int64_t* lohi = (int64_t*)ws;
+    hipLaunchKernelGGL(k_tile_bounds<W>, dim3(grid_for(2 * nt, kBlock)), dim3(kBlock), 0, st,
+                       (const u64*)q, Q, (const u64*)book, M, nt, lohi);
+    hipLaunchKernelGGL(k_lower_bound_sorted<W>, dim3((unsigned)nt), dim3(kBlock), 0, st,
+                       (const u64*)book, M, (const u64*)q, Q, (const int64_t*)lohi, idx, hit,
+                       miss);
+    return launch_status();
+  }
   hipLaunchKernelGGL(k_lower_bound<W>, dim3(grid_for(Q, kBlock)), dim3(kBlock), 0, st,
                      (const u64*)book, M, (const u64*)q, Q, idx, hit, miss);
   return launch_status();
@@ -125,6 +180,7 @@ static int run_lb(const uint64_t* book, int64_t M, const uint64_t* q, int64_t Q,
 struct RunsArg {
   const u64* ptr[PACES_MAX_RUNS];
   int64_t off[PACES_MAX_RUNS + 1];
+  int64_t boff[PACES_MAX_RUNS + 1];  // first workgroup of each run in k_mu_rank's grid
   int K;
 };
 
@@ -134,28 +190,61 @@ __device__ __forceinline__ int run_of(const RunsArg& R, int64_t g) {
   return r;
 }
 
+__device__ __forceinline__ int run_of_block(const RunsArg& R, int64_t b) {
+  int r = 0;
+  for (int s = 1; s < R.K; ++s) r += (b >= R.boff[s]) ? 1 : 0;
+  return r;
+}
+
+// Partition pass of the union: for every tile of every run and every other run s, the lower
+// bounds in s of the tile's first and last rows (one thread each).
 template <int W>
-__global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, int32_t* __restrict__ lbs,
-                                                    uint8_t* __restrict__ kept) {
-  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t C = R.off[R.K];
-  if (g >= C) return;
-  int r = run_of(R, g);
-  int64_t own = g - R.off[r];
+__global__ void __launch_bounds__(kBlock) k_mu_bounds(RunsArg R, int64_t* __restrict__ lohi) {
+  const int64_t nblk = R.boff[R.K];
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nblk * R.K * 2) return;
+  const int64_t b = e / (2 * R.K);
+  const int s = (int)((e / 2) % R.K);
+  const int r = run_of_block(R, b);
+  if (s == r) {
+    lohi[e] = 0;
+    return;
+  }
+  const int64_t len_r = R.off[r + 1] - R.off[r];
+  const int64_t t0 = (b - R.boff[r]) * kBlock;
+  const int64_t own = (e & 1) ? ((t0 + kBlock - 1 < len_r - 1) ? t0 + kBlock - 1 : len_r - 1) : t0;
   Row<W> x = load_row<W>(R.ptr[r], own);
+  lohi[e] = lower_bound_row<W>(R.ptr[s], 0, R.off[s + 1] - R.off[s], x);
+}
+
+// One workgroup per tile of one run: the tile's rows are sorted, so the rows of run s they can
+// land among form the narrow range the partition pass found; it is staged in LDS.
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, const int64_t* __restrict__ lohi,
+                                                    int32_t* __restrict__ lbs,
+                                                    uint8_t* __restrict__ kept) {
+  __shared__ LookupSmem<W> sm;
+  const int r = run_of_block(R, blockIdx.x);
+  const int64_t len_r = R.off[r + 1] - R.off[r];
+  const int64_t own = ((int64_t)blockIdx.x - R.boff[r]) * kBlock + threadIdx.x;
+  const bool v = own < len_r;
+  const int64_t g = R.off[r] + own;
+  Row<W> x;
+#pragma unroll
+  for (int k = 0; k < W; ++k) x.w[k] = 0;
+  if (v) x = load_row<W>(R.ptr[r], own);
   bool keep = true;
   for (int s = 0; s < R.K; ++s) {
-    int64_t len = R.off[s + 1] - R.off[s];
-    int64_t lb;
-    if (s == r) {
-      lb = own;
-    } else {
-      lb = lower_bound_row<W>(R.ptr[s], 0, len, x);
-      if (s < r && lb < len && eq_row<W>(x, R.ptr[s] + lb * W)) keep = false;
+    const int64_t len = R.off[s + 1] - R.off[s];
+    int64_t lb = own;
+    if (s != r) {
+      const int64_t* p = lohi + ((int64_t)blockIdx.x * R.K + s) * 2;
+      lb = block_lower_bound_range<W>(x, v, R.ptr[s], p[0], p[1], sm);
+      if (v && s < r && lb < len && eq_row<W>(x, R.ptr[s] + lb * W)) keep = false;
     }
-    lbs[g * R.K + s] = (int32_t)lb;
+    if (v) lbs[g * R.K + s] = (int32_t)lb;
   }
-  kept[g] = keep ? 1 : 0;
+  if (v) kept[g] = keep ? 1 : 0;
 }
 
 template <int W>
@@ -189,8 +278,10 @@ __global__ void k_write_zero(int32_t* p) { *p = 0; }
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t mu_ws_bytes(int64_t C, int K) {
+  // the partition table holds 2K int64 per tile; runs add at most K partial tiles
+  int64_t ntiles = (C + kBlock - 1) / kBlock + K;
   return align256((size_t)C * K * 4) + align256((size_t)C) + align256((size_t)(C + 1) * 4) +
-         align256(scan_ws_bytes(C));
+         align256((size_t)ntiles * K * 2 * 8) + align256(scan_ws_bytes(C));
 }
 
 template <int W>
@@ -209,7 +300,13 @@ static int run_mu(const RunsArg& R, uint64_t* out, int32_t* out_count, uint8_t* 
   p += align256((size_t)C);
   int32_t* G = (int32_t*)p;
   p += align256((size_t)(C + 1) * 4);
-  hipLaunchKernelGGL(k_mu_rank<W>, dim3(grid_for(C, kBlock)), dim3(kBlock), 0, st, R, lbs, kept);
+  int64_t* lohi = (int64_t*)p;
+  p += align256((size_t)((C + kBlock - 1) / kBlock + R.K) * R.K * 2 * 8);
+  const int64_t nblk = R.boff[R.K];
+  hipLaunchKernelGGL(k_mu_bounds<W>, dim3(grid_for(nblk * R.K * 2, kBlock)), dim3(kBlock), 0, st,
+                     R, lohi);
+  hipLaunchKernelGGL(k_mu_rank<W>, dim3((unsigned)nblk), dim3(kBlock), 0, st, R,
+                     (const int64_t*)lohi, lbs, kept);
   int rc = launch_status();
   if (rc) return rc;
   rc = scan_u8(kept, C, G, p, scan_ws_bytes(C), st);
@@ -228,23 +325,35 @@ template <int W>
 __global__ void __launch_bounds__(kBlock) k_reassign(const u64* __restrict__ old_rows, int64_t M,
                                                      const double2* __restrict__ old_psi,
                                                      const u64* __restrict__ new_rows, int64_t R,
+                                                     const int64_t* __restrict__ lohi,
                                                      double2* __restrict__ new_psi) {
+  __shared__ LookupSmem<W> sm;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= R) return;
-  Row<W> x = load_row<W>(new_rows, i);
-  int64_t lb = lower_bound_row<W>(old_rows, 0, M, x);
-  double2 v = make_double2(0.0, 0.0);
-  if (lb < M && eq_row<W>(x, old_rows + lb * W)) v = old_psi[lb];
-  new_psi[i] = v;
+  const bool v = i < R;
+  Row<W> x;
+#pragma unroll
+  for (int k = 0; k < W; ++k) x.w[k] = 0;
+  if (v) x = load_row<W>(new_rows, i);
+  int64_t lb = block_lower_bound_range<W>(x, v, old_rows, lohi[2 * blockIdx.x],
+                                          lohi[2 * blockIdx.x + 1], sm);
+  if (!v) return;
+  double2 val = make_double2(0.0, 0.0);
+  if (lb < M && eq_row<W>(x, old_rows + lb * W)) val = old_psi[lb];
+  new_psi[i] = val;
 }
 
 template <int W>
 static int run_reassign(const uint64_t* old_rows, int64_t M, const double* old_psi,
-                        const uint64_t* new_rows, int64_t R, double* new_psi, hipStream_t st) {
+                        const uint64_t* new_rows, int64_t R, double* new_psi, void* ws,
+                        hipStream_t st) {
   if (R == 0) return PACES_OK;
-  hipLaunchKernelGGL(k_reassign<W>, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
+  int64_t nt = (R + kBlock - 1) / kBlock;
+  int64_t* lohi = (int64_t*)ws;
+  hipLaunchKernelGGL(k_tile_bounds<W>, dim3(grid_for(2 * nt, kBlock)), dim3(kBlock), 0, st,
+                     (const u64*)new_rows, R, (const u64*)old_rows, M, nt, lohi);
+  hipLaunchKernelGGL(k_reassign<W>, dim3((unsigned)nt), dim3(kBlock), 0, st,
                      (const u64*)old_rows, M, (const double2*)old_psi, (const u64*)new_rows, R,
-                     (double2*)new_psi);
+                     (const int64_t*)lohi, (double2*)new_psi);
   return launch_status();
 }
 
@@ -277,13 +386,15 @@ int paces_add_field(uint64_t* rows, int64_t n, const paces_layout_t* lay, int32_
   PACES_DISPATCH_W(lay->W, run_add, rows, n, lay, field, delta, (hipStream_t)stream);
 }
 
+size_t paces_lower_bound_workspace_bytes(int64_t Q) { return lb_ws_bytes(Q); }
+
 int paces_lower_bound(const uint64_t* book, int64_t M, const uint64_t* q, int64_t Q, int32_t W,
                       int32_t q_sorted, int32_t* out_idx, uint8_t* out_hit, int32_t* miss_count,
-                      void* stream) {
-  (void)q_sorted;
+                      void* ws, size_t ws_bytes, void* stream) {
   if (M < 0 || Q < 0 || M >= 2147483647LL || W < 1) return PACES_E_INVALID;
   if (Q > 0 && (!q || (M > 0 && !book))) return PACES_E_INVALID;
-  PACES_DISPATCH_W(W, run_lb, book, M, q, Q, out_idx, out_hit, miss_count, (hipStream_t)stream);
+  PACES_DISPATCH_W(W, run_lb, book, M, q, Q, q_sorted, out_idx, out_hit, miss_count, ws, ws_bytes,
+                   (hipStream_t)stream);
 }
 
 size_t paces_merge_unique_workspace_bytes(int64_t total, int32_t K) {
@@ -306,18 +417,25 @@ int paces_merge_unique(const paces_runs_t* runs, int32_t W, uint64_t* out_rows,
     R.ptr[s] = nullptr;
     R.off[s + 1] = R.off[R.K];
   }
+  R.boff[0] = 0;
+  for (int s = 0; s < PACES_MAX_RUNS; ++s)
+    R.boff[s + 1] = R.boff[s] + (R.off[s + 1] - R.off[s] + kBlock - 1) / kBlock;
   if (R.off[R.K] >= 2147483647LL) return PACES_E_INVALID;
   if (R.off[R.K] > 0 && !out_rows) return PACES_E_INVALID;
   PACES_DISPATCH_W(W, run_mu, R, out_rows, out_count, out_from_run0, cand_pos, ws, ws_bytes,
                    (hipStream_t)stream);
 }
 
+size_t paces_reassign_workspace_bytes(int64_t R) { return lb_ws_bytes(R); }
+
 int paces_reassign(const uint64_t* old_rows, int64_t M, const double* old_psi,
-                   const uint64_t* new_rows, int64_t R, int32_t W, double* new_psi, void* stream) {
+                   const uint64_t* new_rows, int64_t R, int32_t W, double* new_psi, void* ws,
+                   size_t ws_bytes, void* stream) {
   if (M < 0 || R < 0 || W < 1) return PACES_E_INVALID;
-  if (R > 0 && (!new_rows || !new_psi || (M > 0 && (!old_rows || !old_psi))))
+  if (R > 0 && (!new_rows || !new_psi || !ws || (M > 0 && (!old_rows || !old_psi))))
     return PACES_E_INVALID;
-  PACES_DISPATCH_W(W, run_reassign, old_rows, M, old_psi, new_rows, R, new_psi,
+  if (ws_bytes < lb_ws_bytes(R)) return PACES_E_WORKSPACE;
+  PACES_DISPATCH_W(W, run_reassign, old_rows, M, old_psi, new_rows, R, new_psi, ws,
                    (hipStream_t)stream);
 }
 

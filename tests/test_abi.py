@@ -51,5 +51,5 @@ def test_ctypes_binding_covers_header_and_host_only_calls():
     assert lib.paces_holstein_num_runs(C.byref(h)) == 10
     assert lib.paces_scan_workspace_bytes(10 ** 7) > 0
     # argument validation runs on the host and returns a status, no device work
-    assert lib.paces_lower_bound(None, -1, None, 0, 1, 1, None, None, None, None) == -1
+    assert lib.paces_lower_bound(None, -1, None, 0, 1, 1, None, None, None, None, 0, None) == -1
     assert lib.paces_merge_unique(None, 1, None, None, None, None, None, 0, None) == -1

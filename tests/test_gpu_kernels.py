@@ -70,8 +70,11 @@ def test_lower_bound_bit_exact(env, W):
     absent = rng.integers(0, hi, size=(3000, W)).astype(np.uint64)
     q = np.concatenate([present, absent, book[:1], book[-1:], np.full((1, W), 2 ** 64 - 1,
                                                                       dtype=np.uint64)])
-    for qq in (q, po.unique_rows(q)):
-        idx, hit, miss = eng.lower_bound(_rows(env, book), _rows(env, qq), True, True)
+    for qq, srt in ((q, False), (po.unique_rows(q), False), (po.unique_rows(q), True),
+                    (np.sort(q.view(f"V{8 * W}").ravel()).view(np.uint64).reshape(-1, W), True)):
+        if srt:  # the sorted path needs ascending queries (duplicates allowed)
+            qq = qq[po.lexsort_rows(qq)]
+        idx, hit, miss = eng.lower_bound(_rows(env, book), _rows(env, qq), True, True, srt)
         exp_idx, exp_hit = po.lookup(book, qq)
         np.testing.assert_array_equal(idx.cpu().numpy(), exp_idx)
         np.testing.assert_array_equal(hit.cpu().numpy().astype(bool), exp_hit)
@@ -95,7 +98,7 @@ def test_pack_get_add_fields(env, dims):
     d_off = torch.tensor(lay.offsets, dtype=torch.int32, device=dev)
     d_w = torch.tensor(lay.bitwidths, dtype=torch.int32, device=dev)
     d_m = torch.tensor(lay.max_dims, dtype=torch.int32, device=dev)
-    L = hip.Layout(lay.W, lay.nfields, d_off.data_ptr(), d_w.data_ptr(), d_m.data_ptr())
+    L = hip.Layout(lay.W, lay.nfields, d_off.data_ptr(), d_w.data_ptr(), d_m.data_ptr(), None, 0)
     d_raw = torch.tensor(raw, dtype=torch.int32, device=dev)
     rows = torch.empty((500, lay.W), dtype=torch.int64, device=dev)
     hip.check(lib.paces_pack_rows(hip.ptr(d_raw), 500, C.byref(L), hip.ptr(rows), hip.stream()))
@@ -164,7 +167,8 @@ def test_reassign_and_reductions(env):
     new = _rand_unique_rows(rng, 60000, 3, hi=6)
     vec = rng.standard_normal(old.shape[0]) + 1j * rng.standard_normal(old.shape[0])
     d_old_vec = torch.from_numpy(vec).to(env["dev"])
-    out = eng.reassign(_rows(env, old), d_old_vec, _rows(env, new))
+    out = eng.reassign(_rows(env, old), d_old_vec, _rows(env, new),
+                       env["hip"].Workspace(env["dev"]))
     np.testing.assert_array_equal(out.cpu().numpy(), po.reassign(old, vec, new))
     red = eng.Reducer(env["dev"])
     assert abs(red.norm2(d_old_vec) - np.sum(np.abs(vec) ** 2)) <= 1e-12 * np.sum(np.abs(vec) ** 2)
