@@ -139,9 +139,30 @@ size_t paces_holstein_images_workspace_bytes(int64_t n, const paces_holstein_t* 
 int paces_holstein_images_count(const uint64_t* basis, int64_t n, const paces_layout_t* lay,
                                 const paces_holstein_t* h, int32_t* run_offsets, void* ws,
                                 size_t ws_bytes, void* stream);
+/* out_src (optional) gets the basis index each image came from, out_nup (optional) the upper
+ * phonon count of a vib pair (0 for hops). */
 int paces_holstein_images_emit(const uint64_t* basis, int64_t n, const paces_layout_t* lay,
-                               const paces_holstein_t* h, uint64_t* out_images, void* ws,
-                               size_t ws_bytes, void* stream);
+                               const paces_holstein_t* h, uint64_t* out_images, int32_t* out_src,
+                               uint8_t* out_nup, void* ws, size_t ws_bytes, void* stream);
+
+/* ---- D1/D2 from the union (no lookups): given the union of a basis (run 0, nbasis rows) and its
+ * image runs (paces_merge_unique with cand_pos), the per-term CSR on the R union rows. Each image
+ * of basis row b is the pair {b, image}: entries (b, image) and (image, b), except a minus-op
+ * image that is itself a basis row (the o_star rule of hamiltonian.py:363-374, holstein.py:201-206,
+ * 266-268; those pairs come from the image's plus op). run_offsets is a HOST array of nruns+1
+ * image offsets (from paces_holstein_images_count); in_basis is the union's out_from_run0.
+ * Same two-phase protocol as paces_holstein_csr_*; columns ascend within a row. */
+size_t paces_holstein_pairs_workspace_bytes(int64_t R, const paces_holstein_t* h);
+int paces_holstein_pairs_count(int64_t R, int64_t nbasis, const int64_t* run_offsets,
+                               const int32_t* cand_pos, const uint8_t* in_basis,
+                               const int32_t* img_src, const paces_holstein_t* h,
+                               int32_t* term_nnz, void* ws, size_t ws_bytes, void* stream);
+int paces_holstein_pairs_fill(int64_t R, int64_t nbasis, const int64_t* run_offsets,
+                              const int32_t* cand_pos, const uint8_t* in_basis,
+                              const int32_t* img_src, const uint8_t* img_nup,
+                              const paces_holstein_t* h, int32_t* const* indptr,
+                              int32_t* const* indices, double* const* data, void* ws,
+                              size_t ws_bytes, void* stream);
 
 /* ---- D1/D2: per-term canonical CSR of the Holstein H on the new basis (generate_mel +
  * _add_hc_and_generate_coo + coo_matrix().tocsr(): hamiltonian.py:383-547,

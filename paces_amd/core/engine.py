@@ -74,8 +74,9 @@ def lower_bound(book, q, want_hit=True, count_misses=False, q_sorted=False, ws=N
     return idx, hit, miss
 
 
-def merge_unique(runs, W, ws, want_from0=False):
-    """Sorted union of sorted unique row runs; returns (rows, from_run0 flags or None)."""
+def merge_unique(runs, W, ws, want_from0=False, want_cand_pos=False):
+    """Sorted union of sorted unique row runs; returns (rows, from_run0 flags or None) and, with
+    want_cand_pos, the output index of every input row as a third element."""
     if len(runs) > _hip.MAX_RUNS:
         raise ValueError("too many runs")
     R = _hip.Runs()
@@ -90,10 +91,13 @@ def merge_unique(runs, W, ws, want_from0=False):
     out = torch.empty((max(total, 1), W), dtype=torch.int64, device=dev)
     cnt = torch.empty(1, dtype=torch.int32, device=dev)
     from0 = torch.empty(max(total, 1), dtype=torch.uint8, device=dev) if want_from0 else None
+    cpos = torch.empty(max(total, 1), dtype=torch.int32, device=dev) if want_cand_pos else None
     wsp, wsn = ws.get(lib.paces_merge_unique_workspace_bytes(total, R.K))
-    check(lib.paces_merge_unique(C.byref(R), W, ptr(out), ptr(cnt), ptr(from0), None, wsp, wsn,
-                                 stream()), "merge_unique")
+    check(lib.paces_merge_unique(C.byref(R), W, ptr(out), ptr(cnt), ptr(from0), ptr(cpos), wsp,
+                                 wsn, stream()), "merge_unique")
     n = int(cnt.item())
+    if want_cand_pos:
+        return out[:n], (from0[:n] if want_from0 else None), cpos
     return out[:n], (from0[:n] if want_from0 else None)
 
 

@@ -193,7 +193,9 @@ __global__ void __launch_bounds__(kBlock) k_img_emit(const u64* __restrict__ bas
                                                      OpTable T, LayoutArg L,
                                                      const uint8_t* __restrict__ flags,
                                                      const int32_t* __restrict__ G,
-                                                     u64* __restrict__ out) {
+                                                     u64* __restrict__ out,
+                                                     int32_t* __restrict__ out_src,
+                                                     uint8_t* __restrict__ out_nup) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Row<W> r0 = load_row<W>(basis, i);
@@ -201,9 +203,12 @@ __global__ void __launch_bounds__(kBlock) k_img_emit(const u64* __restrict__ bas
     int64_t f = (int64_t)k * n + i;
     if (!flags[f]) continue;
     Row<W> r = r0;
-    int nup;
+    int nup = 0;
     apply_op<W>(T, k, L, r, nup);
-    store_row<W>(out, G[f], r);
+    const int64_t o = G[f];
+    store_row<W>(out, o, r);
+    if (out_src) out_src[o] = (int32_t)i;
+    if (out_nup) out_nup[o] = (uint8_t)nup;
   }
 }
 
@@ -236,7 +241,8 @@ static int run_img_count(const uint64_t* basis, int64_t n, const LayoutArg& L, c
 
 template <int W>
 static int run_img_emit(const uint64_t* basis, int64_t n, const LayoutArg& L, const OpTable& T,
-                        uint64_t* out, void* ws, hipStream_t st) {
+                        uint64_t* out, int32_t* out_src, uint8_t* out_nup, void* ws,
+                        hipStream_t st) {
   if (n == 0) return PACES_OK;
   int64_t m = (int64_t)T.n * n;
   char* p = (char*)ws;
@@ -244,7 +250,7 @@ static int run_img_emit(const uint64_t* basis, int64_t n, const LayoutArg& L, co
   p += a256((size_t)m);
   const int32_t* G = (const int32_t*)p;
   hipLaunchKernelGGL(k_img_emit<W>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
-                     (const u64*)basis, n, T, L, flags, G, (u64*)out);
+                     (const u64*)basis, n, T, L, flags, G, (u64*)out, out_src, out_nup);
   return launch_status();
 }
 
@@ -290,6 +296,11 @@ __global__ void __launch_bounds__(kBlock) k_csr_count(const u64* __restrict__ ro
   }
   if (v)
     for (int t = 0; t < nterms; ++t) cnt[(int64_t)t * R + i] = c[t];
+}
+
+__global__ void __launch_bounds__(kBlock) k_zero_i32(int32_t* __restrict__ p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
 }
 
 __global__ void k_csr_nnz(const int32_t* __restrict__ G, int64_t R, int nterms,
@@ -390,6 +401,110 @@ static int run_csr_fill(int64_t R, const OpTable& T, int nterms, const double* s
   hipLaunchKernelGGL(k_csr_fill, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st, R, T, nterms,
                      sqrt_tab, nb, nbup, G, out);
   return launch_status();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Per-term CSR from the union's candidate positions (no lookups). Every image candidate e of a
+// basis row b under op k is the pair {b, e}; it is an entry (b, e) and its H.c. (e, b) of term
+// term[k], except for a minus op whose image is itself a basis row: that pair is emitted by the
+// plus op of the image (the o_star rule, hamiltonian.py:363-374; holstein.py:201-206, 266-268).
+// Rows are counted with atomics, filled through atomic cursors, then each row is sorted by column,
+// which gives the canonical coo_matrix(...).tocsr() layout (time_evolution.py:369-371).
+// ------------------------------------------------------------------------------------------------
+
+struct ImgRuns {
+  int n;
+  int64_t off[kMaxOps + 1];  // image index range of each run (host copy of the run offsets)
+};
+
+__device__ __forceinline__ int img_run_of(const ImgRuns& IR, int64_t e) {
+  int k = 0;
+  for (int q = 1; q < IR.n; ++q) k += (e >= IR.off[q]) ? 1 : 0;
+  return k;
+}
+
+__global__ void __launch_bounds__(kBlock) k_pairs_count(int64_t R, int64_t nbasis, ImgRuns IR,
+                                                        OpTable T,
+                                                        const int32_t* __restrict__ cand_pos,
+                                                        const uint8_t* __restrict__ inB,
+                                                        const int32_t* __restrict__ src,
+                                                        int32_t* __restrict__ cnt) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= IR.off[IR.n]) return;
+  const int k = img_run_of(IR, e);
+  const int32_t p = cand_pos[nbasis + e];
+  if (T.dir[k] < 0 && inB[p]) return;
+  const int32_t q = cand_pos[src[e]];
+  const int64_t t = T.term[k];
+  atomicAdd(&cnt[t * R + q], 1);
+  atomicAdd(&cnt[t * R + p], 1);
+}
+
+// indptr of every term from the scan, and the fill cursors (absolute positions)
+__global__ void __launch_bounds__(kBlock) k_pairs_indptr(int64_t R, int nterms,
+                                                         const int32_t* __restrict__ G,
+                                                         CsrOut out, int32_t* __restrict__ cursor) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nterms * R) return;
+  const int t = (int)(e / R);
+  const int64_t i = e - (int64_t)t * R;
+  const int32_t base = G[e] - G[(int64_t)t * R];
+  out.indptr[t][i] = base;
+  cursor[e] = base;
+  if (i == R - 1) out.indptr[t][R] = G[(int64_t)(t + 1) * R] - G[(int64_t)t * R];
+}
+
+__global__ void __launch_bounds__(kBlock) k_pairs_fill(int64_t R, int64_t nbasis, ImgRuns IR,
+                                                       OpTable T,
+                                                       const int32_t* __restrict__ cand_pos,
+                                                       const uint8_t* __restrict__ inB,
+                                                       const int32_t* __restrict__ src,
+                                                       const uint8_t* __restrict__ nup,
+                                                       const double* __restrict__ sqrt_tab,
+                                                       int32_t* __restrict__ cursor, CsrOut out) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= IR.off[IR.n]) return;
+  const int k = img_run_of(IR, e);
+  const int32_t p = cand_pos[nbasis + e];
+  if (T.dir[k] < 0 && inB[p]) return;
+  const int32_t q = cand_pos[src[e]];
+  const int t = T.term[k];
+  const double v = (T.kind[k] == PACES_TERM_HOP) ? T.value[k] : sqrt_tab[nup[e]] * T.value[k];
+  const int32_t a = atomicAdd(&cursor[(int64_t)t * R + q], 1);
+  out.indices[t][a] = p;
+  out.data[t][a] = v;
+  const int32_t b = atomicAdd(&cursor[(int64_t)t * R + p], 1);
+  out.indices[t][b] = q;
+  out.data[t][b] = v;
+}
+
+// selection sort of each row's (few) entries by column, in place
+__global__ void __launch_bounds__(kBlock) k_pairs_sort(int64_t R, int nterms, CsrOut out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  for (int t = 0; t < nterms; ++t) {
+    const int32_t b = out.indptr[t][i], e = out.indptr[t][i + 1];
+    int32_t* ix = out.indices[t];
+    double* dv = out.data[t];
+    for (int32_t a = b; a + 1 < e; ++a) {
+      int32_t m = a;
+      for (int32_t c = a + 1; c < e; ++c)
+        if (ix[c] < ix[m]) m = c;
+      if (m != a) {
+        const int32_t tj = ix[a];
+        const double tv = dv[a];
+        ix[a] = ix[m];
+        dv[a] = dv[m];
+        ix[m] = tj;
+        dv[m] = tv;
+      }
+    }
+  }
+}
+
+static size_t pairs_ws(int64_t R, int nterms) {
+  int64_t m = (int64_t)nterms * R;
+  return a256((size_t)m * 4) + a256((size_t)(m + 1) * 4) + a256(scan_ws_bytes(m));
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -616,8 +731,8 @@ int paces_holstein_images_count(const uint64_t* basis, int64_t n, const paces_la
 }
 
 int paces_holstein_images_emit(const uint64_t* basis, int64_t n, const paces_layout_t* lay,
-                               const paces_holstein_t* h, uint64_t* out_images, void* ws,
-                               size_t ws_bytes, void* stream) {
+                               const paces_holstein_t* h, uint64_t* out_images, int32_t* out_src,
+                               uint8_t* out_nup, void* ws, size_t ws_bytes, void* stream) {
   int rc = check_h(lay, h);
   if (rc) return rc;
   OpTable T;
@@ -625,8 +740,106 @@ int paces_holstein_images_emit(const uint64_t* basis, int64_t n, const paces_lay
   if (rc) return rc;
   if (n < 0 || (n > 0 && (!basis || !out_images)) || !ws) return PACES_E_INVALID;
   if (ws_bytes < images_ws(n, T.n)) return PACES_E_WORKSPACE;
-  PACES_DISPATCH_W(lay->W, run_img_emit, basis, n, larg(lay), T, out_images, ws,
+  PACES_DISPATCH_W(lay->W, run_img_emit, basis, n, larg(lay), T, out_images, out_src, out_nup, ws,
                    (hipStream_t)stream);
+}
+
+static int img_runs_arg(const OpTable& T, const int64_t* run_offsets, ImgRuns* IR) {
+  IR->n = T.n;
+  for (int k = 0; k <= T.n; ++k) IR->off[k] = run_offsets[k];
+  for (int k = T.n + 1; k <= kMaxOps; ++k) IR->off[k] = run_offsets[T.n];
+  for (int k = 0; k < T.n; ++k)
+    if (IR->off[k + 1] < IR->off[k]) return PACES_E_INVALID;
+  return PACES_OK;
+}
+
+size_t paces_holstein_pairs_workspace_bytes(int64_t R, const paces_holstein_t* h) {
+  if (!h) return 0;
+  return pairs_ws(R, h->nterms);
+}
+
+int paces_holstein_pairs_count(int64_t R, int64_t nbasis, const int64_t* run_offsets,
+                               const int32_t* cand_pos, const uint8_t* in_basis,
+                               const int32_t* img_src, const paces_holstein_t* h,
+                               int32_t* term_nnz, void* ws, size_t ws_bytes, void* stream) {
+  OpTable T;
+  if (!h || !run_offsets || !term_nnz || !ws || R < 0 || nbasis < 0) return PACES_E_INVALID;
+  if (h->nterms < 1 || h->nterms > PACES_MAX_TERMS) return PACES_E_UNSUPPORTED;
+  int rc = build_image_ops(h, &T);
+  if (rc) return rc;
+  ImgRuns IR;
+  rc = img_runs_arg(T, run_offsets, &IR);
+  if (rc) return rc;
+  const int64_t nimg = IR.off[IR.n];
+  if (nimg > 0 && (!cand_pos || !in_basis || !img_src)) return PACES_E_INVALID;
+  if ((int64_t)h->nterms * R >= 2147483647LL) return PACES_E_INVALID;
+  if (ws_bytes < pairs_ws(R, h->nterms)) return PACES_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t m = (int64_t)h->nterms * R;
+  char* p = (char*)ws;
+  int32_t* cnt = (int32_t*)p;
+  p += a256((size_t)m * 4);
+  int32_t* G = (int32_t*)p;
+  p += a256((size_t)(m + 1) * 4);
+  if (m > 0) {
+    hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, st, cnt, m);
+  }
+  if (nimg > 0) {
+    hipLaunchKernelGGL(k_pairs_count, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, R,
+                       nbasis, IR, T, cand_pos, in_basis, img_src, cnt);
+  }
+  rc = launch_status();
+  if (rc) return rc;
+  rc = scan_i32(cnt, m, G, p, scan_ws_bytes(m), st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_csr_nnz, dim3(1), dim3(64), 0, st, (const int32_t*)G, R, h->nterms,
+                     term_nnz);
+  return launch_status();
+}
+
+int paces_holstein_pairs_fill(int64_t R, int64_t nbasis, const int64_t* run_offsets,
+                              const int32_t* cand_pos, const uint8_t* in_basis,
+                              const int32_t* img_src, const uint8_t* img_nup,
+                              const paces_holstein_t* h, int32_t* const* indptr,
+                              int32_t* const* indices, double* const* data, void* ws,
+                              size_t ws_bytes, void* stream) {
+  OpTable T;
+  if (!h || !run_offsets || !ws || !indptr || !indices || !data || R < 0) return PACES_E_INVALID;
+  if (h->nterms < 1 || h->nterms > PACES_MAX_TERMS) return PACES_E_UNSUPPORTED;
+  if (!h->d_sqrt_table) return PACES_E_INVALID;
+  int rc = build_image_ops(h, &T);
+  if (rc) return rc;
+  ImgRuns IR;
+  rc = img_runs_arg(T, run_offsets, &IR);
+  if (rc) return rc;
+  const int64_t nimg = IR.off[IR.n];
+  if (nimg > 0 && (!cand_pos || !in_basis || !img_src || !img_nup)) return PACES_E_INVALID;
+  if (ws_bytes < pairs_ws(R, h->nterms)) return PACES_E_WORKSPACE;
+  CsrOut out;
+  for (int t = 0; t < PACES_MAX_TERMS; ++t) {
+    out.indptr[t] = t < h->nterms ? indptr[t] : nullptr;
+    out.indices[t] = t < h->nterms ? indices[t] : nullptr;
+    out.data[t] = t < h->nterms ? data[t] : nullptr;
+    if (t < h->nterms && !out.indptr[t]) return PACES_E_INVALID;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  if (R == 0) {
+    hipLaunchKernelGGL(k_csr_empty, dim3(1), dim3(64), 0, st, out, h->nterms);
+    return launch_status();
+  }
+  const int64_t m = (int64_t)h->nterms * R;
+  char* p = (char*)ws;
+  int32_t* cursor = (int32_t*)p;  // the count array is no longer needed
+  p += a256((size_t)m * 4);
+  const int32_t* G = (const int32_t*)p;
+  hipLaunchKernelGGL(k_pairs_indptr, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, st, R,
+                     h->nterms, G, out, cursor);
+  if (nimg > 0)
+    hipLaunchKernelGGL(k_pairs_fill, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, R, nbasis,
+                       IR, T, cand_pos, in_basis, img_src, img_nup, h->d_sqrt_table, cursor, out);
+  hipLaunchKernelGGL(k_pairs_sort, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st, R, h->nterms,
+                     out);
+  return launch_status();
 }
 
 size_t paces_holstein_csr_workspace_bytes(int64_t R, const paces_holstein_t* h) {

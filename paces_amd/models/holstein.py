@@ -115,8 +115,9 @@ class Hamiltonian(HamiltonianFramework):
         self.add_n_at_site(arr, -1, -1)
 
     # ---- device expansion ---------------------------------------------------------------------------
-    def _images(self, basis, hstruct=None):
-        """Image runs of a sorted unique basis: (images tensor, run offsets list)."""
+    def _images(self, basis, hstruct=None, want_src=False):
+        """Image runs of a sorted unique basis: (images tensor, run offsets list) and, with
+        want_src, the source basis index and vib upper phonon count of every image."""
         h = self._hs if hstruct is None else hstruct
         n = basis.shape[0]
         nruns = _lib.paces_holstein_num_runs(C.byref(h))
@@ -129,9 +130,16 @@ class Hamiltonian(HamiltonianFramework):
         offs_h = offs.tolist()
         imgs = torch.empty((max(offs_h[-1], 1), self.totwordwidth), dtype=self.dtype,
                            device=self.device)
+        src = nup = None
+        if want_src:
+            src = torch.empty(max(offs_h[-1], 1), dtype=torch.int32, device=self.device)
+            nup = torch.empty(max(offs_h[-1], 1), dtype=torch.uint8, device=self.device)
         check(_lib.paces_holstein_images_emit(ptr(basis) if n else None, n,
-                                              C.byref(self._layout), C.byref(h), ptr(imgs), wsp,
-                                              wsn, stream()), "holstein_images_emit")
+                                              C.byref(self._layout), C.byref(h), ptr(imgs),
+                                              ptr(src), ptr(nup), wsp, wsn, stream()),
+              "holstein_images_emit")
+        if want_src:
+            return imgs, offs_h, src, nup
         return imgs, offs_h
 
     def _expand(self, basis, want_from0=False):
@@ -167,17 +175,48 @@ class Hamiltonian(HamiltonianFramework):
         basis = basis_states
         for _ in range(enlarge_steps):
             basis = self.enlarge_basis_set(basis)
-        new_inds, in_basis, offs = self._expand(basis, want_from0=True)
+        imgs, offs, src, nup = self._images(basis, want_src=True)
+        runs = [(basis, 0, basis.shape[0])]
+        runs += [(imgs, offs[k], offs[k + 1] - offs[k]) for k in range(len(offs) - 1)]
+        new_inds, in_basis, cand_pos = engine.merge_unique(runs, self.totwordwidth, self._ws,
+                                                           want_from0=True, want_cand_pos=True)
         self._check_basis_size(new_inds.shape[0])
         diag = self.generate_mel_diag(new_inds)
-        mats = self._csr(new_inds, in_basis)
+        mats = self._pairs_csr(new_inds.shape[0], basis.shape[0], offs, cand_pos, in_basis, src,
+                               nup)
         dbg = {}
         if "vib_coupling" in self.offdiag_terms:
             dbg["vib_coupling"] = [basis.shape[0] - self._vib_plus_count(offs)]
         self.last_basis_e = basis
         return (diag, new_inds), mats, dbg
 
+    def _pairs_csr(self, R, nbasis, offs, cand_pos, in_basis, src, nup):
+        """Per-term CSR from the union's candidate positions (paces_holstein_pairs_*)."""
+        h = self._hs
+        nterms = h.nterms
+        run_off = (C.c_int64 * len(offs))(*offs)
+        nnz = torch.empty(max(nterms, 1), dtype=torch.int32, device=self.device)
+        wsp, wsn = self._ws.get(_lib.paces_holstein_pairs_workspace_bytes(R, C.byref(h)))
+        check(_lib.paces_holstein_pairs_count(R, nbasis, run_off, ptr(cand_pos), ptr(in_basis),
+                                              ptr(src), C.byref(h), ptr(nnz), wsp, wsn, stream()),
+              "holstein_pairs_count")
+        nnz_h = nnz.tolist()
+        mats = {}
+        arrs = [(C.c_void_p * _hip.MAX_TERMS)() for _ in range(3)]
+        for t, name in enumerate(self.offdiag_terms):
+            ip = torch.empty(R + 1, dtype=torch.int32, device=self.device)
+            ix = torch.empty(max(nnz_h[t], 1), dtype=torch.int32, device=self.device)
+            dv = torch.empty(max(nnz_h[t], 1), dtype=torch.float64, device=self.device)
+            arrs[0][t], arrs[1][t], arrs[2][t] = ip.data_ptr(), ix.data_ptr(), dv.data_ptr()
+            mats[name] = CsrMatrix(ip, ix[:nnz_h[t]], dv[:nnz_h[t]], (R, R))
+        check(_lib.paces_holstein_pairs_fill(R, nbasis, run_off, ptr(cand_pos), ptr(in_basis),
+                                             ptr(src), ptr(nup), C.byref(h), arrs[0], arrs[1],
+                                             arrs[2], wsp, wsn, stream()), "holstein_pairs_fill")
+        return mats
+
     def _csr(self, rows, in_basis):
+        """Per-term CSR of the rows by neighbour lookups (paces_holstein_csr_*); equal to
+        _pairs_csr when in_basis marks the basis whose expansion gave the rows."""
         R = rows.shape[0]
         h = self._hs
         nterms = h.nterms

@@ -213,6 +213,13 @@ def test_expand_csr_diag_taylor_observables(env, case):
         np.testing.assert_array_equal(m.data.cpu().numpy(), dv)
     if "vib_coupling" in oham.offdiag_terms:
         assert dbg["vib_coupling"] == rec.dbg["vib_coupling"]
+    # the lookup-based CSR entry point agrees with the union-based one
+    _, in_b, _ = eng.lower_bound(ham.last_basis_e, new_inds, True, False, True)
+    mats2 = ham._csr(new_inds, in_b)
+    for name in oham.offdiag_terms:
+        np.testing.assert_array_equal(mats2[name].indptr.cpu().numpy(), rec.csrs[name][0])
+        np.testing.assert_array_equal(mats2[name].indices.cpu().numpy(), rec.csrs[name][1])
+        np.testing.assert_array_equal(mats2[name].data.cpu().numpy(), rec.csrs[name][2])
     csr_list = [rec.csrs[n] for n in oham.offdiag_terms]
     mat_list = [mats[n] for n in oham.offdiag_terms]
     v = rec.vec_adapted
