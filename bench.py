@@ -126,6 +126,7 @@ def main(argv=None):
     ap.add_argument("--workload", default="ring64", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=25.0)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported "
@@ -138,12 +139,15 @@ def main(argv=None):
     from paces_amd.models import holstein
 
     wl = WORKLOADS[a.workload]
-    devices.configure()
+    import contextlib
+    with contextlib.redirect_stdout(sys.stderr):  # stdout carries only the JSON line
+        devices.configure()
     ham = holstein.Hamiltonian(wl["N"], [wl["dim"]] * wl["N"], geo_dim=wl["geo"],
                                periodic=wl["periodic"], use_terms=use_terms(wl["J"], wl["g"]))
     te = holstein.TimeEvolution(ham, holstein.Observables, ["n_exc", "n_pho"], None,
                                 TimeEvoParams(maxstates=wl["ms"], enlarge_steps=wl["enlarge"],
-                                              shuffle_seed=None), ExpmParams())
+                                              shuffle_seed=None),
+                                ExpmParams(precision=a.precision))
     te.create_nonuniform_basis([1] * wl["N"], minpos=wl["pos0"], maxpos=wl["pos0"] + 1)
     te.grow_optimal_basis(fillfac=wl["fillfac"])
     raw = np.zeros((1, wl["N"] + 1), dtype=np.int64)
@@ -172,20 +176,25 @@ def main(argv=None):
     print(f"timed {a.steps} steps in {elapsed:.3f} s, basis {te.numstates}", file=sys.stderr,
           flush=True)
 
-    sec, nbytes = measure_taylor_kernel(te)
-    achieved = nbytes / sec / 1e9
+    sec, nbytes = measure_taylor_kernel(te) if a.precision == "fp64" else (None, None)
     peak = 8000.0
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": peak, "unit": "GB/s",
-            "frac": round(achieved / peak, 4), "traffic": None,
-            "kernel": "k_tay_term (paces_taylor_term)", "avg_us": round(sec * 1e6, 2),
-            "bytes_per_launch": int(nbytes), "rows": int(te.numstates)}
+    if sec is not None:
+        achieved = nbytes / sec / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": peak, "unit": "GB/s",
+                "frac": round(achieved / peak, 4), "traffic": None,
+                "kernel": "k_tay_term (paces_taylor_term)", "avg_us": round(sec * 1e6, 2),
+                "bytes_per_launch": int(nbytes), "rows": int(te.numstates)}
+    else:
+        roof = None
     cpu = None if a.no_cpu_baseline else cpu_baseline(wl, a.cpu_budget)
 
     line = {"metric": METRIC, "value": units / elapsed, "unit": "state-steps/s", "n_gpus": 1,
             "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * elapsed / a.steps,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64" if a.precision == "fp64" else "c64 terms / f64 state",
             "data": "synthetic",
-            "config": {"workload": f"{a.workload}: {wl['desc']}", "form": "plain",
+            "config": {"workload": f"{a.workload}: {wl['desc']}",
+                       "form": "plain" if a.precision == "fp64" else "plain-mixed",
                        "unit_of_work": "retained basis state advanced one step "
                                        "(min(maxstates, R_prev) per step)",
                        "r_state_steps_per_s": r_units / elapsed,

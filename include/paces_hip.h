@@ -235,6 +235,27 @@ int paces_taylor_term(int64_t R, const double* diag, const paces_csr_set_t* H, c
                       double* x_out, double* res, double c, int32_t j, double tol,
                       paces_taylor_state_t* state, void* ws, size_t ws_bytes, void* stream);
 
+/* ---- N2 mixed precision: complex64 series terms with FP32 matrix values and FP32 accumulation,
+ * summed into the complex128 result in FP64 (no reference counterpart; the FP64 path above is the
+ * reference's arithmetic). Same state and stopping test; tol is the caller's (2^-24 suits FP32
+ * terms). */
+typedef struct {
+  int32_t nterms;
+  const int32_t* indptr[PACES_MAX_TERMS];
+  const int32_t* indices[PACES_MAX_TERMS];
+  const float* data[PACES_MAX_TERMS];
+} paces_csr32_set_t;
+
+int paces_convert_f64_to_f32(int64_t n, const double* in, float* out, void* stream);
+/* res = v; x0 = (complex64) v; state.c1 = ||v||_inf */
+int paces_taylor_begin_mixed(int64_t R, const double* v, double* res, float* x0,
+                             paces_taylor_state_t* state, void* ws, size_t ws_bytes, void* stream);
+/* x_out = (-i*c) * (H32 x_in) in complex64; res += x_out in complex128; stopping test as above */
+int paces_taylor_term_mixed(int64_t R, const float* diag, const paces_csr32_set_t* H,
+                            const float* x_in, float* x_out, double* res, float c, int32_t j,
+                            double tol, paces_taylor_state_t* state, void* ws, size_t ws_bytes,
+                            void* stream);
+
 /* ---- deterministic vector reductions (norms, vdot) and Lanczos helpers (E3,
  * paces/core/time_evolution.py:515-549). out is a device double[2] (re, im) or double[1]. */
 size_t paces_reduce_workspace_bytes(int64_t R);

@@ -68,6 +68,11 @@ class CsrSet(C.Structure):
                 ("indices", C.c_void_p * MAX_TERMS), ("data", C.c_void_p * MAX_TERMS)]
 
 
+class CsrSet32(C.Structure):
+    _fields_ = [("nterms", C.c_int32), ("indptr", C.c_void_p * MAX_TERMS),
+                ("indices", C.c_void_p * MAX_TERMS), ("data", C.c_void_p * MAX_TERMS)]
+
+
 class TaylorState(C.Structure):
     _fields_ = [("max_term2", C.c_uint64), ("max_res2", C.c_uint64), ("c1", C.c_double),
                 ("c1_plus_c2", C.c_double), ("ratio", C.c_double), ("done", C.c_int32),
@@ -120,6 +125,10 @@ _SIGNATURES = {
     "paces_taylor_begin": (C.c_int, [_I64, _P, _P, _P, _P, _SZ, _P]),
     "paces_taylor_term": (C.c_int, [_I64, _P, C.POINTER(CsrSet), _P, _P, _P, _D, _I32, _D, _P,
                                     _P, _SZ, _P]),
+    "paces_convert_f64_to_f32": (C.c_int, [_I64, _P, _P, _P]),
+    "paces_taylor_begin_mixed": (C.c_int, [_I64, _P, _P, _P, _P, _P, _SZ, _P]),
+    "paces_taylor_term_mixed": (C.c_int, [_I64, _P, C.POINTER(CsrSet32), _P, _P, _P, C.c_float,
+                                          _I32, _D, _P, _P, _SZ, _P]),
     "paces_reduce_workspace_bytes": (_SZ, [_I64]),
     "paces_norm2": (C.c_int, [_P, _I64, _P, _P, _SZ, _P]),
     "paces_vdot": (C.c_int, [_P, _P, _I64, _P, _P, _SZ, _P]),

@@ -210,3 +210,67 @@ def simple_taylor(diag, mats, v, delta_t, m_star=100, tol=TAYLOR_TOL, chunk=6, w
         if host.done:
             break
     return res, [bool(host.done), int(host.j_ind), float(host.c1_plus_c2), float(host.ratio)]
+
+
+TAYLOR_TOL_MIXED = 5.960464477539063e-08  # 2**-24: FP32 terms carry no more
+
+
+def to_f32(t):
+    out = torch.empty(t.shape, dtype=torch.float32, device=t.device)
+    n = t.numel()
+    if n:
+        check(lib.paces_convert_f64_to_f32(n, ptr(t), ptr(out), stream()), "convert_f64_to_f32")
+    return out
+
+
+class MixedMats:
+    """FP32 copies of the diagonal and of each term's CSR values for the mixed-precision series."""
+
+    def __init__(self, diag, mats):
+        self.diag = to_f32(diag)
+        self.data = [to_f32(m.data) for m in mats]
+        s = _hip.CsrSet32()
+        s.nterms = len(mats)
+        for t, m in enumerate(mats):
+            s.indptr[t] = m.indptr.data_ptr()
+            s.indices[t] = m.indices.data_ptr() if m.nnz else m.indptr.data_ptr()
+            s.data[t] = self.data[t].data_ptr() if m.nnz else m.indptr.data_ptr()
+        self.cset = s
+
+
+def simple_taylor_mixed(mm: MixedMats, v, delta_t, m_star=100, tol=TAYLOR_TOL_MIXED, chunk=4,
+                        ws=None):
+    """The Taylor series with complex64 terms (FP32 matrix and accumulation) summed into a
+    complex128 result. Returns (res, [converged, j_ind, c1+c2, ratio])."""
+    R = v.shape[0]
+    res = torch.empty_like(v)
+    bufs = (torch.empty(R, dtype=torch.complex64, device=v.device),
+            torch.empty(R, dtype=torch.complex64, device=v.device),
+            torch.empty(R, dtype=torch.complex64, device=v.device))
+    state = torch.zeros(C.sizeof(_hip.TaylorState), dtype=torch.uint8, device=v.device)
+    st = stream()
+    ws = ws if ws is not None else _hip.Workspace(v.device)
+    wsp, wsn = ws.get(lib.paces_taylor_workspace_bytes(R))
+    check(lib.paces_taylor_begin_mixed(R, ptr(torch.view_as_real(v)), ptr(torch.view_as_real(res)),
+                                       ptr(torch.view_as_real(bufs[2])), ptr(state), wsp, wsn, st),
+          "taylor_begin_mixed")
+    x_in = bufs[2]
+    j = 0
+    host = None
+    while j < m_star:
+        for _ in range(chunk):
+            if j >= m_star:
+                break
+            x_out = bufs[j % 2]
+            check(lib.paces_taylor_term_mixed(R, ptr(mm.diag), C.byref(mm.cset),
+                                              ptr(torch.view_as_real(x_in)),
+                                              ptr(torch.view_as_real(x_out)),
+                                              ptr(torch.view_as_real(res)),
+                                              taylor_coeff(delta_t, j), j, tol, ptr(state), wsp,
+                                              wsn, st), "taylor_term_mixed")
+            x_in = x_out
+            j += 1
+        host = _hip.TaylorState.from_buffer_copy(bytes(state.cpu().numpy()))
+        if host.done:
+            break
+    return res, [bool(host.done), int(host.j_ind), float(host.c1_plus_c2), float(host.ratio)]

@@ -50,12 +50,19 @@ class CoeffSaveParams:
 
 @dataclass
 class ExpmParams:
-    """Matrix-exponential parameters (time_evolution.py:57-82)."""
+    """Matrix-exponential parameters (time_evolution.py:57-82).
+
+    Two fields extend the reference: ``precision`` ("fp64": the reference's complex128 series;
+    "mixed": complex64 terms with FP32 matrix values and accumulation, summed into the complex128
+    state) and ``taylor_tol`` (None: 2**-53 for fp64, 2**-24 for mixed).
+    """
     m_star: int = 100
     explosion_cutoff: float = 2.0
     use_scaling: bool = False
     renormalize: bool = False
     lanczos_order: int = 0
+    precision: str = "fp64"
+    taylor_tol: Optional[float] = None
 
 
 @dataclass
@@ -251,8 +258,18 @@ class TimeEvolutionFramework:
 
     # ---- propagators (482-549) ------------------------------------------------------------------------
     def _simple_taylor(self, delta_t):
-        res, dbg = engine.simple_taylor(self.diag_vals, self._mats(), self.vector, delta_t,
-                                        self.expm_params.m_star, ws=self._taylor_ws)
+        ep = self.expm_params
+        if ep.precision == "mixed":
+            mm = engine.MixedMats(self.diag_vals, self._mats())
+            tol = engine.TAYLOR_TOL_MIXED if ep.taylor_tol is None else ep.taylor_tol
+            res, dbg = engine.simple_taylor_mixed(mm, self.vector, delta_t, ep.m_star, tol,
+                                                  ws=self._taylor_ws)
+        elif ep.precision == "fp64":
+            tol = engine.TAYLOR_TOL if ep.taylor_tol is None else ep.taylor_tol
+            res, dbg = engine.simple_taylor(self.diag_vals, self._mats(), self.vector, delta_t,
+                                            ep.m_star, tol, ws=self._taylor_ws)
+        else:
+            raise ValueError(f"Unknown precision {ep.precision!r} (use 'fp64' or 'mixed').")
         if self.debug_verb > EXPM_EVO_LEVEL:
             print(f"         Taylor series: converged={dbg[0]} after j={dbg[1]}.")
         return res, dbg

@@ -86,7 +86,22 @@ __device__ __forceinline__ void reduce_partials(const u64* __restrict__ part, in
                                                 u64* m1) {
   __shared__ u64 sa[1024 / 64], sb[1024 / 64];
   u64 a = 0, b = 0;
-  for (int k = threadIdx.x; k < nblk; k += blockDim.x) {
+  // 8 independent loads in flight per thread
+  int k = threadIdx.x;
+  for (; k + 7 * (int)blockDim.x < nblk; k += 8 * blockDim.x) {
+    u64 x[8], y[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      x[q] = part[2 * (k + q * blockDim.x)];
+      y[q] = part[2 * (k + q * blockDim.x) + 1];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      a = x[q] > a ? x[q] : a;
+      b = y[q] > b ? y[q] : b;
+    }
+  }
+  for (; k < nblk; k += blockDim.x) {
     u64 x = part[2 * k], y = part[2 * k + 1];
     a = x > a ? x : a;
     b = y > b ? y : b;
@@ -172,6 +187,92 @@ __global__ void __launch_bounds__(1024) k_tay_check(paces_taylor_state_t* s,
     s->done = 1;
   else
     s->c1 = c2;
+}
+
+// ---- mixed precision: complex64 terms, FP32 matrix and accumulation, complex128 sum ----------
+
+struct CsrSet32 {
+  int nterms;
+  const int32_t* indptr[PACES_MAX_TERMS];
+  const int32_t* indices[PACES_MAX_TERMS];
+  const float* data[PACES_MAX_TERMS];
+};
+
+__device__ __forceinline__ float2 hx_row32(int64_t i, const float* __restrict__ diag,
+                                           const CsrSet32& H, const float2* __restrict__ x) {
+  const float2 xi = x[i];
+  const float d = diag[i];
+  float yr = d * xi.x, yi = d * xi.y;
+  for (int t = 0; t < H.nterms; ++t) {
+    const int32_t b = H.indptr[t][i], e = H.indptr[t][i + 1];
+    float tr = 0.0f, ti = 0.0f;
+    for (int32_t k = b; k < e; ++k) {
+      const float a = H.data[t][k];
+      const float2 xj = x[H.indices[t][k]];
+      tr = tr + a * xj.x;
+      ti = ti + a * xj.y;
+    }
+    yr = yr + tr;
+    yi = yi + ti;
+  }
+  return make_float2(yr, yi);
+}
+
+// res = v (complex128), x0 = v rounded to complex64, partial maxima of |v|^2
+__global__ void __launch_bounds__(kBlock) k_tay_begin_mixed(int64_t R,
+                                                            const double2* __restrict__ v,
+                                                            double2* __restrict__ res,
+                                                            float2* __restrict__ x0,
+                                                            u64* __restrict__ part) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  u64 m = 0;
+  if (i < R) {
+    double2 a = v[i];
+    res[i] = a;
+    x0[i] = make_float2((float)a.x, (float)a.y);
+    m = (u64)__double_as_longlong(a.x * a.x + a.y * a.y);
+  }
+  block_max2_store(m, 0, part);
+}
+
+__global__ void __launch_bounds__(kBlock) k_tay_term_mixed(
+    int64_t R, const float* __restrict__ diag, CsrSet32 H, const float2* __restrict__ xin,
+    float2* __restrict__ xout, double2* __restrict__ res, float c,
+    const paces_taylor_state_t* __restrict__ s, u64* __restrict__ part) {
+  if (s->done) return;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  u64 mt = 0, mr = 0;
+  if (i < R) {
+    const float2 h = hx_row32(i, diag, H, xin);
+    const float2 t = make_float2(h.y * c, -(h.x * c));
+    xout[i] = t;
+    double2 r = res[i];
+    r.x = r.x + (double)t.x;
+    r.y = r.y + (double)t.y;
+    res[i] = r;
+    const double tx = t.x, ty = t.y;
+    mt = (u64)__double_as_longlong(tx * tx + ty * ty);
+    mr = (u64)__double_as_longlong(r.x * r.x + r.y * r.y);
+  }
+  block_max2_store(mt, mr, part);
+}
+
+__global__ void __launch_bounds__(kBlock) k_f64_to_f32(int64_t n, const double* __restrict__ in,
+                                                       float* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)in[i];
+}
+
+static bool csr_set32(const paces_csr32_set_t* h, CsrSet32* H) {
+  if (!h || h->nterms < 0 || h->nterms > PACES_MAX_TERMS) return false;
+  H->nterms = h->nterms;
+  for (int t = 0; t < PACES_MAX_TERMS; ++t) {
+    H->indptr[t] = t < h->nterms ? h->indptr[t] : nullptr;
+    H->indices[t] = t < h->nterms ? h->indices[t] : nullptr;
+    H->data[t] = t < h->nterms ? h->data[t] : nullptr;
+    if (t < h->nterms && !H->indptr[t]) return false;
+  }
+  return true;
 }
 
 // ---- deterministic reductions ------------------------------------------------------------------
@@ -322,6 +423,52 @@ int paces_taylor_term(int64_t R, const double* diag, const paces_csr_set_t* h, c
     nblk = (int)grid_for(R, kBlock);
     hipLaunchKernelGGL(k_tay_term, dim3(nblk), dim3(kBlock), 0, st, R, diag, H,
                        (const double2*)x_in, (double2*)x_out, (double2*)res, c,
+                       (const paces_taylor_state_t*)state, (u64*)ws);
+  }
+  hipLaunchKernelGGL(k_tay_check, dim3(1), dim3(1024), 0, st, state, (const u64*)ws, nblk,
+                     (int)j, tol);
+  return launch_status();
+}
+
+int paces_convert_f64_to_f32(int64_t n, const double* in, float* out, void* stream) {
+  if (n < 0 || (n > 0 && (!in || !out))) return PACES_E_INVALID;
+  if (n == 0) return PACES_OK;
+  hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, (hipStream_t)stream,
+                     n, in, out);
+  return launch_status();
+}
+
+int paces_taylor_begin_mixed(int64_t R, const double* v, double* res, float* x0,
+                             paces_taylor_state_t* state, void* ws, size_t ws_bytes,
+                             void* stream) {
+  if (R < 0 || !state || !ws || (R > 0 && (!v || !res || !x0))) return PACES_E_INVALID;
+  if (ws_bytes < paces_taylor_workspace_bytes(R)) return PACES_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_tay_zero, dim3(1), dim3(1), 0, st, state);
+  int nblk = 0;
+  if (R > 0) {
+    nblk = (int)grid_for(R, kBlock);
+    hipLaunchKernelGGL(k_tay_begin_mixed, dim3(nblk), dim3(kBlock), 0, st, R, (const double2*)v,
+                       (double2*)res, (float2*)x0, (u64*)ws);
+  }
+  hipLaunchKernelGGL(k_tay_begin_fin, dim3(1), dim3(1024), 0, st, state, (const u64*)ws, nblk);
+  return launch_status();
+}
+
+int paces_taylor_term_mixed(int64_t R, const float* diag, const paces_csr32_set_t* h,
+                            const float* x_in, float* x_out, double* res, float c, int32_t j,
+                            double tol, paces_taylor_state_t* state, void* ws, size_t ws_bytes,
+                            void* stream) {
+  CsrSet32 H;
+  if (R < 0 || !state || !ws || !csr_set32(h, &H)) return PACES_E_INVALID;
+  if (R > 0 && (!diag || !x_in || !x_out || !res)) return PACES_E_INVALID;
+  if (ws_bytes < paces_taylor_workspace_bytes(R)) return PACES_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  int nblk = 0;
+  if (R > 0) {
+    nblk = (int)grid_for(R, kBlock);
+    hipLaunchKernelGGL(k_tay_term_mixed, dim3(nblk), dim3(kBlock), 0, st, R, diag, H,
+                       (const float2*)x_in, (float2*)x_out, (double2*)res, c,
                        (const paces_taylor_state_t*)state, (u64*)ws);
   }
   hipLaunchKernelGGL(k_tay_check, dim3(1), dim3(1024), 0, st, state, (const u64*)ws, nblk,
