@@ -70,6 +70,14 @@ typedef struct {
   double eps_sys, hbar_omega, delta_eps;
   const double* d_sqrt_table; /* device: sqrt(m) for m = 0..sqrt_table_len-1, correctly rounded */
   int32_t sqrt_table_len;
+  /* Batched trajectories (no reference counterpart): field 0 is the trajectory id, field 1 the
+   * exciton position and fields 2.. the phonons of nchain (= largest) sites. Each trajectory has
+   * its own lattice length and term values; geo_dim, periodic, the diagonal and the dims are
+   * shared. */
+  int32_t batched;
+  int32_t ntraj;
+  const int32_t* d_traj_L;    /* device [ntraj] */
+  const double* d_traj_value; /* device [ntraj * nterms], term values in use_terms order */
 } paces_holstein_t;
 
 /* ---- scans (prefix sums behind every mask compaction) ----------------------------------------- */
@@ -157,9 +165,11 @@ int paces_holstein_pairs_count(int64_t R, int64_t nbasis, const int64_t* run_off
                                const int32_t* cand_pos, const uint8_t* in_basis,
                                const int32_t* img_src, const paces_holstein_t* h,
                                int32_t* term_nnz, void* ws, size_t ws_bytes, void* stream);
+/* basis/lay are read only for batched trajectories (the trajectory of each source row). */
 int paces_holstein_pairs_fill(int64_t R, int64_t nbasis, const int64_t* run_offsets,
                               const int32_t* cand_pos, const uint8_t* in_basis,
                               const int32_t* img_src, const uint8_t* img_nup,
+                              const uint64_t* basis, const paces_layout_t* lay,
                               const paces_holstein_t* h, int32_t* const* indptr,
                               int32_t* const* indices, double* const* data, void* ws,
                               size_t ws_bytes, void* stream);
@@ -254,6 +264,50 @@ int paces_taylor_begin_mixed(int64_t R, const double* v, double* res, float* x0,
 int paces_taylor_term_mixed(int64_t R, const float* diag, const paces_csr32_set_t* H,
                             const float* x_in, float* x_out, double* res, float c, int32_t j,
                             double tol, paces_taylor_state_t* state, void* ws, size_t ws_bytes,
+                            void* stream);
+
+/* ---- N1 batched trajectories (no reference counterpart). Field 0 of every row is the
+ * trajectory id (ceil(log2 ntraj) bits), so trajectory t owns the contiguous segment
+ * [seg_off[t], seg_off[t+1]) of the sorted basis and H is block diagonal. Per trajectory:
+ * the top-k select (the rule of paces_select_topk with k_per_traj[t]), the observables and the
+ * Taylor stopping test; the series runs until every trajectory has stopped. */
+typedef struct {
+  double c1, c1_plus_c2, ratio;
+  int32_t done, j_ind;
+} paces_batch_taylor_t;
+typedef struct {
+  int32_t notdone; /* trajectories still running after the last term */
+  int32_t pad[3];
+} paces_batch_taylor_hdr_t;
+
+/* seg_off (device, ntraj+1) from sorted rows */
+int paces_batch_segments(const uint64_t* rows, int64_t R, int32_t W, int32_t ntraj,
+                         int32_t* seg_off, void* stream);
+size_t paces_batch_select_workspace_bytes(int64_t R, int32_t ntraj);
+/* out_rows needs capacity R; *out_count (device) = sum_t min(k_t, R_t) */
+int paces_batch_select_topk(const double* psi, const uint64_t* rows, int64_t R, int32_t W,
+                            const int32_t* seg_off, int32_t ntraj, const int64_t* k_per_traj,
+                            uint64_t* out_rows, int32_t* out_count, void* ws, size_t ws_bytes,
+                            void* stream);
+/* out: ntraj blocks of 2*nchain+1 doubles (n_exc, n_pho, sum |psi|^2); h->batched must be set */
+size_t paces_holstein_observables_batched_workspace_bytes(int64_t R, int32_t ntraj,
+                                                          int32_t nsites);
+int paces_holstein_observables_batched(const uint64_t* rows, const double* psi, int64_t R,
+                                       const int32_t* seg_off, const paces_layout_t* lay,
+                                       const paces_holstein_t* h, double* out, void* ws,
+                                       size_t ws_bytes, void* stream);
+size_t paces_batch_taylor_workspace_bytes(int64_t R, int32_t ntraj);
+/* res = v; x0 = v as complex64 (mixed != 0) or complex128; ts[t].c1 = ||v_t||_inf */
+int paces_batch_taylor_begin(int64_t R, const double* v, double* res, void* x0, int32_t mixed,
+                             const int32_t* seg_off, int32_t ntraj, paces_batch_taylor_t* ts,
+                             void* ws, size_t ws_bytes, void* stream);
+/* one term for the trajectories still running; exactly one of H64 (FP64 values, complex128
+ * terms, double diag) and H32 (FP32 values, complex64 terms, float diag) is non-null */
+int paces_batch_taylor_term(int64_t R, const void* diag, const paces_csr_set_t* H64,
+                            const paces_csr32_set_t* H32, const void* x_in, void* x_out,
+                            double* res, double c, int32_t j, double tol, const int32_t* seg_off,
+                            int32_t ntraj, paces_batch_taylor_t* ts,
+                            paces_batch_taylor_hdr_t* hdr, void* ws, size_t ws_bytes,
                             void* stream);
 
 /* ---- deterministic vector reductions (norms, vdot) and Lanczos helpers (E3,

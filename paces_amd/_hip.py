@@ -36,11 +36,11 @@ class Layout(C.Structure):
                 ("sum_bits", C.c_int32)]
 
 
-def sum_masks(offsets, widths, W):
-    """Per (word, in-field significance) masks over fields 1.. (see paces_layout_t)."""
-    kb = max([int(b) for b in widths[1:]] + [1])
+def sum_masks(offsets, widths, W, first=1):
+    """Per (word, in-field significance) masks over fields first.. (see paces_layout_t)."""
+    kb = max([int(b) for b in widths[first:]] + [1])
     m = np.zeros((W, kb), dtype=np.uint64)
-    for f in range(1, len(widths)):
+    for f in range(first, len(widths)):
         b = int(widths[f])
         for k in range(b):  # significance 2^k is bit (off + b - 1 - k) from the MSB
             pos = int(offsets[f]) + b - 1 - k
@@ -56,7 +56,9 @@ class Holstein(C.Structure):
     _fields_ = [("nchain", C.c_int32), ("lattice_l", C.c_int32), ("geo_dim", C.c_int32),
                 ("periodic", C.c_int32), ("nterms", C.c_int32), ("terms", Term * MAX_TERMS),
                 ("eps_sys", C.c_double), ("hbar_omega", C.c_double), ("delta_eps", C.c_double),
-                ("d_sqrt_table", C.c_void_p), ("sqrt_table_len", C.c_int32)]
+                ("d_sqrt_table", C.c_void_p), ("sqrt_table_len", C.c_int32),
+                ("batched", C.c_int32), ("ntraj", C.c_int32), ("d_traj_L", C.c_void_p),
+                ("d_traj_value", C.c_void_p)]
 
 
 class Runs(C.Structure):
@@ -71,6 +73,11 @@ class CsrSet(C.Structure):
 class CsrSet32(C.Structure):
     _fields_ = [("nterms", C.c_int32), ("indptr", C.c_void_p * MAX_TERMS),
                 ("indices", C.c_void_p * MAX_TERMS), ("data", C.c_void_p * MAX_TERMS)]
+
+
+class BatchTaylor(C.Structure):
+    _fields_ = [("c1", C.c_double), ("c1_plus_c2", C.c_double), ("ratio", C.c_double),
+                ("done", C.c_int32), ("j_ind", C.c_int32)]
 
 
 class TaylorState(C.Structure):
@@ -106,8 +113,9 @@ _SIGNATURES = {
     "paces_holstein_pairs_count": (C.c_int, [_I64, _I64, C.POINTER(C.c_int64), _P, _P, _P,
                                              C.POINTER(Holstein), _P, _P, _SZ, _P]),
     "paces_holstein_pairs_fill": (C.c_int, [_I64, _I64, C.POINTER(C.c_int64), _P, _P, _P, _P,
-                                            C.POINTER(Holstein), C.POINTER(_P), C.POINTER(_P),
-                                            C.POINTER(_P), _P, _SZ, _P]),
+                                            _P, C.POINTER(Layout), C.POINTER(Holstein),
+                                            C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), _P, _SZ,
+                                            _P]),
     "paces_holstein_csr_workspace_bytes": (_SZ, [_I64, C.POINTER(Holstein)]),
     "paces_holstein_csr_count": (C.c_int, [_P, _I64, _P, C.POINTER(Layout), C.POINTER(Holstein),
                                            _P, _P, _SZ, _P]),
@@ -129,6 +137,16 @@ _SIGNATURES = {
     "paces_taylor_begin_mixed": (C.c_int, [_I64, _P, _P, _P, _P, _P, _SZ, _P]),
     "paces_taylor_term_mixed": (C.c_int, [_I64, _P, C.POINTER(CsrSet32), _P, _P, _P, C.c_float,
                                           _I32, _D, _P, _P, _SZ, _P]),
+    "paces_batch_segments": (C.c_int, [_P, _I64, _I32, _I32, _P, _P]),
+    "paces_batch_select_workspace_bytes": (_SZ, [_I64, _I32]),
+    "paces_batch_select_topk": (C.c_int, [_P, _P, _I64, _I32, _P, _I32, _P, _P, _P, _P, _SZ, _P]),
+    "paces_holstein_observables_batched_workspace_bytes": (_SZ, [_I64, _I32, _I32]),
+    "paces_holstein_observables_batched": (C.c_int, [_P, _P, _I64, _P, C.POINTER(Layout),
+                                                     C.POINTER(Holstein), _P, _P, _SZ, _P]),
+    "paces_batch_taylor_workspace_bytes": (_SZ, [_I64, _I32]),
+    "paces_batch_taylor_begin": (C.c_int, [_I64, _P, _P, _P, _I32, _P, _I32, _P, _P, _SZ, _P]),
+    "paces_batch_taylor_term": (C.c_int, [_I64, _P, C.POINTER(CsrSet), C.POINTER(CsrSet32), _P,
+                                          _P, _P, _D, _I32, _D, _P, _I32, _P, _P, _P, _SZ, _P]),
     "paces_reduce_workspace_bytes": (_SZ, [_I64]),
     "paces_norm2": (C.c_int, [_P, _I64, _P, _P, _SZ, _P]),
     "paces_vdot": (C.c_int, [_P, _P, _I64, _P, _P, _SZ, _P]),

@@ -1,0 +1,681 @@
+// N1: many independent trajectories advanced together. All trajectories share one packed basis
+// whose field 0 is the trajectory id, so the basis of trajectory t is the contiguous segment
+// [seg_off[t], seg_off[t+1]) of the sorted rows, and H is block diagonal. The union, the
+// reassignment, the diagonal and the products run on the whole batch unchanged; this file holds
+// the per-trajectory parts: segments, the top-k select (_determine_select_whoami,
+// paces/core/time_evolution.py:833-910, per trajectory), the observables and the Taylor stopping
+// test (time_evolution.py:482-512, per trajectory). Workgroups are aligned to segments: block b
+// covers tile b - boff[t] of trajectory t, with boff[t] = sum_{u<t} ceil(R_u / kTile).
+#include "common.hpp"
+
+namespace paces {
+
+int scan_u8(const uint8_t* in, int64_t n, int32_t* out, void* ws, size_t ws_bytes, hipStream_t st);
+size_t scan_ws_bytes(int64_t n);
+
+static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+constexpr int kTile = kBlock;  // rows per segment-aligned workgroup
+
+// ---- segments -------------------------------------------------------------------------------------
+
+// seg_off[t] = first row whose trajectory id is >= t (t = 0..ntraj); rows are sorted
+template <int W>
+__global__ void k_seg_offsets(const u64* __restrict__ rows, int64_t R, int trajbits, int ntraj,
+                              int32_t* __restrict__ seg_off) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > ntraj) return;
+  if (t == ntraj) {
+    seg_off[t] = (int32_t)R;
+    return;
+  }
+  Row<W> q;
+#pragma unroll
+  for (int k = 0; k < W; ++k) q.w[k] = 0;
+  q.w[0] = trajbits ? ((u64)t << (64 - trajbits)) : 0;
+  seg_off[t] = (int32_t)lower_bound_row<W>(rows, 0, R, q);
+}
+
+// boff[t]: first workgroup of trajectory t; boff[ntraj] = number of workgroups
+__global__ void k_seg_blocks(const int32_t* __restrict__ seg_off, int ntraj, int tile,
+                             int64_t* __restrict__ boff) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int64_t b = 0;
+  for (int t = 0; t < ntraj; ++t) {
+    boff[t] = b;
+    b += (seg_off[t + 1] - seg_off[t] + tile - 1) / tile;
+  }
+  boff[ntraj] = b;
+}
+
+struct SegTile {
+  int t;        // trajectory, -1 if the workgroup is past the last tile
+  int64_t r0;   // first row of the tile
+  int64_t r1;   // one past the last row
+};
+
+__device__ __forceinline__ SegTile seg_tile(const int32_t* __restrict__ seg_off,
+                                            const int64_t* __restrict__ boff, int ntraj, int tile) {
+  SegTile st;
+  const int64_t b = blockIdx.x;
+  if (b >= boff[ntraj]) {
+    st.t = -1;
+    st.r0 = st.r1 = 0;
+    return st;
+  }
+  int lo = 0, hi = ntraj;  // last t with boff[t] <= b (empty trajectories share boff values)
+  while (hi - lo > 1) {
+    int m = (lo + hi) >> 1;
+    if (boff[m] <= b)
+      lo = m;
+    else
+      hi = m;
+  }
+  while (lo + 1 < ntraj && boff[lo + 1] <= b) ++lo;
+  st.t = lo;
+  st.r0 = seg_off[lo] + (b - boff[lo]) * tile;
+  int64_t e = st.r0 + tile;
+  st.r1 = e < seg_off[lo + 1] ? e : seg_off[lo + 1];
+  return st;
+}
+
+static unsigned seg_grid(int64_t R, int ntraj, int tile) {
+  return (unsigned)((R + tile - 1) / tile + ntraj + 1);
+}
+
+// ---- per-trajectory top-k select --------------------------------------------------------------------
+
+struct SegSel {
+  u64 prefix;
+  u64 mask;
+  long long krem;
+  int all;
+  int pad;
+};
+
+__global__ void k_bsel_keys(const double2* __restrict__ psi, int64_t R, u64* __restrict__ keys) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  double2 v = psi[i];
+  double k = v.x * v.x + v.y * v.y;
+  keys[i] = (u64)__double_as_longlong(k);
+}
+
+__global__ void k_bsel_init(const int32_t* __restrict__ seg_off, const int64_t* __restrict__ kpt,
+                            int ntraj, SegSel* __restrict__ st, unsigned* __restrict__ hist) {
+  const int t = blockIdx.x;
+  if (t >= ntraj) return;
+  hist[(int64_t)t * 256 + threadIdx.x] = 0;
+  if (threadIdx.x == 0) {
+    const long long Rt = seg_off[t + 1] - seg_off[t];
+    SegSel s;
+    s.prefix = 0;
+    s.mask = 0;
+    s.all = (kpt[t] >= Rt) ? 1 : 0;
+    s.krem = s.all ? 0 : kpt[t];
+    s.pad = 0;
+    st[t] = s;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_bsel_hist(const u64* __restrict__ keys,
+                                                      const int32_t* __restrict__ seg_off,
+                                                      const int64_t* __restrict__ boff, int ntraj,
+                                                      const SegSel* __restrict__ st, int shift,
+                                                      unsigned* __restrict__ hist) {
+  __shared__ unsigned lh[256];
+  const SegTile T = seg_tile(seg_off, boff, ntraj, kTile);
+  if (T.t < 0 || st[T.t].all || st[T.t].krem == 0) return;
+  lh[threadIdx.x] = 0;
+  __syncthreads();
+  const u64 prefix = st[T.t].prefix, mask = st[T.t].mask;
+  const int64_t i = T.r0 + threadIdx.x;
+  if (i < T.r1) {
+    u64 k = keys[i];
+    if ((k & mask) == prefix) atomicAdd(&lh[(k >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  unsigned c = lh[threadIdx.x];
+  if (c) atomicAdd(&hist[(int64_t)T.t * 256 + threadIdx.x], c);
+}
+
+__global__ void __launch_bounds__(256) k_bsel_pick(SegSel* __restrict__ st,
+                                                   unsigned* __restrict__ hist, int shift) {
+  __shared__ long long suf[257];
+  const int t = blockIdx.x, d = threadIdx.x;
+  unsigned* h = hist + (int64_t)t * 256;
+  const bool skip = st[t].all || st[t].krem == 0;
+  suf[d] = skip ? 0 : h[d];
+  if (d == 0) suf[256] = 0;
+  __syncthreads();
+  if (d == 0)
+    for (int u = 255; u >= 0; --u) suf[u] += suf[u + 1];
+  __syncthreads();
+  if (!skip) {
+    const long long krem = st[t].krem;
+    const bool mine = (suf[d] >= krem) && (suf[d + 1] < krem);
+    __syncthreads();
+    if (mine) {
+      st[t].prefix |= ((u64)d) << shift;
+      st[t].mask |= ((u64)255) << shift;
+      st[t].krem = krem - suf[d + 1];
+    }
+  }
+  h[d] = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_bsel_ties(const u64* __restrict__ keys,
+                                                      const int32_t* __restrict__ seg_off,
+                                                      const int64_t* __restrict__ boff, int ntraj,
+                                                      const SegSel* __restrict__ st,
+                                                      uint8_t* __restrict__ tie) {
+  const SegTile T = seg_tile(seg_off, boff, ntraj, kTile);
+  if (T.t < 0) return;
+  const int64_t i = T.r0 + threadIdx.x;
+  if (i >= T.r1) return;
+  tie[i] = (!st[T.t].all && keys[i] == st[T.t].prefix) ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_bsel_keep(const u64* __restrict__ keys,
+                                                      const int32_t* __restrict__ seg_off,
+                                                      const int64_t* __restrict__ boff, int ntraj,
+                                                      const SegSel* __restrict__ st,
+                                                      const int32_t* __restrict__ TG,
+                                                      uint8_t* __restrict__ keep) {
+  const SegTile T = seg_tile(seg_off, boff, ntraj, kTile);
+  if (T.t < 0) return;
+  const int64_t i = T.r0 + threadIdx.x;
+  if (i >= T.r1) return;
+  const SegSel s = st[T.t];
+  bool kp;
+  if (s.all) {
+    kp = true;
+  } else {
+    const int64_t a = seg_off[T.t], b = seg_off[T.t + 1];
+    const long long nt = TG[b] - TG[a];
+    const long long rank = TG[i] - TG[a];
+    const u64 k = keys[i];
+    kp = (k > s.prefix) || (k == s.prefix && rank >= nt - s.krem);
+  }
+  keep[i] = kp ? 1 : 0;
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_bsel_compact(const u64* __restrict__ rows, int64_t R,
+                                                         const uint8_t* __restrict__ keep,
+                                                         const int32_t* __restrict__ G,
+                                                         u64* __restrict__ out,
+                                                         int32_t* __restrict__ out_count) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *out_count = G[R];
+  if (i >= R || !keep[i]) return;
+  store_row<W>(out, G[i], load_row<W>(rows, i));
+}
+
+size_t bsel_ws_bytes(int64_t R, int ntraj) {
+  return a256((size_t)ntraj * sizeof(SegSel)) + a256((size_t)ntraj * 256 * 4) +
+         a256((size_t)(ntraj + 1) * 8) + a256((size_t)R * 8) + 2 * a256((size_t)R) +
+         2 * a256((size_t)(R + 1) * 4) + a256(scan_ws_bytes(R));
+}
+
+template <int W>
+static int run_bselect(const double* psi, const uint64_t* rows, int64_t R,
+                       const int32_t* seg_off, int ntraj, const int64_t* kpt, uint64_t* out_rows,
+                       int32_t* out_count, void* ws, hipStream_t st) {
+  char* p = (char*)ws;
+  SegSel* s = (SegSel*)p;
+  p += a256((size_t)ntraj * sizeof(SegSel));
+  unsigned* hist = (unsigned*)p;
+  p += a256((size_t)ntraj * 256 * 4);
+  int64_t* boff = (int64_t*)p;
+  p += a256((size_t)(ntraj + 1) * 8);
+  u64* keys = (u64*)p;
+  p += a256((size_t)R * 8);
+  uint8_t* tie = (uint8_t*)p;
+  p += a256((size_t)R);
+  uint8_t* keep = (uint8_t*)p;
+  p += a256((size_t)R);
+  int32_t* TG = (int32_t*)p;
+  p += a256((size_t)(R + 1) * 4);
+  int32_t* KG = (int32_t*)p;
+  p += a256((size_t)(R + 1) * 4);
+  const unsigned gs = seg_grid(R, ntraj, kTile);
+  hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(1), 0, st, seg_off, ntraj, kTile, boff);
+  if (R > 0)
+    hipLaunchKernelGGL(k_bsel_keys, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
+                       (const double2*)psi, R, keys);
+  hipLaunchKernelGGL(k_bsel_init, dim3(ntraj), dim3(256), 0, st, seg_off, kpt, ntraj, s, hist);
+  for (int pass = 0; pass < 8; ++pass) {
+    const int shift = 56 - 8 * pass;
+    hipLaunchKernelGGL(k_bsel_hist, dim3(gs), dim3(kBlock), 0, st, (const u64*)keys, seg_off,
+                       (const int64_t*)boff, ntraj, (const SegSel*)s, shift, hist);
+    hipLaunchKernelGGL(k_bsel_pick, dim3(ntraj), dim3(256), 0, st, s, hist, shift);
+  }
+  hipLaunchKernelGGL(k_bsel_ties, dim3(gs), dim3(kBlock), 0, st, (const u64*)keys, seg_off,
+                     (const int64_t*)boff, ntraj, (const SegSel*)s, tie);
+  int rc = launch_status();
+  if (rc) return rc;
+  rc = scan_u8(tie, R, TG, p, scan_ws_bytes(R), st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_bsel_keep, dim3(gs), dim3(kBlock), 0, st, (const u64*)keys, seg_off,
+                     (const int64_t*)boff, ntraj, (const SegSel*)s, (const int32_t*)TG, keep);
+  rc = launch_status();
+  if (rc) return rc;
+  rc = scan_u8(keep, R, KG, p, scan_ws_bytes(R), st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_bsel_compact<W>, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
+                     (const u64*)rows, R, (const uint8_t*)keep, (const int32_t*)KG, (u64*)out_rows,
+                     out_count);
+  return launch_status();
+}
+
+// ---- per-trajectory observables (Holstein, batched layout: field 0 trajectory, 1 position) -------
+
+constexpr int kObsTileB = 256;
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__ rows,
+                                                         const double2* __restrict__ psi,
+                                                         const int32_t* __restrict__ seg_off,
+                                                         const int64_t* __restrict__ boff,
+                                                         int ntraj, const int32_t* __restrict__ off,
+                                                         const int32_t* __restrict__ width, int N,
+                                                         double* __restrict__ part) {
+  __shared__ u64 srow[kObsTileB * W];
+  __shared__ double sw[kObsTileB];
+  __shared__ int spos[kObsTileB];
+  __shared__ double swn[kBlock / 64];
+  const SegTile T = seg_tile(seg_off, boff, ntraj, kObsTileB);
+  if (T.t < 0) return;
+  const int S = 2 * N + 1;
+  const int nr = (int)(T.r1 - T.r0);
+  for (int e = threadIdx.x; e < nr * W; e += blockDim.x) srow[e] = rows[T.r0 * W + e];
+  __syncthreads();
+  double wsum = 0.0;
+  for (int r = threadIdx.x; r < nr; r += blockDim.x) {
+    double2 v = psi[T.r0 + r];
+    double w = v.x * v.x + v.y * v.y;
+    sw[r] = w;
+    wsum += w;
+    Row<W> rr;
+#pragma unroll
+    for (int k = 0; k < W; ++k) rr.w[k] = srow[r * W + k];
+    spos[r] = (int)get_field<W>(rr, off[1], width[1]);
+  }
+  wsum = wave_sum(wsum);
+  if ((threadIdx.x & 63) == 0) swn[threadIdx.x >> 6] = wsum;
+  __syncthreads();
+  for (int s = threadIdx.x; s < N; s += blockDim.x) {
+    const int fo = off[s + 2], b = width[s + 2];
+    const int k = fo >> 6, o = fo & 63;
+    const bool straddle = o + b > 64;
+    const u64 mask = (b >= 64) ? ~0ull : ((1ull << b) - 1ull);
+    double ae = 0.0, ap = 0.0;
+    for (int r = 0; r < nr; ++r) {
+      const double w = sw[r];
+      ae += (spos[r] == s) ? w : 0.0;
+      u64 n;
+      if (b == 0) {
+        n = 0;
+      } else if (straddle) {
+        const u64 hi = srow[r * W + k], lo = srow[r * W + k + 1];
+        n = ((hi << (o + b - 64)) | (lo >> (128 - o - b))) & mask;
+      } else {
+        n = (srow[r * W + k] >> (64 - o - b)) & mask;
+      }
+      ap += w * (double)n;
+    }
+    part[(int64_t)blockIdx.x * S + s] = ae;
+    part[(int64_t)blockIdx.x * S + N + s] = ap;
+  }
+  if (threadIdx.x == 0) {
+    double a = 0.0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) a += swn[q];
+    part[(int64_t)blockIdx.x * S + 2 * N] = a;
+  }
+}
+
+// one wavefront per (trajectory, output): sum of the trajectory's workgroup partials
+__global__ void __launch_bounds__(64) k_bobs_final(const double* __restrict__ part,
+                                                   const int64_t* __restrict__ boff, int S,
+                                                   double* __restrict__ out) {
+  const int t = blockIdx.x / S, s = blockIdx.x % S;
+  double a = 0.0;
+  for (int64_t b = boff[t] + threadIdx.x; b < boff[t + 1]; b += 64) a += part[b * S + s];
+  a = wave_sum(a);
+  if (threadIdx.x == 0) out[(int64_t)t * S + s] = a;
+}
+
+size_t bobs_ws_bytes(int64_t R, int ntraj, int N) {
+  int64_t nblk = (R + kObsTileB - 1) / kObsTileB + ntraj + 1;
+  return a256((size_t)(ntraj + 1) * 8) + a256((size_t)nblk * (2 * N + 1) * 8);
+}
+
+template <int W>
+static int run_bobs(const uint64_t* rows, const double* psi, int64_t R, const int32_t* seg_off,
+                    int ntraj, const paces_layout_t* lay, int N, double* out, void* ws,
+                    hipStream_t st) {
+  char* p = (char*)ws;
+  int64_t* boff = (int64_t*)p;
+  p += a256((size_t)(ntraj + 1) * 8);
+  double* part = (double*)p;
+  const int S = 2 * N + 1;
+  hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(1), 0, st, seg_off, ntraj, kObsTileB, boff);
+  hipLaunchKernelGGL(k_bobs_partial<W>, dim3(seg_grid(R, ntraj, kObsTileB)), dim3(kBlock), 0, st,
+                     (const u64*)rows, (const double2*)psi, seg_off, (const int64_t*)boff, ntraj,
+                     lay->d_off, lay->d_width, N, part);
+  hipLaunchKernelGGL(k_bobs_final, dim3(ntraj * S), dim3(64), 0, st, (const double*)part,
+                     (const int64_t*)boff, S, out);
+  return launch_status();
+}
+
+// ---- batched Taylor series with a per-trajectory stopping test --------------------------------------
+
+struct CsrSetB64 {
+  int nterms;
+  const int32_t* indptr[PACES_MAX_TERMS];
+  const int32_t* indices[PACES_MAX_TERMS];
+  const void* data[PACES_MAX_TERMS];
+};
+
+template <typename F>
+struct Vec2;
+template <>
+struct Vec2<double> {
+  typedef double2 T;
+};
+template <>
+struct Vec2<float> {
+  typedef float2 T;
+};
+
+template <typename F>
+__device__ __forceinline__ typename Vec2<F>::T hx_row_t(int64_t i, const F* __restrict__ diag,
+                                                        const CsrSetB64& H,
+                                                        const typename Vec2<F>::T* __restrict__ x) {
+  typedef typename Vec2<F>::T V;
+  const V xi = x[i];
+  const F d = diag[i];
+  F yr = d * xi.x, yi = d * xi.y;
+  for (int t = 0; t < H.nterms; ++t) {
+    const int32_t b = H.indptr[t][i], e = H.indptr[t][i + 1];
+    const F* data = (const F*)H.data[t];
+    F tr = 0, ti = 0;
+    for (int32_t k = b; k < e; ++k) {
+      const F a = data[k];
+      const V xj = x[H.indices[t][k]];
+      tr = tr + a * xj.x;
+      ti = ti + a * xj.y;
+    }
+    yr = yr + tr;
+    yi = yi + ti;
+  }
+  V o;
+  o.x = yr;
+  o.y = yi;
+  return o;
+}
+
+__device__ __forceinline__ void blk_max2(u64 a, u64 b, u64* __restrict__ part) {
+  __shared__ u64 sa[kBlock / 64], sb[kBlock / 64];
+  a = wave_max_u64(a);
+  b = wave_max_u64(b);
+  if ((threadIdx.x & 63) == 0) {
+    sa[threadIdx.x >> 6] = a;
+    sb[threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u64 ma = 0, mb = 0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) {
+      ma = sa[q] > ma ? sa[q] : ma;
+      mb = sb[q] > mb ? sb[q] : mb;
+    }
+    part[2 * blockIdx.x] = ma;
+    part[2 * blockIdx.x + 1] = mb;
+  }
+}
+
+template <typename F>
+__global__ void __launch_bounds__(kBlock) k_btay_begin(const double2* __restrict__ v,
+                                                       double2* __restrict__ res,
+                                                       typename Vec2<F>::T* __restrict__ x0,
+                                                       const int32_t* __restrict__ seg_off,
+                                                       const int64_t* __restrict__ boff, int ntraj,
+                                                       u64* __restrict__ part) {
+  const SegTile T = seg_tile(seg_off, boff, ntraj, kTile);
+  if (T.t < 0) return;
+  const int64_t i = T.r0 + threadIdx.x;
+  u64 m = 0;
+  if (i < T.r1) {
+    double2 a = v[i];
+    res[i] = a;
+    typename Vec2<F>::T c;
+    c.x = (F)a.x;
+    c.y = (F)a.y;
+    x0[i] = c;
+    m = (u64)__double_as_longlong(a.x * a.x + a.y * a.y);
+  }
+  blk_max2(m, 0, part);
+}
+
+// per-trajectory reduction of the partials; fin == 1: initialise c1, else the stopping test
+__global__ void __launch_bounds__(64) k_btay_reduce(const u64* __restrict__ part,
+                                                    const int64_t* __restrict__ boff,
+                                                    paces_batch_taylor_t* __restrict__ ts,
+                                                    paces_batch_taylor_hdr_t* __restrict__ hdr,
+                                                    int fin, int j, double tol) {
+  const int t = blockIdx.x;
+  if (!fin && ts[t].done) return;
+  u64 a = 0, b = 0;
+  for (int64_t q = boff[t] + threadIdx.x; q < boff[t + 1]; q += 64) {
+    u64 x = part[2 * q], y = part[2 * q + 1];
+    a = x > a ? x : a;
+    b = y > b ? y : b;
+  }
+  a = wave_max_u64(a);
+  b = wave_max_u64(b);
+  if (threadIdx.x != 0) return;
+  const double m0 = sqrt(__longlong_as_double((long long)a));
+  if (fin) {
+    ts[t].c1 = m0;
+    ts[t].c1_plus_c2 = 0.0;
+    ts[t].ratio = 0.0;
+    ts[t].done = (boff[t + 1] == boff[t]) ? 1 : 0;  // an empty trajectory has nothing to do
+    ts[t].j_ind = 0;
+    return;
+  }
+  const double rn = sqrt(__longlong_as_double((long long)b));
+  const double c1pc2 = ts[t].c1 + m0;
+  const double ratio = c1pc2 / rn;
+  ts[t].c1_plus_c2 = c1pc2;
+  ts[t].ratio = ratio;
+  ts[t].j_ind = j;
+  if (ratio <= tol) {
+    ts[t].done = 1;
+  } else {
+    ts[t].c1 = m0;
+    atomicAdd(&hdr->notdone, 1);
+  }
+}
+
+template <typename F>
+__global__ void __launch_bounds__(kBlock) k_btay_term(
+    const F* __restrict__ diag, CsrSetB64 H, const typename Vec2<F>::T* __restrict__ xin,
+    typename Vec2<F>::T* __restrict__ xout, double2* __restrict__ res, F c,
+    const int32_t* __restrict__ seg_off, const int64_t* __restrict__ boff, int ntraj,
+    const paces_batch_taylor_t* __restrict__ ts, paces_batch_taylor_hdr_t* __restrict__ hdr,
+    u64* __restrict__ part) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) hdr->notdone = 0;
+  const SegTile T = seg_tile(seg_off, boff, ntraj, kTile);
+  if (T.t < 0 || ts[T.t].done) return;
+  const int64_t i = T.r0 + threadIdx.x;
+  u64 mt = 0, mr = 0;
+  if (i < T.r1) {
+    const typename Vec2<F>::T h = hx_row_t<F>(i, diag, H, xin);
+    typename Vec2<F>::T t;
+    t.x = h.y * c;
+    t.y = -(h.x * c);
+    xout[i] = t;
+    double2 r = res[i];
+    r.x = r.x + (double)t.x;
+    r.y = r.y + (double)t.y;
+    res[i] = r;
+    const double tx = t.x, ty = t.y;
+    mt = (u64)__double_as_longlong(tx * tx + ty * ty);
+    mr = (u64)__double_as_longlong(r.x * r.x + r.y * r.y);
+  }
+  blk_max2(mt, mr, part);
+}
+
+size_t btay_ws_bytes(int64_t R, int ntraj) {
+  int64_t nblk = (R + kTile - 1) / kTile + ntraj + 1;
+  return a256((size_t)(ntraj + 1) * 8) + a256((size_t)nblk * 2 * 8);
+}
+
+static bool csr_b(const paces_csr_set_t* h64, const paces_csr32_set_t* h32, CsrSetB64* H) {
+  const int n = h64 ? h64->nterms : (h32 ? h32->nterms : -1);
+  if (n < 0 || n > PACES_MAX_TERMS) return false;
+  H->nterms = n;
+  for (int t = 0; t < PACES_MAX_TERMS; ++t) {
+    H->indptr[t] = t < n ? (h64 ? h64->indptr[t] : h32->indptr[t]) : nullptr;
+    H->indices[t] = t < n ? (h64 ? h64->indices[t] : h32->indices[t]) : nullptr;
+    H->data[t] = t < n ? (h64 ? (const void*)h64->data[t] : (const void*)h32->data[t]) : nullptr;
+    if (t < n && !H->indptr[t]) return false;
+  }
+  return true;
+}
+
+}  // namespace paces
+
+using namespace paces;
+
+extern "C" {
+
+int paces_batch_segments(const uint64_t* rows, int64_t R, int32_t W, int32_t ntraj,
+                         int32_t* seg_off, void* stream) {
+  if (R < 0 || ntraj < 1 || !seg_off || (R > 0 && !rows)) return PACES_E_INVALID;
+  int trajbits = 0;
+  while ((1 << trajbits) < ntraj) ++trajbits;
+  hipStream_t st = (hipStream_t)stream;
+#define SEG_LAUNCH(WW)                                                                          \
+  hipLaunchKernelGGL(k_seg_offsets<WW>, dim3(grid_for(ntraj + 1, kBlock)), dim3(kBlock), 0, st, \
+                     (const u64*)rows, R, trajbits, ntraj, seg_off);                           \
+  return launch_status();
+  switch (W) {
+    case 1: SEG_LAUNCH(1)
+    case 2: SEG_LAUNCH(2)
+    case 3: SEG_LAUNCH(3)
+    case 4: SEG_LAUNCH(4)
+    case 5: SEG_LAUNCH(5)
+    case 6: SEG_LAUNCH(6)
+    case 7: SEG_LAUNCH(7)
+    case 8: SEG_LAUNCH(8)
+    case 9: SEG_LAUNCH(9)
+    case 10: SEG_LAUNCH(10)
+    case 11: SEG_LAUNCH(11)
+    case 12: SEG_LAUNCH(12)
+    case 13: SEG_LAUNCH(13)
+    case 14: SEG_LAUNCH(14)
+    case 15: SEG_LAUNCH(15)
+    case 16: SEG_LAUNCH(16)
+    default: return PACES_E_UNSUPPORTED;
+  }
+#undef SEG_LAUNCH
+}
+
+size_t paces_batch_select_workspace_bytes(int64_t R, int32_t ntraj) {
+  return bsel_ws_bytes(R, ntraj);
+}
+
+int paces_batch_select_topk(const double* psi, const uint64_t* rows, int64_t R, int32_t W,
+                            const int32_t* seg_off, int32_t ntraj, const int64_t* k_per_traj,
+                            uint64_t* out_rows, int32_t* out_count, void* ws, size_t ws_bytes,
+                            void* stream) {
+  if (R < 0 || ntraj < 1 || !seg_off || !k_per_traj || !out_count || !ws) return PACES_E_INVALID;
+  if (R > 0 && (!psi || !rows || !out_rows)) return PACES_E_INVALID;
+  if (R >= 2147483647LL) return PACES_E_INVALID;
+  if (ws_bytes < bsel_ws_bytes(R, ntraj)) return PACES_E_WORKSPACE;
+  PACES_DISPATCH_W(W, run_bselect, psi, rows, R, seg_off, ntraj, k_per_traj, out_rows, out_count,
+                   ws, (hipStream_t)stream);
+}
+
+size_t paces_holstein_observables_batched_workspace_bytes(int64_t R, int32_t ntraj,
+                                                          int32_t nsites) {
+  return bobs_ws_bytes(R, ntraj, nsites);
+}
+
+int paces_holstein_observables_batched(const uint64_t* rows, const double* psi, int64_t R,
+                                       const int32_t* seg_off, const paces_layout_t* lay,
+                                       const paces_holstein_t* h, double* out, void* ws,
+                                       size_t ws_bytes, void* stream) {
+  if (!lay || !h || !h->batched || !seg_off || !out || !ws || R < 0) return PACES_E_INVALID;
+  if (R > 0 && (!rows || !psi)) return PACES_E_INVALID;
+  if (lay->nfields != h->nchain + 2) return PACES_E_INVALID;
+  if (ws_bytes < bobs_ws_bytes(R, h->ntraj, h->nchain)) return PACES_E_WORKSPACE;
+  PACES_DISPATCH_W(lay->W, run_bobs, rows, psi, R, seg_off, h->ntraj, lay, h->nchain, out, ws,
+                   (hipStream_t)stream);
+}
+
+size_t paces_batch_taylor_workspace_bytes(int64_t R, int32_t ntraj) {
+  return btay_ws_bytes(R, ntraj);
+}
+
+int paces_batch_taylor_begin(int64_t R, const double* v, double* res, void* x0, int32_t mixed,
+                             const int32_t* seg_off, int32_t ntraj, paces_batch_taylor_t* ts,
+                             void* ws, size_t ws_bytes, void* stream) {
+  if (R < 0 || ntraj < 1 || !seg_off || !ts || !ws) return PACES_E_INVALID;
+  if (R > 0 && (!v || !res || !x0)) return PACES_E_INVALID;
+  if (ws_bytes < btay_ws_bytes(R, ntraj)) return PACES_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  char* p = (char*)ws;
+  int64_t* boff = (int64_t*)p;
+  p += a256((size_t)(ntraj + 1) * 8);
+  u64* part = (u64*)p;
+  hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(1), 0, st, seg_off, ntraj, kTile, boff);
+  const unsigned gs = seg_grid(R, ntraj, kTile);
+  if (mixed)
+    hipLaunchKernelGGL(k_btay_begin<float>, dim3(gs), dim3(kBlock), 0, st, (const double2*)v,
+                       (double2*)res, (float2*)x0, seg_off, (const int64_t*)boff, ntraj, part);
+  else
+    hipLaunchKernelGGL(k_btay_begin<double>, dim3(gs), dim3(kBlock), 0, st, (const double2*)v,
+                       (double2*)res, (double2*)x0, seg_off, (const int64_t*)boff, ntraj, part);
+  hipLaunchKernelGGL(k_btay_reduce, dim3(ntraj), dim3(64), 0, st, (const u64*)part,
+                     (const int64_t*)boff, ts, (paces_batch_taylor_hdr_t*)nullptr, 1, 0, 0.0);
+  return launch_status();
+}
+
+int paces_batch_taylor_term(int64_t R, const void* diag, const paces_csr_set_t* H64,
+                            const paces_csr32_set_t* H32, const void* x_in, void* x_out,
+                            double* res, double c, int32_t j, double tol, const int32_t* seg_off,
+                            int32_t ntraj, paces_batch_taylor_t* ts,
+                            paces_batch_taylor_hdr_t* hdr, void* ws, size_t ws_bytes,
+                            void* stream) {
+  CsrSetB64 H;
+  const int mixed = H32 != nullptr;
+  if (R < 0 || ntraj < 1 || !seg_off || !ts || !hdr || !ws || !csr_b(H64, H32, &H))
+    return PACES_E_INVALID;
+  if (R > 0 && (!diag || !x_in || !x_out || !res)) return PACES_E_INVALID;
+  if (ws_bytes < btay_ws_bytes(R, ntraj)) return PACES_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  char* p = (char*)ws;
+  const int64_t* boff = (const int64_t*)p;
+  p += a256((size_t)(ntraj + 1) * 8);
+  u64* part = (u64*)p;
+  const unsigned gs = seg_grid(R, ntraj, kTile);
+  if (mixed)
+    hipLaunchKernelGGL(k_btay_term<float>, dim3(gs), dim3(kBlock), 0, st, (const float*)diag, H,
+                       (const float2*)x_in, (float2*)x_out, (double2*)res, (float)c, seg_off,
+                       boff, ntraj, (const paces_batch_taylor_t*)ts, hdr, part);
+  else
+    hipLaunchKernelGGL(k_btay_term<double>, dim3(gs), dim3(kBlock), 0, st, (const double*)diag, H,
+                       (const double2*)x_in, (double2*)x_out, (double2*)res, c, seg_off, boff,
+                       ntraj, (const paces_batch_taylor_t*)ts, hdr, part);
+  hipLaunchKernelGGL(k_btay_reduce, dim3(ntraj), dim3(64), 0, st, (const u64*)part, boff, ts, hdr,
+                     0, (int)j, tol);
+  return launch_status();
+}
+
+}  // extern "C"

@@ -27,10 +27,39 @@ struct OpTable {
   int8_t dir[kMaxOps];
   int8_t wrap[kMaxOps];
   int8_t term[kMaxOps];
+  int8_t dim[kMaxOps];
+  int8_t order[kMaxOps];
   int32_t oe[kMaxOps];
   int32_t span[kMaxOps];
   double value[kMaxOps];
 };
+
+// Model parameters beyond the op table: where the exciton position lives and, for batched
+// trajectories, the per-trajectory lattice length and term values.
+struct ModelArg {
+  int pf;  // field index of the exciton position (0, or 1 when field 0 is the trajectory id)
+  int batched;
+  int nterms;
+  const int32_t* traj_L;
+  const double* traj_val;
+};
+
+// width of the trajectory-id field: ceil(log2(ntraj))
+static int tbits(int ntraj) {
+  int b = 0;
+  while ((1 << b) < ntraj) ++b;
+  return b;
+}
+
+static ModelArg marg(const paces_holstein_t* h) {
+  ModelArg M;
+  M.batched = h->batched ? 1 : 0;
+  M.pf = M.batched;
+  M.nterms = h->nterms;
+  M.traj_L = h->d_traj_L;
+  M.traj_val = h->d_traj_value;
+  return M;
+}
 
 // Image runs (enlarge_basis_set / generate_mel_*(raw_map_to=True)): OBC keeps the in-span moves
 // (holstein.py:185-199); periodic adds the wrapped moves as separate runs so every run stays
@@ -51,6 +80,8 @@ static int build_image_ops(const paces_holstein_t* h, OpTable* T) {
             T->dir[k] = (int8_t)dir;
             T->wrap[k] = (int8_t)wrap;
             T->term[k] = (int8_t)t;
+            T->dim[k] = (int8_t)d;
+            T->order[k] = (int8_t)tm.order;
             T->oe[k] = (int32_t)oe;
             T->span[k] = (int32_t)span;
             T->value[k] = tm.value;
@@ -66,6 +97,8 @@ static int build_image_ops(const paces_holstein_t* h, OpTable* T) {
         T->dir[k] = (int8_t)dir;
         T->wrap[k] = 0;
         T->term[k] = (int8_t)t;
+        T->dim[k] = 0;
+        T->order[k] = 0;
         T->oe[k] = 0;
         T->span[k] = 0;
         T->value[k] = tm.value;
@@ -92,6 +125,8 @@ static int build_nb_ops(const paces_holstein_t* h, OpTable* T) {
           T->dir[k] = (int8_t)dir;
           T->wrap[k] = (int8_t)(h->periodic ? 2 : 0);  // 2: either side, wrap when needed
           T->term[k] = (int8_t)t;
+          T->dim[k] = (int8_t)d;
+          T->order[k] = (int8_t)tm.order;
           T->oe[k] = (int32_t)(tm.order * Ld);
           T->span[k] = (int32_t)(Ld * h->lattice_l);
           T->value[k] = tm.value;
@@ -106,6 +141,8 @@ static int build_nb_ops(const paces_holstein_t* h, OpTable* T) {
         T->dir[k] = (int8_t)dir;
         T->wrap[k] = 0;
         T->term[k] = (int8_t)t;
+        T->dim[k] = 0;
+        T->order[k] = 0;
         T->oe[k] = 0;
         T->span[k] = 0;
         T->value[k] = tm.value;
@@ -127,11 +164,24 @@ struct LayoutArg {
 // `nup` receives the upper phonon count of a vib pair (for the matrix element).
 template <int W>
 __device__ __forceinline__ bool apply_op(const OpTable& T, int k, const LayoutArg& L,
-                                         Row<W>& r, int& nup) {
-  const int pb = L.width[0];
-  const int pos = (int)get_field<W>(r, 0, pb);
+                                         const ModelArg& M, Row<W>& r, int& nup) {
+  const int pf = M.pf;
+  const int po = L.off[pf], pb = L.width[pf];
+  const int pos = (int)get_field<W>(r, po, pb);
   if (T.kind[k] == PACES_TERM_HOP) {
-    const int oe = T.oe[k], span = T.span[k], dir = T.dir[k];
+    int oe, span;
+    if (M.batched) {
+      const int Lt = M.traj_L[get_field<W>(r, L.off[0], L.width[0])];
+      int Ld = 1;
+      for (int d = 0; d < T.dim[k]; ++d) Ld *= Lt;
+      oe = T.order[k] * Ld;
+      span = Ld * Lt;
+      if (oe >= span) return false;  // hop order longer than this trajectory's lattice
+    } else {
+      oe = T.oe[k];
+      span = T.span[k];
+    }
+    const int dir = T.dir[k];
     const int mod = pos % span;
     int npos;
     bool inside = (dir > 0) ? (mod < span - oe) : (mod >= oe);
@@ -144,11 +194,11 @@ __device__ __forceinline__ bool apply_op(const OpTable& T, int k, const LayoutAr
     } else {
       npos = inside ? pos + dir * oe : pos + dir * oe - dir * span;
     }
-    add_field<W>(r, 0, pb, (int64_t)(npos - pos));
+    add_field<W>(r, po, pb, (int64_t)(npos - pos));
     nup = 0;
     return true;
   } else {
-    const int f = pos + 1;
+    const int f = pf + 1 + pos;
     const int off = L.off[f], b = L.width[f];
     const int n = (int)get_field<W>(r, off, b);
     if (T.dir[k] > 0) {
@@ -164,13 +214,20 @@ __device__ __forceinline__ bool apply_op(const OpTable& T, int k, const LayoutAr
   }
 }
 
+// matrix-element value of op k for a row of trajectory traj
+__device__ __forceinline__ double op_value(const OpTable& T, int k, const ModelArg& M, int traj,
+                                           const double* __restrict__ sqrt_tab, int nup) {
+  const double v = M.batched ? M.traj_val[traj * M.nterms + T.term[k]] : T.value[k];
+  return (T.kind[k] == PACES_TERM_HOP) ? v : sqrt_tab[nup] * v;
+}
+
 // ------------------------------------------------------------------------------------------------
 // Image runs: flags -> scan -> emit
 // ------------------------------------------------------------------------------------------------
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_img_flags(const u64* __restrict__ basis, int64_t n,
-                                                      OpTable T, LayoutArg L,
+                                                      OpTable T, LayoutArg L, ModelArg M,
                                                       uint8_t* __restrict__ flags) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -178,7 +235,7 @@ __global__ void __launch_bounds__(kBlock) k_img_flags(const u64* __restrict__ ba
   for (int k = 0; k < T.n; ++k) {
     Row<W> r = r0;
     int nup;
-    flags[(int64_t)k * n + i] = apply_op<W>(T, k, L, r, nup) ? 1 : 0;
+    flags[(int64_t)k * n + i] = apply_op<W>(T, k, L, M, r, nup) ? 1 : 0;
   }
 }
 
@@ -190,7 +247,7 @@ __global__ void k_img_offsets(const int32_t* __restrict__ G, int64_t n, int nops
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_img_emit(const u64* __restrict__ basis, int64_t n,
-                                                     OpTable T, LayoutArg L,
+                                                     OpTable T, LayoutArg L, ModelArg M,
                                                      const uint8_t* __restrict__ flags,
                                                      const int32_t* __restrict__ G,
                                                      u64* __restrict__ out,
@@ -204,7 +261,7 @@ __global__ void __launch_bounds__(kBlock) k_img_emit(const u64* __restrict__ bas
     if (!flags[f]) continue;
     Row<W> r = r0;
     int nup = 0;
-    apply_op<W>(T, k, L, r, nup);
+    apply_op<W>(T, k, L, M, r, nup);
     const int64_t o = G[f];
     store_row<W>(out, o, r);
     if (out_src) out_src[o] = (int32_t)i;
@@ -218,8 +275,9 @@ static size_t images_ws(int64_t n, int nops) {
 }
 
 template <int W>
-static int run_img_count(const uint64_t* basis, int64_t n, const LayoutArg& L, const OpTable& T,
-                         int32_t* run_offsets, void* ws, size_t ws_bytes, hipStream_t st) {
+static int run_img_count(const uint64_t* basis, int64_t n, const LayoutArg& L, const ModelArg& M,
+                         const OpTable& T, int32_t* run_offsets, void* ws, size_t ws_bytes,
+                         hipStream_t st) {
   int64_t m = (int64_t)T.n * n;
   char* p = (char*)ws;
   uint8_t* flags = (uint8_t*)p;
@@ -228,7 +286,7 @@ static int run_img_count(const uint64_t* basis, int64_t n, const LayoutArg& L, c
   p += a256((size_t)(m + 1) * 4);
   if (n > 0) {
     hipLaunchKernelGGL(k_img_flags<W>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
-                       (const u64*)basis, n, T, L, flags);
+                       (const u64*)basis, n, T, L, M, flags);
     int rc = launch_status();
     if (rc) return rc;
   }
@@ -240,9 +298,9 @@ static int run_img_count(const uint64_t* basis, int64_t n, const LayoutArg& L, c
 }
 
 template <int W>
-static int run_img_emit(const uint64_t* basis, int64_t n, const LayoutArg& L, const OpTable& T,
-                        uint64_t* out, int32_t* out_src, uint8_t* out_nup, void* ws,
-                        hipStream_t st) {
+static int run_img_emit(const uint64_t* basis, int64_t n, const LayoutArg& L, const ModelArg& M,
+                        const OpTable& T, uint64_t* out, int32_t* out_src, uint8_t* out_nup,
+                        void* ws, hipStream_t st) {
   if (n == 0) return PACES_OK;
   int64_t m = (int64_t)T.n * n;
   char* p = (char*)ws;
@@ -250,7 +308,7 @@ static int run_img_emit(const uint64_t* basis, int64_t n, const LayoutArg& L, co
   p += a256((size_t)m);
   const int32_t* G = (const int32_t*)p;
   hipLaunchKernelGGL(k_img_emit<W>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
-                     (const u64*)basis, n, T, L, flags, G, (u64*)out, out_src, out_nup);
+                     (const u64*)basis, n, T, L, M, flags, G, (u64*)out, out_src, out_nup);
   return launch_status();
 }
 
@@ -264,7 +322,7 @@ static int run_img_emit(const uint64_t* basis, int64_t n, const LayoutArg& L, co
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_csr_count(const u64* __restrict__ rows, int64_t R,
                                                       const uint8_t* __restrict__ inB, OpTable T,
-                                                      LayoutArg L, int nterms,
+                                                      LayoutArg L, ModelArg M, int nterms,
                                                       int32_t* __restrict__ nb,
                                                       uint8_t* __restrict__ nbup,
                                                       int32_t* __restrict__ cnt) {
@@ -280,7 +338,7 @@ __global__ void __launch_bounds__(kBlock) k_csr_count(const u64* __restrict__ ro
   for (int k = 0; k < T.n; ++k) {
     Row<W> r = r0;
     int nup = 0;
-    const bool has = v && apply_op<W>(T, k, L, r, nup);
+    const bool has = v && apply_op<W>(T, k, L, M, r, nup);
     // the images of consecutive rows under one op are clustered, so the lookup stays local
     const int64_t lb = block_lower_bound<W>(r, has, rows, R, sm);
     int32_t j = -1;
@@ -318,13 +376,16 @@ struct CsrOut {
 // Emits each row's entries of every term in ascending column order by repeated minimum
 // selection over the (at most 8) neighbour slots of the term; the slots were filled by
 // k_csr_count with the neighbour index (or -1) and the upper phonon count of a vib pair.
-__global__ void __launch_bounds__(kBlock) k_csr_fill(int64_t R, OpTable T, int nterms,
+__global__ void __launch_bounds__(kBlock) k_csr_fill(int64_t R, OpTable T, ModelArg M, int nterms,
+                                                     const u64* __restrict__ rows, int W,
+                                                     int trajbits,
                                                      const double* __restrict__ sqrt_tab,
                                                      const int32_t* __restrict__ nb,
                                                      const uint8_t* __restrict__ nbup,
                                                      const int32_t* __restrict__ G, CsrOut out) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= R) return;
+  const int traj = M.batched ? (int)(rows[i * W] >> (64 - trajbits)) : 0;
   for (int t = 0; t < nterms; ++t) {
     const int32_t base0 = G[(int64_t)t * R];
     const int32_t base = G[(int64_t)t * R + i] - base0;
@@ -339,8 +400,7 @@ __global__ void __launch_bounds__(kBlock) k_csr_fill(int64_t R, OpTable T, int n
         const int32_t j = nb[(int64_t)k * R + i];
         if (j > last && j < best) {
           best = j;
-          bv = (T.kind[k] == PACES_TERM_HOP) ? T.value[k]
-                                             : sqrt_tab[nbup[(int64_t)k * R + i]] * T.value[k];
+          bv = op_value(T, k, M, traj, sqrt_tab, nbup[(int64_t)k * R + i]);
         }
       }
       if (best == 0x7fffffff) break;
@@ -363,7 +423,8 @@ static size_t csr_ws(int64_t R, int nops, int nterms) {
 
 template <int W>
 static int run_csr_count(const uint64_t* rows, int64_t R, const uint8_t* inB, const LayoutArg& L,
-                         const OpTable& T, int nterms, int32_t* nnz, void* ws, hipStream_t st) {
+                         const ModelArg& M, const OpTable& T, int nterms, int32_t* nnz, void* ws,
+                         hipStream_t st) {
   char* p = (char*)ws;
   int32_t* nb = (int32_t*)p;
   p += a256((size_t)T.n * R * 4);
@@ -375,7 +436,7 @@ static int run_csr_count(const uint64_t* rows, int64_t R, const uint8_t* inB, co
   p += a256((size_t)(nterms * R + 1) * 4);
   if (R > 0) {
     hipLaunchKernelGGL(k_csr_count<W>, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
-                       (const u64*)rows, R, inB, T, L, nterms, nb, nbup, cnt);
+                       (const u64*)rows, R, inB, T, L, M, nterms, nb, nbup, cnt);
     int rc = launch_status();
     if (rc) return rc;
   }
@@ -385,7 +446,8 @@ static int run_csr_count(const uint64_t* rows, int64_t R, const uint8_t* inB, co
   return launch_status();
 }
 
-static int run_csr_fill(int64_t R, const OpTable& T, int nterms, const double* sqrt_tab,
+static int run_csr_fill(int64_t R, const OpTable& T, const ModelArg& M, int nterms,
+                        const uint64_t* rows, int W, int trajbits, const double* sqrt_tab,
                         const CsrOut& out, void* ws, hipStream_t st) {
   if (R == 0) {
     hipLaunchKernelGGL(k_csr_empty, dim3(1), dim3(64), 0, st, out, nterms);
@@ -398,8 +460,8 @@ static int run_csr_fill(int64_t R, const OpTable& T, int nterms, const double* s
   p += a256((size_t)T.n * R);
   p += a256((size_t)nterms * R * 4);
   const int32_t* G = (const int32_t*)p;
-  hipLaunchKernelGGL(k_csr_fill, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st, R, T, nterms,
-                     sqrt_tab, nb, nbup, G, out);
+  hipLaunchKernelGGL(k_csr_fill, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st, R, T, M, nterms,
+                     (const u64*)rows, W, trajbits, sqrt_tab, nb, nbup, G, out);
   return launch_status();
 }
 
@@ -455,7 +517,9 @@ __global__ void __launch_bounds__(kBlock) k_pairs_indptr(int64_t R, int nterms,
 }
 
 __global__ void __launch_bounds__(kBlock) k_pairs_fill(int64_t R, int64_t nbasis, ImgRuns IR,
-                                                       OpTable T,
+                                                       OpTable T, ModelArg M,
+                                                       const u64* __restrict__ basis, int W,
+                                                       int trajbits,
                                                        const int32_t* __restrict__ cand_pos,
                                                        const uint8_t* __restrict__ inB,
                                                        const int32_t* __restrict__ src,
@@ -469,7 +533,8 @@ __global__ void __launch_bounds__(kBlock) k_pairs_fill(int64_t R, int64_t nbasis
   if (T.dir[k] < 0 && inB[p]) return;
   const int32_t q = cand_pos[src[e]];
   const int t = T.term[k];
-  const double v = (T.kind[k] == PACES_TERM_HOP) ? T.value[k] : sqrt_tab[nup[e]] * T.value[k];
+  const int traj = M.batched ? (int)(basis[(int64_t)src[e] * W] >> (64 - trajbits)) : 0;
+  const double v = op_value(T, k, M, traj, sqrt_tab, nup[e]);
   const int32_t a = atomicAdd(&cursor[(int64_t)t * R + q], 1);
   out.indices[t][a] = p;
   out.data[t][a] = v;
@@ -514,8 +579,8 @@ static size_t pairs_ws(int64_t R, int nterms) {
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_diag(const u64* __restrict__ rows, int64_t R,
-                                                 LayoutArg L, double eps, double hw, double de,
-                                                 double* __restrict__ diag) {
+                                                 LayoutArg L, int pf, double eps, double hw,
+                                                 double de, double* __restrict__ diag) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= R) return;
   const Row<W> r = load_row<W>(rows, i);
@@ -526,7 +591,7 @@ __global__ void __launch_bounds__(kBlock) k_diag(const u64* __restrict__ rows, i
     for (int k = 0; k < L.sum_bits; ++k)
       s += (long long)__popcll(r.w[w] & L.sum_mask[w * L.sum_bits + k]) << k;
   double d = eps + hw * (double)s;
-  if (de != 0.0) d = d + de * (double)get_field<W>(r, 0, L.width[0]);
+  if (de != 0.0) d = d + de * (double)get_field<W>(r, L.off[pf], L.width[pf]);
   diag[i] = d;
 }
 
@@ -662,21 +727,27 @@ static int run_obs(const uint64_t* rows, const double* psi, int64_t R, const Lay
 }
 
 template <int W>
-static int run_diag(const uint64_t* rows, int64_t R, const LayoutArg& L, double eps, double hw,
-                    double de, double* diag, hipStream_t st) {
+static int run_diag(const uint64_t* rows, int64_t R, const LayoutArg& L, int pf, double eps,
+                    double hw, double de, double* diag, hipStream_t st) {
   if (R == 0) return PACES_OK;
   hipLaunchKernelGGL(k_diag<W>, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st, (const u64*)rows,
-                     R, L, eps, hw, de, diag);
+                     R, L, pf, eps, hw, de, diag);
   return launch_status();
 }
 
 static int check_h(const paces_layout_t* lay, const paces_holstein_t* h) {
   if (!lay || !h || !lay->d_off || !lay->d_width || !lay->d_maxdim) return PACES_E_INVALID;
   if (lay->W < 1 || lay->W > PACES_MAX_ROW_WORDS) return PACES_E_UNSUPPORTED;
-  if (h->nchain < 1 || h->nchain > PACES_MAX_SITES || lay->nfields != h->nchain + 1)
+  const int extra = h->batched ? 2 : 1;
+  if (h->nchain < 1 || h->nchain > PACES_MAX_SITES || lay->nfields != h->nchain + extra)
     return PACES_E_INVALID;
   if (h->nterms < 0 || h->nterms > PACES_MAX_TERMS) return PACES_E_UNSUPPORTED;
-  if (h->geo_dim < 1 || h->lattice_l < 1) return PACES_E_INVALID;
+  if (h->geo_dim < 1) return PACES_E_INVALID;
+  if (h->batched) {
+    if (h->ntraj < 1 || !h->d_traj_L || !h->d_traj_value) return PACES_E_INVALID;
+    return PACES_OK;
+  }
+  if (h->lattice_l < 1) return PACES_E_INVALID;
   if (h->periodic && h->lattice_l < 3) return PACES_E_INVALID;
   for (int t = 0; t < h->nterms; ++t) {
     if (h->terms[t].kind == PACES_TERM_HOP &&
@@ -726,8 +797,8 @@ int paces_holstein_images_count(const uint64_t* basis, int64_t n, const paces_la
   if (n < 0 || (n > 0 && !basis) || !run_offsets || !ws) return PACES_E_INVALID;
   if ((int64_t)T.n * n >= 2147483647LL) return PACES_E_INVALID;
   if (ws_bytes < images_ws(n, T.n)) return PACES_E_WORKSPACE;
-  PACES_DISPATCH_W(lay->W, run_img_count, basis, n, larg(lay), T, run_offsets, ws, ws_bytes,
-                   (hipStream_t)stream);
+  PACES_DISPATCH_W(lay->W, run_img_count, basis, n, larg(lay), marg(h), T, run_offsets, ws,
+                   ws_bytes, (hipStream_t)stream);
 }
 
 int paces_holstein_images_emit(const uint64_t* basis, int64_t n, const paces_layout_t* lay,
@@ -740,8 +811,8 @@ int paces_holstein_images_emit(const uint64_t* basis, int64_t n, const paces_lay
   if (rc) return rc;
   if (n < 0 || (n > 0 && (!basis || !out_images)) || !ws) return PACES_E_INVALID;
   if (ws_bytes < images_ws(n, T.n)) return PACES_E_WORKSPACE;
-  PACES_DISPATCH_W(lay->W, run_img_emit, basis, n, larg(lay), T, out_images, out_src, out_nup, ws,
-                   (hipStream_t)stream);
+  PACES_DISPATCH_W(lay->W, run_img_emit, basis, n, larg(lay), marg(h), T, out_images, out_src,
+                   out_nup, ws, (hipStream_t)stream);
 }
 
 static int img_runs_arg(const OpTable& T, const int64_t* run_offsets, ImgRuns* IR) {
@@ -800,11 +871,15 @@ int paces_holstein_pairs_count(int64_t R, int64_t nbasis, const int64_t* run_off
 int paces_holstein_pairs_fill(int64_t R, int64_t nbasis, const int64_t* run_offsets,
                               const int32_t* cand_pos, const uint8_t* in_basis,
                               const int32_t* img_src, const uint8_t* img_nup,
+                              const uint64_t* basis, const paces_layout_t* lay,
                               const paces_holstein_t* h, int32_t* const* indptr,
                               int32_t* const* indices, double* const* data, void* ws,
                               size_t ws_bytes, void* stream) {
   OpTable T;
   if (!h || !run_offsets || !ws || !indptr || !indices || !data || R < 0) return PACES_E_INVALID;
+  if (h->batched && (!basis || !lay || !h->d_traj_value)) return PACES_E_INVALID;
+  const int W = lay ? lay->W : 1;
+  const int trajbits = h->batched ? tbits(h->ntraj) : 0;
   if (h->nterms < 1 || h->nterms > PACES_MAX_TERMS) return PACES_E_UNSUPPORTED;
   if (!h->d_sqrt_table) return PACES_E_INVALID;
   int rc = build_image_ops(h, &T);
@@ -836,7 +911,8 @@ int paces_holstein_pairs_fill(int64_t R, int64_t nbasis, const int64_t* run_offs
                      h->nterms, G, out, cursor);
   if (nimg > 0)
     hipLaunchKernelGGL(k_pairs_fill, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, R, nbasis,
-                       IR, T, cand_pos, in_basis, img_src, img_nup, h->d_sqrt_table, cursor, out);
+                       IR, T, marg(h), (const u64*)basis, W, trajbits, cand_pos, in_basis,
+                       img_src, img_nup, h->d_sqrt_table, cursor, out);
   hipLaunchKernelGGL(k_pairs_sort, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st, R, h->nterms,
                      out);
   return launch_status();
@@ -865,8 +941,8 @@ int paces_holstein_csr_count(const uint64_t* rows, int64_t R, const uint8_t* in_
   if ((int64_t)h->nterms * R >= 2147483647LL || (int64_t)T.n * R >= 2147483647LL)
     return PACES_E_INVALID;
   if (ws_bytes < csr_ws(R, T.n, h->nterms)) return PACES_E_WORKSPACE;
-  PACES_DISPATCH_W(lay->W, run_csr_count, rows, R, in_basis, larg(lay), T, h->nterms, term_nnz, ws,
-                   (hipStream_t)stream);
+  PACES_DISPATCH_W(lay->W, run_csr_count, rows, R, in_basis, larg(lay), marg(h), T, h->nterms,
+                   term_nnz, ws, (hipStream_t)stream);
 }
 
 int paces_holstein_csr_fill(const uint64_t* rows, int64_t R, const paces_layout_t* lay,
@@ -888,7 +964,8 @@ int paces_holstein_csr_fill(const uint64_t* rows, int64_t R, const paces_layout_
     out.data[t] = t < h->nterms ? data[t] : nullptr;
     if (t < h->nterms && !out.indptr[t]) return PACES_E_INVALID;
   }
-  return run_csr_fill(R, T, h->nterms, h->d_sqrt_table, out, ws, (hipStream_t)stream);
+  return run_csr_fill(R, T, marg(h), h->nterms, rows, lay->W, h->batched ? tbits(h->ntraj) : 0,
+                      h->d_sqrt_table, out, ws, (hipStream_t)stream);
 }
 
 int paces_holstein_diag(const uint64_t* rows, int64_t R, const paces_layout_t* lay,
@@ -897,8 +974,8 @@ int paces_holstein_diag(const uint64_t* rows, int64_t R, const paces_layout_t* l
   if (rc) return rc;
   if (R < 0 || (R > 0 && (!rows || !diag))) return PACES_E_INVALID;
   if (!lay->d_sum_mask || lay->sum_bits < 1 || lay->sum_bits > 32) return PACES_E_INVALID;
-  PACES_DISPATCH_W(lay->W, run_diag, rows, R, larg(lay), h->eps_sys, h->hbar_omega, h->delta_eps,
-                   diag, (hipStream_t)stream);
+  PACES_DISPATCH_W(lay->W, run_diag, rows, R, larg(lay), h->batched ? 1 : 0, h->eps_sys,
+                   h->hbar_omega, h->delta_eps, diag, (hipStream_t)stream);
 }
 
 size_t paces_holstein_observables_workspace_bytes(int64_t R, const paces_holstein_t* h) {

@@ -183,14 +183,14 @@ class Hamiltonian(HamiltonianFramework):
         self._check_basis_size(new_inds.shape[0])
         diag = self.generate_mel_diag(new_inds)
         mats = self._pairs_csr(new_inds.shape[0], basis.shape[0], offs, cand_pos, in_basis, src,
-                               nup)
+                               nup, basis)
         dbg = {}
         if "vib_coupling" in self.offdiag_terms:
             dbg["vib_coupling"] = [basis.shape[0] - self._vib_plus_count(offs)]
         self.last_basis_e = basis
         return (diag, new_inds), mats, dbg
 
-    def _pairs_csr(self, R, nbasis, offs, cand_pos, in_basis, src, nup):
+    def _pairs_csr(self, R, nbasis, offs, cand_pos, in_basis, src, nup, basis=None):
         """Per-term CSR from the union's candidate positions (paces_holstein_pairs_*)."""
         h = self._hs
         nterms = h.nterms
@@ -210,8 +210,9 @@ class Hamiltonian(HamiltonianFramework):
             arrs[0][t], arrs[1][t], arrs[2][t] = ip.data_ptr(), ix.data_ptr(), dv.data_ptr()
             mats[name] = CsrMatrix(ip, ix[:nnz_h[t]], dv[:nnz_h[t]], (R, R))
         check(_lib.paces_holstein_pairs_fill(R, nbasis, run_off, ptr(cand_pos), ptr(in_basis),
-                                             ptr(src), ptr(nup), C.byref(h), arrs[0], arrs[1],
-                                             arrs[2], wsp, wsn, stream()), "holstein_pairs_fill")
+                                             ptr(src), ptr(nup), ptr(basis), C.byref(self._layout),
+                                             C.byref(h), arrs[0], arrs[1], arrs[2], wsp, wsn,
+                                             stream()), "holstein_pairs_fill")
         return mats
 
     def _csr(self, rows, in_basis):
@@ -361,3 +362,220 @@ class TimeEvolution(TimeEvolutionFramework):
         raw = cartesian_product(numpy.arange(N), *([numpy.arange(truncate_d)] * N))
         self.whoami = self.hamobj.compress_ind_arr(raw)
         self.numstates = self.whoami.shape[0]
+
+
+# ------------------------------------------------------------------------------------------------
+# Batched trajectories (no reference counterpart): many independent Holstein lattices advanced
+# together. Rows carry the trajectory id as field 0 and the exciton position as field 1, phonon
+# fields are padded to the largest lattice; the per-trajectory lattice length and term values
+# live in device tables.
+# ------------------------------------------------------------------------------------------------
+
+class BatchedHamiltonian(Hamiltonian):
+    """Holstein Hamiltonians of ``len(nchains)`` independent trajectories in one packed basis.
+
+    ``hopping_J`` and ``vib_g`` are per-trajectory values in the reference's convention (the
+    hopping element and the g of g*sqrt(n+1)); the diagonal (``eps_sys``, ``hbar_omega``), the
+    phonon dimension, ``geo_dim`` and ``periodic`` are shared.
+    """
+
+    def __init__(self, nchains, max_ho_dim, hopping_J, vib_g, hbar_omega=1.0, eps_sys=0.0,
+                 geo_dim=1, periodic=False, debug_verb=0):
+        nchains = [int(n) for n in nchains]
+        ntraj = len(nchains)
+        if ntraj < 1 or len(hopping_J) != ntraj or len(vib_g) != ntraj:
+            raise ValueError("One lattice size, J and g per trajectory are required.")
+        self.ntraj = ntraj
+        self.nchains = nchains
+        self.nchain = max(nchains)
+        if self.nchain > _hip.MAX_SITES:
+            raise NotImplementedError(f"At most {_hip.MAX_SITES} sites are supported.")
+        self.geo_dim = int(geo_dim)
+        self.lattice_ls = [int(round(n ** (1 / geo_dim))) for n in nchains]
+        for n, L in zip(nchains, self.lattice_ls):
+            if L ** geo_dim != n:
+                raise ValueError(f"Number of sites {n} is incompatible with dimension {geo_dim}.")
+            if periodic and L < 3:
+                raise ValueError("Periodic hopping needs a lattice length of at least 3.")
+        self.lattice_l = max(self.lattice_ls)
+        self.periodic = bool(periodic)
+        self.max_ho_dims = numpy.full(self.nchain, int(max_ho_dim), dtype=numpy.int64)
+        use_terms = {"diag": {"eps_sys": float(eps_sys), "hbar_omega": float(hbar_omega),
+                              "delta_eps": 0.0},
+                     "hopping": {"J": 1.0}, "vib_coupling": {"g": 1.0}}
+        max_dims = numpy.array([ntraj, self.nchain] + [int(max_ho_dim)] * self.nchain)
+        HamiltonianFramework.__init__(self, max_dims=max_dims, use_terms=use_terms,
+                                      debug_verb=debug_verb)
+        # the diagonal sums the phonon fields only (not the trajectory and position fields)
+        masks, kb = _hip.sum_masks(self.bit_offsets, self.bitwidths, self.totwordwidth, first=2)
+        self._d_sum_mask = torch.from_numpy(masks.view(numpy.int64).copy()).to(self.device)
+        self._layout = _hip.Layout(self.totwordwidth, self.n_sites, self._d_off.data_ptr(),
+                                   self._d_width.data_ptr(), self._d_maxdim.data_ptr(),
+                                   self._d_sum_mask.data_ptr(), kb)
+        self.posbitwidth = int(self.bitwidths[1])
+        self.hopping_J = numpy.asarray(hopping_J, dtype=numpy.float64)
+        self.vib_g = numpy.asarray(vib_g, dtype=numpy.float64)
+        self._d_traj_L = torch.tensor(self.lattice_ls, dtype=torch.int32, device=self.device)
+        vals = numpy.stack([self.hopping_J, self.vib_g], axis=1)
+        self._d_traj_val = torch.tensor(vals, dtype=torch.float64, device=self.device)
+        self._hs = self._make_holstein_struct(self.offdiag_terms)
+        self._hs.batched = 1
+        self._hs.ntraj = ntraj
+        self._hs.d_traj_L = self._d_traj_L.data_ptr()
+        self._hs.d_traj_value = self._d_traj_val.data_ptr()
+
+    def get_pos(self, arr):
+        return self.get_n_at_site(arr, 1)
+
+    def get_traj(self, arr):
+        return self.get_n_at_site(arr, 0)
+
+    def segments(self, rows):
+        """seg_off (device int32, ntraj+1): trajectory t owns rows [seg_off[t], seg_off[t+1])."""
+        seg = torch.empty(self.ntraj + 1, dtype=torch.int32, device=self.device)
+        check(_lib.paces_batch_segments(ptr(rows) if rows.shape[0] else None, rows.shape[0],
+                                        self.totwordwidth, self.ntraj, ptr(seg), stream()),
+              "batch_segments")
+        return seg
+
+    def site_observables_batched(self, rows, vector, seg_off):
+        """(n_exc [ntraj, N], n_pho [ntraj, N], sum|psi|^2 [ntraj]) per trajectory."""
+        R = rows.shape[0]
+        N = self.nchain
+        out = torch.empty(self.ntraj * (2 * N + 1), dtype=torch.float64, device=self.device)
+        wsp, wsn = self._ws.get(
+            _lib.paces_holstein_observables_batched_workspace_bytes(R, self.ntraj, N))
+        check(_lib.paces_holstein_observables_batched(
+            ptr(rows) if R else None, ptr(torch.view_as_real(vector)) if R else None, R,
+            ptr(seg_off), C.byref(self._layout), C.byref(self._hs), ptr(out), wsp, wsn,
+            stream()), "holstein_observables_batched")
+        h = out.cpu().numpy().reshape(self.ntraj, 2 * N + 1)
+        return h[:, :N], h[:, N:2 * N], h[:, 2 * N]
+
+
+class BatchedTimeEvolution:
+    """Adaptive-basis evolution of every trajectory of a BatchedHamiltonian in one set of
+    launches per step: per-trajectory select, one shared expansion/union/CSR/reassignment, and a
+    Taylor series whose stopping test runs per trajectory (the series continues until all stop).
+
+    precision: "mixed" (complex64 terms, FP32 matrix, complex128 state) or "fp64".
+    """
+
+    def __init__(self, hamobj: BatchedHamiltonian, maxstates, enlarge_steps=1,
+                 precision="mixed", m_star=100, taylor_tol=None):
+        self.hamobj = hamobj
+        self.device = hamobj.device
+        ms = numpy.asarray(maxstates, dtype=numpy.int64)
+        if ms.shape != (hamobj.ntraj,) or numpy.any(ms < 1):
+            raise ValueError("maxstates needs one positive value per trajectory.")
+        self.maxstates = ms
+        self._d_ms = torch.tensor(ms, dtype=torch.int64, device=self.device)
+        self.enlarge_steps = int(enlarge_steps)
+        if precision not in ("mixed", "fp64"):
+            raise ValueError("precision must be 'mixed' or 'fp64'.")
+        self.precision = precision
+        self.m_star = m_star
+        self.tol = taylor_tol if taylor_tol is not None else (
+            engine.TAYLOR_TOL_MIXED if precision == "mixed" else engine.TAYLOR_TOL)
+        self._ws = hamobj._ws
+        self._tws = _hip.Workspace(self.device)
+        self.whoami = None
+        self.vector = None
+        self.seg_off = None
+        self.diag_vals = None
+        self.sparse_mats_dict = None
+        self.last_expm = None
+
+    @property
+    def numstates(self):
+        return int(self.whoami.shape[0])
+
+    def create_single_exciton_states(self, positions):
+        """One basis state per trajectory: exciton at positions[t], phonon vacuum, amplitude 1."""
+        h = self.hamobj
+        pos = numpy.asarray(positions, dtype=numpy.int64)
+        if pos.shape != (h.ntraj,) or numpy.any(pos < 0) or numpy.any(pos >= h.nchains):
+            raise ValueError("One exciton position inside each lattice is required.")
+        raw = numpy.zeros((h.ntraj, 2 + h.nchain), dtype=numpy.int64)
+        raw[:, 0] = numpy.arange(h.ntraj)
+        raw[:, 1] = pos
+        self.whoami = h.compress_ind_arr(raw)
+        self.vector = torch.ones(h.ntraj, dtype=torch.complex128, device=self.device)
+        self.seg_off = h.segments(self.whoami)
+
+    def _select(self):
+        R, W = self.whoami.shape
+        out = torch.empty((max(R, 1), W), dtype=torch.int64, device=self.device)
+        cnt = torch.empty(1, dtype=torch.int32, device=self.device)
+        wsp, wsn = self._ws.get(_lib.paces_batch_select_workspace_bytes(R, self.hamobj.ntraj))
+        check(_lib.paces_batch_select_topk(ptr(torch.view_as_real(self.vector)), ptr(self.whoami),
+                                           R, W, ptr(self.seg_off), self.hamobj.ntraj,
+                                           ptr(self._d_ms), ptr(out), ptr(cnt), wsp, wsn,
+                                           stream()), "batch_select_topk")
+        return out[:int(cnt.item())]
+
+    def adapt(self):
+        select = self._select()
+        (diag, new_inds), mats, dbg = self.hamobj.generate_mel(select, self.enlarge_steps)
+        self.vector = engine.reassign(self.whoami, self.vector, new_inds, self._ws)
+        self.whoami = new_inds
+        self.diag_vals = diag
+        self.sparse_mats_dict = mats
+        self.seg_off = self.hamobj.segments(new_inds)
+        return dbg
+
+    def propagate(self, delta_t):
+        h = self.hamobj
+        R = self.numstates
+        mats = [self.sparse_mats_dict[t] for t in h.offdiag_terms]
+        mixed = self.precision == "mixed"
+        vdt = torch.complex64 if mixed else torch.complex128
+        res = torch.empty_like(self.vector)
+        bufs = [torch.empty(R, dtype=vdt, device=self.device) for _ in range(3)]
+        ts = torch.zeros(h.ntraj * C.sizeof(_hip.BatchTaylor), dtype=torch.uint8,
+                         device=self.device)
+        hdr = torch.zeros(16, dtype=torch.uint8, device=self.device)
+        wsp, wsn = self._tws.get(_lib.paces_batch_taylor_workspace_bytes(R, h.ntraj))
+        st = stream()
+        check(_lib.paces_batch_taylor_begin(R, ptr(torch.view_as_real(self.vector)),
+                                            ptr(torch.view_as_real(res)),
+                                            ptr(torch.view_as_real(bufs[2])), 1 if mixed else 0,
+                                            ptr(self.seg_off), h.ntraj, ptr(ts), wsp, wsn, st),
+              "batch_taylor_begin")
+        if mixed:
+            mm = engine.MixedMats(self.diag_vals, mats)
+            h64, h32, dptr = None, C.byref(mm.cset), ptr(mm.diag)
+        else:
+            s64 = engine.csr_set(mats)
+            h64, h32, dptr = C.byref(s64), None, ptr(self.diag_vals)
+        x_in = bufs[2]
+        j = 0
+        while j < self.m_star:
+            for _ in range(4):
+                if j >= self.m_star:
+                    break
+                x_out = bufs[j % 2]
+                check(_lib.paces_batch_taylor_term(
+                    R, dptr, h64, h32, ptr(torch.view_as_real(x_in)),
+                    ptr(torch.view_as_real(x_out)), ptr(torch.view_as_real(res)),
+                    engine.taylor_coeff(delta_t, j), j, self.tol, ptr(self.seg_off), h.ntraj,
+                    ptr(ts), ptr(hdr), wsp, wsn, st), "batch_taylor_term")
+                x_in = x_out
+                j += 1
+            if int(hdr[:4].view(torch.int32).item()) == 0:
+                break
+        self.vector = res
+        info = numpy.frombuffer(ts.cpu().numpy().tobytes(), dtype=numpy.dtype(
+            [("c1", "f8"), ("c1pc2", "f8"), ("ratio", "f8"), ("done", "i4"), ("j", "i4")]))
+        self.last_expm = info
+        return info
+
+    def step(self, i, delta_t):
+        """Step 0 adapts only; later steps adapt then propagate (time_evolution.py:426-458)."""
+        dbg = self.adapt()
+        if i > 0:
+            self.propagate(delta_t)
+        return dbg
+
+    def observables(self):
+        return self.hamobj.site_observables_batched(self.whoami, self.vector, self.seg_off)
