@@ -118,15 +118,110 @@ def cpu_baseline(wl, budget_s=25.0):
                       f"({units} retained state-steps in {elapsed:.1f} s, NumPy single thread)"}
 
 
+def traj256_draws(n=256, seed=0):
+    """Config-3 draws in order: N in {16,24,32,48}, J ~ U[0.5,1.5], g ~ U[0.5,2.0],
+    maxstates = round(10^U[3, log10 2e5])."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        N = int(rng.choice([16, 24, 32, 48]))
+        J = float(rng.uniform(0.5, 1.5))
+        g = float(rng.uniform(0.5, 2.0))
+        ms = int(round(10 ** rng.uniform(3.0, np.log10(2e5))))
+        out.append((N, J, g, ms))
+    return out
+
+
+def run_traj256(a):
+    """256 independent 1D Holstein chains (OBC, cap 7, w0=1), exciton at N//2, dt=0.05,
+    enlarge_steps=1. form "batched": one BatchedTimeEvolution (mixed precision); form "plain":
+    every trajectory stepped by its own plain FP64 engine, one after the other."""
+    import torch
+    from paces_amd.core.time_evolution import ExpmParams, TimeEvoParams
+    from paces_amd.models import holstein
+    draws = traj256_draws(a.ntraj)
+    dt = 0.05
+    if a.form == "batched":
+        bham = holstein.BatchedHamiltonian([d[0] for d in draws], 8, [-d[1] for d in draws],
+                                           [d[2] for d in draws])
+        bte = holstein.BatchedTimeEvolution(bham, [d[3] for d in draws], enlarge_steps=1,
+                                            precision=a.precision_batched)
+        bte.create_single_exciton_states([d[0] // 2 for d in draws])
+        bte.step(0, dt)
+        for i in range(1, a.warmup + 1):
+            bte.step(i, dt)
+            print(f"warmup step {i}: basis {bte.numstates}", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        units = 0
+        t0 = time.perf_counter()
+        for i in range(a.warmup + 1, a.warmup + 1 + a.steps):
+            segs = bte.seg_off.cpu().numpy()
+            prev = np.diff(segs)
+            bte.step(i, dt)
+            units += int(np.minimum(prev, bte.maxstates).sum())
+            print(f"timed step {i}: basis {bte.numstates}", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        rows = bte.numstates
+        dtype = "c64 terms / f64 state" if a.precision_batched == "mixed" else "f64"
+    else:
+        tes = []
+        for (N, J, g, ms) in draws:
+            ham = holstein.Hamiltonian(N, [8] * N, use_terms=use_terms(J, g))
+            te = holstein.TimeEvolution(ham, holstein.Observables, ["n_exc", "n_pho"], None,
+                                        TimeEvoParams(maxstates=ms, enlarge_steps=1,
+                                                      shuffle_seed=None, diagnostics=False),
+                                        ExpmParams())
+            te.create_nonuniform_basis([1] * N, minpos=N // 2, maxpos=N // 2 + 1)
+            raw = np.zeros((1, N + 1), dtype=np.int64)
+            raw[0, 0] = N // 2
+            te.create_initial_vector([1.0], raw)
+            te._perform_first_step(0.0)
+            tes.append(te)
+        for i in range(1, a.warmup + 1):
+            for te in tes:
+                te._evolve_single_step(i, dt * i, dt)
+            print(f"warmup step {i}: basis {sum(te.numstates for te in tes)}", file=sys.stderr,
+                  flush=True)
+        torch.cuda.synchronize()
+        units = 0
+        t0 = time.perf_counter()
+        for i in range(a.warmup + 1, a.warmup + 1 + a.steps):
+            for te, d in zip(tes, draws):
+                prev = te.numstates
+                te._evolve_single_step(i, dt * i, dt)
+                units += min(prev, d[3])
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        rows = sum(te.numstates for te in tes)
+        dtype = "f64"
+    line = {"metric": METRIC, "value": units / elapsed, "unit": "state-steps/s", "n_gpus": 1,
+            "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * elapsed / a.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic",
+            "config": {"workload": f"traj256: {a.ntraj} 1D Holstein chains, N in {{16,24,32,48}}, "
+                                   "J~U[0.5,1.5], g~U[0.5,2], cap 7, maxstates~10^U[3,5.3], "
+                                   "dt=0.05, enlarge_steps=1 (seed 0)",
+                       "form": a.form, "final_basis_rows": int(rows),
+                       "sum_maxstates": int(sum(d[3] for d in draws))},
+            "roofline": None, "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=6)
-    ap.add_argument("--workload", default="ring64", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="ring64", choices=sorted(WORKLOADS) + ["traj256"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=25.0)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--form", default="batched", choices=["batched", "plain"],
+                    help="traj256 only")
+    ap.add_argument("--ntraj", type=int, default=256)
+    ap.add_argument("--precision-batched", default="mixed", choices=["fp64", "mixed"])
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported "
@@ -138,10 +233,12 @@ def main(argv=None):
     from paces_amd.core.time_evolution import ExpmParams, TimeEvoParams
     from paces_amd.models import holstein
 
-    wl = WORKLOADS[a.workload]
     import contextlib
     with contextlib.redirect_stdout(sys.stderr):  # stdout carries only the JSON line
         devices.configure()
+    if a.workload == "traj256":
+        return run_traj256(a)
+    wl = WORKLOADS[a.workload]
     ham = holstein.Hamiltonian(wl["N"], [wl["dim"]] * wl["N"], geo_dim=wl["geo"],
                                periodic=wl["periodic"], use_terms=use_terms(wl["J"], wl["g"]))
     te = holstein.TimeEvolution(ham, holstein.Observables, ["n_exc", "n_pho"], None,

@@ -11,7 +11,6 @@
 #include "paces_hip.h"
 
 typedef unsigned long long u64;
-typedef unsigned __int128 u128;
 
 #define PACES_MAX_W 16
 
@@ -241,22 +240,31 @@ __device__ int64_t block_lower_bound_range(const Row<W>& q, bool valid,
 }
 
 // ---- bit fields ---------------------------------------------------------------------------------
-// A field at bit offset `off` (from the MSB of word 0) and width `b` (1..64).
+// A field at bit offset `off` (from the MSB of word 0) and width `b` (1..64). It lies in word
+// k = off/64 or, when o + b > 64 (o = off%64), straddles words k and k+1. Only 64-bit shifts by
+// 0..63 are used: the 128-bit formulation gave unstable results for straddling fields.
 template <int W>
-__device__ __forceinline__ u64 get_field(const Row<W>& r, int off, int b) {
-  if (b == 0) return 0;
-  int k = off >> 6;
-  int o = off & 63;
-  u64 hi = 0, lo = 0;
+__device__ __forceinline__ void field_words(const Row<W>& r, int k, u64& hi, u64& lo) {
+  hi = 0;
+  lo = 0;
 #pragma unroll
   for (int kk = 0; kk < W; ++kk) {  // select without dynamic register indexing
     hi = (kk == k) ? r.w[kk] : hi;
     lo = (kk == k + 1) ? r.w[kk] : lo;
   }
-  u128 c = ((u128)hi << 64) | (u128)lo;
-  c >>= (128 - o - b);
-  u64 mask = (b == 64) ? ~0ull : ((1ull << b) - 1ull);
-  return (u64)c & mask;
+}
+
+template <int W>
+__device__ __forceinline__ u64 get_field(const Row<W>& r, int off, int b) {
+  if (b == 0) return 0;
+  const int k = off >> 6, o = off & 63;
+  u64 hi, lo;
+  field_words<W>(r, k, hi, lo);
+  const u64 mask = (b == 64) ? ~0ull : ((1ull << b) - 1ull);
+  const int end = o + b;  // 1..127
+  if (end <= 64) return (hi >> (64 - end)) & mask;
+  const int s = end - 64;  // field bits in word k+1, 1..63
+  return ((hi << s) | (lo >> (64 - s))) & mask;
 }
 
 // add a signed delta to a field; the caller guarantees the result stays inside the field, so the
@@ -264,24 +272,34 @@ __device__ __forceinline__ u64 get_field(const Row<W>& r, int off, int b) {
 template <int W>
 __device__ __forceinline__ void add_field(Row<W>& r, int off, int b, int64_t delta) {
   if (b == 0 || delta == 0) return;
-  int k = off >> 6;
-  int o = off & 63;
-  u64 hi = 0, lo = 0;
-#pragma unroll
-  for (int kk = 0; kk < W; ++kk) {
-    hi = (kk == k) ? r.w[kk] : hi;
-    lo = (kk == k + 1) ? r.w[kk] : lo;
+  const int k = off >> 6, o = off & 63;
+  u64 hi, lo;
+  field_words<W>(r, k, hi, lo);
+  const int end = o + b;
+  const bool neg = delta < 0;
+  const u64 d = neg ? (u64)(-delta) : (u64)delta;
+  if (end <= 64) {
+    const u64 a = d << (64 - end);
+    hi = neg ? hi - a : hi + a;
+  } else {
+    const int s = end - 64;            // field LSB is bit 64-s of word k+1
+    const u64 a_lo = d << (64 - s);    // part of d*2^(64-s) inside word k+1
+    const u64 a_hi = d >> s;           // part that lands in word k
+    if (neg) {
+      const u64 borrow = lo < a_lo ? 1ull : 0ull;
+      lo = lo - a_lo;
+      hi = hi - a_hi - borrow;
+    } else {
+      const u64 nlo = lo + a_lo;
+      const u64 carry = nlo < lo ? 1ull : 0ull;
+      lo = nlo;
+      hi = hi + a_hi + carry;
+    }
   }
-  u128 c = ((u128)hi << 64) | (u128)lo;
-  u128 unit = (u128)1 << (128 - o - b);
-  if (delta > 0)
-    c += unit * (u128)(u64)delta;
-  else
-    c -= unit * (u128)(u64)(-delta);
 #pragma unroll
   for (int kk = 0; kk < W; ++kk) {
-    if (kk == k) r.w[kk] = (u64)(c >> 64);
-    if (kk == k + 1) r.w[kk] = (u64)c;
+    if (kk == k) r.w[kk] = hi;
+    if (kk == k + 1 && end > 64) r.w[kk] = lo;
   }
 }
 

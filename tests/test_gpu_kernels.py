@@ -121,6 +121,49 @@ def test_pack_get_add_fields(env, dims):
             np.testing.assert_array_equal(_np(env, d), lay.pack(exp))
 
 
+@pytest.mark.parametrize("dims", [[256, 64] + [8] * 64, [64] + [16] * 64, [2, 4] + [128] * 40])
+def test_fields_at_word_straddles_repeatable(env, dims):
+    """Fields that straddle two uint64 words (e.g. site 16 of the 64-site batched layout, bits
+    62-64) read and step identically on every call: 200k rows, values at the field limits so the
+    carry/borrow crosses the word boundary, four repetitions, each bit-exact against the oracle."""
+    import paces_amd._hip as hip
+    torch, lib = env["torch"], env["lib"]
+    lay = po.Layout(dims)
+    straddle = [f for f in range(lay.nfields)
+                if lay.bitwidths[f] and lay.offsets[f] // 64
+                != (lay.offsets[f] + lay.bitwidths[f] - 1) // 64]
+    assert straddle
+    n = 200_000
+    rng = np.random.default_rng(7)
+    dev = env["dev"]
+    d_off = torch.tensor(lay.offsets, dtype=torch.int32, device=dev)
+    d_w = torch.tensor(lay.bitwidths, dtype=torch.int32, device=dev)
+    d_m = torch.tensor(lay.max_dims, dtype=torch.int32, device=dev)
+    L = hip.Layout(lay.W, lay.nfields, d_off.data_ptr(), d_w.data_ptr(), d_m.data_ptr(), None, 0)
+    out = torch.empty(n, dtype=torch.int32, device=dev)
+    for f in straddle:
+        md = int(lay.max_dims[f])
+        raw = np.stack([rng.integers(0, d, n) for d in lay.max_dims], axis=1)
+        # half the rows sit at a boundary value whose +-1 moves bits across the two words
+        edge = rng.integers(0, 2, n).astype(bool)
+        raw[edge, f] = rng.choice([0, md // 2 - 1, md // 2, md - 1], int(edge.sum()))
+        packed = lay.pack(raw)
+        for rep in range(4):
+            rows = _rows(env, packed)
+            hip.check(lib.paces_get_field(hip.ptr(rows), n, C.byref(L), f, hip.ptr(out),
+                                          hip.stream()))
+            np.testing.assert_array_equal(out.cpu().numpy(), raw[:, f], err_msg=f"{f} {rep}")
+            for delta, sel in ((+1, raw[:, f] + 1 < md), (-1, raw[:, f] > 0)):
+                sub = raw[sel]
+                d = _rows(env, lay.pack(sub))
+                hip.check(lib.paces_add_field(hip.ptr(d), sub.shape[0], C.byref(L), f, delta,
+                                              hip.stream()))
+                exp = sub.copy()
+                exp[:, f] += delta
+                np.testing.assert_array_equal(_np(env, d), lay.pack(exp),
+                                              err_msg=f"{f} {delta} {rep}")
+
+
 @pytest.mark.parametrize("W,K", [(1, 5), (3, 7), (5, 11), (13, 3)])
 def test_merge_unique_matches_numpy_unique(env, W, K):
     eng, hip = env["engine"], env["hip"]
