@@ -1,11 +1,17 @@
 """Benchmark of the adaptive-basis time step on one MI355X.
 
-Default workload: 1D Holstein ring, N=64 sites, periodic, J=1, w0=1, g=1.5, phonon cap 15
-(4 bits; rows of 6+256 bits in 5 uint64 words), exciton at site 32, maxstates=2e6, dt=0.02,
-enlarge_steps=1, plain complex128/FP64 form. One step = select + expansion + CSR + reassignment +
-Taylor propagation + norm + observables, with every buffer in HBM.
+Default workload (traj256): 256 independent 1D Holstein chains (open), N in {16,24,32,48},
+J ~ U[0.5,1.5], w0=1, g ~ U[0.5,2], phonon cap 7, maxstates ~ 10^U[3,5.3], exciton at N//2,
+dt=0.05, enlarge_steps=1, all advanced together by the batched form (complex64 Taylor terms,
+complex128 state). The same line carries the plain FP64 form's throughput on the same
+trajectories (one engine per trajectory), the batched form's maximum site-density error against
+the batched FP64 form, the Taylor term's HBM roofline and the oracle's CPU throughput.
 
-Prints one JSON line. ``--gpus`` above 1 exits: this engine runs on one GPU.
+``--workload ring64|chain16`` runs one plain-form trajectory instead (64-site periodic ring,
+cap 15, maxstates 2e6, dt 0.02; or 16-site open chain, cap 7, maxstates 2e4, dt 0.05).
+
+One step = select + expansion + CSR + reassignment + Taylor propagation, with every buffer in
+HBM. Prints one JSON line. ``--gpus`` above 1 exits: this engine runs on one GPU.
 """
 
 from __future__ import annotations
@@ -119,7 +125,7 @@ def cpu_baseline(wl, budget_s=25.0):
 
 
 def traj256_draws(n=256, seed=0):
-    """Config-3 draws in order: N in {16,24,32,48}, J ~ U[0.5,1.5], g ~ U[0.5,2.0],
+    """Trajectory draws in order: N in {16,24,32,48}, J ~ U[0.5,1.5], g ~ U[0.5,2.0],
     maxstates = round(10^U[3, log10 2e5])."""
     rng = np.random.default_rng(seed)
     out = []
@@ -132,80 +138,260 @@ def traj256_draws(n=256, seed=0):
     return out
 
 
-def run_traj256(a):
-    """256 independent 1D Holstein chains (OBC, cap 7, w0=1), exciton at N//2, dt=0.05,
-    enlarge_steps=1. form "batched": one BatchedTimeEvolution (mixed precision); form "plain":
-    every trajectory stepped by its own plain FP64 engine, one after the other."""
+TRAJ_DT = 0.05
+TRAJ_DESC = ("traj256: {n} independent 1D Holstein chains (open), N in {{16,24,32,48}}, "
+             "J~U[0.5,1.5], w0=1, g~U[0.5,2], phonon cap 7, maxstates~10^U[3,5.3], exciton at "
+             "N//2, dt=0.05, enlarge_steps=1 (draws seeded 0)")
+
+
+def btay_term_bytes(R, mats, mixed):
+    """Algorithmic HBM bytes of one batched Taylor term over R rows: diagonal (4 or 8 per row),
+    each term's indptr (4(R+1)), indices + values (4 + 4 or 8 per nonzero), x_in and x_out
+    (8 or 16 per row each), complex128 partial sum read + write (32 per row)."""
+    f = 4 if mixed else 8
+    nnz = sum(m.nnz for m in mats)
+    return f * R + 4 * (R + 1) * len(mats) + (4 + f) * nnz + 4 * f * R + 32 * R
+
+
+# HBM bytes per row of one k_btay_term<float> launch, from FETCH_SIZE and WRITE_SIZE counter
+# passes over the roofline loop below, each scaled by the counter's reading for a 1 GiB device
+# copy (profiles/k_btay_term_traj256_hbm_traffic.txt). Only the mixed-precision kernel is measured.
+BTAY_TRAFFIC_BYTES_PER_ROW = {"mixed": 101.15}
+
+
+def measure_batched_term(bte, reps=20):
+    """Average duration of paces_batch_taylor_term (k_btay_term + its per-trajectory reduction)
+    at the current basis with the stopping test disabled (tol < 0), so every launch processes
+    every row; HIP events on the stream the kernels run on."""
+    import ctypes as C
+    import torch
+    from paces_amd import _hip
+    from paces_amd.core import engine
+    lib = _hip.load()
+    h = bte.hamobj
+    R = bte.numstates
+    mixed = bte.precision == "mixed"
+    mats = [bte.sparse_mats_dict[t] for t in h.offdiag_terms]
+    vdt = torch.complex64 if mixed else torch.complex128
+    res = torch.empty_like(bte.vector)
+    bufs = [torch.empty(R, dtype=vdt, device=bte.device) for _ in range(3)]
+    ts = torch.zeros(h.ntraj * C.sizeof(_hip.BatchTaylor), dtype=torch.uint8, device=bte.device)
+    hdr = torch.zeros(16, dtype=torch.uint8, device=bte.device)
+    ws = _hip.Workspace(bte.device)
+    wsp, wsn = ws.get(lib.paces_batch_taylor_workspace_bytes(R, h.ntraj))
+    st = torch.cuda.current_stream()
+    sp = C.c_void_p(st.cuda_stream)
+    vr = _hip.ptr(torch.view_as_real(bte.vector))
+    _hip.check(lib.paces_batch_taylor_begin(R, vr, _hip.ptr(torch.view_as_real(res)),
+                                            _hip.ptr(torch.view_as_real(bufs[2])),
+                                            1 if mixed else 0, _hip.ptr(bte.seg_off), h.ntraj,
+                                            _hip.ptr(ts), wsp, wsn, sp))
+    if mixed:
+        mm = engine.MixedMats(bte.diag_vals, mats)
+        h64, h32, dptr = None, C.byref(mm.cset), _hip.ptr(mm.diag)
+    else:
+        s64 = engine.csr_set(mats)
+        h64, h32, dptr = C.byref(s64), None, _hip.ptr(bte.diag_vals)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    x_in = bufs[2]
+    for j in range(reps + 2):
+        if j == 2:
+            e0.record(st)
+        x_out = bufs[j % 2]
+        _hip.check(lib.paces_batch_taylor_term(
+            R, dptr, h64, h32, _hip.ptr(torch.view_as_real(x_in)),
+            _hip.ptr(torch.view_as_real(x_out)), _hip.ptr(torch.view_as_real(res)),
+            engine.taylor_coeff(TRAJ_DT, j), j, -1.0, _hip.ptr(bte.seg_off), h.ntraj,
+            _hip.ptr(ts), _hip.ptr(hdr), wsp, wsn, sp))
+        x_in = x_out
+    e1.record(st)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps * 1e-3, btay_term_bytes(R, mats, mixed), reps
+
+
+def batched_leg(draws, precision, warmup, steps, label):
+    """Batched run: step 0 and ``warmup`` steps untimed, then ``steps`` timed steps. Returns
+    (state-steps, seconds, the evolution object)."""
+    import torch
+    from paces_amd.models import holstein
+    bham = holstein.BatchedHamiltonian([d[0] for d in draws], 8, [-d[1] for d in draws],
+                                       [d[2] for d in draws])
+    bte = holstein.BatchedTimeEvolution(bham, [d[3] for d in draws], enlarge_steps=1,
+                                        precision=precision)
+    bte.create_single_exciton_states([d[0] // 2 for d in draws])
+    bte.step(0, TRAJ_DT)
+    for i in range(1, warmup + 1):
+        bte.step(i, TRAJ_DT)
+        print(f"[{label}] warmup step {i}: basis {bte.numstates}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    units = 0
+    t0 = time.perf_counter()
+    for i in range(warmup + 1, warmup + 1 + steps):
+        prev = np.diff(bte.seg_off.cpu().numpy())
+        bte.step(i, TRAJ_DT)
+        units += int(np.minimum(prev, bte.maxstates).sum())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    print(f"[{label}] timed {steps} steps in {elapsed:.3f} s, basis {bte.numstates}",
+          file=sys.stderr, flush=True)
+    return units, elapsed, bte
+
+
+def plain_leg(draws, warmup, steps):
+    """Every trajectory stepped by its own plain FP64 engine, one after the other."""
     import torch
     from paces_amd.core.time_evolution import ExpmParams, TimeEvoParams
     from paces_amd.models import holstein
+    tes = []
+    for (N, J, g, ms) in draws:
+        ham = holstein.Hamiltonian(N, [8] * N, use_terms=use_terms(J, g))
+        te = holstein.TimeEvolution(ham, holstein.Observables, ["n_exc", "n_pho"], None,
+                                    TimeEvoParams(maxstates=ms, enlarge_steps=1,
+                                                  shuffle_seed=None, diagnostics=False),
+                                    ExpmParams())
+        te.create_nonuniform_basis([1] * N, minpos=N // 2, maxpos=N // 2 + 1)
+        raw = np.zeros((1, N + 1), dtype=np.int64)
+        raw[0, 0] = N // 2
+        te.create_initial_vector([1.0], raw)
+        te._perform_first_step(0.0)
+        tes.append(te)
+    for i in range(1, warmup + 1):
+        for te in tes:
+            te._evolve_single_step(i, TRAJ_DT * i, TRAJ_DT)
+        print(f"[plain] warmup step {i}: basis {sum(te.numstates for te in tes)}",
+              file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    units = 0
+    t0 = time.perf_counter()
+    for i in range(warmup + 1, warmup + 1 + steps):
+        for te, d in zip(tes, draws):
+            prev = te.numstates
+            te._evolve_single_step(i, TRAJ_DT * i, TRAJ_DT)
+            units += min(prev, d[3])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    print(f"[plain] timed {steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    return units, elapsed
+
+
+def cpu_baseline_traj(draws, budget_s=25.0, max_ms=20000):
+    """The oracle on this host, one trajectory at a time, in draw order over the draws with
+    maxstates <= max_ms: each is grown untimed until its basis holds maxstates rows (at most 12
+    steps) plus one step, so that the selection saturates, then two whole steps are timed; stops
+    when the timed budget is used."""
+    from oracle import paces_oracle as po
+    units, elapsed, ntraj, nsteps = 0, 0.0, 0, 0
+    wall0 = time.perf_counter()
+    for (N, J, g, ms) in draws:
+        if ms > max_ms:
+            continue
+        if elapsed >= budget_s or time.perf_counter() - wall0 > 3 * budget_s:
+            break
+        ham = po.HolsteinOracle(N, [8] * N, use_terms=use_terms(J, g))
+        te = po.TimeEvolutionOracle(ham, ms, 1)
+        te.create_single_state_basis(N // 2)
+        raw = np.zeros((1, N + 1), dtype=np.int64)
+        raw[0, 0] = N // 2
+        te.create_initial_vector([1.0], raw)
+        te.step(0, 0.0, 0.0)
+        i = 0
+        while i < 12 and te.whoami.shape[0] < ms:
+            i += 1
+            te.step(i, TRAJ_DT * i, TRAJ_DT)
+        i += 1  # one more: the selection now saturates maxstates
+        te.step(i, TRAJ_DT * i, TRAJ_DT)
+        for _ in range(2):
+            prev = te.whoami.shape[0]
+            i += 1
+            t0 = time.perf_counter()
+            te.step(i, TRAJ_DT * i, TRAJ_DT)
+            elapsed += time.perf_counter() - t0
+            units += min(prev, ms)
+            nsteps += 1
+        ntraj += 1
+    return {"value": units / elapsed, "unit": "state-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (NumPy, one thread) on the first {ntraj} draws with maxstates <= "
+                      f"{max_ms}, each grown untimed to steady state, then {nsteps} timed whole "
+                      f"steps: {units} retained state-steps in {elapsed:.1f} s"}
+
+
+def run_traj256(a):
+    """Headline line: the batched form (default mixed precision) of the 256-trajectory workload,
+    with the plain FP64 form's throughput on the same trajectories, the batched form's density
+    error against the batched FP64 form (which matches the oracle to 1e-10), the Taylor term's
+    roofline and the oracle's CPU throughput."""
+    import torch
     draws = traj256_draws(a.ntraj)
-    dt = 0.05
-    if a.form == "batched":
-        bham = holstein.BatchedHamiltonian([d[0] for d in draws], 8, [-d[1] for d in draws],
-                                           [d[2] for d in draws])
-        bte = holstein.BatchedTimeEvolution(bham, [d[3] for d in draws], enlarge_steps=1,
-                                            precision=a.precision_batched)
-        bte.create_single_exciton_states([d[0] // 2 for d in draws])
-        bte.step(0, dt)
-        for i in range(1, a.warmup + 1):
-            bte.step(i, dt)
-            print(f"warmup step {i}: basis {bte.numstates}", file=sys.stderr, flush=True)
-        torch.cuda.synchronize()
-        units = 0
-        t0 = time.perf_counter()
-        for i in range(a.warmup + 1, a.warmup + 1 + a.steps):
-            segs = bte.seg_off.cpu().numpy()
-            prev = np.diff(segs)
-            bte.step(i, dt)
-            units += int(np.minimum(prev, bte.maxstates).sum())
-            print(f"timed step {i}: basis {bte.numstates}", file=sys.stderr, flush=True)
-        torch.cuda.synchronize()
-        elapsed = time.perf_counter() - t0
-        rows = bte.numstates
-        dtype = "c64 terms / f64 state" if a.precision_batched == "mixed" else "f64"
-    else:
-        tes = []
-        for (N, J, g, ms) in draws:
-            ham = holstein.Hamiltonian(N, [8] * N, use_terms=use_terms(J, g))
-            te = holstein.TimeEvolution(ham, holstein.Observables, ["n_exc", "n_pho"], None,
-                                        TimeEvoParams(maxstates=ms, enlarge_steps=1,
-                                                      shuffle_seed=None, diagnostics=False),
-                                        ExpmParams())
-            te.create_nonuniform_basis([1] * N, minpos=N // 2, maxpos=N // 2 + 1)
-            raw = np.zeros((1, N + 1), dtype=np.int64)
-            raw[0, 0] = N // 2
-            te.create_initial_vector([1.0], raw)
-            te._perform_first_step(0.0)
-            tes.append(te)
-        for i in range(1, a.warmup + 1):
-            for te in tes:
-                te._evolve_single_step(i, dt * i, dt)
-            print(f"warmup step {i}: basis {sum(te.numstates for te in tes)}", file=sys.stderr,
-                  flush=True)
-        torch.cuda.synchronize()
-        units = 0
-        t0 = time.perf_counter()
-        for i in range(a.warmup + 1, a.warmup + 1 + a.steps):
-            for te, d in zip(tes, draws):
-                prev = te.numstates
-                te._evolve_single_step(i, dt * i, dt)
-                units += min(prev, d[3])
-        torch.cuda.synchronize()
-        elapsed = time.perf_counter() - t0
-        rows = sum(te.numstates for te in tes)
-        dtype = "f64"
-    line = {"metric": METRIC, "value": units / elapsed, "unit": "state-steps/s", "n_gpus": 1,
+    prec = a.precision_batched
+    units, elapsed, bte = batched_leg(draws, prec, a.warmup, a.steps, "batched-" + prec)
+    rows = bte.numstates
+    value = units / elapsed
+    sec, nbytes, reps = measure_batched_term(bte)
+    achieved = nbytes / sec / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(achieved / 8000.0, 4),
+            "traffic": (int(BTAY_TRAFFIC_BYTES_PER_ROW[prec] * rows)
+                        if prec in BTAY_TRAFFIC_BYTES_PER_ROW else None),
+            "kernel": "k_btay_term<%s> (paces_batch_taylor_term, stopping test off)"
+                      % ("float" if prec == "mixed" else "double"),
+            "avg_us": round(sec * 1e6, 2), "bytes_per_launch": int(nbytes), "rows": int(rows),
+            "launches_timed": reps}
+    n_exc_b = bte.observables()[0]
+    del bte
+    torch.cuda.empty_cache()
+    extra = {}
+    if not a.no_check and prec != "fp64":
+        _, _, ref = batched_leg(draws, "fp64", a.warmup, a.steps, "batched-fp64")
+        n_exc_r = ref.observables()[0]
+        extra["density_err_vs_fp64"] = float(np.max(np.abs(n_exc_b - n_exc_r)))
+        del ref
+        torch.cuda.empty_cache()
+    if not a.no_plain:
+        pu, pe = plain_leg(draws, a.warmup, a.plain_steps)
+        extra["plain_gpu_state_steps_per_s"] = pu / pe
+        extra["plain_ms_per_step"] = 1e3 * pe / a.plain_steps
+        extra["speedup_vs_plain"] = value / (pu / pe)
+        torch.cuda.empty_cache()
+    cpu = None if a.no_cpu_baseline else cpu_baseline_traj(draws, a.cpu_budget)
+    dtype = "c64 terms / f64 state" if prec == "mixed" else "f64"
+    line = {"metric": METRIC, "value": value, "unit": "state-steps/s", "n_gpus": 1,
             "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * elapsed / a.steps,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
             "data": "synthetic",
-            "config": {"workload": f"traj256: {a.ntraj} 1D Holstein chains, N in {{16,24,32,48}}, "
-                                   "J~U[0.5,1.5], g~U[0.5,2], cap 7, maxstates~10^U[3,5.3], "
-                                   "dt=0.05, enlarge_steps=1 (seed 0)",
-                       "form": a.form, "final_basis_rows": int(rows),
-                       "sum_maxstates": int(sum(d[3] for d in draws))},
+            "config": dict({"workload": TRAJ_DESC.format(n=a.ntraj), "form": "batched-" + prec,
+                            "unit_of_work": "retained basis state advanced one step "
+                                            "(sum over trajectories of min(maxstates, R_prev))",
+                            "final_basis_rows": int(rows),
+                            "sum_maxstates": int(sum(d[3] for d in draws))}, **extra),
+            "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def run_plain_form(a):
+    """The plain form on its own (per-trajectory engines), for a separate line."""
+    draws = traj256_draws(a.ntraj)
+    pu, pe = plain_leg(draws, a.warmup, a.steps)
+    line = {"metric": METRIC, "value": pu / pe, "unit": "state-steps/s", "n_gpus": 1,
+            "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * pe / a.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": TRAJ_DESC.format(n=a.ntraj), "form": "plain"},
             "roofline": None, "cpu_baseline": None}
     print(json.dumps(line), flush=True)
+    return 0
+
+
+def calibrate_copy(nbytes=1 << 30, reps=5):
+    """A plain device copy of known size, for scaling FETCH_SIZE / WRITE_SIZE counter readings:
+    each launch reads and writes ``nbytes``."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "launches": reps}), flush=True)
     return 0
 
 
@@ -213,19 +399,29 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=6)
-    ap.add_argument("--workload", default="ring64", choices=sorted(WORKLOADS) + ["traj256"])
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps after step 0 (default 8 for traj256, 6 otherwise)")
+    ap.add_argument("--workload", default="traj256", choices=sorted(WORKLOADS) + ["traj256"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=25.0)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--form", default="batched", choices=["batched", "plain"],
-                    help="traj256 only")
+                    help="traj256 only: 'plain' times the per-trajectory engines alone")
     ap.add_argument("--ntraj", type=int, default=256)
     ap.add_argument("--precision-batched", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--plain-steps", type=int, default=2,
+                    help="timed steps of the plain form beside the batched line")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-check", action="store_true",
+                    help="skip the batched FP64 run that gives the density error")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a 1 GiB device copy (counter calibration)")
     a = ap.parse_args(argv)
+    if a.warmup is None:
+        a.warmup = 8 if a.workload == "traj256" else 6
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported "
-              "(trajectories shard across GPUs, see DESIGN.md).", file=sys.stderr)
+              "(independent trajectories shard one process per GPU).", file=sys.stderr)
         return 2
 
     import torch
@@ -236,8 +432,10 @@ def main(argv=None):
     import contextlib
     with contextlib.redirect_stdout(sys.stderr):  # stdout carries only the JSON line
         devices.configure()
+    if a.calibrate_copy:
+        return calibrate_copy()
     if a.workload == "traj256":
-        return run_traj256(a)
+        return run_plain_form(a) if a.form == "plain" else run_traj256(a)
     wl = WORKLOADS[a.workload]
     ham = holstein.Hamiltonian(wl["N"], [wl["dim"]] * wl["N"], geo_dim=wl["geo"],
                                periodic=wl["periodic"], use_terms=use_terms(wl["J"], wl["g"]))

@@ -16,13 +16,13 @@ def terms(J=1.0, g=1.0, w0=1.0, eps=0.0, de=0.0, J2=None):
 
 
 CASES = {
-    # config 1: 1D OBC chain N=16, cap 7, J=1, g=1, exciton at 8
-    "cfg1": dict(N=16, dim=8, geo=1, periodic=False, terms=terms(1.0, 1.0), pos0=8,
+    # 1D OBC chain N=16, cap 7, J=1, g=1, exciton at 8
+    "chain16": dict(N=16, dim=8, geo=1, periodic=False, terms=terms(1.0, 1.0), pos0=8,
                  ms=20000, dt=0.05, steps=12, enlarge=1, fillfac=0.1),
-    # config 2 layout (64-site ring, cap 15, g=1.5, five uint64 words) at a reduced maxstates
-    "cfg2_small": dict(N=64, dim=16, geo=1, periodic=True, terms=terms(1.0, 1.5), pos0=32,
+    # 64-site ring layout (cap 15, g=1.5, five uint64 words) at a reduced maxstates
+    "ring64_small": dict(N=64, dim=16, geo=1, periodic=True, terms=terms(1.0, 1.5), pos0=32,
                        ms=4000, dt=0.02, steps=4, enlarge=1, fillfac=0.1),
-    # config 5 physics on a 6x6 torus, cap 7, g=1.2 (multi-word 2D rows)
+    # 2D periodic lattice: 6x6 torus, cap 7, g=1.2 (multi-word 2D rows)
     "torus6": dict(N=36, dim=8, geo=2, periodic=True, terms=terms(1.0, 1.2), pos0=21,
                    ms=3000, dt=0.02, steps=3, enlarge=0, fillfac=0.1),
     # OBC 2D lattice with next-nearest hopping and a site bias

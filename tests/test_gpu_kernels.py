@@ -224,7 +224,7 @@ def test_reassign_and_reductions(env):
                                rtol=0, atol=1e-13)
 
 
-GEN_CASES = ["cfg1", "cfg2_small", "torus6", "square4_hop2"]
+GEN_CASES = ["chain16", "ring64_small", "torus6", "square4_hop2"]
 
 
 @pytest.mark.parametrize("case", GEN_CASES)

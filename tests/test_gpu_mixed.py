@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 DENSITY_TOL = 1e-4
 
 
-@pytest.mark.parametrize("case,steps", [("cfg1", 60), ("cfg2_small", 40)])
+@pytest.mark.parametrize("case,steps", [("chain16", 60), ("ring64_small", 40)])
 def test_mixed_precision_density_error(case, steps):
     import torch
     if not torch.cuda.is_available():

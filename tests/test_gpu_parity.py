@@ -27,7 +27,7 @@ def torch_mod():
     return torch
 
 
-@pytest.mark.parametrize("case", ["cfg1", "cfg2_small", "torus6", "square4_hop2"])
+@pytest.mark.parametrize("case", ["chain16", "ring64_small", "torus6", "square4_hop2"])
 def test_timeline_parity(torch_mod, case):
     oham, recs, t = oracle_run(case)
     ham, te = gpu_setup(case)
@@ -60,7 +60,7 @@ def _oracle_h(rec):
 
 def test_lanczos_matches_oracle(torch_mod):
     from oracle import paces_oracle as po
-    case = dict(CASES["cfg1"])
+    case = dict(CASES["chain16"])
     case["steps"] = 3
     oham, recs, t = oracle_run(case)
     ham, te = gpu_setup(case)

@@ -19,11 +19,11 @@ def _rand_raw(rng, lay, n):
 
 
 LAYOUTS = [
-    [16] + [8] * 16,        # config 1: 4 + 48 bits, one word
-    [64] + [16] * 64,       # config 2: 6 + 256 bits, five words, fields straddle words
-    [256] + [8] * 256,      # config 5: 8 + 768 bits, 13 words
+    [16] + [8] * 16,        # 16-site chain, cap 7: 4 + 48 bits, one word
+    [64] + [16] * 64,       # 64-site ring, cap 15: 6 + 256 bits, five words, fields straddle words
+    [256] + [8] * 256,      # 16x16 torus, cap 7: 8 + 768 bits, 13 words
     [5] + [3, 7, 1, 2, 9],  # ragged widths incl. a zero-bit field
-    [24] + [8] * 24,        # config 3 draw N=24: 77 bits
+    [24] + [8] * 24,        # 24-site chain, cap 7: 77 bits
 ]
 
 
