@@ -310,6 +310,34 @@ int paces_batch_taylor_term(int64_t R, const void* diag, const paces_csr_set_t* 
                             paces_batch_taylor_hdr_t* hdr, void* ws, size_t ws_bytes,
                             void* stream);
 
+/* ---- N3 candidate scorer (no reference counterpart): 64 inputs -> 3 x 256 GELU -> 1 logit,
+ * BF16 matrix operands with FP32 accumulation on the matrix cores. params is the FP32 master
+ * vector (PACES_MLP_NPARAM floats):
+ *   Wt1 [256 x 64] | b1 [256] | Wt2 [256 x 256] | b2 | Wt3 [256 x 256] | b3 | w4 [256] | b4 [1]
+ * (Wt output-major); wbf16 holds BF16 copies of Wt1 | Wt2 | Wt3 (PACES_MLP_NBF16 values).
+ * Inputs X are BF16 rows of PACES_MLP_IN values. */
+#define PACES_MLP_IN 64
+#define PACES_MLP_HIDDEN 256
+#define PACES_MLP_NPARAM 148481
+#define PACES_MLP_NBF16 147456
+
+/* wbf16 = bf16(Wt1 | Wt2 | Wt3) of params */
+int paces_mlp_weights_bf16(const float* params, uint16_t* wbf16, void* stream);
+size_t paces_mlp_train_workspace_bytes(int64_t B);
+/* Gradient of the weighted mean binary cross-entropy (1/B) sum_b w_b (softplus(z_b) - y_b z_b),
+ * w_b = pos_weight for y_b = 1, else 1, over B rows X[idx[b]] with labels labels[idx[b]] (idx may
+ * be null: rows 0..B-1). grads receives PACES_MLP_NPARAM floats; *loss (optional, device) the
+ * loss. Deterministic (fixed reduction order). */
+int paces_mlp_grad(const uint16_t* X, const int32_t* idx, const uint8_t* labels, int64_t B,
+                   float pos_weight, const float* params, const uint16_t* wbf16, float* grads,
+                   float* loss, void* ws, size_t ws_bytes, void* stream);
+/* One Adam step (step counted from 1) on params with moments m, v; refreshes wbf16. */
+int paces_mlp_adam(float* params, const float* grads, float* m, float* v, int32_t step, float lr,
+                   float beta1, float beta2, float eps, uint16_t* wbf16, void* stream);
+/* prob[i] = sigmoid(z_i) and/or logit[i] = z_i for n rows of X (all layers fused). */
+int paces_mlp_infer(const uint16_t* X, int64_t n, const float* params, const uint16_t* wbf16,
+                    float* prob, float* logit, void* stream);
+
 /* ---- deterministic vector reductions (norms, vdot) and Lanczos helpers (E3,
  * paces/core/time_evolution.py:515-549). out is a device double[2] (re, im) or double[1]. */
 size_t paces_reduce_workspace_bytes(int64_t R);

@@ -1,0 +1,145 @@
+"""NumPy oracle of the candidate scorer (test infrastructure only: imported by tests/ and nothing
+in the product path).
+
+The scorer has no counterpart in the reference. It is the network described for the learned
+proposal:
+* 64 inputs, three hidden layers of 256 with GELU (erf form), and one sigmoid output;
+* BF16 weights and activations as matrix operands, with FP32 accumulation;
+* trained with a weighted binary cross-entropy and Adam (Kingma & Ba 2015, Algorithm 1).
+
+``bf16=True`` rounds every matrix operand to bfloat16 (round to nearest even) at the points
+where the HIP kernels do, so the two can be compared closely. ``bf16=False`` gives the plain
+float64 network that the central-difference gradient check pins.
+
+Parameter vector layout (float32, shared with the kernels):
+    Wt1 [256 x 64], b1 [256], Wt2 [256 x 256], b2 [256], Wt3 [256 x 256], b3 [256], w4 [256], b4 [1]
+Wt is stored output-major (row n holds the weights into output n).
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+IN, HID = 64, 256
+SHAPES = [("Wt1", (HID, IN)), ("b1", (HID,)), ("Wt2", (HID, HID)), ("b2", (HID,)),
+          ("Wt3", (HID, HID)), ("b3", (HID,)), ("w4", (HID,)), ("b4", (1,))]
+OFFSETS = {}
+_o = 0
+for _name, _shape in SHAPES:
+    OFFSETS[_name] = (_o, _shape)
+    _o += int(np.prod(_shape))
+NPARAM = _o
+WEIGHTS_BF16 = ["Wt1", "Wt2", "Wt3"]  # the matrices the kernels read as bf16 operands
+
+
+def unpack(theta):
+    return {k: theta[o:o + int(np.prod(s))].reshape(s) for k, (o, s) in OFFSETS.items()}
+
+
+def pack(d):
+    out = np.zeros(NPARAM, dtype=np.float64)
+    for k, (o, s) in OFFSETS.items():
+        out[o:o + int(np.prod(s))] = np.asarray(d[k], dtype=np.float64).ravel()
+    return out
+
+
+def init_params(seed=0):
+    """He-uniform hidden weights, small output weights, zero biases (float32)."""
+    rng = np.random.default_rng(seed)
+    d = {}
+    for name, shape in SHAPES:
+        if name.startswith("Wt"):
+            lim = math.sqrt(6.0 / shape[1])
+            d[name] = rng.uniform(-lim, lim, shape)
+        elif name == "w4":
+            lim = math.sqrt(3.0 / shape[0])
+            d[name] = rng.uniform(-lim, lim, shape)
+        else:
+            d[name] = np.zeros(shape)
+    return pack(d).astype(np.float32)
+
+
+def bf16(x):
+    """Round float values to bfloat16 (nearest even), returned as float64."""
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    u = a.view(np.uint32).astype(np.uint64)
+    nan = (u & 0x7FFFFFFF) > 0x7F800000
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    u = np.where(nan, 0x7FC0, u).astype(np.uint32) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+_erf = np.vectorize(math.erf, otypes=[np.float64])
+
+
+def gelu(a):
+    return 0.5 * a * (1.0 + _erf(a / math.sqrt(2.0)))
+
+
+def gelu_grad(a):
+    cdf = 0.5 * (1.0 + _erf(a / math.sqrt(2.0)))
+    pdf = np.exp(-0.5 * a * a) / math.sqrt(2.0 * math.pi)
+    return cdf + a * pdf
+
+
+def forward(theta, X, bf16_ops=True):
+    """Returns a dict with pre-activations a1..a3, activations h0..h3 (the operands) and logits z."""
+    P = unpack(np.asarray(theta, dtype=np.float64))
+    r = bf16 if bf16_ops else (lambda v: np.asarray(v, dtype=np.float64))
+    h0 = r(X)
+    a1 = h0 @ r(P["Wt1"]).T + P["b1"]
+    h1 = r(gelu(a1))
+    a2 = h1 @ r(P["Wt2"]).T + P["b2"]
+    h2 = r(gelu(a2))
+    a3 = h2 @ r(P["Wt3"]).T + P["b3"]
+    h3 = r(gelu(a3))
+    z = h3 @ P["w4"] + P["b4"][0]
+    return dict(h0=h0, a1=a1, h1=h1, a2=a2, h2=h2, a3=a3, h3=h3, z=z)
+
+
+def sigmoid(z):
+    return 1.0 / (1.0 + np.exp(-z))
+
+
+def loss(theta, X, y, pos_weight=1.0, bf16_ops=True):
+    """Weighted mean binary cross-entropy with logits: (1/B) sum w_b (softplus(z) - y z)."""
+    z = forward(theta, X, bf16_ops)["z"]
+    w = np.where(y > 0, pos_weight, 1.0)
+    sp = np.maximum(z, 0) + np.log1p(np.exp(-np.abs(z)))
+    return float(np.sum(w * (sp - y * z)) / len(z))
+
+
+def backward(theta, X, y, pos_weight=1.0, bf16_ops=True):
+    """Gradient of ``loss`` as a parameter vector (float64), with the kernels' rounding points:
+    the back-propagated pre-activation gradients are bf16 operands of the next products."""
+    P = unpack(np.asarray(theta, dtype=np.float64))
+    r = bf16 if bf16_ops else (lambda v: np.asarray(v, dtype=np.float64))
+    F = forward(theta, X, bf16_ops)
+    B = len(y)
+    w = np.where(y > 0, pos_weight, 1.0)
+    dz = w * (sigmoid(F["z"]) - y) / B
+    g = {}
+    g["w4"] = F["h3"].T @ dz
+    g["b4"] = np.array([dz.sum()])
+    da3 = r(dz[:, None] * P["w4"][None, :] * gelu_grad(F["a3"]))
+    g["Wt3"] = da3.T @ F["h2"]
+    g["b3"] = da3.sum(0)
+    da2 = r((da3 @ r(P["Wt3"])) * gelu_grad(F["a2"]))
+    g["Wt2"] = da2.T @ F["h1"]
+    g["b2"] = da2.sum(0)
+    da1 = r((da2 @ r(P["Wt2"])) * gelu_grad(F["a1"]))
+    g["Wt1"] = da1.T @ F["h0"]
+    g["b1"] = da1.sum(0)
+    return pack(g)
+
+
+def adam(theta, grad, m, v, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam update (Kingma & Ba 2015, Algorithm 1), step counted from 1. Returns new
+    (theta, m, v) in float64."""
+    m = b1 * m + (1.0 - b1) * grad
+    v = b2 * v + (1.0 - b2) * grad * grad
+    mh = m / (1.0 - b1 ** step)
+    vh = v / (1.0 - b2 ** step)
+    return theta - lr * mh / (np.sqrt(vh) + eps), m, v

@@ -86,7 +86,8 @@ class TaylorState(C.Structure):
                 ("j_ind", C.c_int32), ("pad", C.c_int32 * 4)]
 
 
-_P, _I64, _I32, _D, _SZ = C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_size_t
+_P, _I64, _I32, _D, _SZ, _F = (C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_size_t,
+                               C.c_float)
 
 _SIGNATURES = {
     "paces_strerror": (C.c_char_p, [C.c_int]),
@@ -151,6 +152,11 @@ _SIGNATURES = {
     "paces_norm2": (C.c_int, [_P, _I64, _P, _P, _SZ, _P]),
     "paces_vdot": (C.c_int, [_P, _P, _I64, _P, _P, _SZ, _P]),
     "paces_axpby": (C.c_int, [_I64, _D, _D, _P, _D, _D, _P, _P]),
+    "paces_mlp_weights_bf16": (C.c_int, [_P, _P, _P]),
+    "paces_mlp_train_workspace_bytes": (_SZ, [_I64]),
+    "paces_mlp_grad": (C.c_int, [_P, _P, _P, _I64, _F, _P, _P, _P, _P, _P, _SZ, _P]),
+    "paces_mlp_adam": (C.c_int, [_P, _P, _P, _P, _I32, _F, _F, _F, _F, _P, _P]),
+    "paces_mlp_infer": (C.c_int, [_P, _I64, _P, _P, _P, _P, _P]),
 }
 
 
