@@ -338,10 +338,45 @@ int paces_mlp_adam(float* params, const float* grads, float* m, float* v, int32_
 int paces_mlp_infer(const uint16_t* X, int64_t n, const float* params, const uint16_t* wbf16,
                     float* prob, float* logit, void* stream);
 
+/* ---- N3 candidate features and pruning (no reference counterpart).
+ * paces_holstein_features writes one row of PACES_MLP_IN bf16 features per union row (R rows,
+ * amplitudes psi and diagonal diag on the union): the phonon window of +-16 sites around the
+ * exciton over (dim - 1) (-1 outside an open lattice), log10 of the largest neighbour |psi|
+ * through the image pairs (nbasis, host run_offsets, cand_pos and img_src as for
+ * paces_holstein_pairs_count), the scaled diagonal, log10 |psi| of the row, the vibronic and
+ * hopping values and pos / nchain. Single trajectories only. */
+size_t paces_holstein_features_workspace_bytes(int64_t R);
+int paces_holstein_features(const uint64_t* rows, int64_t R, const double* psi,
+                            const double* diag, int64_t nbasis, const int64_t* run_offsets,
+                            const int32_t* cand_pos, const int32_t* img_src,
+                            const paces_layout_t* lay, const paces_holstein_t* h, uint16_t* X,
+                            void* ws, size_t ws_bytes, void* stream);
+/* keep[i] = 1 when prob[i] >= threshold or i is one of forced_idx; else 2 when sample_bits >= 0
+ * and the low sample_bits bits of a hash of (i, seed) are zero (a sample of the pruned rows kept
+ * to measure their weight); else 0. */
+int paces_prune_plan(const float* prob, int64_t n, float threshold, const int32_t* forced_idx,
+                     int64_t nforced, int32_t sample_bits, uint64_t seed, uint8_t* keep,
+                     void* stream);
+/* out[idx[j]] = src[idx[j]] (or 1 when src is null) for j < m; other entries are untouched. */
+int paces_mark_indices(const int32_t* idx, int64_t m, const uint8_t* src, uint8_t* out,
+                       void* stream);
+/* Rows with keep != 0 in order, with their psi (optional), in_basis flag (optional) and
+ * out_sample = (keep == 2); out_cand_pos[g] = the kept index of cand_pos[g], or -1 when that row
+ * was dropped (the pairs kernels then skip the pair). *out_count (device) = kept rows. */
+size_t paces_compact_workspace_bytes(int64_t n);
+int paces_compact_kept(const uint8_t* keep, int64_t n, const uint64_t* rows, int32_t W,
+                       const double* psi, const uint8_t* in_basis, const int32_t* cand_pos,
+                       int64_t C, uint64_t* out_rows, double* out_psi, uint8_t* out_in_basis,
+                       uint8_t* out_sample, int32_t* out_cand_pos, int32_t* out_count, void* ws,
+                       size_t ws_bytes, void* stream);
+
 /* ---- deterministic vector reductions (norms, vdot) and Lanczos helpers (E3,
  * paces/core/time_evolution.py:515-549). out is a device double[2] (re, im) or double[1]. */
 size_t paces_reduce_workspace_bytes(int64_t R);
 int paces_norm2(const double* x, int64_t R, double* out, void* ws, size_t ws_bytes, void* stream);
+/* sum of |x_i|^2 over the rows with mask[i] != 0 (N3: weight of the sampled pruned rows) */
+int paces_norm2_masked(const double* x, const uint8_t* mask, int64_t R, double* out, void* ws,
+                       size_t ws_bytes, void* stream);
 int paces_vdot(const double* x, const double* y, int64_t R, double* out, void* ws,
                size_t ws_bytes, void* stream);
 /* y = a*x + b*y with complex scalars given on the host; with b == 0, y is only written */

@@ -157,6 +157,15 @@ _SIGNATURES = {
     "paces_mlp_grad": (C.c_int, [_P, _P, _P, _I64, _F, _P, _P, _P, _P, _P, _SZ, _P]),
     "paces_mlp_adam": (C.c_int, [_P, _P, _P, _P, _I32, _F, _F, _F, _F, _P, _P]),
     "paces_mlp_infer": (C.c_int, [_P, _I64, _P, _P, _P, _P, _P]),
+    "paces_holstein_features_workspace_bytes": (_SZ, [_I64]),
+    "paces_holstein_features": (C.c_int, [_P, _I64, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P,
+                                          _SZ, _P]),
+    "paces_prune_plan": (C.c_int, [_P, _I64, _F, _P, _I64, _I32, C.c_uint64, _P, _P]),
+    "paces_norm2_masked": (C.c_int, [_P, _P, _I64, _P, _P, _SZ, _P]),
+    "paces_mark_indices": (C.c_int, [_P, _I64, _P, _P, _P]),
+    "paces_compact_workspace_bytes": (_SZ, [_I64]),
+    "paces_compact_kept": (C.c_int, [_P, _I64, _P, _I32, _P, _P, _P, _I64, _P, _P, _P, _P, _P,
+                                     _P, _P, _SZ, _P]),
 }
 
 

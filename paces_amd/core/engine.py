@@ -101,15 +101,20 @@ def merge_unique(runs, W, ws, want_from0=False, want_cand_pos=False):
     return out[:n], (from0[:n] if want_from0 else None)
 
 
-def select_topk(psi, rows, k, ws):
+def select_topk(psi, rows, k, ws, want_index=False):
+    """Rows of the k largest |psi| (ties keep the highest indices), in index order; with
+    want_index also their indices (int32)."""
     R, W = rows.shape
     k = int(min(k, R))
     out = torch.empty((k, W), dtype=torch.int64, device=rows.device)
-    if k == 0:
-        return out
-    wsp, wsn = ws.get(lib.paces_select_workspace_bytes(R))
-    check(lib.paces_select_topk(ptr(torch.view_as_real(psi)), ptr(rows), R, W, k, ptr(out), None,
-                                wsp, wsn, stream()), "select_topk")
+    idx = torch.empty(max(k, 1), dtype=torch.int32, device=rows.device) if want_index else None
+    if k > 0:
+        wsp, wsn = ws.get(lib.paces_select_workspace_bytes(R))
+        check(lib.paces_select_topk(ptr(torch.view_as_real(psi)), ptr(rows), R, W, k, ptr(out),
+                                    ptr(idx) if want_index else None, wsp, wsn, stream()),
+              "select_topk")
+    if want_index:
+        return out, idx[:k]
     return out
 
 
@@ -150,6 +155,15 @@ class Reducer:
 
     def norm(self, x):
         return math.sqrt(self.norm2(x))
+
+    def norm2_masked(self, x, mask):
+        """sum of |x_i|^2 over rows with mask[i] != 0 (uint8 mask) -> float."""
+        R = x.shape[0]
+        wsp, wsn = self.ws.get(lib.paces_reduce_workspace_bytes(R))
+        check(lib.paces_norm2_masked(ptr(torch.view_as_real(x)) if R else None,
+                                     ptr(mask) if R else None, R, ptr(self.out), wsp, wsn,
+                                     stream()), "norm2_masked")
+        return float(self.out[0].item())
 
     def vdot(self, x, y):
         R = x.shape[0]

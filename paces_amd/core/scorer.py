@@ -112,17 +112,15 @@ class CandidateScorer:
                                         ptr(out) if want_logit else None, stream()), "mlp_infer")
         return out[:n]
 
-    def train(self, X, labels, epochs=1, batch=8192, lr=1e-3, pos_weight=None, seed=0,
+    def train(self, X, labels, epochs=1, batch=8192, lr=1e-3, pos_weight=1.0, seed=0,
               log_every=0):
         """Minibatch Adam over (X, labels) on the device; rows are drawn by a seeded permutation
-        per epoch and gathered by the kernels. pos_weight defaults to the negative/positive
-        ratio (capped at 100). Returns the mean loss of each epoch."""
+        per epoch and gathered by the kernels. The default unweighted loss keeps the output a
+        calibrated retain probability, so a pruning threshold reads as one. Returns the mean
+        loss of each epoch."""
         n = int(X.shape[0])
         if n == 0:
             return []
-        if pos_weight is None:
-            npos = int(labels.sum().item())
-            pos_weight = min(100.0, (n - npos) / max(npos, 1))
         gen = torch.Generator(device="cpu")
         gen.manual_seed(seed)
         hist = []

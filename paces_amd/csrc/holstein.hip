@@ -495,8 +495,9 @@ __global__ void __launch_bounds__(kBlock) k_pairs_count(int64_t R, int64_t nbasi
   if (e >= IR.off[IR.n]) return;
   const int k = img_run_of(IR, e);
   const int32_t p = cand_pos[nbasis + e];
-  if (T.dir[k] < 0 && inB[p]) return;
   const int32_t q = cand_pos[src[e]];
+  if (p < 0 || q < 0) return;  // an endpoint pruned from the basis
+  if (T.dir[k] < 0 && inB[p]) return;
   const int64_t t = T.term[k];
   atomicAdd(&cnt[t * R + q], 1);
   atomicAdd(&cnt[t * R + p], 1);
@@ -530,8 +531,9 @@ __global__ void __launch_bounds__(kBlock) k_pairs_fill(int64_t R, int64_t nbasis
   if (e >= IR.off[IR.n]) return;
   const int k = img_run_of(IR, e);
   const int32_t p = cand_pos[nbasis + e];
-  if (T.dir[k] < 0 && inB[p]) return;
   const int32_t q = cand_pos[src[e]];
+  if (p < 0 || q < 0) return;
+  if (T.dir[k] < 0 && inB[p]) return;
   const int t = T.term[k];
   const int traj = M.batched ? (int)(basis[(int64_t)src[e] * W] >> (64 - trajbits)) : 0;
   const double v = op_value(T, k, M, traj, sqrt_tab, nup[e]);
@@ -771,6 +773,116 @@ static LayoutArg larg(const paces_layout_t* lay) {
 
 using namespace paces;
 
+// ------------------------------------------------------------------------------------------------
+// N3 candidate features (no reference counterpart): one row of PACES_MLP_IN bf16 values per union
+// row, the input of the candidate scorer.
+//   [0, 33)  phonon count at sites pos-16 .. pos+16 over (dim - 1); -1 outside an open lattice
+//   33       log10 of the largest |psi| among the row's H-neighbours through the image pairs, / 10
+//   34       (diag - eps_sys) / (hbar_omega * nchain)
+//   35       log10 |psi| of the row itself, / 10 (amplitudes below 1e-20 count as 1e-20)
+//   36, 37   the vibronic and hopping term values; 38 pos / nchain; the rest 0
+// ------------------------------------------------------------------------------------------------
+
+constexpr int kFeatWin = 16;
+
+__global__ void __launch_bounds__(kBlock) k_zero_u64(u64* __restrict__ p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// par[i] = max |psi_j|^2 over the pairs {i, j} of the images (bit patterns of non-negative
+// doubles order like the values)
+__global__ void __launch_bounds__(kBlock) k_parent_amp(int64_t nbasis, int64_t nimg,
+                                                       const int32_t* __restrict__ cand_pos,
+                                                       const int32_t* __restrict__ src,
+                                                       const double2* __restrict__ psi,
+                                                       u64* __restrict__ par) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nimg) return;
+  const int32_t p = cand_pos[nbasis + e];
+  const int32_t q = cand_pos[src[e]];
+  if (p < 0 || q < 0) return;
+  const double2 a = psi[q], b = psi[p];
+  atomicMax(&par[p], (u64)__double_as_longlong(a.x * a.x + a.y * a.y));
+  atomicMax(&par[q], (u64)__double_as_longlong(b.x * b.x + b.y * b.y));
+}
+
+__device__ __forceinline__ uint16_t feat_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float feat_log_amp(double w2) {
+  double l = (w2 > 1e-40) ? 0.5 * log10(w2) : -20.0;
+  return (float)(l / 10.0);
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_features(const u64* __restrict__ rows, int64_t R,
+                                                     const double2* __restrict__ psi,
+                                                     const double* __restrict__ diag,
+                                                     const u64* __restrict__ par, LayoutArg L,
+                                                     int N, int periodic, double eps,
+                                                     double inv_scale, float gfeat, float jfeat,
+                                                     uint16_t* __restrict__ X) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const Row<W> r = load_row<W>(rows, i);
+  const int pos = (int)get_field<W>(r, L.off[0], L.width[0]);
+  const double2 a = psi[i];
+  uint32_t packed[PACES_MLP_IN / 2];
+#pragma unroll
+  for (int c2 = 0; c2 < PACES_MLP_IN / 2; ++c2) packed[c2] = 0;
+  for (int c = 0; c < PACES_MLP_IN; ++c) {
+    float v = 0.f;
+    if (c < 2 * kFeatWin + 1) {
+      int s = pos + c - kFeatWin;
+      bool in = true;
+      if (periodic) {
+        s %= N;
+        if (s < 0) s += N;
+      } else {
+        in = s >= 0 && s < N;
+      }
+      if (in) {
+        const int md = L.maxdim[s + 1];
+        const int n = (int)get_field<W>(r, L.off[s + 1], L.width[s + 1]);
+        v = md > 1 ? (float)n / (float)(md - 1) : 0.f;
+      } else {
+        v = -1.f;
+      }
+    } else if (c == 33) {
+      v = feat_log_amp(__longlong_as_double((long long)par[i]));
+    } else if (c == 34) {
+      v = (float)((diag[i] - eps) * inv_scale);
+    } else if (c == 35) {
+      v = feat_log_amp(a.x * a.x + a.y * a.y);
+    } else if (c == 36) {
+      v = gfeat;
+    } else if (c == 37) {
+      v = jfeat;
+    } else if (c == 38) {
+      v = (float)pos / (float)N;
+    }
+    packed[c >> 1] |= (uint32_t)feat_bf16(v) << (16 * (c & 1));
+  }
+  uint4* dst = (uint4*)(X + i * PACES_MLP_IN);
+#pragma unroll
+  for (int q = 0; q < PACES_MLP_IN / 8; ++q)
+    dst[q] = make_uint4(packed[4 * q], packed[4 * q + 1], packed[4 * q + 2], packed[4 * q + 3]);
+}
+
+template <int W>
+static int run_features(const uint64_t* rows, int64_t R, const double* psi, const double* diag,
+                        const u64* par, LayoutArg L, int N, int periodic, double eps,
+                        double inv_scale, float gfeat, float jfeat, uint16_t* X, hipStream_t st) {
+  hipLaunchKernelGGL(k_features<W>, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
+                     (const u64*)rows, R, (const double2*)psi, diag, par, L, N, periodic, eps,
+                     inv_scale, gfeat, jfeat, X);
+  return launch_status();
+}
+
 extern "C" {
 
 int paces_holstein_num_runs(const paces_holstein_t* h) {
@@ -994,4 +1106,44 @@ int paces_holstein_observables(const uint64_t* rows, const double* psi, int64_t 
                    (hipStream_t)stream);
 }
 
+size_t paces_holstein_features_workspace_bytes(int64_t R) { return a256((size_t)(R > 0 ? R : 1) * 8); }
+
+int paces_holstein_features(const uint64_t* rows, int64_t R, const double* psi,
+                            const double* diag, int64_t nbasis, const int64_t* run_offsets,
+                            const int32_t* cand_pos, const int32_t* img_src,
+                            const paces_layout_t* lay, const paces_holstein_t* h, uint16_t* X,
+                            void* ws, size_t ws_bytes, void* stream) {
+  int rc = check_h(lay, h);
+  if (rc) return rc;
+  if (h->batched) return PACES_E_UNSUPPORTED;
+  if (R < 0 || nbasis < 0 || !run_offsets || !ws) return PACES_E_INVALID;
+  if (R == 0) return PACES_OK;
+  if (!rows || !psi || !diag || !X) return PACES_E_INVALID;
+  if (ws_bytes < paces_holstein_features_workspace_bytes(R)) return PACES_E_WORKSPACE;
+  OpTable T;
+  rc = build_image_ops(h, &T);
+  if (rc) return rc;
+  const int64_t nimg = run_offsets[T.n] - run_offsets[0];
+  if (nimg < 0) return PACES_E_INVALID;
+  if (nimg > 0 && (!cand_pos || !img_src)) return PACES_E_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  u64* par = (u64*)ws;
+  hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st, par, R);
+  if (nimg > 0)
+    hipLaunchKernelGGL(k_parent_amp, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, nbasis,
+                       nimg, cand_pos, img_src, (const double2*)psi, par);
+  rc = launch_status();
+  if (rc) return rc;
+  float gfeat = 0.f, jfeat = 0.f;
+  for (int t = h->nterms - 1; t >= 0; --t) {
+    if (h->terms[t].kind == PACES_TERM_VIB) gfeat = (float)h->terms[t].value;
+    if (h->terms[t].kind == PACES_TERM_HOP && h->terms[t].order == 1)
+      jfeat = (float)h->terms[t].value;
+  }
+  const double inv_scale = 1.0 / (h->hbar_omega * (double)h->nchain);
+  PACES_DISPATCH_W(lay->W, run_features, rows, R, psi, diag, (const u64*)par, larg(lay),
+                   h->nchain, h->periodic ? 1 : 0, h->eps_sys, inv_scale, gfeat, jfeat, X, st);
+}
+
 }  // extern "C"
+

@@ -357,6 +357,112 @@ static int run_reassign(const uint64_t* old_rows, int64_t M, const double* old_p
   return launch_status();
 }
 
+// ------------------------------------------------------------------------------------------------
+// N3 pruning (no reference counterpart): which union rows to keep after scoring, and the
+// compaction of the kept rows with the candidate positions remapped into the kept numbering.
+// ------------------------------------------------------------------------------------------------
+
+// finaliser of MurmurHash3 (Appleby), 64-bit variant
+__device__ __forceinline__ uint64_t fmix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdULL;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ULL;
+  x ^= x >> 33;
+  return x;
+}
+
+__global__ void __launch_bounds__(kBlock) k_prune_plan(const float* __restrict__ prob, int64_t n,
+                                                       float thr, int sample, uint64_t smask,
+                                                       uint64_t seed, uint8_t* __restrict__ keep) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint8_t k = prob[i] >= thr ? 1 : 0;
+  if (!k && sample && (fmix64((uint64_t)i ^ seed) & smask) == 0) k = 2;
+  keep[i] = k;
+}
+
+__global__ void __launch_bounds__(kBlock) k_force_keep(const int32_t* __restrict__ idx, int64_t m,
+                                                       uint8_t* __restrict__ keep) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m) keep[idx[j]] = 1;
+}
+
+__global__ void __launch_bounds__(kBlock) k_keep_flags(const uint8_t* __restrict__ keep, int64_t n,
+                                                       uint8_t* __restrict__ flag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = keep[i] != 0 ? 1 : 0;
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_compact(const uint8_t* __restrict__ keep, int64_t n,
+                                                    const int32_t* __restrict__ G,
+                                                    const u64* __restrict__ rows,
+                                                    const double2* __restrict__ psi,
+                                                    const uint8_t* __restrict__ inb,
+                                                    u64* __restrict__ orows,
+                                                    double2* __restrict__ opsi,
+                                                    uint8_t* __restrict__ oinb,
+                                                    uint8_t* __restrict__ osample,
+                                                    int32_t* __restrict__ ocount) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *ocount = G[n];
+  if (i >= n || !keep[i]) return;
+  const int32_t j = G[i];
+  store_row<W>(orows, j, load_row<W>(rows, i));
+  if (opsi) opsi[j] = psi[i];
+  if (oinb) oinb[j] = inb[i];
+  if (osample) osample[j] = keep[i] == 2 ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_remap_pos(const int32_t* __restrict__ cand_pos,
+                                                      int64_t C, const uint8_t* __restrict__ keep,
+                                                      const int32_t* __restrict__ G,
+                                                      int32_t* __restrict__ out) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= C) return;
+  const int32_t p = cand_pos[g];
+  out[g] = keep[p] ? G[p] : -1;
+}
+
+__global__ void __launch_bounds__(kBlock) k_mark_indices(const int32_t* __restrict__ idx,
+                                                         int64_t m, const uint8_t* __restrict__ src,
+                                                         uint8_t* __restrict__ out) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const int32_t i = idx[j];
+  out[i] = src ? src[i] : 1;
+}
+
+size_t compact_ws_bytes(int64_t n) {
+  return align256((size_t)(n > 0 ? n : 1)) + align256((size_t)(n + 1) * 4) +
+         align256(scan_ws_bytes(n));
+}
+
+template <int W>
+static int run_compact(const uint8_t* keep, int64_t n, const uint64_t* rows, const double* psi,
+                       const uint8_t* inb, const int32_t* cand_pos, int64_t C, uint64_t* orows,
+                       double* opsi, uint8_t* oinb, uint8_t* osample, int32_t* ocand,
+                       int32_t* ocount, void* ws, hipStream_t st) {
+  char* p = (char*)ws;
+  uint8_t* flag = (uint8_t*)p;
+  p += align256((size_t)(n > 0 ? n : 1));
+  int32_t* G = (int32_t*)p;
+  p += align256((size_t)(n + 1) * 4);
+  hipLaunchKernelGGL(k_keep_flags, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, keep, n, flag);
+  int rc = launch_status();
+  if (rc) return rc;
+  rc = scan_u8(flag, n, G, p, scan_ws_bytes(n), st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_compact<W>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, keep, n,
+                     (const int32_t*)G, (const u64*)rows, (const double2*)psi, inb, (u64*)orows,
+                     (double2*)opsi, oinb, osample, ocount);
+  if (C > 0)
+    hipLaunchKernelGGL(k_remap_pos, dim3(grid_for(C, kBlock)), dim3(kBlock), 0, st, cand_pos, C,
+                       keep, (const int32_t*)G, ocand);
+  return launch_status();
+}
+
 }  // namespace paces
 
 using namespace paces;
@@ -439,4 +545,48 @@ int paces_reassign(const uint64_t* old_rows, int64_t M, const double* old_psi,
                    (hipStream_t)stream);
 }
 
+int paces_prune_plan(const float* prob, int64_t n, float threshold, const int32_t* forced_idx,
+                     int64_t nforced, int32_t sample_bits, uint64_t seed, uint8_t* keep,
+                     void* stream) {
+  if (n < 0 || nforced < 0 || sample_bits > 62) return PACES_E_INVALID;
+  if (n > 0 && (!prob || !keep)) return PACES_E_INVALID;
+  if (nforced > 0 && !forced_idx) return PACES_E_INVALID;
+  if (n == 0) return PACES_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const uint64_t smask = sample_bits > 0 ? ((1ULL << sample_bits) - 1ULL) : 0ULL;
+  hipLaunchKernelGGL(k_prune_plan, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, prob, n,
+                     threshold, sample_bits >= 0 ? 1 : 0, smask, seed, keep);
+  if (nforced > 0)
+    hipLaunchKernelGGL(k_force_keep, dim3(grid_for(nforced, kBlock)), dim3(kBlock), 0, st,
+                       forced_idx, nforced, keep);
+  return launch_status();
+}
+
+int paces_mark_indices(const int32_t* idx, int64_t m, const uint8_t* src, uint8_t* out,
+                       void* stream) {
+  if (m < 0 || (m > 0 && (!idx || !out))) return PACES_E_INVALID;
+  if (m == 0) return PACES_OK;
+  hipLaunchKernelGGL(k_mark_indices, dim3(grid_for(m, kBlock)), dim3(kBlock), 0,
+                     (hipStream_t)stream, idx, m, src, out);
+  return launch_status();
+}
+
+size_t paces_compact_workspace_bytes(int64_t n) { return compact_ws_bytes(n); }
+
+int paces_compact_kept(const uint8_t* keep, int64_t n, const uint64_t* rows, int32_t W,
+                       const double* psi, const uint8_t* in_basis, const int32_t* cand_pos,
+                       int64_t C, uint64_t* out_rows, double* out_psi, uint8_t* out_in_basis,
+                       uint8_t* out_sample, int32_t* out_cand_pos, int32_t* out_count, void* ws,
+                       size_t ws_bytes, void* stream) {
+  if (n < 0 || C < 0 || !out_count || !ws) return PACES_E_INVALID;
+  if (n > 0 && (!keep || !rows || !out_rows)) return PACES_E_INVALID;
+  if ((psi && !out_psi) || (in_basis && !out_in_basis)) return PACES_E_INVALID;
+  if (C > 0 && (!cand_pos || !out_cand_pos)) return PACES_E_INVALID;
+  if (n >= 2147483647LL) return PACES_E_INVALID;
+  if (ws_bytes < compact_ws_bytes(n)) return PACES_E_WORKSPACE;
+  PACES_DISPATCH_W(W, run_compact, keep, n, rows, psi, in_basis, cand_pos, C, out_rows, out_psi,
+                   out_in_basis, out_sample, out_cand_pos, out_count, ws, (hipStream_t)stream);
+}
+
 }  // extern "C"
+

@@ -282,7 +282,8 @@ constexpr int kRedTile = kBlock * 4;
 // mode 0: sum |x|^2 -> part[b]; mode 1: sum conj(x) y -> part[2b], part[2b+1]
 __global__ void __launch_bounds__(kBlock) k_red_partial(const double2* __restrict__ x,
                                                         const double2* __restrict__ y, int64_t R,
-                                                        int mode, double* __restrict__ part) {
+                                                        int mode, double* __restrict__ part,
+                                                        const uint8_t* __restrict__ mask) {
   __shared__ double red[2][kBlock / 64];
   const int64_t base = (int64_t)blockIdx.x * kRedTile;
   double a = 0.0, b = 0.0;
@@ -292,7 +293,7 @@ __global__ void __launch_bounds__(kBlock) k_red_partial(const double2* __restric
     if (i < R) {
       double2 u = x[i];
       if (mode == 0) {
-        a += u.x * u.x + u.y * u.y;
+        if (!mask || mask[i]) a += u.x * u.x + u.y * u.y;
       } else {
         double2 v = y[i];
         a += u.x * v.x + u.y * v.y;
@@ -484,7 +485,20 @@ int paces_norm2(const double* x, int64_t R, double* out, void* ws, size_t ws_byt
   hipStream_t st = (hipStream_t)stream;
   int64_t nb = red_blocks(R);
   hipLaunchKernelGGL(k_red_partial, dim3((unsigned)nb), dim3(kBlock), 0, st, (const double2*)x,
-                     (const double2*)nullptr, R, 0, (double*)ws);
+                     (const double2*)nullptr, R, 0, (double*)ws, (const uint8_t*)nullptr);
+  hipLaunchKernelGGL(k_red_final, dim3(1), dim3(kBlock), 0, st, (const double*)ws, (int)nb, 1,
+                     out);
+  return launch_status();
+}
+
+int paces_norm2_masked(const double* x, const uint8_t* mask, int64_t R, double* out, void* ws,
+                       size_t ws_bytes, void* stream) {
+  if (R < 0 || !out || !ws || (R > 0 && (!x || !mask))) return PACES_E_INVALID;
+  if (ws_bytes < paces_reduce_workspace_bytes(R)) return PACES_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  int64_t nb = red_blocks(R);
+  hipLaunchKernelGGL(k_red_partial, dim3((unsigned)nb), dim3(kBlock), 0, st, (const double2*)x,
+                     (const double2*)nullptr, R, 0, (double*)ws, mask);
   hipLaunchKernelGGL(k_red_final, dim3(1), dim3(kBlock), 0, st, (const double*)ws, (int)nb, 1,
                      out);
   return launch_status();
@@ -497,7 +511,7 @@ int paces_vdot(const double* x, const double* y, int64_t R, double* out, void* w
   hipStream_t st = (hipStream_t)stream;
   int64_t nb = red_blocks(R);
   hipLaunchKernelGGL(k_red_partial, dim3((unsigned)nb), dim3(kBlock), 0, st, (const double2*)x,
-                     (const double2*)y, R, 1, (double*)ws);
+                     (const double2*)y, R, 1, (double*)ws, (const uint8_t*)nullptr);
   hipLaunchKernelGGL(k_red_final, dim3(2), dim3(kBlock), 0, st, (const double*)ws, (int)nb, 2,
                      out);
   return launch_status();

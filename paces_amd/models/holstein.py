@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import inspect
+import math
 
 import numpy
 import torch
@@ -251,6 +252,22 @@ class Hamiltonian(HamiltonianFramework):
                                        ptr(out) if R else None, stream()), "holstein_diag")
         return out
 
+    def features(self, rows, psi, diag, nbasis, offs, cand_pos, src):
+        """Scorer inputs of the union rows (paces_holstein_features): (R, 64) bf16 bits in int16.
+        nbasis, offs, cand_pos, src: the expanded basis size, image run offsets (host), the
+        union's candidate positions and the images' source rows, as for the pairs CSR."""
+        R = rows.shape[0]
+        X = torch.empty((max(R, 1), 64), dtype=torch.int16, device=self.device)
+        run_off = (C.c_int64 * len(offs))(*offs)
+        wsp, wsn = self._ws.get(_lib.paces_holstein_features_workspace_bytes(R))
+        check(_lib.paces_holstein_features(ptr(rows) if R else None, R,
+                                           ptr(torch.view_as_real(psi)) if R else None,
+                                           ptr(diag) if R else None, nbasis, run_off,
+                                           ptr(cand_pos), ptr(src), C.byref(self._layout),
+                                           C.byref(self._hs), ptr(X), wsp, wsn, stream()),
+              "holstein_features")
+        return X[:R]
+
     # ---- plugin methods with the reference's contract (holstein.py:151-277) ----------------------
     def _term_images(self, basis, name):
         h = self._make_holstein_struct({name: self.offdiag_terms[name]})
@@ -370,6 +387,223 @@ class TimeEvolution(TimeEvolutionFramework):
 # fields are padded to the largest lattice; the per-trajectory lattice length and term values
 # live in device tables.
 # ------------------------------------------------------------------------------------------------
+
+class LabelLog:
+    """Retained/discarded labels of candidate rows, logged from steps that keep every row.
+
+    At step i the union rows outside the selection are staged with their scorer features; at
+    step i+1 each staged row is labelled 1 when the new selection kept it, else 0. Labels and
+    features accumulate on the device up to ``capacity`` rows."""
+
+    def __init__(self, device, capacity):
+        self.device = device
+        self.capacity = int(capacity)
+        self.X = torch.empty((self.capacity, 64), dtype=torch.int16, device=device)
+        self.y = torch.empty(self.capacity, dtype=torch.uint8, device=device)
+        self.n = 0
+        self._pending = None
+
+    @property
+    def full(self):
+        return self.n >= self.capacity
+
+    def stage(self, union_rows, X, select):
+        if self.full:
+            self._pending = None
+            return
+        _, hit, _ = engine.lower_bound(select, union_rows, want_hit=True)
+        cand = torch.nonzero(hit == 0).flatten()
+        self._pending = (union_rows.index_select(0, cand), X.index_select(0, cand))
+
+    def resolve(self, select):
+        if self._pending is None:
+            return
+        rows, X = self._pending
+        self._pending = None
+        if rows.shape[0] == 0:
+            return
+        _, hit, _ = engine.lower_bound(select, rows, want_hit=True, q_sorted=True)
+        m = min(rows.shape[0], self.capacity - self.n)
+        self.X[self.n:self.n + m].copy_(X[:m])
+        self.y[self.n:self.n + m].copy_(hit[:m])
+        self.n += m
+
+    def data(self):
+        return self.X[:self.n], self.y[:self.n]
+
+
+class LearnedTimeEvolution(TimeEvolution):
+    """The adaptive-basis step with the candidate scorer (no counterpart in the reference).
+
+    Per step:
+    * select, expand, and build the images and their union, as the plain step does;
+    * carry the amplitudes over onto the union and score every union row outside the selection;
+    * drop rows whose predicted retain probability is below ``threshold`` before the CSR build;
+    * keep a hashed sample of 2^-sample_bits of the dropped rows in the basis, so the weight
+      that all dropped rows would have received is measured: the sample's weight after
+      propagation times (dropped / sampled);
+    * when that estimate exceeds ``fallback_tol``, redo the step with the full union, which is
+      the plain step.
+
+    With ``scorer=None`` every row is kept (the plain step through the same code). A
+    ``label_log`` then records training labels. ``stats`` holds one dict per step: full and kept
+    rows, sampled rows, estimated dropped weight and whether the step fell back.
+    """
+
+    def __init__(self, *args, scorer=None, threshold=0.01, sample_bits=6, fallback_tol=1e-6,
+                 label_log=None, **kwargs):
+        super().__init__(*args, **kwargs)
+        self.scorer = scorer
+        self.threshold = float(threshold)
+        self.sample_bits = int(sample_bits)
+        self.fallback_tol = float(fallback_tol)
+        self.label_log = label_log
+        self.stats = []
+        self._full = None
+        self._sample = None
+        self._next_select = None
+
+    def _determine_select_whoami(self, vector, use_weight_function=False, tau=0.0,
+                                 garbage_tol=-1):
+        # the selection computed right after the last propagation (same rows and amplitudes)
+        if self._next_select is not None:
+            sel, self._next_select = self._next_select, None
+            return sel
+        return super()._determine_select_whoami(vector, use_weight_function, tau, garbage_tol)
+
+    def _install(self, rows, in_basis, cand_pos, psi, diag, basis, offs, src, nup):
+        ham = self.hamobj
+        mats = ham._pairs_csr(rows.shape[0], basis.shape[0], offs, cand_pos, in_basis, src, nup,
+                              basis)
+        self.numstates = rows.shape[0]
+        self.create_matrices(diag, mats)
+        self.vector = psi
+        self.whoami = rows
+
+    def _generate_new_hilbert_space(self, enlarge_steps):
+        ham = self.hamobj
+        W = ham.totwordwidth
+        select = self._determine_select_whoami(self.vector)
+        self.last_select = select
+        if self.label_log is not None:
+            self.label_log.resolve(select)
+        basis = select
+        for _ in range(enlarge_steps):
+            basis = ham.enlarge_basis_set(basis)
+        imgs, offs, src, nup = ham._images(basis, want_src=True)
+        runs = [(basis, 0, basis.shape[0])]
+        runs += [(imgs, offs[k], offs[k + 1] - offs[k]) for k in range(len(offs) - 1)]
+        U, in_basis, cand_pos = engine.merge_unique(runs, W, self._ws, want_from0=True,
+                                                    want_cand_pos=True)
+        ham._check_basis_size(U.shape[0])
+        psiU = engine.reassign(self.whoami, self.vector, U, self._ws)
+        diagU = ham.generate_mel_diag(U)
+        R = U.shape[0]
+        X = None
+        if self.scorer is not None or (self.label_log is not None and not self.label_log.full):
+            X = ham.features(U, psiU, diagU, basis.shape[0], offs, cand_pos, src)
+        if self.label_log is not None and X is not None:
+            self.label_log.stage(U, X, select)
+        st = {"rows_full": R, "rows_kept": R, "sampled": 0, "dropped_weight": 0.0,
+              "dropped_retained_weight": 0.0, "fallback": False}
+        self._sample = None
+        self._full = None
+        if self.scorer is None:
+            self._install(U, in_basis, cand_pos, psiU, diagU, basis, offs, src, nup)
+        else:
+            prob = self.scorer.infer(X)
+            forced, _, _ = engine.lower_bound(U, select, want_hit=False, q_sorted=True)
+            keep = torch.empty(max(R, 1), dtype=torch.uint8, device=self.device)
+            # sample 2^-bits of the dropped rows, with bits lowered on small unions so the sample
+            # holds a few hundred rows
+            bits = max(0, min(self.sample_bits, int(math.log2(max(R, 1) / 4096.0))
+                              if R >= 4096 else 0))
+            check(_lib.paces_prune_plan(ptr(prob), R, self.threshold, ptr(forced),
+                                        select.shape[0], bits,
+                                        (self._step_index + 1) * 0x9E3779B97F4A7C15 % (1 << 64),
+                                        ptr(keep), stream()), "prune_plan")
+            nC = cand_pos.shape[0]
+            rows_k = torch.empty((max(R, 1), W), dtype=U.dtype, device=self.device)
+            psi_k = torch.empty(max(R, 1), dtype=torch.complex128, device=self.device)
+            inb_k = torch.empty(max(R, 1), dtype=torch.uint8, device=self.device)
+            smp_k = torch.empty(max(R, 1), dtype=torch.uint8, device=self.device)
+            pos_k = torch.empty(max(nC, 1), dtype=torch.int32, device=self.device)
+            cnt = torch.empty(1, dtype=torch.int32, device=self.device)
+            wsp, wsn = self._ws.get(_lib.paces_compact_workspace_bytes(R))
+            check(_lib.paces_compact_kept(ptr(keep), R, ptr(U), W,
+                                          ptr(torch.view_as_real(psiU)), ptr(in_basis),
+                                          ptr(cand_pos), nC, ptr(rows_k),
+                                          ptr(torch.view_as_real(psi_k)), ptr(inb_k), ptr(smp_k),
+                                          ptr(pos_k), ptr(cnt), wsp, wsn, stream()),
+                  "compact_kept")
+            Rk = int(cnt.item())
+            rows_k, psi_k, inb_k, smp_k = rows_k[:Rk], psi_k[:Rk], inb_k[:Rk], smp_k[:Rk]
+            nsample = int(smp_k.sum().item()) if Rk else 0
+            st.update(rows_kept=Rk, sampled=nsample)
+            self._sample = (smp_k, nsample, R - Rk + nsample)
+            self._full = (U, in_basis, cand_pos, psiU, diagU, basis, offs, src, nup)
+            self._install(rows_k, inb_k, pos_k, psi_k, ham.generate_mel_diag(rows_k), basis,
+                          offs, src, nup)
+        self.stats.append(st)
+        dbg = {}
+        if "vib_coupling" in ham.offdiag_terms:
+            dbg["vib_coupling"] = [basis.shape[0] - ham._vib_plus_count(offs)]
+        return dbg, self._post_assignment_diagnostics()
+
+    def _check_dropped(self, vec, expm_dbg, delta_t):
+        """Estimate, from the sampled dropped rows, the weight of dropped rows that the next
+        selection would keep; fall back to the full union when it exceeds fallback_tol."""
+        smp, nsample, ndropped = self._sample
+        st = self.stats[-1]
+        if ndropped == 0:
+            return vec, expm_dbg
+        R = self.whoami.shape[0]
+        ms = self.te_params.maxstates
+        sel, sidx = engine.select_topk(vec, self.whoami, ms, self._ws, want_index=True)
+        if nsample:
+            scale = ndropped / nsample
+            mark = torch.zeros(R, dtype=torch.uint8, device=self.device)
+            check(_lib.paces_mark_indices(ptr(sidx), sidx.shape[0], ptr(smp), ptr(mark),
+                                          stream()), "mark_indices")
+            st["dropped_weight"] = self._reducer.norm2_masked(vec, smp) * scale
+            est = self._reducer.norm2_masked(vec, mark) * scale
+        else:
+            est = float("inf")  # rows were dropped but none sampled: no estimate
+        st["dropped_retained_weight"] = est
+        if est > self.fallback_tol:
+            st["fallback"] = True
+            self._install(*self._full)
+            return self._propagate(delta_t)
+        if not self.expm_params.renormalize:
+            self._next_select = sel
+        return vec, expm_dbg
+
+    def _propagate(self, delta_t):
+        if self.expm_params.lanczos_order > 0:
+            return self._lanczos(delta_t)
+        return self._simple_taylor(delta_t)
+
+    def _evolve_single_step(self, i, t_curr, delta_t, diag_coo_debug=None):
+        dbg_dict, post = self._generate_new_hilbert_space(self.te_params.enlarge_steps)
+        vec, expm_dbg = self._propagate(delta_t)
+        if self._sample is not None:
+            vec, expm_dbg = self._check_dropped(vec, expm_dbg, delta_t)
+        self._full = None
+        self._sample = None
+        self.vector = vec
+        norm = self._reducer.norm(self.vector)
+        if self.expm_params.renormalize:
+            engine.axpby(1.0 / norm, self.vector, 0.0, self.vector)
+            norm = self._reducer.norm(self.vector)
+        elif norm > self.expm_params.explosion_cutoff:
+            raise RuntimeError("Norm has exceeded preset explosion cutoff value"
+                               f" ({self.expm_params.explosion_cutoff})!")
+        self.last_expm_dbg = expm_dbg
+        self._postprocess(t_curr, norm, i)
+        self._save_diagnostic_data(t_curr, norm, post, expm_dbg, dbg_dict)
+        self._step_index = i
+        return None
+
 
 class BatchedHamiltonian(Hamiltonian):
     """Holstein Hamiltonians of ``len(nchains)`` independent trajectories in one packed basis.
