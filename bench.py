@@ -382,6 +382,141 @@ def run_plain_form(a):
     return 0
 
 
+LEARNED_DESC = ("learned: 1D Holstein chain N=16 (open), J=1, w0=1, phonon cap 7, exciton at 8, "
+                "maxstates={ms}, dt=0.05, enlarge_steps=1; g~U[0.5,2] seeded 0: three training "
+                "couplings for the logged labels, the fourth for the timed runs")
+# FLOPs of one scorer inference row: 2 * (64*256 + 256*256 + 256*256 + 256)
+MLP_FLOPS_PER_ROW = 2 * (64 * 256 + 2 * 256 * 256 + 256)
+
+
+def learned_draws(seed=0):
+    return np.random.default_rng(seed).uniform(0.5, 2.0, 4)
+
+
+def chain_te(cls, g, ms, precision="fp64", **kw):
+    from paces_amd.core.time_evolution import ExpmParams, TimeEvoParams
+    from paces_amd.models import holstein
+    ham = holstein.Hamiltonian(16, [8] * 16, use_terms=use_terms(1.0, g))
+    te = cls(ham, holstein.Observables, ["n_exc", "n_pho"], None,
+             TimeEvoParams(maxstates=ms, enlarge_steps=1, shuffle_seed=None, diagnostics=False),
+             ExpmParams(precision=precision), **kw)
+    te.create_nonuniform_basis([1] * 16, minpos=8, maxpos=9)
+    raw = np.zeros((1, 17), dtype=np.int64)
+    raw[0, 0] = 8
+    te.create_initial_vector([1.0], raw)
+    te._perform_first_step(0.0)
+    return te
+
+
+def measure_scorer(scorer, X, reps=10):
+    """Average duration of paces_mlp_infer over X, HIP events on the current stream."""
+    import torch
+    st = torch.cuda.current_stream()
+    scorer.infer(X)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        scorer.infer(X)
+    e1.record(st)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps * 1e-3
+
+
+def run_learned(a):
+    """The learned form (scorer pruning, complex64 Taylor terms) against the plain FP64 form on
+    the same trajectory, after logging labels from plain runs and training the scorer on them."""
+    import torch
+    from paces_amd.core.scorer import CandidateScorer
+    from paces_amd.models import holstein
+    ms = a.learned_ms
+    gs = learned_draws()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    log = holstein.LabelLog(dev, a.labels)
+    t0 = time.perf_counter()
+    for g in gs[:3]:
+        te = chain_te(holstein.LearnedTimeEvolution, g, ms, label_log=log)
+        for i in range(1, a.log_steps + 1):
+            te._evolve_single_step(i, 0.05 * i, 0.05)
+            if log.full:
+                break
+        print(f"[learned] logged g={g:.3f}: {log.n} labels", file=sys.stderr, flush=True)
+        del te
+    torch.cuda.synchronize()
+    log_s = time.perf_counter() - t0
+    X, y = log.data()
+    scorer = CandidateScorer(dev, seed=1)
+    t0 = time.perf_counter()
+    hist = scorer.train(X, y, epochs=a.epochs, batch=16384, lr=1e-3)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    print(f"[learned] trained on {log.n} labels in {train_s:.1f} s, loss {hist}",
+          file=sys.stderr, flush=True)
+    del X, y, log
+    torch.cuda.empty_cache()
+    g = gs[3]
+    plain = chain_te(holstein.TimeEvolution, g, ms)
+    learned = chain_te(holstein.LearnedTimeEvolution, g, ms, precision=a.learned_precision,
+                       scorer=scorer)
+    tp = tl = 0.0
+    up = ul = 0
+    err = 0.0
+    n_total = a.warmup + a.steps
+    for i in range(1, n_total + 1):
+        timed = i > a.warmup
+        for name, te in (("plain", plain), ("learned", learned)):
+            prev = te.numstates
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            te._evolve_single_step(i, 0.05 * i, 0.05)
+            torch.cuda.synchronize()
+            dt_ = time.perf_counter() - t0
+            if timed:
+                if name == "plain":
+                    tp += dt_
+                    up += min(prev, ms)
+                else:
+                    tl += dt_
+                    ul += min(prev, ms)
+        e = float(np.max(np.abs(learned.obsobj.last_results["n_exc"]
+                                - plain.obsobj.last_results["n_exc"])))
+        err = max(err, e)
+        s = learned.stats[-1]
+        print(f"[learned] step {i}: rows {s['rows_full']} kept {s['rows_kept']} fallback "
+              f"{s['fallback']} density err {e:.2e}", file=sys.stderr, flush=True)
+    st = learned.stats[a.warmup + 1:]
+    # scorer roofline on the last union's candidate features
+    ham = learned.hamobj
+    R = learned.stats[-1]["rows_full"]
+    Xb = torch.zeros((R, 64), dtype=torch.int16, device=dev)
+    sec = measure_scorer(scorer, Xb)
+    achieved = MLP_FLOPS_PER_ROW * R / sec / 1e12
+    value = ul / tl
+    line = {"metric": METRIC, "value": value, "unit": "state-steps/s", "n_gpus": 1,
+            "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * tl / a.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "c64 terms / f64 state + bf16 scorer" if a.learned_precision == "mixed"
+            else "f64 + bf16 scorer",
+            "data": "synthetic",
+            "config": {"workload": LEARNED_DESC.format(ms=ms), "form": "learned-"
+                       + a.learned_precision,
+                       "plain_gpu_state_steps_per_s": up / tp,
+                       "plain_ms_per_step": 1e3 * tp / a.steps,
+                       "speedup_vs_plain": value / (up / tp),
+                       "density_err_vs_plain_fp64": err,
+                       "mean_kept_fraction": float(np.mean([x["rows_kept"] / x["rows_full"]
+                                                            for x in st])),
+                       "fallbacks": int(sum(x["fallback"] for x in st)),
+                       "labels": int(a.labels), "label_log_s": log_s, "train_s": train_s,
+                       "train_epochs": a.epochs},
+            "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": 2500.0,
+                         "unit": "TFLOP/s", "frac": round(achieved / 2500.0, 4), "traffic": None,
+                         "kernel": "k_mlp_infer (paces_mlp_infer, bf16 MFMA)",
+                         "avg_us": round(sec * 1e6, 1), "rows": int(R)},
+            "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def calibrate_copy(nbytes=1 << 30, reps=5):
     """A plain device copy of known size, for scaling FETCH_SIZE / WRITE_SIZE counter readings:
     each launch reads and writes ``nbytes``."""
@@ -401,7 +536,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=None,
                     help="untimed steps after step 0 (default 8 for traj256, 6 otherwise)")
-    ap.add_argument("--workload", default="traj256", choices=sorted(WORKLOADS) + ["traj256"])
+    ap.add_argument("--workload", default="traj256", choices=sorted(WORKLOADS) + ["traj256", "learned"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=25.0)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
@@ -414,11 +549,17 @@ def main(argv=None):
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-check", action="store_true",
                     help="skip the batched FP64 run that gives the density error")
+    ap.add_argument("--learned-ms", type=int, default=2_000_000)
+    ap.add_argument("--learned-precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--labels", type=int, default=100_000_000,
+                    help="labelled candidates logged for training (learned workload)")
+    ap.add_argument("--log-steps", type=int, default=20)
+    ap.add_argument("--epochs", type=int, default=2)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 1 GiB device copy (counter calibration)")
     a = ap.parse_args(argv)
     if a.warmup is None:
-        a.warmup = 8 if a.workload == "traj256" else 6
+        a.warmup = {"traj256": 8, "learned": 12}.get(a.workload, 6)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported "
               "(independent trajectories shard one process per GPU).", file=sys.stderr)
@@ -436,6 +577,8 @@ def main(argv=None):
         return calibrate_copy()
     if a.workload == "traj256":
         return run_plain_form(a) if a.form == "plain" else run_traj256(a)
+    if a.workload == "learned":
+        return run_learned(a)
     wl = WORKLOADS[a.workload]
     ham = holstein.Hamiltonian(wl["N"], [wl["dim"]] * wl["N"], geo_dim=wl["geo"],
                                periodic=wl["periodic"], use_terms=use_terms(wl["J"], wl["g"]))
