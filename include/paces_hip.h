@@ -128,6 +128,21 @@ int paces_merge_unique(const paces_runs_t* runs, int32_t W, uint64_t* out_rows,
                        int32_t* out_count, uint8_t* out_from_run0, int32_t* cand_pos, void* ws,
                        size_t ws_bytes, void* stream);
 
+/* ---- A7 general sort + dedup (cupy_unique, paces/aux/helpers.py:63-75, as called by
+ * enlarge_basis_set, paces/core/hamiltonian.py:550-577), for rows in no particular order:
+ * LSD radix sort on 4-bit digits with a stable scatter per pass. paces_rows_varying_bits
+ * writes (device, W words) the bits that differ between rows; handing a HOST copy of it to
+ * paces_sort_rows skips every digit at which all rows agree (null: all 16 W digits). out_rows
+ * (capacity n, not aliasing rows) receives the sorted rows, out_perm (optional) the source index
+ * of each; with unique != 0 only the first of equal rows is kept and *out_count (device)
+ * receives the number of rows written. */
+int paces_rows_varying_bits(const uint64_t* rows, int64_t n, int32_t W, uint64_t* mask,
+                            void* stream);
+size_t paces_sort_rows_workspace_bytes(int64_t n, int32_t W);
+int paces_sort_rows(const uint64_t* rows, int64_t n, int32_t W, const uint64_t* varying,
+                    int32_t unique, uint64_t* out_rows, int32_t* out_perm, int32_t* out_count,
+                    void* ws, size_t ws_bytes, void* stream);
+
 /* ---- F1: top-k select (_determine_select_whoami, paces/core/time_evolution.py:833-910,
  * shuffle_seed=None). Keeps the k rows of largest |psi|^2 = re*re + im*im; ties at the cut keep
  * the highest indices (stable ascending argsort, last k). Writes min(k, R) rows in index order,
@@ -245,6 +260,19 @@ int paces_taylor_term(int64_t R, const double* diag, const paces_csr_set_t* H, c
                       double* x_out, double* res, double c, int32_t j, double tol,
                       paces_taylor_state_t* state, void* ws, size_t ws_bytes, void* stream);
 
+/* ---- N4 launch shapes (no reference counterpart): the Taylor-term entry points below take the
+ * workgroup size (128, 256, 512 or 1024) and rows per thread (1, 2 or 4) the autotuner picked;
+ * a null cfg is 256 x 1, the shape of the entry points without the suffix. Every shape gives
+ * bit-identical results. */
+typedef struct {
+  int32_t block;
+  int32_t rows_per_thread;
+} paces_launch_cfg_t;
+int paces_taylor_term_tuned(int64_t R, const double* diag, const paces_csr_set_t* H,
+                            const double* x_in, double* x_out, double* res, double c, int32_t j,
+                            double tol, paces_taylor_state_t* state, void* ws, size_t ws_bytes,
+                            const paces_launch_cfg_t* cfg, void* stream);
+
 /* ---- N2 mixed precision: complex64 series terms with FP32 matrix values and FP32 accumulation,
  * summed into the complex128 result in FP64 (no reference counterpart; the FP64 path above is the
  * reference's arithmetic). Same state and stopping test; tol is the caller's (2^-24 suits FP32
@@ -265,6 +293,10 @@ int paces_taylor_term_mixed(int64_t R, const float* diag, const paces_csr32_set_
                             const float* x_in, float* x_out, double* res, float c, int32_t j,
                             double tol, paces_taylor_state_t* state, void* ws, size_t ws_bytes,
                             void* stream);
+int paces_taylor_term_mixed_tuned(int64_t R, const float* diag, const paces_csr32_set_t* H,
+                                  const float* x_in, float* x_out, double* res, float c,
+                                  int32_t j, double tol, paces_taylor_state_t* state, void* ws,
+                                  size_t ws_bytes, const paces_launch_cfg_t* cfg, void* stream);
 
 /* ---- N1 batched trajectories (no reference counterpart). Field 0 of every row is the
  * trajectory id (ceil(log2 ntraj) bits), so trajectory t owns the contiguous segment
@@ -301,6 +333,12 @@ size_t paces_batch_taylor_workspace_bytes(int64_t R, int32_t ntraj);
 int paces_batch_taylor_begin(int64_t R, const double* v, double* res, void* x0, int32_t mixed,
                              const int32_t* seg_off, int32_t ntraj, paces_batch_taylor_t* ts,
                              void* ws, size_t ws_bytes, void* stream);
+/* N4: the same with a launch shape (see paces_launch_cfg_t); begin and every term of one series
+ * must use the same shape */
+int paces_batch_taylor_begin_tuned(int64_t R, const double* v, double* res, void* x0,
+                                   int32_t mixed, const int32_t* seg_off, int32_t ntraj,
+                                   paces_batch_taylor_t* ts, void* ws, size_t ws_bytes,
+                                   const paces_launch_cfg_t* cfg, void* stream);
 /* one term for the trajectories still running; exactly one of H64 (FP64 values, complex128
  * terms, double diag) and H32 (FP32 values, complex64 terms, float diag) is non-null */
 int paces_batch_taylor_term(int64_t R, const void* diag, const paces_csr_set_t* H64,
@@ -309,6 +347,12 @@ int paces_batch_taylor_term(int64_t R, const void* diag, const paces_csr_set_t* 
                             int32_t ntraj, paces_batch_taylor_t* ts,
                             paces_batch_taylor_hdr_t* hdr, void* ws, size_t ws_bytes,
                             void* stream);
+int paces_batch_taylor_term_tuned(int64_t R, const void* diag, const paces_csr_set_t* H64,
+                                  const paces_csr32_set_t* H32, const void* x_in, void* x_out,
+                                  double* res, double c, int32_t j, double tol,
+                                  const int32_t* seg_off, int32_t ntraj, paces_batch_taylor_t* ts,
+                                  paces_batch_taylor_hdr_t* hdr, void* ws, size_t ws_bytes,
+                                  const paces_launch_cfg_t* cfg, void* stream);
 
 /* ---- N3 candidate scorer (no reference counterpart): 64 inputs -> 3 x 256 GELU -> 1 logit,
  * BF16 matrix operands with FP32 accumulation on the matrix cores. params is the FP32 master
@@ -337,6 +381,10 @@ int paces_mlp_adam(float* params, const float* grads, float* m, float* v, int32_
 /* prob[i] = sigmoid(z_i) and/or logit[i] = z_i for n rows of X (all layers fused). */
 int paces_mlp_infer(const uint16_t* X, int64_t n, const float* params, const uint16_t* wbf16,
                     float* prob, float* logit, void* stream);
+/* the same with tile_rows (64 or 128; 0 = default) rows per workgroup (N4 tuning knob) */
+int paces_mlp_infer_tiled(const uint16_t* X, int64_t n, const float* params,
+                          const uint16_t* wbf16, float* prob, float* logit, int32_t tile_rows,
+                          void* stream);
 
 /* ---- N3 candidate features and pruning (no reference counterpart).
  * paces_holstein_features writes one row of PACES_MLP_IN bf16 features per union row (R rows,

@@ -80,6 +80,10 @@ class BatchTaylor(C.Structure):
                 ("done", C.c_int32), ("j_ind", C.c_int32)]
 
 
+class LaunchCfg(C.Structure):
+    _fields_ = [("block", C.c_int32), ("rows_per_thread", C.c_int32)]
+
+
 class TaylorState(C.Structure):
     _fields_ = [("max_term2", C.c_uint64), ("max_res2", C.c_uint64), ("c1", C.c_double),
                 ("c1_plus_c2", C.c_double), ("ratio", C.c_double), ("done", C.c_int32),
@@ -152,11 +156,26 @@ _SIGNATURES = {
     "paces_norm2": (C.c_int, [_P, _I64, _P, _P, _SZ, _P]),
     "paces_vdot": (C.c_int, [_P, _P, _I64, _P, _P, _SZ, _P]),
     "paces_axpby": (C.c_int, [_I64, _D, _D, _P, _D, _D, _P, _P]),
+    "paces_taylor_term_tuned": (C.c_int, [_I64, _P, C.POINTER(CsrSet), _P, _P, _P, _D, _I32,
+                                          _D, _P, _P, _SZ, C.POINTER(LaunchCfg), _P]),
+    "paces_taylor_term_mixed_tuned": (C.c_int, [_I64, _P, C.POINTER(CsrSet32), _P, _P, _P, _F,
+                                                _I32, _D, _P, _P, _SZ, C.POINTER(LaunchCfg),
+                                                _P]),
+    "paces_batch_taylor_begin_tuned": (C.c_int, [_I64, _P, _P, _P, _I32, _P, _I32, _P, _P, _SZ,
+                                                 C.POINTER(LaunchCfg), _P]),
+    "paces_batch_taylor_term_tuned": (C.c_int, [_I64, _P, C.POINTER(CsrSet),
+                                                C.POINTER(CsrSet32), _P, _P, _P, _D, _I32, _D,
+                                                _P, _I32, _P, _P, _P, _SZ, C.POINTER(LaunchCfg),
+                                                _P]),
+    "paces_rows_varying_bits": (C.c_int, [_P, _I64, _I32, _P, _P]),
+    "paces_sort_rows_workspace_bytes": (_SZ, [_I64, _I32]),
+    "paces_sort_rows": (C.c_int, [_P, _I64, _I32, _P, _I32, _P, _P, _P, _P, _SZ, _P]),
     "paces_mlp_weights_bf16": (C.c_int, [_P, _P, _P]),
     "paces_mlp_train_workspace_bytes": (_SZ, [_I64]),
     "paces_mlp_grad": (C.c_int, [_P, _P, _P, _I64, _F, _P, _P, _P, _P, _P, _SZ, _P]),
     "paces_mlp_adam": (C.c_int, [_P, _P, _P, _P, _I32, _F, _F, _F, _F, _P, _P]),
     "paces_mlp_infer": (C.c_int, [_P, _I64, _P, _P, _P, _P, _P]),
+    "paces_mlp_infer_tiled": (C.c_int, [_P, _I64, _P, _P, _P, _P, _I32, _P]),
     "paces_holstein_features_workspace_bytes": (_SZ, [_I64]),
     "paces_holstein_features": (C.c_int, [_P, _I64, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P,
                                           _SZ, _P]),

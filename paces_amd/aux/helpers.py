@@ -41,3 +41,12 @@ def debug_lister(dbg_list):
 def flatten_dbg_dict(d):
     """Flatten a dict of debug lists into one list (helpers.py:58-60)."""
     return [x for k in d for x in d[k]]
+
+
+def unique_rows(rows, ws=None):
+    """Sorted unique rows of an (n, W) device tensor (cupy_unique, paces/aux/helpers.py:63-75),
+    by the library's LSD radix sort."""
+    from .. import _hip
+    from ..core import engine
+    return engine.sort_rows(rows.contiguous(), ws if ws is not None else _hip.Workspace(rows.device),
+                            unique=True)

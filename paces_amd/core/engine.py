@@ -101,6 +101,27 @@ def merge_unique(runs, W, ws, want_from0=False, want_cand_pos=False):
     return out[:n], (from0[:n] if want_from0 else None)
 
 
+def sort_rows(rows, ws, unique=True, want_perm=False):
+    """Lexicographically sorted rows (LSD radix sort on the device), duplicates dropped when
+    unique; with want_perm also the source index of each output row. The digits at which all
+    rows agree are found first (one small device-to-host copy) and skipped."""
+    n, W = rows.shape
+    out = torch.empty((max(n, 1), W), dtype=torch.int64, device=rows.device)
+    perm = torch.empty(max(n, 1), dtype=torch.int32, device=rows.device) if want_perm else None
+    cnt = torch.empty(1, dtype=torch.int32, device=rows.device)
+    if n == 0:
+        return (out[:0], perm[:0]) if want_perm else out[:0]
+    mask = torch.empty(W, dtype=torch.int64, device=rows.device)
+    check(lib.paces_rows_varying_bits(ptr(rows), n, W, ptr(mask), stream()), "rows_varying_bits")
+    host = (C.c_uint64 * W)(*[int(x) & 0xFFFFFFFFFFFFFFFF for x in mask.cpu().tolist()])
+    wsp, wsn = ws.get(lib.paces_sort_rows_workspace_bytes(n, W))
+    check(lib.paces_sort_rows(ptr(rows), n, W, host, 1 if unique else 0, ptr(out),
+                              ptr(perm) if want_perm else None, ptr(cnt), wsp, wsn, stream()),
+          "sort_rows")
+    m = int(cnt.item()) if unique else n
+    return (out[:m], perm[:m]) if want_perm else out[:m]
+
+
 def select_topk(psi, rows, k, ws, want_index=False):
     """Rows of the k largest |psi| (ties keep the highest indices), in index order; with
     want_index also their indices (int32)."""
@@ -193,9 +214,22 @@ def taylor_coeff(delta_t, j):
 TAYLOR_TOL = 1.1102230246251565e-16  # 2**-53 (paces/core/time_evolution.py:484)
 
 
-def simple_taylor(diag, mats, v, delta_t, m_star=100, tol=TAYLOR_TOL, chunk=6, ws=None):
+def _tuned_shape(tuner, kind, R, nterms, run):
+    """Launch shape for a Taylor term over R rows: the tuner's choice (timing run(shape) on
+    scratch buffers the first time a size bucket is seen), or the default shape."""
+    if tuner is None or R == 0:
+        return None
+    from . import autotune
+    key = tuner.key(kind, R, nterms)
+    shape = tuner.pick(key, autotune.SHAPES, run)
+    return autotune.launch_cfg(shape)
+
+
+def simple_taylor(diag, mats, v, delta_t, m_star=100, tol=TAYLOR_TOL, chunk=6, ws=None,
+                  tuner=None):
     """res = sum_j (-i dt H)^j v / j! until (|t_j| + |t_{j+1}|)_inf / |res|_inf <= tol
-    (paces/core/time_evolution.py:482-512). Returns (res, [converged, j_ind, c1+c2, ratio])."""
+    (paces/core/time_evolution.py:482-512). Returns (res, [converged, j_ind, c1+c2, ratio]).
+    With a tuner, the term kernel's launch shape is the tuned one (identical results)."""
     R = v.shape[0]
     res = torch.empty_like(v)
     bufs = (torch.empty_like(v), torch.empty_like(v))
@@ -204,6 +238,18 @@ def simple_taylor(diag, mats, v, delta_t, m_star=100, tol=TAYLOR_TOL, chunk=6, w
     st = stream()
     ws = ws if ws is not None else _hip.Workspace(v.device)
     wsp, wsn = ws.get(lib.paces_taylor_workspace_bytes(R))
+
+    def run(shape):
+        # tol < 0 never stops, so every launch does the full term; scratch result and state
+        scratch = torch.empty_like(v)
+        sstate = torch.zeros_like(state)
+        cfg = _hip.LaunchCfg(*shape)
+        return lambda: check(lib.paces_taylor_term_tuned(
+            R, ptr(diag), C.byref(s), ptr(torch.view_as_real(v)), ptr(torch.view_as_real(bufs[0])),
+            ptr(torch.view_as_real(scratch)), 1e-3, 0, -1.0, ptr(sstate), wsp, wsn,
+            C.byref(cfg), st), "taylor_term_tuned")
+
+    cfg = _tuned_shape(tuner, "tay64", R, len(mats), run)
     check(lib.paces_taylor_begin(R, ptr(torch.view_as_real(v)), ptr(torch.view_as_real(res)),
                                  ptr(state), wsp, wsn, st), "taylor_begin")
     x_in = v
@@ -214,10 +260,13 @@ def simple_taylor(diag, mats, v, delta_t, m_star=100, tol=TAYLOR_TOL, chunk=6, w
             if j >= m_star:
                 break
             x_out = bufs[j % 2]
-            check(lib.paces_taylor_term(R, ptr(diag), C.byref(s), ptr(torch.view_as_real(x_in)),
-                                        ptr(torch.view_as_real(x_out)),
-                                        ptr(torch.view_as_real(res)), taylor_coeff(delta_t, j), j,
-                                        tol, ptr(state), wsp, wsn, st), "taylor_term")
+            check(lib.paces_taylor_term_tuned(R, ptr(diag), C.byref(s),
+                                              ptr(torch.view_as_real(x_in)),
+                                              ptr(torch.view_as_real(x_out)),
+                                              ptr(torch.view_as_real(res)),
+                                              taylor_coeff(delta_t, j), j, tol, ptr(state), wsp,
+                                              wsn, C.byref(cfg) if cfg else None, st),
+                  "taylor_term")
             x_in = x_out
             j += 1
         host = _hip.TaylorState.from_buffer_copy(bytes(state.cpu().numpy()))
@@ -253,7 +302,7 @@ class MixedMats:
 
 
 def simple_taylor_mixed(mm: MixedMats, v, delta_t, m_star=100, tol=TAYLOR_TOL_MIXED, chunk=4,
-                        ws=None):
+                        ws=None, tuner=None):
     """The Taylor series with complex64 terms (FP32 matrix and accumulation) summed into a
     complex128 result. Returns (res, [converged, j_ind, c1+c2, ratio])."""
     R = v.shape[0]
@@ -268,6 +317,17 @@ def simple_taylor_mixed(mm: MixedMats, v, delta_t, m_star=100, tol=TAYLOR_TOL_MI
     check(lib.paces_taylor_begin_mixed(R, ptr(torch.view_as_real(v)), ptr(torch.view_as_real(res)),
                                        ptr(torch.view_as_real(bufs[2])), ptr(state), wsp, wsn, st),
           "taylor_begin_mixed")
+
+    def run(shape):
+        scratch = torch.empty_like(v)
+        sstate = torch.zeros_like(state)
+        cfg = _hip.LaunchCfg(*shape)
+        return lambda: check(lib.paces_taylor_term_mixed_tuned(
+            R, ptr(mm.diag), C.byref(mm.cset), ptr(torch.view_as_real(bufs[2])),
+            ptr(torch.view_as_real(bufs[0])), ptr(torch.view_as_real(scratch)), 1e-3, 0, -1.0,
+            ptr(sstate), wsp, wsn, C.byref(cfg), st), "taylor_term_mixed_tuned")
+
+    cfg = _tuned_shape(tuner, "tay32", R, mm.cset.nterms, run)
     x_in = bufs[2]
     j = 0
     host = None
@@ -276,12 +336,13 @@ def simple_taylor_mixed(mm: MixedMats, v, delta_t, m_star=100, tol=TAYLOR_TOL_MI
             if j >= m_star:
                 break
             x_out = bufs[j % 2]
-            check(lib.paces_taylor_term_mixed(R, ptr(mm.diag), C.byref(mm.cset),
-                                              ptr(torch.view_as_real(x_in)),
-                                              ptr(torch.view_as_real(x_out)),
-                                              ptr(torch.view_as_real(res)),
-                                              taylor_coeff(delta_t, j), j, tol, ptr(state), wsp,
-                                              wsn, st), "taylor_term_mixed")
+            check(lib.paces_taylor_term_mixed_tuned(R, ptr(mm.diag), C.byref(mm.cset),
+                                                    ptr(torch.view_as_real(x_in)),
+                                                    ptr(torch.view_as_real(x_out)),
+                                                    ptr(torch.view_as_real(res)),
+                                                    taylor_coeff(delta_t, j), j, tol, ptr(state),
+                                                    wsp, wsn, C.byref(cfg) if cfg else None, st),
+                  "taylor_term_mixed")
             x_in = x_out
             j += 1
         host = _hip.TaylorState.from_buffer_copy(bytes(state.cpu().numpy()))

@@ -147,8 +147,16 @@ class HamiltonianFramework:
         return
 
     def enlarge_basis_set(self, basis_states):
-        """Basis plus its first-degree neighbours (hamiltonian.py:550-577)."""
-        raise NotImplementedError("This model does not provide a device expansion.")
+        """Basis plus its first-degree neighbours (hamiltonian.py:550-577): every off-diagonal
+        term's images (generate_mel_<term>(basis, raw_map_to=True)) are concatenated after the
+        basis and sorted with duplicates dropped on the device (paces_sort_rows). Models whose
+        image runs are sorted override this with the union of runs."""
+        parts = [basis_states]
+        for term in self.offdiag_terms:
+            imgs = getattr(self, "generate_mel_" + term)(basis_states, raw_map_to=True)
+            parts.extend(x for x in imgs if x.shape[0])
+        allrows = torch.cat(parts, dim=0) if len(parts) > 1 else basis_states
+        return engine.sort_rows(allrows.contiguous(), self._ws, unique=True)
 
     def generate_mel(self, basis_states, enlarge_steps=0, log_ind=1.2):
         """((diag_vals, new_inds), {term: CsrMatrix}, dbg_dict) for the given basis

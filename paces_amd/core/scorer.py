@@ -65,6 +65,7 @@ class CandidateScorer:
         self.grads = torch.zeros(NPARAM, dtype=torch.float32, device=device)
         self.loss = torch.zeros(1, dtype=torch.float32, device=device)
         self.step = 0
+        self.tile_rows = 0  # 0: the library default; the autotuner may set 64 or 128
         self._ws = _hip.Workspace(device)
         self._lib = _hip.load()
         self.refresh_bf16()
@@ -104,12 +105,16 @@ class CandidateScorer:
                                        float(beta2), float(eps), ptr(self.wbf), stream()),
               "mlp_adam")
 
-    def infer(self, X, want_logit=False):
+    def infer(self, X, want_logit=False, tile_rows=None):
+        """Retain probabilities (or logits) of the rows of X. tile_rows (64 or 128) overrides
+        the workgroup tile; by default the autotuned choice, else the library default."""
         n = int(X.shape[0])
         out = torch.empty(max(n, 1), dtype=torch.float32, device=self.device)
-        check(self._lib.paces_mlp_infer(ptr(X) if n else None, n, ptr(self.params),
-                                        ptr(self.wbf), None if want_logit else ptr(out),
-                                        ptr(out) if want_logit else None, stream()), "mlp_infer")
+        tile = self.tile_rows if tile_rows is None else int(tile_rows)
+        check(self._lib.paces_mlp_infer_tiled(ptr(X) if n else None, n, ptr(self.params),
+                                              ptr(self.wbf), None if want_logit else ptr(out),
+                                              ptr(out) if want_logit else None, tile, stream()),
+              "mlp_infer")
         return out[:n]
 
     def train(self, X, labels, epochs=1, batch=8192, lr=1e-3, pos_weight=1.0, seed=0,

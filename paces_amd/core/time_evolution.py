@@ -52,9 +52,10 @@ class CoeffSaveParams:
 class ExpmParams:
     """Matrix-exponential parameters (time_evolution.py:57-82).
 
-    Two fields extend the reference: ``precision`` ("fp64": the reference's complex128 series;
+    Three fields extend the reference: ``precision`` ("fp64": the reference's complex128 series;
     "mixed": complex64 terms with FP32 matrix values and accumulation, summed into the complex128
-    state) and ``taylor_tol`` (None: 2**-53 for fp64, 2**-24 for mixed).
+    state), ``taylor_tol`` (None: 2**-53 for fp64, 2**-24 for mixed) and ``autotune`` (pick the
+    Taylor-term launch shape by timing the candidates; results are identical).
     """
     m_star: int = 100
     explosion_cutoff: float = 2.0
@@ -63,6 +64,7 @@ class ExpmParams:
     lanczos_order: int = 0
     precision: str = "fp64"
     taylor_tol: Optional[float] = None
+    autotune: bool = False
 
 
 @dataclass
@@ -263,16 +265,23 @@ class TimeEvolutionFramework:
             mm = engine.MixedMats(self.diag_vals, self._mats())
             tol = engine.TAYLOR_TOL_MIXED if ep.taylor_tol is None else ep.taylor_tol
             res, dbg = engine.simple_taylor_mixed(mm, self.vector, delta_t, ep.m_star, tol,
-                                                  ws=self._taylor_ws)
+                                                  ws=self._taylor_ws, tuner=self._tuner())
         elif ep.precision == "fp64":
             tol = engine.TAYLOR_TOL if ep.taylor_tol is None else ep.taylor_tol
             res, dbg = engine.simple_taylor(self.diag_vals, self._mats(), self.vector, delta_t,
-                                            ep.m_star, tol, ws=self._taylor_ws)
+                                            ep.m_star, tol, ws=self._taylor_ws,
+                                            tuner=self._tuner())
         else:
             raise ValueError(f"Unknown precision {ep.precision!r} (use 'fp64' or 'mixed').")
         if self.debug_verb > EXPM_EVO_LEVEL:
             print(f"         Taylor series: converged={dbg[0]} after j={dbg[1]}.")
         return res, dbg
+
+    def _tuner(self):
+        if not getattr(self.expm_params, "autotune", False):
+            return None
+        from . import autotune
+        return autotune.default_tuner()
 
     def _lanczos(self, delta_t):
         """Krylov propagator without reorthogonalisation (time_evolution.py:515-549). The

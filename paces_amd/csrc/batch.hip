@@ -417,8 +417,9 @@ __device__ __forceinline__ typename Vec2<F>::T hx_row_t(int64_t i, const F* __re
   return o;
 }
 
+template <int BS>
 __device__ __forceinline__ void blk_max2(u64 a, u64 b, u64* __restrict__ part) {
-  __shared__ u64 sa[kBlock / 64], sb[kBlock / 64];
+  __shared__ u64 sa[BS / 64], sb[BS / 64];
   a = wave_max_u64(a);
   b = wave_max_u64(b);
   if ((threadIdx.x & 63) == 0) {
@@ -429,7 +430,7 @@ __device__ __forceinline__ void blk_max2(u64 a, u64 b, u64* __restrict__ part) {
   if (threadIdx.x == 0) {
     u64 ma = 0, mb = 0;
 #pragma unroll
-    for (int q = 0; q < kBlock / 64; ++q) {
+    for (int q = 0; q < BS / 64; ++q) {
       ma = sa[q] > ma ? sa[q] : ma;
       mb = sb[q] > mb ? sb[q] : mb;
     }
@@ -438,27 +439,33 @@ __device__ __forceinline__ void blk_max2(u64 a, u64 b, u64* __restrict__ part) {
   }
 }
 
-template <typename F>
-__global__ void __launch_bounds__(kBlock) k_btay_begin(const double2* __restrict__ v,
+// Launch shape (tuned): BS threads per workgroup, RPT rows per thread (t, t + BS, ...), a
+// segment-aligned tile of BS * RPT rows; begin and every term of one series use the same shape.
+template <typename F, int BS, int RPT>
+__global__ void __launch_bounds__(BS) k_btay_begin(const double2* __restrict__ v,
                                                        double2* __restrict__ res,
                                                        typename Vec2<F>::T* __restrict__ x0,
                                                        const int32_t* __restrict__ seg_off,
                                                        const int64_t* __restrict__ boff, int ntraj,
                                                        u64* __restrict__ part) {
-  const SegTile T = seg_tile(seg_off, boff, ntraj, kTile);
+  const SegTile T = seg_tile(seg_off, boff, ntraj, BS * RPT);
   if (T.t < 0) return;
-  const int64_t i = T.r0 + threadIdx.x;
   u64 m = 0;
-  if (i < T.r1) {
-    double2 a = v[i];
-    res[i] = a;
-    typename Vec2<F>::T c;
-    c.x = (F)a.x;
-    c.y = (F)a.y;
-    x0[i] = c;
-    m = (u64)__double_as_longlong(a.x * a.x + a.y * a.y);
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int64_t i = T.r0 + q * BS + threadIdx.x;
+    if (i < T.r1) {
+      double2 a = v[i];
+      res[i] = a;
+      typename Vec2<F>::T c;
+      c.x = (F)a.x;
+      c.y = (F)a.y;
+      x0[i] = c;
+      const u64 w = (u64)__double_as_longlong(a.x * a.x + a.y * a.y);
+      m = w > m ? w : m;
+    }
   }
-  blk_max2(m, 0, part);
+  blk_max2<BS>(m, 0, part);
 }
 
 // per-trajectory reduction of the partials; fin == 1: initialise c1, else the stopping test
@@ -501,37 +508,42 @@ __global__ void __launch_bounds__(64) k_btay_reduce(const u64* __restrict__ part
   }
 }
 
-template <typename F>
-__global__ void __launch_bounds__(kBlock) k_btay_term(
+template <typename F, int BS, int RPT>
+__global__ void __launch_bounds__(BS) k_btay_term(
     const F* __restrict__ diag, CsrSetB64 H, const typename Vec2<F>::T* __restrict__ xin,
     typename Vec2<F>::T* __restrict__ xout, double2* __restrict__ res, F c,
     const int32_t* __restrict__ seg_off, const int64_t* __restrict__ boff, int ntraj,
     const paces_batch_taylor_t* __restrict__ ts, paces_batch_taylor_hdr_t* __restrict__ hdr,
     u64* __restrict__ part) {
   if (blockIdx.x == 0 && threadIdx.x == 0) hdr->notdone = 0;
-  const SegTile T = seg_tile(seg_off, boff, ntraj, kTile);
+  const SegTile T = seg_tile(seg_off, boff, ntraj, BS * RPT);
   if (T.t < 0 || ts[T.t].done) return;
-  const int64_t i = T.r0 + threadIdx.x;
   u64 mt = 0, mr = 0;
-  if (i < T.r1) {
-    const typename Vec2<F>::T h = hx_row_t<F>(i, diag, H, xin);
-    typename Vec2<F>::T t;
-    t.x = h.y * c;
-    t.y = -(h.x * c);
-    xout[i] = t;
-    double2 r = res[i];
-    r.x = r.x + (double)t.x;
-    r.y = r.y + (double)t.y;
-    res[i] = r;
-    const double tx = t.x, ty = t.y;
-    mt = (u64)__double_as_longlong(tx * tx + ty * ty);
-    mr = (u64)__double_as_longlong(r.x * r.x + r.y * r.y);
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int64_t i = T.r0 + q * BS + threadIdx.x;
+    if (i < T.r1) {
+      const typename Vec2<F>::T h = hx_row_t<F>(i, diag, H, xin);
+      typename Vec2<F>::T t;
+      t.x = h.y * c;
+      t.y = -(h.x * c);
+      xout[i] = t;
+      double2 r = res[i];
+      r.x = r.x + (double)t.x;
+      r.y = r.y + (double)t.y;
+      res[i] = r;
+      const double tx = t.x, ty = t.y;
+      const u64 a = (u64)__double_as_longlong(tx * tx + ty * ty);
+      const u64 b = (u64)__double_as_longlong(r.x * r.x + r.y * r.y);
+      mt = a > mt ? a : mt;
+      mr = b > mr ? b : mr;
+    }
   }
-  blk_max2(mt, mr, part);
+  blk_max2<BS>(mt, mr, part);
 }
 
 size_t btay_ws_bytes(int64_t R, int ntraj) {
-  int64_t nblk = (R + kTile - 1) / kTile + ntraj + 1;
+  int64_t nblk = (R + 127) / 128 + ntraj + 1;  // the smallest tile of any launch shape
   return a256((size_t)(ntraj + 1) * 8) + a256((size_t)nblk * 2 * 8);
 }
 
@@ -626,7 +638,15 @@ size_t paces_batch_taylor_workspace_bytes(int64_t R, int32_t ntraj) {
 int paces_batch_taylor_begin(int64_t R, const double* v, double* res, void* x0, int32_t mixed,
                              const int32_t* seg_off, int32_t ntraj, paces_batch_taylor_t* ts,
                              void* ws, size_t ws_bytes, void* stream) {
-  if (R < 0 || ntraj < 1 || !seg_off || !ts || !ws) return PACES_E_INVALID;
+  return paces_batch_taylor_begin_tuned(R, v, res, x0, mixed, seg_off, ntraj, ts, ws, ws_bytes,
+                                        nullptr, stream);
+}
+
+int paces_batch_taylor_begin_tuned(int64_t R, const double* v, double* res, void* x0,
+                                   int32_t mixed, const int32_t* seg_off, int32_t ntraj,
+                                   paces_batch_taylor_t* ts, void* ws, size_t ws_bytes,
+                                   const paces_launch_cfg_t* cfg, void* stream) {
+  if (R < 0 || ntraj < 1 || !seg_off || !ts || !ws || !cfg_ok(cfg)) return PACES_E_INVALID;
   if (R > 0 && (!v || !res || !x0)) return PACES_E_INVALID;
   if (ws_bytes < btay_ws_bytes(R, ntraj)) return PACES_E_WORKSPACE;
   hipStream_t st = (hipStream_t)stream;
@@ -634,14 +654,21 @@ int paces_batch_taylor_begin(int64_t R, const double* v, double* res, void* x0, 
   int64_t* boff = (int64_t*)p;
   p += a256((size_t)(ntraj + 1) * 8);
   u64* part = (u64*)p;
-  hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(1), 0, st, seg_off, ntraj, kTile, boff);
-  const unsigned gs = seg_grid(R, ntraj, kTile);
-  if (mixed)
-    hipLaunchKernelGGL(k_btay_begin<float>, dim3(gs), dim3(kBlock), 0, st, (const double2*)v,
-                       (double2*)res, (float2*)x0, seg_off, (const int64_t*)boff, ntraj, part);
-  else
-    hipLaunchKernelGGL(k_btay_begin<double>, dim3(gs), dim3(kBlock), 0, st, (const double2*)v,
-                       (double2*)res, (double2*)x0, seg_off, (const int64_t*)boff, ntraj, part);
+  int B, RR;
+  cfg_get(cfg, &B, &RR);
+  hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(1), 0, st, seg_off, ntraj, B * RR, boff);
+  const unsigned gs = seg_grid(R, ntraj, B * RR);
+#define BB_LAUNCH(BB, QQ)                                                                      \
+  if (mixed)                                                                                   \
+    hipLaunchKernelGGL((k_btay_begin<float, BB, QQ>), dim3(gs), dim3(BB), 0, st,               \
+                       (const double2*)v, (double2*)res, (float2*)x0, seg_off,                 \
+                       (const int64_t*)boff, ntraj, part);                                     \
+  else                                                                                         \
+    hipLaunchKernelGGL((k_btay_begin<double, BB, QQ>), dim3(gs), dim3(BB), 0, st,              \
+                       (const double2*)v, (double2*)res, (double2*)x0, seg_off,                \
+                       (const int64_t*)boff, ntraj, part)
+  PACES_CFG_SWITCH(B, RR, BB_LAUNCH)
+#undef BB_LAUNCH
   hipLaunchKernelGGL(k_btay_reduce, dim3(ntraj), dim3(64), 0, st, (const u64*)part,
                      (const int64_t*)boff, ts, (paces_batch_taylor_hdr_t*)nullptr, 1, 0, 0.0);
   return launch_status();
@@ -653,9 +680,20 @@ int paces_batch_taylor_term(int64_t R, const void* diag, const paces_csr_set_t* 
                             int32_t ntraj, paces_batch_taylor_t* ts,
                             paces_batch_taylor_hdr_t* hdr, void* ws, size_t ws_bytes,
                             void* stream) {
+  return paces_batch_taylor_term_tuned(R, diag, H64, H32, x_in, x_out, res, c, j, tol, seg_off,
+                                       ntraj, ts, hdr, ws, ws_bytes, nullptr, stream);
+}
+
+int paces_batch_taylor_term_tuned(int64_t R, const void* diag, const paces_csr_set_t* H64,
+                                  const paces_csr32_set_t* H32, const void* x_in, void* x_out,
+                                  double* res, double c, int32_t j, double tol,
+                                  const int32_t* seg_off, int32_t ntraj, paces_batch_taylor_t* ts,
+                                  paces_batch_taylor_hdr_t* hdr, void* ws, size_t ws_bytes,
+                                  const paces_launch_cfg_t* cfg, void* stream) {
   CsrSetB64 H;
   const int mixed = H32 != nullptr;
-  if (R < 0 || ntraj < 1 || !seg_off || !ts || !hdr || !ws || !csr_b(H64, H32, &H))
+  if (R < 0 || ntraj < 1 || !seg_off || !ts || !hdr || !ws || !csr_b(H64, H32, &H) ||
+      !cfg_ok(cfg))
     return PACES_E_INVALID;
   if (R > 0 && (!diag || !x_in || !x_out || !res)) return PACES_E_INVALID;
   if (ws_bytes < btay_ws_bytes(R, ntraj)) return PACES_E_WORKSPACE;
@@ -664,15 +702,22 @@ int paces_batch_taylor_term(int64_t R, const void* diag, const paces_csr_set_t* 
   const int64_t* boff = (const int64_t*)p;
   p += a256((size_t)(ntraj + 1) * 8);
   u64* part = (u64*)p;
-  const unsigned gs = seg_grid(R, ntraj, kTile);
-  if (mixed)
-    hipLaunchKernelGGL(k_btay_term<float>, dim3(gs), dim3(kBlock), 0, st, (const float*)diag, H,
-                       (const float2*)x_in, (float2*)x_out, (double2*)res, (float)c, seg_off,
-                       boff, ntraj, (const paces_batch_taylor_t*)ts, hdr, part);
-  else
-    hipLaunchKernelGGL(k_btay_term<double>, dim3(gs), dim3(kBlock), 0, st, (const double*)diag, H,
-                       (const double2*)x_in, (double2*)x_out, (double2*)res, c, seg_off, boff,
-                       ntraj, (const paces_batch_taylor_t*)ts, hdr, part);
+  int B, RR;
+  cfg_get(cfg, &B, &RR);
+  const unsigned gs = seg_grid(R, ntraj, B * RR);
+#define BT_LAUNCH(BB, QQ)                                                                      \
+  if (mixed)                                                                                   \
+    hipLaunchKernelGGL((k_btay_term<float, BB, QQ>), dim3(gs), dim3(BB), 0, st,                \
+                       (const float*)diag, H, (const float2*)x_in, (float2*)x_out,             \
+                       (double2*)res, (float)c, seg_off, boff, ntraj,                          \
+                       (const paces_batch_taylor_t*)ts, hdr, part);                            \
+  else                                                                                         \
+    hipLaunchKernelGGL((k_btay_term<double, BB, QQ>), dim3(gs), dim3(BB), 0, st,               \
+                       (const double*)diag, H, (const double2*)x_in, (double2*)x_out,          \
+                       (double2*)res, c, seg_off, boff, ntraj,                                 \
+                       (const paces_batch_taylor_t*)ts, hdr, part)
+  PACES_CFG_SWITCH(B, RR, BT_LAUNCH)
+#undef BT_LAUNCH
   hipLaunchKernelGGL(k_btay_reduce, dim3(ntraj), dim3(64), 0, st, (const u64*)part, boff, ts, hdr,
                      0, (int)j, tol);
   return launch_status();

@@ -332,6 +332,34 @@ __device__ __forceinline__ int wave_sum_i(int v) {
   return v;
 }
 
+// ---- launch shapes (N4): workgroup sizes 128..1024 x rows per thread 1, 2, 4 ----------------------
+inline bool cfg_ok(const paces_launch_cfg_t* cfg) {
+  if (!cfg) return true;
+  const int b = cfg->block, r = cfg->rows_per_thread;
+  return (b == 128 || b == 256 || b == 512 || b == 1024) && (r == 1 || r == 2 || r == 4);
+}
+inline void cfg_get(const paces_launch_cfg_t* cfg, int* b, int* r) {
+  *b = cfg ? cfg->block : kBlock;
+  *r = cfg ? cfg->rows_per_thread : 1;
+}
+
+#define PACES_CFG_SWITCH(B, RR, LAUNCH)                            \
+  switch ((B) * 8 + (RR)) {                                       \
+    case 128 * 8 + 1: LAUNCH(128, 1); break;                      \
+    case 128 * 8 + 2: LAUNCH(128, 2); break;                      \
+    case 128 * 8 + 4: LAUNCH(128, 4); break;                      \
+    case 256 * 8 + 1: LAUNCH(256, 1); break;                      \
+    case 256 * 8 + 2: LAUNCH(256, 2); break;                      \
+    case 256 * 8 + 4: LAUNCH(256, 4); break;                      \
+    case 512 * 8 + 1: LAUNCH(512, 1); break;                      \
+    case 512 * 8 + 2: LAUNCH(512, 2); break;                      \
+    case 512 * 8 + 4: LAUNCH(512, 4); break;                      \
+    case 1024 * 8 + 1: LAUNCH(1024, 1); break;                    \
+    case 1024 * 8 + 2: LAUNCH(1024, 2); break;                    \
+    case 1024 * 8 + 4: LAUNCH(1024, 4); break;                    \
+    default: return PACES_E_INVALID;                              \
+  }
+
 }  // namespace paces
 
 // Dispatch a templated launcher over the supported row widths.

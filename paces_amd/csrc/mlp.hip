@@ -14,7 +14,7 @@
 //   k_head_train    output layer, loss, and the gradient into the last hidden layer
 //   k_colsum / k_sum_parts   deterministic bias-gradient and split-K reductions
 //   k_adam          Adam update of the FP32 master and refresh of the BF16 copies
-//   k_mlp_infer     all layers fused for a tile of 64 rows, activations kept in LDS
+//   k_mlp_infer<RB> all layers fused for a tile of 64 RB rows, activations kept in LDS
 
 #include "common.hpp"
 
@@ -34,20 +34,32 @@ typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
+// float -> bf16 bits, round to nearest even (the conversion gfx950 does in hardware)
 __device__ __forceinline__ uint16_t f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+  return __builtin_bit_cast(uint16_t, (__bf16)f);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// erf by Abramowitz & Stegun, Handbook of Mathematical Functions, formula 7.1.26:
+// erf(x) = 1 - (a1 t + a2 t^2 + a3 t^3 + a4 t^4 + a5 t^5) exp(-x^2), t = 1 / (1 + p x), x >= 0,
+// |error| <= 1.5e-7 -- far below the bf16 resolution of the activations it feeds.
+__device__ __forceinline__ float erf_as(float x) {
+  const float ax = fabsf(x);
+  const float t = __frcp_rn(1.0f + 0.3275911f * ax);
+  const float poly =
+      t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f +
+                                                                      t * 1.061405429f))));
+  const float y = 1.0f - poly * __expf(-ax * ax);
+  return copysignf(y, x);
+}
+
+// GELU(a) = a * Phi(a) = 0.5 a (1 + erf(a / sqrt 2)) and its derivative Phi(a) + a phi(a)
 __device__ __forceinline__ float gelu_f(float a) {
-  return 0.5f * a * (1.0f + erff(a * 0.70710678118654752f));
+  return 0.5f * a * (1.0f + erf_as(a * 0.70710678118654752f));
 }
 __device__ __forceinline__ float gelu_grad_f(float a) {
-  const float cdf = 0.5f * (1.0f + erff(a * 0.70710678118654752f));
-  const float pdf = expf(-0.5f * a * a) * 0.39894228040143268f;
+  const float cdf = 0.5f * (1.0f + erf_as(a * 0.70710678118654752f));
+  const float pdf = __expf(-0.5f * a * a) * 0.39894228040143268f;
   return cdf + a * pdf;
 }
 
@@ -335,62 +347,78 @@ __global__ void __launch_bounds__(256) k_adam(float* __restrict__ params,
   if (q >= 0) wbf[q] = f2bf(p);
 }
 
-// ---- fused inference: 64 rows per workgroup, activations in LDS -------------------------------
+// ---- fused inference: RB x 64 rows per workgroup of 4 RB waves, activations in LDS ----------
+// Wave w computes rows [64 (w / 4), +64) x columns [64 (w % 4), +64) of each layer; the two (RB = 2)
+// waves that share a column block read the same weight fragments, so L2 traffic per row halves.
 constexpr int kXs = IN + 8, kHs = HID + 8;  // padded LDS row lengths (16-byte aligned rows)
 
-// one hidden layer for the workgroup's 64 rows: wave w computes output columns [64w, 64w + 64)
 template <int K>
 __device__ __forceinline__ void infer_layer(const uint16_t* src, int ld,
                                             const uint16_t* __restrict__ Wt, f32x4 (&acc)[4][4]) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int rb = w >> 2, cb = w & 3;
   const int lr = lane & 15, lk = 8 * (lane >> 4);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
+  const uint16_t* wrow = Wt + (int64_t)(64 * cb + lr) * K + lk;
+  const uint16_t* arow = src + (64 * rb + lr) * ld + lk;
+  bf16x8 b[4], bn[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) b[j] = glb_frag(wrow + (int64_t)16 * j * K);
+#pragma unroll
   for (int k0 = 0; k0 < K; k0 += 32) {
-    bf16x8 a[4], b[4];
+    if (k0 + 32 < K) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) b[j] = glb_frag(Wt + (int64_t)(64 * w + 16 * j + lr) * K + k0 + lk);
+      for (int j = 0; j < 4; ++j) bn[j] = glb_frag(wrow + (int64_t)16 * j * K + k0 + 32);
+    }
+    bf16x8 a[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) a[i] = lds_frag(src + (16 * i + lr) * ld + k0 + lk);
+    for (int i = 0; i < 4; ++i) a[i] = lds_frag(arow + 16 * i * ld + k0);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+    if (k0 + 32 < K) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = bn[j];
+    }
   }
 }
 
 __device__ __forceinline__ void infer_store(f32x4 (&acc)[4][4], const float* __restrict__ bias,
                                             uint16_t* Hs) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int rb = w >> 2, cb = w & 3;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int j = 0; j < 4; ++j) {
+    const int n = 64 * cb + 16 * j + (lane & 15);
+    const float bn = bias[n];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int n = 64 * w + 16 * j + (lane & 15);
-      const float bn = bias[n];
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = 16 * i + 4 * (lane >> 4) + r;
+        const int row = 64 * rb + 16 * i + 4 * (lane >> 4) + r;
         Hs[row * kHs + n] = f2bf(gelu_f(acc[i][j][r] + bn));
       }
-    }
+  }
 }
 
-__global__ void __launch_bounds__(256) k_mlp_infer(const uint16_t* __restrict__ X, int64_t n,
-                                                   const float* __restrict__ params,
-                                                   const uint16_t* __restrict__ wbf,
-                                                   float* __restrict__ prob,
-                                                   float* __restrict__ logit) {
-  __shared__ __attribute__((aligned(16))) uint16_t Xs[64 * kXs];
-  __shared__ __attribute__((aligned(16))) uint16_t Hs[64 * kHs];
-  __shared__ float zp[4][64];
+template <int RB>
+__global__ void __launch_bounds__(256 * RB) k_mlp_infer(const uint16_t* __restrict__ X, int64_t n,
+                                                        const float* __restrict__ params,
+                                                        const uint16_t* __restrict__ wbf,
+                                                        float* __restrict__ prob,
+                                                        float* __restrict__ logit) {
+  constexpr int ROWS = 64 * RB;
+  __shared__ __attribute__((aligned(16))) uint16_t Xs[ROWS * kXs];
+  __shared__ __attribute__((aligned(16))) uint16_t Hs[ROWS * kHs];
+  __shared__ float zp[4][ROWS];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int64_t m0 = (int64_t)blockIdx.x * 64;
-  // stage 64 x 64 inputs: 512 16-byte pieces, two per thread
-  for (int e = t; e < 64 * IN / 8; e += 256) {
+  const int rb = w >> 2, cb = w & 3;
+  const int64_t m0 = (int64_t)blockIdx.x * ROWS;
+  for (int e = t; e < ROWS * IN / 8; e += 256 * RB) {
     const int r = e / (IN / 8), c8 = (e % (IN / 8)) * 8;
     u32x4 v = {0u, 0u, 0u, 0u};
     if (m0 + r < n) v = *(const u32x4*)(X + (m0 + r) * IN + c8);
@@ -416,7 +444,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const uint16_t* __restrict__ 
     for (int r = 0; r < 4; ++r) part[i][r] = 0.f;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const int col = 64 * w + 16 * j + (lane & 15);
+    const int col = 64 * cb + 16 * j + (lane & 15);
     const float bn = b3[col], wn = w4[col];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -433,13 +461,13 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const uint16_t* __restrict__ 
       v += __shfl_xor(v, 2, 64);
       v += __shfl_xor(v, 4, 64);
       v += __shfl_xor(v, 8, 64);
-      if ((lane & 15) == 0) zp[w][16 * i + 4 * (lane >> 4) + r] = v;
+      if ((lane & 15) == 0) zp[cb][64 * rb + 16 * i + 4 * (lane >> 4) + r] = v;
     }
   __syncthreads();
-  if (t < 64 && m0 + t < n) {
+  if (t < ROWS && m0 + t < n) {
     const float z = zp[0][t] + zp[1][t] + zp[2][t] + zp[3][t] + params[OFF_B4];
     if (logit) logit[m0 + t] = z;
-    if (prob) prob[m0 + t] = 1.0f / (1.0f + expf(-z));
+    if (prob) prob[m0 + t] = 1.0f / (1.0f + __expf(-z));
   }
 }
 
@@ -447,6 +475,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const uint16_t* __restrict__ 
 static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 constexpr int64_t kSplitK = 512;  // batch rows per weight-gradient partial
+constexpr int kInferTileDefault = 64;  // rows per workgroup of the fused inference
 
 struct TrainWs {
   float *a1, *a2, *a3;
@@ -643,8 +672,10 @@ int paces_mlp_grad(const uint16_t* X, const int32_t* idx, const uint8_t* labels,
   dw_layer(p.d0, X, IN, idx, B, p.part, grads + OFF_W1, st);
   bias_grad(p.d0, B, p.colpart, grads + OFF_B1, st);
   // db4 and the loss
-  hipMemcpyAsync(grads + OFF_B4, p.scal, 4, hipMemcpyDeviceToDevice, st);
-  if (loss) hipMemcpyAsync(loss, p.scal + 1, 4, hipMemcpyDeviceToDevice, st);
+  if (hipMemcpyAsync(grads + OFF_B4, p.scal, 4, hipMemcpyDeviceToDevice, st) != hipSuccess)
+    return PACES_E_HIP;
+  if (loss && hipMemcpyAsync(loss, p.scal + 1, 4, hipMemcpyDeviceToDevice, st) != hipSuccess)
+    return PACES_E_HIP;
   return launch_status();
 }
 
@@ -660,12 +691,26 @@ int paces_mlp_adam(float* params, const float* grads, float* m, float* v, int32_
 
 int paces_mlp_infer(const uint16_t* X, int64_t n, const float* params, const uint16_t* wbf16,
                     float* prob, float* logit, void* stream) {
+  return paces_mlp_infer_tiled(X, n, params, wbf16, prob, logit, 0, stream);
+}
+
+int paces_mlp_infer_tiled(const uint16_t* X, int64_t n, const float* params,
+                          const uint16_t* wbf16, float* prob, float* logit, int32_t tile_rows,
+                          void* stream) {
   if (n < 0 || !params || !wbf16 || (!prob && !logit)) return PACES_E_INVALID;
+  if (tile_rows == 0) tile_rows = kInferTileDefault;
+  if (tile_rows != 64 && tile_rows != 128) return PACES_E_INVALID;
   if (n == 0) return PACES_OK;
   if (!X) return PACES_E_INVALID;
-  hipLaunchKernelGGL(k_mlp_infer, dim3(grid_for(n, 64)), dim3(256), 0, (hipStream_t)stream, X, n,
-                     params, wbf16, prob, logit);
+  hipStream_t st = (hipStream_t)stream;
+  if (tile_rows == 128)
+    hipLaunchKernelGGL(k_mlp_infer<2>, dim3(grid_for(n, 128)), dim3(512), 0, st, X, n, params,
+                       wbf16, prob, logit);
+  else
+    hipLaunchKernelGGL(k_mlp_infer<1>, dim3(grid_for(n, 64)), dim3(256), 0, st, X, n, params,
+                       wbf16, prob, logit);
   return launch_status();
 }
+
 
 }  // extern "C"

@@ -60,6 +60,28 @@ __global__ void k_tay_zero(paces_taylor_state_t* s) {
 
 // Block-level max of two non-negative doubles given as bit patterns; thread 0 writes the pair
 // to part[2*blockIdx.x]. Non-negative doubles order like their bit patterns.
+template <int BS>
+__device__ __forceinline__ void block_max2_store_t(u64 a, u64 b, u64* __restrict__ part) {
+  __shared__ u64 sa[BS / 64], sb[BS / 64];
+  a = wave_max_u64(a);
+  b = wave_max_u64(b);
+  if ((threadIdx.x & 63) == 0) {
+    sa[threadIdx.x >> 6] = a;
+    sb[threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u64 ma = 0, mb = 0;
+#pragma unroll
+    for (int q = 0; q < BS / 64; ++q) {
+      ma = sa[q] > ma ? sa[q] : ma;
+      mb = sb[q] > mb ? sb[q] : mb;
+    }
+    part[2 * blockIdx.x] = ma;
+    part[2 * blockIdx.x + 1] = mb;
+  }
+}
+
 __device__ __forceinline__ void block_max2_store(u64 a, u64 b, u64* __restrict__ part) {
   __shared__ u64 sa[kBlock / 64], sb[kBlock / 64];
   a = wave_max_u64(a);
@@ -145,28 +167,39 @@ __global__ void __launch_bounds__(1024) k_tay_begin_fin(paces_taylor_state_t* s,
   if (threadIdx.x == 0) s->c1 = sqrt(__longlong_as_double((long long)m0));
 }
 
-__global__ void __launch_bounds__(kBlock) k_tay_term(int64_t R, const double* __restrict__ diag,
-                                                     CsrSet H, const double2* __restrict__ xin,
-                                                     double2* __restrict__ xout,
-                                                     double2* __restrict__ res, double c,
-                                                     const paces_taylor_state_t* __restrict__ s,
-                                                     u64* __restrict__ part) {
+// One Taylor term j: x_out = (-i c) H x_in, res += x_out, per-workgroup maxima of |x_out|^2 and
+// |res|^2 for the stopping test. (a + ib) * (-0 - ic) = (b*c, -(a*c)) exactly. Launch shape: BS
+// threads per workgroup, each taking RPT rows (t, t + BS, ...) of the workgroup's BS * RPT rows;
+// per-row arithmetic does not depend on the shape and the block maxima are order-independent, so
+// every shape gives identical results.
+template <int BS, int RPT>
+__global__ void __launch_bounds__(BS) k_tay_term_t(int64_t R, const double* __restrict__ diag,
+                                                   CsrSet H, const double2* __restrict__ xin,
+                                                   double2* __restrict__ xout,
+                                                   double2* __restrict__ res, double c,
+                                                   const paces_taylor_state_t* __restrict__ s,
+                                                   u64* __restrict__ part) {
   if (s->done) return;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * (BS * RPT) + threadIdx.x;
   u64 mt = 0, mr = 0;
-  if (i < R) {
-    const double2 h = hx_row(i, diag, H, xin);
-    // (a + ib) * (-0 - ic) = (b*c, -(a*c)) exactly
-    const double2 t = make_double2(h.y * c, -(h.x * c));
-    xout[i] = t;
-    double2 r = res[i];
-    r.x = r.x + t.x;
-    r.y = r.y + t.y;
-    res[i] = r;
-    mt = (u64)__double_as_longlong(t.x * t.x + t.y * t.y);
-    mr = (u64)__double_as_longlong(r.x * r.x + r.y * r.y);
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int64_t i = base + (int64_t)q * BS;
+    if (i < R) {
+      const double2 h = hx_row(i, diag, H, xin);
+      const double2 t = make_double2(h.y * c, -(h.x * c));
+      xout[i] = t;
+      double2 r = res[i];
+      r.x = r.x + t.x;
+      r.y = r.y + t.y;
+      res[i] = r;
+      const u64 a = (u64)__double_as_longlong(t.x * t.x + t.y * t.y);
+      const u64 b = (u64)__double_as_longlong(r.x * r.x + r.y * r.y);
+      mt = a > mt ? a : mt;
+      mr = b > mr ? b : mr;
+    }
   }
-  block_max2_store(mt, mr, part);
+  block_max2_store_t<BS>(mt, mr, part);
 }
 
 __global__ void __launch_bounds__(1024) k_tay_check(paces_taylor_state_t* s,
@@ -235,26 +268,33 @@ __global__ void __launch_bounds__(kBlock) k_tay_begin_mixed(int64_t R,
   block_max2_store(m, 0, part);
 }
 
-__global__ void __launch_bounds__(kBlock) k_tay_term_mixed(
+template <int BS, int RPT>
+__global__ void __launch_bounds__(BS) k_tay_term_mixed_t(
     int64_t R, const float* __restrict__ diag, CsrSet32 H, const float2* __restrict__ xin,
     float2* __restrict__ xout, double2* __restrict__ res, float c,
     const paces_taylor_state_t* __restrict__ s, u64* __restrict__ part) {
   if (s->done) return;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * (BS * RPT) + threadIdx.x;
   u64 mt = 0, mr = 0;
-  if (i < R) {
-    const float2 h = hx_row32(i, diag, H, xin);
-    const float2 t = make_float2(h.y * c, -(h.x * c));
-    xout[i] = t;
-    double2 r = res[i];
-    r.x = r.x + (double)t.x;
-    r.y = r.y + (double)t.y;
-    res[i] = r;
-    const double tx = t.x, ty = t.y;
-    mt = (u64)__double_as_longlong(tx * tx + ty * ty);
-    mr = (u64)__double_as_longlong(r.x * r.x + r.y * r.y);
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int64_t i = base + (int64_t)q * BS;
+    if (i < R) {
+      const float2 h = hx_row32(i, diag, H, xin);
+      const float2 t = make_float2(h.y * c, -(h.x * c));
+      xout[i] = t;
+      double2 r = res[i];
+      r.x = r.x + (double)t.x;
+      r.y = r.y + (double)t.y;
+      res[i] = r;
+      const double tx = t.x, ty = t.y;
+      const u64 a = (u64)__double_as_longlong(tx * tx + ty * ty);
+      const u64 b = (u64)__double_as_longlong(r.x * r.x + r.y * r.y);
+      mt = a > mt ? a : mt;
+      mr = b > mr ? b : mr;
+    }
   }
-  block_max2_store(mt, mr, part);
+  block_max2_store_t<BS>(mt, mr, part);
 }
 
 __global__ void __launch_bounds__(kBlock) k_f64_to_f32(int64_t n, const double* __restrict__ in,
@@ -391,7 +431,7 @@ int paces_spmv(int64_t R, const double* diag, const paces_csr_set_t* h, const do
 }
 
 size_t paces_taylor_workspace_bytes(int64_t R) {
-  int64_t nb = (R + kBlock - 1) / kBlock;
+  int64_t nb = (R + 127) / 128;  // partial pairs for the smallest tile of any launch shape
   return a256((size_t)(nb < 1 ? 1 : nb) * 2 * sizeof(u64));
 }
 
@@ -414,17 +454,30 @@ int paces_taylor_begin(int64_t R, const double* v, double* res, paces_taylor_sta
 int paces_taylor_term(int64_t R, const double* diag, const paces_csr_set_t* h, const double* x_in,
                       double* x_out, double* res, double c, int32_t j, double tol,
                       paces_taylor_state_t* state, void* ws, size_t ws_bytes, void* stream) {
+  return paces_taylor_term_tuned(R, diag, h, x_in, x_out, res, c, j, tol, state, ws, ws_bytes,
+                                 nullptr, stream);
+}
+
+int paces_taylor_term_tuned(int64_t R, const double* diag, const paces_csr_set_t* h,
+                            const double* x_in, double* x_out, double* res, double c, int32_t j,
+                            double tol, paces_taylor_state_t* state, void* ws, size_t ws_bytes,
+                            const paces_launch_cfg_t* cfg, void* stream) {
   CsrSet H;
-  if (R < 0 || !state || !ws || !csr_set(h, &H)) return PACES_E_INVALID;
+  if (R < 0 || !state || !ws || !csr_set(h, &H) || !cfg_ok(cfg)) return PACES_E_INVALID;
   if (R > 0 && (!diag || !x_in || !x_out || !res)) return PACES_E_INVALID;
   if (ws_bytes < paces_taylor_workspace_bytes(R)) return PACES_E_WORKSPACE;
   hipStream_t st = (hipStream_t)stream;
   int nblk = 0;
   if (R > 0) {
-    nblk = (int)grid_for(R, kBlock);
-    hipLaunchKernelGGL(k_tay_term, dim3(nblk), dim3(kBlock), 0, st, R, diag, H,
-                       (const double2*)x_in, (double2*)x_out, (double2*)res, c,
-                       (const paces_taylor_state_t*)state, (u64*)ws);
+    int B, RR;
+    cfg_get(cfg, &B, &RR);
+    nblk = (int)grid_for(R, B * RR);
+#define TAY_LAUNCH(BB, QQ)                                                                  \
+  hipLaunchKernelGGL((k_tay_term_t<BB, QQ>), dim3(nblk), dim3(BB), 0, st, R, diag, H,       \
+                     (const double2*)x_in, (double2*)x_out, (double2*)res, c,              \
+                     (const paces_taylor_state_t*)state, (u64*)ws)
+    PACES_CFG_SWITCH(B, RR, TAY_LAUNCH)
+#undef TAY_LAUNCH
   }
   hipLaunchKernelGGL(k_tay_check, dim3(1), dim3(1024), 0, st, state, (const u64*)ws, nblk,
                      (int)j, tol);
@@ -460,17 +513,30 @@ int paces_taylor_term_mixed(int64_t R, const float* diag, const paces_csr32_set_
                             const float* x_in, float* x_out, double* res, float c, int32_t j,
                             double tol, paces_taylor_state_t* state, void* ws, size_t ws_bytes,
                             void* stream) {
+  return paces_taylor_term_mixed_tuned(R, diag, h, x_in, x_out, res, c, j, tol, state, ws,
+                                       ws_bytes, nullptr, stream);
+}
+
+int paces_taylor_term_mixed_tuned(int64_t R, const float* diag, const paces_csr32_set_t* h,
+                                  const float* x_in, float* x_out, double* res, float c,
+                                  int32_t j, double tol, paces_taylor_state_t* state, void* ws,
+                                  size_t ws_bytes, const paces_launch_cfg_t* cfg, void* stream) {
   CsrSet32 H;
-  if (R < 0 || !state || !ws || !csr_set32(h, &H)) return PACES_E_INVALID;
+  if (R < 0 || !state || !ws || !csr_set32(h, &H) || !cfg_ok(cfg)) return PACES_E_INVALID;
   if (R > 0 && (!diag || !x_in || !x_out || !res)) return PACES_E_INVALID;
   if (ws_bytes < paces_taylor_workspace_bytes(R)) return PACES_E_WORKSPACE;
   hipStream_t st = (hipStream_t)stream;
   int nblk = 0;
   if (R > 0) {
-    nblk = (int)grid_for(R, kBlock);
-    hipLaunchKernelGGL(k_tay_term_mixed, dim3(nblk), dim3(kBlock), 0, st, R, diag, H,
-                       (const float2*)x_in, (float2*)x_out, (double2*)res, c,
-                       (const paces_taylor_state_t*)state, (u64*)ws);
+    int B, RR;
+    cfg_get(cfg, &B, &RR);
+    nblk = (int)grid_for(R, B * RR);
+#define TAYM_LAUNCH(BB, QQ)                                                                 \
+  hipLaunchKernelGGL((k_tay_term_mixed_t<BB, QQ>), dim3(nblk), dim3(BB), 0, st, R, diag, H, \
+                     (const float2*)x_in, (float2*)x_out, (double2*)res, c,                \
+                     (const paces_taylor_state_t*)state, (u64*)ws)
+    PACES_CFG_SWITCH(B, RR, TAYM_LAUNCH)
+#undef TAYM_LAUNCH
   }
   hipLaunchKernelGGL(k_tay_check, dim3(1), dim3(1024), 0, st, state, (const u64*)ws, nblk,
                      (int)j, tol);

@@ -511,6 +511,12 @@ class LearnedTimeEvolution(TimeEvolution):
         if self.scorer is None:
             self._install(U, in_basis, cand_pos, psiU, diagU, basis, offs, src, nup)
         else:
+            tuner = self._tuner()
+            if tuner is not None and R:
+                from ..core import autotune
+                self.scorer.tile_rows = tuner.pick(
+                    tuner.key("mlp", R), autotune.MLP_TILES,
+                    lambda t: (lambda: self.scorer.infer(X, tile_rows=t)))
             prob = self.scorer.infer(X)
             forced, _, _ = engine.lower_bound(U, select, want_hit=False, q_sorted=True)
             keep = torch.empty(max(R, 1), dtype=torch.uint8, device=self.device)
@@ -696,7 +702,7 @@ class BatchedTimeEvolution:
     """
 
     def __init__(self, hamobj: BatchedHamiltonian, maxstates, enlarge_steps=1,
-                 precision="mixed", m_star=100, taylor_tol=None):
+                 precision="mixed", m_star=100, taylor_tol=None, autotune=False):
         self.hamobj = hamobj
         self.device = hamobj.device
         ms = numpy.asarray(maxstates, dtype=numpy.int64)
@@ -713,6 +719,7 @@ class BatchedTimeEvolution:
             engine.TAYLOR_TOL_MIXED if precision == "mixed" else engine.TAYLOR_TOL)
         self._ws = hamobj._ws
         self._tws = _hip.Workspace(self.device)
+        self.autotune = bool(autotune)
         self.whoami = None
         self.vector = None
         self.seg_off = None
@@ -771,17 +778,20 @@ class BatchedTimeEvolution:
         hdr = torch.zeros(16, dtype=torch.uint8, device=self.device)
         wsp, wsn = self._tws.get(_lib.paces_batch_taylor_workspace_bytes(R, h.ntraj))
         st = stream()
-        check(_lib.paces_batch_taylor_begin(R, ptr(torch.view_as_real(self.vector)),
-                                            ptr(torch.view_as_real(res)),
-                                            ptr(torch.view_as_real(bufs[2])), 1 if mixed else 0,
-                                            ptr(self.seg_off), h.ntraj, ptr(ts), wsp, wsn, st),
-              "batch_taylor_begin")
         if mixed:
             mm = engine.MixedMats(self.diag_vals, mats)
             h64, h32, dptr = None, C.byref(mm.cset), ptr(mm.diag)
         else:
             s64 = engine.csr_set(mats)
             h64, h32, dptr = C.byref(s64), None, ptr(self.diag_vals)
+        cfg = self._launch_shape(R, mats, mixed, dptr, h64, h32, wsp, wsn, st)
+        cref = C.byref(cfg) if cfg is not None else None
+        check(_lib.paces_batch_taylor_begin_tuned(R, ptr(torch.view_as_real(self.vector)),
+                                                  ptr(torch.view_as_real(res)),
+                                                  ptr(torch.view_as_real(bufs[2])),
+                                                  1 if mixed else 0, ptr(self.seg_off), h.ntraj,
+                                                  ptr(ts), wsp, wsn, cref, st),
+              "batch_taylor_begin")
         x_in = bufs[2]
         j = 0
         while j < self.m_star:
@@ -789,11 +799,11 @@ class BatchedTimeEvolution:
                 if j >= self.m_star:
                     break
                 x_out = bufs[j % 2]
-                check(_lib.paces_batch_taylor_term(
+                check(_lib.paces_batch_taylor_term_tuned(
                     R, dptr, h64, h32, ptr(torch.view_as_real(x_in)),
                     ptr(torch.view_as_real(x_out)), ptr(torch.view_as_real(res)),
                     engine.taylor_coeff(delta_t, j), j, self.tol, ptr(self.seg_off), h.ntraj,
-                    ptr(ts), ptr(hdr), wsp, wsn, st), "batch_taylor_term")
+                    ptr(ts), ptr(hdr), wsp, wsn, cref, st), "batch_taylor_term")
                 x_in = x_out
                 j += 1
             if int(hdr[:4].view(torch.int32).item()) == 0:
@@ -803,6 +813,36 @@ class BatchedTimeEvolution:
             [("c1", "f8"), ("c1pc2", "f8"), ("ratio", "f8"), ("done", "i4"), ("j", "i4")]))
         self.last_expm = info
         return info
+
+    def _launch_shape(self, R, mats, mixed, dptr, h64, h32, wsp, wsn, st):
+        """The tuned launch shape of the batched term (None: the default), timed on scratch
+        buffers with the stopping test off the first time a size bucket is seen."""
+        if not self.autotune or R == 0:
+            return None
+        from ..core import autotune
+        h = self.hamobj
+        vdt = torch.complex64 if mixed else torch.complex128
+        tuner = autotune.default_tuner()
+        key = tuner.key("btay32" if mixed else "btay64", R, len(mats), h.ntraj)
+
+        def run(shape):
+            cfg = _hip.LaunchCfg(*shape)
+            res = torch.empty_like(self.vector)
+            xa = torch.empty(R, dtype=vdt, device=self.device)
+            xb = torch.empty(R, dtype=vdt, device=self.device)
+            ts = torch.zeros(h.ntraj * C.sizeof(_hip.BatchTaylor), dtype=torch.uint8,
+                             device=self.device)
+            hdr = torch.zeros(16, dtype=torch.uint8, device=self.device)
+            check(_lib.paces_batch_taylor_begin_tuned(
+                R, ptr(torch.view_as_real(self.vector)), ptr(torch.view_as_real(res)),
+                ptr(torch.view_as_real(xa)), 1 if mixed else 0, ptr(self.seg_off), h.ntraj,
+                ptr(ts), wsp, wsn, C.byref(cfg), st), "batch_taylor_begin_tuned")
+            return lambda: check(_lib.paces_batch_taylor_term_tuned(
+                R, dptr, h64, h32, ptr(torch.view_as_real(xa)), ptr(torch.view_as_real(xb)),
+                ptr(torch.view_as_real(res)), 1e-3, 0, -1.0, ptr(self.seg_off), h.ntraj, ptr(ts),
+                ptr(hdr), wsp, wsn, C.byref(cfg), st), "batch_taylor_term_tuned")
+
+        return autotune.launch_cfg(tuner.pick(key, autotune.SHAPES, run))
 
     def step(self, i, delta_t):
         """Step 0 adapts only; later steps adapt then propagate (time_evolution.py:426-458)."""

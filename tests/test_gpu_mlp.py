@@ -42,14 +42,15 @@ def _theta(seed):
     return th.astype(np.float32)
 
 
+@pytest.mark.parametrize("tile", [64, 128])
 @pytest.mark.parametrize("n", [1, 64, 1000, 4099])
-def test_infer_matches_oracle(env, n):
+def test_infer_matches_oracle(env, n, tile):
     sc = env["scorer"]
     th = _theta(n)
     X = np.random.default_rng(n).uniform(-1, 1, (n, mo.IN)).astype(np.float32)
     s = sc.CandidateScorer(env["dev"], params=th)
-    z = s.infer(_bits(env, X), want_logit=True).cpu().numpy()
-    p = s.infer(_bits(env, X)).cpu().numpy()
+    z = s.infer(_bits(env, X), want_logit=True, tile_rows=tile).cpu().numpy()
+    p = s.infer(_bits(env, X), tile_rows=tile).cpu().numpy()
     zo = mo.forward(th.astype(np.float64), X)["z"]
     assert np.all(np.abs(z - zo) <= 1e-2 + 1e-2 * np.abs(zo)), np.max(np.abs(z - zo))
     np.testing.assert_allclose(p, 1 / (1 + np.exp(-z.astype(np.float64))), rtol=1e-5, atol=1e-6)
