@@ -115,15 +115,16 @@ int paces_lower_bound(const uint64_t* book, int64_t M, const uint64_t* q, int64_
 
 /* ---- A7: union of K sorted unique runs (cupy_unique, paces/aux/helpers.py:63-75, applied to
  * the concatenation built in hamiltonian.py:434-452 and 563-577). out_rows needs capacity for
- * sum(len) rows; *out_count (device) receives the number of unique rows; out_from_run0[p] = 1 when
- * output row p came from run 0; cand_pos (optional, sum(len) entries) receives the output index
- * of every input row, duplicates included. */
+ * sum(len) rows; *out_count (device) receives the number of unique rows; out_from_run0[p] = 1
+ * when output row p came from run 0; cand_pos (optional, sum(len) entries) receives the output
+ * index of every input row, duplicates included. The workspace size depends on the total, the
+ * run count and the row width. */
 typedef struct {
   const uint64_t* ptr[PACES_MAX_RUNS];
   int64_t len[PACES_MAX_RUNS];
   int32_t K;
 } paces_runs_t;
-size_t paces_merge_unique_workspace_bytes(int64_t total, int32_t K);
+size_t paces_merge_unique_workspace_bytes(int64_t total, int32_t K, int32_t W);
 int paces_merge_unique(const paces_runs_t* runs, int32_t W, uint64_t* out_rows,
                        int32_t* out_count, uint8_t* out_from_run0, int32_t* cand_pos, void* ws,
                        size_t ws_bytes, void* stream);

@@ -104,7 +104,7 @@ _SIGNATURES = {
     "paces_add_field": (C.c_int, [_P, _I64, C.POINTER(Layout), _I32, _I32, _P]),
     "paces_lower_bound_workspace_bytes": (_SZ, [_I64]),
     "paces_lower_bound": (C.c_int, [_P, _I64, _P, _I64, _I32, _I32, _P, _P, _P, _P, _SZ, _P]),
-    "paces_merge_unique_workspace_bytes": (_SZ, [_I64, _I32]),
+    "paces_merge_unique_workspace_bytes": (_SZ, [_I64, _I32, _I32]),
     "paces_merge_unique": (C.c_int, [C.POINTER(Runs), _I32, _P, _P, _P, _P, _P, _SZ, _P]),
     "paces_select_workspace_bytes": (_SZ, [_I64]),
     "paces_select_topk": (C.c_int, [_P, _P, _I64, _I32, _I64, _P, _P, _P, _SZ, _P]),

@@ -92,7 +92,7 @@ def merge_unique(runs, W, ws, want_from0=False, want_cand_pos=False):
     cnt = torch.empty(1, dtype=torch.int32, device=dev)
     from0 = torch.empty(max(total, 1), dtype=torch.uint8, device=dev) if want_from0 else None
     cpos = torch.empty(max(total, 1), dtype=torch.int32, device=dev) if want_cand_pos else None
-    wsp, wsn = ws.get(lib.paces_merge_unique_workspace_bytes(total, R.K))
+    wsp, wsn = ws.get(lib.paces_merge_unique_workspace_bytes(total, R.K, W))
     check(lib.paces_merge_unique(C.byref(R), W, ptr(out), ptr(cnt), ptr(from0), ptr(cpos), wsp,
                                  wsn, stream()), "merge_unique")
     n = int(cnt.item())

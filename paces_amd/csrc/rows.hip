@@ -5,6 +5,7 @@
 namespace paces {
 
 int scan_u8(const uint8_t* in, int64_t n, int32_t* out, void* ws, size_t ws_bytes, hipStream_t st);
+int scan_i32(const int32_t* in, int64_t n, int32_t* out, void* ws, size_t ws_bytes, hipStream_t st);
 size_t scan_ws_bytes(int64_t n);
 
 // ------------------------------------------------------------------------------------------------
@@ -503,7 +504,8 @@ int paces_lower_bound(const uint64_t* book, int64_t M, const uint64_t* q, int64_
                    (hipStream_t)stream);
 }
 
-size_t paces_merge_unique_workspace_bytes(int64_t total, int32_t K) {
+size_t paces_merge_unique_workspace_bytes(int64_t total, int32_t K, int32_t W) {
+  if (total < 0 || K < 1 || W < 1) return 0;
   return mu_ws_bytes(total, K);
 }
 
@@ -528,6 +530,7 @@ int paces_merge_unique(const paces_runs_t* runs, int32_t W, uint64_t* out_rows,
     R.boff[s + 1] = R.boff[s] + (R.off[s + 1] - R.off[s] + kBlock - 1) / kBlock;
   if (R.off[R.K] >= 2147483647LL) return PACES_E_INVALID;
   if (R.off[R.K] > 0 && !out_rows) return PACES_E_INVALID;
+  if (W < 1 || W > PACES_MAX_ROW_WORDS) return PACES_E_UNSUPPORTED;
   PACES_DISPATCH_W(W, run_mu, R, out_rows, out_count, out_from_run0, cand_pos, ws, ws_bytes,
                    (hipStream_t)stream);
 }

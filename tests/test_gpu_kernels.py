@@ -189,6 +189,38 @@ def test_merge_unique_matches_numpy_unique(env, W, K):
     assert out.shape[0] == 0
 
 
+@pytest.mark.parametrize("W,K", [(1, 5), (3, 5), (3, 9), (5, 3), (13, 6)])
+def test_merge_unique_large(env, W, K):
+    """Unions of 2^16 and more rows: rows, run-0 flags and the output index of every input row
+    against NumPy. One run is a dense block of consecutive rows (a narrow key range where the
+    partition table's ranges are uneven), one is empty."""
+    eng, hip = env["engine"], env["hip"]
+    rng = np.random.default_rng(W * 1000 + K)
+    pool = _rand_unique_rows(rng, 400000, W)
+    runs = []
+    for k in range(K):
+        if k == 1:
+            a = int(rng.integers(0, pool.shape[0] - 60000))
+            r = pool[a:a + 60000]  # a dense block of consecutive rows
+        elif k == K - 1:
+            r = pool[:0]
+        else:
+            n = int(rng.integers(20000, 120000))
+            r = pool[np.sort(rng.choice(pool.shape[0], n, replace=False))]
+        runs.append(r)
+    d_runs = [(_rows(env, r) if r.shape[0] else _rows(env, pool[:1]), 0, r.shape[0])
+              for r in runs]
+    ws = hip.Workspace(env["dev"])
+    out, from0, cpos = eng.merge_unique(d_runs, W, ws, want_from0=True, want_cand_pos=True)
+    cat = np.concatenate(runs)
+    exp = po.unique_rows(cat)
+    assert cat.shape[0] >= 1 << 16
+    np.testing.assert_array_equal(_np(env, out), exp)
+    _, in0 = po.lookup(runs[0], exp)
+    np.testing.assert_array_equal(from0.cpu().numpy().astype(bool), in0)
+    np.testing.assert_array_equal(cpos.cpu().numpy(), po.lower_bound(exp, cat))
+
+
 @pytest.mark.parametrize("R,k", [(1, 1), (10, 3), (5000, 1), (5000, 4999), (5000, 5000),
                                  (200000, 20000), (200000, 300000)])
 def test_select_topk_stable_ties(env, R, k):
