@@ -209,7 +209,7 @@ def measure_batched_term(bte, reps=20):
     return e0.elapsed_time(e1) / reps * 1e-3, btay_term_bytes(R, mats, mixed), reps
 
 
-def batched_leg(draws, precision, warmup, steps, label):
+def batched_leg(draws, precision, warmup, steps, label, graph=False, autotune=False):
     """Batched run: step 0 and ``warmup`` steps untimed, then ``steps`` timed steps. Returns
     (state-steps, seconds, the evolution object)."""
     import torch
@@ -217,7 +217,7 @@ def batched_leg(draws, precision, warmup, steps, label):
     bham = holstein.BatchedHamiltonian([d[0] for d in draws], 8, [-d[1] for d in draws],
                                        [d[2] for d in draws])
     bte = holstein.BatchedTimeEvolution(bham, [d[3] for d in draws], enlarge_steps=1,
-                                        precision=precision)
+                                        precision=precision, graph=graph, autotune=autotune)
     bte.create_single_exciton_states([d[0] // 2 for d in draws])
     bte.step(0, TRAJ_DT)
     for i in range(1, warmup + 1):
@@ -323,7 +323,8 @@ def run_traj256(a):
     import torch
     draws = traj256_draws(a.ntraj)
     prec = a.precision_batched
-    units, elapsed, bte = batched_leg(draws, prec, a.warmup, a.steps, "batched-" + prec)
+    units, elapsed, bte = batched_leg(draws, prec, a.warmup, a.steps, "batched-" + prec,
+                                      graph=a.graph, autotune=a.autotune)
     rows = bte.numstates
     value = units / elapsed
     sec, nbytes, reps = measure_batched_term(bte)
@@ -359,6 +360,7 @@ def run_traj256(a):
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
             "data": "synthetic",
             "config": dict({"workload": TRAJ_DESC.format(n=a.ntraj), "form": "batched-" + prec,
+                            "series_graph": bool(a.graph), "autotune": bool(a.autotune),
                             "unit_of_work": "retained basis state advanced one step "
                                             "(sum over trajectories of min(maxstates, R_prev))",
                             "final_basis_rows": int(rows),
@@ -549,6 +551,10 @@ def main(argv=None):
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-check", action="store_true",
                     help="skip the batched FP64 run that gives the density error")
+    ap.add_argument("--graph", type=int, default=1,
+                    help="traj256: run each step's Taylor series as one replayed HIP graph")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="traj256: tune the Taylor-term launch shape")
     ap.add_argument("--learned-ms", type=int, default=2_000_000)
     ap.add_argument("--learned-precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--labels", type=int, default=100_000_000,

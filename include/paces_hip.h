@@ -355,6 +355,38 @@ int paces_batch_taylor_term_tuned(int64_t R, const void* diag, const paces_csr_s
                                   paces_batch_taylor_hdr_t* hdr, void* ws, size_t ws_bytes,
                                   const paces_launch_cfg_t* cfg, void* stream);
 
+/* ---- N1 the batched series as one graph-capturable launch sequence (no reference
+ * counterpart). Every argument lives in a device-resident descriptor that the caller rewrites
+ * before each step, so a HIP graph captured once from paces_batch_series_launch is replayed at
+ * every step. Grids are sized for the capacity R_cap (>= the step's rows); workgroups past the
+ * step's segments exit. The launch runs begin and terms 0 .. nterms-1 (term j reads x[2] for
+ * j == 0, else x[(j-1) % 2], and writes x[j % 2]); terms of stopped trajectories do nothing, and
+ * a series not finished after nterms continues with paces_batch_taylor_term_tuned from term
+ * nterms with the same state. */
+#define PACES_SERIES_MAX_TERMS 64
+typedef struct {
+  int64_t R;                              /* rows this step (informational; grids use R_cap) */
+  const void* diag;                       /* float (mixed) or double */
+  int32_t nterms_h;                       /* Hamiltonian terms in the arrays below */
+  int32_t pad0;
+  const int32_t* indptr[PACES_MAX_TERMS];
+  const int32_t* indices[PACES_MAX_TERMS];
+  const void* data[PACES_MAX_TERMS];      /* float (mixed) or double values */
+  const double* v;                        /* start vector, complex128 */
+  double* res;                            /* result, complex128 */
+  void* x[3];                             /* series buffers (complex64 or complex128) */
+  const int32_t* seg_off;                 /* ntraj + 1 */
+  paces_batch_taylor_t* ts;               /* ntraj */
+  paces_batch_taylor_hdr_t* hdr;
+  void* ws;                               /* paces_batch_taylor_workspace_bytes(R_cap, ntraj) */
+  double tol;
+  double c[PACES_SERIES_MAX_TERMS];       /* term coefficients fl(dt / (j + 1)) */
+} paces_batch_series_t;
+size_t paces_batch_series_desc_bytes(void); /* sizeof(paces_batch_series_t), for bindings */
+int paces_batch_series_launch(const paces_batch_series_t* desc, int64_t R_cap, int32_t ntraj,
+                              int32_t mixed, int32_t nterms, const paces_launch_cfg_t* cfg,
+                              void* stream);
+
 /* ---- N3 candidate scorer (no reference counterpart): 64 inputs -> 3 x 256 GELU -> 1 logit,
  * BF16 matrix operands with FP32 accumulation on the matrix cores. params is the FP32 master
  * vector (PACES_MLP_NPARAM floats):

@@ -84,6 +84,18 @@ class LaunchCfg(C.Structure):
     _fields_ = [("block", C.c_int32), ("rows_per_thread", C.c_int32)]
 
 
+SERIES_MAX_TERMS = 64
+
+
+class BatchSeries(C.Structure):
+    _fields_ = [("R", C.c_int64), ("diag", C.c_void_p), ("nterms_h", C.c_int32),
+                ("pad0", C.c_int32), ("indptr", C.c_void_p * MAX_TERMS),
+                ("indices", C.c_void_p * MAX_TERMS), ("data", C.c_void_p * MAX_TERMS),
+                ("v", C.c_void_p), ("res", C.c_void_p), ("x", C.c_void_p * 3),
+                ("seg_off", C.c_void_p), ("ts", C.c_void_p), ("hdr", C.c_void_p),
+                ("ws", C.c_void_p), ("tol", C.c_double), ("c", C.c_double * SERIES_MAX_TERMS)]
+
+
 class TaylorState(C.Structure):
     _fields_ = [("max_term2", C.c_uint64), ("max_res2", C.c_uint64), ("c1", C.c_double),
                 ("c1_plus_c2", C.c_double), ("ratio", C.c_double), ("done", C.c_int32),
@@ -170,6 +182,9 @@ _SIGNATURES = {
     "paces_rows_varying_bits": (C.c_int, [_P, _I64, _I32, _P, _P]),
     "paces_sort_rows_workspace_bytes": (_SZ, [_I64, _I32]),
     "paces_sort_rows": (C.c_int, [_P, _I64, _I32, _P, _I32, _P, _P, _P, _P, _SZ, _P]),
+    "paces_batch_series_desc_bytes": (_SZ, []),
+    "paces_batch_series_launch": (C.c_int, [_P, _I64, _I32, _I32, _I32, C.POINTER(LaunchCfg),
+                                            _P]),
     "paces_mlp_weights_bf16": (C.c_int, [_P, _P, _P]),
     "paces_mlp_train_workspace_bytes": (_SZ, [_I64]),
     "paces_mlp_grad": (C.c_int, [_P, _P, _P, _I64, _F, _P, _P, _P, _P, _P, _SZ, _P]),

@@ -6,6 +6,8 @@
 // paces/core/time_evolution.py:833-910, per trajectory), the observables and the Taylor stopping
 // test (time_evolution.py:482-512, per trajectory). Workgroups are aligned to segments: block b
 // covers tile b - boff[t] of trajectory t, with boff[t] = sum_{u<t} ceil(R_u / kTile).
+#include <stddef.h>
+
 #include "common.hpp"
 
 namespace paces {
@@ -398,7 +400,11 @@ __device__ __forceinline__ typename Vec2<F>::T hx_row_t(int64_t i, const F* __re
   const V xi = x[i];
   const F d = diag[i];
   F yr = d * xi.x, yi = d * xi.y;
-  for (int t = 0; t < H.nterms; ++t) {
+  // unrolled over the compile-time maximum so every H field index is static (registers, not
+  // scratch, when H is built inside the kernel); the order of the sums is the term order
+#pragma unroll
+  for (int t = 0; t < PACES_MAX_TERMS; ++t) {
+    if (t >= H.nterms) break;
     const int32_t b = H.indptr[t][i], e = H.indptr[t][i + 1];
     const F* data = (const F*)H.data[t];
     F tr = 0, ti = 0;
@@ -442,12 +448,12 @@ __device__ __forceinline__ void blk_max2(u64 a, u64 b, u64* __restrict__ part) {
 // Launch shape (tuned): BS threads per workgroup, RPT rows per thread (t, t + BS, ...), a
 // segment-aligned tile of BS * RPT rows; begin and every term of one series use the same shape.
 template <typename F, int BS, int RPT>
-__global__ void __launch_bounds__(BS) k_btay_begin(const double2* __restrict__ v,
-                                                       double2* __restrict__ res,
-                                                       typename Vec2<F>::T* __restrict__ x0,
-                                                       const int32_t* __restrict__ seg_off,
-                                                       const int64_t* __restrict__ boff, int ntraj,
-                                                       u64* __restrict__ part) {
+__device__ __forceinline__ void btay_begin_body(const double2* __restrict__ v,
+                                                double2* __restrict__ res,
+                                                typename Vec2<F>::T* __restrict__ x0,
+                                                const int32_t* __restrict__ seg_off,
+                                                const int64_t* __restrict__ boff, int ntraj,
+                                                u64* __restrict__ part) {
   const SegTile T = seg_tile(seg_off, boff, ntraj, BS * RPT);
   if (T.t < 0) return;
   u64 m = 0;
@@ -469,11 +475,11 @@ __global__ void __launch_bounds__(BS) k_btay_begin(const double2* __restrict__ v
 }
 
 // per-trajectory reduction of the partials; fin == 1: initialise c1, else the stopping test
-__global__ void __launch_bounds__(64) k_btay_reduce(const u64* __restrict__ part,
-                                                    const int64_t* __restrict__ boff,
-                                                    paces_batch_taylor_t* __restrict__ ts,
-                                                    paces_batch_taylor_hdr_t* __restrict__ hdr,
-                                                    int fin, int j, double tol) {
+__device__ __forceinline__ void btay_reduce_body(const u64* __restrict__ part,
+                                                 const int64_t* __restrict__ boff,
+                                                 paces_batch_taylor_t* __restrict__ ts,
+                                                 paces_batch_taylor_hdr_t* __restrict__ hdr,
+                                                 int fin, int j, double tol) {
   const int t = blockIdx.x;
   if (!fin && ts[t].done) return;
   u64 a = 0, b = 0;
@@ -509,8 +515,8 @@ __global__ void __launch_bounds__(64) k_btay_reduce(const u64* __restrict__ part
 }
 
 template <typename F, int BS, int RPT>
-__global__ void __launch_bounds__(BS) k_btay_term(
-    const F* __restrict__ diag, CsrSetB64 H, const typename Vec2<F>::T* __restrict__ xin,
+__device__ __forceinline__ void btay_term_body(
+    const F* __restrict__ diag, const CsrSetB64& H, const typename Vec2<F>::T* __restrict__ xin,
     typename Vec2<F>::T* __restrict__ xout, double2* __restrict__ res, F c,
     const int32_t* __restrict__ seg_off, const int64_t* __restrict__ boff, int ntraj,
     const paces_batch_taylor_t* __restrict__ ts, paces_batch_taylor_hdr_t* __restrict__ hdr,
@@ -540,6 +546,128 @@ __global__ void __launch_bounds__(BS) k_btay_term(
     }
   }
   blk_max2<BS>(mt, mr, part);
+}
+
+// Launch shape (tuned): BS threads per workgroup, RPT rows per thread (t, t + BS, ...), a
+// segment-aligned tile of BS * RPT rows; begin and every term of one series use the same shape.
+template <typename F, int BS, int RPT>
+__global__ void __launch_bounds__(BS) k_btay_begin(const double2* __restrict__ v,
+                                                   double2* __restrict__ res,
+                                                   typename Vec2<F>::T* __restrict__ x0,
+                                                   const int32_t* __restrict__ seg_off,
+                                                   const int64_t* __restrict__ boff, int ntraj,
+                                                   u64* __restrict__ part) {
+  btay_begin_body<F, BS, RPT>(v, res, x0, seg_off, boff, ntraj, part);
+}
+
+__global__ void __launch_bounds__(64) k_btay_reduce(const u64* __restrict__ part,
+                                                    const int64_t* __restrict__ boff,
+                                                    paces_batch_taylor_t* __restrict__ ts,
+                                                    paces_batch_taylor_hdr_t* __restrict__ hdr,
+                                                    int fin, int j, double tol) {
+  btay_reduce_body(part, boff, ts, hdr, fin, j, tol);
+}
+
+template <typename F, int BS, int RPT>
+__global__ void __launch_bounds__(BS) k_btay_term(
+    const F* __restrict__ diag, CsrSetB64 H, const typename Vec2<F>::T* __restrict__ xin,
+    typename Vec2<F>::T* __restrict__ xout, double2* __restrict__ res, F c,
+    const int32_t* __restrict__ seg_off, const int64_t* __restrict__ boff, int ntraj,
+    const paces_batch_taylor_t* __restrict__ ts, paces_batch_taylor_hdr_t* __restrict__ hdr,
+    u64* __restrict__ part) {
+  btay_term_body<F, BS, RPT>(diag, H, xin, xout, res, c, seg_off, boff, ntraj, ts, hdr, part);
+}
+
+// ---- the series from a device descriptor (N1: captured once into a HIP graph, replayed every
+// step after the caller rewrites the descriptor). Grids are sized for a row capacity; workgroups
+// past the step's segments exit (seg_tile), so one graph serves every basis up to the capacity.
+// The descriptor as the kernels read it: the same layout as paces_batch_series_t, with every
+// pointer declared in the global address space, so loads through them are global (not flat)
+// accesses, as for kernel-argument pointers.
+#define PACES_GPTR(T) __attribute__((address_space(1))) T*
+struct SeriesDev {
+  int64_t R;
+  PACES_GPTR(const void) diag;
+  int32_t nterms_h;
+  int32_t pad0;
+  PACES_GPTR(const int32_t) indptr[PACES_MAX_TERMS];
+  PACES_GPTR(const int32_t) indices[PACES_MAX_TERMS];
+  PACES_GPTR(const void) data[PACES_MAX_TERMS];
+  PACES_GPTR(const double) v;
+  PACES_GPTR(double) res;
+  PACES_GPTR(void) x[3];
+  PACES_GPTR(const int32_t) seg_off;
+  PACES_GPTR(paces_batch_taylor_t) ts;
+  PACES_GPTR(paces_batch_taylor_hdr_t) hdr;
+  PACES_GPTR(void) ws;
+  double tol;
+  double c[PACES_SERIES_MAX_TERMS];
+};
+static_assert(sizeof(SeriesDev) == sizeof(paces_batch_series_t), "descriptor layout");
+static_assert(offsetof(SeriesDev, c) == offsetof(paces_batch_series_t, c), "descriptor layout");
+static_assert(offsetof(SeriesDev, ws) == offsetof(paces_batch_series_t, ws), "descriptor layout");
+
+template <typename T>
+__device__ __forceinline__ T* gen(PACES_GPTR(T) p) {
+  return (T*)p;
+}
+
+__device__ __forceinline__ const int64_t* ser_boff(const SeriesDev* d) {
+  return gen((PACES_GPTR(const int64_t))d->ws);
+}
+__device__ __forceinline__ u64* ser_part(const SeriesDev* d, int ntraj) {
+  return gen((PACES_GPTR(u64))((PACES_GPTR(char))d->ws +
+                               (((size_t)(ntraj + 1) * 8 + 255) & ~(size_t)255)));
+}
+
+__global__ void k_bser_segblocks(const paces_batch_series_t* __restrict__ desc, int ntraj,
+                                 int tile) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const SeriesDev* d = (const SeriesDev*)desc;
+  int64_t* boff = gen((PACES_GPTR(int64_t))d->ws);
+  const int32_t* seg = gen(d->seg_off);
+  int64_t b = 0;
+  for (int t = 0; t < ntraj; ++t) {
+    boff[t] = b;
+    b += (seg[t + 1] - seg[t] + tile - 1) / tile;
+  }
+  boff[ntraj] = b;
+}
+
+template <typename F, int BS, int RPT>
+__global__ void __launch_bounds__(BS) k_bser_begin(const paces_batch_series_t* __restrict__ desc,
+                                                   int ntraj) {
+  const SeriesDev* d = (const SeriesDev*)desc;
+  typedef typename Vec2<F>::T V;
+  btay_begin_body<F, BS, RPT>(gen((PACES_GPTR(const double2))d->v),
+                              gen((PACES_GPTR(double2))d->res), gen((PACES_GPTR(V))d->x[2]),
+                              gen(d->seg_off), ser_boff(d), ntraj, ser_part(d, ntraj));
+}
+
+__global__ void __launch_bounds__(64) k_bser_reduce(const paces_batch_series_t* __restrict__ desc,
+                                                    int ntraj, int fin, int j) {
+  const SeriesDev* d = (const SeriesDev*)desc;
+  btay_reduce_body(ser_part(d, ntraj), ser_boff(d), gen(d->ts), gen(d->hdr), fin, j, d->tol);
+}
+
+template <typename F, int BS, int RPT>
+__global__ void __launch_bounds__(BS) k_bser_term(const paces_batch_series_t* __restrict__ desc,
+                                                  int ntraj, int j) {
+  const SeriesDev* d = (const SeriesDev*)desc;
+  CsrSetB64 H;
+  H.nterms = d->nterms_h;
+#pragma unroll
+  for (int t = 0; t < PACES_MAX_TERMS; ++t) {
+    H.indptr[t] = gen(d->indptr[t]);
+    H.indices[t] = gen(d->indices[t]);
+    H.data[t] = gen(d->data[t]);
+  }
+  typedef typename Vec2<F>::T V;
+  const V* xin = gen((PACES_GPTR(const V))(j == 0 ? d->x[2] : d->x[(j - 1) & 1]));
+  V* xout = gen((PACES_GPTR(V))d->x[j & 1]);
+  btay_term_body<F, BS, RPT>(gen((PACES_GPTR(const F))d->diag), H, xin, xout,
+                             gen((PACES_GPTR(double2))d->res), (F)d->c[j], gen(d->seg_off),
+                             ser_boff(d), ntraj, gen(d->ts), gen(d->hdr), ser_part(d, ntraj));
 }
 
 size_t btay_ws_bytes(int64_t R, int ntraj) {
@@ -671,6 +799,42 @@ int paces_batch_taylor_begin_tuned(int64_t R, const double* v, double* res, void
 #undef BB_LAUNCH
   hipLaunchKernelGGL(k_btay_reduce, dim3(ntraj), dim3(64), 0, st, (const u64*)part,
                      (const int64_t*)boff, ts, (paces_batch_taylor_hdr_t*)nullptr, 1, 0, 0.0);
+  return launch_status();
+}
+
+size_t paces_batch_series_desc_bytes(void) { return sizeof(paces_batch_series_t); }
+
+int paces_batch_series_launch(const paces_batch_series_t* desc, int64_t R_cap, int32_t ntraj,
+                              int32_t mixed, int32_t nterms, const paces_launch_cfg_t* cfg,
+                              void* stream) {
+  if (!desc || R_cap < 0 || ntraj < 1 || nterms < 0 || nterms > PACES_SERIES_MAX_TERMS ||
+      !cfg_ok(cfg))
+    return PACES_E_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  int B, RR;
+  cfg_get(cfg, &B, &RR);
+  const unsigned gs = seg_grid(R_cap, ntraj, B * RR);
+  hipLaunchKernelGGL(k_bser_segblocks, dim3(1), dim3(1), 0, st, desc, ntraj, B * RR);
+#define SB_LAUNCH(BB, QQ)                                                                      \
+  if (mixed)                                                                                   \
+    hipLaunchKernelGGL((k_bser_begin<float, BB, QQ>), dim3(gs), dim3(BB), 0, st, desc, ntraj); \
+  else                                                                                         \
+    hipLaunchKernelGGL((k_bser_begin<double, BB, QQ>), dim3(gs), dim3(BB), 0, st, desc, ntraj)
+  PACES_CFG_SWITCH(B, RR, SB_LAUNCH)
+#undef SB_LAUNCH
+  hipLaunchKernelGGL(k_bser_reduce, dim3(ntraj), dim3(64), 0, st, desc, ntraj, 1, 0);
+  for (int j = 0; j < nterms; ++j) {
+#define ST_LAUNCH(BB, QQ)                                                                      \
+  if (mixed)                                                                                   \
+    hipLaunchKernelGGL((k_bser_term<float, BB, QQ>), dim3(gs), dim3(BB), 0, st, desc, ntraj,   \
+                       j);                                                                     \
+  else                                                                                         \
+    hipLaunchKernelGGL((k_bser_term<double, BB, QQ>), dim3(gs), dim3(BB), 0, st, desc, ntraj,  \
+                       j)
+    PACES_CFG_SWITCH(B, RR, ST_LAUNCH)
+#undef ST_LAUNCH
+    hipLaunchKernelGGL(k_bser_reduce, dim3(ntraj), dim3(64), 0, st, desc, ntraj, 0, j);
+  }
   return launch_status();
 }
 

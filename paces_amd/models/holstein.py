@@ -702,7 +702,7 @@ class BatchedTimeEvolution:
     """
 
     def __init__(self, hamobj: BatchedHamiltonian, maxstates, enlarge_steps=1,
-                 precision="mixed", m_star=100, taylor_tol=None, autotune=False):
+                 precision="mixed", m_star=100, taylor_tol=None, autotune=False, graph=False):
         self.hamobj = hamobj
         self.device = hamobj.device
         ms = numpy.asarray(maxstates, dtype=numpy.int64)
@@ -720,6 +720,12 @@ class BatchedTimeEvolution:
         self._ws = hamobj._ws
         self._tws = _hip.Workspace(self.device)
         self.autotune = bool(autotune)
+        # graph=True: the Taylor series runs as one HIP graph per step (captured once per row
+        # capacity and term count, replayed with a rewritten device descriptor)
+        self.graph = bool(graph)
+        self._graphs = {}
+        self._desc_dev = None
+        self._graph_terms = 8 if precision == "mixed" else 16
         self.whoami = None
         self.vector = None
         self.seg_off = None
@@ -765,6 +771,54 @@ class BatchedTimeEvolution:
         self.seg_off = self.hamobj.segments(new_inds)
         return dbg
 
+    def _propagate_graph(self, delta_t, mats, mixed, res, bufs, ts, hdr, dptr, h64, h32, cfg, st):
+        """Terms 0 .. n-1 by replaying a captured graph; returns n (the eager loop continues
+        from there if some trajectory has not stopped)."""
+        h = self.hamobj
+        R = self.numstates
+        # capacity: R rounded up in sixteenths of its octave (at most 6.25% spare rows)
+        q = 1 << max(8, R.bit_length() - 4)
+        cap = ((R + q - 1) // q) * q
+        nterms = max(1, min(self.m_star, self._graph_terms, _hip.SERIES_MAX_TERMS))
+        wsp, wsn = self._tws.get(_lib.paces_batch_taylor_workspace_bytes(cap, h.ntraj))
+        d = _hip.BatchSeries()
+        d.R = R
+        d.diag = dptr
+        sset = mats
+        d.nterms_h = sset.nterms
+        for t in range(sset.nterms):
+            d.indptr[t] = sset.indptr[t]
+            d.indices[t] = sset.indices[t]
+            d.data[t] = sset.data[t]
+        d.v = ptr(torch.view_as_real(self.vector))
+        d.res = ptr(torch.view_as_real(res))
+        for k in range(3):
+            d.x[k] = ptr(torch.view_as_real(bufs[k]))
+        d.seg_off = ptr(self.seg_off)
+        d.ts = ptr(ts)
+        d.hdr = ptr(hdr)
+        d.ws = wsp.value
+        d.tol = self.tol
+        for j in range(_hip.SERIES_MAX_TERMS):
+            d.c[j] = engine.taylor_coeff(delta_t, j)
+        if self._desc_dev is None:
+            self._desc_dev = torch.empty(C.sizeof(_hip.BatchSeries), dtype=torch.uint8,
+                                         device=self.device)
+        self._desc_dev.copy_(torch.frombuffer(bytearray(bytes(d)), dtype=torch.uint8))
+        shape = (cfg.block, cfg.rows_per_thread) if cfg is not None else None
+        key = (cap, nterms, mixed, shape)
+        g = self._graphs.get(key)
+        if g is None:
+            g = torch.cuda.CUDAGraph()
+            cref = C.byref(cfg) if cfg is not None else None
+            with torch.cuda.graph(g):
+                check(_lib.paces_batch_series_launch(ptr(self._desc_dev), cap, h.ntraj,
+                                                     1 if mixed else 0, nterms, cref, stream()),
+                      "batch_series_launch")
+            self._graphs[key] = g
+        g.replay()
+        return nterms
+
     def propagate(self, delta_t):
         h = self.hamobj
         R = self.numstates
@@ -786,15 +840,23 @@ class BatchedTimeEvolution:
             h64, h32, dptr = C.byref(s64), None, ptr(self.diag_vals)
         cfg = self._launch_shape(R, mats, mixed, dptr, h64, h32, wsp, wsn, st)
         cref = C.byref(cfg) if cfg is not None else None
-        check(_lib.paces_batch_taylor_begin_tuned(R, ptr(torch.view_as_real(self.vector)),
-                                                  ptr(torch.view_as_real(res)),
-                                                  ptr(torch.view_as_real(bufs[2])),
-                                                  1 if mixed else 0, ptr(self.seg_off), h.ntraj,
-                                                  ptr(ts), wsp, wsn, cref, st),
-              "batch_taylor_begin")
-        x_in = bufs[2]
         j = 0
+        if self.graph:
+            j = self._propagate_graph(delta_t, mm.cset if mixed else s64, mixed, res, bufs, ts,
+                                      hdr, dptr, h64, h32, cfg, st)
+            wsp, wsn = self._tws.get(0)  # the graph's workspace (same buffer, grown if needed)
+            x_in = bufs[(j - 1) % 2]
+        else:
+            check(_lib.paces_batch_taylor_begin_tuned(R, ptr(torch.view_as_real(self.vector)),
+                                                      ptr(torch.view_as_real(res)),
+                                                      ptr(torch.view_as_real(bufs[2])),
+                                                      1 if mixed else 0, ptr(self.seg_off),
+                                                      h.ntraj, ptr(ts), wsp, wsn, cref, st),
+                  "batch_taylor_begin")
+            x_in = bufs[2]
         while j < self.m_star:
+            if self.graph and int(hdr[:4].view(torch.int32).item()) == 0:
+                break
             for _ in range(4):
                 if j >= self.m_star:
                     break
@@ -812,6 +874,9 @@ class BatchedTimeEvolution:
         info = numpy.frombuffer(ts.cpu().numpy().tobytes(), dtype=numpy.dtype(
             [("c1", "f8"), ("c1pc2", "f8"), ("ratio", "f8"), ("done", "i4"), ("j", "i4")]))
         self.last_expm = info
+        if self.graph and info.size:
+            # terms 0 .. j of the slowest trajectory; a longer series continues eagerly
+            self._graph_terms = int(info["j"].max()) + 1
         return info
 
     def _launch_shape(self, R, mats, mixed, dptr, h64, h32, wsp, wsn, st):

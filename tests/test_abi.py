@@ -53,3 +53,14 @@ def test_ctypes_binding_covers_header_and_host_only_calls():
     # argument validation runs on the host and returns a status, no device work
     assert lib.paces_lower_bound(None, -1, None, 0, 1, 1, None, None, None, None, 0, None) == -1
     assert lib.paces_merge_unique(None, 1, None, None, None, None, None, 0, None) == -1
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_struct_layouts_match_the_library():
+    """ctypes mirrors of structs the library reads from device memory have the C sizes."""
+    import ctypes as C
+    from paces_amd import _hip
+    lib = _hip.load()
+    assert lib.paces_batch_series_desc_bytes() == C.sizeof(_hip.BatchSeries)
+    assert C.sizeof(_hip.BatchTaylor) == 32
+    assert C.sizeof(_hip.TaylorState) == 64

@@ -82,3 +82,35 @@ def test_batched_trajectories_match_per_trajectory_oracle(precision):
             else:
                 assert err <= 1e-4, (i, t, err)
     print(f"[batched {precision}] max density error {worst:.3e}")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_series_equals_eager(precision):
+    """The series replayed from a HIP graph (captured once per row capacity and term count, the
+    device descriptor rewritten every step) gives the eager launches' state bit for bit at every
+    step, across growing bases (new captures) and repeated replays."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from paces_amd.config import devices
+    from paces_amd.models import holstein
+    if not devices.configure_called:
+        devices.configure()
+    draws = _draws(7, seed=5)
+    runs = []
+    for graph in (False, True):
+        bham = holstein.BatchedHamiltonian([d[0] for d in draws], 8, [-d[1] for d in draws],
+                                           [d[2] for d in draws])
+        bte = holstein.BatchedTimeEvolution(bham, [d[3] for d in draws], enlarge_steps=1,
+                                            precision=precision, graph=graph)
+        bte.create_single_exciton_states([d[0] // 2 for d in draws])
+        runs.append(bte)
+    for i in range(14):
+        for bte in runs:
+            bte.step(i, 0.05)
+        a, b = runs
+        assert torch.equal(a.whoami, b.whoami), i
+        assert torch.equal(a.vector, b.vector), i
+        if i > 0:
+            assert a.last_expm.tobytes() == b.last_expm.tobytes(), i
+    assert len(runs[1]._graphs) >= 2  # several capacities were captured, then replayed
