@@ -120,21 +120,29 @@ __global__ void k_bsel_init(const int32_t* __restrict__ seg_off, const int64_t* 
   }
 }
 
+// histogram passes take kSelTile rows per workgroup (8 per thread, strided for coalescing), so
+// each workgroup's LDS histogram is flushed to the trajectory's global one 8x less often
+constexpr int kSelRpt = 8;
+constexpr int kSelTile = kBlock * kSelRpt;
+
 __global__ void __launch_bounds__(kBlock) k_bsel_hist(const u64* __restrict__ keys,
                                                       const int32_t* __restrict__ seg_off,
                                                       const int64_t* __restrict__ boff, int ntraj,
                                                       const SegSel* __restrict__ st, int shift,
                                                       unsigned* __restrict__ hist) {
   __shared__ unsigned lh[256];
-  const SegTile T = seg_tile(seg_off, boff, ntraj, kTile);
+  const SegTile T = seg_tile(seg_off, boff, ntraj, kSelTile);
   if (T.t < 0 || st[T.t].all || st[T.t].krem == 0) return;
   lh[threadIdx.x] = 0;
   __syncthreads();
   const u64 prefix = st[T.t].prefix, mask = st[T.t].mask;
-  const int64_t i = T.r0 + threadIdx.x;
-  if (i < T.r1) {
-    u64 k = keys[i];
-    if ((k & mask) == prefix) atomicAdd(&lh[(k >> shift) & 255u], 1u);
+#pragma unroll
+  for (int q = 0; q < kSelRpt; ++q) {
+    const int64_t i = T.r0 + q * kBlock + threadIdx.x;
+    if (i < T.r1) {
+      const u64 k = keys[i];
+      if ((k & mask) == prefix) atomicAdd(&lh[(k >> shift) & 255u], 1u);
+    }
   }
   __syncthreads();
   unsigned c = lh[threadIdx.x];
@@ -216,7 +224,7 @@ __global__ void __launch_bounds__(kBlock) k_bsel_compact(const u64* __restrict__
 
 size_t bsel_ws_bytes(int64_t R, int ntraj) {
   return a256((size_t)ntraj * sizeof(SegSel)) + a256((size_t)ntraj * 256 * 4) +
-         a256((size_t)(ntraj + 1) * 8) + a256((size_t)R * 8) + 2 * a256((size_t)R) +
+         2 * a256((size_t)(ntraj + 1) * 8) + a256((size_t)R * 8) + 2 * a256((size_t)R) +
          2 * a256((size_t)(R + 1) * 4) + a256(scan_ws_bytes(R));
 }
 
@@ -231,6 +239,8 @@ static int run_bselect(const double* psi, const uint64_t* rows, int64_t R,
   p += a256((size_t)ntraj * 256 * 4);
   int64_t* boff = (int64_t*)p;
   p += a256((size_t)(ntraj + 1) * 8);
+  int64_t* boffh = (int64_t*)p;  // workgroup offsets of the histogram passes' larger tiles
+  p += a256((size_t)(ntraj + 1) * 8);
   u64* keys = (u64*)p;
   p += a256((size_t)R * 8);
   uint8_t* tie = (uint8_t*)p;
@@ -242,15 +252,17 @@ static int run_bselect(const double* psi, const uint64_t* rows, int64_t R,
   int32_t* KG = (int32_t*)p;
   p += a256((size_t)(R + 1) * 4);
   const unsigned gs = seg_grid(R, ntraj, kTile);
+  const unsigned gsh = seg_grid(R, ntraj, kSelTile);
   hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(1), 0, st, seg_off, ntraj, kTile, boff);
+  hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(1), 0, st, seg_off, ntraj, kSelTile, boffh);
   if (R > 0)
     hipLaunchKernelGGL(k_bsel_keys, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
                        (const double2*)psi, R, keys);
   hipLaunchKernelGGL(k_bsel_init, dim3(ntraj), dim3(256), 0, st, seg_off, kpt, ntraj, s, hist);
   for (int pass = 0; pass < 8; ++pass) {
     const int shift = 56 - 8 * pass;
-    hipLaunchKernelGGL(k_bsel_hist, dim3(gs), dim3(kBlock), 0, st, (const u64*)keys, seg_off,
-                       (const int64_t*)boff, ntraj, (const SegSel*)s, shift, hist);
+    hipLaunchKernelGGL(k_bsel_hist, dim3(gsh), dim3(kBlock), 0, st, (const u64*)keys, seg_off,
+                       (const int64_t*)boffh, ntraj, (const SegSel*)s, shift, hist);
     hipLaunchKernelGGL(k_bsel_pick, dim3(ntraj), dim3(256), 0, st, s, hist, shift);
   }
   hipLaunchKernelGGL(k_bsel_ties, dim3(gs), dim3(kBlock), 0, st, (const u64*)keys, seg_off,
