@@ -220,10 +220,16 @@ __global__ void __launch_bounds__(kBlock) k_mu_bounds(RunsArg R, int64_t* __rest
 
 // One workgroup per tile of one run: the tile's rows are sorted, so the rows of run s they can
 // land among form the narrow range the partition pass found; it is staged in LDS.
+// Each row x of run r gets its position in the merge of all runs with ties broken by run order:
+//   m(x) = own index + sum_{s < r} upper_bound_s(x) + sum_{s > r} lower_bound_s(x),
+// a permutation of 0 .. C-1 since every run is sorted unique. x is kept (its first occurrence)
+// when no earlier run holds it, and its kept flag is written at position m(x) of the merged
+// order; an exclusive scan of that array then gives every kept row's output index directly.
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, const int64_t* __restrict__ lohi,
-                                                    int32_t* __restrict__ lbs,
-                                                    uint8_t* __restrict__ kept) {
+                                                    int32_t* __restrict__ mpos,
+                                                    uint8_t* __restrict__ kept,
+                                                    uint8_t* __restrict__ merged_kept) {
   __shared__ LookupSmem<W> sm;
   const int r = run_of_block(R, blockIdx.x);
   const int64_t len_r = R.off[r + 1] - R.off[r];
@@ -235,21 +241,30 @@ __global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, const int64_t* __
   for (int k = 0; k < W; ++k) x.w[k] = 0;
   if (v) x = load_row<W>(R.ptr[r], own);
   bool keep = true;
+  int64_t m = own;
   for (int s = 0; s < R.K; ++s) {
+    if (s == r) continue;
     const int64_t len = R.off[s + 1] - R.off[s];
-    int64_t lb = own;
-    if (s != r) {
-      const int64_t* p = lohi + ((int64_t)blockIdx.x * R.K + s) * 2;
-      lb = block_lower_bound_range<W>(x, v, R.ptr[s], p[0], p[1], sm);
-      if (v && s < r && lb < len && eq_row<W>(x, R.ptr[s] + lb * W)) keep = false;
+    const int64_t* p = lohi + ((int64_t)blockIdx.x * R.K + s) * 2;
+    const int64_t lb = block_lower_bound_range<W>(x, v, R.ptr[s], p[0], p[1], sm);
+    if (v && s < r && lb < len && eq_row<W>(x, R.ptr[s] + lb * W)) {
+      keep = false;
+      m += lb + 1;  // the equal row of the earlier run comes first
+    } else {
+      m += lb;
     }
-    if (v) lbs[g * R.K + s] = (int32_t)lb;
   }
-  if (v) kept[g] = keep ? 1 : 0;
+  if (v) {
+    mpos[g] = (int32_t)m;
+    kept[g] = keep ? 1 : 0;
+    merged_kept[m] = keep ? 1 : 0;
+  }
 }
 
+// output index of every input row from the scan of the kept flags in merged order: a kept row's
+// own entry; a repeat's kept copy sits just before it in merged order (index - 1)
 template <int W>
-__global__ void __launch_bounds__(kBlock) k_mu_scatter(RunsArg R, const int32_t* __restrict__ lbs,
+__global__ void __launch_bounds__(kBlock) k_mu_scatter(RunsArg R, const int32_t* __restrict__ mpos,
                                                        const uint8_t* __restrict__ kept,
                                                        const int32_t* __restrict__ G,
                                                        u64* __restrict__ out,
@@ -260,13 +275,10 @@ __global__ void __launch_bounds__(kBlock) k_mu_scatter(RunsArg R, const int32_t*
   int64_t C = R.off[R.K];
   if (g == 0) *out_count = G[C];
   if (g >= C) return;
-  int64_t pos = 0;
-  for (int s = 0; s < R.K; ++s) {
-    int64_t o = R.off[s];
-    pos += (int64_t)G[o + lbs[g * R.K + s]] - (int64_t)G[o];
-  }
-  if (cand_pos) cand_pos[g] = (int32_t)pos;
-  if (kept[g]) {
+  const bool k = kept[g] != 0;
+  const int32_t pos = G[mpos[g]] - (k ? 0 : 1);
+  if (cand_pos) cand_pos[g] = pos;
+  if (k) {
     int r = run_of(R, g);
     Row<W> x = load_row<W>(R.ptr[r], g - R.off[r]);
     store_row<W>(out, pos, x);
@@ -276,13 +288,21 @@ __global__ void __launch_bounds__(kBlock) k_mu_scatter(RunsArg R, const int32_t*
 
 __global__ void k_write_zero(int32_t* p) { *p = 0; }
 
+// merged-order flags start at 0, so a position no row lands on (possible only for runs that
+// repeat a row) adds nothing and every output index stays below C
+__global__ void __launch_bounds__(kBlock) k_zero_u8(uint8_t* __restrict__ p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t mu_ws_bytes(int64_t C, int K) {
   // the partition table holds 2K int64 per tile; runs add at most K partial tiles
   int64_t ntiles = (C + kBlock - 1) / kBlock + K;
-  return align256((size_t)C * K * 4) + align256((size_t)C) + align256((size_t)(C + 1) * 4) +
-         align256((size_t)ntiles * K * 2 * 8) + align256(scan_ws_bytes(C));
+  return align256((size_t)C * 4) + align256((size_t)C) + align256((size_t)C) +
+         align256((size_t)(C + 1) * 4) + align256((size_t)ntiles * K * 2 * 8) +
+         align256(scan_ws_bytes(C));
 }
 
 template <int W>
@@ -295,25 +315,28 @@ static int run_mu(const RunsArg& R, uint64_t* out, int32_t* out_count, uint8_t* 
     return launch_status();
   }
   char* p = (char*)ws;
-  int32_t* lbs = (int32_t*)p;
-  p += align256((size_t)C * R.K * 4);
+  int32_t* mpos = (int32_t*)p;
+  p += align256((size_t)C * 4);
   uint8_t* kept = (uint8_t*)p;
+  p += align256((size_t)C);
+  uint8_t* mkept = (uint8_t*)p;
   p += align256((size_t)C);
   int32_t* G = (int32_t*)p;
   p += align256((size_t)(C + 1) * 4);
   int64_t* lohi = (int64_t*)p;
   p += align256((size_t)((C + kBlock - 1) / kBlock + R.K) * R.K * 2 * 8);
   const int64_t nblk = R.boff[R.K];
+  hipLaunchKernelGGL(k_zero_u8, dim3(grid_for(C, kBlock)), dim3(kBlock), 0, st, mkept, C);
   hipLaunchKernelGGL(k_mu_bounds<W>, dim3(grid_for(nblk * R.K * 2, kBlock)), dim3(kBlock), 0, st,
                      R, lohi);
   hipLaunchKernelGGL(k_mu_rank<W>, dim3((unsigned)nblk), dim3(kBlock), 0, st, R,
-                     (const int64_t*)lohi, lbs, kept);
+                     (const int64_t*)lohi, mpos, kept, mkept);
   int rc = launch_status();
   if (rc) return rc;
-  rc = scan_u8(kept, C, G, p, scan_ws_bytes(C), st);
+  rc = scan_u8(mkept, C, G, p, scan_ws_bytes(C), st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_mu_scatter<W>, dim3(grid_for(C, kBlock)), dim3(kBlock), 0, st, R,
-                     (const int32_t*)lbs, (const uint8_t*)kept, (const int32_t*)G, (u64*)out,
+                     (const int32_t*)mpos, (const uint8_t*)kept, (const int32_t*)G, (u64*)out,
                      from0, cand_pos, out_count);
   return launch_status();
 }

@@ -40,16 +40,49 @@ __device__ __forceinline__ int block_excl_scan(int v, int* tot) {
   return off + inc - v;
 }
 
+// a thread's kItems consecutive inputs: one 8-byte load (u8) or two 16-byte loads (int) when the
+// run is whole and aligned, else element by element
+__device__ __forceinline__ void load_items(const uint8_t* __restrict__ in, int64_t base, int64_t n,
+                                           int (&v)[kItems]) {
+  const uint8_t* p = in + base;
+  if (base + kItems <= n && (((uintptr_t)p) & 7) == 0) {
+    const uint64_t w = *(const uint64_t*)p;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) v[k] = (int)((w >> (8 * k)) & 255u);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) v[k] = (base + k < n) ? (int)p[k] : 0;
+  }
+}
+
+__device__ __forceinline__ void load_items(const int* __restrict__ in, int64_t base, int64_t n,
+                                           int (&v)[kItems]) {
+  const int* p = in + base;
+  if (base + kItems <= n && (((uintptr_t)p) & 15) == 0) {
+    const int4 a = *(const int4*)p, b = *(const int4*)(p + 4);
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = a.z;
+    v[3] = a.w;
+    v[4] = b.x;
+    v[5] = b.y;
+    v[6] = b.z;
+    v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) v[k] = (base + k < n) ? p[k] : 0;
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_tile_reduce(const T* __restrict__ in, int64_t n,
                                                         int* __restrict__ sums) {
   int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kItems;
+  int v[kItems];
+  load_items(in, base, n, v);
   int s = 0;
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    int64_t i = base + k;
-    s += (i < n) ? (int)in[i] : 0;
-  }
+  for (int k = 0; k < kItems; ++k) s += v[k];
   int tot;
   block_excl_scan(s, &tot);
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
@@ -62,20 +95,26 @@ __global__ void __launch_bounds__(kBlock) k_tile_scan(const T* __restrict__ in, 
                                                       int* __restrict__ out) {
   int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kItems;
   int v[kItems];
+  load_items(in, base, n, v);
   int s = 0;
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    int64_t i = base + k;
-    v[k] = (i < n) ? (int)in[i] : 0;
-    s += v[k];
-  }
+  for (int k = 0; k < kItems; ++k) s += v[k];
   int tot;
   int run = block_excl_scan(s, &tot) + (tile_off ? tile_off[blockIdx.x] : 0);
+  int o[kItems];
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
-    int64_t i = base + k;
-    if (i < n) out[i] = run;
+    o[k] = run;
     run += v[k];
+  }
+  int* q = out + base;
+  if (base + kItems <= n && (((uintptr_t)q) & 15) == 0) {
+    *(int4*)q = make_int4(o[0], o[1], o[2], o[3]);
+    *(int4*)(q + 4) = make_int4(o[4], o[5], o[6], o[7]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+      if (base + k < n) q[k] = o[k];
   }
   // the thread owning element n-1 (or thread 0 of an empty range) writes the total
   if (n == 0) {
