@@ -153,16 +153,18 @@ def btay_term_bytes(R, mats, mixed):
     return f * R + 4 * (R + 1) * len(mats) + (4 + f) * nnz + 4 * f * R + 32 * R
 
 
-# HBM bytes per row of one k_btay_term<float> launch, from FETCH_SIZE and WRITE_SIZE counter
-# passes over the roofline loop below, each scaled by the counter's reading for a 1 GiB device
-# copy (profiles/k_btay_term_traj256_hbm_traffic.txt). Only the mixed-precision kernel is measured.
-BTAY_TRAFFIC_BYTES_PER_ROW = {"mixed": 101.15}
+# HBM bytes per row of one k_btay_term<float, 512, 4> launch (the autotuned shape), from
+# FETCH_SIZE and WRITE_SIZE counter passes over the roofline loop below, each scaled by the
+# counter's reading for a 1 GiB device copy (profiles/k_btay_term_traj256_hbm_traffic.txt). Only
+# the mixed-precision kernel is measured.
+BTAY_TRAFFIC_BYTES_PER_ROW = {"mixed": 101.56}
 
 
 def measure_batched_term(bte, reps=20):
-    """Average duration of paces_batch_taylor_term (k_btay_term + its per-trajectory reduction)
-    at the current basis with the stopping test disabled (tol < 0), so every launch processes
-    every row; HIP events on the stream the kernels run on."""
+    """Average duration of paces_batch_taylor_term_tuned (k_btay_term + its per-trajectory
+    reduction) at the current basis, in the launch shape the timed steps used, with the stopping
+    test disabled (tol < 0) so every launch processes every row; HIP events on the stream the
+    kernels run on."""
     import ctypes as C
     import torch
     from paces_amd import _hip
@@ -181,11 +183,14 @@ def measure_batched_term(bte, reps=20):
     wsp, wsn = ws.get(lib.paces_batch_taylor_workspace_bytes(R, h.ntraj))
     st = torch.cuda.current_stream()
     sp = C.c_void_p(st.cuda_stream)
+    shape = bte.last_launch_shape
+    cfg = _hip.LaunchCfg(*shape) if shape else None
+    cref = C.byref(cfg) if cfg is not None else None
     vr = _hip.ptr(torch.view_as_real(bte.vector))
-    _hip.check(lib.paces_batch_taylor_begin(R, vr, _hip.ptr(torch.view_as_real(res)),
-                                            _hip.ptr(torch.view_as_real(bufs[2])),
-                                            1 if mixed else 0, _hip.ptr(bte.seg_off), h.ntraj,
-                                            _hip.ptr(ts), wsp, wsn, sp))
+    _hip.check(lib.paces_batch_taylor_begin_tuned(R, vr, _hip.ptr(torch.view_as_real(res)),
+                                                  _hip.ptr(torch.view_as_real(bufs[2])),
+                                                  1 if mixed else 0, _hip.ptr(bte.seg_off),
+                                                  h.ntraj, _hip.ptr(ts), wsp, wsn, cref, sp))
     if mixed:
         mm = engine.MixedMats(bte.diag_vals, mats)
         h64, h32, dptr = None, C.byref(mm.cset), _hip.ptr(mm.diag)
@@ -198,15 +203,15 @@ def measure_batched_term(bte, reps=20):
         if j == 2:
             e0.record(st)
         x_out = bufs[j % 2]
-        _hip.check(lib.paces_batch_taylor_term(
+        _hip.check(lib.paces_batch_taylor_term_tuned(
             R, dptr, h64, h32, _hip.ptr(torch.view_as_real(x_in)),
             _hip.ptr(torch.view_as_real(x_out)), _hip.ptr(torch.view_as_real(res)),
             engine.taylor_coeff(TRAJ_DT, j), j, -1.0, _hip.ptr(bte.seg_off), h.ntraj,
-            _hip.ptr(ts), _hip.ptr(hdr), wsp, wsn, sp))
+            _hip.ptr(ts), _hip.ptr(hdr), wsp, wsn, cref, sp))
         x_in = x_out
     e1.record(st)
     e1.synchronize()
-    return e0.elapsed_time(e1) / reps * 1e-3, btay_term_bytes(R, mats, mixed), reps
+    return e0.elapsed_time(e1) / reps * 1e-3, btay_term_bytes(R, mats, mixed), reps, shape
 
 
 def batched_leg(draws, precision, warmup, steps, label, graph=False, autotune=False):
@@ -327,14 +332,16 @@ def run_traj256(a):
                                       graph=a.graph, autotune=a.autotune)
     rows = bte.numstates
     value = units / elapsed
-    sec, nbytes, reps = measure_batched_term(bte)
+    sec, nbytes, reps, shape = measure_batched_term(bte)
+    kshape = "%d, %d" % shape if shape else "256, 1"
     achieved = nbytes / sec / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
             "frac": round(achieved / 8000.0, 4),
             "traffic": (int(BTAY_TRAFFIC_BYTES_PER_ROW[prec] * rows)
                         if prec in BTAY_TRAFFIC_BYTES_PER_ROW else None),
-            "kernel": "k_btay_term<%s> (paces_batch_taylor_term, stopping test off)"
-                      % ("float" if prec == "mixed" else "double"),
+            "kernel": "k_btay_term<%s, %s> (paces_batch_taylor_term_tuned, the timed steps' "
+                      "launch shape, stopping test off)" % ("float" if prec == "mixed" else "double",
+                                                            kshape),
             "avg_us": round(sec * 1e6, 2), "bytes_per_launch": int(nbytes), "rows": int(rows),
             "launches_timed": reps}
     n_exc_b = bte.observables()[0]

@@ -726,6 +726,7 @@ class BatchedTimeEvolution:
         self._graphs = {}
         self._desc_dev = None
         self._graph_terms = 8 if precision == "mixed" else 16
+        self.last_launch_shape = None
         self.whoami = None
         self.vector = None
         self.seg_off = None
@@ -839,6 +840,7 @@ class BatchedTimeEvolution:
             s64 = engine.csr_set(mats)
             h64, h32, dptr = C.byref(s64), None, ptr(self.diag_vals)
         cfg = self._launch_shape(R, mats, mixed, dptr, h64, h32, wsp, wsn, st)
+        self.last_launch_shape = (cfg.block, cfg.rows_per_thread) if cfg is not None else None
         cref = C.byref(cfg) if cfg is not None else None
         j = 0
         if self.graph:
