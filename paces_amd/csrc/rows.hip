@@ -225,34 +225,89 @@ __global__ void __launch_bounds__(kBlock) k_mu_bounds(RunsArg R, int64_t* __rest
 // a permutation of 0 .. C-1 since every run is sorted unique. x is kept (its first occurrence)
 // when no earlier run holds it, and its kept flag is written at position m(x) of the merged
 // order; an exclusive scan of that array then gives every kept row's output index directly.
+constexpr int kMuWords = 3264;  // 25.5 KiB (6 workgroups per CU): several runs' ranges at once
+
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, const int64_t* __restrict__ lohi,
                                                     int32_t* __restrict__ mpos,
                                                     uint8_t* __restrict__ kept,
                                                     uint8_t* __restrict__ merged_kept) {
-  __shared__ LookupSmem<W> sm;
+  // one LDS buffer: several runs' ranges staged together, or (for a range larger than the
+  // buffer) the fenced single-run search's table
+  __shared__ u64 buf[kMuWords];
+  static_assert(sizeof(LookupSmem<W>) <= sizeof(u64) * kMuWords, "LDS buffer");
+  LookupSmem<W>& sm = *reinterpret_cast<LookupSmem<W>*>(buf);
   const int r = run_of_block(R, blockIdx.x);
   const int64_t len_r = R.off[r + 1] - R.off[r];
   const int64_t own = ((int64_t)blockIdx.x - R.boff[r]) * kBlock + threadIdx.x;
   const bool v = own < len_r;
   const int64_t g = R.off[r] + own;
+  const int64_t* bl = lohi + (int64_t)blockIdx.x * R.K * 2;
   Row<W> x;
 #pragma unroll
   for (int k = 0; k < W; ++k) x.w[k] = 0;
   if (v) x = load_row<W>(R.ptr[r], own);
   bool keep = true;
   int64_t m = own;
-  for (int s = 0; s < R.K; ++s) {
-    if (s == r) continue;
-    const int64_t len = R.off[s + 1] - R.off[s];
-    const int64_t* p = lohi + ((int64_t)blockIdx.x * R.K + s) * 2;
-    const int64_t lb = block_lower_bound_range<W>(x, v, R.ptr[s], p[0], p[1], sm);
-    if (v && s < r && lb < len && eq_row<W>(x, R.ptr[s] + lb * W)) {
-      keep = false;
-      m += lb + 1;  // the equal row of the earlier run comes first
-    } else {
-      m += lb;
+  int s = 0;
+  while (s < R.K) {
+    if (s == r) {
+      ++s;
+      continue;
     }
+    // the group [s, e): consecutive runs (skipping r) whose ranges fit the buffer together
+    int e = s;
+    int64_t words = 0;
+    while (e < R.K) {
+      if (e == r) {
+        ++e;
+        continue;
+      }
+      const int64_t w = (bl[2 * e + 1] - bl[2 * e]) * W;
+      if (words + w > kMuWords) break;
+      words += w;
+      ++e;
+    }
+    if (e == s) {  // one range larger than the buffer: the fenced search
+      const int64_t len = R.off[s + 1] - R.off[s];
+      const int64_t lb = block_lower_bound_range<W>(x, v, R.ptr[s], bl[2 * s], bl[2 * s + 1], sm);
+      if (v && s < r && lb < len && eq_row<W>(x, R.ptr[s] + lb * W)) {
+        keep = false;
+        m += lb + 1;
+      } else {
+        m += lb;
+      }
+      ++s;
+      continue;
+    }
+    // stage the group's ranges back to back
+    int64_t base = 0;
+    for (int q = s; q < e; ++q) {
+      if (q == r) continue;
+      const int64_t lo = bl[2 * q], n = (bl[2 * q + 1] - lo) * W;
+      const u64* src = R.ptr[q] + lo * W;
+      for (int64_t i = threadIdx.x; i < n; i += blockDim.x) buf[base + i] = src[i];
+      base += n;
+    }
+    __syncthreads();
+    if (v) {
+      base = 0;
+      for (int q = s; q < e; ++q) {
+        if (q == r) continue;
+        const int64_t lo = bl[2 * q], span = bl[2 * q + 1] - lo;
+        const int64_t lb = lo + lower_bound_lds<W>(buf + base, span, x);
+        const int64_t len = R.off[q + 1] - R.off[q];
+        if (q < r && lb < len && eq_row<W>(x, R.ptr[q] + lb * W)) {
+          keep = false;
+          m += lb + 1;  // the equal row of the earlier run comes first
+        } else {
+          m += lb;
+        }
+        base += span * W;
+      }
+    }
+    __syncthreads();
+    s = e;
   }
   if (v) {
     mpos[g] = (int32_t)m;
