@@ -596,7 +596,6 @@ __global__ void __launch_bounds__(BS) k_btay_term(
 // The descriptor as the kernels read it: the same layout as paces_batch_series_t, with every
 // pointer declared in the global address space, so loads through them are global (not flat)
 // accesses, as for kernel-argument pointers.
-#define PACES_GPTR(T) __attribute__((address_space(1))) T*
 struct SeriesDev {
   int64_t R;
   PACES_GPTR(const void) diag;

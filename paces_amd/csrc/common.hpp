@@ -12,6 +12,12 @@
 
 typedef unsigned long long u64;
 
+// A pointer in the global address space: loads through it are global, not flat, accesses (a
+// flat load also counts against the LDS wait counter, so later LDS reads would wait for it).
+#define PACES_GPTR(T) __attribute__((address_space(1))) T*
+// A pointer into LDS: 32-bit address arithmetic.
+#define PACES_LPTR(T) __attribute__((address_space(3))) T*
+
 #define PACES_MAX_W 16
 
 namespace paces {

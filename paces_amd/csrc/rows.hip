@@ -218,99 +218,189 @@ __global__ void __launch_bounds__(kBlock) k_mu_bounds(RunsArg R, int64_t* __rest
   lohi[e] = lower_bound_row<W>(R.ptr[s], 0, R.off[s + 1] - R.off[s], x);
 }
 
+// a < b over W words, word 0 most significant: the borrow out of the multiword a - b.
+template <int W>
+__device__ __forceinline__ bool lex_less(const u64 (&a)[W], const u64 (&b)[W]) {
+  unsigned long long borrow = 0;
+#pragma unroll
+  for (int k = W - 1; k >= 0; --k) (void)__builtin_subcll(a[k], b[k], borrow, &borrow);
+  return borrow != 0;
+}
+
+template <int W>
+__device__ __forceinline__ void load_lds_row(PACES_LPTR(const unsigned char) rows, unsigned i,
+                                             u64 (&w)[W]) {
+  const PACES_LPTR(const u64) p = (PACES_LPTR(const u64))(rows + __umul24(i, W * 8));
+#pragma unroll
+  for (int k = 0; k < W; ++k) w[k] = p[k];
+}
+
+// Bound of q in n sorted unique rows staged in LDS (byte address rows): the number of rows below
+// q, or with UPPER the number of rows at most q. Binary lifting with a trip count of
+// floor(log2 n) + 1 for every lane (n is uniform across the workgroup).
+template <int W, bool UPPER>
+__device__ __forceinline__ int bound_lds(PACES_LPTR(const unsigned char) rows, int n,
+                                         const Row<W>& q) {
+  int pos = 0;
+  for (int step = n > 0 ? (int)(1u << (31 - __builtin_clz((unsigned)n))) : 0; step > 0;
+       step >>= 1) {
+    const int probe = pos + step - 1;
+    const bool in = probe < n;
+    u64 w[W];
+    load_lds_row<W>(rows, (unsigned)(in ? probe : n - 1), w);
+    const bool pred = UPPER ? !lex_less<W>(q.w, w) : lex_less<W>(w, q.w);
+    if (in & pred) pos += step;
+  }
+  return pos;
+}
+
 // One workgroup per tile of one run: the tile's rows are sorted, so the rows of run s they can
-// land among form the narrow range the partition pass found; it is staged in LDS.
+// land among form the narrow range [lo, hi] the partition pass found (hi = lower bound of the
+// tile's last row); that range, with the row at hi, is staged in LDS.
 // Each row x of run r gets its position in the merge of all runs with ties broken by run order:
 //   m(x) = own index + sum_{s < r} upper_bound_s(x) + sum_{s > r} lower_bound_s(x),
 // a permutation of 0 .. C-1 since every run is sorted unique. x is kept (its first occurrence)
 // when no earlier run holds it, and its kept flag is written at position m(x) of the merged
 // order; an exclusive scan of that array then gives every kept row's output index directly.
-constexpr int kMuWords = 3264;  // 25.5 KiB (6 workgroups per CU): several runs' ranges at once
+// The other runs' ranges are packed into groups that fit the LDS buffer together; each group is
+// copied global -> LDS by direct loads (16 bytes per lane, no registers), then searched, within
+// one barrier pair. A range is copied as the 16-byte blocks that hold it: a block never crosses
+// a page, so the few bytes read beyond the range cannot fault.
+constexpr int kMuWords = 3136;  // 24.5 KiB: with the run table, 6 workgroups per CU
+
+struct MuTable {
+  PACES_GPTR(const unsigned char) blk[PACES_MAX_RUNS];  // first 16-byte block of each range
+  long long lohi[2 * PACES_MAX_RUNS];                   // this tile's partition row
+  int q[PACES_MAX_RUNS];       // slot -> run (every run but the tile's own, in order)
+  int lo[PACES_MAX_RUNS];      // first staged row (index in its run)
+  int n[PACES_MAX_RUNS];       // rows staged
+  int nblk[PACES_MAX_RUNS];    // 16-byte blocks copied
+  int base[PACES_MAX_RUNS];    // LDS word offset of the first block (even)
+  int row0[PACES_MAX_RUNS];    // LDS word offset of row lo
+  int gs[PACES_MAX_RUNS + 1];  // group g = slots [gs[g], gs[g + 1])
+  int ng;
+};
+
+template <int W>
+constexpr int mu_buf_words() {
+  return (int)(sizeof(LookupSmem<W>) + 7) / 8 > kMuWords ? (int)(sizeof(LookupSmem<W>) + 7) / 8
+                                                         : kMuWords;
+}
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, const int64_t* __restrict__ lohi,
                                                     int32_t* __restrict__ mpos,
                                                     uint8_t* __restrict__ kept,
                                                     uint8_t* __restrict__ merged_kept) {
-  // one LDS buffer: several runs' ranges staged together, or (for a range larger than the
-  // buffer) the fenced single-run search's table
-  __shared__ u64 buf[kMuWords];
-  static_assert(sizeof(LookupSmem<W>) <= sizeof(u64) * kMuWords, "LDS buffer");
+  // one LDS buffer: a group of ranges, or (for a range larger than kMuWords) the fenced
+  // single-run search's table
+  __shared__ __align__(16) u64 buf[mu_buf_words<W>()];
+  __shared__ MuTable T;
   LookupSmem<W>& sm = *reinterpret_cast<LookupSmem<W>*>(buf);
+  const PACES_LPTR(const unsigned char) lbuf = (PACES_LPTR(const unsigned char))buf;
   const int r = run_of_block(R, blockIdx.x);
+  const int K = R.K;
+  const int64_t* bl = lohi + (int64_t)blockIdx.x * K * 2;
+  if ((int)threadIdx.x < 2 * K) T.lohi[threadIdx.x] = bl[threadIdx.x];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int i = 0, g = 0, words = 0;
+    T.gs[0] = 0;
+    for (int s = 0; s < K; ++s) {
+      if (s == r) continue;
+      const int lo = (int)T.lohi[2 * s], hi = (int)T.lohi[2 * s + 1];
+      const int len = (int)(R.off[s + 1] - R.off[s]);
+      const int n = (hi < len ? hi + 1 : len) - lo;
+      const uintptr_t a0 = (uintptr_t)(R.ptr[s] + (int64_t)lo * W);
+      const uintptr_t b0 = a0 & ~(uintptr_t)15, b1 = (a0 + (uintptr_t)n * W * 8 + 15) & ~(uintptr_t)15;
+      const int nb = n > 0 ? (int)((b1 - b0) >> 4) : 0;
+      const int w = 2 * nb;
+      if (i > T.gs[g] && words + w > kMuWords) {
+        T.gs[++g] = i;
+        words = 0;
+      }
+      T.q[i] = s;
+      T.lo[i] = lo;
+      T.n[i] = n;
+      T.nblk[i] = nb;
+      T.blk[i] = (PACES_GPTR(const unsigned char))b0;
+      T.base[i] = words;
+      T.row0[i] = words + (int)((a0 - b0) >> 3);
+      words += w;
+      ++i;
+      if (w > kMuWords) {  // a range too large for the buffer stands alone
+        T.gs[++g] = i;
+        words = 0;
+      }
+    }
+    if (i > T.gs[g]) T.gs[++g] = i;
+    T.ng = g;
+  }
+  __syncthreads();
   const int64_t len_r = R.off[r + 1] - R.off[r];
-  const int64_t own = ((int64_t)blockIdx.x - R.boff[r]) * kBlock + threadIdx.x;
+  const int own = (int)(((int64_t)blockIdx.x - R.boff[r]) * kBlock) + (int)threadIdx.x;
   const bool v = own < len_r;
-  const int64_t g = R.off[r] + own;
-  const int64_t* bl = lohi + (int64_t)blockIdx.x * R.K * 2;
   Row<W> x;
 #pragma unroll
   for (int k = 0; k < W; ++k) x.w[k] = 0;
   if (v) x = load_row<W>(R.ptr[r], own);
   bool keep = true;
-  int64_t m = own;
-  int s = 0;
-  while (s < R.K) {
-    if (s == r) {
-      ++s;
-      continue;
-    }
-    // the group [s, e): consecutive runs (skipping r) whose ranges fit the buffer together
-    int e = s;
-    int64_t words = 0;
-    while (e < R.K) {
-      if (e == r) {
-        ++e;
-        continue;
-      }
-      const int64_t w = (bl[2 * e + 1] - bl[2 * e]) * W;
-      if (words + w > kMuWords) break;
-      words += w;
-      ++e;
-    }
-    if (e == s) {  // one range larger than the buffer: the fenced search
+  int m = own;
+  const int ng = T.ng;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int g = 0; g < ng; ++g) {
+    const int a = T.gs[g], b = T.gs[g + 1];
+    if (2 * T.nblk[a] > kMuWords) {  // the fenced search over the partition range
+      const int s = T.q[a];
       const int64_t len = R.off[s + 1] - R.off[s];
-      const int64_t lb = block_lower_bound_range<W>(x, v, R.ptr[s], bl[2 * s], bl[2 * s + 1], sm);
+      const int64_t lb =
+          block_lower_bound_range<W>(x, v, R.ptr[s], T.lohi[2 * s], T.lohi[2 * s + 1], sm);
       if (v && s < r && lb < len && eq_row<W>(x, R.ptr[s] + lb * W)) {
         keep = false;
-        m += lb + 1;
+        m += (int)lb + 1;  // the equal row of the earlier run comes first
       } else {
-        m += lb;
+        m += (int)lb;
       }
-      ++s;
       continue;
     }
-    // stage the group's ranges back to back
-    int64_t base = 0;
-    for (int q = s; q < e; ++q) {
-      if (q == r) continue;
-      const int64_t lo = bl[2 * q], n = (bl[2 * q + 1] - lo) * W;
-      const u64* src = R.ptr[q] + lo * W;
-      for (int64_t i = threadIdx.x; i < n; i += blockDim.x) buf[base + i] = src[i];
-      base += n;
+    // each wave copies 64 blocks (1 KiB) per load: lane l's block lands at the wave's base + 16 l
+    for (int k = a; k < b; ++k) {
+      const int nb = T.nblk[k];
+      PACES_GPTR(const unsigned char) src = T.blk[k];
+      const int base = T.base[k];
+      for (int c0 = wave * 64; c0 < nb; c0 += kBlock) {
+        if (c0 + lane < nb)
+          __builtin_amdgcn_global_load_lds((PACES_GPTR(const void))(src + (size_t)(c0 + lane) * 16),
+                                           (PACES_LPTR(void))(buf + base + 2 * c0), 16, 0, 0);
+      }
     }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (v) {
-      base = 0;
-      for (int q = s; q < e; ++q) {
-        if (q == r) continue;
-        const int64_t lo = bl[2 * q], span = bl[2 * q + 1] - lo;
-        const int64_t lb = lo + lower_bound_lds<W>(buf + base, span, x);
-        const int64_t len = R.off[q + 1] - R.off[q];
-        if (q < r && lb < len && eq_row<W>(x, R.ptr[q] + lb * W)) {
-          keep = false;
-          m += lb + 1;  // the equal row of the earlier run comes first
+      for (int k = a; k < b; ++k) {
+        const PACES_LPTR(const unsigned char) rows = lbuf + 8 * T.row0[k];
+        const int n = T.n[k];
+        if (T.q[k] < r) {  // earlier run: upper bound, and an equal row there drops x
+          const int pos = bound_lds<W, true>(rows, n, x);
+          if (pos > 0) {
+            u64 w[W];
+            load_lds_row<W>(rows, (unsigned)(pos - 1), w);
+            bool eq = true;
+#pragma unroll
+            for (int j = 0; j < W; ++j) eq = eq & (w[j] == x.w[j]);
+            keep = keep & !eq;
+          }
+          m += T.lo[k] + pos;
         } else {
-          m += lb;
+          m += T.lo[k] + bound_lds<W, false>(rows, n, x);
         }
-        base += span * W;
       }
     }
     __syncthreads();
-    s = e;
   }
   if (v) {
-    mpos[g] = (int32_t)m;
+    const int64_t g = R.off[r] + own;
+    mpos[g] = m;
     kept[g] = keep ? 1 : 0;
     merged_kept[m] = keep ? 1 : 0;
   }
