@@ -287,6 +287,10 @@ static int run_bselect(const double* psi, const uint64_t* rows, int64_t R,
 
 constexpr int kObsTileB = 256;
 
+// Partial sums of one tile (rows of one trajectory): sum of weights, weight at each exciton site
+// and weighted phonons at each site. The (site, row) work is spread over the whole workgroup:
+// with N <= kBlock sites, P = kBlock / N threads share each site, each taking every P-th row,
+// and their partials meet in LDS.
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__ rows,
                                                          const double2* __restrict__ psi,
@@ -299,11 +303,24 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
   __shared__ double sw[kObsTileB];
   __shared__ int spos[kObsTileB];
   __shared__ double swn[kBlock / 64];
+  __shared__ double rae[kBlock], rap[kBlock];
   const SegTile T = seg_tile(seg_off, boff, ntraj, kObsTileB);
   if (T.t < 0) return;
   const int S = 2 * N + 1;
   const int nr = (int)(T.r1 - T.r0);
-  for (int e = threadIdx.x; e < nr * W; e += blockDim.x) srow[e] = rows[T.r0 * W + e];
+  {
+    u64 tmp[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const int e = (int)threadIdx.x + j * kBlock;
+      tmp[j] = e < nr * W ? rows[T.r0 * W + e] : 0ull;
+    }
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const int e = (int)threadIdx.x + j * kBlock;
+      if (e < nr * W) srow[e] = tmp[j];
+    }
+  }
   __syncthreads();
   double wsum = 0.0;
   for (int r = threadIdx.x; r < nr; r += blockDim.x) {
@@ -319,28 +336,52 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
   wsum = wave_sum(wsum);
   if ((threadIdx.x & 63) == 0) swn[threadIdx.x >> 6] = wsum;
   __syncthreads();
-  for (int s = threadIdx.x; s < N; s += blockDim.x) {
-    const int fo = off[s + 2], b = width[s + 2];
-    const int k = fo >> 6, o = fo & 63;
-    const bool straddle = o + b > 64;
-    const u64 mask = (b >= 64) ? ~0ull : ((1ull << b) - 1ull);
-    double ae = 0.0, ap = 0.0;
-    for (int r = 0; r < nr; ++r) {
-      const double w = sw[r];
-      ae += (spos[r] == s) ? w : 0.0;
-      u64 n;
-      if (b == 0) {
-        n = 0;
-      } else if (straddle) {
-        const u64 hi = srow[r * W + k], lo = srow[r * W + k + 1];
-        n = ((hi << (o + b - 64)) | (lo >> (128 - o - b))) & mask;
-      } else {
-        n = (srow[r * W + k] >> (64 - o - b)) & mask;
+  const bool sliced = N <= kBlock;
+  const int P = sliced ? kBlock / N : 1;
+  const int s0 = sliced ? (int)threadIdx.x % N : (int)threadIdx.x;
+  const int slice = sliced ? (int)threadIdx.x / N : 0;
+  if (slice < P) {
+    for (int s = s0; s < N; s += kBlock) {
+      const int fo = off[s + 2], b = width[s + 2];
+      const int k = fo >> 6, o = fo & 63;
+      const bool straddle = o + b > 64;
+      const u64 mask = (b >= 64) ? ~0ull : ((1ull << b) - 1ull);
+      double ae = 0.0, ap = 0.0;
+      for (int r = slice; r < nr; r += P) {
+        const double w = sw[r];
+        ae += (spos[r] == s) ? w : 0.0;
+        u64 n;
+        if (b == 0) {
+          n = 0;
+        } else if (straddle) {
+          const u64 hi = srow[r * W + k], lo = srow[r * W + k + 1];
+          n = ((hi << (o + b - 64)) | (lo >> (128 - o - b))) & mask;
+        } else {
+          n = (srow[r * W + k] >> (64 - o - b)) & mask;
+        }
+        ap += w * (double)n;
       }
-      ap += w * (double)n;
+      if (sliced) {
+        rae[slice * N + s] = ae;
+        rap[slice * N + s] = ap;
+      } else {
+        part[(int64_t)blockIdx.x * S + s] = ae;
+        part[(int64_t)blockIdx.x * S + N + s] = ap;
+      }
     }
-    part[(int64_t)blockIdx.x * S + s] = ae;
-    part[(int64_t)blockIdx.x * S + N + s] = ap;
+  }
+  if (sliced) {
+    __syncthreads();
+    if ((int)threadIdx.x < N) {
+      const int s = threadIdx.x;
+      double ae = 0.0, ap = 0.0;
+      for (int q = 0; q < P; ++q) {
+        ae += rae[q * N + s];
+        ap += rap[q * N + s];
+      }
+      part[(int64_t)blockIdx.x * S + s] = ae;
+      part[(int64_t)blockIdx.x * S + N + s] = ap;
+    }
   }
   if (threadIdx.x == 0) {
     double a = 0.0;

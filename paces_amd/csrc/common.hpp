@@ -23,6 +23,7 @@ typedef unsigned long long u64;
 namespace paces {
 
 constexpr int kBlock = 256;
+constexpr int kScanTile = 2048;  // elements per workgroup of the device-wide scans
 
 // ---- status plumbing --------------------------------------------------------------------------
 inline int launch_status() {

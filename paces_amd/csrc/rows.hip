@@ -7,6 +7,9 @@ namespace paces {
 int scan_u8(const uint8_t* in, int64_t n, int32_t* out, void* ws, size_t ws_bytes, hipStream_t st);
 int scan_i32(const int32_t* in, int64_t n, int32_t* out, void* ws, size_t ws_bytes, hipStream_t st);
 size_t scan_ws_bytes(int64_t n);
+size_t scan_tile_offsets_ws_bytes(int64_t n);
+int scan_tile_offsets_u8(const uint8_t* in, int64_t n, int32_t* offs, void* ws, size_t ws_bytes,
+                         hipStream_t st);
 
 // ------------------------------------------------------------------------------------------------
 // A2-A5 (paces/core/hamiltonian.py:218-360, paces/models/_kernels/_holstein_kernels.py:96-179)
@@ -171,11 +174,11 @@ static int run_lb(const uint64_t* book, int64_t M, const uint64_t* q, int64_t Q,
 
 // ------------------------------------------------------------------------------------------------
 // A7 union of K sorted unique runs (paces/aux/helpers.py:63-75 on the concatenation of
-// paces/core/hamiltonian.py:434-452). Element x of run r goes to output index
-//   pos(x) = sum_s keptprefix_s[lb_s(x)]
-// where lb_s is its lower_bound in run s (its own index for s == r) and keptprefix_s counts the
-// rows of run s that are not duplicates of an earlier run. Rows of run r equal to a row of an
-// earlier run are dropped; the formula gives them the index of the kept copy.
+// paces/core/hamiltonian.py:434-452). Every element gets its position in the merge of all runs
+// (ties by run order); an element equal to one of an earlier run is dropped, and the output index
+// of a position is the number of kept elements before it. Dropped elements report the index of
+// their kept copy. Passes: partition bounds, merged positions (k_mu_rank), tile sums of the kept
+// flags, and the output in merged order (k_mu_emit).
 // ------------------------------------------------------------------------------------------------
 
 struct RunsArg {
@@ -260,8 +263,8 @@ __device__ __forceinline__ int bound_lds(PACES_LPTR(const unsigned char) rows, i
 // Each row x of run r gets its position in the merge of all runs with ties broken by run order:
 //   m(x) = own index + sum_{s < r} upper_bound_s(x) + sum_{s > r} lower_bound_s(x),
 // a permutation of 0 .. C-1 since every run is sorted unique. x is kept (its first occurrence)
-// when no earlier run holds it, and its kept flag is written at position m(x) of the merged
-// order; an exclusive scan of that array then gives every kept row's output index directly.
+// when no earlier run holds it; its kept flag and its candidate index are written at position
+// m(x) of the merged order, which the output pass then walks in order.
 // The other runs' ranges are packed into groups that fit the LDS buffer together; each group is
 // copied global -> LDS by direct loads (16 bytes per lane, no registers), then searched, within
 // one barrier pair. A range is copied as the 16-byte blocks that hold it: a block never crosses
@@ -289,9 +292,8 @@ constexpr int mu_buf_words() {
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, const int64_t* __restrict__ lohi,
-                                                    int32_t* __restrict__ mpos,
-                                                    uint8_t* __restrict__ kept,
-                                                    uint8_t* __restrict__ merged_kept) {
+                                                    uint8_t* __restrict__ merged_kept,
+                                                    int32_t* __restrict__ inv) {
   // one LDS buffer: a group of ranges, or (for a range larger than kMuWords) the fenced
   // single-run search's table
   __shared__ __align__(16) u64 buf[mu_buf_words<W>()];
@@ -399,55 +401,96 @@ __global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, const int64_t* __
     __syncthreads();
   }
   if (v) {
-    const int64_t g = R.off[r] + own;
-    mpos[g] = m;
-    kept[g] = keep ? 1 : 0;
     merged_kept[m] = keep ? 1 : 0;
+    inv[m] = (int32_t)(R.off[r] + own);
   }
 }
 
-// output index of every input row from the scan of the kept flags in merged order: a kept row's
-// own entry; a repeat's kept copy sits just before it in merged order (index - 1)
+// The union's output in merged order. Workgroup t takes merged positions [kScanTile t,
+// kScanTile (t + 1)); thread x takes positions kScanTile t + kBlock k + x for k < kSteps, so a
+// wave covers 64 consecutive positions at each step: the flag and index loads and the row stores
+// are contiguous. A position's exclusive prefix of kept flags is the tile's offset, plus the wave
+// totals before the wave's step, plus the kept lanes below it (mbcnt of the ballot). That prefix
+// is a kept row's output index; a repeat's kept copy sits just before it in merged order, so a
+// repeat's index is the prefix - 1.
 template <int W>
-__global__ void __launch_bounds__(kBlock) k_mu_scatter(RunsArg R, const int32_t* __restrict__ mpos,
-                                                       const uint8_t* __restrict__ kept,
-                                                       const int32_t* __restrict__ G,
-                                                       u64* __restrict__ out,
-                                                       uint8_t* __restrict__ from0,
-                                                       int32_t* __restrict__ cand_pos,
-                                                       int32_t* __restrict__ out_count) {
-  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t C = R.off[R.K];
-  if (g == 0) *out_count = G[C];
-  if (g >= C) return;
-  const bool k = kept[g] != 0;
-  const int32_t pos = G[mpos[g]] - (k ? 0 : 1);
-  if (cand_pos) cand_pos[g] = pos;
-  if (k) {
-    int r = run_of(R, g);
-    Row<W> x = load_row<W>(R.ptr[r], g - R.off[r]);
-    store_row<W>(out, pos, x);
-    if (from0) from0[pos] = (r == 0) ? 1 : 0;
+__global__ void __launch_bounds__(kBlock) k_mu_emit(RunsArg R, const uint8_t* __restrict__ mk,
+                                                    const int32_t* __restrict__ inv,
+                                                    const int32_t* __restrict__ offs,
+                                                    u64* __restrict__ out,
+                                                    uint8_t* __restrict__ from0,
+                                                    int32_t* __restrict__ cand_pos,
+                                                    int32_t* __restrict__ out_count) {
+  constexpr int kSteps = kScanTile / kBlock;
+  constexpr int kWaves = kBlock / 64;
+  __shared__ int wpre[kSteps * kWaves];
+  const int64_t C = R.off[R.K];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t t0 = (int64_t)blockIdx.x * kScanTile + threadIdx.x;
+  int f[kSteps], g[kSteps], pre[kSteps];
+#pragma unroll
+  for (int k = 0; k < kSteps; ++k) {
+    const int64_t m = t0 + k * kBlock;
+    f[k] = m < C ? (int)mk[m] : 0;
+    g[k] = m < C ? inv[m] : -1;
   }
+#pragma unroll
+  for (int k = 0; k < kSteps; ++k) {
+    const u64 b = __ballot(f[k] != 0);
+    pre[k] = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b >> 32),
+                                            __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
+    if (lane == 0) wpre[k * kWaves + wave] = __popcll(b);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {  // exclusive prefix of the wave totals, in position order
+    int run = 0;
+    for (int i = 0; i < kSteps * kWaves; ++i) {
+      const int c = wpre[i];
+      wpre[i] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  const int base = offs[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kSteps; ++k) {
+    if (g[k] < 0) continue;  // no candidate landed here (only for runs that repeat a row)
+    const int G = base + wpre[k * kWaves + wave] + pre[k];
+    const int pos = f[k] ? G : G - 1;
+    if (cand_pos) cand_pos[g[k]] = pos;
+    if (f[k]) {
+      const int r = run_of(R, g[k]);
+      const Row<W> x = load_row<W>(R.ptr[r], g[k] - R.off[r]);
+      store_row<W>(out, pos, x);
+      if (from0) from0[pos] = (r == 0) ? 1 : 0;
+    }
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *out_count = offs[gridDim.x];
 }
 
 __global__ void k_write_zero(int32_t* p) { *p = 0; }
 
-// merged-order flags start at 0, so a position no row lands on (possible only for runs that
-// repeat a row) adds nothing and every output index stays below C
-__global__ void __launch_bounds__(kBlock) k_zero_u8(uint8_t* __restrict__ p, int64_t n) {
+// merged-order flags start at 0 and indices at -1, so a position no row lands on (possible only
+// for runs that repeat a row) adds nothing and is skipped
+__global__ void __launch_bounds__(kBlock) k_mu_fill(uint8_t* __restrict__ mk,
+                                                    int32_t* __restrict__ inv, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
+  if (i < n) {
+    mk[i] = 0;
+    inv[i] = -1;
+  }
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+static int64_t mu_scan_tiles(int64_t C) { return C <= 0 ? 1 : (C + kScanTile - 1) / kScanTile; }
+
 size_t mu_ws_bytes(int64_t C, int K) {
   // the partition table holds 2K int64 per tile; runs add at most K partial tiles
   int64_t ntiles = (C + kBlock - 1) / kBlock + K;
-  return align256((size_t)C * 4) + align256((size_t)C) + align256((size_t)C) +
-         align256((size_t)(C + 1) * 4) + align256((size_t)ntiles * K * 2 * 8) +
-         align256(scan_ws_bytes(C));
+  return align256((size_t)C * 4) + align256((size_t)C) +
+         align256((size_t)(mu_scan_tiles(C) + 1) * 4) + align256((size_t)ntiles * K * 2 * 8) +
+         align256(scan_tile_offsets_ws_bytes(C));
 }
 
 template <int W>
@@ -460,28 +503,27 @@ static int run_mu(const RunsArg& R, uint64_t* out, int32_t* out_count, uint8_t* 
     return launch_status();
   }
   char* p = (char*)ws;
-  int32_t* mpos = (int32_t*)p;
+  int32_t* inv = (int32_t*)p;
   p += align256((size_t)C * 4);
-  uint8_t* kept = (uint8_t*)p;
-  p += align256((size_t)C);
   uint8_t* mkept = (uint8_t*)p;
   p += align256((size_t)C);
-  int32_t* G = (int32_t*)p;
-  p += align256((size_t)(C + 1) * 4);
+  const int64_t nst = mu_scan_tiles(C);
+  int32_t* offs = (int32_t*)p;
+  p += align256((size_t)(nst + 1) * 4);
   int64_t* lohi = (int64_t*)p;
   p += align256((size_t)((C + kBlock - 1) / kBlock + R.K) * R.K * 2 * 8);
   const int64_t nblk = R.boff[R.K];
-  hipLaunchKernelGGL(k_zero_u8, dim3(grid_for(C, kBlock)), dim3(kBlock), 0, st, mkept, C);
+  hipLaunchKernelGGL(k_mu_fill, dim3(grid_for(C, kBlock)), dim3(kBlock), 0, st, mkept, inv, C);
   hipLaunchKernelGGL(k_mu_bounds<W>, dim3(grid_for(nblk * R.K * 2, kBlock)), dim3(kBlock), 0, st,
                      R, lohi);
   hipLaunchKernelGGL(k_mu_rank<W>, dim3((unsigned)nblk), dim3(kBlock), 0, st, R,
-                     (const int64_t*)lohi, mpos, kept, mkept);
+                     (const int64_t*)lohi, mkept, inv);
   int rc = launch_status();
   if (rc) return rc;
-  rc = scan_u8(mkept, C, G, p, scan_ws_bytes(C), st);
+  rc = scan_tile_offsets_u8(mkept, C, offs, p, scan_tile_offsets_ws_bytes(C), st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_mu_scatter<W>, dim3(grid_for(C, kBlock)), dim3(kBlock), 0, st, R,
-                     (const int32_t*)mpos, (const uint8_t*)kept, (const int32_t*)G, (u64*)out,
+  hipLaunchKernelGGL(k_mu_emit<W>, dim3((unsigned)nst), dim3(kBlock), 0, st, R,
+                     (const uint8_t*)mkept, (const int32_t*)inv, (const int32_t*)offs, (u64*)out,
                      from0, cand_pos, out_count);
   return launch_status();
 }

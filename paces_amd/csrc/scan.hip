@@ -10,6 +10,7 @@ namespace paces {
 
 constexpr int kItems = 8;
 constexpr int kTile = kBlock * kItems;  // 2048 elements per workgroup
+static_assert(kTile == kScanTile, "scan tile");
 
 __device__ __forceinline__ int wave_incl_scan(int v) {
   const int lane = threadIdx.x & 63;
@@ -175,6 +176,26 @@ int scan_i32(const int32_t* in, int64_t n, int32_t* out, void* ws, size_t ws_byt
 }
 
 size_t scan_ws_bytes(int64_t n) { return scan_ws_ints(n) * sizeof(int); }
+
+// The scan's first half for a caller that rescans the tiles itself: offs[t] = number of set
+// bytes before tile t (kScanTile elements each), for t in [0, tiles]; offs[tiles] is the total.
+size_t scan_tile_offsets_ws_bytes(int64_t n) {
+  const int64_t nt = n_tiles(n);
+  return ((size_t)nt + scan_ws_ints(nt)) * sizeof(int);
+}
+
+int scan_tile_offsets_u8(const uint8_t* in, int64_t n, int32_t* offs, void* ws, size_t ws_bytes,
+                         hipStream_t st) {
+  if (n < 0 || !offs || (n > 0 && !in)) return PACES_E_INVALID;
+  if (n >= 2147483647LL) return PACES_E_INVALID;
+  if (ws_bytes < scan_tile_offsets_ws_bytes(n)) return PACES_E_WORKSPACE;
+  const int64_t nt = n_tiles(n);
+  int* sums = (int*)ws;
+  hipLaunchKernelGGL(k_tile_reduce<uint8_t>, dim3((unsigned)nt), dim3(kBlock), 0, st, in, n, sums);
+  int rc = launch_status();
+  if (rc) return rc;
+  return scan_rec<int>(sums, nt, offs, sums + nt, st);
+}
 
 }  // namespace paces
 
