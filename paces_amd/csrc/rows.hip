@@ -8,8 +8,8 @@ int scan_u8(const uint8_t* in, int64_t n, int32_t* out, void* ws, size_t ws_byte
 int scan_i32(const int32_t* in, int64_t n, int32_t* out, void* ws, size_t ws_bytes, hipStream_t st);
 size_t scan_ws_bytes(int64_t n);
 size_t scan_tile_offsets_ws_bytes(int64_t n);
-int scan_tile_offsets_u8(const uint8_t* in, int64_t n, int32_t* offs, void* ws, size_t ws_bytes,
-                         hipStream_t st);
+int scan_tile_offsets_bit0(const uint32_t* in, int64_t n, int32_t* offs, void* ws, size_t ws_bytes,
+                           hipStream_t st);
 
 // ------------------------------------------------------------------------------------------------
 // A2-A5 (paces/core/hamiltonian.py:218-360, paces/models/_kernels/_holstein_kernels.py:96-179)
@@ -263,8 +263,8 @@ __device__ __forceinline__ int bound_lds(PACES_LPTR(const unsigned char) rows, i
 // Each row x of run r gets its position in the merge of all runs with ties broken by run order:
 //   m(x) = own index + sum_{s < r} upper_bound_s(x) + sum_{s > r} lower_bound_s(x),
 // a permutation of 0 .. C-1 since every run is sorted unique. x is kept (its first occurrence)
-// when no earlier run holds it; its kept flag and its candidate index are written at position
-// m(x) of the merged order, which the output pass then walks in order.
+// when no earlier run holds it. Position m(x) of the merged order receives one 32-bit code,
+// (candidate index << 1) | kept, which the output pass then walks in order.
 // The other runs' ranges are packed into groups that fit the LDS buffer together; each group is
 // copied global -> LDS by direct loads (16 bytes per lane, no registers), then searched, within
 // one barrier pair. A range is copied as the 16-byte blocks that hold it: a block never crosses
@@ -292,8 +292,7 @@ constexpr int mu_buf_words() {
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, const int64_t* __restrict__ lohi,
-                                                    uint8_t* __restrict__ merged_kept,
-                                                    int32_t* __restrict__ inv) {
+                                                    uint32_t* __restrict__ code) {
   // one LDS buffer: a group of ranges, or (for a range larger than kMuWords) the fenced
   // single-run search's table
   __shared__ __align__(16) u64 buf[mu_buf_words<W>()];
@@ -400,10 +399,7 @@ __global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, const int64_t* __
     }
     __syncthreads();
   }
-  if (v) {
-    merged_kept[m] = keep ? 1 : 0;
-    inv[m] = (int32_t)(R.off[r] + own);
-  }
+  if (v) code[m] = ((uint32_t)(R.off[r] + own) << 1) | (keep ? 1u : 0u);
 }
 
 // The union's output in merged order. Workgroup t takes merged positions [kScanTile t,
@@ -413,9 +409,10 @@ __global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, const int64_t* __
 // totals before the wave's step, plus the kept lanes below it (mbcnt of the ballot). That prefix
 // is a kept row's output index; a repeat's kept copy sits just before it in merged order, so a
 // repeat's index is the prefix - 1.
+constexpr uint32_t kMuNoCode = 0xFFFFFFFEu;  // flag 0, index 2^31 - 1: above every candidate
+
 template <int W>
-__global__ void __launch_bounds__(kBlock) k_mu_emit(RunsArg R, const uint8_t* __restrict__ mk,
-                                                    const int32_t* __restrict__ inv,
+__global__ void __launch_bounds__(kBlock) k_mu_emit(RunsArg R, const uint32_t* __restrict__ code,
                                                     const int32_t* __restrict__ offs,
                                                     u64* __restrict__ out,
                                                     uint8_t* __restrict__ from0,
@@ -431,8 +428,9 @@ __global__ void __launch_bounds__(kBlock) k_mu_emit(RunsArg R, const uint8_t* __
 #pragma unroll
   for (int k = 0; k < kSteps; ++k) {
     const int64_t m = t0 + k * kBlock;
-    f[k] = m < C ? (int)mk[m] : 0;
-    g[k] = m < C ? inv[m] : -1;
+    const uint32_t c = m < C ? code[m] : kMuNoCode;
+    f[k] = (int)(c & 1u);
+    g[k] = c == kMuNoCode ? -1 : (int)(c >> 1);
   }
 #pragma unroll
   for (int k = 0; k < kSteps; ++k) {
@@ -470,15 +468,11 @@ __global__ void __launch_bounds__(kBlock) k_mu_emit(RunsArg R, const uint8_t* __
 
 __global__ void k_write_zero(int32_t* p) { *p = 0; }
 
-// merged-order flags start at 0 and indices at -1, so a position no row lands on (possible only
-// for runs that repeat a row) adds nothing and is skipped
-__global__ void __launch_bounds__(kBlock) k_mu_fill(uint8_t* __restrict__ mk,
-                                                    int32_t* __restrict__ inv, int64_t n) {
+// merged-order codes start empty, so a position no row lands on (possible only for runs that
+// repeat a row) adds nothing and is skipped
+__global__ void __launch_bounds__(kBlock) k_mu_fill(uint32_t* __restrict__ code, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    mk[i] = 0;
-    inv[i] = -1;
-  }
+  if (i < n) code[i] = kMuNoCode;
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -488,8 +482,7 @@ static int64_t mu_scan_tiles(int64_t C) { return C <= 0 ? 1 : (C + kScanTile - 1
 size_t mu_ws_bytes(int64_t C, int K) {
   // the partition table holds 2K int64 per tile; runs add at most K partial tiles
   int64_t ntiles = (C + kBlock - 1) / kBlock + K;
-  return align256((size_t)C * 4) + align256((size_t)C) +
-         align256((size_t)(mu_scan_tiles(C) + 1) * 4) + align256((size_t)ntiles * K * 2 * 8) +
+  return align256((size_t)C * 4) + align256((size_t)(mu_scan_tiles(C) + 1) * 4) + align256((size_t)ntiles * K * 2 * 8) +
          align256(scan_tile_offsets_ws_bytes(C));
 }
 
@@ -503,27 +496,25 @@ static int run_mu(const RunsArg& R, uint64_t* out, int32_t* out_count, uint8_t* 
     return launch_status();
   }
   char* p = (char*)ws;
-  int32_t* inv = (int32_t*)p;
+  uint32_t* code = (uint32_t*)p;
   p += align256((size_t)C * 4);
-  uint8_t* mkept = (uint8_t*)p;
-  p += align256((size_t)C);
   const int64_t nst = mu_scan_tiles(C);
   int32_t* offs = (int32_t*)p;
   p += align256((size_t)(nst + 1) * 4);
   int64_t* lohi = (int64_t*)p;
   p += align256((size_t)((C + kBlock - 1) / kBlock + R.K) * R.K * 2 * 8);
   const int64_t nblk = R.boff[R.K];
-  hipLaunchKernelGGL(k_mu_fill, dim3(grid_for(C, kBlock)), dim3(kBlock), 0, st, mkept, inv, C);
+  hipLaunchKernelGGL(k_mu_fill, dim3(grid_for(C, kBlock)), dim3(kBlock), 0, st, code, C);
   hipLaunchKernelGGL(k_mu_bounds<W>, dim3(grid_for(nblk * R.K * 2, kBlock)), dim3(kBlock), 0, st,
                      R, lohi);
   hipLaunchKernelGGL(k_mu_rank<W>, dim3((unsigned)nblk), dim3(kBlock), 0, st, R,
-                     (const int64_t*)lohi, mkept, inv);
+                     (const int64_t*)lohi, code);
   int rc = launch_status();
   if (rc) return rc;
-  rc = scan_tile_offsets_u8(mkept, C, offs, p, scan_tile_offsets_ws_bytes(C), st);
+  rc = scan_tile_offsets_bit0(code, C, offs, p, scan_tile_offsets_ws_bytes(C), st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_mu_emit<W>, dim3((unsigned)nst), dim3(kBlock), 0, st, R,
-                     (const uint8_t*)mkept, (const int32_t*)inv, (const int32_t*)offs, (u64*)out,
+                     (const uint32_t*)code, (const int32_t*)offs, (u64*)out,
                      from0, cand_pos, out_count);
   return launch_status();
 }

@@ -75,6 +75,26 @@ __device__ __forceinline__ void load_items(const int* __restrict__ in, int64_t b
   }
 }
 
+// a thread's kItems consecutive codes reduced to their low bits (two 16-byte loads when aligned)
+__device__ __forceinline__ void load_items(const uint32_t* __restrict__ in, int64_t base, int64_t n,
+                                           int (&v)[kItems]) {
+  const uint32_t* p = in + base;
+  if (base + kItems <= n && (((uintptr_t)p) & 15) == 0) {
+    const uint4 a = *(const uint4*)p, b = *(const uint4*)(p + 4);
+    v[0] = (int)(a.x & 1u);
+    v[1] = (int)(a.y & 1u);
+    v[2] = (int)(a.z & 1u);
+    v[3] = (int)(a.w & 1u);
+    v[4] = (int)(b.x & 1u);
+    v[5] = (int)(b.y & 1u);
+    v[6] = (int)(b.z & 1u);
+    v[7] = (int)(b.w & 1u);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) v[k] = (base + k < n) ? (int)(p[k] & 1u) : 0;
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_tile_reduce(const T* __restrict__ in, int64_t n,
                                                         int* __restrict__ sums) {
@@ -177,21 +197,23 @@ int scan_i32(const int32_t* in, int64_t n, int32_t* out, void* ws, size_t ws_byt
 
 size_t scan_ws_bytes(int64_t n) { return scan_ws_ints(n) * sizeof(int); }
 
-// The scan's first half for a caller that rescans the tiles itself: offs[t] = number of set
-// bytes before tile t (kScanTile elements each), for t in [0, tiles]; offs[tiles] is the total.
+// The scan's first half for a caller that rescans the tiles itself: offs[t] = number of flags
+// set before tile t (kScanTile elements each), for t in [0, tiles]; offs[tiles] is the total.
 size_t scan_tile_offsets_ws_bytes(int64_t n) {
   const int64_t nt = n_tiles(n);
   return ((size_t)nt + scan_ws_ints(nt)) * sizeof(int);
 }
 
-int scan_tile_offsets_u8(const uint8_t* in, int64_t n, int32_t* offs, void* ws, size_t ws_bytes,
-                         hipStream_t st) {
+// (the low bit of each 32-bit code is its flag)
+int scan_tile_offsets_bit0(const uint32_t* in, int64_t n, int32_t* offs, void* ws, size_t ws_bytes,
+                           hipStream_t st) {
   if (n < 0 || !offs || (n > 0 && !in)) return PACES_E_INVALID;
   if (n >= 2147483647LL) return PACES_E_INVALID;
   if (ws_bytes < scan_tile_offsets_ws_bytes(n)) return PACES_E_WORKSPACE;
   const int64_t nt = n_tiles(n);
   int* sums = (int*)ws;
-  hipLaunchKernelGGL(k_tile_reduce<uint8_t>, dim3((unsigned)nt), dim3(kBlock), 0, st, in, n, sums);
+  hipLaunchKernelGGL(k_tile_reduce<uint32_t>, dim3((unsigned)nt), dim3(kBlock), 0, st, in, n,
+                     sums);
   int rc = launch_status();
   if (rc) return rc;
   return scan_rec<int>(sums, nt, offs, sums + nt, st);
