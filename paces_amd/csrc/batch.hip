@@ -285,12 +285,18 @@ static int run_bselect(const double* psi, const uint64_t* rows, int64_t R,
 
 // ---- per-trajectory observables (Holstein, batched layout: field 0 trajectory, 1 position) -------
 
-constexpr int kObsTileB = 256;
+constexpr int kObsTileB = 256;  // the smallest tile (sizes the workspace)
 
-// Partial sums of one tile (rows of one trajectory): sum of weights, weight at each exciton site
-// and weighted phonons at each site. The (site, row) work is spread over the whole workgroup:
-// with N <= kBlock sites, P = kBlock / N threads share each site, each taking every P-th row,
-// and their partials meet in LDS.
+template <int W>
+constexpr int obs_tile() {
+  return W <= 6 ? 512 : kObsTileB;
+}
+
+// Partial sums of one tile (rows of one trajectory): the sum of weights, the weight at each
+// exciton site and the weighted phonons at each site. Wave w takes sites w, w + 4, ..., so a
+// site's field offset and width are uniform across the wave; lane l holds rows l, l + 64, ...
+// with their weights and exciton positions in registers, and a site's sum ends in a wave
+// reduction. Only sites inside the tile's range of exciton positions can hold exciton weight.
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__ rows,
                                                          const double2* __restrict__ psi,
@@ -299,95 +305,111 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
                                                          int ntraj, const int32_t* __restrict__ off,
                                                          const int32_t* __restrict__ width, int N,
                                                          double* __restrict__ part) {
-  __shared__ u64 srow[kObsTileB * W];
-  __shared__ double sw[kObsTileB];
-  __shared__ int spos[kObsTileB];
-  __shared__ double swn[kBlock / 64];
-  __shared__ double rae[kBlock], rap[kBlock];
-  const SegTile T = seg_tile(seg_off, boff, ntraj, kObsTileB);
+  constexpr int TB = obs_tile<W>(), RPL = TB / 64, PER = TB * W / kBlock;
+  __shared__ u64 srow[TB * W];
+  __shared__ int spr[2];
+  const SegTile T = seg_tile(seg_off, boff, ntraj, TB);
   if (T.t < 0) return;
+  if (threadIdx.x == 0) {
+    spr[0] = 0x7fffffff;
+    spr[1] = -1;
+  }
   const int S = 2 * N + 1;
   const int nr = (int)(T.r1 - T.r0);
   {
-    u64 tmp[W];
+    u64 tmp[PER];
 #pragma unroll
-    for (int j = 0; j < W; ++j) {
+    for (int j = 0; j < PER; ++j) {
       const int e = (int)threadIdx.x + j * kBlock;
       tmp[j] = e < nr * W ? rows[T.r0 * W + e] : 0ull;
     }
 #pragma unroll
-    for (int j = 0; j < W; ++j) {
+    for (int j = 0; j < PER; ++j) {
       const int e = (int)threadIdx.x + j * kBlock;
       if (e < nr * W) srow[e] = tmp[j];
     }
   }
   __syncthreads();
-  double wsum = 0.0;
-  for (int r = threadIdx.x; r < nr; r += blockDim.x) {
-    double2 v = psi[T.r0 + r];
-    double w = v.x * v.x + v.y * v.y;
-    sw[r] = w;
-    wsum += w;
-    Row<W> rr;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double w[RPL];
+  int pos[RPL];
 #pragma unroll
-    for (int k = 0; k < W; ++k) rr.w[k] = srow[r * W + k];
-    spos[r] = (int)get_field<W>(rr, off[1], width[1]);
-  }
-  wsum = wave_sum(wsum);
-  if ((threadIdx.x & 63) == 0) swn[threadIdx.x >> 6] = wsum;
-  __syncthreads();
-  const bool sliced = N <= kBlock;
-  const int P = sliced ? kBlock / N : 1;
-  const int s0 = sliced ? (int)threadIdx.x % N : (int)threadIdx.x;
-  const int slice = sliced ? (int)threadIdx.x / N : 0;
-  if (slice < P) {
-    for (int s = s0; s < N; s += kBlock) {
-      const int fo = off[s + 2], b = width[s + 2];
-      const int k = fo >> 6, o = fo & 63;
-      const bool straddle = o + b > 64;
-      const u64 mask = (b >= 64) ? ~0ull : ((1ull << b) - 1ull);
-      double ae = 0.0, ap = 0.0;
-      for (int r = slice; r < nr; r += P) {
-        const double w = sw[r];
-        ae += (spos[r] == s) ? w : 0.0;
-        u64 n;
-        if (b == 0) {
-          n = 0;
-        } else if (straddle) {
-          const u64 hi = srow[r * W + k], lo = srow[r * W + k + 1];
-          n = ((hi << (o + b - 64)) | (lo >> (128 - o - b))) & mask;
-        } else {
-          n = (srow[r * W + k] >> (64 - o - b)) & mask;
-        }
-        ap += w * (double)n;
-      }
-      if (sliced) {
-        rae[slice * N + s] = ae;
-        rap[slice * N + s] = ap;
-      } else {
-        part[(int64_t)blockIdx.x * S + s] = ae;
-        part[(int64_t)blockIdx.x * S + N + s] = ap;
-      }
+  for (int j = 0; j < RPL; ++j) {
+    const int r = lane + 64 * j;
+    w[j] = 0.0;
+    pos[j] = -1;
+    if (r < nr) {
+      const double2 v = psi[T.r0 + r];
+      w[j] = v.x * v.x + v.y * v.y;
+      Row<W> rr;
+#pragma unroll
+      for (int k = 0; k < W; ++k) rr.w[k] = srow[r * W + k];
+      pos[j] = (int)get_field<W>(rr, off[1], width[1]);
     }
   }
-  if (sliced) {
-    __syncthreads();
-    if ((int)threadIdx.x < N) {
-      const int s = threadIdx.x;
-      double ae = 0.0, ap = 0.0;
-      for (int q = 0; q < P; ++q) {
-        ae += rae[q * N + s];
-        ap += rap[q * N + s];
+  {
+    int lo = 0x7fffffff, hi = -1;
+#pragma unroll
+    for (int j = 0; j < RPL; ++j) {
+      if (pos[j] >= 0) {
+        lo = pos[j] < lo ? pos[j] : lo;
+        hi = pos[j] > hi ? pos[j] : hi;
       }
+    }
+    if (hi >= 0) {
+      atomicMin(&spr[0], lo);
+      atomicMax(&spr[1], hi);
+    }
+  }
+  if (wave == 0) {
+    double a = 0.0;
+#pragma unroll
+    for (int j = 0; j < RPL; ++j) a += w[j];
+    a = wave_sum(a);
+    if (lane == 0) part[(int64_t)blockIdx.x * S + 2 * N] = a;
+  }
+  __syncthreads();
+  const int pmin = spr[0], pmax = spr[1];
+  for (int s = wave; s < N; s += kBlock / 64) {
+    const int fo = off[s + 2], b = width[s + 2];
+    const int k = fo >> 6, o = fo & 63;
+    double ap = 0.0;
+    if (b > 0 && o + b > 64) {  // the field straddles two words
+      const u64 mask = (b >= 64) ? ~0ull : ((1ull << b) - 1ull);
+#pragma unroll
+      for (int j = 0; j < RPL; ++j) {
+        const int r = lane + 64 * j;
+        const u64 n = ((srow[r * W + k] << (o + b - 64)) | (srow[r * W + k + 1] >> (128 - o - b))) &
+                      mask;
+        ap += w[j] * (double)n;
+      }
+    } else if (b > 0) {
+      const u64 mask = (b >= 64) ? ~0ull : ((1ull << b) - 1ull);
+      if (b <= 32) {
+#pragma unroll
+        for (int j = 0; j < RPL; ++j) {
+          const int r = lane + 64 * j;
+          ap += w[j] * (double)(uint32_t)((srow[r * W + k] >> (64 - o - b)) & mask);
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < RPL; ++j) {
+          const int r = lane + 64 * j;
+          ap += w[j] * (double)((srow[r * W + k] >> (64 - o - b)) & mask);
+        }
+      }
+    }
+    ap = wave_sum(ap);
+    double ae = 0.0;
+    if (s >= pmin && s <= pmax) {
+#pragma unroll
+      for (int j = 0; j < RPL; ++j) ae += (pos[j] == s) ? w[j] : 0.0;
+      ae = wave_sum(ae);
+    }
+    if (lane == 0) {
       part[(int64_t)blockIdx.x * S + s] = ae;
       part[(int64_t)blockIdx.x * S + N + s] = ap;
     }
-  }
-  if (threadIdx.x == 0) {
-    double a = 0.0;
-#pragma unroll
-    for (int q = 0; q < kBlock / 64; ++q) a += swn[q];
-    part[(int64_t)blockIdx.x * S + 2 * N] = a;
   }
 }
 
@@ -416,8 +438,9 @@ static int run_bobs(const uint64_t* rows, const double* psi, int64_t R, const in
   p += a256((size_t)(ntraj + 1) * 8);
   double* part = (double*)p;
   const int S = 2 * N + 1;
-  hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(1), 0, st, seg_off, ntraj, kObsTileB, boff);
-  hipLaunchKernelGGL(k_bobs_partial<W>, dim3(seg_grid(R, ntraj, kObsTileB)), dim3(kBlock), 0, st,
+  constexpr int TB = obs_tile<W>();
+  hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(1), 0, st, seg_off, ntraj, TB, boff);
+  hipLaunchKernelGGL(k_bobs_partial<W>, dim3(seg_grid(R, ntraj, TB)), dim3(kBlock), 0, st,
                      (const u64*)rows, (const double2*)psi, seg_off, (const int64_t*)boff, ntraj,
                      lay->d_off, lay->d_width, N, part);
   hipLaunchKernelGGL(k_bobs_final, dim3(ntraj * S), dim3(64), 0, st, (const double*)part,
