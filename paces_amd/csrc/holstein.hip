@@ -470,8 +470,10 @@ static int run_csr_fill(int64_t R, const OpTable& T, const ModelArg& M, int nter
 // basis row b under op k is the pair {b, e}; it is an entry (b, e) and its H.c. (e, b) of term
 // term[k], except for a minus op whose image is itself a basis row: that pair is emitted by the
 // plus op of the image (the o_star rule, hamiltonian.py:363-374; holstein.py:201-206, 266-268).
-// Rows are counted with atomics, filled through atomic cursors, then each row is sorted by column,
-// which gives the canonical coo_matrix(...).tocsr() layout (time_evolution.py:369-371).
+// Each row's entries come out in ascending column order, the canonical coo_matrix(...).tocsr()
+// layout (time_evolution.py:369-371): through fixed slots where the column order of a term's ops
+// is strict for every row (k_pairs_mark / k_pairs_place), else counted and filled through atomic
+// cursors and then sorted per row (k_pairs_count / k_pairs_fill / k_pairs_sort).
 // ------------------------------------------------------------------------------------------------
 
 struct ImgRuns {
@@ -503,7 +505,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs_count(int64_t R, int64_t nbasi
   atomicAdd(&cnt[t * R + p], 1);
 }
 
-// indptr of every term from the scan, and the fill cursors (absolute positions)
+// indptr of every term from the scan, and (when cursor is given) the fill cursors
 __global__ void __launch_bounds__(kBlock) k_pairs_indptr(int64_t R, int nterms,
                                                          const int32_t* __restrict__ G,
                                                          CsrOut out, int32_t* __restrict__ cursor) {
@@ -513,7 +515,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs_indptr(int64_t R, int nterms,
   const int64_t i = e - (int64_t)t * R;
   const int32_t base = G[e] - G[(int64_t)t * R];
   out.indptr[t][i] = base;
-  cursor[e] = base;
+  if (cursor) cursor[e] = base;
   if (i == R - 1) out.indptr[t][R] = G[(int64_t)(t + 1) * R] - G[(int64_t)t * R];
 }
 
@@ -569,9 +571,139 @@ __global__ void __launch_bounds__(kBlock) k_pairs_sort(int64_t R, int nterms, Cs
   }
 }
 
-static size_t pairs_ws(int64_t R, int nterms) {
+// Fixed slots instead of atomics and the sort, where the column order is known in advance. An
+// entry (row, neighbour) belongs to the op that maps row to neighbour: the image pair {b, e} of
+// op k is slot k of row q = pos(b) and slot k^1 (the inverse op: ops come in (+, -) pairs) of row
+// p = pos(e). Each (row, slot) is written by one pair at most, so a byte per slot marks the
+// occupied ones without atomics. A hop moves only the exciton position and a vib op only the
+// phonon count at the exciton site, so the column order of a row's neighbours is the order of
+// the ops' key changes: rank[k] is op k's place in that order within its term. An entry's
+// position is its row's start plus its occupied slots of lower rank; rows come out sorted.
+struct SlotArg {
+  int8_t k0[PACES_MAX_TERMS];     // first op of each term
+  int8_t S[PACES_MAX_TERMS];      // ops (slots) of each term
+  int8_t rank[kMaxOps];           // column rank of op k's neighbour among its term's slots
+  int32_t foff[PACES_MAX_TERMS];  // slots of the terms before t (flag offset = R * foff[t])
+};
+
+// the slot table, or false when some term's key changes do not give a strict column order for
+// every row (periodic batched lattices, whose spans differ by trajectory; ties on tiny rings)
+static bool build_slots(const OpTable& T, const paces_holstein_t* h, SlotArg* SL) {
+  if (h->batched && h->periodic) return false;
+  int k = 0, f = 0;
+  for (int t = 0; t < h->nterms; ++t) {
+    SL->k0[t] = (int8_t)k;
+    SL->foff[t] = f;
+    int S = 0;
+    while (k + S < T.n && T.term[k + S] == t) ++S;
+    if (S % 2 != 0 || S > 8) return false;
+    for (int j = 0; j < S; j += 2)
+      if (T.dir[k + j] != 1 || T.dir[k + j + 1] != -1) return false;
+    long long d[8];
+    for (int j = 0; j < S; ++j) {
+      const int q = k + j;
+      d[j] = T.kind[q] == PACES_TERM_HOP
+                 ? (long long)T.dir[q] * T.oe[q] - (T.wrap[q] ? (long long)T.dir[q] * T.span[q] : 0)
+                 : (long long)T.dir[q];
+    }
+    for (int j = 0; j < S; ++j) {
+      int r = 0;
+      for (int i = 0; i < S; ++i) {
+        if (i != j && d[i] == d[j]) return false;
+        r += d[i] < d[j] ? 1 : 0;
+      }
+      SL->rank[k + j] = (int8_t)r;
+    }
+    SL->S[t] = (int8_t)S;
+    k += S;
+    f += S;
+  }
+  return k == T.n;
+}
+
+__global__ void __launch_bounds__(kBlock) k_pairs_mark(int64_t R, int64_t nbasis, ImgRuns IR,
+                                                       OpTable T, SlotArg SL,
+                                                       const int32_t* __restrict__ cand_pos,
+                                                       const uint8_t* __restrict__ inB,
+                                                       const int32_t* __restrict__ src,
+                                                       uint8_t* __restrict__ flags) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= IR.off[IR.n]) return;
+  const int k = img_run_of(IR, e);
+  const int32_t p = cand_pos[nbasis + e];
+  const int32_t q = cand_pos[src[e]];
+  if (p < 0 || q < 0) return;
+  if (T.dir[k] < 0 && inB[p]) return;
+  const int t = T.term[k], S = SL.S[t];
+  uint8_t* f = flags + R * SL.foff[t];
+  f[(int64_t)q * S + (k - SL.k0[t])] = 1;
+  f[(int64_t)p * S + ((k ^ 1) - SL.k0[t])] = 1;
+}
+
+// entries of every (term, row): its occupied slots
+__global__ void __launch_bounds__(kBlock) k_pairs_rowcount(int64_t R, int nterms, SlotArg SL,
+                                                           const uint8_t* __restrict__ flags,
+                                                           int32_t* __restrict__ cnt) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nterms * R) return;
+  const int t = (int)(e / R);
+  const int64_t i = e - (int64_t)t * R;
+  const int S = SL.S[t];
+  const uint8_t* f = flags + R * SL.foff[t] + i * S;
+  int c = 0;
+  for (int j = 0; j < S; ++j) c += f[j];
+  cnt[e] = c;
+}
+
+__device__ __forceinline__ int32_t slot_place(const SlotArg& SL, int t, const uint8_t* f, int S,
+                                              int rk, int32_t start) {
+  int32_t a = start;
+  for (int j = 0; j < S; ++j) a += (SL.rank[SL.k0[t] + j] < rk) ? f[j] : 0;
+  return a;
+}
+
+__global__ void __launch_bounds__(kBlock) k_pairs_place(int64_t R, int64_t nbasis, ImgRuns IR,
+                                                        OpTable T, SlotArg SL, ModelArg M,
+                                                        const u64* __restrict__ basis, int W,
+                                                        int trajbits,
+                                                        const int32_t* __restrict__ cand_pos,
+                                                        const uint8_t* __restrict__ inB,
+                                                        const int32_t* __restrict__ src,
+                                                        const uint8_t* __restrict__ nup,
+                                                        const double* __restrict__ sqrt_tab,
+                                                        const uint8_t* __restrict__ flags,
+                                                        CsrOut out) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= IR.off[IR.n]) return;
+  const int k = img_run_of(IR, e);
+  const int32_t p = cand_pos[nbasis + e];
+  const int32_t q = cand_pos[src[e]];
+  if (p < 0 || q < 0) return;
+  if (T.dir[k] < 0 && inB[p]) return;
+  const int t = T.term[k], S = SL.S[t];
+  const int traj = M.batched ? (int)(basis[(int64_t)src[e] * W] >> (64 - trajbits)) : 0;
+  const double v = op_value(T, k, M, traj, sqrt_tab, nup[e]);
+  const uint8_t* f = flags + R * SL.foff[t];
+  const int32_t a =
+      slot_place(SL, t, f + (int64_t)q * S, S, SL.rank[k], out.indptr[t][q]);
+  out.indices[t][a] = p;
+  out.data[t][a] = v;
+  const int32_t b =
+      slot_place(SL, t, f + (int64_t)p * S, S, SL.rank[k ^ 1], out.indptr[t][p]);
+  out.indices[t][b] = q;
+  out.data[t][b] = v;
+}
+
+static size_t pairs_ws(int64_t R, int nterms, int nops) {
   int64_t m = (int64_t)nterms * R;
-  return a256((size_t)m * 4) + a256((size_t)(m + 1) * 4) + a256(scan_ws_bytes(m));
+  return a256((size_t)m * 4) + a256((size_t)(m + 1) * 4) + a256(scan_ws_bytes(m)) +
+         a256((size_t)R * nops);
+}
+
+// the slot flags sit after the count, prefix and scan areas of the workspace
+static uint8_t* pairs_flags(void* ws, int64_t R, int nterms) {
+  int64_t m = (int64_t)nterms * R;
+  return (uint8_t*)ws + a256((size_t)m * 4) + a256((size_t)(m + 1) * 4) + a256(scan_ws_bytes(m));
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -937,8 +1069,9 @@ static int img_runs_arg(const OpTable& T, const int64_t* run_offsets, ImgRuns* I
 }
 
 size_t paces_holstein_pairs_workspace_bytes(int64_t R, const paces_holstein_t* h) {
-  if (!h) return 0;
-  return pairs_ws(R, h->nterms);
+  OpTable T;
+  if (!h || build_image_ops(h, &T)) return 0;
+  return pairs_ws(R, h->nterms, T.n);
 }
 
 int paces_holstein_pairs_count(int64_t R, int64_t nbasis, const int64_t* run_offsets,
@@ -956,7 +1089,7 @@ int paces_holstein_pairs_count(int64_t R, int64_t nbasis, const int64_t* run_off
   const int64_t nimg = IR.off[IR.n];
   if (nimg > 0 && (!cand_pos || !in_basis || !img_src)) return PACES_E_INVALID;
   if ((int64_t)h->nterms * R >= 2147483647LL) return PACES_E_INVALID;
-  if (ws_bytes < pairs_ws(R, h->nterms)) return PACES_E_WORKSPACE;
+  if (ws_bytes < pairs_ws(R, h->nterms, T.n)) return PACES_E_WORKSPACE;
   hipStream_t st = (hipStream_t)stream;
   const int64_t m = (int64_t)h->nterms * R;
   char* p = (char*)ws;
@@ -964,12 +1097,27 @@ int paces_holstein_pairs_count(int64_t R, int64_t nbasis, const int64_t* run_off
   p += a256((size_t)m * 4);
   int32_t* G = (int32_t*)p;
   p += a256((size_t)(m + 1) * 4);
-  if (m > 0) {
-    hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, st, cnt, m);
-  }
-  if (nimg > 0) {
-    hipLaunchKernelGGL(k_pairs_count, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, R,
-                       nbasis, IR, T, cand_pos, in_basis, img_src, cnt);
+  SlotArg SL;
+  if (build_slots(T, h, &SL)) {
+    uint8_t* flags = pairs_flags(ws, R, h->nterms);
+    const int64_t nw = ((int64_t)R * T.n + 7) / 8;
+    if (nw > 0)
+      hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(nw, kBlock)), dim3(kBlock), 0, st, (u64*)flags,
+                         nw);
+    if (nimg > 0)
+      hipLaunchKernelGGL(k_pairs_mark, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, R,
+                         nbasis, IR, T, SL, cand_pos, in_basis, img_src, flags);
+    if (m > 0)
+      hipLaunchKernelGGL(k_pairs_rowcount, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, st, R,
+                         h->nterms, SL, (const uint8_t*)flags, cnt);
+  } else {
+    if (m > 0) {
+      hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, st, cnt, m);
+    }
+    if (nimg > 0) {
+      hipLaunchKernelGGL(k_pairs_count, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, R,
+                         nbasis, IR, T, cand_pos, in_basis, img_src, cnt);
+    }
   }
   rc = launch_status();
   if (rc) return rc;
@@ -1001,7 +1149,7 @@ int paces_holstein_pairs_fill(int64_t R, int64_t nbasis, const int64_t* run_offs
   if (rc) return rc;
   const int64_t nimg = IR.off[IR.n];
   if (nimg > 0 && (!cand_pos || !in_basis || !img_src || !img_nup)) return PACES_E_INVALID;
-  if (ws_bytes < pairs_ws(R, h->nterms)) return PACES_E_WORKSPACE;
+  if (ws_bytes < pairs_ws(R, h->nterms, T.n)) return PACES_E_WORKSPACE;
   CsrOut out;
   for (int t = 0; t < PACES_MAX_TERMS; ++t) {
     out.indptr[t] = t < h->nterms ? indptr[t] : nullptr;
@@ -1019,6 +1167,17 @@ int paces_holstein_pairs_fill(int64_t R, int64_t nbasis, const int64_t* run_offs
   int32_t* cursor = (int32_t*)p;  // the count array is no longer needed
   p += a256((size_t)m * 4);
   const int32_t* G = (const int32_t*)p;
+  SlotArg SL;
+  if (build_slots(T, h, &SL)) {  // the same choice as the count call
+    hipLaunchKernelGGL(k_pairs_indptr, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, st, R,
+                       h->nterms, G, out, (int32_t*)nullptr);
+    if (nimg > 0)
+      hipLaunchKernelGGL(k_pairs_place, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, R,
+                         nbasis, IR, T, SL, marg(h), (const u64*)basis, W, trajbits, cand_pos,
+                         in_basis, img_src, img_nup, h->d_sqrt_table,
+                         (const uint8_t*)pairs_flags(ws, R, h->nterms), out);
+    return launch_status();
+  }
   hipLaunchKernelGGL(k_pairs_indptr, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, st, R,
                      h->nterms, G, out, cursor);
   if (nimg > 0)
