@@ -257,6 +257,43 @@ __device__ __forceinline__ int bound_lds(PACES_LPTR(const unsigned char) rows, i
   return pos;
 }
 
+// Merged-order schedule of k_mu_rank's tiles. A tile's rows land in the merged order near
+// est = its first row's index + sum over the other runs of that row's lower bound there. Tiles
+// sorted by est (buckets of 256 positions; order inside a bucket does not matter) and dealt to
+// the XCDs in contiguous slices put the tiles that stage the same ranges of the other runs, and
+// write the same stretch of the merged order, on one L2 at the same time.
+constexpr int kXcds = 8;
+constexpr int kSchedShift = 8;
+
+__global__ void __launch_bounds__(kBlock) k_mu_sched_est(RunsArg R, const int64_t* __restrict__ lohi,
+                                                         int32_t* __restrict__ est,
+                                                         int32_t* __restrict__ cnt) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= R.boff[R.K]) return;
+  const int r = run_of_block(R, b);
+  int64_t e = (b - R.boff[r]) * kBlock;
+  for (int s = 0; s < R.K; ++s)
+    if (s != r) e += lohi[b * R.K * 2 + 2 * s];
+  est[b] = (int32_t)e;
+  atomicAdd(&cnt[e >> kSchedShift], 1);
+}
+
+__global__ void __launch_bounds__(kBlock) k_mu_sched_place(int64_t nblk,
+                                                           const int32_t* __restrict__ est,
+                                                           const int32_t* __restrict__ start,
+                                                           int32_t* __restrict__ cnt,
+                                                           int32_t* __restrict__ sched) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblk) return;
+  const int bk = est[b] >> kSchedShift;
+  sched[start[bk] + atomicSub(&cnt[bk], 1) - 1] = (int32_t)b;
+}
+
+__global__ void __launch_bounds__(kBlock) k_zero_i32r(int32_t* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 // One workgroup per tile of one run: the tile's rows are sorted, so the rows of run s they can
 // land among form the narrow range [lo, hi] the partition pass found (hi = lower bound of the
 // tile's last row); that range, with the row at hi, is staged in LDS.
@@ -292,6 +329,7 @@ constexpr int mu_buf_words() {
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, const int64_t* __restrict__ lohi,
+                                                    const int32_t* __restrict__ sched, int per,
                                                     uint32_t* __restrict__ code) {
   // one LDS buffer: a group of ranges, or (for a range larger than kMuWords) the fenced
   // single-run search's table
@@ -299,9 +337,13 @@ __global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, const int64_t* __
   __shared__ MuTable T;
   LookupSmem<W>& sm = *reinterpret_cast<LookupSmem<W>*>(buf);
   const PACES_LPTR(const unsigned char) lbuf = (PACES_LPTR(const unsigned char))buf;
-  const int r = run_of_block(R, blockIdx.x);
+  // workgroups go to the XCDs round robin: XCD x runs the x-th slice of the merged-order schedule
+  const int j = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+  if (j >= (int)R.boff[R.K]) return;
+  const int64_t tile = sched[j];
+  const int r = run_of_block(R, tile);
   const int K = R.K;
-  const int64_t* bl = lohi + (int64_t)blockIdx.x * K * 2;
+  const int64_t* bl = lohi + tile * K * 2;
   if ((int)threadIdx.x < 2 * K) T.lohi[threadIdx.x] = bl[threadIdx.x];
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -339,7 +381,7 @@ __global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, const int64_t* __
   }
   __syncthreads();
   const int64_t len_r = R.off[r + 1] - R.off[r];
-  const int own = (int)(((int64_t)blockIdx.x - R.boff[r]) * kBlock) + (int)threadIdx.x;
+  const int own = (int)((tile - R.boff[r]) * kBlock) + (int)threadIdx.x;
   const bool v = own < len_r;
   Row<W> x;
 #pragma unroll
@@ -479,11 +521,17 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 static int64_t mu_scan_tiles(int64_t C) { return C <= 0 ? 1 : (C + kScanTile - 1) / kScanTile; }
 
+static int64_t mu_sched_buckets(int64_t C) { return (C >> kSchedShift) + 1; }
+
 size_t mu_ws_bytes(int64_t C, int K) {
   // the partition table holds 2K int64 per tile; runs add at most K partial tiles
   int64_t ntiles = (C + kBlock - 1) / kBlock + K;
-  return align256((size_t)C * 4) + align256((size_t)(mu_scan_tiles(C) + 1) * 4) + align256((size_t)ntiles * K * 2 * 8) +
-         align256(scan_tile_offsets_ws_bytes(C));
+  const int64_t nbk = mu_sched_buckets(C);
+  const size_t sched = 2 * align256((size_t)ntiles * 4) + align256((size_t)nbk * 4) +
+                       align256((size_t)(nbk + 1) * 4) + align256(scan_ws_bytes(nbk));
+  const size_t scan = scan_tile_offsets_ws_bytes(C);
+  return align256((size_t)C * 4) + align256((size_t)(mu_scan_tiles(C) + 1) * 4) +
+         align256((size_t)ntiles * K * 2 * 8) + align256(sched > scan ? sched : scan);
 }
 
 template <int W>
@@ -507,8 +555,30 @@ static int run_mu(const RunsArg& R, uint64_t* out, int32_t* out_count, uint8_t* 
   hipLaunchKernelGGL(k_mu_fill, dim3(grid_for(C, kBlock)), dim3(kBlock), 0, st, code, C);
   hipLaunchKernelGGL(k_mu_bounds<W>, dim3(grid_for(nblk * R.K * 2, kBlock)), dim3(kBlock), 0, st,
                      R, lohi);
-  hipLaunchKernelGGL(k_mu_rank<W>, dim3((unsigned)nblk), dim3(kBlock), 0, st, R,
-                     (const int64_t*)lohi, code);
+  {  // the schedule shares the area the tile-offset scan uses afterwards
+    char* q = p;
+    int32_t* est = (int32_t*)q;
+    q += align256((size_t)nblk * 4);
+    int32_t* sched = (int32_t*)q;
+    q += align256((size_t)nblk * 4);
+    const int64_t nbk = mu_sched_buckets(C);
+    int32_t* cnt = (int32_t*)q;
+    q += align256((size_t)nbk * 4);
+    int32_t* start = (int32_t*)q;
+    q += align256((size_t)(nbk + 1) * 4);
+    hipLaunchKernelGGL(k_zero_i32r, dim3(grid_for(nbk, kBlock)), dim3(kBlock), 0, st, cnt, nbk);
+    hipLaunchKernelGGL(k_mu_sched_est, dim3(grid_for(nblk, kBlock)), dim3(kBlock), 0, st, R,
+                       (const int64_t*)lohi, est, cnt);
+    int rc = launch_status();
+    if (rc) return rc;
+    rc = scan_i32(cnt, nbk, start, q, scan_ws_bytes(nbk), st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_mu_sched_place, dim3(grid_for(nblk, kBlock)), dim3(kBlock), 0, st, nblk,
+                       (const int32_t*)est, (const int32_t*)start, cnt, sched);
+    const int per = (int)((nblk + kXcds - 1) / kXcds);
+    hipLaunchKernelGGL(k_mu_rank<W>, dim3((unsigned)(per * kXcds)), dim3(kBlock), 0, st, R,
+                       (const int64_t*)lohi, (const int32_t*)sched, per, code);
+  }
   int rc = launch_status();
   if (rc) return rc;
   rc = scan_tile_offsets_bit0(code, C, offs, p, scan_tile_offsets_ws_bytes(C), st);
