@@ -344,6 +344,14 @@ __global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, const int64_t* __
   const int r = run_of_block(R, tile);
   const int K = R.K;
   const int64_t* bl = lohi + tile * K * 2;
+  // the tile's own row is loaded first, so its latency overlaps the table's
+  const int64_t len_r = R.off[r + 1] - R.off[r];
+  const int own = (int)((tile - R.boff[r]) * kBlock) + (int)threadIdx.x;
+  const bool v = own < len_r;
+  Row<W> x;
+#pragma unroll
+  for (int k = 0; k < W; ++k) x.w[k] = 0;
+  if (v) x = load_row<W>(R.ptr[r], own);
   if ((int)threadIdx.x < 2 * K) T.lohi[threadIdx.x] = bl[threadIdx.x];
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -380,13 +388,6 @@ __global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, const int64_t* __
     T.ng = g;
   }
   __syncthreads();
-  const int64_t len_r = R.off[r + 1] - R.off[r];
-  const int own = (int)((tile - R.boff[r]) * kBlock) + (int)threadIdx.x;
-  const bool v = own < len_r;
-  Row<W> x;
-#pragma unroll
-  for (int k = 0; k < W; ++k) x.w[k] = 0;
-  if (v) x = load_row<W>(R.ptr[r], own);
   bool keep = true;
   int m = own;
   const int ng = T.ng;
