@@ -153,11 +153,12 @@ def btay_term_bytes(R, mats, mixed):
     return f * R + 4 * (R + 1) * len(mats) + (4 + f) * nnz + 4 * f * R + 32 * R
 
 
-# HBM bytes per row of one k_btay_term<float, 512, 4> launch (the autotuned shape), from
-# FETCH_SIZE and WRITE_SIZE counter passes over the roofline loop below, each scaled by the
-# counter's reading for a 1 GiB device copy (profiles/k_btay_term_traj256_hbm_traffic.txt). Only
-# the mixed-precision kernel is measured.
-BTAY_TRAFFIC_BYTES_PER_ROW = {"mixed": 101.56}
+# HBM bytes per row of one k_btay_term<float, B, RR> launch, per launch shape (the tuner picks
+# among shapes within a few percent of each other), from FETCH_SIZE and WRITE_SIZE counter passes
+# over the roofline loop below with the shape fixed (--term-shape), each scaled by the counter's
+# reading for a 1 GiB device copy (profiles/k_btay_term_traj256_hbm_traffic.txt). Only the
+# mixed-precision kernel is measured; other shapes report null.
+BTAY_TRAFFIC_BYTES_PER_ROW = {("mixed", (512, 4)): 101.56, ("mixed", (256, 4)): 104.90}
 
 
 def measure_batched_term(bte, reps=20):
@@ -214,7 +215,8 @@ def measure_batched_term(bte, reps=20):
     return e0.elapsed_time(e1) / reps * 1e-3, btay_term_bytes(R, mats, mixed), reps, shape
 
 
-def batched_leg(draws, precision, warmup, steps, label, graph=False, autotune=False):
+def batched_leg(draws, precision, warmup, steps, label, graph=False, autotune=False,
+                launch_shape=None):
     """Batched run: step 0 and ``warmup`` steps untimed, then ``steps`` timed steps. Returns
     (state-steps, seconds, the evolution object)."""
     import torch
@@ -222,7 +224,8 @@ def batched_leg(draws, precision, warmup, steps, label, graph=False, autotune=Fa
     bham = holstein.BatchedHamiltonian([d[0] for d in draws], 8, [-d[1] for d in draws],
                                        [d[2] for d in draws])
     bte = holstein.BatchedTimeEvolution(bham, [d[3] for d in draws], enlarge_steps=1,
-                                        precision=precision, graph=graph, autotune=autotune)
+                                        precision=precision, graph=graph, autotune=autotune,
+                                        launch_shape=launch_shape)
     bte.create_single_exciton_states([d[0] // 2 for d in draws])
     bte.step(0, TRAJ_DT)
     for i in range(1, warmup + 1):
@@ -329,7 +332,8 @@ def run_traj256(a):
     draws = traj256_draws(a.ntraj)
     prec = a.precision_batched
     units, elapsed, bte = batched_leg(draws, prec, a.warmup, a.steps, "batched-" + prec,
-                                      graph=a.graph, autotune=a.autotune)
+                                      graph=a.graph, autotune=a.autotune,
+                                      launch_shape=a.term_shape)
     rows = bte.numstates
     value = units / elapsed
     sec, nbytes, reps, shape = measure_batched_term(bte)
@@ -337,8 +341,8 @@ def run_traj256(a):
     achieved = nbytes / sec / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
             "frac": round(achieved / 8000.0, 4),
-            "traffic": (int(BTAY_TRAFFIC_BYTES_PER_ROW[prec] * rows)
-                        if prec in BTAY_TRAFFIC_BYTES_PER_ROW else None),
+            "traffic": (int(BTAY_TRAFFIC_BYTES_PER_ROW[(prec, shape)] * rows)
+                        if (prec, shape) in BTAY_TRAFFIC_BYTES_PER_ROW else None),
             "kernel": "k_btay_term<%s, %s> (paces_batch_taylor_term_tuned, the timed steps' "
                       "launch shape, stopping test off)" % ("float" if prec == "mixed" else "double",
                                                             kshape),
@@ -368,6 +372,7 @@ def run_traj256(a):
             "data": "synthetic",
             "config": dict({"workload": TRAJ_DESC.format(n=a.ntraj), "form": "batched-" + prec,
                             "series_graph": bool(a.graph), "autotune": bool(a.autotune),
+                            "term_shape": list(a.term_shape) if a.term_shape else "tuned",
                             "unit_of_work": "retained basis state advanced one step "
                                             "(sum over trajectories of min(maxstates, R_prev))",
                             "final_basis_rows": int(rows),
@@ -560,6 +565,9 @@ def main(argv=None):
                     help="skip the batched FP64 run that gives the density error")
     ap.add_argument("--graph", type=int, default=1,
                     help="traj256: run each step's Taylor series as one replayed HIP graph")
+    ap.add_argument("--term-shape", default=None,
+                    help="traj256: fix the Taylor term's launch shape, e.g. 256,4 (threads, rows "
+                         "per thread), instead of autotuning it (for reproducible profiles)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="traj256: tune the Taylor-term launch shape")
     ap.add_argument("--learned-ms", type=int, default=2_000_000)
@@ -571,6 +579,8 @@ def main(argv=None):
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 1 GiB device copy (counter calibration)")
     a = ap.parse_args(argv)
+    if a.term_shape is not None:
+        a.term_shape = tuple(int(v) for v in a.term_shape.split(","))
     if a.warmup is None:
         a.warmup = {"traj256": 8, "learned": 12}.get(a.workload, 6)
     if a.gpus != 1:

@@ -702,7 +702,8 @@ class BatchedTimeEvolution:
     """
 
     def __init__(self, hamobj: BatchedHamiltonian, maxstates, enlarge_steps=1,
-                 precision="mixed", m_star=100, taylor_tol=None, autotune=False, graph=False):
+                 precision="mixed", m_star=100, taylor_tol=None, autotune=False, graph=False,
+                 launch_shape=None):
         self.hamobj = hamobj
         self.device = hamobj.device
         ms = numpy.asarray(maxstates, dtype=numpy.int64)
@@ -720,6 +721,14 @@ class BatchedTimeEvolution:
         self._ws = hamobj._ws
         self._tws = _hip.Workspace(self.device)
         self.autotune = bool(autotune)
+        # launch_shape=(threads, rows per thread): a fixed Taylor-term shape instead of the
+        # default or the tuned one (every shape gives identical results)
+        if launch_shape is not None:
+            launch_shape = (int(launch_shape[0]), int(launch_shape[1]))
+            from ..core import autotune as _at
+            if launch_shape not in _at.SHAPES:
+                raise ValueError(f"launch_shape must be one of {_at.SHAPES}")
+        self.launch_shape = launch_shape
         # graph=True: the Taylor series runs as one HIP graph per step (captured once per row
         # capacity and term count, replayed with a rewritten device descriptor)
         self.graph = bool(graph)
@@ -884,6 +893,9 @@ class BatchedTimeEvolution:
     def _launch_shape(self, R, mats, mixed, dptr, h64, h32, wsp, wsn, st):
         """The tuned launch shape of the batched term (None: the default), timed on scratch
         buffers with the stopping test off the first time a size bucket is seen."""
+        if self.launch_shape is not None:
+            from ..core import autotune
+            return autotune.launch_cfg(self.launch_shape)
         if not self.autotune or R == 0:
             return None
         from ..core import autotune
