@@ -145,12 +145,13 @@ TRAJ_DESC = ("traj256: {n} independent 1D Holstein chains (open), N in {{16,24,3
 
 
 def btay_term_bytes(R, mats, mixed):
-    """Algorithmic HBM bytes of one batched Taylor term over R rows: diagonal (4 or 8 per row),
-    each term's indptr (4(R+1)), indices + values (4 + 4 or 8 per nonzero), x_in and x_out
-    (8 or 16 per row each), complex128 partial sum read + write (32 per row)."""
+    """Algorithmic HBM bytes of one batched Taylor term over R rows, averaged over an even and an
+    odd term (the roofline loop times both alike): diagonal (4 or 8 per row), each term's indptr
+    (4(R+1)), indices + values (4 + 4 or 8 per nonzero), x_in and x_out (8 or 16 per row each),
+    the complex128 partial sum read on every term and written on odd terms (16 + 8 per row)."""
     f = 4 if mixed else 8
     nnz = sum(m.nnz for m in mats)
-    return f * R + 4 * (R + 1) * len(mats) + (4 + f) * nnz + 4 * f * R + 32 * R
+    return f * R + 4 * (R + 1) * len(mats) + (4 + f) * nnz + 4 * f * R + 24 * R
 
 
 # HBM bytes per row of one k_btay_term<float, B, RR> launch, per launch shape (the tuner picks
@@ -158,7 +159,7 @@ def btay_term_bytes(R, mats, mixed):
 # over the roofline loop below with the shape fixed (--term-shape), each scaled by the counter's
 # reading for a 1 GiB device copy (profiles/k_btay_term_traj256_hbm_traffic.txt). Only the
 # mixed-precision kernel is measured; other shapes report null.
-BTAY_TRAFFIC_BYTES_PER_ROW = {("mixed", (512, 4)): 101.56, ("mixed", (256, 4)): 104.90}
+BTAY_TRAFFIC_BYTES_PER_ROW = {}
 
 
 def measure_batched_term(bte, reps=20):
@@ -232,14 +233,14 @@ def batched_leg(draws, precision, warmup, steps, label, graph=False, autotune=Fa
         bte.step(i, TRAJ_DT)
         print(f"[{label}] warmup step {i}: basis {bte.numstates}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
-    units = 0
+    prevs = []  # each step's incoming segment offsets, counted after the timed region
     t0 = time.perf_counter()
     for i in range(warmup + 1, warmup + 1 + steps):
-        prev = np.diff(bte.seg_off.cpu().numpy())
+        prevs.append(bte.seg_off)
         bte.step(i, TRAJ_DT)
-        units += int(np.minimum(prev, bte.maxstates).sum())
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    units = sum(int(np.minimum(np.diff(p.cpu().numpy()), bte.maxstates).sum()) for p in prevs)
     print(f"[{label}] timed {steps} steps in {elapsed:.3f} s, basis {bte.numstates}",
           file=sys.stderr, flush=True)
     return units, elapsed, bte

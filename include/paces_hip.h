@@ -341,7 +341,10 @@ int paces_batch_taylor_begin_tuned(int64_t R, const double* v, double* res, void
                                    paces_batch_taylor_t* ts, void* ws, size_t ws_bytes,
                                    const paces_launch_cfg_t* cfg, void* stream);
 /* one term for the trajectories still running; exactly one of H64 (FP64 values, complex128
- * terms, double diag) and H32 (FP32 values, complex64 terms, float diag) is non-null */
+ * terms, double diag) and H32 (FP32 values, complex64 terms, float diag) is non-null.
+ * res is stored on odd terms j only (an even term's sum is formed for the stopping test and
+ * completed by the next term, with the same roundings); after the last term call
+ * paces_batch_taylor_finish. Even terms must write their x_out to one buffer (x_even). */
 int paces_batch_taylor_term(int64_t R, const void* diag, const paces_csr_set_t* H64,
                             const paces_csr32_set_t* H32, const void* x_in, void* x_out,
                             double* res, double c, int32_t j, double tol, const int32_t* seg_off,
@@ -354,13 +357,21 @@ int paces_batch_taylor_term_tuned(int64_t R, const void* diag, const paces_csr_s
                                   const int32_t* seg_off, int32_t ntraj, paces_batch_taylor_t* ts,
                                   paces_batch_taylor_hdr_t* hdr, void* ws, size_t ws_bytes,
                                   const paces_launch_cfg_t* cfg, void* stream);
+/* after the series (terms and/or a replayed paces_batch_series_launch): adds to res the last
+ * term of every trajectory whose last term index ts[t].j_ind is even, from x_even (the buffer
+ * every even term wrote); same launch shape and workspace as the series */
+int paces_batch_taylor_finish(int64_t R, const void* x_even, int32_t mixed, double* res,
+                              const int32_t* seg_off, int32_t ntraj,
+                              const paces_batch_taylor_t* ts, void* ws, size_t ws_bytes,
+                              const paces_launch_cfg_t* cfg, void* stream);
 
 /* ---- N1 the batched series as one graph-capturable launch sequence (no reference
  * counterpart). Every argument lives in a device-resident descriptor that the caller rewrites
  * before each step, so a HIP graph captured once from paces_batch_series_launch is replayed at
  * every step. Grids are sized for the capacity R_cap (>= the step's rows); workgroups past the
  * step's segments exit. The launch runs begin and terms 0 .. nterms-1 (term j reads x[2] for
- * j == 0, else x[(j-1) % 2], and writes x[j % 2]); terms of stopped trajectories do nothing, and
+ * j == 0, else x[(j-1) % 2], and writes x[j % 2], so x_even = x[0]); terms of stopped
+ * trajectories do nothing, and
  * a series not finished after nterms continues with paces_batch_taylor_term_tuned from term
  * nterms with the same state. */
 #define PACES_SERIES_MAX_TERMS 64

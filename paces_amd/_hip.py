@@ -179,6 +179,8 @@ _SIGNATURES = {
                                                 C.POINTER(CsrSet32), _P, _P, _P, _D, _I32, _D,
                                                 _P, _I32, _P, _P, _P, _SZ, C.POINTER(LaunchCfg),
                                                 _P]),
+    "paces_batch_taylor_finish": (C.c_int, [_I64, _P, _I32, _P, _P, _I32, _P, _P, _SZ,
+                                            C.POINTER(LaunchCfg), _P]),
     "paces_rows_varying_bits": (C.c_int, [_P, _I64, _I32, _P, _P]),
     "paces_sort_rows_workspace_bytes": (_SZ, [_I64, _I32]),
     "paces_sort_rows": (C.c_int, [_P, _I64, _I32, _P, _I32, _P, _P, _P, _P, _SZ, _P]),

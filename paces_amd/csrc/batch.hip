@@ -471,9 +471,11 @@ struct Vec2<float> {
 template <typename F>
 __device__ __forceinline__ typename Vec2<F>::T hx_row_t(int64_t i, const F* __restrict__ diag,
                                                         const CsrSetB64& H,
-                                                        const typename Vec2<F>::T* __restrict__ x) {
+                                                        const typename Vec2<F>::T* __restrict__ x,
+                                                        typename Vec2<F>::T& xi_out) {
   typedef typename Vec2<F>::T V;
   const V xi = x[i];
+  xi_out = xi;
   const F d = diag[i];
   F yr = d * xi.x, yi = d * xi.y;
   // unrolled over the compile-time maximum so every H field index is static (registers, not
@@ -590,30 +592,40 @@ __device__ __forceinline__ void btay_reduce_body(const u64* __restrict__ part,
   }
 }
 
+// The partial sum res is stored on odd terms only. On an even term j the sum res + t_j is formed
+// in registers for the stopping test; the next (odd) term adds its own input (t_j) and then its
+// output, (res + t_j) + t_(j+1), the same roundings in the same order as storing every term. A
+// trajectory whose series ends on an even term gets its last term added by k_btay_finish.
 template <typename F, int BS, int RPT>
 __device__ __forceinline__ void btay_term_body(
     const F* __restrict__ diag, const CsrSetB64& H, const typename Vec2<F>::T* __restrict__ xin,
-    typename Vec2<F>::T* __restrict__ xout, double2* __restrict__ res, F c,
+    typename Vec2<F>::T* __restrict__ xout, double2* __restrict__ res, F c, int j,
     const int32_t* __restrict__ seg_off, const int64_t* __restrict__ boff, int ntraj,
     const paces_batch_taylor_t* __restrict__ ts, paces_batch_taylor_hdr_t* __restrict__ hdr,
     u64* __restrict__ part) {
   if (blockIdx.x == 0 && threadIdx.x == 0) hdr->notdone = 0;
   const SegTile T = seg_tile(seg_off, boff, ntraj, BS * RPT);
   if (T.t < 0 || ts[T.t].done) return;
+  const bool odd = (j & 1) != 0;
   u64 mt = 0, mr = 0;
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
     const int64_t i = T.r0 + q * BS + threadIdx.x;
     if (i < T.r1) {
-      const typename Vec2<F>::T h = hx_row_t<F>(i, diag, H, xin);
+      typename Vec2<F>::T xi;
+      const typename Vec2<F>::T h = hx_row_t<F>(i, diag, H, xin, xi);
       typename Vec2<F>::T t;
       t.x = h.y * c;
       t.y = -(h.x * c);
       xout[i] = t;
       double2 r = res[i];
+      if (odd) {
+        r.x = r.x + (double)xi.x;
+        r.y = r.y + (double)xi.y;
+      }
       r.x = r.x + (double)t.x;
       r.y = r.y + (double)t.y;
-      res[i] = r;
+      if (odd) res[i] = r;
       const double tx = t.x, ty = t.y;
       const u64 a = (u64)__double_as_longlong(tx * tx + ty * ty);
       const u64 b = (u64)__double_as_longlong(r.x * r.x + r.y * r.y);
@@ -647,11 +659,34 @@ __global__ void __launch_bounds__(64) k_btay_reduce(const u64* __restrict__ part
 template <typename F, int BS, int RPT>
 __global__ void __launch_bounds__(BS) k_btay_term(
     const F* __restrict__ diag, CsrSetB64 H, const typename Vec2<F>::T* __restrict__ xin,
-    typename Vec2<F>::T* __restrict__ xout, double2* __restrict__ res, F c,
+    typename Vec2<F>::T* __restrict__ xout, double2* __restrict__ res, F c, int j,
     const int32_t* __restrict__ seg_off, const int64_t* __restrict__ boff, int ntraj,
     const paces_batch_taylor_t* __restrict__ ts, paces_batch_taylor_hdr_t* __restrict__ hdr,
     u64* __restrict__ part) {
-  btay_term_body<F, BS, RPT>(diag, H, xin, xout, res, c, seg_off, boff, ntraj, ts, hdr, part);
+  btay_term_body<F, BS, RPT>(diag, H, xin, xout, res, c, j, seg_off, boff, ntraj, ts, hdr, part);
+}
+
+// After the series: a trajectory whose last term j is even has res one term behind; its last
+// term is still in x_even (even terms write there, and later terms skip stopped trajectories).
+template <typename F, int BS, int RPT>
+__global__ void __launch_bounds__(BS) k_btay_finish(const typename Vec2<F>::T* __restrict__ x_even,
+                                                    double2* __restrict__ res,
+                                                    const int32_t* __restrict__ seg_off,
+                                                    const int64_t* __restrict__ boff, int ntraj,
+                                                    const paces_batch_taylor_t* __restrict__ ts) {
+  const SegTile T = seg_tile(seg_off, boff, ntraj, BS * RPT);
+  if (T.t < 0 || (ts[T.t].j_ind & 1) != 0) return;
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int64_t i = T.r0 + q * BS + threadIdx.x;
+    if (i < T.r1) {
+      const typename Vec2<F>::T x = x_even[i];
+      double2 r = res[i];
+      r.x = r.x + (double)x.x;
+      r.y = r.y + (double)x.y;
+      res[i] = r;
+    }
+  }
 }
 
 // ---- the series from a device descriptor (N1: captured once into a HIP graph, replayed every
@@ -741,7 +776,7 @@ __global__ void __launch_bounds__(BS) k_bser_term(const paces_batch_series_t* __
   const V* xin = gen((PACES_GPTR(const V))(j == 0 ? d->x[2] : d->x[(j - 1) & 1]));
   V* xout = gen((PACES_GPTR(V))d->x[j & 1]);
   btay_term_body<F, BS, RPT>(gen((PACES_GPTR(const F))d->diag), H, xin, xout,
-                             gen((PACES_GPTR(double2))d->res), (F)d->c[j], gen(d->seg_off),
+                             gen((PACES_GPTR(double2))d->res), (F)d->c[j], j, gen(d->seg_off),
                              ser_boff(d), ntraj, gen(d->ts), gen(d->hdr), ser_part(d, ntraj));
 }
 
@@ -948,17 +983,41 @@ int paces_batch_taylor_term_tuned(int64_t R, const void* diag, const paces_csr_s
   if (mixed)                                                                                   \
     hipLaunchKernelGGL((k_btay_term<float, BB, QQ>), dim3(gs), dim3(BB), 0, st,                \
                        (const float*)diag, H, (const float2*)x_in, (float2*)x_out,             \
-                       (double2*)res, (float)c, seg_off, boff, ntraj,                          \
+                       (double2*)res, (float)c, j, seg_off, boff, ntraj,                       \
                        (const paces_batch_taylor_t*)ts, hdr, part);                            \
   else                                                                                         \
     hipLaunchKernelGGL((k_btay_term<double, BB, QQ>), dim3(gs), dim3(BB), 0, st,               \
                        (const double*)diag, H, (const double2*)x_in, (double2*)x_out,          \
-                       (double2*)res, c, seg_off, boff, ntraj,                                 \
+                       (double2*)res, c, j, seg_off, boff, ntraj,                              \
                        (const paces_batch_taylor_t*)ts, hdr, part)
   PACES_CFG_SWITCH(B, RR, BT_LAUNCH)
 #undef BT_LAUNCH
   hipLaunchKernelGGL(k_btay_reduce, dim3(ntraj), dim3(64), 0, st, (const u64*)part, boff, ts, hdr,
                      0, (int)j, tol);
+  return launch_status();
+}
+
+int paces_batch_taylor_finish(int64_t R, const void* x_even, int32_t mixed, double* res,
+                              const int32_t* seg_off, int32_t ntraj,
+                              const paces_batch_taylor_t* ts, void* ws, size_t ws_bytes,
+                              const paces_launch_cfg_t* cfg, void* stream) {
+  if (R < 0 || ntraj < 1 || !seg_off || !ts || !ws || !cfg_ok(cfg)) return PACES_E_INVALID;
+  if (R > 0 && (!x_even || !res)) return PACES_E_INVALID;
+  if (ws_bytes < btay_ws_bytes(R, ntraj)) return PACES_E_WORKSPACE;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t* boff = (const int64_t*)ws;  // the series' tile offsets (same shape)
+  int B, RR;
+  cfg_get(cfg, &B, &RR);
+  const unsigned gs = seg_grid(R, ntraj, B * RR);
+#define BF_LAUNCH(BB, QQ)                                                                      \
+  if (mixed)                                                                                   \
+    hipLaunchKernelGGL((k_btay_finish<float, BB, QQ>), dim3(gs), dim3(BB), 0, st,              \
+                       (const float2*)x_even, (double2*)res, seg_off, boff, ntraj, ts);        \
+  else                                                                                         \
+    hipLaunchKernelGGL((k_btay_finish<double, BB, QQ>), dim3(gs), dim3(BB), 0, st,             \
+                       (const double2*)x_even, (double2*)res, seg_off, boff, ntraj, ts)
+  PACES_CFG_SWITCH(B, RR, BF_LAUNCH)
+#undef BF_LAUNCH
   return launch_status();
 }
 

@@ -741,7 +741,8 @@ class BatchedTimeEvolution:
         self.seg_off = None
         self.diag_vals = None
         self.sparse_mats_dict = None
-        self.last_expm = None
+        self._expm_pending = None
+        self._expm_info = None
 
     @property
     def numstates(self):
@@ -758,7 +759,7 @@ class BatchedTimeEvolution:
         raw[:, 1] = pos
         self.whoami = h.compress_ind_arr(raw)
         self.vector = torch.ones(h.ntraj, dtype=torch.complex128, device=self.device)
-        self.seg_off = h.segments(self.whoami)
+        self._set_segments(h.segments(self.whoami))
 
     def _select(self):
         R, W = self.whoami.shape
@@ -769,7 +770,10 @@ class BatchedTimeEvolution:
                                            R, W, ptr(self.seg_off), self.hamobj.ntraj,
                                            ptr(self._d_ms), ptr(out), ptr(cnt), wsp, wsn,
                                            stream()), "batch_select_topk")
-        return out[:int(cnt.item())]
+        host, ev = self._seg_host
+        ev.synchronize()  # recorded a step ago: long complete
+        n = int(numpy.minimum(numpy.diff(host.numpy()), self.maxstates).sum())
+        return out[:n]
 
     def adapt(self):
         select = self._select()
@@ -778,8 +782,18 @@ class BatchedTimeEvolution:
         self.whoami = new_inds
         self.diag_vals = diag
         self.sparse_mats_dict = mats
-        self.seg_off = self.hamobj.segments(new_inds)
+        self._set_segments(self.hamobj.segments(new_inds))
         return dbg
+
+    def _set_segments(self, seg_off):
+        """The segment offsets, with a pinned host copy made without a stream sync (the next
+        select sizes its output from it: sum over trajectories of min(maxstates, rows))."""
+        self.seg_off = seg_off
+        host = torch.empty(seg_off.shape, dtype=seg_off.dtype, pin_memory=True)
+        host.copy_(seg_off, non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream())
+        self._seg_host = (host, ev)
 
     def _propagate_graph(self, delta_t, mats, mixed, res, bufs, ts, hdr, dptr, h64, h32, cfg, st):
         """Terms 0 .. n-1 by replaying a captured graph; returns n (the eager loop continues
@@ -789,6 +803,10 @@ class BatchedTimeEvolution:
         # capacity: R rounded up in sixteenths of its octave (at most 6.25% spare rows)
         q = 1 << max(8, R.bit_length() - 4)
         cap = ((R + q - 1) // q) * q
+        info = self.last_expm  # the previous step's series (arrived long ago: no stall)
+        if info is not None and info.size:
+            # terms 0 .. j of the slowest trajectory; a longer series continues eagerly
+            self._graph_terms = int(info["j"].max()) + 1
         nterms = max(1, min(self.m_star, self._graph_terms, _hip.SERIES_MAX_TERMS))
         wsp, wsn = self._tws.get(_lib.paces_batch_taylor_workspace_bytes(cap, h.ntraj))
         d = _hip.BatchSeries()
@@ -881,14 +899,35 @@ class BatchedTimeEvolution:
                 j += 1
             if int(hdr[:4].view(torch.int32).item()) == 0:
                 break
+        # trajectories whose series ended on an even term get that term added (the sum is
+        # stored on odd terms); even terms wrote bufs[0]
+        check(_lib.paces_batch_taylor_finish(R, ptr(torch.view_as_real(bufs[0])),
+                                             1 if mixed else 0, ptr(torch.view_as_real(res)),
+                                             ptr(self.seg_off), h.ntraj, ptr(ts), wsp, wsn, cref,
+                                             st), "batch_taylor_finish")
         self.vector = res
-        info = numpy.frombuffer(ts.cpu().numpy().tobytes(), dtype=numpy.dtype(
-            [("c1", "f8"), ("c1pc2", "f8"), ("ratio", "f8"), ("done", "i4"), ("j", "i4")]))
-        self.last_expm = info
-        if self.graph and info.size:
-            # terms 0 .. j of the slowest trajectory; a longer series continues eagerly
-            self._graph_terms = int(info["j"].max()) + 1
-        return info
+        # the per-trajectory series states go to pinned host memory without a stream sync; they
+        # are parsed when first read (last_expm, or the next step's graph size)
+        host = torch.empty(ts.shape, dtype=ts.dtype, pin_memory=True)
+        host.copy_(ts, non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream())
+        self._expm_pending = (host, ev)
+        self._expm_info = None
+
+    _EXPM_DTYPE = numpy.dtype([("c1", "f8"), ("c1pc2", "f8"), ("ratio", "f8"), ("done", "i4"),
+                               ("j", "i4")])
+
+    @property
+    def last_expm(self):
+        """The last series' per-trajectory states: c1, c1 + c2, ratio, done and the last term
+        index j (a numpy structured array), or None before the first propagation."""
+        if self._expm_info is None and self._expm_pending is not None:
+            host, ev = self._expm_pending
+            ev.synchronize()
+            self._expm_info = numpy.frombuffer(host.numpy().tobytes(), dtype=self._EXPM_DTYPE)
+            self._expm_pending = None
+        return self._expm_info
 
     def _launch_shape(self, R, mats, mixed, dptr, h64, h32, wsp, wsn, st):
         """The tuned launch shape of the batched term (None: the default), timed on scratch
