@@ -617,16 +617,8 @@ __device__ __forceinline__ void btay_row_out(int64_t i, typename Vec2<F>::T h,
   mr = b > mr ? b : mr;
 }
 
-// H.x for the RPT rows of one thread in phases, so a thread keeps many independent loads in
-// flight instead of walking each row's dependent chain (bounds, entries, gathers) in turn:
-// 1. own x, diagonal, partial sum and every term's row bounds of every row;
-// 2. the first kEnt entries (value, column) of every (row, term);
-// 3. the gathers of x at those columns;
-// then the sums, in the same order as hx_row_t (diagonal, then each term's entries in column
-// order). Entries past the first kEnt of a row's term are taken in a loop. NT = the number of
-// off-diagonal terms (a template argument so the per-term arrays stay in registers).
-constexpr int kEnt = 2;
-
+// The batched term's rows of one thread: the partial sums are loaded first (independent of H.x),
+// then H.x in load phases (hx_rows_phased), then each row's output.
 template <typename F, int BS, int RPT, int NT>
 __device__ __forceinline__ void btay_rows_phased(const F* __restrict__ diag, const CsrSetB64& H,
                                                  const typename Vec2<F>::T* __restrict__ xin,
@@ -636,80 +628,18 @@ __device__ __forceinline__ void btay_rows_phased(const F* __restrict__ diag, con
   typedef typename Vec2<F>::T V;
   int64_t ii[RPT];
   bool vv[RPT];
-  V xi[RPT];
-  F d[RPT];
   double2 rr[RPT];
-  int32_t b[RPT][NT], e[RPT][NT];
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
     ii[q] = T.r0 + q * BS + threadIdx.x;
     vv[q] = ii[q] < T.r1;
-    const int64_t i = vv[q] ? ii[q] : T.r0;  // rows past the tile read its first row, unused
-    xi[q] = xin[i];
-    d[q] = diag[i];
-    rr[q] = res[i];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      b[q][t] = H.indptr[t][i];
-      e[q][t] = H.indptr[t][i + 1];
-    }
+    rr[q] = res[vv[q] ? ii[q] : T.r0];
   }
-  // phases 2 and 3 load unconditionally (an absent entry reads the term's indptr, an absent
-  // gather the thread's own row; both in bounds, results unused), so the code has no branches
-  // and each use waits only for its own load
-  F a[RPT][NT][kEnt];
-  uint32_t ix[RPT][NT][kEnt];  // unsigned: no sign extension (and wait) before the gathers
-  bool ok[RPT][NT][kEnt];
+  V h[RPT], xi[RPT];
+  hx_rows_phased<F, V, RPT, NT>(ii, vv, T.r0, diag, H, xin, h, xi);
 #pragma unroll
   for (int q = 0; q < RPT; ++q)
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-      for (int m = 0; m < kEnt; ++m) {
-        const int32_t k = b[q][t] + m;
-        ok[q][t][m] = vv[q] & (k < e[q][t]);
-        const F* pa = ok[q][t][m] ? ((const F*)H.data[t]) + k : (const F*)H.indptr[t];
-        const int32_t* pi = ok[q][t][m] ? H.indices[t] + k : H.indptr[t];
-        a[q][t][m] = *pa;
-        ix[q][t][m] = (uint32_t)*pi;
-      }
-  V xj[RPT][NT][kEnt];
-#pragma unroll
-  for (int q = 0; q < RPT; ++q)
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-      for (int m = 0; m < kEnt; ++m)
-        xj[q][t][m] = xin[ok[q][t][m] ? (int64_t)ix[q][t][m] : (vv[q] ? ii[q] : T.r0)];
-#pragma unroll
-  for (int q = 0; q < RPT; ++q) {
-    if (!vv[q]) continue;
-    F yr = d[q] * xi[q].x, yi = d[q] * xi[q].y;
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      F tr = 0, ti = 0;
-#pragma unroll
-      for (int m = 0; m < kEnt; ++m) {
-        if (ok[q][t][m]) {
-          tr = tr + a[q][t][m] * xj[q][t][m].x;
-          ti = ti + a[q][t][m] * xj[q][t][m].y;
-        }
-      }
-      const F* data = (const F*)H.data[t];
-      for (int32_t k = b[q][t] + kEnt; k < e[q][t]; ++k) {
-        const F av = data[k];
-        const V xv = xin[H.indices[t][k]];
-        tr = tr + av * xv.x;
-        ti = ti + av * xv.y;
-      }
-      yr = yr + tr;
-      yi = yi + ti;
-    }
-    V h;
-    h.x = yr;
-    h.y = yi;
-    btay_row_out<F>(ii[q], h, xi[q], rr[q], c, odd, xout, res, mt, mr);
-  }
+    if (vv[q]) btay_row_out<F>(ii[q], h[q], xi[q], rr[q], c, odd, xout, res, mt, mr);
 }
 
 // The partial sum res is stored on odd terms only. On an even term j the sum res + t_j is formed

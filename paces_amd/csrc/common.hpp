@@ -367,6 +367,94 @@ inline void cfg_get(const paces_launch_cfg_t* cfg, int* b, int* r) {
     default: return PACES_E_INVALID;                              \
   }
 
+// ---- sparse H.x of several rows per thread --------------------------------------------------------
+// H.x for the RPT rows ii[] of one thread (diag*x plus each term's CSR row), in load phases so a
+// thread keeps many independent loads in flight instead of walking each row's dependent chain
+// (bounds, entries, gathers) in turn:
+//   1. own x, diagonal and every term's row bounds of every row;
+//   2. the first kHxEnt entries (value, column) of every (row, term);
+//   3. the gathers of x at those columns.
+// Every load is unconditional and in bounds (rows past the tile read row `spare`, an absent
+// entry reads the term's indptr, an absent gather row `spare`; the results are unused), so the
+// code has no branches and each use waits only for its own load. The sums are formed in the
+// reference order: y = d*x; per term t = 0, t = t + a_k x_k in column order, y = y + t. Entries
+// past the first kHxEnt of a row's term are taken in a loop. NT = the number of off-diagonal terms
+// (a template argument, so the per-term arrays stay in registers). CSR: indptr[t], indices[t],
+// data[t] (F values; data may be typed or void*).
+constexpr int kHxEnt = 2;
+
+template <typename F, typename V, int RPT, int NT, typename CSR>
+__device__ __forceinline__ void hx_rows_phased(const int64_t (&ii)[RPT], const bool (&vv)[RPT],
+                                               int64_t spare, const F* __restrict__ diag,
+                                               const CSR& H, const V* __restrict__ x,
+                                               V (&h)[RPT], V (&xi)[RPT]) {
+  F d[RPT];
+  int32_t b[RPT][NT], e[RPT][NT];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int64_t i = vv[q] ? ii[q] : spare;
+    xi[q] = x[i];
+    d[q] = diag[i];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      b[q][t] = H.indptr[t][i];
+      e[q][t] = H.indptr[t][i + 1];
+    }
+  }
+  F a[RPT][NT][kHxEnt];
+  uint32_t ix[RPT][NT][kHxEnt];  // unsigned: no sign extension (and wait) before the gathers
+  bool ok[RPT][NT][kHxEnt];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q)
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int m = 0; m < kHxEnt; ++m) {
+        const int32_t k = b[q][t] + m;
+        ok[q][t][m] = vv[q] & (k < e[q][t]);
+        const F* pa = ok[q][t][m] ? ((const F*)H.data[t]) + k : (const F*)H.indptr[t];
+        const int32_t* pi = ok[q][t][m] ? H.indices[t] + k : H.indptr[t];
+        a[q][t][m] = *pa;
+        ix[q][t][m] = (uint32_t)*pi;
+      }
+  V xj[RPT][NT][kHxEnt];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q)
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int m = 0; m < kHxEnt; ++m)
+        xj[q][t][m] = x[ok[q][t][m] ? (int64_t)ix[q][t][m] : spare];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    F yr = d[q] * xi[q].x, yi = d[q] * xi[q].y;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      F tr = 0, ti = 0;
+#pragma unroll
+      for (int m = 0; m < kHxEnt; ++m) {
+        if (ok[q][t][m]) {
+          tr = tr + a[q][t][m] * xj[q][t][m].x;
+          ti = ti + a[q][t][m] * xj[q][t][m].y;
+        }
+      }
+      if (vv[q]) {
+        const F* data = (const F*)H.data[t];
+        for (int32_t k = b[q][t] + kHxEnt; k < e[q][t]; ++k) {
+          const F av = data[k];
+          const V xv = x[H.indices[t][k]];
+          tr = tr + av * xv.x;
+          ti = ti + av * xv.y;
+        }
+      }
+      yr = yr + tr;
+      yi = yi + ti;
+    }
+    h[q].x = yr;
+    h[q].y = yi;
+  }
+}
+
 }  // namespace paces
 
 // Dispatch a templated launcher over the supported row widths.

@@ -42,6 +42,18 @@ __global__ void __launch_bounds__(kBlock) k_spmv(int64_t R, const double* __rest
                                                  CsrSet H, const double2* __restrict__ x,
                                                  double2* __restrict__ y) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (H.nterms == 1 || H.nterms == 2) {  // one row per thread, its loads in phases
+    const int64_t ii[1] = {i};
+    const bool vv[1] = {i < R};
+    double2 h[1], xi[1];
+    const int64_t spare = (int64_t)blockIdx.x * blockDim.x;  // the block's first row (< R)
+    if (H.nterms == 2)
+      hx_rows_phased<double, double2, 1, 2>(ii, vv, spare, diag, H, x, h, xi);
+    else
+      hx_rows_phased<double, double2, 1, 1>(ii, vv, spare, diag, H, x, h, xi);
+    if (vv[0]) y[i] = h[0];
+    return;
+  }
   if (i >= R) return;
   y[i] = hx_row(i, diag, H, x);
 }
@@ -182,6 +194,40 @@ __global__ void __launch_bounds__(BS) k_tay_term_t(int64_t R, const double* __re
   if (s->done) return;
   const int64_t base = (int64_t)blockIdx.x * (BS * RPT) + threadIdx.x;
   u64 mt = 0, mr = 0;
+  // rows in load phases (hx_rows_phased) for one or two terms, where the registers allow
+  constexpr bool phased = BS * RPT <= 2048;
+  if (phased && (H.nterms == 1 || H.nterms == 2)) {
+    int64_t ii[RPT];
+    bool vv[RPT];
+    double2 rr[RPT], h[RPT], xi[RPT];
+    const int64_t spare = (int64_t)blockIdx.x * (BS * RPT);  // the block's first row (< R)
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      ii[q] = base + (int64_t)q * BS;
+      vv[q] = ii[q] < R;
+      rr[q] = res[vv[q] ? ii[q] : spare];
+    }
+    if (H.nterms == 2)
+      hx_rows_phased<double, double2, RPT, 2>(ii, vv, spare, diag, H, xin, h, xi);
+    else
+      hx_rows_phased<double, double2, RPT, 1>(ii, vv, spare, diag, H, xin, h, xi);
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      if (!vv[q]) continue;
+      const double2 t = make_double2(h[q].y * c, -(h[q].x * c));
+      xout[ii[q]] = t;
+      double2 r = rr[q];
+      r.x = r.x + t.x;
+      r.y = r.y + t.y;
+      res[ii[q]] = r;
+      const u64 a = (u64)__double_as_longlong(t.x * t.x + t.y * t.y);
+      const u64 b = (u64)__double_as_longlong(r.x * r.x + r.y * r.y);
+      mt = a > mt ? a : mt;
+      mr = b > mr ? b : mr;
+    }
+    block_max2_store_t<BS>(mt, mr, part);
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
     const int64_t i = base + (int64_t)q * BS;
@@ -276,6 +322,40 @@ __global__ void __launch_bounds__(BS) k_tay_term_mixed_t(
   if (s->done) return;
   const int64_t base = (int64_t)blockIdx.x * (BS * RPT) + threadIdx.x;
   u64 mt = 0, mr = 0;
+  if (H.nterms == 1 || H.nterms == 2) {  // rows in load phases (hx_rows_phased)
+    int64_t ii[RPT];
+    bool vv[RPT];
+    double2 rr[RPT];
+    float2 h[RPT], xi[RPT];
+    const int64_t spare = (int64_t)blockIdx.x * (BS * RPT);  // the block's first row (< R)
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      ii[q] = base + (int64_t)q * BS;
+      vv[q] = ii[q] < R;
+      rr[q] = res[vv[q] ? ii[q] : spare];
+    }
+    if (H.nterms == 2)
+      hx_rows_phased<float, float2, RPT, 2>(ii, vv, spare, diag, H, xin, h, xi);
+    else
+      hx_rows_phased<float, float2, RPT, 1>(ii, vv, spare, diag, H, xin, h, xi);
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      if (!vv[q]) continue;
+      const float2 t = make_float2(h[q].y * c, -(h[q].x * c));
+      xout[ii[q]] = t;
+      double2 r = rr[q];
+      r.x = r.x + (double)t.x;
+      r.y = r.y + (double)t.y;
+      res[ii[q]] = r;
+      const double tx = t.x, ty = t.y;
+      const u64 a = (u64)__double_as_longlong(tx * tx + ty * ty);
+      const u64 b = (u64)__double_as_longlong(r.x * r.x + r.y * r.y);
+      mt = a > mt ? a : mt;
+      mr = b > mr ? b : mr;
+    }
+    block_max2_store_t<BS>(mt, mr, part);
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
     const int64_t i = base + (int64_t)q * BS;
