@@ -256,16 +256,31 @@ __global__ void __launch_bounds__(kBlock) k_img_emit(const u64* __restrict__ bas
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Row<W> r0 = load_row<W>(basis, i);
-  for (int k = 0; k < T.n; ++k) {
-    int64_t f = (int64_t)k * n + i;
-    if (!flags[f]) continue;
-    Row<W> r = r0;
-    int nup = 0;
-    apply_op<W>(T, k, L, M, r, nup);
-    const int64_t o = G[f];
-    store_row<W>(out, o, r);
-    if (out_src) out_src[o] = (int32_t)i;
-    if (out_nup) out_nup[o] = (uint8_t)nup;
+  // ops in groups of 4: every flag and output index of a group is loaded (unconditionally, in
+  // bounds) before any is used, so the loads overlap
+  constexpr int kG = 4;
+  for (int k0 = 0; k0 < T.n; k0 += kG) {
+    uint8_t fl[kG];
+    int32_t og[kG];
+#pragma unroll
+    for (int j = 0; j < kG; ++j) {
+      const int k = (k0 + j < T.n) ? k0 + j : k0;
+      const int64_t f = (int64_t)k * n + i;
+      fl[j] = flags[f];
+      og[j] = G[f];
+    }
+#pragma unroll
+    for (int j = 0; j < kG; ++j) {
+      const int k = k0 + j;
+      if (k >= T.n || !fl[j]) continue;
+      Row<W> r = r0;
+      int nup = 0;
+      apply_op<W>(T, k, L, M, r, nup);
+      const int64_t o = og[j];
+      store_row<W>(out, o, r);
+      if (out_src) out_src[o] = (int32_t)i;
+      if (out_nup) out_nup[o] = (uint8_t)nup;
+    }
   }
 }
 
@@ -676,20 +691,41 @@ __global__ void __launch_bounds__(kBlock) k_pairs_place(int64_t R, int64_t nbasi
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= IR.off[IR.n]) return;
   const int k = img_run_of(IR, e);
-  const int32_t p = cand_pos[nbasis + e];
-  const int32_t q = cand_pos[src[e]];
-  if (p < 0 || q < 0) return;
-  if (T.dir[k] < 0 && inB[p]) return;
   const int t = T.term[k], S = SL.S[t];
-  const int traj = M.batched ? (int)(basis[(int64_t)src[e] * W] >> (64 - trajbits)) : 0;
-  const double v = op_value(T, k, M, traj, sqrt_tab, nup[e]);
+  // every load is issued before the validity tests (indices clamped in bounds), so the
+  // dependent levels overlap: (p, src, nup) -> (q, traj word, inB[p], p's slots and start) ->
+  // (q's slots and start)
+  const int32_t p = cand_pos[nbasis + e];
+  const int32_t s = src[e];
+  const int nu = nup[e];
+  const int32_t q = cand_pos[s];
+  const int32_t pc = p < 0 ? 0 : p, qc = q < 0 ? 0 : q;
+  const u64 w0 = M.batched ? basis[(int64_t)s * W] : 0;
+  const bool pin = inB[pc] != 0;
   const uint8_t* f = flags + R * SL.foff[t];
-  const int32_t a =
-      slot_place(SL, t, f + (int64_t)q * S, S, SL.rank[k], out.indptr[t][q]);
+  uint8_t fp[8], fq[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    fp[j] = j < S ? f[(int64_t)pc * S + j] : 0;
+    fq[j] = j < S ? f[(int64_t)qc * S + j] : 0;
+  }
+  const int32_t sp = out.indptr[t][pc], sq = out.indptr[t][qc];
+  if (p < 0 || q < 0) return;
+  if (T.dir[k] < 0 && pin) return;
+  const int traj = M.batched ? (int)(w0 >> (64 - trajbits)) : 0;
+  const double v = op_value(T, k, M, traj, sqrt_tab, nu);
+  int32_t a = sq, b = sp;
+  const int rk = SL.rank[k], rinv = SL.rank[k ^ 1];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    if (j < S) {
+      const int rj = SL.rank[SL.k0[t] + j];
+      a += rj < rk ? fq[j] : 0;
+      b += rj < rinv ? fp[j] : 0;
+    }
+  }
   out.indices[t][a] = p;
   out.data[t][a] = v;
-  const int32_t b =
-      slot_place(SL, t, f + (int64_t)p * S, S, SL.rank[k ^ 1], out.indptr[t][p]);
   out.indices[t][b] = q;
   out.data[t][b] = v;
 }
