@@ -25,8 +25,8 @@ def _draws(n, seed=3):
     return out
 
 
-def _oracle(N, J, g, ms, dim, steps, dt):
-    ham = po.HolsteinOracle(N, [dim] * N, use_terms={
+def _oracle(N, J, g, ms, dim, steps, dt, periodic=False):
+    ham = po.HolsteinOracle(N, [dim] * N, periodic=periodic, use_terms={
         "diag": {"eps_sys": 0.0, "hbar_omega": 1.0, "delta_eps": 0.0},
         "hopping": {"J": -J}, "vib_coupling": {"g": g}})
     te = po.TimeEvolutionOracle(ham, ms, 1)
@@ -82,6 +82,40 @@ def test_batched_trajectories_match_per_trajectory_oracle(precision):
             else:
                 assert err <= 1e-4, (i, t, err)
     print(f"[batched {precision}] max density error {worst:.3e}")
+
+
+def test_batched_periodic_rings_match_per_trajectory_oracle():
+    """FP64 batched rings of different lengths (periodic hops, whose wrap moves differ by
+    trajectory, so the CSR is built through atomic cursors and a per-row sort rather than fixed
+    slots): basis, amplitudes and observables equal each ring's own oracle run."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from paces_amd.config import devices
+    from paces_amd.models import holstein
+    if not devices.configure_called:
+        devices.configure()
+    draws = [(N, J, g, ms) for (N, J, g, ms) in _draws(4, seed=5)]
+    dim, steps, dt = 8, 6, 0.05
+    oracles = [_oracle(N, J, g, ms, dim, steps, dt, periodic=True) for (N, J, g, ms) in draws]
+    bham = holstein.BatchedHamiltonian([d[0] for d in draws], dim, [-d[1] for d in draws],
+                                       [d[2] for d in draws], periodic=True)
+    bte = holstein.BatchedTimeEvolution(bham, [d[3] for d in draws], enlarge_steps=1,
+                                        precision="fp64")
+    bte.create_single_exciton_states([d[0] // 2 for d in draws])
+    for i in range(steps + 1):
+        print(f"[batched periodic] step {i}", flush=True)
+        bte.step(i, dt)
+        n_exc, n_pho, nrm2 = bte.observables()
+        raw = bham.decompress_ind_arr(bte.whoami).cpu().numpy()
+        vec = bte.vector.cpu().numpy()
+        for t, ((N, J, g, ms), (oham, recs)) in enumerate(zip(draws, oracles)):
+            rec = recs[i]
+            sel = raw[:, 0] == t
+            np.testing.assert_array_equal(raw[sel][:, 1:2 + N], oham.layout.unpack(rec.whoami))
+            assert np.max(np.abs(vec[sel] - rec.vec), initial=0) <= 1e-10, (i, t)
+            assert np.max(np.abs(n_exc[t, :N] - rec.obs["n_exc"])) <= 1e-10, (i, t)
+            assert np.max(np.abs(n_pho[t, :N] - rec.obs["n_pho"])) <= 1e-10, (i, t)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
