@@ -102,9 +102,65 @@ __device__ __forceinline__ int64_t lower_bound_row(const u64* __restrict__ book,
 
 constexpr int kLkWords = 3072;  // 24 KiB of staged rows per workgroup
 
+// a < b over W words, word 0 most significant: the borrow out of the multiword a - b.
+template <int W>
+__device__ __forceinline__ bool lex_less(const u64 (&a)[W], const u64 (&b)[W]) {
+  unsigned long long borrow = 0;
+#pragma unroll
+  for (int k = W - 1; k >= 0; --k) (void)__builtin_subcll(a[k], b[k], borrow, &borrow);
+  return borrow != 0;
+}
+
+template <int W>
+__device__ __forceinline__ void load_lds_row(PACES_LPTR(const unsigned char) rows, unsigned i,
+                                             u64 (&w)[W]) {
+  const PACES_LPTR(const u64) p = (PACES_LPTR(const u64))(rows + __umul24(i, W * 8));
+#pragma unroll
+  for (int k = 0; k < W; ++k) w[k] = p[k];
+}
+
+// Bound of q in n sorted unique rows staged in LDS (byte address rows): the number of rows below
+// q, or with UPPER the number of rows at most q. Binary lifting with a trip count of
+// floor(log2 n) + 1 for every lane (n is uniform across the workgroup).
+template <int W, bool UPPER>
+__device__ __forceinline__ int bound_lds(PACES_LPTR(const unsigned char) rows, int n,
+                                         const Row<W>& q) {
+  int pos = 0;
+  for (int step = n > 0 ? (int)(1u << (31 - __builtin_clz((unsigned)n))) : 0; step > 0;
+       step >>= 1) {
+    const int probe = pos + step - 1;
+    const bool in = probe < n;
+    u64 w[W];
+    load_lds_row<W>(rows, (unsigned)(in ? probe : n - 1), w);
+    const bool pred = UPPER ? !lex_less<W>(q.w, w) : lex_less<W>(w, q.w);
+    if (in & pred) pos += step;
+  }
+  return pos;
+}
+
+// Copy the 16-byte blocks holding bytes [src, src + nbytes) global -> LDS at dst (16-byte
+// aligned) by direct loads (global_load_lds_dwordx4: lane l's block lands at the wave's base +
+// 16 l, no registers); returns the number of blocks. A block never crosses a page, so the bytes
+// read beyond the range cannot fault. Every wave of the workgroup takes part; the caller waits
+// (vmcnt 0) and synchronises the workgroup before reading dst.
+__device__ __forceinline__ int dma_blocks_to_lds(const void* src, size_t nbytes, u64* dst,
+                                                 int& lead_words) {
+  const uintptr_t a0 = (uintptr_t)src;
+  const uintptr_t b0 = a0 & ~(uintptr_t)15, b1 = (a0 + nbytes + 15) & ~(uintptr_t)15;
+  const int nb = nbytes ? (int)((b1 - b0) >> 4) : 0;
+  lead_words = (int)((a0 - b0) >> 3);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  PACES_GPTR(const unsigned char) g = (PACES_GPTR(const unsigned char))b0;
+  for (int c0 = wave * 64; c0 < nb; c0 += (int)blockDim.x)
+    if (c0 + lane < nb)
+      __builtin_amdgcn_global_load_lds((PACES_GPTR(const void))(g + (size_t)(c0 + lane) * 16),
+                                       (PACES_LPTR(void))(dst + 2 * c0), 16, 0, 0);
+  return nb;
+}
+
 template <int W>
 struct LookupSmem {
-  u64 rows[kLkWords];
+  __align__(16) u64 rows[kLkWords];
   u64 wmin[kBlock / 64][W];
   u64 wmax[kBlock / 64][W];
   int wany[kBlock / 64];
@@ -186,6 +242,18 @@ __device__ int64_t block_lower_bound(const Row<W>& q, bool valid, const u64* __r
   if (!sm.any) return 0;
   const int64_t lo = sm.lo, hi = sm.hi;
   const int64_t span = hi - lo;
+  if (span * W + 3 <= kLkWords) {  // the whole range fits: direct loads, branchless search
+    int lead;
+    dma_blocks_to_lds(book + lo * W, (size_t)span * W * 8, sm.rows, lead);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    int64_t res = 0;
+    if (valid)
+      res = lo + bound_lds<W, false>((PACES_LPTR(const unsigned char))(sm.rows + lead),
+                                     (int)span, q);
+    __syncthreads();
+    return res;
+  }
   constexpr int cap = kLkWords / W;
   int64_t stride = 1, nst = span;
   if (span > cap) {
@@ -220,6 +288,18 @@ __device__ int64_t block_lower_bound_range(const Row<W>& q, bool valid,
                                            const u64* __restrict__ book, int64_t lo, int64_t hi,
                                            LookupSmem<W>& sm) {
   const int64_t span = hi - lo;
+  if (span * W + 3 <= kLkWords) {  // the whole range fits: direct loads, branchless search
+    int lead;
+    dma_blocks_to_lds(book + lo * W, (size_t)span * W * 8, sm.rows, lead);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    int64_t res = 0;
+    if (valid)
+      res = lo + bound_lds<W, false>((PACES_LPTR(const unsigned char))(sm.rows + lead),
+                                     (int)span, q);
+    __syncthreads();
+    return res;
+  }
   constexpr int cap = kLkWords / W;
   int64_t stride = 1, nst = span;
   if (span > cap) {

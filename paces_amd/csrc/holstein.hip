@@ -521,13 +521,14 @@ __global__ void __launch_bounds__(kBlock) k_pairs_count(int64_t R, int64_t nbasi
 }
 
 // indptr of every term from the scan, and (when cursor is given) the fill cursors
+// (grid: x over the rows, y over the terms)
 __global__ void __launch_bounds__(kBlock) k_pairs_indptr(int64_t R, int nterms,
                                                          const int32_t* __restrict__ G,
                                                          CsrOut out, int32_t* __restrict__ cursor) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)nterms * R) return;
-  const int t = (int)(e / R);
-  const int64_t i = e - (int64_t)t * R;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.y;
+  if (i >= R) return;
+  const int64_t e = (int64_t)t * R + i;
   const int32_t base = G[e] - G[(int64_t)t * R];
   out.indptr[t][i] = base;
   if (cursor) cursor[e] = base;
@@ -645,29 +646,30 @@ __global__ void __launch_bounds__(kBlock) k_pairs_mark(int64_t R, int64_t nbasis
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= IR.off[IR.n]) return;
   const int k = img_run_of(IR, e);
+  // both levels of loads issued before the tests (p clamped in bounds)
   const int32_t p = cand_pos[nbasis + e];
   const int32_t q = cand_pos[src[e]];
+  const bool pin = inB[p < 0 ? 0 : p] != 0;
   if (p < 0 || q < 0) return;
-  if (T.dir[k] < 0 && inB[p]) return;
+  if (T.dir[k] < 0 && pin) return;
   const int t = T.term[k], S = SL.S[t];
   uint8_t* f = flags + R * SL.foff[t];
   f[(int64_t)q * S + (k - SL.k0[t])] = 1;
   f[(int64_t)p * S + ((k ^ 1) - SL.k0[t])] = 1;
 }
 
-// entries of every (term, row): its occupied slots
+// entries of every (term, row): its occupied slots (grid: x over the rows, y over the terms)
 __global__ void __launch_bounds__(kBlock) k_pairs_rowcount(int64_t R, int nterms, SlotArg SL,
                                                            const uint8_t* __restrict__ flags,
                                                            int32_t* __restrict__ cnt) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)nterms * R) return;
-  const int t = (int)(e / R);
-  const int64_t i = e - (int64_t)t * R;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.y;
+  if (i >= R) return;
   const int S = SL.S[t];
   const uint8_t* f = flags + R * SL.foff[t] + i * S;
   int c = 0;
   for (int j = 0; j < S; ++j) c += f[j];
-  cnt[e] = c;
+  cnt[(int64_t)t * R + i] = c;
 }
 
 __device__ __forceinline__ int32_t slot_place(const SlotArg& SL, int t, const uint8_t* f, int S,
@@ -1144,7 +1146,8 @@ int paces_holstein_pairs_count(int64_t R, int64_t nbasis, const int64_t* run_off
       hipLaunchKernelGGL(k_pairs_mark, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, R,
                          nbasis, IR, T, SL, cand_pos, in_basis, img_src, flags);
     if (m > 0)
-      hipLaunchKernelGGL(k_pairs_rowcount, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, st, R,
+      hipLaunchKernelGGL(k_pairs_rowcount, dim3(grid_for(R, kBlock), h->nterms), dim3(kBlock), 0,
+                         st, R,
                          h->nterms, SL, (const uint8_t*)flags, cnt);
   } else {
     if (m > 0) {
@@ -1205,7 +1208,7 @@ int paces_holstein_pairs_fill(int64_t R, int64_t nbasis, const int64_t* run_offs
   const int32_t* G = (const int32_t*)p;
   SlotArg SL;
   if (build_slots(T, h, &SL)) {  // the same choice as the count call
-    hipLaunchKernelGGL(k_pairs_indptr, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, st, R,
+    hipLaunchKernelGGL(k_pairs_indptr, dim3(grid_for(R, kBlock), h->nterms), dim3(kBlock), 0, st, R,
                        h->nterms, G, out, (int32_t*)nullptr);
     if (nimg > 0)
       hipLaunchKernelGGL(k_pairs_place, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, R,
@@ -1214,7 +1217,7 @@ int paces_holstein_pairs_fill(int64_t R, int64_t nbasis, const int64_t* run_offs
                          (const uint8_t*)pairs_flags(ws, R, h->nterms), out);
     return launch_status();
   }
-  hipLaunchKernelGGL(k_pairs_indptr, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, st, R,
+  hipLaunchKernelGGL(k_pairs_indptr, dim3(grid_for(R, kBlock), h->nterms), dim3(kBlock), 0, st, R,
                      h->nterms, G, out, cursor);
   if (nimg > 0)
     hipLaunchKernelGGL(k_pairs_fill, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, R, nbasis,

@@ -221,42 +221,6 @@ __global__ void __launch_bounds__(kBlock) k_mu_bounds(RunsArg R, int64_t* __rest
   lohi[e] = lower_bound_row<W>(R.ptr[s], 0, R.off[s + 1] - R.off[s], x);
 }
 
-// a < b over W words, word 0 most significant: the borrow out of the multiword a - b.
-template <int W>
-__device__ __forceinline__ bool lex_less(const u64 (&a)[W], const u64 (&b)[W]) {
-  unsigned long long borrow = 0;
-#pragma unroll
-  for (int k = W - 1; k >= 0; --k) (void)__builtin_subcll(a[k], b[k], borrow, &borrow);
-  return borrow != 0;
-}
-
-template <int W>
-__device__ __forceinline__ void load_lds_row(PACES_LPTR(const unsigned char) rows, unsigned i,
-                                             u64 (&w)[W]) {
-  const PACES_LPTR(const u64) p = (PACES_LPTR(const u64))(rows + __umul24(i, W * 8));
-#pragma unroll
-  for (int k = 0; k < W; ++k) w[k] = p[k];
-}
-
-// Bound of q in n sorted unique rows staged in LDS (byte address rows): the number of rows below
-// q, or with UPPER the number of rows at most q. Binary lifting with a trip count of
-// floor(log2 n) + 1 for every lane (n is uniform across the workgroup).
-template <int W, bool UPPER>
-__device__ __forceinline__ int bound_lds(PACES_LPTR(const unsigned char) rows, int n,
-                                         const Row<W>& q) {
-  int pos = 0;
-  for (int step = n > 0 ? (int)(1u << (31 - __builtin_clz((unsigned)n))) : 0; step > 0;
-       step >>= 1) {
-    const int probe = pos + step - 1;
-    const bool in = probe < n;
-    u64 w[W];
-    load_lds_row<W>(rows, (unsigned)(in ? probe : n - 1), w);
-    const bool pred = UPPER ? !lex_less<W>(q.w, w) : lex_less<W>(w, q.w);
-    if (in & pred) pos += step;
-  }
-  return pos;
-}
-
 // Merged-order schedule of k_mu_rank's tiles. A tile's rows land in the merged order near
 // est = its first row's index + sum over the other runs of that row's lower bound there. Tiles
 // sorted by est (buckets of 256 positions; order inside a bucket does not matter) and dealt to
