@@ -159,7 +159,8 @@ def btay_term_bytes(R, mats, mixed):
 # over the roofline loop below with the shape fixed (--term-shape), each scaled by the counter's
 # reading for a 1 GiB device copy (profiles/k_btay_term_traj256_hbm_traffic.txt). Only the
 # mixed-precision kernel is measured; other shapes report null.
-BTAY_TRAFFIC_BYTES_PER_ROW = {}
+BTAY_TRAFFIC_BYTES_PER_ROW = {("mixed", (256, 2)): 89.72, ("mixed", (256, 4)): 88.19,
+                              ("mixed", (512, 4)): 87.42}
 
 
 def measure_batched_term(bte, reps=20):
