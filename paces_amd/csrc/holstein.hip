@@ -781,75 +781,112 @@ struct ObsTile {
                               : (kObsLdsBytes / (8 * W + 12)) >= 512 ? 512 : 256;
 };
 
+// Partial sums of one tile of TR rows: the sum of weights, the weight at each exciton site and
+// the weighted phonons at each site. Wave w takes sites w, w + 4, ..., so a site's field offset and
+// width are uniform across the wave; lane l holds rows l, l + 64, ... with their weights and
+// exciton positions in registers, and a site's sum ends in a wave reduction. Only sites inside
+// the tile's range of exciton positions can hold exciton weight.
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_obs_partial(const u64* __restrict__ rows,
                                                         const double2* __restrict__ psi, int64_t R,
                                                         LayoutArg L, int N,
                                                         double* __restrict__ part) {
-  constexpr int TR = ObsTile<W>::rows;
+  constexpr int TR = ObsTile<W>::rows, RPL = TR / 64, PER = TR * W / kBlock;
   __shared__ u64 srow[TR * W];
-  __shared__ double sw[TR];
-  __shared__ int spos[TR];
-  __shared__ double sgrp[4 * 2 * PACES_MAX_SITES / 4 + 8];
-  __shared__ double swn[kBlock / 64];
+  __shared__ int spr[2];
   const int S = 2 * N + 1;
   const int64_t base = (int64_t)blockIdx.x * TR;
   const int nr = (int)((R - base) < TR ? (R - base) : TR);
-  for (int e = threadIdx.x; e < nr * W; e += blockDim.x) srow[e] = rows[base * W + e];
-  __syncthreads();
-  double wsum = 0.0;
-  for (int r = threadIdx.x; r < nr; r += blockDim.x) {
-    double2 v = psi[base + r];
-    double w = v.x * v.x + v.y * v.y;
-    sw[r] = w;
-    wsum += w;
-    Row<W> rr;
-#pragma unroll
-    for (int k = 0; k < W; ++k) rr.w[k] = srow[r * W + k];
-    spos[r] = (int)get_field<W>(rr, 0, L.width[0]);
-  }
-  wsum = wave_sum(wsum);
-  if ((threadIdx.x & 63) == 0) swn[threadIdx.x >> 6] = wsum;
-  __syncthreads();
-  // groups of gsz threads split the tile's rows; each thread owns sites gt, gt + gsz, ...
-  const int gsz = N <= 64 ? 64 : (N <= 128 ? 128 : 256);
-  const int ngrp = kBlock / gsz;
-  const int grp = threadIdx.x / gsz, gt = threadIdx.x % gsz;
-  const int r0 = (nr * grp) / ngrp, r1 = (nr * (grp + 1)) / ngrp;
-  for (int s = gt; s < N; s += gsz) {
-    const int off = L.off[s + 1], b = L.width[s + 1];
-    const int k = off >> 6, o = off & 63;
-    const bool straddle = o + b > 64;
-    const u64 mask = (b >= 64) ? ~0ull : ((1ull << b) - 1ull);
-    double ae = 0.0, ap = 0.0;
-    for (int r = r0; r < r1; ++r) {
-      const double w = sw[r];
-      ae += (spos[r] == s) ? w : 0.0;
-      u64 n;
-      if (b == 0) {
-        n = 0;
-      } else if (straddle) {
-        const u64 hi = srow[r * W + k], lo = srow[r * W + k + 1];
-        n = ((hi << (o + b - 64)) | (lo >> (128 - o - b))) & mask;
-      } else {
-        n = (srow[r * W + k] >> (64 - o - b)) & mask;
-      }
-      ap += w * (double)n;
-    }
-    sgrp[grp * 2 * N + s] = ae;
-    sgrp[grp * 2 * N + N + s] = ap;
-  }
-  __syncthreads();
-  for (int s = threadIdx.x; s < 2 * N; s += blockDim.x) {
-    double a = 0.0;
-    for (int q = 0; q < ngrp; ++q) a += sgrp[q * 2 * N + s];
-    part[(int64_t)blockIdx.x * S + s] = a;
-  }
   if (threadIdx.x == 0) {
+    spr[0] = 0x7fffffff;
+    spr[1] = -1;
+  }
+  {
+    u64 tmp[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int e = (int)threadIdx.x + j * kBlock;
+      tmp[j] = e < nr * W ? rows[base * W + e] : 0ull;
+    }
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int e = (int)threadIdx.x + j * kBlock;
+      if (e < nr * W) srow[e] = tmp[j];
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double w[RPL];
+  int pos[RPL];
+  int lo = 0x7fffffff, hi = -1;
+#pragma unroll
+  for (int j = 0; j < RPL; ++j) {
+    const int r = lane + 64 * j;
+    w[j] = 0.0;
+    pos[j] = -1;
+    if (r < nr) {
+      const double2 v = psi[base + r];
+      w[j] = v.x * v.x + v.y * v.y;
+      Row<W> rr;
+#pragma unroll
+      for (int k = 0; k < W; ++k) rr.w[k] = srow[r * W + k];
+      pos[j] = (int)get_field<W>(rr, 0, L.width[0]);
+      lo = pos[j] < lo ? pos[j] : lo;
+      hi = pos[j] > hi ? pos[j] : hi;
+    }
+  }
+  if (hi >= 0) {
+    atomicMin(&spr[0], lo);
+    atomicMax(&spr[1], hi);
+  }
+  if (wave == 0) {
     double a = 0.0;
 #pragma unroll
-    for (int q = 0; q < kBlock / 64; ++q) a += swn[q];
-    part[(int64_t)blockIdx.x * S + 2 * N] = a;
+    for (int j = 0; j < RPL; ++j) a += w[j];
+    a = wave_sum(a);
+    if (lane == 0) part[(int64_t)blockIdx.x * S + 2 * N] = a;
+  }
+  __syncthreads();
+  const int pmin = spr[0], pmax = spr[1];
+  for (int s = wave; s < N; s += kBlock / 64) {
+    const int fo = L.off[s + 1], b = L.width[s + 1];
+    const int k = fo >> 6, o = fo & 63;
+    double ap = 0.0;
+    if (b > 0 && o + b > 64) {  // the field straddles two words
+      const u64 mask = (b >= 64) ? ~0ull : ((1ull << b) - 1ull);
+#pragma unroll
+      for (int j = 0; j < RPL; ++j) {
+        const int r = lane + 64 * j;  // rows past the tile read stale LDS, weighted by 0
+        const u64 n = ((srow[r * W + k] << (o + b - 64)) | (srow[r * W + k + 1] >> (128 - o - b))) &
+                      mask;
+        ap += w[j] * (double)n;
+      }
+    } else if (b > 0 && b <= 32) {
+      const u64 mask = (1ull << b) - 1ull;
+#pragma unroll
+      for (int j = 0; j < RPL; ++j) {
+        const int r = lane + 64 * j;
+        ap += w[j] * (double)(uint32_t)((srow[r * W + k] >> (64 - o - b)) & mask);
+      }
+    } else if (b > 0) {
+      const u64 mask = (b >= 64) ? ~0ull : ((1ull << b) - 1ull);
+#pragma unroll
+      for (int j = 0; j < RPL; ++j) {
+        const int r = lane + 64 * j;
+        ap += w[j] * (double)((srow[r * W + k] >> (64 - o - b)) & mask);
+      }
+    }
+    ap = wave_sum(ap);
+    double ae = 0.0;
+    if (s >= pmin && s <= pmax) {
+#pragma unroll
+      for (int j = 0; j < RPL; ++j) ae += (pos[j] == s) ? w[j] : 0.0;
+      ae = wave_sum(ae);
+    }
+    if (lane == 0) {
+      part[(int64_t)blockIdx.x * S + s] = ae;
+      part[(int64_t)blockIdx.x * S + N + s] = ap;
+    }
   }
 }
 
