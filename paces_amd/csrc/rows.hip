@@ -194,6 +194,21 @@ __device__ __forceinline__ int run_of(const RunsArg& R, int64_t g) {
   return r;
 }
 
+// the address of candidate g's row and its run, by a uniform loop of selects over the runs (no
+// per-lane indexing of the argument struct, which would be a dependent load)
+template <int W>
+__device__ __forceinline__ const u64* cand_row(const RunsArg& R, int64_t g, int& run) {
+  const u64* p = R.ptr[0] + g * W;
+  int r = 0;
+  for (int s = 1; s < R.K; ++s) {
+    const bool in = g >= R.off[s];
+    p = in ? R.ptr[s] + (g - R.off[s]) * W : p;
+    r = in ? s : r;
+  }
+  run = r;
+  return p;
+}
+
 __device__ __forceinline__ int run_of_block(const RunsArg& R, int64_t b) {
   int r = 0;
   for (int s = 1; s < R.K; ++s) r += (b >= R.boff[s]) ? 1 : 0;
@@ -457,6 +472,15 @@ __global__ void __launch_bounds__(kBlock) k_mu_emit(RunsArg R, const uint32_t* _
   }
   __syncthreads();
   const int base = offs[blockIdx.x];
+  // every step's row is loaded before any is stored (a position without a kept row reads
+  // candidate 0's row, unused), so the gathers overlap
+  Row<W> x[kSteps];
+  int rr[kSteps];
+#pragma unroll
+  for (int k = 0; k < kSteps; ++k) {
+    const int gc = (f[k] && g[k] >= 0) ? g[k] : 0;
+    x[k] = load_row<W>(cand_row<W>(R, gc, rr[k]), 0);
+  }
 #pragma unroll
   for (int k = 0; k < kSteps; ++k) {
     if (g[k] < 0) continue;  // no candidate landed here (only for runs that repeat a row)
@@ -464,10 +488,8 @@ __global__ void __launch_bounds__(kBlock) k_mu_emit(RunsArg R, const uint32_t* _
     const int pos = f[k] ? G : G - 1;
     if (cand_pos) cand_pos[g[k]] = pos;
     if (f[k]) {
-      const int r = run_of(R, g[k]);
-      const Row<W> x = load_row<W>(R.ptr[r], g[k] - R.off[r]);
-      store_row<W>(out, pos, x);
-      if (from0) from0[pos] = (r == 0) ? 1 : 0;
+      store_row<W>(out, pos, x[k]);
+      if (from0) from0[pos] = (rr[k] == 0) ? 1 : 0;
     }
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *out_count = offs[gridDim.x];
