@@ -120,22 +120,27 @@ __device__ __forceinline__ void load_lds_row(PACES_LPTR(const unsigned char) row
 }
 
 // Bound of q in n sorted unique rows staged in LDS (byte address rows): the number of rows below
-// q, or with UPPER the number of rows at most q. Binary lifting with a trip count of
-// floor(log2 n) + 1 for every lane (n is uniform across the workgroup).
+// q, or with UPPER the number of rows at most q. Halving search: the window [base, base + len)
+// holds the answer's neighbourhood, every probe (base + len/2) is in bounds, and the window
+// lengths depend on n only, so every lane takes ceil(log2 n) steps plus one final compare
+// (n is uniform across the workgroup).
 template <int W, bool UPPER>
 __device__ __forceinline__ int bound_lds(PACES_LPTR(const unsigned char) rows, int n,
                                          const Row<W>& q) {
-  int pos = 0;
-  for (int step = n > 0 ? (int)(1u << (31 - __builtin_clz((unsigned)n))) : 0; step > 0;
-       step >>= 1) {
-    const int probe = pos + step - 1;
-    const bool in = probe < n;
+  if (n <= 0) return 0;
+  int base = 0;
+  for (int len = n; len > 1;) {
+    const int half = len >> 1;
     u64 w[W];
-    load_lds_row<W>(rows, (unsigned)(in ? probe : n - 1), w);
+    load_lds_row<W>(rows, (unsigned)(base + half), w);
     const bool pred = UPPER ? !lex_less<W>(q.w, w) : lex_less<W>(w, q.w);
-    if (in & pred) pos += step;
+    base = pred ? base + half : base;
+    len -= half;
   }
-  return pos;
+  u64 w[W];
+  load_lds_row<W>(rows, (unsigned)base, w);
+  const bool pred = UPPER ? !lex_less<W>(q.w, w) : lex_less<W>(w, q.w);
+  return base + (pred ? 1 : 0);
 }
 
 // Copy the 16-byte blocks holding bytes [src, src + nbytes) global -> LDS at dst (16-byte
