@@ -103,11 +103,19 @@ __global__ void k_bsel_keys(const double2* __restrict__ psi, int64_t R, u64* __r
   keys[i] = (u64)__double_as_longlong(k);
 }
 
+// Radix select over the 64-bit keys in digits of up to 11 bits, most significant first: five
+// 11-bit passes and a last 9-bit one (6 passes), with 2048-bin per-trajectory histograms.
+constexpr int kSelBits = 11;
+constexpr int kSelBins = 1 << kSelBits;
+constexpr int kSelPasses = 6;
+__host__ __device__ constexpr int sel_shift(int pass) { return pass < 5 ? 53 - 11 * pass : 0; }
+__host__ __device__ constexpr int sel_width(int pass) { return pass < 5 ? 11 : 9; }
+
 __global__ void k_bsel_init(const int32_t* __restrict__ seg_off, const int64_t* __restrict__ kpt,
                             int ntraj, SegSel* __restrict__ st, unsigned* __restrict__ hist) {
   const int t = blockIdx.x;
   if (t >= ntraj) return;
-  hist[(int64_t)t * 256 + threadIdx.x] = 0;
+  for (int d = threadIdx.x; d < kSelBins; d += blockDim.x) hist[(int64_t)t * kSelBins + d] = 0;
   if (threadIdx.x == 0) {
     const long long Rt = seg_off[t + 1] - seg_off[t];
     SegSel s;
@@ -129,49 +137,79 @@ __global__ void __launch_bounds__(kBlock) k_bsel_hist(const u64* __restrict__ ke
                                                       const int32_t* __restrict__ seg_off,
                                                       const int64_t* __restrict__ boff, int ntraj,
                                                       const SegSel* __restrict__ st, int shift,
-                                                      unsigned* __restrict__ hist) {
-  __shared__ unsigned lh[256];
+                                                      int width, unsigned* __restrict__ hist) {
+  __shared__ unsigned lh[kSelBins];
   const SegTile T = seg_tile(seg_off, boff, ntraj, kSelTile);
   if (T.t < 0 || st[T.t].all || st[T.t].krem == 0) return;
-  lh[threadIdx.x] = 0;
+  for (int d = threadIdx.x; d < kSelBins; d += kBlock) lh[d] = 0;
   __syncthreads();
   const u64 prefix = st[T.t].prefix, mask = st[T.t].mask;
+  const u64 dm = (1ull << width) - 1ull;
+  u64 k[kSelRpt];
+#pragma unroll
+  for (int q = 0; q < kSelRpt; ++q) {  // every key loaded before any is counted
+    const int64_t i = T.r0 + q * kBlock + threadIdx.x;
+    k[q] = keys[i < T.r1 ? i : T.r0];
+  }
 #pragma unroll
   for (int q = 0; q < kSelRpt; ++q) {
     const int64_t i = T.r0 + q * kBlock + threadIdx.x;
-    if (i < T.r1) {
-      const u64 k = keys[i];
-      if ((k & mask) == prefix) atomicAdd(&lh[(k >> shift) & 255u], 1u);
-    }
+    if (i < T.r1 && (k[q] & mask) == prefix) atomicAdd(&lh[(k[q] >> shift) & dm], 1u);
   }
   __syncthreads();
-  unsigned c = lh[threadIdx.x];
-  if (c) atomicAdd(&hist[(int64_t)T.t * 256 + threadIdx.x], c);
+  for (int d = threadIdx.x; d < kSelBins; d += kBlock) {
+    const unsigned c = lh[d];
+    if (c) atomicAdd(&hist[(int64_t)T.t * kSelBins + d], c);
+  }
 }
 
+// the digit holding the krem-th largest remaining key: per thread 8 consecutive bins, a suffix
+// scan over the threads, then the thread whose bins cross krem finds the digit
 __global__ void __launch_bounds__(256) k_bsel_pick(SegSel* __restrict__ st,
-                                                   unsigned* __restrict__ hist, int shift) {
+                                                   unsigned* __restrict__ hist, int shift,
+                                                   int width) {
+  constexpr int kPer = kSelBins / 256;
   __shared__ long long suf[257];
-  const int t = blockIdx.x, d = threadIdx.x;
-  unsigned* h = hist + (int64_t)t * 256;
+  const int t = blockIdx.x, d0 = threadIdx.x * kPer;
+  unsigned* h = hist + (int64_t)t * kSelBins;
   const bool skip = st[t].all || st[t].krem == 0;
-  suf[d] = skip ? 0 : h[d];
-  if (d == 0) suf[256] = 0;
+  unsigned c[kPer];
+  long long tot = 0;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    c[j] = skip ? 0u : h[d0 + j];
+    tot += c[j];
+  }
+  suf[threadIdx.x] = tot;
+  if (threadIdx.x == 0) suf[256] = 0;
   __syncthreads();
-  if (d == 0)
+  if (threadIdx.x == 0)
     for (int u = 255; u >= 0; --u) suf[u] += suf[u + 1];
   __syncthreads();
   if (!skip) {
     const long long krem = st[t].krem;
-    const bool mine = (suf[d] >= krem) && (suf[d + 1] < krem);
+    // keys in bins above this thread's: suf[tid + 1]; walk this thread's bins from the top
+    long long above = suf[threadIdx.x + 1];
+    int pick = -1;
+    long long above_pick = 0;
+#pragma unroll
+    for (int j = kPer - 1; j >= 0; --j) {
+      const long long with = above + c[j];
+      if (pick < 0 && with >= krem && above < krem) {
+        pick = d0 + j;
+        above_pick = above;
+      }
+      above = with;
+    }
     __syncthreads();
-    if (mine) {
-      st[t].prefix |= ((u64)d) << shift;
-      st[t].mask |= ((u64)255) << shift;
-      st[t].krem = krem - suf[d + 1];
+    if (pick >= 0) {
+      st[t].prefix |= ((u64)pick) << shift;
+      st[t].mask |= ((1ull << width) - 1ull) << shift;
+      st[t].krem = krem - above_pick;
     }
   }
-  h[d] = 0;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) h[d0 + j] = 0;
 }
 
 __global__ void __launch_bounds__(kBlock) k_bsel_ties(const u64* __restrict__ keys,
@@ -223,7 +261,7 @@ __global__ void __launch_bounds__(kBlock) k_bsel_compact(const u64* __restrict__
 }
 
 size_t bsel_ws_bytes(int64_t R, int ntraj) {
-  return a256((size_t)ntraj * sizeof(SegSel)) + a256((size_t)ntraj * 256 * 4) +
+  return a256((size_t)ntraj * sizeof(SegSel)) + a256((size_t)ntraj * kSelBins * 4) +
          2 * a256((size_t)(ntraj + 1) * 8) + a256((size_t)R * 8) + 2 * a256((size_t)R) +
          2 * a256((size_t)(R + 1) * 4) + a256(scan_ws_bytes(R));
 }
@@ -236,7 +274,7 @@ static int run_bselect(const double* psi, const uint64_t* rows, int64_t R,
   SegSel* s = (SegSel*)p;
   p += a256((size_t)ntraj * sizeof(SegSel));
   unsigned* hist = (unsigned*)p;
-  p += a256((size_t)ntraj * 256 * 4);
+  p += a256((size_t)ntraj * kSelBins * 4);
   int64_t* boff = (int64_t*)p;
   p += a256((size_t)(ntraj + 1) * 8);
   int64_t* boffh = (int64_t*)p;  // workgroup offsets of the histogram passes' larger tiles
@@ -259,11 +297,11 @@ static int run_bselect(const double* psi, const uint64_t* rows, int64_t R,
     hipLaunchKernelGGL(k_bsel_keys, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
                        (const double2*)psi, R, keys);
   hipLaunchKernelGGL(k_bsel_init, dim3(ntraj), dim3(256), 0, st, seg_off, kpt, ntraj, s, hist);
-  for (int pass = 0; pass < 8; ++pass) {
-    const int shift = 56 - 8 * pass;
+  for (int pass = 0; pass < kSelPasses; ++pass) {
+    const int shift = sel_shift(pass), width = sel_width(pass);
     hipLaunchKernelGGL(k_bsel_hist, dim3(gsh), dim3(kBlock), 0, st, (const u64*)keys, seg_off,
-                       (const int64_t*)boffh, ntraj, (const SegSel*)s, shift, hist);
-    hipLaunchKernelGGL(k_bsel_pick, dim3(ntraj), dim3(256), 0, st, s, hist, shift);
+                       (const int64_t*)boffh, ntraj, (const SegSel*)s, shift, width, hist);
+    hipLaunchKernelGGL(k_bsel_pick, dim3(ntraj), dim3(256), 0, st, s, hist, shift, width);
   }
   hipLaunchKernelGGL(k_bsel_ties, dim3(gs), dim3(kBlock), 0, st, (const u64*)keys, seg_off,
                      (const int64_t*)boff, ntraj, (const SegSel*)s, tie);

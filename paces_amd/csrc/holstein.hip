@@ -668,7 +668,12 @@ __global__ void __launch_bounds__(kBlock) k_pairs_rowcount(int64_t R, int nterms
   const int S = SL.S[t];
   const uint8_t* f = flags + R * SL.foff[t] + i * S;
   int c = 0;
-  for (int j = 0; j < S; ++j) c += f[j];
+  if (S == 2) {  // slot counts are even, so a term's flags start 2-byte aligned
+    const uint16_t v = *(const uint16_t*)f;
+    c = (v & 0xff) + (v >> 8);
+  } else {
+    for (int j = 0; j < S; ++j) c += f[j];
+  }
   cnt[(int64_t)t * R + i] = c;
 }
 
