@@ -13,6 +13,7 @@
 namespace paces {
 
 int scan_u8(const uint8_t* in, int64_t n, int32_t* out, void* ws, size_t ws_bytes, hipStream_t st);
+int scan_i32(const int32_t* in, int64_t n, int32_t* out, void* ws, size_t ws_bytes, hipStream_t st);
 size_t scan_ws_bytes(int64_t n);
 
 static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -212,86 +213,231 @@ __global__ void __launch_bounds__(256) k_bsel_pick(SegSel* __restrict__ st,
   for (int j = 0; j < kPer; ++j) h[d0 + j] = 0;
 }
 
-__global__ void __launch_bounds__(kBlock) k_bsel_ties(const u64* __restrict__ keys,
-                                                      const int32_t* __restrict__ seg_off,
-                                                      const int64_t* __restrict__ boff, int ntraj,
-                                                      const SegSel* __restrict__ st,
-                                                      uint8_t* __restrict__ tie) {
-  const SegTile T = seg_tile(seg_off, boff, ntraj, kTile);
-  if (T.t < 0) return;
-  const int64_t i = T.r0 + threadIdx.x;
-  if (i >= T.r1) return;
-  tie[i] = (!st[T.t].all && keys[i] == st[T.t].prefix) ? 1 : 0;
+// ---- the selection's output: the keys above each trajectory's threshold key, and the last krem
+// of the keys equal to it (among equal keys the later rows are kept: the stable order of the
+// reference's sort). Three passes over kSelTile-row tiles of one trajectory each:
+// 1. k_bsel_count: the tile's numbers of keys above and equal to the threshold;
+// 2. k_bsel_tiles (one workgroup per trajectory): each tile's tie offset within its trajectory
+//    and its kept count; a scan of the kept counts then gives each tile's output offset;
+// 3. k_bsel_emit: each tile writes its kept rows in row order (in-tile prefixes from ballots).
+// A thread takes rows r0 + q kBlock + x (q < kSelRpt), so row order is step-major.
+
+__global__ void __launch_bounds__(kBlock) k_bsel_count(const u64* __restrict__ keys,
+                                                       const int32_t* __restrict__ seg_off,
+                                                       const int64_t* __restrict__ boff, int ntraj,
+                                                       const SegSel* __restrict__ st,
+                                                       int2* __restrict__ tc,
+                                                       int32_t* __restrict__ kc) {
+  __shared__ int sg[kBlock / 64], se[kBlock / 64];
+  const SegTile T = seg_tile(seg_off, boff, ntraj, kSelTile);
+  if (T.t < 0) {
+    if (threadIdx.x == 0) kc[blockIdx.x] = 0;  // tiles past the last trajectory keep nothing
+    return;
+  }
+  const SegSel s = st[T.t];
+  u64 k[kSelRpt];
+#pragma unroll
+  for (int q = 0; q < kSelRpt; ++q) {
+    const int64_t i = T.r0 + q * kBlock + threadIdx.x;
+    k[q] = keys[i < T.r1 ? i : T.r0];
+  }
+  int ng = 0, ne = 0;
+#pragma unroll
+  for (int q = 0; q < kSelRpt; ++q) {
+    const int64_t i = T.r0 + q * kBlock + threadIdx.x;
+    if (i < T.r1) {
+      ng += (s.all || k[q] > s.prefix) ? 1 : 0;
+      ne += (!s.all && k[q] == s.prefix) ? 1 : 0;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    ng += __shfl_xor(ng, o, 64);
+    ne += __shfl_xor(ne, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sg[threadIdx.x >> 6] = ng;
+    se[threadIdx.x >> 6] = ne;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int a = 0, b = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+      a += sg[w];
+      b += se[w];
+    }
+    tc[blockIdx.x] = make_int2(a, b);
+  }
 }
 
-__global__ void __launch_bounds__(kBlock) k_bsel_keep(const u64* __restrict__ keys,
-                                                      const int32_t* __restrict__ seg_off,
-                                                      const int64_t* __restrict__ boff, int ntraj,
-                                                      const SegSel* __restrict__ st,
-                                                      const int32_t* __restrict__ TG,
-                                                      uint8_t* __restrict__ keep) {
-  const SegTile T = seg_tile(seg_off, boff, ntraj, kTile);
-  if (T.t < 0) return;
-  const int64_t i = T.r0 + threadIdx.x;
-  if (i >= T.r1) return;
-  const SegSel s = st[T.t];
-  bool kp;
-  if (s.all) {
-    kp = true;
-  } else {
-    const int64_t a = seg_off[T.t], b = seg_off[T.t + 1];
-    const long long nt = TG[b] - TG[a];
-    const long long rank = TG[i] - TG[a];
-    const u64 k = keys[i];
-    kp = (k > s.prefix) || (k == s.prefix && rank >= nt - s.krem);
+// one workgroup per trajectory: tie offsets (exclusive, within the trajectory) and kept counts of
+// its tiles, and the tie rank from which ties are kept (thr = ties - krem)
+__global__ void __launch_bounds__(256) k_bsel_tiles(const int64_t* __restrict__ boff,
+                                                    const SegSel* __restrict__ st,
+                                                    const int2* __restrict__ tc,
+                                                    int32_t* __restrict__ TO,
+                                                    int32_t* __restrict__ kc,
+                                                    long long* __restrict__ thr) {
+  __shared__ int wsum[4];
+  __shared__ long long carry_s;
+  const int t = blockIdx.x;
+  const int64_t b0 = boff[t], b1 = boff[t + 1];
+  const SegSel s = st[t];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) carry_s = 0;
+  __syncthreads();
+  for (int64_t c0 = b0; c0 < b1; c0 += 256) {  // exclusive scan of the tie counts
+    const int64_t b = c0 + threadIdx.x;
+    const int v = b < b1 ? tc[b].y : 0;
+    int inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += u;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    int before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      before += w < wave ? wsum[w] : 0;
+      total += wsum[w];
+    }
+    const long long carry = carry_s;
+    if (b < b1) TO[b] = (int32_t)(carry + before + inc - v);
+    __syncthreads();
+    if (threadIdx.x == 0) carry_s = carry + total;
+    __syncthreads();
   }
-  keep[i] = kp ? 1 : 0;
+  const long long th = s.all ? 0 : carry_s - s.krem;
+  if (threadIdx.x == 0) thr[t] = th;
+  for (int64_t b = b0 + threadIdx.x; b < b1; b += 256) {
+    const int2 c = tc[b];
+    int keep = c.x;
+    if (!s.all) {
+      const long long lo = TO[b], hi = lo + c.y;  // this tile's tie ranks [lo, hi)
+      const long long from = lo > th ? lo : th;
+      keep += (int)(hi > from ? hi - from : 0);
+    }
+    kc[b] = keep;
+  }
 }
 
 template <int W>
-__global__ void __launch_bounds__(kBlock) k_bsel_compact(const u64* __restrict__ rows, int64_t R,
-                                                         const uint8_t* __restrict__ keep,
-                                                         const int32_t* __restrict__ G,
-                                                         u64* __restrict__ out,
-                                                         int32_t* __restrict__ out_count) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *out_count = G[R];
-  if (i >= R || !keep[i]) return;
-  store_row<W>(out, G[i], load_row<W>(rows, i));
+__global__ void __launch_bounds__(kBlock) k_bsel_emit(const u64* __restrict__ keys,
+                                                      const u64* __restrict__ rows,
+                                                      const int32_t* __restrict__ seg_off,
+                                                      const int64_t* __restrict__ boff, int ntraj,
+                                                      const SegSel* __restrict__ st,
+                                                      const int32_t* __restrict__ TO,
+                                                      const long long* __restrict__ thr,
+                                                      const int32_t* __restrict__ KO, int ntiles,
+                                                      u64* __restrict__ out,
+                                                      int32_t* __restrict__ out_count) {
+  constexpr int kWaves = kBlock / 64;
+  __shared__ int wt[kSelRpt * kWaves];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *out_count = KO[ntiles];
+  const SegTile T = seg_tile(seg_off, boff, ntraj, kSelTile);
+  if (T.t < 0) return;
+  const SegSel s = st[T.t];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  u64 k[kSelRpt];
+  Row<W> x[kSelRpt];
+  bool vv[kSelRpt];
+#pragma unroll
+  for (int q = 0; q < kSelRpt; ++q) {
+    const int64_t i = T.r0 + q * kBlock + threadIdx.x;
+    vv[q] = i < T.r1;
+    const int64_t ic = vv[q] ? i : T.r0;
+    k[q] = keys[ic];
+    x[q] = load_row<W>(rows, ic);
+  }
+  // tie ranks: exclusive prefix of the tie flags in row order, plus the tile's tie offset
+  int pre[kSelRpt];
+  bool tie[kSelRpt];
+#pragma unroll
+  for (int q = 0; q < kSelRpt; ++q) {
+    tie[q] = vv[q] && !s.all && k[q] == s.prefix;
+    const u64 bl = __ballot(tie[q]);
+    pre[q] = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bl >> 32),
+                                            __builtin_amdgcn_mbcnt_lo((unsigned)bl, 0u));
+    if (lane == 0) wt[q * kWaves + wave] = __popcll(bl);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int u = 0; u < kSelRpt * kWaves; ++u) {
+      const int c = wt[u];
+      wt[u] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  const long long th = thr[T.t];
+  const int to = TO[blockIdx.x];
+  bool keep[kSelRpt];
+#pragma unroll
+  for (int q = 0; q < kSelRpt; ++q) {
+    const long long rank = (long long)to + wt[q * kWaves + wave] + pre[q];
+    keep[q] = vv[q] && (s.all || k[q] > s.prefix || (tie[q] && rank >= th));
+  }
+  __syncthreads();  // the tie prefixes are read; wt is reused for the kept counts
+#pragma unroll
+  for (int q = 0; q < kSelRpt; ++q) {
+    const u64 bl = __ballot(keep[q]);
+    pre[q] = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bl >> 32),
+                                            __builtin_amdgcn_mbcnt_lo((unsigned)bl, 0u));
+    if (lane == 0) wt[q * kWaves + wave] = __popcll(bl);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int u = 0; u < kSelRpt * kWaves; ++u) {
+      const int c = wt[u];
+      wt[u] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  const int ko = KO[blockIdx.x];
+#pragma unroll
+  for (int q = 0; q < kSelRpt; ++q)
+    if (keep[q]) store_row<W>(out, ko + wt[q * kWaves + wave] + pre[q], x[q]);
 }
 
 size_t bsel_ws_bytes(int64_t R, int ntraj) {
+  const int64_t nt = seg_grid(R, ntraj, kSelTile);
   return a256((size_t)ntraj * sizeof(SegSel)) + a256((size_t)ntraj * kSelBins * 4) +
-         2 * a256((size_t)(ntraj + 1) * 8) + a256((size_t)R * 8) + 2 * a256((size_t)R) +
-         2 * a256((size_t)(R + 1) * 4) + a256(scan_ws_bytes(R));
+         a256((size_t)(ntraj + 1) * 8) + a256((size_t)R * 8) + a256((size_t)nt * 8) +
+         2 * a256((size_t)nt * 4) + a256((size_t)(nt + 1) * 4) + a256((size_t)ntraj * 8) +
+         a256(scan_ws_bytes(nt));
 }
 
 template <int W>
 static int run_bselect(const double* psi, const uint64_t* rows, int64_t R,
                        const int32_t* seg_off, int ntraj, const int64_t* kpt, uint64_t* out_rows,
                        int32_t* out_count, void* ws, hipStream_t st) {
+  const unsigned gsh = seg_grid(R, ntraj, kSelTile);
   char* p = (char*)ws;
   SegSel* s = (SegSel*)p;
   p += a256((size_t)ntraj * sizeof(SegSel));
   unsigned* hist = (unsigned*)p;
   p += a256((size_t)ntraj * kSelBins * 4);
-  int64_t* boff = (int64_t*)p;
-  p += a256((size_t)(ntraj + 1) * 8);
-  int64_t* boffh = (int64_t*)p;  // workgroup offsets of the histogram passes' larger tiles
+  int64_t* boffh = (int64_t*)p;  // workgroup offsets of the kSelTile-row tiles
   p += a256((size_t)(ntraj + 1) * 8);
   u64* keys = (u64*)p;
   p += a256((size_t)R * 8);
-  uint8_t* tie = (uint8_t*)p;
-  p += a256((size_t)R);
-  uint8_t* keep = (uint8_t*)p;
-  p += a256((size_t)R);
-  int32_t* TG = (int32_t*)p;
-  p += a256((size_t)(R + 1) * 4);
-  int32_t* KG = (int32_t*)p;
-  p += a256((size_t)(R + 1) * 4);
-  const unsigned gs = seg_grid(R, ntraj, kTile);
-  const unsigned gsh = seg_grid(R, ntraj, kSelTile);
-  hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(1), 0, st, seg_off, ntraj, kTile, boff);
+  int2* tc = (int2*)p;
+  p += a256((size_t)gsh * 8);
+  int32_t* TO = (int32_t*)p;
+  p += a256((size_t)gsh * 4);
+  int32_t* kc = (int32_t*)p;
+  p += a256((size_t)gsh * 4);
+  int32_t* KO = (int32_t*)p;
+  p += a256((size_t)(gsh + 1) * 4);
+  long long* thr = (long long*)p;
+  p += a256((size_t)ntraj * 8);
   hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(1), 0, st, seg_off, ntraj, kSelTile, boffh);
   if (R > 0)
     hipLaunchKernelGGL(k_bsel_keys, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
@@ -303,21 +449,18 @@ static int run_bselect(const double* psi, const uint64_t* rows, int64_t R,
                        (const int64_t*)boffh, ntraj, (const SegSel*)s, shift, width, hist);
     hipLaunchKernelGGL(k_bsel_pick, dim3(ntraj), dim3(256), 0, st, s, hist, shift, width);
   }
-  hipLaunchKernelGGL(k_bsel_ties, dim3(gs), dim3(kBlock), 0, st, (const u64*)keys, seg_off,
-                     (const int64_t*)boff, ntraj, (const SegSel*)s, tie);
+  hipLaunchKernelGGL(k_bsel_count, dim3(gsh), dim3(kBlock), 0, st, (const u64*)keys, seg_off,
+                     (const int64_t*)boffh, ntraj, (const SegSel*)s, tc, kc);
+  hipLaunchKernelGGL(k_bsel_tiles, dim3(ntraj), dim3(256), 0, st, (const int64_t*)boffh,
+                     (const SegSel*)s, (const int2*)tc, TO, kc, thr);
   int rc = launch_status();
   if (rc) return rc;
-  rc = scan_u8(tie, R, TG, p, scan_ws_bytes(R), st);
+  rc = scan_i32(kc, gsh, KO, p, scan_ws_bytes(gsh), st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_bsel_keep, dim3(gs), dim3(kBlock), 0, st, (const u64*)keys, seg_off,
-                     (const int64_t*)boff, ntraj, (const SegSel*)s, (const int32_t*)TG, keep);
-  rc = launch_status();
-  if (rc) return rc;
-  rc = scan_u8(keep, R, KG, p, scan_ws_bytes(R), st);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_bsel_compact<W>, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
-                     (const u64*)rows, R, (const uint8_t*)keep, (const int32_t*)KG, (u64*)out_rows,
-                     out_count);
+  hipLaunchKernelGGL(k_bsel_emit<W>, dim3(gsh), dim3(kBlock), 0, st, (const u64*)keys,
+                     (const u64*)rows, seg_off, (const int64_t*)boffh, ntraj, (const SegSel*)s,
+                     (const int32_t*)TO, (const long long*)thr, (const int32_t*)KO, (int)gsh,
+                     (u64*)out_rows, out_count);
   return launch_status();
 }
 
