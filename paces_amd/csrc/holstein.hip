@@ -658,23 +658,102 @@ __global__ void __launch_bounds__(kBlock) k_pairs_mark(int64_t R, int64_t nbasis
   f[(int64_t)p * S + ((k ^ 1) - SL.k0[t])] = 1;
 }
 
-// entries of every (term, row): its occupied slots (grid: x over the rows, y over the terms)
-__global__ void __launch_bounds__(kBlock) k_pairs_rowcount(int64_t R, int nterms, SlotArg SL,
-                                                           const uint8_t* __restrict__ flags,
-                                                           int32_t* __restrict__ cnt) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int t = blockIdx.y;
-  if (i >= R) return;
-  const int S = SL.S[t];
-  const uint8_t* f = flags + R * SL.foff[t] + i * S;
-  int c = 0;
+// Row counts of the slot path in tiles of kPairTile rows: the count pass sums each tile's slot
+// flags per term (a scan of those sums gives every tile's first entry and each term's nnz), the
+// indptr pass then writes each row's start from its tile's offset plus the in-tile prefix. A
+// thread takes kPairRpt consecutive rows. Grids: x over the tiles, y over the terms.
+constexpr int kPairRpt = 8;  // (the indptr stores below are written for 8)
+constexpr int kPairTile = kBlock * kPairRpt;
+static_assert(kPairRpt == 8, "indptr stores");
+
+__device__ __forceinline__ int slot_row_count(const uint8_t* f, int S) {
   if (S == 2) {  // slot counts are even, so a term's flags start 2-byte aligned
     const uint16_t v = *(const uint16_t*)f;
-    c = (v & 0xff) + (v >> 8);
-  } else {
-    for (int j = 0; j < S; ++j) c += f[j];
+    return (v & 0xff) + (v >> 8);
   }
-  cnt[(int64_t)t * R + i] = c;
+  int c = 0;
+  for (int j = 0; j < S; ++j) c += f[j];
+  return c;
+}
+
+__global__ void __launch_bounds__(kBlock) k_pairs_tilesum(int64_t R, SlotArg SL,
+                                                          const uint8_t* __restrict__ flags,
+                                                          int32_t* __restrict__ ts) {
+  __shared__ int ws[kBlock / 64];
+  const int t = blockIdx.y;
+  const int S = SL.S[t];
+  const uint8_t* f = flags + R * SL.foff[t];
+  const int64_t r0 = (int64_t)blockIdx.x * kPairTile + (int64_t)threadIdx.x * kPairRpt;
+  int c = 0;
+#pragma unroll
+  for (int j = 0; j < kPairRpt; ++j)
+    if (r0 + j < R) c += slot_row_count(f + (r0 + j) * S, S);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int a = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) a += ws[w];
+    ts[(int64_t)t * gridDim.x + blockIdx.x] = a;
+  }
+}
+
+__global__ void k_pairs_tile_nnz(const int32_t* __restrict__ TS, int ntl, int nterms,
+                                 int32_t* __restrict__ nnz) {
+  const int t = threadIdx.x;
+  if (t < nterms) nnz[t] = TS[(int64_t)(t + 1) * ntl] - TS[(int64_t)t * ntl];
+}
+
+__global__ void __launch_bounds__(kBlock) k_pairs_indptr_tiles(int64_t R, SlotArg SL,
+                                                               const uint8_t* __restrict__ flags,
+                                                               const int32_t* __restrict__ TS,
+                                                               CsrOut out) {
+  __shared__ int wsum[kBlock / 64];
+  const int t = blockIdx.y;
+  const int ntl = gridDim.x;
+  const int S = SL.S[t];
+  const uint8_t* f = flags + R * SL.foff[t];
+  const int64_t r0 = (int64_t)blockIdx.x * kPairTile + (int64_t)threadIdx.x * kPairRpt;
+  int c[kPairRpt];
+  int tot = 0;
+#pragma unroll
+  for (int j = 0; j < kPairRpt; ++j) {
+    c[j] = (r0 + j < R) ? slot_row_count(f + (r0 + j) * S, S) : 0;
+    tot += c[j];
+  }
+  // exclusive prefix of the thread totals across the workgroup
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int inc = tot;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += u;
+  }
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  int before = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) before += w < wave ? wsum[w] : 0;
+  int run = TS[(int64_t)t * ntl + blockIdx.x] - TS[(int64_t)t * ntl] + before + inc - tot;
+  int32_t* ip = out.indptr[t];
+  int o[kPairRpt];
+#pragma unroll
+  for (int j = 0; j < kPairRpt; ++j) {
+    o[j] = run;
+    run += c[j];
+  }
+  if (r0 + kPairRpt <= R && (((uintptr_t)(ip + r0)) & 15) == 0) {  // two 16-byte stores
+    *(int4*)(ip + r0) = make_int4(o[0], o[1], o[2], o[3]);
+    *(int4*)(ip + r0 + 4) = make_int4(o[4], o[5], o[6], o[7]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < kPairRpt; ++j)
+      if (r0 + j < R) ip[r0 + j] = o[j];
+  }
+  if (r0 <= R - 1 && R - 1 < r0 + kPairRpt)
+    ip[R] = TS[(int64_t)(t + 1) * ntl] - TS[(int64_t)t * ntl];
 }
 
 __device__ __forceinline__ int32_t slot_place(const SlotArg& SL, int t, const uint8_t* f, int S,
@@ -1187,10 +1266,23 @@ int paces_holstein_pairs_count(int64_t R, int64_t nbasis, const int64_t* run_off
     if (nimg > 0)
       hipLaunchKernelGGL(k_pairs_mark, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, R,
                          nbasis, IR, T, SL, cand_pos, in_basis, img_src, flags);
-    if (m > 0)
-      hipLaunchKernelGGL(k_pairs_rowcount, dim3(grid_for(R, kBlock), h->nterms), dim3(kBlock), 0,
-                         st, R,
-                         h->nterms, SL, (const uint8_t*)flags, cnt);
+    // tile sums (in the count area) and their scan (in the prefix area) stand in for the row
+    // counts and their scan; the fill call reads the same places
+    const int ntl = (int)((R + kPairTile - 1) / kPairTile);
+    if (R > 0) {
+      hipLaunchKernelGGL(k_pairs_tilesum, dim3(ntl, h->nterms), dim3(kBlock), 0, st, R, SL,
+                         (const uint8_t*)flags, cnt);
+      rc = launch_status();
+      if (rc) return rc;
+      rc = scan_i32(cnt, (int64_t)ntl * h->nterms, G, p,
+                    scan_ws_bytes((int64_t)ntl * h->nterms), st);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_pairs_tile_nnz, dim3(1), dim3(64), 0, st, (const int32_t*)G, ntl,
+                         h->nterms, term_nnz);
+    } else {
+      hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(kBlock), 0, st, term_nnz, (int64_t)h->nterms);
+    }
+    return launch_status();
   } else {
     if (m > 0) {
       hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, st, cnt, m);
@@ -1250,8 +1342,9 @@ int paces_holstein_pairs_fill(int64_t R, int64_t nbasis, const int64_t* run_offs
   const int32_t* G = (const int32_t*)p;
   SlotArg SL;
   if (build_slots(T, h, &SL)) {  // the same choice as the count call
-    hipLaunchKernelGGL(k_pairs_indptr, dim3(grid_for(R, kBlock), h->nterms), dim3(kBlock), 0, st, R,
-                       h->nterms, G, out, (int32_t*)nullptr);
+    const int ntl = (int)((R + kPairTile - 1) / kPairTile);
+    hipLaunchKernelGGL(k_pairs_indptr_tiles, dim3(ntl, h->nterms), dim3(kBlock), 0, st, R, SL,
+                       (const uint8_t*)pairs_flags(ws, R, h->nterms), G, out);
     if (nimg > 0)
       hipLaunchKernelGGL(k_pairs_place, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, R,
                          nbasis, IR, T, SL, marg(h), (const u64*)basis, W, trajbits, cand_pos,
