@@ -348,15 +348,16 @@ __device__ __forceinline__ void field_words(const Row<W>& r, int k, u64& hi, u64
 
 template <int W>
 __device__ __forceinline__ u64 get_field(const Row<W>& r, int off, int b) {
-  if (b == 0) return 0;
+  // branch-free (selects only), so loads of later rows are not held behind this one's branches
   const int k = off >> 6, o = off & 63;
   u64 hi, lo;
   field_words<W>(r, k, hi, lo);
-  const u64 mask = (b == 64) ? ~0ull : ((1ull << b) - 1ull);
-  const int end = o + b;  // 1..127
-  if (end <= 64) return (hi >> (64 - end)) & mask;
-  const int s = end - 64;  // field bits in word k+1, 1..63
-  return ((hi << s) | (lo >> (64 - s))) & mask;
+  const u64 mask = (b >= 64) ? ~0ull : ((1ull << (b & 63)) - 1ull);  // 0 for an empty field
+  const int end = o + b;   // 0..127
+  const int s = end - 64;  // when end > 64: the field's bits in word k+1, 1..63
+  const u64 one = hi >> ((64 - end) & 63);
+  const u64 two = (hi << (s & 63)) | (lo >> ((64 - s) & 63));
+  return (end <= 64 ? one : two) & mask;
 }
 
 // add a signed delta to a field; the caller guarantees the result stays inside the field, so the

@@ -23,12 +23,14 @@ constexpr int kMaxOps = 32;
 // Vib ops add dir to the phonon count at the exciton site.
 struct OpTable {
   int n;
-  int8_t kind[kMaxOps];
-  int8_t dir[kMaxOps];
-  int8_t wrap[kMaxOps];
-  int8_t term[kMaxOps];
-  int8_t dim[kMaxOps];
-  int8_t order[kMaxOps];
+  int has_vib;  // some op is a vib op (the phonon fields exist)
+  // 32-bit fields: a uniform dynamic index then reads them with scalar loads
+  int32_t kind[kMaxOps];
+  int32_t dir[kMaxOps];
+  int32_t wrap[kMaxOps];
+  int32_t term[kMaxOps];
+  int32_t dim[kMaxOps];
+  int32_t order[kMaxOps];
   int32_t oe[kMaxOps];
   int32_t span[kMaxOps];
   double value[kMaxOps];
@@ -66,6 +68,7 @@ static ModelArg marg(const paces_holstein_t* h) {
 // sorted.
 static int build_image_ops(const paces_holstein_t* h, OpTable* T) {
   T->n = 0;
+  T->has_vib = 0;
   for (int t = 0; t < h->nterms; ++t) {
     const paces_term_t& tm = h->terms[t];
     if (tm.kind == PACES_TERM_HOP) {
@@ -94,6 +97,7 @@ static int build_image_ops(const paces_holstein_t* h, OpTable* T) {
         if (T->n >= kMaxOps) return PACES_E_UNSUPPORTED;
         int k = T->n++;
         T->kind[k] = PACES_TERM_VIB;
+        T->has_vib = 1;
         T->dir[k] = (int8_t)dir;
         T->wrap[k] = 0;
         T->term[k] = (int8_t)t;
@@ -113,6 +117,7 @@ static int build_image_ops(const paces_holstein_t* h, OpTable* T) {
 // Matrix-element neighbours (one per direction; the periodic wrap is folded in), grouped by term.
 static int build_nb_ops(const paces_holstein_t* h, OpTable* T) {
   T->n = 0;
+  T->has_vib = 0;
   for (int t = 0; t < h->nterms; ++t) {
     const paces_term_t& tm = h->terms[t];
     if (tm.kind == PACES_TERM_HOP) {
@@ -138,6 +143,7 @@ static int build_nb_ops(const paces_holstein_t* h, OpTable* T) {
         if (T->n >= kMaxOps) return PACES_E_UNSUPPORTED;
         int k = T->n++;
         T->kind[k] = PACES_TERM_VIB;
+        T->has_vib = 1;
         T->dir[k] = (int8_t)dir;
         T->wrap[k] = 0;
         T->term[k] = (int8_t)t;
@@ -160,58 +166,165 @@ struct LayoutArg {
   int sum_bits;
 };
 
-// Apply op k to row r. Returns false when the image does not exist (boundary or ceiling).
-// `nup` receives the upper phonon count of a vib pair (for the matrix element).
+// What every op of a row reads: the exciton position, the row's lattice length (batched) and the
+// phonon field at the exciton. A kernel that applies several ops to one row forms it once.
+struct RowCtx {
+  int pos;
+  int Lt;
+  int voff, vb, vmax, n;
+};
+
+// Offset and width of the exciton-position and trajectory-id fields, read once per kernel as
+// wave-uniform values. (The compiler cannot keep table reads in scalar registers across the
+// kernel's own stores, which might alias the table, so kernels that store read them up front.)
+struct RowFields {
+  int po, pb, to, tb;
+};
+
+__device__ __forceinline__ RowFields row_fields(const LayoutArg& L, const ModelArg& M) {
+  RowFields F;
+  F.po = __builtin_amdgcn_readfirstlane(L.off[M.pf]);
+  F.pb = __builtin_amdgcn_readfirstlane(L.width[M.pf]);
+  F.to = M.batched ? __builtin_amdgcn_readfirstlane(L.off[0]) : 0;
+  F.tb = M.batched ? __builtin_amdgcn_readfirstlane(L.width[0]) : 0;
+  return F;
+}
+
+template <int W>
+__device__ __forceinline__ RowCtx row_ctx(const OpTable& T, const LayoutArg& L, const ModelArg& M,
+                                          const RowFields& F, const Row<W>& r) {
+  RowCtx c;
+  c.pos = (int)get_field<W>(r, F.po, F.pb);
+  c.Lt = M.batched ? M.traj_L[get_field<W>(r, F.to, F.tb)] : 0;
+  c.voff = c.vb = c.vmax = c.n = 0;
+  if (T.has_vib) {  // the phonon fields exist only for models with a vib term
+    const int f = M.pf + 1 + c.pos;
+    c.voff = L.off[f];
+    c.vb = L.width[f];
+    c.vmax = L.maxdim[f];
+    c.n = (int)get_field<W>(r, c.voff, c.vb);
+  }
+  return c;
+}
+
+// row_ctx for NQ rows at once, in phases (field extraction, then every table gather, then the
+// phonon counts), so that the gathers of all NQ rows are in flight together
+template <int W, int NQ>
+__device__ __forceinline__ void row_ctx_n(const OpTable& T, const LayoutArg& L, const ModelArg& M,
+                                          const RowFields& F, const Row<W> (&r)[NQ],
+                                          RowCtx (&c)[NQ]) {
+  int tr[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    c[q].pos = (int)get_field<W>(r[q], F.po, F.pb);
+    tr[q] = (int)get_field<W>(r[q], F.to, F.tb);
+    c[q].Lt = 0;
+    c[q].voff = c[q].vb = c[q].vmax = c[q].n = 0;
+  }
+  if (M.batched) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) c[q].Lt = M.traj_L[tr[q]];
+  }
+  if (T.has_vib) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int f = M.pf + 1 + c[q].pos;
+      c[q].voff = L.off[f];
+      c[q].vb = L.width[f];
+      c[q].vmax = L.maxdim[f];
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) c[q].n = (int)get_field<W>(r[q], c[q].voff, c[q].vb);
+  }
+}
+
+// The fields of op k, read once per op as wave-uniform values (scalar registers), so a loop over
+// rows does not reload them.
+struct OpView {
+  int kind, dir, wrap, dim, order, oe, span;
+  int po, pb;  // offset and width of the exciton-position field
+};
+
+__device__ __forceinline__ OpView op_view(const OpTable& T, int k, const RowFields& F) {
+  OpView o;
+  o.po = F.po;
+  o.pb = F.pb;
+  o.kind = __builtin_amdgcn_readfirstlane((int)T.kind[k]);
+  o.dir = __builtin_amdgcn_readfirstlane((int)T.dir[k]);
+  o.wrap = __builtin_amdgcn_readfirstlane((int)T.wrap[k]);
+  o.dim = __builtin_amdgcn_readfirstlane((int)T.dim[k]);
+  o.order = __builtin_amdgcn_readfirstlane((int)T.order[k]);
+  o.oe = __builtin_amdgcn_readfirstlane(T.oe[k]);
+  o.span = __builtin_amdgcn_readfirstlane(T.span[k]);
+  return o;
+}
+
+// whether op o has an image of the row whose context is c, and (for a hop) the new position
+__device__ __forceinline__ bool op_target(const OpView& o, const ModelArg& M, const RowCtx& c,
+                                          int& npos) {
+  npos = c.pos;
+  if (o.kind == PACES_TERM_HOP) {
+    int oe, span;
+    if (M.batched) {
+      int Ld = 1;
+      for (int d = 0; d < o.dim; ++d) Ld *= c.Lt;
+      oe = o.order * Ld;
+      span = Ld * c.Lt;
+      if (oe >= span) return false;  // hop order longer than this trajectory's lattice
+    } else {
+      oe = o.oe;
+      span = o.span;
+    }
+    int mod = c.pos;
+    if (mod >= span) mod = c.pos % span;  // the outermost dimension needs no division
+    const bool inside = (o.dir > 0) ? (mod < span - oe) : (mod >= oe);
+    const int step = o.dir * oe;
+    const int wrapped = step - o.dir * span;
+    if (o.wrap == 0) {
+      npos = c.pos + step;
+      return inside;
+    }
+    if (o.wrap == 1) {
+      npos = c.pos + wrapped;
+      return !inside;
+    }
+    npos = c.pos + (inside ? step : wrapped);
+    return true;
+  }
+  return o.dir > 0 ? c.n + 1 < c.vmax : c.n > 0;
+}
+
+__device__ __forceinline__ bool op_exists(const OpView& o, const ModelArg& M, const RowCtx& c) {
+  int npos;
+  return op_target(o, M, c, npos);
+}
+
+// Apply op o to row r, whose context is c. Returns false when the image does not exist (boundary
+// or ceiling). `nup` receives the upper phonon count of a vib pair (for the matrix element).
+template <int W>
+__device__ __forceinline__ bool apply_op_ctx(const OpView& o, const LayoutArg& L,
+                                             const ModelArg& M, const RowCtx& c, Row<W>& r,
+                                             int& nup) {
+  int npos;
+  if (!op_target(o, M, c, npos)) return false;
+  if (o.kind == PACES_TERM_HOP) {
+    add_field<W>(r, o.po, o.pb, (int64_t)(npos - c.pos));
+    nup = 0;
+  } else if (o.dir > 0) {
+    add_field<W>(r, c.voff, c.vb, 1);
+    nup = c.n + 1;
+  } else {
+    add_field<W>(r, c.voff, c.vb, -1);
+    nup = c.n;
+  }
+  return true;
+}
+
 template <int W>
 __device__ __forceinline__ bool apply_op(const OpTable& T, int k, const LayoutArg& L,
                                          const ModelArg& M, Row<W>& r, int& nup) {
-  const int pf = M.pf;
-  const int po = L.off[pf], pb = L.width[pf];
-  const int pos = (int)get_field<W>(r, po, pb);
-  if (T.kind[k] == PACES_TERM_HOP) {
-    int oe, span;
-    if (M.batched) {
-      const int Lt = M.traj_L[get_field<W>(r, L.off[0], L.width[0])];
-      int Ld = 1;
-      for (int d = 0; d < T.dim[k]; ++d) Ld *= Lt;
-      oe = T.order[k] * Ld;
-      span = Ld * Lt;
-      if (oe >= span) return false;  // hop order longer than this trajectory's lattice
-    } else {
-      oe = T.oe[k];
-      span = T.span[k];
-    }
-    const int dir = T.dir[k];
-    const int mod = pos % span;
-    int npos;
-    bool inside = (dir > 0) ? (mod < span - oe) : (mod >= oe);
-    if (T.wrap[k] == 0) {
-      if (!inside) return false;
-      npos = pos + dir * oe;
-    } else if (T.wrap[k] == 1) {
-      if (inside) return false;
-      npos = pos + dir * oe - dir * span;
-    } else {
-      npos = inside ? pos + dir * oe : pos + dir * oe - dir * span;
-    }
-    add_field<W>(r, po, pb, (int64_t)(npos - pos));
-    nup = 0;
-    return true;
-  } else {
-    const int f = pf + 1 + pos;
-    const int off = L.off[f], b = L.width[f];
-    const int n = (int)get_field<W>(r, off, b);
-    if (T.dir[k] > 0) {
-      if (n + 1 >= L.maxdim[f]) return false;
-      add_field<W>(r, off, b, 1);
-      nup = n + 1;
-    } else {
-      if (n <= 0) return false;
-      add_field<W>(r, off, b, -1);
-      nup = n;
-    }
-    return true;
-  }
+  const RowFields F = row_fields(L, M);
+  return apply_op_ctx<W>(op_view(T, k, F), L, M, row_ctx<W>(T, L, M, F, r), r, nup);
 }
 
 // matrix-element value of op k for a row of trajectory traj
@@ -222,92 +335,164 @@ __device__ __forceinline__ double op_value(const OpTable& T, int k, const ModelA
 }
 
 // ------------------------------------------------------------------------------------------------
-// Image runs: flags -> scan -> emit
+// Image runs: tile counts -> scan -> tile emit
 // ------------------------------------------------------------------------------------------------
 
-template <int W>
-__global__ void __launch_bounds__(kBlock) k_img_flags(const u64* __restrict__ basis, int64_t n,
-                                                      OpTable T, LayoutArg L, ModelArg M,
-                                                      uint8_t* __restrict__ flags) {
+__global__ void __launch_bounds__(kBlock) k_zero_i32(int32_t* __restrict__ p, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const Row<W> r0 = load_row<W>(basis, i);
-  for (int k = 0; k < T.n; ++k) {
-    Row<W> r = r0;
-    int nup;
-    flags[(int64_t)k * n + i] = apply_op<W>(T, k, L, M, r, nup) ? 1 : 0;
-  }
+  if (i < n) p[i] = 0;
 }
 
-__global__ void k_img_offsets(const int32_t* __restrict__ G, int64_t n, int nops,
-                              int32_t* __restrict__ run_offsets) {
-  int k = threadIdx.x;
-  if (k <= nops) run_offsets[k] = G[(int64_t)k * n];
-}
+// Image runs in wave tiles of kImgTile basis rows (lane x of the wave takes rows
+// t0 + q 64 + x, q < kImgSteps, so the tile's row order is step-major). The count pass counts each
+// tile's valid images per op; a scan of those counts in op-major order gives each (op, tile) its
+// first output index, and the run offsets. The emit pass applies the ops again and writes each
+// tile's images of op k in row order from there, with in-wave prefixes from ballots: no barrier
+// in either pass. The per-row reads every op shares (exciton position, lattice length, phonon
+// field) are formed once per row.
+constexpr int kImgSteps = 8;
+constexpr int kImgTile = 64 * kImgSteps;
+constexpr int kImgWaves = kBlock / 64;
+// rows of up to kImgHoldWords words are loaded for all steps up front; wider ones step by step
+constexpr int kImgHoldWords = 4;
 
-template <int W>
-__global__ void __launch_bounds__(kBlock) k_img_emit(const u64* __restrict__ basis, int64_t n,
-                                                     OpTable T, LayoutArg L, ModelArg M,
-                                                     const uint8_t* __restrict__ flags,
-                                                     const int32_t* __restrict__ G,
-                                                     u64* __restrict__ out,
-                                                     int32_t* __restrict__ out_src,
-                                                     uint8_t* __restrict__ out_nup) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const Row<W> r0 = load_row<W>(basis, i);
-  // ops in groups of 4: every flag and output index of a group is loaded (unconditionally, in
-  // bounds) before any is used, so the loads overlap
-  constexpr int kG = 4;
-  for (int k0 = 0; k0 < T.n; k0 += kG) {
-    uint8_t fl[kG];
-    int32_t og[kG];
-#pragma unroll
-    for (int j = 0; j < kG; ++j) {
-      const int k = (k0 + j < T.n) ? k0 + j : k0;
-      const int64_t f = (int64_t)k * n + i;
-      fl[j] = flags[f];
-      og[j] = G[f];
-    }
-#pragma unroll
-    for (int j = 0; j < kG; ++j) {
-      const int k = k0 + j;
-      if (k >= T.n || !fl[j]) continue;
-      Row<W> r = r0;
-      int nup = 0;
-      apply_op<W>(T, k, L, M, r, nup);
-      const int64_t o = og[j];
-      store_row<W>(out, o, r);
-      if (out_src) out_src[o] = (int32_t)i;
-      if (out_nup) out_nup[o] = (uint8_t)nup;
-    }
-  }
-}
+static int64_t img_tiles(int64_t n) { return (n + kImgTile - 1) / kImgTile; }
 
 static size_t images_ws(int64_t n, int nops) {
-  int64_t m = (int64_t)nops * n;
-  return a256((size_t)m) + a256((size_t)(m + 1) * 4) + a256(scan_ws_bytes(m));
+  const int64_t m = (int64_t)nops * (img_tiles(n) > 0 ? img_tiles(n) : 1);
+  return a256((size_t)m * 4) + a256((size_t)(m + 1) * 4) + a256(scan_ws_bytes(m));
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_img_tcount(const u64* __restrict__ basis, int64_t n,
+                                                       int64_t ntl, OpTable T, LayoutArg L,
+                                                       ModelArg M, int32_t* __restrict__ tcnt) {
+  const int64_t wt = (int64_t)blockIdx.x * kImgWaves + (threadIdx.x >> 6);
+  if (wt >= ntl) return;  // whole waves
+  const int lane = threadIdx.x & 63;
+  const RowFields F = row_fields(L, M);
+  RowCtx cx[kImgSteps];
+  bool vv[kImgSteps];
+  if (W <= kImgHoldWords) {  // every row's load in flight before any is used
+    Row<W> rr[kImgSteps];
+#pragma unroll
+    for (int q = 0; q < kImgSteps; ++q) {
+      const int64_t i = wt * kImgTile + q * 64 + lane;
+      vv[q] = i < n;
+      rr[q] = load_row<W>(basis, vv[q] ? i : 0);
+    }
+    row_ctx_n<W, kImgSteps>(T, L, M, F, rr, cx);
+  } else {
+#pragma unroll
+    for (int q = 0; q < kImgSteps; ++q) {
+      const int64_t i = wt * kImgTile + q * 64 + lane;
+      vv[q] = i < n;
+      cx[q] = row_ctx<W>(T, L, M, F, load_row<W>(basis, vv[q] ? i : 0));
+    }
+  }
+  for (int k = 0; k < T.n; ++k) {
+    const OpView o = op_view(T, k, F);
+    int c = 0;
+#pragma unroll
+    for (int q = 0; q < kImgSteps; ++q) c += __popcll(__ballot(op_exists(o, M, cx[q]) & vv[q]));
+    if (lane == 0) tcnt[(int64_t)k * ntl + wt] = c;
+  }
+}
+
+__global__ void k_img_run_offsets(const int32_t* __restrict__ TS, int64_t ntl, int nops,
+                                  int32_t* __restrict__ run_offsets) {
+  const int k = threadIdx.x;
+  if (k <= nops) run_offsets[k] = TS[(int64_t)k * ntl];
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_img_temit(const u64* __restrict__ basis, int64_t n,
+                                                      int64_t ntl, OpTable T, LayoutArg L,
+                                                      ModelArg M, const int32_t* __restrict__ TS,
+                                                      u64* __restrict__ out,
+                                                      int32_t* __restrict__ out_src,
+                                                      uint8_t* __restrict__ out_nup) {
+  constexpr bool kHold = W <= kImgHoldWords;
+  constexpr int kH = kHold ? kImgSteps : 1;
+  // per wave: the next output index of each op, and the staging of one step's images (so the
+  // global stores are lane-contiguous words)
+  __shared__ int nxt[kImgWaves][kMaxOps];
+  __shared__ u64 stage[kImgWaves][64 * W];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t wt = (int64_t)blockIdx.x * kImgWaves + wave;
+  if (wt >= ntl) return;  // whole waves
+  const RowFields F = row_fields(L, M);
+  if (lane < T.n) nxt[wave][lane] = TS[(int64_t)lane * ntl + wt];
+  Row<W> rh[kH];
+  RowCtx ch[kH];
+  if (kHold) {  // every row and its context before the first store
+#pragma unroll
+    for (int q = 0; q < kImgSteps; ++q) {
+      const int64_t i = wt * kImgTile + q * 64 + lane;
+      rh[kHold ? q : 0] = load_row<W>(basis, i < n ? i : 0);
+    }
+    row_ctx_n<W, kH>(T, L, M, F, rh, ch);
+  }
+  u64* const sw = stage[wave];
+#pragma unroll
+  for (int q = 0; q < kImgSteps; ++q) {
+    const int64_t i = wt * kImgTile + q * 64 + lane;
+    const bool v = i < n;
+    const Row<W> r0 = kHold ? rh[kHold ? q : 0] : load_row<W>(basis, v ? i : 0);
+    const RowCtx cx = kHold ? ch[kHold ? q : 0] : row_ctx<W>(T, L, M, F, r0);
+    for (int k = 0; k < T.n; ++k) {
+      const OpView o = op_view(T, k, F);
+      const bool ok = op_exists(o, M, cx) & v;
+      const u64 bl = __ballot(ok);
+      if (bl == 0) continue;  // wave-uniform
+      const int cnt = __popcll(bl);
+      const int pre = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bl >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((unsigned)bl, 0u));
+      const int o0 = nxt[wave][k];
+      if (ok) {
+        Row<W> r = r0;
+        int nup = 0;
+        apply_op_ctx<W>(o, L, M, cx, r, nup);
+#pragma unroll
+        for (int w = 0; w < W; ++w) sw[pre * W + w] = r.w[w];
+        if (out_src) out_src[o0 + pre] = (int32_t)i;
+        if (out_nup) out_nup[o0 + pre] = (uint8_t)nup;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      u64* const dst = out + (int64_t)o0 * W;
+      for (int j = lane; j < cnt * W; j += 64) dst[j] = sw[j];
+      if (lane == 0) nxt[wave][k] = o0 + cnt;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
 }
 
 template <int W>
 static int run_img_count(const uint64_t* basis, int64_t n, const LayoutArg& L, const ModelArg& M,
                          const OpTable& T, int32_t* run_offsets, void* ws, size_t ws_bytes,
                          hipStream_t st) {
-  int64_t m = (int64_t)T.n * n;
+  const int64_t ntl = img_tiles(n) > 0 ? img_tiles(n) : 1;
+  const int64_t m = (int64_t)T.n * ntl;
   char* p = (char*)ws;
-  uint8_t* flags = (uint8_t*)p;
-  p += a256((size_t)m);
-  int32_t* G = (int32_t*)p;
+  int32_t* tcnt = (int32_t*)p;
+  p += a256((size_t)m * 4);
+  int32_t* TS = (int32_t*)p;
   p += a256((size_t)(m + 1) * 4);
   if (n > 0) {
-    hipLaunchKernelGGL(k_img_flags<W>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
-                       (const u64*)basis, n, T, L, M, flags);
-    int rc = launch_status();
-    if (rc) return rc;
+    hipLaunchKernelGGL(k_img_tcount<W>, dim3(grid_for(ntl, kImgWaves)), dim3(kBlock), 0, st,
+                       (const u64*)basis, n, ntl, T, L, M, tcnt);
+  } else {
+    hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, st, tcnt, m);
   }
-  int rc = scan_u8(flags, m, G, p, scan_ws_bytes(m), st);
+  int rc = launch_status();
   if (rc) return rc;
-  hipLaunchKernelGGL(k_img_offsets, dim3(1), dim3(64), 0, st, (const int32_t*)G, n, T.n,
+  rc = scan_i32(tcnt, m, TS, p, scan_ws_bytes(m), st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_img_run_offsets, dim3(1), dim3(64), 0, st, (const int32_t*)TS, ntl, T.n,
                      run_offsets);
   return launch_status();
 }
@@ -317,13 +502,13 @@ static int run_img_emit(const uint64_t* basis, int64_t n, const LayoutArg& L, co
                         const OpTable& T, uint64_t* out, int32_t* out_src, uint8_t* out_nup,
                         void* ws, hipStream_t st) {
   if (n == 0) return PACES_OK;
-  int64_t m = (int64_t)T.n * n;
+  const int64_t ntl = img_tiles(n);
+  const int64_t m = (int64_t)T.n * ntl;
   char* p = (char*)ws;
-  const uint8_t* flags = (const uint8_t*)p;
-  p += a256((size_t)m);
-  const int32_t* G = (const int32_t*)p;
-  hipLaunchKernelGGL(k_img_emit<W>, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st,
-                     (const u64*)basis, n, T, L, M, flags, G, (u64*)out, out_src, out_nup);
+  p += a256((size_t)m * 4);
+  const int32_t* TS = (const int32_t*)p;
+  hipLaunchKernelGGL(k_img_temit<W>, dim3(grid_for(ntl, kImgWaves)), dim3(kBlock), 0, st,
+                     (const u64*)basis, n, ntl, T, L, M, TS, (u64*)out, out_src, out_nup);
   return launch_status();
 }
 
@@ -371,10 +556,6 @@ __global__ void __launch_bounds__(kBlock) k_csr_count(const u64* __restrict__ ro
     for (int t = 0; t < nterms; ++t) cnt[(int64_t)t * R + i] = c[t];
 }
 
-__global__ void __launch_bounds__(kBlock) k_zero_i32(int32_t* __restrict__ p, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
 
 __global__ void k_csr_nnz(const int32_t* __restrict__ G, int64_t R, int nterms,
                           int32_t* __restrict__ nnz) {
