@@ -189,6 +189,16 @@ int paces_holstein_pairs_fill(int64_t R, int64_t nbasis, const int64_t* run_offs
                               const paces_holstein_t* h, int32_t* const* indptr,
                               int32_t* const* indices, double* const* data, void* ws,
                               size_t ws_bytes, void* stream);
+/* The same entries with FP32 values (each the FP64 value rounded to nearest), for the
+ * mixed-precision series, which reads FP32 values only (no reference counterpart: the reference's
+ * matrices are FP64). */
+int paces_holstein_pairs_fill32(int64_t R, int64_t nbasis, const int64_t* run_offsets,
+                                const int32_t* cand_pos, const uint8_t* in_basis,
+                                const int32_t* img_src, const uint8_t* img_nup,
+                                const uint64_t* basis, const paces_layout_t* lay,
+                                const paces_holstein_t* h, int32_t* const* indptr,
+                                int32_t* const* indices, float* const* data, void* ws,
+                                size_t ws_bytes, void* stream);
 
 /* ---- D1/D2: per-term canonical CSR of the Holstein H on the new basis (generate_mel +
  * _add_hc_and_generate_coo + coo_matrix().tocsr(): hamiltonian.py:383-547,

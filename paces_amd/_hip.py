@@ -133,6 +133,10 @@ _SIGNATURES = {
                                             _P, C.POINTER(Layout), C.POINTER(Holstein),
                                             C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), _P, _SZ,
                                             _P]),
+    "paces_holstein_pairs_fill32": (C.c_int, [_I64, _I64, C.POINTER(C.c_int64), _P, _P, _P, _P,
+                                              _P, C.POINTER(Layout), C.POINTER(Holstein),
+                                              C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), _P,
+                                              _SZ, _P]),
     "paces_holstein_csr_workspace_bytes": (_SZ, [_I64, C.POINTER(Holstein)]),
     "paces_holstein_csr_count": (C.c_int, [_P, _I64, _P, C.POINTER(Layout), C.POINTER(Holstein),
                                            _P, _P, _SZ, _P]),

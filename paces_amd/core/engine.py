@@ -18,7 +18,8 @@ lib = _hip.load()
 
 
 class CsrMatrix:
-    """Canonical CSR of one Hamiltonian term (int32 indptr/indices, float64 data)."""
+    """Canonical CSR of one Hamiltonian term (int32 indptr/indices, float64 data; float32 data
+    for the batched mixed-precision engine, which reads nothing else)."""
 
     def __init__(self, indptr, indices, data, shape):
         self.indptr = indptr
@@ -33,7 +34,9 @@ class CsrMatrix:
     def dot(self, x):
         """M.x for a complex128 vector (the cuSPARSE product the reference calls)."""
         zero = torch.zeros(self.shape[0], dtype=torch.float64, device=x.device)
-        return spmv(zero, [self], x)
+        m = self if self.data.dtype == torch.float64 else \
+            CsrMatrix(self.indptr, self.indices, self.data.double(), self.shape)
+        return spmv(zero, [m], x)
 
     def to_scipy(self):
         import scipy.sparse
@@ -47,6 +50,8 @@ def csr_set(mats):
     if len(mats) > _hip.MAX_TERMS:
         raise ValueError("too many Hamiltonian terms")
     for t, m in enumerate(mats):
+        if m.data.dtype != torch.float64:
+            raise ValueError("the FP64 engine needs float64 CSR values")
         s.indptr[t] = m.indptr.data_ptr()
         s.indices[t] = m.indices.data_ptr() if m.nnz else m.indptr.data_ptr()
         s.data[t] = m.data.data_ptr() if m.nnz else m.indptr.data_ptr()
@@ -279,6 +284,9 @@ TAYLOR_TOL_MIXED = 5.960464477539063e-08  # 2**-24: FP32 terms carry no more
 
 
 def to_f32(t):
+    """FP32 copy of an FP64 tensor (round to nearest); an FP32 tensor is returned as is."""
+    if t.dtype == torch.float32:
+        return t
     out = torch.empty(t.shape, dtype=torch.float32, device=t.device)
     n = t.numel()
     if n:
@@ -287,7 +295,8 @@ def to_f32(t):
 
 
 class MixedMats:
-    """FP32 copies of the diagonal and of each term's CSR values for the mixed-precision series."""
+    """FP32 copies of the diagonal and of each term's CSR values for the mixed-precision series
+    (values built in FP32 already are used as they are)."""
 
     def __init__(self, diag, mats):
         self.diag = to_f32(diag)

@@ -567,7 +567,16 @@ struct CsrOut {
   int32_t* indptr[PACES_MAX_TERMS];
   int32_t* indices[PACES_MAX_TERMS];
   double* data[PACES_MAX_TERMS];
+  float* data32[PACES_MAX_TERMS];  // when set, FP32 values (rounded from the FP64 value) instead
 };
+
+// entry value v of term t at position i, in the output's precision
+__device__ __forceinline__ void put_val(const CsrOut& o, int t, int64_t i, double v) {
+  if (o.data32[t])
+    o.data32[t][i] = (float)v;
+  else
+    o.data[t][i] = v;
+}
 
 // Emits each row's entries of every term in ascending column order by repeated minimum
 // selection over the (at most 8) neighbour slots of the term; the slots were filled by
@@ -601,7 +610,7 @@ __global__ void __launch_bounds__(kBlock) k_csr_fill(int64_t R, OpTable T, Model
       }
       if (best == 0x7fffffff) break;
       out.indices[t][base + e] = best;
-      out.data[t][base + e] = bv;
+      put_val(out, t, base + e, bv);
       last = best;
     }
   }
@@ -738,10 +747,10 @@ __global__ void __launch_bounds__(kBlock) k_pairs_fill(int64_t R, int64_t nbasis
   const double v = op_value(T, k, M, traj, sqrt_tab, nup[e]);
   const int32_t a = atomicAdd(&cursor[(int64_t)t * R + q], 1);
   out.indices[t][a] = p;
-  out.data[t][a] = v;
+  put_val(out, t, a, v);
   const int32_t b = atomicAdd(&cursor[(int64_t)t * R + p], 1);
   out.indices[t][b] = q;
-  out.data[t][b] = v;
+  put_val(out, t, b, v);
 }
 
 // selection sort of each row's (few) entries by column, in place
@@ -752,17 +761,24 @@ __global__ void __launch_bounds__(kBlock) k_pairs_sort(int64_t R, int nterms, Cs
     const int32_t b = out.indptr[t][i], e = out.indptr[t][i + 1];
     int32_t* ix = out.indices[t];
     double* dv = out.data[t];
+    float* fv = out.data32[t];
     for (int32_t a = b; a + 1 < e; ++a) {
       int32_t m = a;
       for (int32_t c = a + 1; c < e; ++c)
         if (ix[c] < ix[m]) m = c;
       if (m != a) {
         const int32_t tj = ix[a];
-        const double tv = dv[a];
         ix[a] = ix[m];
-        dv[a] = dv[m];
         ix[m] = tj;
-        dv[m] = tv;
+        if (fv) {
+          const float tv = fv[a];
+          fv[a] = fv[m];
+          fv[m] = tv;
+        } else {
+          const double tv = dv[a];
+          dv[a] = dv[m];
+          dv[m] = tv;
+        }
       }
     }
   }
@@ -992,9 +1008,9 @@ __global__ void __launch_bounds__(kBlock) k_pairs_place(int64_t R, int64_t nbasi
     }
   }
   out.indices[t][a] = p;
-  out.data[t][a] = v;
+  put_val(out, t, a, v);
   out.indices[t][b] = q;
-  out.data[t][b] = v;
+  put_val(out, t, b, v);
 }
 
 static size_t pairs_ws(int64_t R, int nterms, int nops) {
@@ -1482,15 +1498,15 @@ int paces_holstein_pairs_count(int64_t R, int64_t nbasis, const int64_t* run_off
   return launch_status();
 }
 
-int paces_holstein_pairs_fill(int64_t R, int64_t nbasis, const int64_t* run_offsets,
-                              const int32_t* cand_pos, const uint8_t* in_basis,
-                              const int32_t* img_src, const uint8_t* img_nup,
-                              const uint64_t* basis, const paces_layout_t* lay,
-                              const paces_holstein_t* h, int32_t* const* indptr,
-                              int32_t* const* indices, double* const* data, void* ws,
-                              size_t ws_bytes, void* stream) {
+static int pairs_fill_impl(int64_t R, int64_t nbasis, const int64_t* run_offsets,
+                           const int32_t* cand_pos, const uint8_t* in_basis,
+                           const int32_t* img_src, const uint8_t* img_nup, const uint64_t* basis,
+                           const paces_holstein_t* h, const paces_layout_t* lay,
+                           int32_t* const* indptr, int32_t* const* indices, double* const* data,
+                           float* const* data32, void* ws, size_t ws_bytes, void* stream) {
   OpTable T;
-  if (!h || !run_offsets || !ws || !indptr || !indices || !data || R < 0) return PACES_E_INVALID;
+  if (!h || !run_offsets || !ws || !indptr || !indices || (!data && !data32) || R < 0)
+    return PACES_E_INVALID;
   if (h->batched && (!basis || !lay || !h->d_traj_value)) return PACES_E_INVALID;
   const int W = lay ? lay->W : 1;
   const int trajbits = h->batched ? tbits(h->ntraj) : 0;
@@ -1508,7 +1524,8 @@ int paces_holstein_pairs_fill(int64_t R, int64_t nbasis, const int64_t* run_offs
   for (int t = 0; t < PACES_MAX_TERMS; ++t) {
     out.indptr[t] = t < h->nterms ? indptr[t] : nullptr;
     out.indices[t] = t < h->nterms ? indices[t] : nullptr;
-    out.data[t] = t < h->nterms ? data[t] : nullptr;
+    out.data[t] = (data && t < h->nterms) ? data[t] : nullptr;
+    out.data32[t] = (data32 && t < h->nterms) ? data32[t] : nullptr;
     if (t < h->nterms && !out.indptr[t]) return PACES_E_INVALID;
   }
   hipStream_t st = (hipStream_t)stream;
@@ -1542,6 +1559,30 @@ int paces_holstein_pairs_fill(int64_t R, int64_t nbasis, const int64_t* run_offs
   hipLaunchKernelGGL(k_pairs_sort, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st, R, h->nterms,
                      out);
   return launch_status();
+}
+
+int paces_holstein_pairs_fill(int64_t R, int64_t nbasis, const int64_t* run_offsets,
+                              const int32_t* cand_pos, const uint8_t* in_basis,
+                              const int32_t* img_src, const uint8_t* img_nup,
+                              const uint64_t* basis, const paces_layout_t* lay,
+                              const paces_holstein_t* h, int32_t* const* indptr,
+                              int32_t* const* indices, double* const* data, void* ws,
+                              size_t ws_bytes, void* stream) {
+  if (!data) return PACES_E_INVALID;
+  return pairs_fill_impl(R, nbasis, run_offsets, cand_pos, in_basis, img_src, img_nup, basis, h,
+                         lay, indptr, indices, data, nullptr, ws, ws_bytes, stream);
+}
+
+int paces_holstein_pairs_fill32(int64_t R, int64_t nbasis, const int64_t* run_offsets,
+                                const int32_t* cand_pos, const uint8_t* in_basis,
+                                const int32_t* img_src, const uint8_t* img_nup,
+                                const uint64_t* basis, const paces_layout_t* lay,
+                                const paces_holstein_t* h, int32_t* const* indptr,
+                                int32_t* const* indices, float* const* data, void* ws,
+                                size_t ws_bytes, void* stream) {
+  if (!data) return PACES_E_INVALID;
+  return pairs_fill_impl(R, nbasis, run_offsets, cand_pos, in_basis, img_src, img_nup, basis, h,
+                         lay, indptr, indices, nullptr, data, ws, ws_bytes, stream);
 }
 
 size_t paces_holstein_csr_workspace_bytes(int64_t R, const paces_holstein_t* h) {
@@ -1588,6 +1629,7 @@ int paces_holstein_csr_fill(const uint64_t* rows, int64_t R, const paces_layout_
     out.indptr[t] = t < h->nterms ? indptr[t] : nullptr;
     out.indices[t] = t < h->nterms ? indices[t] : nullptr;
     out.data[t] = t < h->nterms ? data[t] : nullptr;
+    out.data32[t] = nullptr;
     if (t < h->nterms && !out.indptr[t]) return PACES_E_INVALID;
   }
   return run_csr_fill(R, T, marg(h), h->nterms, rows, lay->W, h->batched ? tbits(h->ntraj) : 0,

@@ -165,11 +165,13 @@ class Hamiltonian(HamiltonianFramework):
                 return offs[k + 1] - offs[k]
         return None
 
-    def generate_mel(self, basis_states, enlarge_steps=0, log_ind=1.2):
+    def generate_mel(self, basis_states, enlarge_steps=0, log_ind=1.2,
+                     value_dtype=torch.float64):
         """((diag, new_inds), {term: CsrMatrix}, dbg) for the basis (hamiltonian.py:383-492).
 
         new_inds = the basis (after ``enlarge_steps`` expansions) plus its neighbours; each term's
-        CSR holds the entries with at least one endpoint in that basis.
+        CSR holds the entries with at least one endpoint in that basis. value_dtype=torch.float32
+        gives the CSR values rounded to FP32 (what the mixed-precision series reads).
         """
         if basis_states.shape[1] != self.totwordwidth:
             raise ValueError("Shape of basis states does not match the specified number of bits.")
@@ -184,15 +186,19 @@ class Hamiltonian(HamiltonianFramework):
         self._check_basis_size(new_inds.shape[0])
         diag = self.generate_mel_diag(new_inds)
         mats = self._pairs_csr(new_inds.shape[0], basis.shape[0], offs, cand_pos, in_basis, src,
-                               nup, basis)
+                               nup, basis, value_dtype)
         dbg = {}
         if "vib_coupling" in self.offdiag_terms:
             dbg["vib_coupling"] = [basis.shape[0] - self._vib_plus_count(offs)]
         self.last_basis_e = basis
         return (diag, new_inds), mats, dbg
 
-    def _pairs_csr(self, R, nbasis, offs, cand_pos, in_basis, src, nup, basis=None):
-        """Per-term CSR from the union's candidate positions (paces_holstein_pairs_*)."""
+    def _pairs_csr(self, R, nbasis, offs, cand_pos, in_basis, src, nup, basis=None,
+                   value_dtype=torch.float64):
+        """Per-term CSR from the union's candidate positions (paces_holstein_pairs_*), with FP64
+        or FP32 values."""
+        if value_dtype not in (torch.float64, torch.float32):
+            raise ValueError("CSR values are float64 or float32.")
         h = self._hs
         nterms = h.nterms
         run_off = (C.c_int64 * len(offs))(*offs)
@@ -207,13 +213,14 @@ class Hamiltonian(HamiltonianFramework):
         for t, name in enumerate(self.offdiag_terms):
             ip = torch.empty(R + 1, dtype=torch.int32, device=self.device)
             ix = torch.empty(max(nnz_h[t], 1), dtype=torch.int32, device=self.device)
-            dv = torch.empty(max(nnz_h[t], 1), dtype=torch.float64, device=self.device)
+            dv = torch.empty(max(nnz_h[t], 1), dtype=value_dtype, device=self.device)
             arrs[0][t], arrs[1][t], arrs[2][t] = ip.data_ptr(), ix.data_ptr(), dv.data_ptr()
             mats[name] = CsrMatrix(ip, ix[:nnz_h[t]], dv[:nnz_h[t]], (R, R))
-        check(_lib.paces_holstein_pairs_fill(R, nbasis, run_off, ptr(cand_pos), ptr(in_basis),
-                                             ptr(src), ptr(nup), ptr(basis), C.byref(self._layout),
-                                             C.byref(h), arrs[0], arrs[1], arrs[2], wsp, wsn,
-                                             stream()), "holstein_pairs_fill")
+        fill = _lib.paces_holstein_pairs_fill if value_dtype == torch.float64 else \
+            _lib.paces_holstein_pairs_fill32
+        check(fill(R, nbasis, run_off, ptr(cand_pos), ptr(in_basis), ptr(src), ptr(nup),
+                   ptr(basis), C.byref(self._layout), C.byref(h), arrs[0], arrs[1], arrs[2], wsp,
+                   wsn, stream()), "holstein_pairs_fill")
         return mats
 
     def _csr(self, rows, in_basis):
@@ -777,7 +784,10 @@ class BatchedTimeEvolution:
 
     def adapt(self):
         select = self._select()
-        (diag, new_inds), mats, dbg = self.hamobj.generate_mel(select, self.enlarge_steps)
+        # the mixed-precision series reads FP32 values only: the CSR is built with them
+        vdt = torch.float32 if self.precision == "mixed" else torch.float64
+        (diag, new_inds), mats, dbg = self.hamobj.generate_mel(select, self.enlarge_steps,
+                                                               value_dtype=vdt)
         self.vector = engine.reassign(self.whoami, self.vector, new_inds, self._ws)
         self.whoami = new_inds
         self.diag_vals = diag

@@ -261,7 +261,8 @@ GEN_CASES = ["chain16", "ring64_small", "torus6", "square4_hop2"]
 
 @pytest.mark.parametrize("case", GEN_CASES)
 def test_expand_csr_diag_taylor_observables(env, case):
-    """generate_mel on the oracle's own step-1 basis: rows, per-term CSR, diagonal bit-exact;
+    """generate_mel on the oracle's own step-1 basis: rows, per-term CSR (FP64 values, and FP32
+    values equal to their rounding), diagonal bit-exact;
     H.x and the Taylor series bit-exact; observables to 1e-12."""
     import sys
     import os
@@ -288,6 +289,15 @@ def test_expand_csr_diag_taylor_observables(env, case):
         np.testing.assert_array_equal(m.data.cpu().numpy(), dv)
     if "vib_coupling" in oham.offdiag_terms:
         assert dbg["vib_coupling"] == rec.dbg["vib_coupling"]
+    # FP32 values (the mixed-precision engine's): the same entries, each value rounded to nearest
+    _, mats32, _ = ham.generate_mel(sel, enlarge_steps=c["enlarge"], value_dtype=torch.float32)
+    for name in oham.offdiag_terms:
+        ip, ix, dv = rec.csrs[name]
+        m = mats32[name]
+        assert m.data.dtype == torch.float32
+        np.testing.assert_array_equal(m.indptr.cpu().numpy(), ip)
+        np.testing.assert_array_equal(m.indices.cpu().numpy(), ix)
+        np.testing.assert_array_equal(m.data.cpu().numpy(), dv.astype(np.float32))
     # the lookup-based CSR entry point agrees with the union-based one
     _, in_b, _ = eng.lower_bound(ham.last_basis_e, new_inds, True, False, True)
     mats2 = ham._csr(new_inds, in_b)
