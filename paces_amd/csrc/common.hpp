@@ -111,6 +111,76 @@ __device__ __forceinline__ bool lex_less(const u64 (&a)[W], const u64 (&b)[W]) {
   return borrow != 0;
 }
 
+// ((a < b) != INV) ? x1 : x0, for W <= 4 by the borrow chain of a - b in VCC and one select
+// (VALU only: the compiler's form of a multiword compare combines per-word lane masks with
+// scalar instructions, which a binary search repeats at every probe).
+template <int W, bool INV>
+__device__ __forceinline__ uint32_t less_select(const u64 (&a)[W], const u64 (&b)[W], uint32_t x1,
+                                                uint32_t x0) {
+#define PACES_LO(x) (uint32_t)(x)
+#define PACES_HI(x) (uint32_t)((x) >> 32)
+  // v_cndmask_b32 d, s0, s1, vcc: d = vcc ? s1 : s0, and vcc = borrow = (a < b)
+  const uint32_t on_borrow = INV ? x0 : x1, on_none = INV ? x1 : x0;
+  uint32_t t, f;
+  if constexpr (W == 1) {
+    asm("v_sub_co_u32 %0, vcc, %2, %3\n\t"
+        "v_subb_co_u32 %0, vcc, %4, %5, vcc\n\t"
+        "v_cndmask_b32 %1, %6, %7, vcc"
+        : "=&v"(t), "=v"(f)
+        : "v"(PACES_LO(a[0])), "v"(PACES_LO(b[0])), "v"(PACES_HI(a[0])), "v"(PACES_HI(b[0])),
+          "v"(on_none), "v"(on_borrow)
+        : "vcc");
+  } else if constexpr (W == 2) {
+    asm("v_sub_co_u32 %0, vcc, %2, %3\n\t"
+        "v_subb_co_u32 %0, vcc, %4, %5, vcc\n\t"
+        "v_subb_co_u32 %0, vcc, %6, %7, vcc\n\t"
+        "v_subb_co_u32 %0, vcc, %8, %9, vcc\n\t"
+        "v_cndmask_b32 %1, %10, %11, vcc"
+        : "=&v"(t), "=v"(f)
+        : "v"(PACES_LO(a[1])), "v"(PACES_LO(b[1])), "v"(PACES_HI(a[1])), "v"(PACES_HI(b[1])),
+          "v"(PACES_LO(a[0])), "v"(PACES_LO(b[0])), "v"(PACES_HI(a[0])), "v"(PACES_HI(b[0])),
+          "v"(on_none), "v"(on_borrow)
+        : "vcc");
+  } else if constexpr (W == 3) {
+    asm("v_sub_co_u32 %0, vcc, %2, %3\n\t"
+        "v_subb_co_u32 %0, vcc, %4, %5, vcc\n\t"
+        "v_subb_co_u32 %0, vcc, %6, %7, vcc\n\t"
+        "v_subb_co_u32 %0, vcc, %8, %9, vcc\n\t"
+        "v_subb_co_u32 %0, vcc, %10, %11, vcc\n\t"
+        "v_subb_co_u32 %0, vcc, %12, %13, vcc\n\t"
+        "v_cndmask_b32 %1, %14, %15, vcc"
+        : "=&v"(t), "=v"(f)
+        : "v"(PACES_LO(a[2])), "v"(PACES_LO(b[2])), "v"(PACES_HI(a[2])), "v"(PACES_HI(b[2])),
+          "v"(PACES_LO(a[1])), "v"(PACES_LO(b[1])), "v"(PACES_HI(a[1])), "v"(PACES_HI(b[1])),
+          "v"(PACES_LO(a[0])), "v"(PACES_LO(b[0])), "v"(PACES_HI(a[0])), "v"(PACES_HI(b[0])),
+          "v"(on_none), "v"(on_borrow)
+        : "vcc");
+  } else if constexpr (W == 4) {
+    asm("v_sub_co_u32 %0, vcc, %2, %3\n\t"
+        "v_subb_co_u32 %0, vcc, %4, %5, vcc\n\t"
+        "v_subb_co_u32 %0, vcc, %6, %7, vcc\n\t"
+        "v_subb_co_u32 %0, vcc, %8, %9, vcc\n\t"
+        "v_subb_co_u32 %0, vcc, %10, %11, vcc\n\t"
+        "v_subb_co_u32 %0, vcc, %12, %13, vcc\n\t"
+        "v_subb_co_u32 %0, vcc, %14, %15, vcc\n\t"
+        "v_subb_co_u32 %0, vcc, %16, %17, vcc\n\t"
+        "v_cndmask_b32 %1, %18, %19, vcc"
+        : "=&v"(t), "=v"(f)
+        : "v"(PACES_LO(a[3])), "v"(PACES_LO(b[3])), "v"(PACES_HI(a[3])), "v"(PACES_HI(b[3])),
+          "v"(PACES_LO(a[2])), "v"(PACES_LO(b[2])), "v"(PACES_HI(a[2])), "v"(PACES_HI(b[2])),
+          "v"(PACES_LO(a[1])), "v"(PACES_LO(b[1])), "v"(PACES_HI(a[1])), "v"(PACES_HI(b[1])),
+          "v"(PACES_LO(a[0])), "v"(PACES_LO(b[0])), "v"(PACES_HI(a[0])), "v"(PACES_HI(b[0])),
+          "v"(on_none), "v"(on_borrow)
+        : "vcc");
+  } else {
+    (void)t;
+    f = lex_less<W>(a, b) ? on_borrow : on_none;
+  }
+#undef PACES_LO
+#undef PACES_HI
+  return f;
+}
+
 template <int W>
 __device__ __forceinline__ void load_lds_row(PACES_LPTR(const unsigned char) rows, unsigned i,
                                              u64 (&w)[W]) {
@@ -128,19 +198,28 @@ template <int W, bool UPPER>
 __device__ __forceinline__ int bound_lds(PACES_LPTR(const unsigned char) rows, int n,
                                          const Row<W>& q) {
   if (n <= 0) return 0;
-  int base = 0;
+  // the window's first row is tracked by its LDS address: one add forms a probe's address, and
+  // one select (on the compare's borrow) moves the window to it when q lies above it (lower
+  // bound) or not below it (upper bound)
+  typedef PACES_LPTR(const unsigned char) lptr;
+  lptr a = rows;
   for (int len = n; len > 1;) {
     const int half = len >> 1;
+    uint32_t hb = (uint32_t)half * (W * 8u);  // uniform: kept in a scalar register, so the
+    asm("" : "+s"(hb));                       // probe address is one 32-bit add (not a mad)
+    const lptr p = a + hb;
     u64 w[W];
-    load_lds_row<W>(rows, (unsigned)(base + half), w);
-    const bool pred = UPPER ? !lex_less<W>(q.w, w) : lex_less<W>(w, q.w);
-    base = pred ? base + half : base;
+    load_lds_row<W>(p, 0u, w);
+    const uint32_t pa = (uint32_t)(size_t)p, aa = (uint32_t)(size_t)a;  // LDS addresses: 32-bit
+    a = (lptr)(size_t)(UPPER ? less_select<W, true>(q.w, w, pa, aa)
+                             : less_select<W, false>(w, q.w, pa, aa));
     len -= half;
   }
   u64 w[W];
-  load_lds_row<W>(rows, (unsigned)base, w);
-  const bool pred = UPPER ? !lex_less<W>(q.w, w) : lex_less<W>(w, q.w);
-  return base + (pred ? 1 : 0);
+  load_lds_row<W>(a, 0u, w);
+  const uint32_t pred = UPPER ? less_select<W, true>(q.w, w, 1u, 0u)
+                              : less_select<W, false>(w, q.w, 1u, 0u);
+  return (int)((uint32_t)(a - rows) / (W * 8) + pred);
 }
 
 // Copy the 16-byte blocks holding bytes [src, src + nbytes) global -> LDS at dst (16-byte
