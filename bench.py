@@ -144,14 +144,19 @@ TRAJ_DESC = ("traj256: {n} independent 1D Holstein chains (open), N in {{16,24,3
              "N//2, dt=0.05, enlarge_steps=1 (draws seeded 0)")
 
 
-def btay_term_bytes(R, mats, mixed):
+def btay_term_bytes(R, mats, mixed, coded=False):
     """Algorithmic HBM bytes of one batched Taylor term over R rows, averaged over an even and an
     odd term (the roofline loop times both alike): diagonal (4 or 8 per row), each term's indptr
-    (4(R+1)), indices + values (4 + 4 or 8 per nonzero), x_in and x_out (8 or 16 per row each),
-    the complex128 partial sum read on every term and written on odd terms (16 + 8 per row)."""
+    (4(R+1)), indices + values (4 + 4 or 8 per nonzero; coded: 4 + 1 per vib nonzero and 4 per
+    hopping nonzero), x_in and x_out (8 or 16 per row each), the complex128 partial sum read on
+    every term and written on odd terms (16 + 8 per row)."""
     f = 4 if mixed else 8
     nnz = sum(m.nnz for m in mats)
-    return f * R + 4 * (R + 1) * len(mats) + (4 + f) * nnz + 4 * f * R + 24 * R
+    if coded:
+        vals = sum(m.nnz for m in mats if getattr(m, "code", None) is not None)
+    else:
+        vals = f * nnz
+    return f * R + 4 * (R + 1) * len(mats) + 4 * nnz + vals + 4 * f * R + 24 * R
 
 
 # HBM bytes per row of one k_btay_term<float, B, RR> launch, per launch shape (the tuner picks
@@ -159,8 +164,9 @@ def btay_term_bytes(R, mats, mixed):
 # over the roofline loop below with the shape fixed (--term-shape), each scaled by the counter's
 # reading for a 1 GiB device copy (profiles/k_btay_term_traj256_hbm_traffic.txt). Only the
 # mixed-precision kernel is measured; other shapes report null.
-BTAY_TRAFFIC_BYTES_PER_ROW = {("mixed", (256, 2)): 89.72, ("mixed", (256, 4)): 88.19,
-                              ("mixed", (512, 4)): 87.42}
+BTAY_TRAFFIC_BYTES_PER_ROW = {("mixed", (256, 2), "explicit"): 89.72,
+                              ("mixed", (256, 4), "explicit"): 88.19,
+                              ("mixed", (512, 4), "explicit"): 87.42}
 
 
 def measure_batched_term(bte, reps=20):
@@ -194,31 +200,27 @@ def measure_batched_term(bte, reps=20):
                                                   _hip.ptr(torch.view_as_real(bufs[2])),
                                                   1 if mixed else 0, _hip.ptr(bte.seg_off),
                                                   h.ntraj, _hip.ptr(ts), wsp, wsn, cref, sp))
-    if mixed:
-        mm = engine.MixedMats(bte.diag_vals, mats)
-        h64, h32, dptr = None, C.byref(mm.cset), _hip.ptr(mm.diag)
-    else:
-        s64 = engine.csr_set(mats)
-        h64, h32, dptr = C.byref(s64), None, _hip.ptr(bte.diag_vals)
+    # the same term operator (coded or explicit CSR values) the timed steps launched
+    diag, keep, term, _ = bte._term_operator(mats, mixed)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     x_in = bufs[2]
     for j in range(reps + 2):
         if j == 2:
             e0.record(st)
         x_out = bufs[j % 2]
-        _hip.check(lib.paces_batch_taylor_term_tuned(
-            R, dptr, h64, h32, _hip.ptr(torch.view_as_real(x_in)),
-            _hip.ptr(torch.view_as_real(x_out)), _hip.ptr(torch.view_as_real(res)),
-            engine.taylor_coeff(TRAJ_DT, j), j, -1.0, _hip.ptr(bte.seg_off), h.ntraj,
-            _hip.ptr(ts), _hip.ptr(hdr), wsp, wsn, cref, sp))
+        term(R, _hip.ptr(torch.view_as_real(x_in)), _hip.ptr(torch.view_as_real(x_out)),
+             _hip.ptr(torch.view_as_real(res)), engine.taylor_coeff(TRAJ_DT, j), j, -1.0,
+             _hip.ptr(ts), _hip.ptr(hdr), wsp, wsn, cref, sp)
         x_in = x_out
     e1.record(st)
     e1.synchronize()
-    return e0.elapsed_time(e1) / reps * 1e-3, btay_term_bytes(R, mats, mixed), reps, shape
+    del keep, diag
+    return (e0.elapsed_time(e1) / reps * 1e-3,
+            btay_term_bytes(R, mats, mixed, bte.csr_values == "coded"), reps, shape)
 
 
 def batched_leg(draws, precision, warmup, steps, label, graph=False, autotune=False,
-                launch_shape=None):
+                launch_shape=None, csr_values="coded"):
     """Batched run: step 0 and ``warmup`` steps untimed, then ``steps`` timed steps. Returns
     (state-steps, seconds, the evolution object)."""
     import torch
@@ -227,7 +229,7 @@ def batched_leg(draws, precision, warmup, steps, label, graph=False, autotune=Fa
                                        [d[2] for d in draws])
     bte = holstein.BatchedTimeEvolution(bham, [d[3] for d in draws], enlarge_steps=1,
                                         precision=precision, graph=graph, autotune=autotune,
-                                        launch_shape=launch_shape)
+                                        launch_shape=launch_shape, csr_values=csr_values)
     bte.create_single_exciton_states([d[0] // 2 for d in draws])
     bte.step(0, TRAJ_DT)
     for i in range(1, warmup + 1):
@@ -335,19 +337,21 @@ def run_traj256(a):
     prec = a.precision_batched
     units, elapsed, bte = batched_leg(draws, prec, a.warmup, a.steps, "batched-" + prec,
                                       graph=a.graph, autotune=a.autotune,
-                                      launch_shape=a.term_shape)
+                                      launch_shape=a.term_shape, csr_values=a.csr_values)
     rows = bte.numstates
     value = units / elapsed
+    bte_csr_values = bte.csr_values
     sec, nbytes, reps, shape = measure_batched_term(bte)
     kshape = "%d, %d" % shape if shape else "256, 1"
     achieved = nbytes / sec / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
             "frac": round(achieved / 8000.0, 4),
-            "traffic": (int(BTAY_TRAFFIC_BYTES_PER_ROW[(prec, shape)] * rows)
-                        if (prec, shape) in BTAY_TRAFFIC_BYTES_PER_ROW else None),
-            "kernel": "k_btay_term<%s, %s> (paces_batch_taylor_term_tuned, the timed steps' "
-                      "launch shape, stopping test off)" % ("float" if prec == "mixed" else "double",
-                                                            kshape),
+            "traffic": (int(BTAY_TRAFFIC_BYTES_PER_ROW[(prec, shape, bte.csr_values)] * rows)
+                        if (prec, shape, bte.csr_values) in BTAY_TRAFFIC_BYTES_PER_ROW else None),
+            "kernel": "k_btay_term<%s, %s> (paces_batch_taylor_term_%s, %s CSR values, the timed "
+                      "steps' launch shape, stopping test off)"
+                      % ("float" if prec == "mixed" else "double", kshape,
+                         "coded" if bte.csr_values == "coded" else "tuned", bte.csr_values),
             "avg_us": round(sec * 1e6, 2), "bytes_per_launch": int(nbytes), "rows": int(rows),
             "launches_timed": reps}
     n_exc_b = bte.observables()[0]
@@ -375,6 +379,7 @@ def run_traj256(a):
             "config": dict({"workload": TRAJ_DESC.format(n=a.ntraj), "form": "batched-" + prec,
                             "series_graph": bool(a.graph), "autotune": bool(a.autotune),
                             "term_shape": list(a.term_shape) if a.term_shape else "tuned",
+                            "csr_values": bte_csr_values,
                             "unit_of_work": "retained basis state advanced one step "
                                             "(sum over trajectories of min(maxstates, R_prev))",
                             "final_basis_rows": int(rows),
@@ -570,6 +575,9 @@ def main(argv=None):
     ap.add_argument("--term-shape", default=None,
                     help="traj256: fix the Taylor term's launch shape, e.g. 256,4 (threads, rows "
                          "per thread), instead of autotuning it (for reproducible profiles)")
+    ap.add_argument("--csr-values", default="coded", choices=["coded", "explicit"],
+                    help="traj256: the batched series reads coded CSR values (a code byte per "
+                         "vib entry) or explicit FP32/FP64 values (identical results)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="traj256: tune the Taylor-term launch shape")
     ap.add_argument("--learned-ms", type=int, default=2_000_000)

@@ -189,6 +189,15 @@ int paces_holstein_pairs_fill(int64_t R, int64_t nbasis, const int64_t* run_offs
                               const paces_holstein_t* h, int32_t* const* indptr,
                               int32_t* const* indices, double* const* data, void* ws,
                               size_t ws_bytes, void* stream);
+/* The same entries with coded values (paces_csr_coded_t): code[t] receives the upper phonon count
+ * of each entry of a vib term t (hopping terms write none; their code[t] may be null). Batched
+ * trajectories only, at most PACES_CODE_TABLE sqrt_table entries. */
+int paces_holstein_pairs_fill_coded(int64_t R, int64_t nbasis, const int64_t* run_offsets,
+                                    const int32_t* cand_pos, const uint8_t* in_basis,
+                                    const int32_t* img_src, const uint8_t* img_nup,
+                                    const paces_holstein_t* h, int32_t* const* indptr,
+                                    int32_t* const* indices, uint8_t* const* code, void* ws,
+                                    size_t ws_bytes, void* stream);
 /* The same entries with FP32 values (each the FP64 value rounded to nearest), for the
  * mixed-precision series, which reads FP32 values only (no reference counterpart: the reference's
  * matrices are FP64). */
@@ -367,6 +376,29 @@ int paces_batch_taylor_term_tuned(int64_t R, const void* diag, const paces_csr_s
                                   const int32_t* seg_off, int32_t ntraj, paces_batch_taylor_t* ts,
                                   paces_batch_taylor_hdr_t* hdr, void* ws, size_t ws_bytes,
                                   const paces_launch_cfg_t* cfg, void* stream);
+/* Coded values (no reference counterpart). In the rows of trajectory r, every entry of a hopping
+ * term t has the value v = term_value[r * nterms + t], and every entry of a vib term the value
+ * sqrt_table[m] * v, m the upper phonon count of its pair: the values paces_holstein_pairs_fill
+ * writes. A coded CSR stores m as one byte per vib entry (code[t]; null for a hopping term) in
+ * place of the values, and the Taylor term forms each trajectory's few values in LDS. */
+#define PACES_CODE_TABLE 64 /* codes 0 .. PACES_CODE_TABLE - 1: sqrt_len is at most this */
+typedef struct {
+  int32_t nterms;
+  int32_t sqrt_len;                      /* entries of sqrt_table */
+  const int32_t* indptr[PACES_MAX_TERMS];
+  const int32_t* indices[PACES_MAX_TERMS];
+  const uint8_t* code[PACES_MAX_TERMS];  /* null for a hopping term; else one byte per entry */
+  const double* term_value;              /* device, ntraj x nterms */
+  const double* sqrt_table;              /* device, sqrt_len */
+} paces_csr_coded_t;
+/* paces_batch_taylor_term_tuned with a coded CSR (complex64 terms when mixed != 0, with a float
+ * diag; else complex128 and a double diag); the same results bit for bit. */
+int paces_batch_taylor_term_coded(int64_t R, const void* diag, int32_t mixed,
+                                  const paces_csr_coded_t* H, const void* x_in, void* x_out,
+                                  double* res, double c, int32_t j, double tol,
+                                  const int32_t* seg_off, int32_t ntraj, paces_batch_taylor_t* ts,
+                                  paces_batch_taylor_hdr_t* hdr, void* ws, size_t ws_bytes,
+                                  const paces_launch_cfg_t* cfg, void* stream);
 /* after the series (terms and/or a replayed paces_batch_series_launch): adds to res the last
  * term of every trajectory whose last term index ts[t].j_ind is even, from x_even (the buffer
  * every even term wrote); same launch shape and workspace as the series */
@@ -402,6 +434,12 @@ typedef struct {
   void* ws;                               /* paces_batch_taylor_workspace_bytes(R_cap, ntraj) */
   double tol;
   double c[PACES_SERIES_MAX_TERMS];       /* term coefficients fl(dt / (j + 1)) */
+  /* coded != 0: the Hamiltonian is the coded CSR below (paces_csr_coded_t; data[] unused) */
+  int32_t coded;
+  int32_t sqrt_len;
+  const uint8_t* code[PACES_MAX_TERMS];
+  const double* term_value;
+  const double* sqrt_table;
 } paces_batch_series_t;
 size_t paces_batch_series_desc_bytes(void); /* sizeof(paces_batch_series_t), for bindings */
 int paces_batch_series_launch(const paces_batch_series_t* desc, int64_t R_cap, int32_t ntraj,

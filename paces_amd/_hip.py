@@ -75,6 +75,16 @@ class CsrSet32(C.Structure):
                 ("indices", C.c_void_p * MAX_TERMS), ("data", C.c_void_p * MAX_TERMS)]
 
 
+CODE_TABLE = 64
+
+
+class CsrCoded(C.Structure):
+    _fields_ = [("nterms", C.c_int32), ("sqrt_len", C.c_int32),
+                ("indptr", C.c_void_p * MAX_TERMS), ("indices", C.c_void_p * MAX_TERMS),
+                ("code", C.c_void_p * MAX_TERMS), ("term_value", C.c_void_p),
+                ("sqrt_table", C.c_void_p)]
+
+
 class BatchTaylor(C.Structure):
     _fields_ = [("c1", C.c_double), ("c1_plus_c2", C.c_double), ("ratio", C.c_double),
                 ("done", C.c_int32), ("j_ind", C.c_int32)]
@@ -93,7 +103,9 @@ class BatchSeries(C.Structure):
                 ("indices", C.c_void_p * MAX_TERMS), ("data", C.c_void_p * MAX_TERMS),
                 ("v", C.c_void_p), ("res", C.c_void_p), ("x", C.c_void_p * 3),
                 ("seg_off", C.c_void_p), ("ts", C.c_void_p), ("hdr", C.c_void_p),
-                ("ws", C.c_void_p), ("tol", C.c_double), ("c", C.c_double * SERIES_MAX_TERMS)]
+                ("ws", C.c_void_p), ("tol", C.c_double), ("c", C.c_double * SERIES_MAX_TERMS),
+                ("coded", C.c_int32), ("sqrt_len", C.c_int32), ("code", C.c_void_p * MAX_TERMS),
+                ("term_value", C.c_void_p), ("sqrt_table", C.c_void_p)]
 
 
 class TaylorState(C.Structure):
@@ -137,6 +149,9 @@ _SIGNATURES = {
                                               _P, C.POINTER(Layout), C.POINTER(Holstein),
                                               C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), _P,
                                               _SZ, _P]),
+    "paces_holstein_pairs_fill_coded": (C.c_int, [_I64, _I64, C.POINTER(C.c_int64), _P, _P, _P,
+                                                  _P, C.POINTER(Holstein), C.POINTER(_P),
+                                                  C.POINTER(_P), C.POINTER(_P), _P, _SZ, _P]),
     "paces_holstein_csr_workspace_bytes": (_SZ, [_I64, C.POINTER(Holstein)]),
     "paces_holstein_csr_count": (C.c_int, [_P, _I64, _P, C.POINTER(Layout), C.POINTER(Holstein),
                                            _P, _P, _SZ, _P]),
@@ -183,6 +198,9 @@ _SIGNATURES = {
                                                 C.POINTER(CsrSet32), _P, _P, _P, _D, _I32, _D,
                                                 _P, _I32, _P, _P, _P, _SZ, C.POINTER(LaunchCfg),
                                                 _P]),
+    "paces_batch_taylor_term_coded": (C.c_int, [_I64, _P, _I32, C.POINTER(CsrCoded), _P, _P,
+                                                _P, _D, _I32, _D, _P, _I32, _P, _P, _P, _SZ,
+                                                C.POINTER(LaunchCfg), _P]),
     "paces_batch_taylor_finish": (C.c_int, [_I64, _P, _I32, _P, _P, _I32, _P, _P, _SZ,
                                             C.POINTER(LaunchCfg), _P]),
     "paces_rows_varying_bits": (C.c_int, [_P, _I64, _I32, _P, _P]),

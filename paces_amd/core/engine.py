@@ -44,6 +44,55 @@ class CsrMatrix:
                                         self.indptr.cpu().numpy()), shape=self.shape)
 
 
+class CodedCsrMatrix:
+    """A term's CSR with coded values (paces_csr_coded_t): int32 indptr/indices and, for a vib
+    term, one code byte per entry (the pair's upper phonon count; None for a hopping term). The
+    float64 values -- the trajectory's term value, times sqrt(code) for a vib term -- are formed
+    on first use of ``data`` by ``value_fn(self)``."""
+
+    def __init__(self, indptr, indices, code, shape, term, value_fn):
+        self.indptr = indptr
+        self.indices = indices
+        self.code = code
+        self.shape = shape
+        self.term = term
+        self._value_fn = value_fn
+        self._data = None
+
+    @property
+    def nnz(self):
+        return int(self.indices.numel())
+
+    @property
+    def data(self):
+        if self._data is None:
+            self._data = self._value_fn(self)
+        return self._data
+
+    def dot(self, x):
+        return CsrMatrix(self.indptr, self.indices, self.data, self.shape).dot(x)
+
+    def to_scipy(self):
+        return CsrMatrix(self.indptr, self.indices, self.data, self.shape).to_scipy()
+
+
+def csr_coded(mats, sqrt_len, term_value_ptr, sqrt_table_ptr):
+    """paces_csr_coded_t of a list of CodedCsrMatrix (term order)."""
+    s = _hip.CsrCoded()
+    s.nterms = len(mats)
+    if len(mats) > _hip.MAX_TERMS:
+        raise ValueError("too many Hamiltonian terms")
+    s.sqrt_len = int(sqrt_len)
+    for t, m in enumerate(mats):
+        s.indptr[t] = m.indptr.data_ptr()
+        s.indices[t] = m.indices.data_ptr() if m.nnz else m.indptr.data_ptr()
+        s.code[t] = (m.code.data_ptr() if m.nnz else m.indptr.data_ptr()) \
+            if m.code is not None else None
+    s.term_value = term_value_ptr
+    s.sqrt_table = sqrt_table_ptr
+    return s
+
+
 def csr_set(mats):
     s = _hip.CsrSet()
     s.nterms = len(mats)

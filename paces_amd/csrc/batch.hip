@@ -636,6 +636,30 @@ struct CsrSetB64 {
   const int32_t* indptr[PACES_MAX_TERMS];
   const int32_t* indices[PACES_MAX_TERMS];
   const void* data[PACES_MAX_TERMS];
+  // coded values (paces_csr_coded_t): data unused
+  int coded;
+  int sqrt_len;
+  const uint8_t* code[PACES_MAX_TERMS];
+  const double* term_value;
+  const double* sqrt_table;
+};
+
+// Coded entry values: a vib entry's raw load is its code byte (an absent entry reads a byte of
+// the term's indptr; the mask keeps any byte inside the table), a hopping entry loads nothing;
+// the value is the workgroup's LDS table entry of the row's trajectory.
+template <typename F>
+struct CodedVals {
+  typedef uint32_t raw_t;
+  const F* vt;  // LDS: [term][code]
+  __device__ __forceinline__ uint32_t load(const CsrSetB64& H, int t, int32_t k, bool ok) const {
+    const uint8_t* c = H.code[t];
+    if (!c) return 0u;
+    const uint8_t* pc = ok ? c + k : (const uint8_t*)H.indptr[t];
+    return (uint32_t)*pc & (PACES_CODE_TABLE - 1);
+  }
+  __device__ __forceinline__ F value(int t, uint32_t raw) const {
+    return vt[t * PACES_CODE_TABLE + raw];
+  }
 };
 
 template <typename F>
@@ -649,11 +673,12 @@ struct Vec2<float> {
   typedef float2 T;
 };
 
-template <typename F>
+template <typename F, typename VL>
 __device__ __forceinline__ typename Vec2<F>::T hx_row_t(int64_t i, const F* __restrict__ diag,
                                                         const CsrSetB64& H,
                                                         const typename Vec2<F>::T* __restrict__ x,
-                                                        typename Vec2<F>::T& xi_out) {
+                                                        typename Vec2<F>::T& xi_out,
+                                                        const VL& vl) {
   typedef typename Vec2<F>::T V;
   const V xi = x[i];
   xi_out = xi;
@@ -665,10 +690,9 @@ __device__ __forceinline__ typename Vec2<F>::T hx_row_t(int64_t i, const F* __re
   for (int t = 0; t < PACES_MAX_TERMS; ++t) {
     if (t >= H.nterms) break;
     const int32_t b = H.indptr[t][i], e = H.indptr[t][i + 1];
-    const F* data = (const F*)H.data[t];
     F tr = 0, ti = 0;
     for (int32_t k = b; k < e; ++k) {
-      const F a = data[k];
+      const F a = vl.value(t, vl.load(H, t, k, true));
       const V xj = x[H.indices[t][k]];
       tr = tr + a * xj.x;
       ti = ti + a * xj.y;
@@ -800,12 +824,13 @@ __device__ __forceinline__ void btay_row_out(int64_t i, typename Vec2<F>::T h,
 
 // The batched term's rows of one thread: the partial sums are loaded first (independent of H.x),
 // then H.x in load phases (hx_rows_phased), then each row's output.
-template <typename F, int BS, int RPT, int NT>
+template <typename F, int BS, int RPT, int NT, typename VL>
 __device__ __forceinline__ void btay_rows_phased(const F* __restrict__ diag, const CsrSetB64& H,
                                                  const typename Vec2<F>::T* __restrict__ xin,
                                                  typename Vec2<F>::T* __restrict__ xout,
                                                  double2* __restrict__ res, F c, bool odd,
-                                                 const SegTile& T, u64& mt, u64& mr) {
+                                                 const SegTile& T, u64& mt, u64& mr,
+                                                 const VL& vl) {
   typedef typename Vec2<F>::T V;
   int64_t ii[RPT];
   bool vv[RPT];
@@ -817,10 +842,36 @@ __device__ __forceinline__ void btay_rows_phased(const F* __restrict__ diag, con
     rr[q] = res[vv[q] ? ii[q] : T.r0];
   }
   V h[RPT], xi[RPT];
-  hx_rows_phased<F, V, RPT, NT>(ii, vv, T.r0, diag, H, xin, h, xi);
+  hx_rows_phased<F, V, RPT, NT>(ii, vv, T.r0, diag, H, xin, h, xi, vl);
 #pragma unroll
   for (int q = 0; q < RPT; ++q)
     if (vv[q]) btay_row_out<F>(ii[q], h[q], xi[q], rr[q], c, odd, xout, res, mt, mr);
+}
+
+// A tile's rows of one term: the phased form for one or two Hamiltonian terms, else row by row.
+template <typename F, int BS, int RPT, typename VL>
+__device__ __forceinline__ void btay_rows_all(const F* __restrict__ diag, const CsrSetB64& H,
+                                              const typename Vec2<F>::T* __restrict__ xin,
+                                              typename Vec2<F>::T* __restrict__ xout,
+                                              double2* __restrict__ res, F c, bool odd,
+                                              const SegTile& T, u64& mt, u64& mr, const VL& vl) {
+  // the phased form needs about 40 VGPRs per FP64 row: not at 1024 threads x 4 rows (128 max)
+  constexpr bool phased = !(sizeof(F) == 8 && BS * RPT > 2048);
+  if (phased && H.nterms == 2) {
+    btay_rows_phased<F, BS, RPT, 2>(diag, H, xin, xout, res, c, odd, T, mt, mr, vl);
+  } else if (phased && H.nterms == 1) {
+    btay_rows_phased<F, BS, RPT, 1>(diag, H, xin, xout, res, c, odd, T, mt, mr, vl);
+  } else {
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int64_t i = T.r0 + q * BS + threadIdx.x;
+      if (i < T.r1) {
+        typename Vec2<F>::T xi;
+        const typename Vec2<F>::T h = hx_row_t<F>(i, diag, H, xin, xi, vl);
+        btay_row_out<F>(i, h, xi, res[i], c, odd, xout, res, mt, mr);
+      }
+    }
+  }
 }
 
 // The partial sum res is stored on odd terms only. On an even term j the sum res + t_j is formed
@@ -839,22 +890,26 @@ __device__ __forceinline__ void btay_term_body(
   if (T.t < 0 || ts[T.t].done) return;
   const bool odd = (j & 1) != 0;
   u64 mt = 0, mr = 0;
-  // the phased form needs about 40 VGPRs per FP64 row: not at 1024 threads x 4 rows (128 max)
-  constexpr bool phased = !(sizeof(F) == 8 && BS * RPT > 2048);
-  if (phased && H.nterms == 2) {
-    btay_rows_phased<F, BS, RPT, 2>(diag, H, xin, xout, res, c, odd, T, mt, mr);
-  } else if (phased && H.nterms == 1) {
-    btay_rows_phased<F, BS, RPT, 1>(diag, H, xin, xout, res, c, odd, T, mt, mr);
-  } else {
+  if (H.coded) {
+    // the tile's trajectory's entry values, per term and code (the values the pairs fill
+    // computes: sqrt_table[m] * v for a vib term, v for a hopping term)
+    // (unrolled over the terms: every H field index is static, so H stays in registers when the
+    // kernel builds it from a descriptor)
+    __shared__ F vt[PACES_MAX_TERMS * PACES_CODE_TABLE];
+    static_assert(BS >= PACES_CODE_TABLE, "one code per thread");
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-      const int64_t i = T.r0 + q * BS + threadIdx.x;
-      if (i < T.r1) {
-        typename Vec2<F>::T xi;
-        const typename Vec2<F>::T h = hx_row_t<F>(i, diag, H, xin, xi);
-        btay_row_out<F>(i, h, xi, res[i], c, odd, xout, res, mt, mr);
-      }
+    for (int t = 0; t < PACES_MAX_TERMS; ++t) {
+      if (t >= H.nterms) break;
+      const double v = H.term_value[(int64_t)T.t * H.nterms + t];
+      const int m = threadIdx.x;
+      if (m < PACES_CODE_TABLE)
+        vt[t * PACES_CODE_TABLE + m] =
+            (F)(H.code[t] ? (m < H.sqrt_len ? H.sqrt_table[m] * v : 0.0) : v);
     }
+    __syncthreads();
+    btay_rows_all<F, BS, RPT>(diag, H, xin, xout, res, c, odd, T, mt, mr, CodedVals<F>{vt});
+  } else {
+    btay_rows_all<F, BS, RPT>(diag, H, xin, xout, res, c, odd, T, mt, mr, PlainVals<F>{});
   }
   blk_max2<BS>(mt, mr, part);
 }
@@ -935,8 +990,15 @@ struct SeriesDev {
   PACES_GPTR(void) ws;
   double tol;
   double c[PACES_SERIES_MAX_TERMS];
+  int32_t coded;
+  int32_t sqrt_len;
+  PACES_GPTR(const uint8_t) code[PACES_MAX_TERMS];
+  PACES_GPTR(const double) term_value;
+  PACES_GPTR(const double) sqrt_table;
 };
 static_assert(sizeof(SeriesDev) == sizeof(paces_batch_series_t), "descriptor layout");
+static_assert(offsetof(SeriesDev, code) == offsetof(paces_batch_series_t, code),
+              "descriptor layout");
 static_assert(offsetof(SeriesDev, c) == offsetof(paces_batch_series_t, c), "descriptor layout");
 static_assert(offsetof(SeriesDev, ws) == offsetof(paces_batch_series_t, ws), "descriptor layout");
 
@@ -989,11 +1051,16 @@ __global__ void __launch_bounds__(BS) k_bser_term(const paces_batch_series_t* __
   const SeriesDev* d = (const SeriesDev*)desc;
   CsrSetB64 H;
   H.nterms = d->nterms_h;
+  H.coded = d->coded;
+  H.sqrt_len = d->sqrt_len;
+  H.term_value = gen(d->term_value);
+  H.sqrt_table = gen(d->sqrt_table);
 #pragma unroll
   for (int t = 0; t < PACES_MAX_TERMS; ++t) {
     H.indptr[t] = gen(d->indptr[t]);
     H.indices[t] = gen(d->indices[t]);
     H.data[t] = gen(d->data[t]);
+    H.code[t] = gen(d->code[t]);
   }
   typedef typename Vec2<F>::T V;
   const V* xin = gen((PACES_GPTR(const V))(j == 0 ? d->x[2] : d->x[(j - 1) & 1]));
@@ -1012,6 +1079,11 @@ static bool csr_b(const paces_csr_set_t* h64, const paces_csr32_set_t* h32, CsrS
   const int n = h64 ? h64->nterms : (h32 ? h32->nterms : -1);
   if (n < 0 || n > PACES_MAX_TERMS) return false;
   H->nterms = n;
+  H->coded = 0;
+  H->sqrt_len = 0;
+  H->term_value = nullptr;
+  H->sqrt_table = nullptr;
+  for (int t = 0; t < PACES_MAX_TERMS; ++t) H->code[t] = nullptr;
   for (int t = 0; t < PACES_MAX_TERMS; ++t) {
     H->indptr[t] = t < n ? (h64 ? h64->indptr[t] : h32->indptr[t]) : nullptr;
     H->indices[t] = t < n ? (h64 ? h64->indices[t] : h32->indices[t]) : nullptr;
@@ -1021,9 +1093,38 @@ static bool csr_b(const paces_csr_set_t* h64, const paces_csr32_set_t* h32, CsrS
   return true;
 }
 
+static bool csr_coded(const paces_csr_coded_t* hc, CsrSetB64* H) {
+  if (!hc || hc->nterms < 0 || hc->nterms > PACES_MAX_TERMS) return false;
+  if (hc->sqrt_len < 0 || hc->sqrt_len > PACES_CODE_TABLE) return false;
+  if (hc->nterms > 0 && !hc->term_value) return false;
+  H->nterms = hc->nterms;
+  H->coded = 1;
+  H->sqrt_len = hc->sqrt_len;
+  H->term_value = hc->term_value;
+  H->sqrt_table = hc->sqrt_table;
+  bool vib = false;
+  for (int t = 0; t < PACES_MAX_TERMS; ++t) {
+    const bool in = t < hc->nterms;
+    H->indptr[t] = in ? hc->indptr[t] : nullptr;
+    H->indices[t] = in ? hc->indices[t] : nullptr;
+    H->data[t] = nullptr;
+    H->code[t] = in ? hc->code[t] : nullptr;
+    if (in && !H->indptr[t]) return false;
+    vib = vib || H->code[t];
+  }
+  return !vib || hc->sqrt_table;
+}
+
 }  // namespace paces
 
 using namespace paces;
+
+// one batched Taylor term (paces_batch_taylor_term_tuned / _coded) over the CSR set H
+static int btay_term_launch(int64_t R, const void* diag, int mixed, const CsrSetB64& H,
+                            const void* x_in, void* x_out, double* res, double c, int32_t j,
+                            double tol, const int32_t* seg_off, int32_t ntraj,
+                            paces_batch_taylor_t* ts, paces_batch_taylor_hdr_t* hdr, void* ws,
+                            size_t ws_bytes, const paces_launch_cfg_t* cfg, void* stream);
 
 extern "C" {
 
@@ -1189,8 +1290,31 @@ int paces_batch_taylor_term_tuned(int64_t R, const void* diag, const paces_csr_s
                                   const paces_launch_cfg_t* cfg, void* stream) {
   CsrSetB64 H;
   const int mixed = H32 != nullptr;
-  if (R < 0 || ntraj < 1 || !seg_off || !ts || !hdr || !ws || !csr_b(H64, H32, &H) ||
-      !cfg_ok(cfg))
+  if (!csr_b(H64, H32, &H)) return PACES_E_INVALID;
+  return btay_term_launch(R, diag, mixed, H, x_in, x_out, res, c, j, tol, seg_off, ntraj, ts,
+                          hdr, ws, ws_bytes, cfg, stream);
+}
+
+int paces_batch_taylor_term_coded(int64_t R, const void* diag, int32_t mixed,
+                                  const paces_csr_coded_t* Hc, const void* x_in, void* x_out,
+                                  double* res, double c, int32_t j, double tol,
+                                  const int32_t* seg_off, int32_t ntraj, paces_batch_taylor_t* ts,
+                                  paces_batch_taylor_hdr_t* hdr, void* ws, size_t ws_bytes,
+                                  const paces_launch_cfg_t* cfg, void* stream) {
+  CsrSetB64 H;
+  if (!csr_coded(Hc, &H)) return PACES_E_INVALID;
+  return btay_term_launch(R, diag, mixed ? 1 : 0, H, x_in, x_out, res, c, j, tol, seg_off, ntraj,
+                          ts, hdr, ws, ws_bytes, cfg, stream);
+}
+
+}  // extern "C"
+
+static int btay_term_launch(int64_t R, const void* diag, int mixed, const CsrSetB64& H,
+                            const void* x_in, void* x_out, double* res, double c, int32_t j,
+                            double tol, const int32_t* seg_off, int32_t ntraj,
+                            paces_batch_taylor_t* ts, paces_batch_taylor_hdr_t* hdr, void* ws,
+                            size_t ws_bytes, const paces_launch_cfg_t* cfg, void* stream) {
+  if (R < 0 || ntraj < 1 || !seg_off || !ts || !hdr || !ws || !cfg_ok(cfg))
     return PACES_E_INVALID;
   if (R > 0 && (!diag || !x_in || !x_out || !res)) return PACES_E_INVALID;
   if (ws_bytes < btay_ws_bytes(R, ntraj)) return PACES_E_WORKSPACE;
@@ -1219,6 +1343,8 @@ int paces_batch_taylor_term_tuned(int64_t R, const void* diag, const paces_csr_s
                      0, (int)j, tol);
   return launch_status();
 }
+
+extern "C" {
 
 int paces_batch_taylor_finish(int64_t R, const void* x_even, int32_t mixed, double* res,
                               const int32_t* seg_off, int32_t ntraj,

@@ -548,11 +548,24 @@ inline void cfg_get(const paces_launch_cfg_t* cfg, int* b, int* r) {
 // data[t] (F values; data may be typed or void*).
 constexpr int kHxEnt = 2;
 
-template <typename F, typename V, int RPT, int NT, typename CSR>
+// Entry values read from the CSR's data arrays. (A value loader gives each entry's raw load --
+// issued in the entry phase, in bounds: an absent entry reads the term's indptr -- and its value.)
+template <typename F>
+struct PlainVals {
+  typedef F raw_t;
+  template <typename CSR>
+  __device__ __forceinline__ F load(const CSR& H, int t, int32_t k, bool ok) const {
+    const F* pa = ok ? ((const F*)H.data[t]) + k : (const F*)H.indptr[t];
+    return *pa;
+  }
+  __device__ __forceinline__ F value(int, F raw) const { return raw; }
+};
+
+template <typename F, typename V, int RPT, int NT, typename CSR, typename VL = PlainVals<F>>
 __device__ __forceinline__ void hx_rows_phased(const int64_t (&ii)[RPT], const bool (&vv)[RPT],
                                                int64_t spare, const F* __restrict__ diag,
                                                const CSR& H, const V* __restrict__ x,
-                                               V (&h)[RPT], V (&xi)[RPT]) {
+                                               V (&h)[RPT], V (&xi)[RPT], const VL& vl = VL()) {
   F d[RPT];
   int32_t b[RPT][NT], e[RPT][NT];
 #pragma unroll
@@ -566,7 +579,7 @@ __device__ __forceinline__ void hx_rows_phased(const int64_t (&ii)[RPT], const b
       e[q][t] = H.indptr[t][i + 1];
     }
   }
-  F a[RPT][NT][kHxEnt];
+  typename VL::raw_t a[RPT][NT][kHxEnt];
   uint32_t ix[RPT][NT][kHxEnt];  // unsigned: no sign extension (and wait) before the gathers
   bool ok[RPT][NT][kHxEnt];
 #pragma unroll
@@ -577,9 +590,8 @@ __device__ __forceinline__ void hx_rows_phased(const int64_t (&ii)[RPT], const b
       for (int m = 0; m < kHxEnt; ++m) {
         const int32_t k = b[q][t] + m;
         ok[q][t][m] = vv[q] & (k < e[q][t]);
-        const F* pa = ok[q][t][m] ? ((const F*)H.data[t]) + k : (const F*)H.indptr[t];
         const int32_t* pi = ok[q][t][m] ? H.indices[t] + k : H.indptr[t];
-        a[q][t][m] = *pa;
+        a[q][t][m] = vl.load(H, t, k, ok[q][t][m]);
         ix[q][t][m] = (uint32_t)*pi;
       }
   V xj[RPT][NT][kHxEnt];
@@ -599,14 +611,14 @@ __device__ __forceinline__ void hx_rows_phased(const int64_t (&ii)[RPT], const b
 #pragma unroll
       for (int m = 0; m < kHxEnt; ++m) {
         if (ok[q][t][m]) {
-          tr = tr + a[q][t][m] * xj[q][t][m].x;
-          ti = ti + a[q][t][m] * xj[q][t][m].y;
+          const F av = vl.value(t, a[q][t][m]);
+          tr = tr + av * xj[q][t][m].x;
+          ti = ti + av * xj[q][t][m].y;
         }
       }
       if (vv[q]) {
-        const F* data = (const F*)H.data[t];
         for (int32_t k = b[q][t] + kHxEnt; k < e[q][t]; ++k) {
-          const F av = data[k];
+          const F av = vl.value(t, vl.load(H, t, k, true));
           const V xv = x[H.indices[t][k]];
           tr = tr + av * xv.x;
           ti = ti + av * xv.y;

@@ -568,6 +568,8 @@ struct CsrOut {
   int32_t* indices[PACES_MAX_TERMS];
   double* data[PACES_MAX_TERMS];
   float* data32[PACES_MAX_TERMS];  // when set, FP32 values (rounded from the FP64 value) instead
+  uint8_t* code[PACES_MAX_TERMS];  // coded: the vib pair's upper phonon count (hopping: none)
+  int coded;                       // values as codes, not numbers (data, data32 unused)
 };
 
 // entry value v of term t at position i, in the output's precision
@@ -576,6 +578,15 @@ __device__ __forceinline__ void put_val(const CsrOut& o, int t, int64_t i, doubl
     o.data32[t][i] = (float)v;
   else
     o.data[t][i] = v;
+}
+
+// an entry's value (v, or the code nup of a vib pair for a coded output) at position i of term t
+__device__ __forceinline__ void put_entry(const CsrOut& o, int t, int64_t i, double v, int nup) {
+  if (o.coded) {
+    if (o.code[t]) o.code[t][i] = (uint8_t)nup;
+  } else {
+    put_val(o, t, i, v);
+  }
 }
 
 // Emits each row's entries of every term in ascending column order by repeated minimum
@@ -743,14 +754,18 @@ __global__ void __launch_bounds__(kBlock) k_pairs_fill(int64_t R, int64_t nbasis
   if (p < 0 || q < 0) return;
   if (T.dir[k] < 0 && inB[p]) return;
   const int t = T.term[k];
-  const int traj = M.batched ? (int)(basis[(int64_t)src[e] * W] >> (64 - trajbits)) : 0;
-  const double v = op_value(T, k, M, traj, sqrt_tab, nup[e]);
+  const int nu = nup[e];
+  double v = 0.0;
+  if (!out.coded) {
+    const int traj = M.batched ? (int)(basis[(int64_t)src[e] * W] >> (64 - trajbits)) : 0;
+    v = op_value(T, k, M, traj, sqrt_tab, nu);
+  }
   const int32_t a = atomicAdd(&cursor[(int64_t)t * R + q], 1);
   out.indices[t][a] = p;
-  put_val(out, t, a, v);
+  put_entry(out, t, a, v, nu);
   const int32_t b = atomicAdd(&cursor[(int64_t)t * R + p], 1);
   out.indices[t][b] = q;
-  put_val(out, t, b, v);
+  put_entry(out, t, b, v, nu);
 }
 
 // selection sort of each row's (few) entries by column, in place
@@ -762,6 +777,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs_sort(int64_t R, int nterms, Cs
     int32_t* ix = out.indices[t];
     double* dv = out.data[t];
     float* fv = out.data32[t];
+    uint8_t* cv = out.code[t];
     for (int32_t a = b; a + 1 < e; ++a) {
       int32_t m = a;
       for (int32_t c = a + 1; c < e; ++c)
@@ -770,7 +786,13 @@ __global__ void __launch_bounds__(kBlock) k_pairs_sort(int64_t R, int nterms, Cs
         const int32_t tj = ix[a];
         ix[a] = ix[m];
         ix[m] = tj;
-        if (fv) {
+        if (out.coded) {
+          if (cv) {
+            const uint8_t tv = cv[a];
+            cv[a] = cv[m];
+            cv[m] = tv;
+          }
+        } else if (fv) {
           const float tv = fv[a];
           fv[a] = fv[m];
           fv[m] = tv;
@@ -983,7 +1005,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs_place(int64_t R, int64_t nbasi
   const int nu = nup[e];
   const int32_t q = cand_pos[s];
   const int32_t pc = p < 0 ? 0 : p, qc = q < 0 ? 0 : q;
-  const u64 w0 = M.batched ? basis[(int64_t)s * W] : 0;
+  const u64 w0 = (M.batched && !out.coded) ? basis[(int64_t)s * W] : 0;
   const bool pin = inB[pc] != 0;
   const uint8_t* f = flags + R * SL.foff[t];
   uint8_t fp[8], fq[8];
@@ -996,7 +1018,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs_place(int64_t R, int64_t nbasi
   if (p < 0 || q < 0) return;
   if (T.dir[k] < 0 && pin) return;
   const int traj = M.batched ? (int)(w0 >> (64 - trajbits)) : 0;
-  const double v = op_value(T, k, M, traj, sqrt_tab, nu);
+  const double v = out.coded ? 0.0 : op_value(T, k, M, traj, sqrt_tab, nu);
   int32_t a = sq, b = sp;
   const int rk = SL.rank[k], rinv = SL.rank[k ^ 1];
 #pragma unroll
@@ -1008,9 +1030,9 @@ __global__ void __launch_bounds__(kBlock) k_pairs_place(int64_t R, int64_t nbasi
     }
   }
   out.indices[t][a] = p;
-  put_val(out, t, a, v);
+  put_entry(out, t, a, v, nu);
   out.indices[t][b] = q;
-  put_val(out, t, b, v);
+  put_entry(out, t, b, v, nu);
 }
 
 static size_t pairs_ws(int64_t R, int nterms, int nops) {
@@ -1503,11 +1525,13 @@ static int pairs_fill_impl(int64_t R, int64_t nbasis, const int64_t* run_offsets
                            const int32_t* img_src, const uint8_t* img_nup, const uint64_t* basis,
                            const paces_holstein_t* h, const paces_layout_t* lay,
                            int32_t* const* indptr, int32_t* const* indices, double* const* data,
-                           float* const* data32, void* ws, size_t ws_bytes, void* stream) {
+                           float* const* data32, uint8_t* const* code, void* ws, size_t ws_bytes,
+                           void* stream) {
   OpTable T;
-  if (!h || !run_offsets || !ws || !indptr || !indices || (!data && !data32) || R < 0)
+  if (!h || !run_offsets || !ws || !indptr || !indices || (!data && !data32 && !code) || R < 0)
     return PACES_E_INVALID;
-  if (h->batched && (!basis || !lay || !h->d_traj_value)) return PACES_E_INVALID;
+  if (h->batched && !code && (!basis || !lay || !h->d_traj_value)) return PACES_E_INVALID;
+  if (code && (!h->batched || h->sqrt_table_len > PACES_CODE_TABLE)) return PACES_E_UNSUPPORTED;
   const int W = lay ? lay->W : 1;
   const int trajbits = h->batched ? tbits(h->ntraj) : 0;
   if (h->nterms < 1 || h->nterms > PACES_MAX_TERMS) return PACES_E_UNSUPPORTED;
@@ -1526,8 +1550,12 @@ static int pairs_fill_impl(int64_t R, int64_t nbasis, const int64_t* run_offsets
     out.indices[t] = t < h->nterms ? indices[t] : nullptr;
     out.data[t] = (data && t < h->nterms) ? data[t] : nullptr;
     out.data32[t] = (data32 && t < h->nterms) ? data32[t] : nullptr;
+    out.code[t] = (code && t < h->nterms && h->terms[t].kind == PACES_TERM_VIB) ? code[t] : nullptr;
     if (t < h->nterms && !out.indptr[t]) return PACES_E_INVALID;
+    if (code && t < h->nterms && h->terms[t].kind == PACES_TERM_VIB && !out.code[t])
+      return PACES_E_INVALID;
   }
+  out.coded = code ? 1 : 0;
   hipStream_t st = (hipStream_t)stream;
   if (R == 0) {
     hipLaunchKernelGGL(k_csr_empty, dim3(1), dim3(64), 0, st, out, h->nterms);
@@ -1570,7 +1598,7 @@ int paces_holstein_pairs_fill(int64_t R, int64_t nbasis, const int64_t* run_offs
                               size_t ws_bytes, void* stream) {
   if (!data) return PACES_E_INVALID;
   return pairs_fill_impl(R, nbasis, run_offsets, cand_pos, in_basis, img_src, img_nup, basis, h,
-                         lay, indptr, indices, data, nullptr, ws, ws_bytes, stream);
+                         lay, indptr, indices, data, nullptr, nullptr, ws, ws_bytes, stream);
 }
 
 int paces_holstein_pairs_fill32(int64_t R, int64_t nbasis, const int64_t* run_offsets,
@@ -1582,7 +1610,19 @@ int paces_holstein_pairs_fill32(int64_t R, int64_t nbasis, const int64_t* run_of
                                 size_t ws_bytes, void* stream) {
   if (!data) return PACES_E_INVALID;
   return pairs_fill_impl(R, nbasis, run_offsets, cand_pos, in_basis, img_src, img_nup, basis, h,
-                         lay, indptr, indices, nullptr, data, ws, ws_bytes, stream);
+                         lay, indptr, indices, nullptr, data, nullptr, ws, ws_bytes, stream);
+}
+
+int paces_holstein_pairs_fill_coded(int64_t R, int64_t nbasis, const int64_t* run_offsets,
+                                    const int32_t* cand_pos, const uint8_t* in_basis,
+                                    const int32_t* img_src, const uint8_t* img_nup,
+                                    const paces_holstein_t* h, int32_t* const* indptr,
+                                    int32_t* const* indices, uint8_t* const* code, void* ws,
+                                    size_t ws_bytes, void* stream) {
+  if (!code) return PACES_E_INVALID;
+  return pairs_fill_impl(R, nbasis, run_offsets, cand_pos, in_basis, img_src, img_nup, nullptr,
+                         h, nullptr, indptr, indices, nullptr, nullptr, code, ws, ws_bytes,
+                         stream);
 }
 
 size_t paces_holstein_csr_workspace_bytes(int64_t R, const paces_holstein_t* h) {
@@ -1630,8 +1670,10 @@ int paces_holstein_csr_fill(const uint64_t* rows, int64_t R, const paces_layout_
     out.indices[t] = t < h->nterms ? indices[t] : nullptr;
     out.data[t] = t < h->nterms ? data[t] : nullptr;
     out.data32[t] = nullptr;
+    out.code[t] = nullptr;
     if (t < h->nterms && !out.indptr[t]) return PACES_E_INVALID;
   }
+  out.coded = 0;
   return run_csr_fill(R, T, marg(h), h->nterms, rows, lay->W, h->batched ? tbits(h->ntraj) : 0,
                       h->d_sqrt_table, out, ws, (hipStream_t)stream);
 }

@@ -19,7 +19,7 @@ from .._hip import check, ptr, stream
 from ..aux.helpers import cartesian_product, debug_lister, obs_attrs
 from ..config import INIT_VERBOSITY_LEVEL, devices
 from ..core import engine
-from ..core.engine import CsrMatrix
+from ..core.engine import CodedCsrMatrix, CsrMatrix
 from ..core.hamiltonian import HamiltonianFramework, MelTriple
 from ..core.observables import ObservablesFramework
 from ..core.time_evolution import TimeEvolutionFramework
@@ -171,7 +171,8 @@ class Hamiltonian(HamiltonianFramework):
 
         new_inds = the basis (after ``enlarge_steps`` expansions) plus its neighbours; each term's
         CSR holds the entries with at least one endpoint in that basis. value_dtype=torch.float32
-        gives the CSR values rounded to FP32 (what the mixed-precision series reads).
+        gives the CSR values rounded to FP32 (what the mixed-precision series reads);
+        value_dtype="coded" (batched trajectories) gives coded CSRs (engine.CodedCsrMatrix).
         """
         if basis_states.shape[1] != self.totwordwidth:
             raise ValueError("Shape of basis states does not match the specified number of bits.")
@@ -186,7 +187,7 @@ class Hamiltonian(HamiltonianFramework):
         self._check_basis_size(new_inds.shape[0])
         diag = self.generate_mel_diag(new_inds)
         mats = self._pairs_csr(new_inds.shape[0], basis.shape[0], offs, cand_pos, in_basis, src,
-                               nup, basis, value_dtype)
+                               nup, basis, value_dtype, rows=new_inds)
         dbg = {}
         if "vib_coupling" in self.offdiag_terms:
             dbg["vib_coupling"] = [basis.shape[0] - self._vib_plus_count(offs)]
@@ -194,11 +195,14 @@ class Hamiltonian(HamiltonianFramework):
         return (diag, new_inds), mats, dbg
 
     def _pairs_csr(self, R, nbasis, offs, cand_pos, in_basis, src, nup, basis=None,
-                   value_dtype=torch.float64):
+                   value_dtype=torch.float64, rows=None):
         """Per-term CSR from the union's candidate positions (paces_holstein_pairs_*), with FP64
-        or FP32 values."""
+        or FP32 values, or coded values ("coded": batched trajectories; rows = the new basis,
+        for the values formed on demand)."""
+        if value_dtype == "coded":
+            return self._pairs_csr_coded(R, nbasis, offs, cand_pos, in_basis, src, nup, rows)
         if value_dtype not in (torch.float64, torch.float32):
-            raise ValueError("CSR values are float64 or float32.")
+            raise ValueError("CSR values are float64, float32 or coded.")
         h = self._hs
         nterms = h.nterms
         run_off = (C.c_int64 * len(offs))(*offs)
@@ -222,6 +226,49 @@ class Hamiltonian(HamiltonianFramework):
                    ptr(basis), C.byref(self._layout), C.byref(h), arrs[0], arrs[1], arrs[2], wsp,
                    wsn, stream()), "holstein_pairs_fill")
         return mats
+
+    def _pairs_csr_coded(self, R, nbasis, offs, cand_pos, in_basis, src, nup, rows):
+        """Coded per-term CSR (paces_holstein_pairs_fill_coded): a code byte per vib entry."""
+        h = self._hs
+        if not h.batched:
+            raise NotImplementedError("Coded CSR values are for batched trajectories.")
+        run_off = (C.c_int64 * len(offs))(*offs)
+        nnz = torch.empty(max(h.nterms, 1), dtype=torch.int32, device=self.device)
+        wsp, wsn = self._ws.get(_lib.paces_holstein_pairs_workspace_bytes(R, C.byref(h)))
+        check(_lib.paces_holstein_pairs_count(R, nbasis, run_off, ptr(cand_pos), ptr(in_basis),
+                                              ptr(src), C.byref(h), ptr(nnz), wsp, wsn, stream()),
+              "holstein_pairs_count")
+        nnz_h = nnz.tolist()
+        mats = {}
+        arrs = [(C.c_void_p * _hip.MAX_TERMS)() for _ in range(3)]
+        for t, name in enumerate(self.offdiag_terms):
+            ip = torch.empty(R + 1, dtype=torch.int32, device=self.device)
+            ix = torch.empty(max(nnz_h[t], 1), dtype=torch.int32, device=self.device)
+            cd = None
+            if name == "vib_coupling":
+                cd = torch.empty(max(nnz_h[t], 1), dtype=torch.uint8, device=self.device)
+                arrs[2][t] = cd.data_ptr()
+            arrs[0][t], arrs[1][t] = ip.data_ptr(), ix.data_ptr()
+            mats[name] = CodedCsrMatrix(ip, ix[:nnz_h[t]], cd[:nnz_h[t]] if cd is not None
+                                        else None, (R, R), t,
+                                        lambda m, rows=rows: self._coded_values(m, rows))
+        check(_lib.paces_holstein_pairs_fill_coded(R, nbasis, run_off, ptr(cand_pos),
+                                                   ptr(in_basis), ptr(src), ptr(nup), C.byref(h),
+                                                   arrs[0], arrs[1], arrs[2], wsp, wsn, stream()),
+              "holstein_pairs_fill_coded")
+        return mats
+
+    def _coded_values(self, m, rows):
+        """float64 values of a coded CSR: the term value of each entry's row's trajectory, times
+        sqrt(code) for a vib term (the products paces_holstein_pairs_fill forms)."""
+        R = m.shape[0]
+        counts = (m.indptr[1:] - m.indptr[:-1]).long()
+        traj = self.get_traj(rows).long()
+        te = torch.repeat_interleave(traj, counts, output_size=m.nnz)
+        v = self._d_traj_val[te, m.term]
+        if m.code is not None:
+            v = self._d_sqrt[m.code.long()] * v
+        return v.contiguous()
 
     def _csr(self, rows, in_basis):
         """Per-term CSR of the rows by neighbour lookups (paces_holstein_csr_*); equal to
@@ -710,7 +757,7 @@ class BatchedTimeEvolution:
 
     def __init__(self, hamobj: BatchedHamiltonian, maxstates, enlarge_steps=1,
                  precision="mixed", m_star=100, taylor_tol=None, autotune=False, graph=False,
-                 launch_shape=None):
+                 launch_shape=None, csr_values="coded"):
         self.hamobj = hamobj
         self.device = hamobj.device
         ms = numpy.asarray(maxstates, dtype=numpy.int64)
@@ -736,6 +783,14 @@ class BatchedTimeEvolution:
             if launch_shape not in _at.SHAPES:
                 raise ValueError(f"launch_shape must be one of {_at.SHAPES}")
         self.launch_shape = launch_shape
+        # csr_values="coded": the series reads each vib entry's code byte and forms the values
+        # per trajectory in LDS (paces_csr_coded_t); "explicit": FP32 (mixed) or FP64 values.
+        # Both give identical results.
+        if csr_values not in ("coded", "explicit"):
+            raise ValueError("csr_values must be 'coded' or 'explicit'.")
+        if csr_values == "coded" and hamobj._hs.sqrt_table_len > _hip.CODE_TABLE:
+            csr_values = "explicit"  # codes index a table of at most CODE_TABLE entries
+        self.csr_values = csr_values
         # graph=True: the Taylor series runs as one HIP graph per step (captured once per row
         # capacity and term count, replayed with a rewritten device descriptor)
         self.graph = bool(graph)
@@ -786,6 +841,8 @@ class BatchedTimeEvolution:
         select = self._select()
         # the mixed-precision series reads FP32 values only: the CSR is built with them
         vdt = torch.float32 if self.precision == "mixed" else torch.float64
+        if self.csr_values == "coded":
+            vdt = "coded"
         (diag, new_inds), mats, dbg = self.hamobj.generate_mel(select, self.enlarge_steps,
                                                                value_dtype=vdt)
         self.vector = engine.reassign(self.whoami, self.vector, new_inds, self._ws)
@@ -805,7 +862,58 @@ class BatchedTimeEvolution:
         ev.record(torch.cuda.current_stream())
         self._seg_host = (host, ev)
 
-    def _propagate_graph(self, delta_t, mats, mixed, res, bufs, ts, hdr, dptr, h64, h32, cfg, st):
+    def _term_operator(self, mats, mixed):
+        """What one batched Taylor term needs, for the CSR form in use: (diag tensor, keep-alive
+        objects, call(R, x_in, x_out, res, c, j, tol, ts, hdr, wsp, wsn, cfgref, st) launching one
+        term, fill(desc) writing the Hamiltonian into a series descriptor)."""
+        h = self.hamobj
+        if self.csr_values == "coded":
+            diag = engine.to_f32(self.diag_vals) if mixed else self.diag_vals
+            hc = engine.csr_coded(mats, h._hs.sqrt_table_len, h._hs.d_traj_value,
+                                  h._hs.d_sqrt_table)
+
+            def call(R, x_in, x_out, res, c, j, tol, ts, hdr, wsp, wsn, cref, st):
+                check(_lib.paces_batch_taylor_term_coded(
+                    R, ptr(diag), 1 if mixed else 0, C.byref(hc), x_in, x_out, res, c, j, tol,
+                    ptr(self.seg_off), h.ntraj, ts, hdr, wsp, wsn, cref, st),
+                    "batch_taylor_term_coded")
+
+            def fill(d):
+                d.nterms_h = hc.nterms
+                for t in range(hc.nterms):
+                    d.indptr[t] = hc.indptr[t]
+                    d.indices[t] = hc.indices[t]
+                    d.data[t] = None
+                    d.code[t] = hc.code[t]
+                d.coded = 1
+                d.sqrt_len = hc.sqrt_len
+                d.term_value = hc.term_value
+                d.sqrt_table = hc.sqrt_table
+            return diag, (hc,), call, fill
+        if mixed:
+            mm = engine.MixedMats(self.diag_vals, mats)
+            diag, sset, h64, h32 = mm.diag, mm.cset, None, C.byref(mm.cset)
+            keep = (mm,)
+        else:
+            sset = engine.csr_set(mats)
+            diag, h64, h32 = self.diag_vals, C.byref(sset), None
+            keep = (sset,)
+
+        def call(R, x_in, x_out, res, c, j, tol, ts, hdr, wsp, wsn, cref, st):
+            check(_lib.paces_batch_taylor_term_tuned(
+                R, ptr(diag), h64, h32, x_in, x_out, res, c, j, tol, ptr(self.seg_off), h.ntraj,
+                ts, hdr, wsp, wsn, cref, st), "batch_taylor_term")
+
+        def fill(d):
+            d.nterms_h = sset.nterms
+            for t in range(sset.nterms):
+                d.indptr[t] = sset.indptr[t]
+                d.indices[t] = sset.indices[t]
+                d.data[t] = sset.data[t]
+            d.coded = 0
+        return diag, keep, call, fill
+
+    def _propagate_graph(self, delta_t, fill, mixed, res, bufs, ts, hdr, diag, cfg, st):
         """Terms 0 .. n-1 by replaying a captured graph; returns n (the eager loop continues
         from there if some trajectory has not stopped)."""
         h = self.hamobj
@@ -821,13 +929,8 @@ class BatchedTimeEvolution:
         wsp, wsn = self._tws.get(_lib.paces_batch_taylor_workspace_bytes(cap, h.ntraj))
         d = _hip.BatchSeries()
         d.R = R
-        d.diag = dptr
-        sset = mats
-        d.nterms_h = sset.nterms
-        for t in range(sset.nterms):
-            d.indptr[t] = sset.indptr[t]
-            d.indices[t] = sset.indices[t]
-            d.data[t] = sset.data[t]
+        d.diag = ptr(diag)
+        fill(d)
         d.v = ptr(torch.view_as_real(self.vector))
         d.res = ptr(torch.view_as_real(res))
         for k in range(3):
@@ -870,19 +973,13 @@ class BatchedTimeEvolution:
         hdr = torch.zeros(16, dtype=torch.uint8, device=self.device)
         wsp, wsn = self._tws.get(_lib.paces_batch_taylor_workspace_bytes(R, h.ntraj))
         st = stream()
-        if mixed:
-            mm = engine.MixedMats(self.diag_vals, mats)
-            h64, h32, dptr = None, C.byref(mm.cset), ptr(mm.diag)
-        else:
-            s64 = engine.csr_set(mats)
-            h64, h32, dptr = C.byref(s64), None, ptr(self.diag_vals)
-        cfg = self._launch_shape(R, mats, mixed, dptr, h64, h32, wsp, wsn, st)
+        diag, keep, term, fill = self._term_operator(mats, mixed)
+        cfg = self._launch_shape(R, mats, mixed, term, wsp, wsn, st)
         self.last_launch_shape = (cfg.block, cfg.rows_per_thread) if cfg is not None else None
         cref = C.byref(cfg) if cfg is not None else None
         j = 0
         if self.graph:
-            j = self._propagate_graph(delta_t, mm.cset if mixed else s64, mixed, res, bufs, ts,
-                                      hdr, dptr, h64, h32, cfg, st)
+            j = self._propagate_graph(delta_t, fill, mixed, res, bufs, ts, hdr, diag, cfg, st)
             wsp, wsn = self._tws.get(0)  # the graph's workspace (same buffer, grown if needed)
             x_in = bufs[(j - 1) % 2]
         else:
@@ -900,11 +997,9 @@ class BatchedTimeEvolution:
                 if j >= self.m_star:
                     break
                 x_out = bufs[j % 2]
-                check(_lib.paces_batch_taylor_term_tuned(
-                    R, dptr, h64, h32, ptr(torch.view_as_real(x_in)),
-                    ptr(torch.view_as_real(x_out)), ptr(torch.view_as_real(res)),
-                    engine.taylor_coeff(delta_t, j), j, self.tol, ptr(self.seg_off), h.ntraj,
-                    ptr(ts), ptr(hdr), wsp, wsn, cref, st), "batch_taylor_term")
+                term(R, ptr(torch.view_as_real(x_in)), ptr(torch.view_as_real(x_out)),
+                     ptr(torch.view_as_real(res)), engine.taylor_coeff(delta_t, j), j, self.tol,
+                     ptr(ts), ptr(hdr), wsp, wsn, cref, st)
                 x_in = x_out
                 j += 1
             if int(hdr[:4].view(torch.int32).item()) == 0:
@@ -915,6 +1010,7 @@ class BatchedTimeEvolution:
                                              1 if mixed else 0, ptr(torch.view_as_real(res)),
                                              ptr(self.seg_off), h.ntraj, ptr(ts), wsp, wsn, cref,
                                              st), "batch_taylor_finish")
+        del keep
         self.vector = res
         # the per-trajectory series states go to pinned host memory without a stream sync; they
         # are parsed when first read (last_expm, or the next step's graph size)
@@ -939,7 +1035,7 @@ class BatchedTimeEvolution:
             self._expm_pending = None
         return self._expm_info
 
-    def _launch_shape(self, R, mats, mixed, dptr, h64, h32, wsp, wsn, st):
+    def _launch_shape(self, R, mats, mixed, term, wsp, wsn, st):
         """The tuned launch shape of the batched term (None: the default), timed on scratch
         buffers with the stopping test off the first time a size bucket is seen."""
         if self.launch_shape is not None:
@@ -951,7 +1047,7 @@ class BatchedTimeEvolution:
         h = self.hamobj
         vdt = torch.complex64 if mixed else torch.complex128
         tuner = autotune.default_tuner()
-        key = tuner.key("btay32" if mixed else "btay64", R, len(mats), h.ntraj)
+        key = tuner.key("btay32" if mixed else "btay64", R, len(mats), h.ntraj, self.csr_values)
 
         def run(shape):
             cfg = _hip.LaunchCfg(*shape)
@@ -965,10 +1061,9 @@ class BatchedTimeEvolution:
                 R, ptr(torch.view_as_real(self.vector)), ptr(torch.view_as_real(res)),
                 ptr(torch.view_as_real(xa)), 1 if mixed else 0, ptr(self.seg_off), h.ntraj,
                 ptr(ts), wsp, wsn, C.byref(cfg), st), "batch_taylor_begin_tuned")
-            return lambda: check(_lib.paces_batch_taylor_term_tuned(
-                R, dptr, h64, h32, ptr(torch.view_as_real(xa)), ptr(torch.view_as_real(xb)),
-                ptr(torch.view_as_real(res)), 1e-3, 0, -1.0, ptr(self.seg_off), h.ntraj, ptr(ts),
-                ptr(hdr), wsp, wsn, C.byref(cfg), st), "batch_taylor_term_tuned")
+            return lambda: term(R, ptr(torch.view_as_real(xa)), ptr(torch.view_as_real(xb)),
+                                ptr(torch.view_as_real(res)), 1e-3, 0, -1.0, ptr(ts), ptr(hdr),
+                                wsp, wsn, C.byref(cfg), st)
 
         return autotune.launch_cfg(tuner.pick(key, autotune.SHAPES, run))
 

@@ -148,3 +148,44 @@ def test_graph_series_equals_eager(precision):
         if i > 0:
             assert a.last_expm.tobytes() == b.last_expm.tobytes(), i
     assert len(runs[1]._graphs) >= 2  # several capacities were captured, then replayed
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("graph", [False, True])
+def test_coded_values_equal_explicit_values(precision, graph):
+    """The series over coded CSR values (a code byte per vib entry, each trajectory's values
+    formed in LDS) gives the explicit-value series' basis, amplitudes and stopping states bit for
+    bit at every step; the coded matrices' values, formed on demand, equal the FP64 fill's (and,
+    in FP32, the FP32 fill's) bit for bit."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from paces_amd.config import devices
+    from paces_amd.models import holstein
+    if not devices.configure_called:
+        devices.configure()
+    draws = _draws(6, seed=9)
+    runs = []
+    for csr in ("explicit", "coded"):
+        bham = holstein.BatchedHamiltonian([d[0] for d in draws], 8, [-d[1] for d in draws],
+                                           [d[2] for d in draws])
+        bte = holstein.BatchedTimeEvolution(bham, [d[3] for d in draws], enlarge_steps=1,
+                                            precision=precision, graph=graph, csr_values=csr)
+        bte.create_single_exciton_states([d[0] // 2 for d in draws])
+        runs.append(bte)
+    assert runs[1].csr_values == "coded"
+    for i in range(10):
+        for bte in runs:
+            bte.step(i, 0.05)
+        a, b = runs
+        assert torch.equal(a.whoami, b.whoami), i
+        assert torch.equal(a.vector, b.vector), i
+        if i > 0:
+            assert a.last_expm.tobytes() == b.last_expm.tobytes(), i
+    a, b = runs
+    for name in a.hamobj.offdiag_terms:
+        ma, mb = a.sparse_mats_dict[name], b.sparse_mats_dict[name]
+        assert torch.equal(ma.indptr, mb.indptr) and torch.equal(ma.indices, mb.indices), name
+        assert (mb.code is not None) == (name == "vib_coupling")
+        assert mb.data.dtype == torch.float64
+        assert torch.equal(mb.data.to(ma.data.dtype), ma.data), name
