@@ -160,11 +160,15 @@ def btay_term_bytes(R, mats, mixed, coded=False):
 
 
 # HBM bytes per row of one k_btay_term<float, B, RR> launch, per launch shape (the tuner picks
-# among shapes within a few percent of each other), from FETCH_SIZE and WRITE_SIZE counter passes
+# among shapes within a few percent of each other) and CSR value form (coded is the default; the
+# explicit-value figures are from before it), from FETCH_SIZE and WRITE_SIZE counter passes
 # over the roofline loop below with the shape fixed (--term-shape), each scaled by the counter's
 # reading for a 1 GiB device copy (profiles/k_btay_term_traj256_hbm_traffic.txt). Only the
 # mixed-precision kernel is measured; other shapes report null.
-BTAY_TRAFFIC_BYTES_PER_ROW = {("mixed", (256, 2), "explicit"): 89.72,
+BTAY_TRAFFIC_BYTES_PER_ROW = {("mixed", (256, 2), "coded"): 84.57,
+                              ("mixed", (256, 4), "coded"): 85.16,
+                              ("mixed", (512, 4), "coded"): 83.91,
+                              ("mixed", (256, 2), "explicit"): 89.72,
                               ("mixed", (256, 4), "explicit"): 88.19,
                               ("mixed", (512, 4), "explicit"): 87.42}
 
