@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(kBlock) k_zero_i32r(int32_t* __restrict__ p, i
 // copied global -> LDS by direct loads (16 bytes per lane, no registers), then searched, within
 // one barrier pair. A range is copied as the 16-byte blocks that hold it: a block never crosses
 // a page, so the few bytes read beyond the range cannot fault.
-constexpr int kMuWords = 3136;  // 24.5 KiB: with the run table, 6 workgroups per CU
+constexpr int kMuWords = 2560;  // 20 KiB: with the run table, 7 workgroups per CU
 
 struct MuTable {
   PACES_GPTR(const unsigned char) blk[PACES_MAX_RUNS];  // first 16-byte block of each range
@@ -301,20 +301,13 @@ struct MuTable {
 };
 
 template <int W>
-constexpr int mu_buf_words() {
-  return (int)(sizeof(LookupSmem<W>) + 7) / 8 > kMuWords ? (int)(sizeof(LookupSmem<W>) + 7) / 8
-                                                         : kMuWords;
-}
-
-template <int W>
 __global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, const int64_t* __restrict__ lohi,
                                                     const int32_t* __restrict__ sched, int per,
                                                     uint32_t* __restrict__ code) {
-  // one LDS buffer: a group of ranges, or (for a range larger than kMuWords) the fenced
-  // single-run search's table
-  __shared__ __align__(16) u64 buf[mu_buf_words<W>()];
+  // one LDS buffer for a group of ranges (a range larger than the buffer is searched in global
+  // memory instead: LDS per workgroup stays at the buffer, for occupancy)
+  __shared__ __align__(16) u64 buf[kMuWords];
   __shared__ MuTable T;
-  LookupSmem<W>& sm = *reinterpret_cast<LookupSmem<W>*>(buf);
   const PACES_LPTR(const unsigned char) lbuf = (PACES_LPTR(const unsigned char))buf;
   // workgroups go to the XCDs round robin: XCD x runs the x-th slice of the merged-order schedule
   const int j = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
@@ -373,11 +366,11 @@ __global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, const int64_t* __
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int g = 0; g < ng; ++g) {
     const int a = T.gs[g], b = T.gs[g + 1];
-    if (2 * T.nblk[a] > kMuWords) {  // the fenced search over the partition range
+    if (2 * T.nblk[a] > kMuWords) {  // a per-thread search over the partition range
       const int s = T.q[a];
       const int64_t len = R.off[s + 1] - R.off[s];
-      const int64_t lb =
-          block_lower_bound_range<W>(x, v, R.ptr[s], T.lohi[2 * s], T.lohi[2 * s + 1], sm);
+      const int64_t lb = v ? lower_bound_row<W>(R.ptr[s], T.lohi[2 * s], T.lohi[2 * s + 1], x)
+                           : 0;
       if (v && s < r && lb < len && eq_row<W>(x, R.ptr[s] + lb * W)) {
         keep = false;
         m += (int)lb + 1;  // the equal row of the earlier run comes first
