@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(kBlock) k_zero_i32r(int32_t* __restrict__ p, i
 // copied global -> LDS by direct loads (16 bytes per lane, no registers), then searched, within
 // one barrier pair. A range is copied as the 16-byte blocks that hold it: a block never crosses
 // a page, so the few bytes read beyond the range cannot fault.
-constexpr int kMuWords = 2560;  // 20 KiB: with the run table, 7 workgroups per CU
+constexpr int kMuWords = 2048;  // 16 KiB: with the run table, LDS leaves room for 9 workgroups per CU
 
 struct MuTable {
   PACES_GPTR(const unsigned char) blk[PACES_MAX_RUNS];  // first 16-byte block of each range
