@@ -349,8 +349,9 @@ __global__ void __launch_bounds__(kBlock) k_zero_i32(int32_t* __restrict__ p, in
 // first output index, and the run offsets. The emit pass applies the ops again and writes each
 // tile's images of op k in row order from there, with in-wave prefixes from ballots: no barrier
 // in either pass. The per-row reads every op shares (exciton position, lattice length, phonon
-// field) are formed once per row.
-constexpr int kImgSteps = 8;
+// field) are formed once per row. One row per lane measured fastest (more waves in flight to
+// hide the latencies; eight or two rows per lane were slower).
+constexpr int kImgSteps = 1;
 constexpr int kImgTile = 64 * kImgSteps;
 constexpr int kImgWaves = kBlock / 64;
 // rows of up to kImgHoldWords words are loaded for all steps up front; wider ones step by step
