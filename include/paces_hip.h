@@ -228,6 +228,9 @@ int paces_holstein_csr_fill(const uint64_t* rows, int64_t R, const paces_layout_
  * paces/models/holstein.py:280-287; sum_occ_pte_kernel, _holstein_kernels.py:183-210). */
 int paces_holstein_diag(const uint64_t* rows, int64_t R, const paces_layout_t* lay,
                         const paces_holstein_t* h, double* diag, void* stream);
+/* The same diagonal rounded to FP32 (for the mixed-precision series; no reference counterpart). */
+int paces_holstein_diag32(const uint64_t* rows, int64_t R, const paces_layout_t* lay,
+                          const paces_holstein_t* h, float* diag, void* stream);
 
 /* ---- H1: observables n_exc[N], n_pho[N], sum |psi|^2 (calculate_n_exc / calculate_n_pho,
  * paces/models/holstein.py:311-322; calc_bath_n_b_kernel, _holstein_kernels.py:215-255).

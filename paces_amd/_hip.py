@@ -159,6 +159,8 @@ _SIGNATURES = {
                                           C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), _P, _SZ,
                                           _P]),
     "paces_holstein_diag": (C.c_int, [_P, _I64, C.POINTER(Layout), C.POINTER(Holstein), _P, _P]),
+    "paces_holstein_diag32": (C.c_int, [_P, _I64, C.POINTER(Layout), C.POINTER(Holstein), _P,
+                                        _P]),
     "paces_holstein_observables_workspace_bytes": (_SZ, [_I64, C.POINTER(Holstein)]),
     "paces_holstein_observables": (C.c_int, [_P, _P, _I64, C.POINTER(Layout),
                                              C.POINTER(Holstein), _P, _P, _SZ, _P]),

@@ -1053,10 +1053,10 @@ static uint8_t* pairs_flags(void* ws, int64_t R, int nterms) {
 // sum is exact in FP64, so the result is the reference's rounding of the same expression.
 // ------------------------------------------------------------------------------------------------
 
-template <int W>
+template <int W, typename D>
 __global__ void __launch_bounds__(kBlock) k_diag(const u64* __restrict__ rows, int64_t R,
                                                  LayoutArg L, int pf, double eps, double hw,
-                                                 double de, double* __restrict__ diag) {
+                                                 double de, D* __restrict__ diag) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= R) return;
   const Row<W> r = load_row<W>(rows, i);
@@ -1068,7 +1068,7 @@ __global__ void __launch_bounds__(kBlock) k_diag(const u64* __restrict__ rows, i
       s += (long long)__popcll(r.w[w] & L.sum_mask[w * L.sum_bits + k]) << k;
   double d = eps + hw * (double)s;
   if (de != 0.0) d = d + de * (double)get_field<W>(r, L.off[pf], L.width[pf]);
-  diag[i] = d;
+  diag[i] = (D)d;  // FP32: the FP64 value rounded to nearest
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1241,10 +1241,14 @@ static int run_obs(const uint64_t* rows, const double* psi, int64_t R, const Lay
 
 template <int W>
 static int run_diag(const uint64_t* rows, int64_t R, const LayoutArg& L, int pf, double eps,
-                    double hw, double de, double* diag, hipStream_t st) {
+                    double hw, double de, double* diag, float* diag32, hipStream_t st) {
   if (R == 0) return PACES_OK;
-  hipLaunchKernelGGL(k_diag<W>, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st, (const u64*)rows,
-                     R, L, pf, eps, hw, de, diag);
+  if (diag32)
+    hipLaunchKernelGGL((k_diag<W, float>), dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
+                       (const u64*)rows, R, L, pf, eps, hw, de, diag32);
+  else
+    hipLaunchKernelGGL((k_diag<W, double>), dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
+                       (const u64*)rows, R, L, pf, eps, hw, de, diag);
   return launch_status();
 }
 
@@ -1686,7 +1690,17 @@ int paces_holstein_diag(const uint64_t* rows, int64_t R, const paces_layout_t* l
   if (R < 0 || (R > 0 && (!rows || !diag))) return PACES_E_INVALID;
   if (!lay->d_sum_mask || lay->sum_bits < 1 || lay->sum_bits > 32) return PACES_E_INVALID;
   PACES_DISPATCH_W(lay->W, run_diag, rows, R, larg(lay), h->batched ? 1 : 0, h->eps_sys,
-                   h->hbar_omega, h->delta_eps, diag, (hipStream_t)stream);
+                   h->hbar_omega, h->delta_eps, diag, nullptr, (hipStream_t)stream);
+}
+
+int paces_holstein_diag32(const uint64_t* rows, int64_t R, const paces_layout_t* lay,
+                          const paces_holstein_t* h, float* diag, void* stream) {
+  int rc = check_h(lay, h);
+  if (rc) return rc;
+  if (R < 0 || (R > 0 && (!rows || !diag))) return PACES_E_INVALID;
+  if (!lay->d_sum_mask || lay->sum_bits < 1 || lay->sum_bits > 32) return PACES_E_INVALID;
+  PACES_DISPATCH_W(lay->W, run_diag, rows, R, larg(lay), h->batched ? 1 : 0, h->eps_sys,
+                   h->hbar_omega, h->delta_eps, nullptr, diag, (hipStream_t)stream);
 }
 
 size_t paces_holstein_observables_workspace_bytes(int64_t R, const paces_holstein_t* h) {

@@ -166,13 +166,14 @@ class Hamiltonian(HamiltonianFramework):
         return None
 
     def generate_mel(self, basis_states, enlarge_steps=0, log_ind=1.2,
-                     value_dtype=torch.float64):
+                     value_dtype=torch.float64, diag_dtype=torch.float64):
         """((diag, new_inds), {term: CsrMatrix}, dbg) for the basis (hamiltonian.py:383-492).
 
         new_inds = the basis (after ``enlarge_steps`` expansions) plus its neighbours; each term's
         CSR holds the entries with at least one endpoint in that basis. value_dtype=torch.float32
         gives the CSR values rounded to FP32 (what the mixed-precision series reads);
-        value_dtype="coded" (batched trajectories) gives coded CSRs (engine.CodedCsrMatrix).
+        value_dtype="coded" (batched trajectories) gives coded CSRs (engine.CodedCsrMatrix);
+        diag_dtype=torch.float32 the diagonal rounded to FP32.
         """
         if basis_states.shape[1] != self.totwordwidth:
             raise ValueError("Shape of basis states does not match the specified number of bits.")
@@ -185,7 +186,7 @@ class Hamiltonian(HamiltonianFramework):
         new_inds, in_basis, cand_pos = engine.merge_unique(runs, self.totwordwidth, self._ws,
                                                            want_from0=True, want_cand_pos=True)
         self._check_basis_size(new_inds.shape[0])
-        diag = self.generate_mel_diag(new_inds)
+        diag = self.generate_mel_diag(new_inds, diag_dtype)
         mats = self._pairs_csr(new_inds.shape[0], basis.shape[0], offs, cand_pos, in_basis, src,
                                nup, basis, value_dtype, rows=new_inds)
         dbg = {}
@@ -297,13 +298,14 @@ class Hamiltonian(HamiltonianFramework):
                                            stream()), "holstein_csr_fill")
         return mats
 
-    def generate_mel_diag(self, basis_states):
-        """eps_sys + hbar_omega * sum(n) (+ delta_eps * pos) (holstein.py:280-287)."""
+    def generate_mel_diag(self, basis_states, dtype=torch.float64):
+        """eps_sys + hbar_omega * sum(n) (+ delta_eps * pos) (holstein.py:280-287); float32
+        gives each value rounded to nearest (what the mixed-precision series reads)."""
         R = basis_states.shape[0]
-        out = torch.empty(R, dtype=torch.float64, device=self.device)
-        check(_lib.paces_holstein_diag(ptr(basis_states) if R else None, R,
-                                       C.byref(self._layout), C.byref(self._hs),
-                                       ptr(out) if R else None, stream()), "holstein_diag")
+        out = torch.empty(R, dtype=dtype, device=self.device)
+        fn = _lib.paces_holstein_diag if dtype == torch.float64 else _lib.paces_holstein_diag32
+        check(fn(ptr(basis_states) if R else None, R, C.byref(self._layout), C.byref(self._hs),
+                 ptr(out) if R else None, stream()), "holstein_diag")
         return out
 
     def features(self, rows, psi, diag, nbasis, offs, cand_pos, src):
@@ -841,10 +843,11 @@ class BatchedTimeEvolution:
         select = self._select()
         # the mixed-precision series reads FP32 values only: the CSR is built with them
         vdt = torch.float32 if self.precision == "mixed" else torch.float64
+        ddt = vdt  # the diagonal too (diag_vals is FP32 in the mixed form)
         if self.csr_values == "coded":
             vdt = "coded"
         (diag, new_inds), mats, dbg = self.hamobj.generate_mel(select, self.enlarge_steps,
-                                                               value_dtype=vdt)
+                                                               value_dtype=vdt, diag_dtype=ddt)
         self.vector = engine.reassign(self.whoami, self.vector, new_inds, self._ws)
         self.whoami = new_inds
         self.diag_vals = diag

@@ -262,7 +262,7 @@ GEN_CASES = ["chain16", "ring64_small", "torus6", "square4_hop2"]
 @pytest.mark.parametrize("case", GEN_CASES)
 def test_expand_csr_diag_taylor_observables(env, case):
     """generate_mel on the oracle's own step-1 basis: rows, per-term CSR (FP64 values, and FP32
-    values equal to their rounding), diagonal bit-exact;
+    values equal to their rounding), diagonal bit-exact (and its FP32 rounding);
     H.x and the Taylor series bit-exact; observables to 1e-12."""
     import sys
     import os
@@ -281,6 +281,8 @@ def test_expand_csr_diag_taylor_observables(env, case):
     np.testing.assert_array_equal(_np(env, ham.last_basis_e), rec.basis_e)
     np.testing.assert_array_equal(_np(env, new_inds), rec.whoami)
     np.testing.assert_array_equal(diag.cpu().numpy(), rec.diag)
+    d32 = ham.generate_mel_diag(new_inds, torch.float32)
+    np.testing.assert_array_equal(d32.cpu().numpy(), rec.diag.astype(np.float32))
     for name in oham.offdiag_terms:
         ip, ix, dv = rec.csrs[name]
         m = mats[name]
