@@ -168,6 +168,17 @@ int paces_holstein_images_count(const uint64_t* basis, int64_t n, const paces_la
 int paces_holstein_images_emit(const uint64_t* basis, int64_t n, const paces_layout_t* lay,
                                const paces_holstein_t* h, uint64_t* out_images, int32_t* out_src,
                                uint8_t* out_nup, void* ws, size_t ws_bytes, void* stream);
+/* The same over the rows i with skip[i] == 0 only (the images of the rows an expansion added: the
+ * other rows' images are already in the expanded basis; no reference counterpart). out_src is
+ * the row's index in basis. Same workspace. */
+int paces_holstein_images_count_skip(const uint64_t* basis, int64_t n, const uint8_t* skip,
+                                     const paces_layout_t* lay, const paces_holstein_t* h,
+                                     int32_t* run_offsets, void* ws, size_t ws_bytes,
+                                     void* stream);
+int paces_holstein_images_emit_skip(const uint64_t* basis, int64_t n, const uint8_t* skip,
+                                    const paces_layout_t* lay, const paces_holstein_t* h,
+                                    uint64_t* out_images, int32_t* out_src, uint8_t* out_nup,
+                                    void* ws, size_t ws_bytes, void* stream);
 
 /* ---- D1/D2 from the union (no lookups): given the union of a basis (run 0, nbasis rows) and its
  * image runs (paces_merge_unique with cand_pos), the per-term CSR on the R union rows. Each image
@@ -208,6 +219,37 @@ int paces_holstein_pairs_fill32(int64_t R, int64_t nbasis, const int64_t* run_of
                                 const paces_holstein_t* h, int32_t* const* indptr,
                                 int32_t* const* indices, float* const* data, void* ws,
                                 size_t ws_bytes, void* stream);
+/* The image list in two parts (no reference counterpart: the reference forms the images of the
+ * whole expanded basis again). When the nbasis rows are themselves the union of a previous basis
+ * (nprev rows, run 0) with that basis's images, the images of those nprev rows are already
+ * known. The second union then takes only the images of the rows the expansion added
+ * (paces_holstein_images_*_skip with the expansion's out_from_run0 as skip); run_offsets,
+ * cand_pos, img_src and img_nup describe that union as above. prior describes the first:
+ * run_offsets (host) and img_src/img_nup of the previous basis's images, and cand_pos, that
+ * union's candidate positions (in the nbasis rows). The pairs then take every image of the
+ * nbasis rows; the output is the same matrix as from one full image list. For the fill, exactly
+ * one of data (FP64), data32 (FP32) or code (coded; basis and lay are then not read) is given. */
+typedef struct {
+  int64_t nprev;
+  const int64_t* run_offsets;
+  const int32_t* cand_pos;
+  const int32_t* img_src;
+  const uint8_t* img_nup;
+} paces_prior_images_t;
+int paces_holstein_pairs_count_split(int64_t R, int64_t nbasis, const int64_t* run_offsets,
+                                     const int32_t* cand_pos, const uint8_t* in_basis,
+                                     const int32_t* img_src, const paces_prior_images_t* prior,
+                                     const paces_holstein_t* h, int32_t* term_nnz, void* ws,
+                                     size_t ws_bytes, void* stream);
+int paces_holstein_pairs_fill_split(int64_t R, int64_t nbasis, const int64_t* run_offsets,
+                                    const int32_t* cand_pos, const uint8_t* in_basis,
+                                    const int32_t* img_src, const uint8_t* img_nup,
+                                    const paces_prior_images_t* prior, const uint64_t* basis,
+                                    const paces_layout_t* lay, const paces_holstein_t* h,
+                                    int32_t* const* indptr, int32_t* const* indices,
+                                    double* const* data, float* const* data32,
+                                    uint8_t* const* code, void* ws, size_t ws_bytes,
+                                    void* stream);
 
 /* ---- D1/D2: per-term canonical CSR of the Holstein H on the new basis (generate_mel +
  * _add_hc_and_generate_coo + coo_matrix().tocsr(): hamiltonian.py:383-547,

@@ -85,6 +85,12 @@ class CsrCoded(C.Structure):
                 ("sqrt_table", C.c_void_p)]
 
 
+class PriorImages(C.Structure):
+    """paces_prior_images_t: the previous expansion's part of the image list."""
+    _fields_ = [("nprev", C.c_int64), ("run_offsets", C.POINTER(C.c_int64)),
+                ("cand_pos", C.c_void_p), ("img_src", C.c_void_p), ("img_nup", C.c_void_p)]
+
+
 class BatchTaylor(C.Structure):
     _fields_ = [("c1", C.c_double), ("c1_plus_c2", C.c_double), ("ratio", C.c_double),
                 ("done", C.c_int32), ("j_ind", C.c_int32)]
@@ -134,6 +140,10 @@ _SIGNATURES = {
     "paces_select_topk": (C.c_int, [_P, _P, _I64, _I32, _I64, _P, _P, _P, _SZ, _P]),
     "paces_holstein_num_runs": (C.c_int, [C.POINTER(Holstein)]),
     "paces_holstein_images_workspace_bytes": (_SZ, [_I64, C.POINTER(Holstein)]),
+    "paces_holstein_images_count_skip": (C.c_int, [_P, _I64, _P, C.POINTER(Layout),
+                                                   C.POINTER(Holstein), _P, _P, _SZ, _P]),
+    "paces_holstein_images_emit_skip": (C.c_int, [_P, _I64, _P, C.POINTER(Layout),
+                                                  C.POINTER(Holstein), _P, _P, _P, _P, _SZ, _P]),
     "paces_holstein_images_count": (C.c_int, [_P, _I64, C.POINTER(Layout), C.POINTER(Holstein),
                                               _P, _P, _SZ, _P]),
     "paces_holstein_images_emit": (C.c_int, [_P, _I64, C.POINTER(Layout), C.POINTER(Holstein),
@@ -151,6 +161,14 @@ _SIGNATURES = {
                                               _SZ, _P]),
     "paces_holstein_pairs_fill_coded": (C.c_int, [_I64, _I64, C.POINTER(C.c_int64), _P, _P, _P,
                                                   _P, C.POINTER(Holstein), C.POINTER(_P),
+                                                  C.POINTER(_P), C.POINTER(_P), _P, _SZ, _P]),
+    "paces_holstein_pairs_count_split": (C.c_int, [_I64, _I64, C.POINTER(C.c_int64), _P, _P,
+                                                   _P, C.POINTER(PriorImages),
+                                                   C.POINTER(Holstein), _P, _P, _SZ, _P]),
+    "paces_holstein_pairs_fill_split": (C.c_int, [_I64, _I64, C.POINTER(C.c_int64), _P, _P, _P,
+                                                  _P, C.POINTER(PriorImages), _P,
+                                                  C.POINTER(Layout), C.POINTER(Holstein),
+                                                  C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
                                                   C.POINTER(_P), C.POINTER(_P), _P, _SZ, _P]),
     "paces_holstein_csr_workspace_bytes": (_SZ, [_I64, C.POINTER(Holstein)]),
     "paces_holstein_csr_count": (C.c_int, [_P, _I64, _P, C.POINTER(Layout), C.POINTER(Holstein),

@@ -366,6 +366,7 @@ static size_t images_ws(int64_t n, int nops) {
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_img_tcount(const u64* __restrict__ basis, int64_t n,
+                                                       const uint8_t* __restrict__ skip,
                                                        int64_t ntl, OpTable T, LayoutArg L,
                                                        ModelArg M, int32_t* __restrict__ tcnt) {
   const int64_t wt = (int64_t)blockIdx.x * kImgWaves + (threadIdx.x >> 6);
@@ -379,16 +380,16 @@ __global__ void __launch_bounds__(kBlock) k_img_tcount(const u64* __restrict__ b
 #pragma unroll
     for (int q = 0; q < kImgSteps; ++q) {
       const int64_t i = wt * kImgTile + q * 64 + lane;
-      vv[q] = i < n;
-      rr[q] = load_row<W>(basis, vv[q] ? i : 0);
+      vv[q] = i < n && !(skip && skip[i]);
+      rr[q] = load_row<W>(basis, i < n ? i : 0);
     }
     row_ctx_n<W, kImgSteps>(T, L, M, F, rr, cx);
   } else {
 #pragma unroll
     for (int q = 0; q < kImgSteps; ++q) {
       const int64_t i = wt * kImgTile + q * 64 + lane;
-      vv[q] = i < n;
-      cx[q] = row_ctx<W>(T, L, M, F, load_row<W>(basis, vv[q] ? i : 0));
+      vv[q] = i < n && !(skip && skip[i]);
+      cx[q] = row_ctx<W>(T, L, M, F, load_row<W>(basis, i < n ? i : 0));
     }
   }
   for (int k = 0; k < T.n; ++k) {
@@ -408,6 +409,7 @@ __global__ void k_img_run_offsets(const int32_t* __restrict__ TS, int64_t ntl, i
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_img_temit(const u64* __restrict__ basis, int64_t n,
+                                                      const uint8_t* __restrict__ skip,
                                                       int64_t ntl, OpTable T, LayoutArg L,
                                                       ModelArg M, const int32_t* __restrict__ TS,
                                                       u64* __restrict__ out,
@@ -438,8 +440,8 @@ __global__ void __launch_bounds__(kBlock) k_img_temit(const u64* __restrict__ ba
 #pragma unroll
   for (int q = 0; q < kImgSteps; ++q) {
     const int64_t i = wt * kImgTile + q * 64 + lane;
-    const bool v = i < n;
-    const Row<W> r0 = kHold ? rh[kHold ? q : 0] : load_row<W>(basis, v ? i : 0);
+    const bool v = i < n && !(skip && skip[i]);
+    const Row<W> r0 = kHold ? rh[kHold ? q : 0] : load_row<W>(basis, i < n ? i : 0);
     const RowCtx cx = kHold ? ch[kHold ? q : 0] : row_ctx<W>(T, L, M, F, r0);
     for (int k = 0; k < T.n; ++k) {
       const OpView o = op_view(T, k, F);
@@ -473,9 +475,9 @@ __global__ void __launch_bounds__(kBlock) k_img_temit(const u64* __restrict__ ba
 }
 
 template <int W>
-static int run_img_count(const uint64_t* basis, int64_t n, const LayoutArg& L, const ModelArg& M,
-                         const OpTable& T, int32_t* run_offsets, void* ws, size_t ws_bytes,
-                         hipStream_t st) {
+static int run_img_count(const uint64_t* basis, int64_t n, const uint8_t* skip, const LayoutArg& L,
+                         const ModelArg& M, const OpTable& T, int32_t* run_offsets, void* ws,
+                         size_t ws_bytes, hipStream_t st) {
   const int64_t ntl = img_tiles(n) > 0 ? img_tiles(n) : 1;
   const int64_t m = (int64_t)T.n * ntl;
   char* p = (char*)ws;
@@ -485,7 +487,7 @@ static int run_img_count(const uint64_t* basis, int64_t n, const LayoutArg& L, c
   p += a256((size_t)(m + 1) * 4);
   if (n > 0) {
     hipLaunchKernelGGL(k_img_tcount<W>, dim3(grid_for(ntl, kImgWaves)), dim3(kBlock), 0, st,
-                       (const u64*)basis, n, ntl, T, L, M, tcnt);
+                       (const u64*)basis, n, skip, ntl, T, L, M, tcnt);
   } else {
     hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, st, tcnt, m);
   }
@@ -499,9 +501,9 @@ static int run_img_count(const uint64_t* basis, int64_t n, const LayoutArg& L, c
 }
 
 template <int W>
-static int run_img_emit(const uint64_t* basis, int64_t n, const LayoutArg& L, const ModelArg& M,
-                        const OpTable& T, uint64_t* out, int32_t* out_src, uint8_t* out_nup,
-                        void* ws, hipStream_t st) {
+static int run_img_emit(const uint64_t* basis, int64_t n, const uint8_t* skip, const LayoutArg& L,
+                        const ModelArg& M, const OpTable& T, uint64_t* out, int32_t* out_src,
+                        uint8_t* out_nup, void* ws, hipStream_t st) {
   if (n == 0) return PACES_OK;
   const int64_t ntl = img_tiles(n);
   const int64_t m = (int64_t)T.n * ntl;
@@ -509,7 +511,7 @@ static int run_img_emit(const uint64_t* basis, int64_t n, const LayoutArg& L, co
   p += a256((size_t)m * 4);
   const int32_t* TS = (const int32_t*)p;
   hipLaunchKernelGGL(k_img_temit<W>, dim3(grid_for(ntl, kImgWaves)), dim3(kBlock), 0, st,
-                     (const u64*)basis, n, ntl, T, L, M, TS, (u64*)out, out_src, out_nup);
+                     (const u64*)basis, n, skip, ntl, T, L, M, TS, (u64*)out, out_src, out_nup);
   return launch_status();
 }
 
@@ -704,17 +706,56 @@ __device__ __forceinline__ int img_run_of(const ImgRuns& IR, int64_t e) {
   return k;
 }
 
+// A split image list (paces_prior_images_t): images e < A.n are the previous basis' images (the
+// last expansion's runs A.IR), then come the images of the rows that expansion added. A prior
+// image's position in the expanded basis is A.cpos[A.nprev + e] and its source's is
+// A.cpos[A.src[e]]; both map into the new basis through cand_pos (the union with the expanded
+// basis as run 0). A.n == 0: a plain list.
+struct PriorImgs {
+  int64_t n, nprev;
+  const int32_t* cpos;
+  const int32_t* src;
+  const uint8_t* nup;
+  ImgRuns IR;
+};
+
+// image e's op k, the new-basis positions p of the image and q of its source, and the source's
+// expanded-basis index s (its trajectory id)
+__device__ __forceinline__ void image_ends(int64_t e, const PriorImgs& A, const ImgRuns& IR,
+                                           int64_t nbasis, const int32_t* __restrict__ cand_pos,
+                                           const int32_t* __restrict__ src, int& k, int32_t& p,
+                                           int32_t& q, int32_t& s) {
+  if (e < A.n) {
+    k = img_run_of(A.IR, e);
+    const int32_t ca = A.cpos[A.nprev + e];
+    s = A.cpos[A.src[e]];
+    p = cand_pos[ca];
+    q = cand_pos[s];
+  } else {
+    const int64_t e2 = e - A.n;
+    k = img_run_of(IR, e2);
+    p = cand_pos[nbasis + e2];
+    s = src[e2];
+    q = cand_pos[s];
+  }
+}
+
+__device__ __forceinline__ int image_nup(int64_t e, const PriorImgs& A,
+                                         const uint8_t* __restrict__ nup) {
+  return e < A.n ? A.nup[e] : nup[e - A.n];
+}
+
 __global__ void __launch_bounds__(kBlock) k_pairs_count(int64_t R, int64_t nbasis, ImgRuns IR,
-                                                        OpTable T,
+                                                        PriorImgs A, OpTable T,
                                                         const int32_t* __restrict__ cand_pos,
                                                         const uint8_t* __restrict__ inB,
                                                         const int32_t* __restrict__ src,
                                                         int32_t* __restrict__ cnt) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= IR.off[IR.n]) return;
-  const int k = img_run_of(IR, e);
-  const int32_t p = cand_pos[nbasis + e];
-  const int32_t q = cand_pos[src[e]];
+  if (e >= A.n + IR.off[IR.n]) return;
+  int k;
+  int32_t p, q, s;
+  image_ends(e, A, IR, nbasis, cand_pos, src, k, p, q, s);
   if (p < 0 || q < 0) return;  // an endpoint pruned from the basis
   if (T.dir[k] < 0 && inB[p]) return;
   const int64_t t = T.term[k];
@@ -738,7 +779,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs_indptr(int64_t R, int nterms,
 }
 
 __global__ void __launch_bounds__(kBlock) k_pairs_fill(int64_t R, int64_t nbasis, ImgRuns IR,
-                                                       OpTable T, ModelArg M,
+                                                       PriorImgs A, OpTable T, ModelArg M,
                                                        const u64* __restrict__ basis, int W,
                                                        int trajbits,
                                                        const int32_t* __restrict__ cand_pos,
@@ -748,17 +789,17 @@ __global__ void __launch_bounds__(kBlock) k_pairs_fill(int64_t R, int64_t nbasis
                                                        const double* __restrict__ sqrt_tab,
                                                        int32_t* __restrict__ cursor, CsrOut out) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= IR.off[IR.n]) return;
-  const int k = img_run_of(IR, e);
-  const int32_t p = cand_pos[nbasis + e];
-  const int32_t q = cand_pos[src[e]];
+  if (e >= A.n + IR.off[IR.n]) return;
+  int k;
+  int32_t p, q, s;
+  image_ends(e, A, IR, nbasis, cand_pos, src, k, p, q, s);
   if (p < 0 || q < 0) return;
   if (T.dir[k] < 0 && inB[p]) return;
   const int t = T.term[k];
-  const int nu = nup[e];
+  const int nu = image_nup(e, A, nup);
   double v = 0.0;
   if (!out.coded) {
-    const int traj = M.batched ? (int)(basis[(int64_t)src[e] * W] >> (64 - trajbits)) : 0;
+    const int traj = M.batched ? (int)(basis[(int64_t)s * W] >> (64 - trajbits)) : 0;
     v = op_value(T, k, M, traj, sqrt_tab, nu);
   }
   const int32_t a = atomicAdd(&cursor[(int64_t)t * R + q], 1);
@@ -858,17 +899,16 @@ static bool build_slots(const OpTable& T, const paces_holstein_t* h, SlotArg* SL
 }
 
 __global__ void __launch_bounds__(kBlock) k_pairs_mark(int64_t R, int64_t nbasis, ImgRuns IR,
-                                                       OpTable T, SlotArg SL,
+                                                       PriorImgs A, OpTable T, SlotArg SL,
                                                        const int32_t* __restrict__ cand_pos,
                                                        const uint8_t* __restrict__ inB,
                                                        const int32_t* __restrict__ src,
                                                        uint8_t* __restrict__ flags) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= IR.off[IR.n]) return;
-  const int k = img_run_of(IR, e);
-  // both levels of loads issued before the tests (p clamped in bounds)
-  const int32_t p = cand_pos[nbasis + e];
-  const int32_t q = cand_pos[src[e]];
+  if (e >= A.n + IR.off[IR.n]) return;
+  int k;
+  int32_t p, q, s;
+  image_ends(e, A, IR, nbasis, cand_pos, src, k, p, q, s);
   const bool pin = inB[p < 0 ? 0 : p] != 0;
   if (p < 0 || q < 0) return;
   if (T.dir[k] < 0 && pin) return;
@@ -984,7 +1024,8 @@ __device__ __forceinline__ int32_t slot_place(const SlotArg& SL, int t, const ui
 }
 
 __global__ void __launch_bounds__(kBlock) k_pairs_place(int64_t R, int64_t nbasis, ImgRuns IR,
-                                                        OpTable T, SlotArg SL, ModelArg M,
+                                                        PriorImgs A, OpTable T, SlotArg SL,
+                                                        ModelArg M,
                                                         const u64* __restrict__ basis, int W,
                                                         int trajbits,
                                                         const int32_t* __restrict__ cand_pos,
@@ -995,16 +1036,15 @@ __global__ void __launch_bounds__(kBlock) k_pairs_place(int64_t R, int64_t nbasi
                                                         const uint8_t* __restrict__ flags,
                                                         CsrOut out) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= IR.off[IR.n]) return;
-  const int k = img_run_of(IR, e);
-  const int t = T.term[k], S = SL.S[t];
+  if (e >= A.n + IR.off[IR.n]) return;
   // every load is issued before the validity tests (indices clamped in bounds), so the
   // dependent levels overlap: (p, src, nup) -> (q, traj word, inB[p], p's slots and start) ->
-  // (q's slots and start)
-  const int32_t p = cand_pos[nbasis + e];
-  const int32_t s = src[e];
-  const int nu = nup[e];
-  const int32_t q = cand_pos[s];
+  // (q's slots and start); a prior image has one more level in front (its expanded-basis rows)
+  int k;
+  int32_t p, q, s;
+  image_ends(e, A, IR, nbasis, cand_pos, src, k, p, q, s);
+  const int nu = image_nup(e, A, nup);
+  const int t = T.term[k], S = SL.S[t];
   const int32_t pc = p < 0 ? 0 : p, qc = q < 0 ? 0 : q;
   const u64 w0 = (M.batched && !out.coded) ? basis[(int64_t)s * W] : 0;
   const bool pin = inB[pc] != 0;
@@ -1416,6 +1456,14 @@ size_t paces_holstein_images_workspace_bytes(int64_t n, const paces_holstein_t* 
 int paces_holstein_images_count(const uint64_t* basis, int64_t n, const paces_layout_t* lay,
                                 const paces_holstein_t* h, int32_t* run_offsets, void* ws,
                                 size_t ws_bytes, void* stream) {
+  return paces_holstein_images_count_skip(basis, n, nullptr, lay, h, run_offsets, ws, ws_bytes,
+                                          stream);
+}
+
+int paces_holstein_images_count_skip(const uint64_t* basis, int64_t n, const uint8_t* skip,
+                                     const paces_layout_t* lay, const paces_holstein_t* h,
+                                     int32_t* run_offsets, void* ws, size_t ws_bytes,
+                                     void* stream) {
   int rc = check_h(lay, h);
   if (rc) return rc;
   OpTable T;
@@ -1424,13 +1472,21 @@ int paces_holstein_images_count(const uint64_t* basis, int64_t n, const paces_la
   if (n < 0 || (n > 0 && !basis) || !run_offsets || !ws) return PACES_E_INVALID;
   if ((int64_t)T.n * n >= 2147483647LL) return PACES_E_INVALID;
   if (ws_bytes < images_ws(n, T.n)) return PACES_E_WORKSPACE;
-  PACES_DISPATCH_W(lay->W, run_img_count, basis, n, larg(lay), marg(h), T, run_offsets, ws,
+  PACES_DISPATCH_W(lay->W, run_img_count, basis, n, skip, larg(lay), marg(h), T, run_offsets, ws,
                    ws_bytes, (hipStream_t)stream);
 }
 
 int paces_holstein_images_emit(const uint64_t* basis, int64_t n, const paces_layout_t* lay,
                                const paces_holstein_t* h, uint64_t* out_images, int32_t* out_src,
                                uint8_t* out_nup, void* ws, size_t ws_bytes, void* stream) {
+  return paces_holstein_images_emit_skip(basis, n, nullptr, lay, h, out_images, out_src, out_nup,
+                                         ws, ws_bytes, stream);
+}
+
+int paces_holstein_images_emit_skip(const uint64_t* basis, int64_t n, const uint8_t* skip,
+                                    const paces_layout_t* lay, const paces_holstein_t* h,
+                                    uint64_t* out_images, int32_t* out_src, uint8_t* out_nup,
+                                    void* ws, size_t ws_bytes, void* stream) {
   int rc = check_h(lay, h);
   if (rc) return rc;
   OpTable T;
@@ -1438,8 +1494,8 @@ int paces_holstein_images_emit(const uint64_t* basis, int64_t n, const paces_lay
   if (rc) return rc;
   if (n < 0 || (n > 0 && (!basis || !out_images)) || !ws) return PACES_E_INVALID;
   if (ws_bytes < images_ws(n, T.n)) return PACES_E_WORKSPACE;
-  PACES_DISPATCH_W(lay->W, run_img_emit, basis, n, larg(lay), marg(h), T, out_images, out_src,
-                   out_nup, ws, (hipStream_t)stream);
+  PACES_DISPATCH_W(lay->W, run_img_emit, basis, n, skip, larg(lay), marg(h), T, out_images,
+                   out_src, out_nup, ws, (hipStream_t)stream);
 }
 
 static int img_runs_arg(const OpTable& T, const int64_t* run_offsets, ImgRuns* IR) {
@@ -1451,16 +1507,42 @@ static int img_runs_arg(const OpTable& T, const int64_t* run_offsets, ImgRuns* I
   return PACES_OK;
 }
 
+// the prior part of the image list (none without prior): run offsets on the host, checked as
+// the second part's are
+static int prior_arg(const OpTable& T, const paces_prior_images_t* pr, PriorImgs* A) {
+  A->n = 0;
+  A->nprev = 0;
+  A->cpos = nullptr;
+  A->src = nullptr;
+  A->nup = nullptr;
+  A->IR.n = T.n;
+  for (int k = 0; k <= kMaxOps; ++k) A->IR.off[k] = 0;
+  if (!pr) return PACES_OK;
+  if (!pr->run_offsets || pr->nprev < 0) return PACES_E_INVALID;
+  int rc = img_runs_arg(T, pr->run_offsets, &A->IR);
+  if (rc) return rc;
+  if (A->IR.off[0] != 0) return PACES_E_INVALID;
+  A->n = A->IR.off[A->IR.n];
+  A->nprev = pr->nprev;
+  A->cpos = pr->cand_pos;
+  A->src = pr->img_src;
+  A->nup = pr->img_nup;
+  if (A->n > 0 && (!A->cpos || !A->src || !A->nup)) return PACES_E_INVALID;
+  if (A->nprev + A->n >= 2147483647LL) return PACES_E_INVALID;
+  return PACES_OK;
+}
+
 size_t paces_holstein_pairs_workspace_bytes(int64_t R, const paces_holstein_t* h) {
   OpTable T;
   if (!h || build_image_ops(h, &T)) return 0;
   return pairs_ws(R, h->nterms, T.n);
 }
 
-int paces_holstein_pairs_count(int64_t R, int64_t nbasis, const int64_t* run_offsets,
-                               const int32_t* cand_pos, const uint8_t* in_basis,
-                               const int32_t* img_src, const paces_holstein_t* h,
-                               int32_t* term_nnz, void* ws, size_t ws_bytes, void* stream) {
+static int pairs_count_impl(int64_t R, int64_t nbasis, const int64_t* run_offsets,
+                            const int32_t* cand_pos, const uint8_t* in_basis,
+                            const int32_t* img_src, const paces_prior_images_t* prior,
+                            const paces_holstein_t* h, int32_t* term_nnz, void* ws,
+                            size_t ws_bytes, void* stream) {
   OpTable T;
   if (!h || !run_offsets || !term_nnz || !ws || R < 0 || nbasis < 0) return PACES_E_INVALID;
   if (h->nterms < 1 || h->nterms > PACES_MAX_TERMS) return PACES_E_UNSUPPORTED;
@@ -1469,8 +1551,12 @@ int paces_holstein_pairs_count(int64_t R, int64_t nbasis, const int64_t* run_off
   ImgRuns IR;
   rc = img_runs_arg(T, run_offsets, &IR);
   if (rc) return rc;
-  const int64_t nimg = IR.off[IR.n];
-  if (nimg > 0 && (!cand_pos || !in_basis || !img_src)) return PACES_E_INVALID;
+  PriorImgs A;
+  rc = prior_arg(T, prior, &A);
+  if (rc) return rc;
+  const int64_t nimg = A.n + IR.off[IR.n];
+  if (nimg > 0 && (!cand_pos || !in_basis || (IR.off[IR.n] > 0 && !img_src)))
+    return PACES_E_INVALID;
   if ((int64_t)h->nterms * R >= 2147483647LL) return PACES_E_INVALID;
   if (ws_bytes < pairs_ws(R, h->nterms, T.n)) return PACES_E_WORKSPACE;
   hipStream_t st = (hipStream_t)stream;
@@ -1489,7 +1575,7 @@ int paces_holstein_pairs_count(int64_t R, int64_t nbasis, const int64_t* run_off
                          nw);
     if (nimg > 0)
       hipLaunchKernelGGL(k_pairs_mark, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, R,
-                         nbasis, IR, T, SL, cand_pos, in_basis, img_src, flags);
+                         nbasis, IR, A, T, SL, cand_pos, in_basis, img_src, flags);
     // tile sums (in the count area) and their scan (in the prefix area) stand in for the row
     // counts and their scan; the fill call reads the same places
     const int ntl = (int)((R + kPairTile - 1) / kPairTile);
@@ -1513,7 +1599,7 @@ int paces_holstein_pairs_count(int64_t R, int64_t nbasis, const int64_t* run_off
     }
     if (nimg > 0) {
       hipLaunchKernelGGL(k_pairs_count, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, R,
-                         nbasis, IR, T, cand_pos, in_basis, img_src, cnt);
+                         nbasis, IR, A, T, cand_pos, in_basis, img_src, cnt);
     }
   }
   rc = launch_status();
@@ -1525,9 +1611,28 @@ int paces_holstein_pairs_count(int64_t R, int64_t nbasis, const int64_t* run_off
   return launch_status();
 }
 
+int paces_holstein_pairs_count(int64_t R, int64_t nbasis, const int64_t* run_offsets,
+                               const int32_t* cand_pos, const uint8_t* in_basis,
+                               const int32_t* img_src, const paces_holstein_t* h,
+                               int32_t* term_nnz, void* ws, size_t ws_bytes, void* stream) {
+  return pairs_count_impl(R, nbasis, run_offsets, cand_pos, in_basis, img_src, nullptr, h,
+                          term_nnz, ws, ws_bytes, stream);
+}
+
+int paces_holstein_pairs_count_split(int64_t R, int64_t nbasis, const int64_t* run_offsets,
+                                     const int32_t* cand_pos, const uint8_t* in_basis,
+                                     const int32_t* img_src, const paces_prior_images_t* prior,
+                                     const paces_holstein_t* h, int32_t* term_nnz, void* ws,
+                                     size_t ws_bytes, void* stream) {
+  if (!prior) return PACES_E_INVALID;
+  return pairs_count_impl(R, nbasis, run_offsets, cand_pos, in_basis, img_src, prior, h,
+                          term_nnz, ws, ws_bytes, stream);
+}
+
 static int pairs_fill_impl(int64_t R, int64_t nbasis, const int64_t* run_offsets,
                            const int32_t* cand_pos, const uint8_t* in_basis,
-                           const int32_t* img_src, const uint8_t* img_nup, const uint64_t* basis,
+                           const int32_t* img_src, const uint8_t* img_nup,
+                           const paces_prior_images_t* prior, const uint64_t* basis,
                            const paces_holstein_t* h, const paces_layout_t* lay,
                            int32_t* const* indptr, int32_t* const* indices, double* const* data,
                            float* const* data32, uint8_t* const* code, void* ws, size_t ws_bytes,
@@ -1546,8 +1651,12 @@ static int pairs_fill_impl(int64_t R, int64_t nbasis, const int64_t* run_offsets
   ImgRuns IR;
   rc = img_runs_arg(T, run_offsets, &IR);
   if (rc) return rc;
-  const int64_t nimg = IR.off[IR.n];
-  if (nimg > 0 && (!cand_pos || !in_basis || !img_src || !img_nup)) return PACES_E_INVALID;
+  PriorImgs A;
+  rc = prior_arg(T, prior, &A);
+  if (rc) return rc;
+  const int64_t nimg = A.n + IR.off[IR.n];
+  if (nimg > 0 && (!cand_pos || !in_basis || (IR.off[IR.n] > 0 && (!img_src || !img_nup))))
+    return PACES_E_INVALID;
   if (ws_bytes < pairs_ws(R, h->nterms, T.n)) return PACES_E_WORKSPACE;
   CsrOut out;
   for (int t = 0; t < PACES_MAX_TERMS; ++t) {
@@ -1578,7 +1687,7 @@ static int pairs_fill_impl(int64_t R, int64_t nbasis, const int64_t* run_offsets
                        (const uint8_t*)pairs_flags(ws, R, h->nterms), G, out);
     if (nimg > 0)
       hipLaunchKernelGGL(k_pairs_place, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, R,
-                         nbasis, IR, T, SL, marg(h), (const u64*)basis, W, trajbits, cand_pos,
+                         nbasis, IR, A, T, SL, marg(h), (const u64*)basis, W, trajbits, cand_pos,
                          in_basis, img_src, img_nup, h->d_sqrt_table,
                          (const uint8_t*)pairs_flags(ws, R, h->nterms), out);
     return launch_status();
@@ -1587,7 +1696,7 @@ static int pairs_fill_impl(int64_t R, int64_t nbasis, const int64_t* run_offsets
                      h->nterms, G, out, cursor);
   if (nimg > 0)
     hipLaunchKernelGGL(k_pairs_fill, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, R, nbasis,
-                       IR, T, marg(h), (const u64*)basis, W, trajbits, cand_pos, in_basis,
+                       IR, A, T, marg(h), (const u64*)basis, W, trajbits, cand_pos, in_basis,
                        img_src, img_nup, h->d_sqrt_table, cursor, out);
   hipLaunchKernelGGL(k_pairs_sort, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st, R, h->nterms,
                      out);
@@ -1602,8 +1711,9 @@ int paces_holstein_pairs_fill(int64_t R, int64_t nbasis, const int64_t* run_offs
                               int32_t* const* indices, double* const* data, void* ws,
                               size_t ws_bytes, void* stream) {
   if (!data) return PACES_E_INVALID;
-  return pairs_fill_impl(R, nbasis, run_offsets, cand_pos, in_basis, img_src, img_nup, basis, h,
-                         lay, indptr, indices, data, nullptr, nullptr, ws, ws_bytes, stream);
+  return pairs_fill_impl(R, nbasis, run_offsets, cand_pos, in_basis, img_src, img_nup, nullptr,
+                         basis, h, lay, indptr, indices, data, nullptr, nullptr, ws, ws_bytes,
+                         stream);
 }
 
 int paces_holstein_pairs_fill32(int64_t R, int64_t nbasis, const int64_t* run_offsets,
@@ -1614,8 +1724,9 @@ int paces_holstein_pairs_fill32(int64_t R, int64_t nbasis, const int64_t* run_of
                                 int32_t* const* indices, float* const* data, void* ws,
                                 size_t ws_bytes, void* stream) {
   if (!data) return PACES_E_INVALID;
-  return pairs_fill_impl(R, nbasis, run_offsets, cand_pos, in_basis, img_src, img_nup, basis, h,
-                         lay, indptr, indices, nullptr, data, nullptr, ws, ws_bytes, stream);
+  return pairs_fill_impl(R, nbasis, run_offsets, cand_pos, in_basis, img_src, img_nup, nullptr,
+                         basis, h, lay, indptr, indices, nullptr, data, nullptr, ws, ws_bytes,
+                         stream);
 }
 
 int paces_holstein_pairs_fill_coded(int64_t R, int64_t nbasis, const int64_t* run_offsets,
@@ -1626,8 +1737,23 @@ int paces_holstein_pairs_fill_coded(int64_t R, int64_t nbasis, const int64_t* ru
                                     size_t ws_bytes, void* stream) {
   if (!code) return PACES_E_INVALID;
   return pairs_fill_impl(R, nbasis, run_offsets, cand_pos, in_basis, img_src, img_nup, nullptr,
-                         h, nullptr, indptr, indices, nullptr, nullptr, code, ws, ws_bytes,
-                         stream);
+                         nullptr, h, nullptr, indptr, indices, nullptr, nullptr, code, ws,
+                         ws_bytes, stream);
+}
+
+int paces_holstein_pairs_fill_split(int64_t R, int64_t nbasis, const int64_t* run_offsets,
+                                    const int32_t* cand_pos, const uint8_t* in_basis,
+                                    const int32_t* img_src, const uint8_t* img_nup,
+                                    const paces_prior_images_t* prior, const uint64_t* basis,
+                                    const paces_layout_t* lay, const paces_holstein_t* h,
+                                    int32_t* const* indptr, int32_t* const* indices,
+                                    double* const* data, float* const* data32,
+                                    uint8_t* const* code, void* ws, size_t ws_bytes,
+                                    void* stream) {
+  if (!prior || (!data) + (!data32) + (!code) != 2) return PACES_E_INVALID;
+  return pairs_fill_impl(R, nbasis, run_offsets, cand_pos, in_basis, img_src, img_nup, prior,
+                         code ? nullptr : basis, h, code ? nullptr : lay, indptr, indices, data,
+                         data32, code, ws, ws_bytes, stream);
 }
 
 size_t paces_holstein_csr_workspace_bytes(int64_t R, const paces_holstein_t* h) {

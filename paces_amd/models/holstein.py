@@ -116,18 +116,24 @@ class Hamiltonian(HamiltonianFramework):
         self.add_n_at_site(arr, -1, -1)
 
     # ---- device expansion ---------------------------------------------------------------------------
-    def _images(self, basis, hstruct=None, want_src=False):
+    def _images(self, basis, hstruct=None, want_src=False, skip=None):
         """Image runs of a sorted unique basis: (images tensor, run offsets list) and, with
-        want_src, the source basis index and vib upper phonon count of every image."""
+        want_src, the source basis index and vib upper phonon count of every image. With skip
+        (uint8 per row), only the rows with skip == 0 give images."""
         h = self._hs if hstruct is None else hstruct
         n = basis.shape[0]
         nruns = _lib.paces_holstein_num_runs(C.byref(h))
         check(min(nruns, 0), "holstein_num_runs")
         offs = torch.empty(nruns + 1, dtype=torch.int32, device=self.device)
         wsp, wsn = self._ws.get(_lib.paces_holstein_images_workspace_bytes(n, C.byref(h)))
-        check(_lib.paces_holstein_images_count(ptr(basis) if n else None, n,
-                                               C.byref(self._layout), C.byref(h), ptr(offs), wsp,
-                                               wsn, stream()), "holstein_images_count")
+        if skip is not None:
+            check(_lib.paces_holstein_images_count_skip(
+                ptr(basis) if n else None, n, ptr(skip) if n else None, C.byref(self._layout),
+                C.byref(h), ptr(offs), wsp, wsn, stream()), "holstein_images_count_skip")
+        else:
+            check(_lib.paces_holstein_images_count(ptr(basis) if n else None, n,
+                                                   C.byref(self._layout), C.byref(h), ptr(offs),
+                                                   wsp, wsn, stream()), "holstein_images_count")
         offs_h = offs.tolist()
         imgs = torch.empty((max(offs_h[-1], 1), self.totwordwidth), dtype=self.dtype,
                            device=self.device)
@@ -135,10 +141,16 @@ class Hamiltonian(HamiltonianFramework):
         if want_src:
             src = torch.empty(max(offs_h[-1], 1), dtype=torch.int32, device=self.device)
             nup = torch.empty(max(offs_h[-1], 1), dtype=torch.uint8, device=self.device)
-        check(_lib.paces_holstein_images_emit(ptr(basis) if n else None, n,
-                                              C.byref(self._layout), C.byref(h), ptr(imgs),
-                                              ptr(src), ptr(nup), wsp, wsn, stream()),
-              "holstein_images_emit")
+        if skip is not None:
+            check(_lib.paces_holstein_images_emit_skip(
+                ptr(basis) if n else None, n, ptr(skip) if n else None, C.byref(self._layout),
+                C.byref(h), ptr(imgs), ptr(src), ptr(nup), wsp, wsn, stream()),
+                "holstein_images_emit_skip")
+        else:
+            check(_lib.paces_holstein_images_emit(ptr(basis) if n else None, n,
+                                                  C.byref(self._layout), C.byref(h), ptr(imgs),
+                                                  ptr(src), ptr(nup), wsp, wsn, stream()),
+                  "holstein_images_emit")
         if want_src:
             return imgs, offs_h, src, nup
         return imgs, offs_h
@@ -178,41 +190,85 @@ class Hamiltonian(HamiltonianFramework):
         if basis_states.shape[1] != self.totwordwidth:
             raise ValueError("Shape of basis states does not match the specified number of bits.")
         basis = basis_states
+        last = None  # the last expansion's images, with their union positions
+        prior = None
         for _ in range(enlarge_steps):
-            basis = self.enlarge_basis_set(basis)
-        imgs, offs, src, nup = self._images(basis, want_src=True)
-        runs = [(basis, 0, basis.shape[0])]
-        runs += [(imgs, offs[k], offs[k + 1] - offs[k]) for k in range(len(offs) - 1)]
-        new_inds, in_basis, cand_pos = engine.merge_unique(runs, self.totwordwidth, self._ws,
-                                                           want_from0=True, want_cand_pos=True)
+            imgs, offs, src, nup = self._images(basis, want_src=True)
+            runs = [(basis, 0, basis.shape[0])]
+            runs += [(imgs, offs[k], offs[k + 1] - offs[k]) for k in range(len(offs) - 1)]
+            rows, from0, cpos = engine.merge_unique(runs, self.totwordwidth, self._ws,
+                                                    want_from0=True, want_cand_pos=True)
+            last = (basis.shape[0], offs, src, nup, cpos, from0)
+            basis = rows
+        if last is None:
+            imgs, offs, src, nup = self._images(basis, want_src=True)
+            runs = [(basis, 0, basis.shape[0])]
+            runs += [(imgs, offs[k], offs[k + 1] - offs[k]) for k in range(len(offs) - 1)]
+            new_inds, in_basis, cand_pos = engine.merge_unique(
+                runs, self.totwordwidth, self._ws, want_from0=True, want_cand_pos=True)
+        else:
+            # The expanded basis already holds the images of the rows it was expanded from, so
+            # the new rows are its union with the images of the rows the expansion added only;
+            # the pairs still take every image of the expanded basis.
+            n_prev, offs_a, src_a, nup_a, cpos_a, from0_a = last
+            prior = (n_prev, offs_a, cpos_a, src_a, nup_a)
+            imgs, offs, src, nup = self._images(basis, want_src=True, skip=from0_a)
+            runs = [(basis, 0, basis.shape[0])]
+            runs += [(imgs, offs[k], offs[k + 1] - offs[k]) for k in range(len(offs) - 1)]
+            new_inds, in_basis, cand_pos = engine.merge_unique(
+                runs, self.totwordwidth, self._ws, want_from0=True, want_cand_pos=True)
         self._check_basis_size(new_inds.shape[0])
         diag = self.generate_mel_diag(new_inds, diag_dtype)
         mats = self._pairs_csr(new_inds.shape[0], basis.shape[0], offs, cand_pos, in_basis, src,
-                               nup, basis, value_dtype, rows=new_inds)
+                               nup, basis, value_dtype, rows=new_inds, prior=prior)
         dbg = {}
         if "vib_coupling" in self.offdiag_terms:
-            dbg["vib_coupling"] = [basis.shape[0] - self._vib_plus_count(offs)]
+            offs_all = offs if prior is None else [a + b for a, b in zip(prior[1], offs)]
+            dbg["vib_coupling"] = [basis.shape[0] - self._vib_plus_count(offs_all)]
         self.last_basis_e = basis
         return (diag, new_inds), mats, dbg
 
+    @staticmethod
+    def _prior_arg(prior):
+        """paces_prior_images_t from (n_prev, run offsets, union positions, source, phonon count)
+        of the previous expansion's images, or None."""
+        if prior is None:
+            return None
+        n_prev, offs_a, cpos_a, src_a, nup_a = prior
+        return _hip.PriorImages(n_prev, (C.c_int64 * len(offs_a))(*offs_a), ptr(cpos_a),
+                                ptr(src_a), ptr(nup_a))
+
+    def _pairs_count(self, R, nbasis, run_off, cand_pos, in_basis, src, pr):
+        """Per-term nnz (host list) and the workspace the fill call reuses."""
+        h = self._hs
+        nnz = torch.empty(max(h.nterms, 1), dtype=torch.int32, device=self.device)
+        wsp, wsn = self._ws.get(_lib.paces_holstein_pairs_workspace_bytes(R, C.byref(h)))
+        if pr is None:
+            check(_lib.paces_holstein_pairs_count(R, nbasis, run_off, ptr(cand_pos),
+                                                  ptr(in_basis), ptr(src), C.byref(h), ptr(nnz),
+                                                  wsp, wsn, stream()), "holstein_pairs_count")
+        else:
+            check(_lib.paces_holstein_pairs_count_split(R, nbasis, run_off, ptr(cand_pos),
+                                                        ptr(in_basis), ptr(src), C.byref(pr),
+                                                        C.byref(h), ptr(nnz), wsp, wsn,
+                                                        stream()), "holstein_pairs_count_split")
+        return nnz.tolist(), wsp, wsn
+
     def _pairs_csr(self, R, nbasis, offs, cand_pos, in_basis, src, nup, basis=None,
-                   value_dtype=torch.float64, rows=None):
+                   value_dtype=torch.float64, rows=None, prior=None):
         """Per-term CSR from the union's candidate positions (paces_holstein_pairs_*), with FP64
         or FP32 values, or coded values ("coded": batched trajectories; rows = the new basis,
-        for the values formed on demand)."""
+        for the values formed on demand). With prior (the previous expansion's images, see
+        _prior_arg), the image list comes in two parts (paces_holstein_pairs_*_split)."""
         if value_dtype == "coded":
-            return self._pairs_csr_coded(R, nbasis, offs, cand_pos, in_basis, src, nup, rows)
+            return self._pairs_csr_coded(R, nbasis, offs, cand_pos, in_basis, src, nup, rows,
+                                         prior)
         if value_dtype not in (torch.float64, torch.float32):
             raise ValueError("CSR values are float64, float32 or coded.")
         h = self._hs
-        nterms = h.nterms
         run_off = (C.c_int64 * len(offs))(*offs)
-        nnz = torch.empty(max(nterms, 1), dtype=torch.int32, device=self.device)
-        wsp, wsn = self._ws.get(_lib.paces_holstein_pairs_workspace_bytes(R, C.byref(h)))
-        check(_lib.paces_holstein_pairs_count(R, nbasis, run_off, ptr(cand_pos), ptr(in_basis),
-                                              ptr(src), C.byref(h), ptr(nnz), wsp, wsn, stream()),
-              "holstein_pairs_count")
-        nnz_h = nnz.tolist()
+        pr = self._prior_arg(prior)
+        nnz_h, wsp, wsn = self._pairs_count(R, nbasis, run_off, cand_pos, in_basis, src, pr)
         mats = {}
         arrs = [(C.c_void_p * _hip.MAX_TERMS)() for _ in range(3)]
         for t, name in enumerate(self.offdiag_terms):
@@ -221,6 +277,14 @@ class Hamiltonian(HamiltonianFramework):
             dv = torch.empty(max(nnz_h[t], 1), dtype=value_dtype, device=self.device)
             arrs[0][t], arrs[1][t], arrs[2][t] = ip.data_ptr(), ix.data_ptr(), dv.data_ptr()
             mats[name] = CsrMatrix(ip, ix[:nnz_h[t]], dv[:nnz_h[t]], (R, R))
+        if pr is not None:
+            f64 = value_dtype == torch.float64
+            check(_lib.paces_holstein_pairs_fill_split(
+                R, nbasis, run_off, ptr(cand_pos), ptr(in_basis), ptr(src), ptr(nup),
+                C.byref(pr), ptr(basis), C.byref(self._layout), C.byref(h), arrs[0], arrs[1],
+                arrs[2] if f64 else None, None if f64 else arrs[2], None, wsp, wsn, stream()),
+                "holstein_pairs_fill_split")
+            return mats
         fill = _lib.paces_holstein_pairs_fill if value_dtype == torch.float64 else \
             _lib.paces_holstein_pairs_fill32
         check(fill(R, nbasis, run_off, ptr(cand_pos), ptr(in_basis), ptr(src), ptr(nup),
@@ -228,18 +292,14 @@ class Hamiltonian(HamiltonianFramework):
                    wsn, stream()), "holstein_pairs_fill")
         return mats
 
-    def _pairs_csr_coded(self, R, nbasis, offs, cand_pos, in_basis, src, nup, rows):
+    def _pairs_csr_coded(self, R, nbasis, offs, cand_pos, in_basis, src, nup, rows, prior=None):
         """Coded per-term CSR (paces_holstein_pairs_fill_coded): a code byte per vib entry."""
         h = self._hs
         if not h.batched:
             raise NotImplementedError("Coded CSR values are for batched trajectories.")
         run_off = (C.c_int64 * len(offs))(*offs)
-        nnz = torch.empty(max(h.nterms, 1), dtype=torch.int32, device=self.device)
-        wsp, wsn = self._ws.get(_lib.paces_holstein_pairs_workspace_bytes(R, C.byref(h)))
-        check(_lib.paces_holstein_pairs_count(R, nbasis, run_off, ptr(cand_pos), ptr(in_basis),
-                                              ptr(src), C.byref(h), ptr(nnz), wsp, wsn, stream()),
-              "holstein_pairs_count")
-        nnz_h = nnz.tolist()
+        pr = self._prior_arg(prior)
+        nnz_h, wsp, wsn = self._pairs_count(R, nbasis, run_off, cand_pos, in_basis, src, pr)
         mats = {}
         arrs = [(C.c_void_p * _hip.MAX_TERMS)() for _ in range(3)]
         for t, name in enumerate(self.offdiag_terms):
@@ -253,6 +313,12 @@ class Hamiltonian(HamiltonianFramework):
             mats[name] = CodedCsrMatrix(ip, ix[:nnz_h[t]], cd[:nnz_h[t]] if cd is not None
                                         else None, (R, R), t,
                                         lambda m, rows=rows: self._coded_values(m, rows))
+        if pr is not None:
+            check(_lib.paces_holstein_pairs_fill_split(
+                R, nbasis, run_off, ptr(cand_pos), ptr(in_basis), ptr(src), ptr(nup),
+                C.byref(pr), None, None, C.byref(h), arrs[0], arrs[1], None, None, arrs[2], wsp,
+                wsn, stream()), "holstein_pairs_fill_split")
+            return mats
         check(_lib.paces_holstein_pairs_fill_coded(R, nbasis, run_off, ptr(cand_pos),
                                                    ptr(in_basis), ptr(src), ptr(nup), C.byref(h),
                                                    arrs[0], arrs[1], arrs[2], wsp, wsn, stream()),

@@ -323,6 +323,41 @@ def test_expand_csr_diag_taylor_observables(env, case):
     assert abs(np.sqrt(nrm2) - rec.obs["norm"]) < 1e-12
 
 
+@pytest.mark.parametrize("case", ["chain16", "square4_hop2"])
+def test_generate_mel_two_expansions_match_oracle(env, case):
+    """Two expansions before the pairs: the second union takes the images of the rows the last
+    expansion added, and the pairs take the earlier images as a separate part
+    (paces_holstein_pairs_*_split). Rows, expanded basis, diagonal, CSR entries and values (FP64
+    and FP32) and the vib ceiling count must equal the oracle's full-image-list result."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from holstein_cases import CASES, oracle_run
+    torch = env["torch"]
+    c = dict(CASES[case])
+    c["steps"] = 1
+    oham, recs, _ = oracle_run(c)
+    from paces_amd.models import holstein
+    ham = holstein.Hamiltonian(c["N"], [c["dim"]] * c["N"], geo_dim=c["geo"],
+                               periodic=c["periodic"], use_terms=c["terms"])
+    sel = recs[-1].select
+    diag, new_inds, coo, dbg, basis_e = oham.generate_mel(sel, 2)
+    R = new_inds.shape[0]
+    (gdiag, gnew), mats, gdbg = ham.generate_mel(_rows(env, sel), enlarge_steps=2)
+    np.testing.assert_array_equal(_np(env, ham.last_basis_e), basis_e)
+    np.testing.assert_array_equal(_np(env, gnew), new_inds)
+    np.testing.assert_array_equal(gdiag.cpu().numpy(), diag)
+    _, mats32, _ = ham.generate_mel(_rows(env, sel), enlarge_steps=2, value_dtype=torch.float32)
+    for name in oham.offdiag_terms:
+        ip, ix, dv = po.coo_to_csr(*coo[name], R)
+        for m, vals in ((mats[name], dv), (mats32[name], dv.astype(np.float32))):
+            np.testing.assert_array_equal(m.indptr.cpu().numpy(), ip)
+            np.testing.assert_array_equal(m.indices.cpu().numpy(), ix)
+            np.testing.assert_array_equal(m.data.cpu().numpy(), vals)
+    if "vib_coupling" in oham.offdiag_terms:
+        assert gdbg["vib_coupling"] == dbg["vib_coupling"]
+
+
 @pytest.mark.parametrize("layout_case", ["ring64_cap15", "torus16_cap7"])
 def test_generate_mel_at_word_straddling_fields(env, layout_case):
     """Bases concentrated on sites whose phonon field straddles a uint64 word boundary, with
