@@ -707,28 +707,37 @@ __device__ __forceinline__ int img_run_of(const ImgRuns& IR, int64_t e) {
 }
 
 // A split image list (paces_prior_images_t): images e < A.n are the previous basis' images (the
-// last expansion's runs A.IR), then come the images of the rows that expansion added. A prior
-// image's position in the expanded basis is A.cpos[A.nprev + e] and its source's is
-// A.cpos[A.src[e]]; both map into the new basis through cand_pos (the union with the expanded
-// basis as run 0). A.n == 0: a plain list.
+// last expansion's), then come the images of the rows that expansion added. A prior image's
+// position in the expanded basis is A.cpos[A.nprev + ea] and its source's is A.cpos[A.src[ea]]
+// (ea its index in the prior arrays); both map into the new basis through cand_pos (the union
+// with the expanded basis as run 0). Every prior image is a row of that basis, so the image of
+// an op of negative direction gives no entry (the pair comes from the inverse op's image of it,
+// as for any image inside the basis): A's index space covers the positive-direction runs only
+// (A.IR, with the other runs empty), and A.base[k] is where op k's run starts in the prior
+// arrays. A.n == 0: a plain list.
 struct PriorImgs {
   int64_t n, nprev;
   const int32_t* cpos;
   const int32_t* src;
   const uint8_t* nup;
   ImgRuns IR;
+  int64_t base[kMaxOps];
 };
 
-// image e's op k, the new-basis positions p of the image and q of its source, and the source's
-// expanded-basis index s (its trajectory id)
+// image e's op k, the new-basis positions p of the image and q of its source, the source's
+// expanded-basis index s (its trajectory id) and the image's upper phonon count nu (unused
+// outputs cost no load)
 __device__ __forceinline__ void image_ends(int64_t e, const PriorImgs& A, const ImgRuns& IR,
                                            int64_t nbasis, const int32_t* __restrict__ cand_pos,
-                                           const int32_t* __restrict__ src, int& k, int32_t& p,
-                                           int32_t& q, int32_t& s) {
+                                           const int32_t* __restrict__ src,
+                                           const uint8_t* __restrict__ nup, int& k, int32_t& p,
+                                           int32_t& q, int32_t& s, int& nu) {
   if (e < A.n) {
     k = img_run_of(A.IR, e);
-    const int32_t ca = A.cpos[A.nprev + e];
-    s = A.cpos[A.src[e]];
+    const int64_t ea = e - A.IR.off[k] + A.base[k];
+    const int32_t ca = A.cpos[A.nprev + ea];
+    s = A.cpos[A.src[ea]];
+    nu = A.nup[ea];
     p = cand_pos[ca];
     q = cand_pos[s];
   } else {
@@ -736,13 +745,9 @@ __device__ __forceinline__ void image_ends(int64_t e, const PriorImgs& A, const 
     k = img_run_of(IR, e2);
     p = cand_pos[nbasis + e2];
     s = src[e2];
+    nu = nup ? nup[e2] : 0;
     q = cand_pos[s];
   }
-}
-
-__device__ __forceinline__ int image_nup(int64_t e, const PriorImgs& A,
-                                         const uint8_t* __restrict__ nup) {
-  return e < A.n ? A.nup[e] : nup[e - A.n];
 }
 
 __global__ void __launch_bounds__(kBlock) k_pairs_count(int64_t R, int64_t nbasis, ImgRuns IR,
@@ -753,9 +758,9 @@ __global__ void __launch_bounds__(kBlock) k_pairs_count(int64_t R, int64_t nbasi
                                                         int32_t* __restrict__ cnt) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= A.n + IR.off[IR.n]) return;
-  int k;
+  int k, nu;
   int32_t p, q, s;
-  image_ends(e, A, IR, nbasis, cand_pos, src, k, p, q, s);
+  image_ends(e, A, IR, nbasis, cand_pos, src, nullptr, k, p, q, s, nu);
   if (p < 0 || q < 0) return;  // an endpoint pruned from the basis
   if (T.dir[k] < 0 && inB[p]) return;
   const int64_t t = T.term[k];
@@ -790,13 +795,12 @@ __global__ void __launch_bounds__(kBlock) k_pairs_fill(int64_t R, int64_t nbasis
                                                        int32_t* __restrict__ cursor, CsrOut out) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= A.n + IR.off[IR.n]) return;
-  int k;
+  int k, nu;
   int32_t p, q, s;
-  image_ends(e, A, IR, nbasis, cand_pos, src, k, p, q, s);
+  image_ends(e, A, IR, nbasis, cand_pos, src, nup, k, p, q, s, nu);
   if (p < 0 || q < 0) return;
   if (T.dir[k] < 0 && inB[p]) return;
   const int t = T.term[k];
-  const int nu = image_nup(e, A, nup);
   double v = 0.0;
   if (!out.coded) {
     const int traj = M.batched ? (int)(basis[(int64_t)s * W] >> (64 - trajbits)) : 0;
@@ -906,9 +910,9 @@ __global__ void __launch_bounds__(kBlock) k_pairs_mark(int64_t R, int64_t nbasis
                                                        uint8_t* __restrict__ flags) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= A.n + IR.off[IR.n]) return;
-  int k;
+  int k, nu;
   int32_t p, q, s;
-  image_ends(e, A, IR, nbasis, cand_pos, src, k, p, q, s);
+  image_ends(e, A, IR, nbasis, cand_pos, src, nullptr, k, p, q, s, nu);
   const bool pin = inB[p < 0 ? 0 : p] != 0;
   if (p < 0 || q < 0) return;
   if (T.dir[k] < 0 && pin) return;
@@ -1040,10 +1044,9 @@ __global__ void __launch_bounds__(kBlock) k_pairs_place(int64_t R, int64_t nbasi
   // every load is issued before the validity tests (indices clamped in bounds), so the
   // dependent levels overlap: (p, src, nup) -> (q, traj word, inB[p], p's slots and start) ->
   // (q's slots and start); a prior image has one more level in front (its expanded-basis rows)
-  int k;
+  int k, nu;
   int32_t p, q, s;
-  image_ends(e, A, IR, nbasis, cand_pos, src, k, p, q, s);
-  const int nu = image_nup(e, A, nup);
+  image_ends(e, A, IR, nbasis, cand_pos, src, nup, k, p, q, s, nu);
   const int t = T.term[k], S = SL.S[t];
   const int32_t pc = p < 0 ? 0 : p, qc = q < 0 ? 0 : q;
   const u64 w0 = (M.batched && !out.coded) ? basis[(int64_t)s * W] : 0;
@@ -1075,6 +1078,11 @@ __global__ void __launch_bounds__(kBlock) k_pairs_place(int64_t R, int64_t nbasi
   out.indices[t][b] = q;
   put_entry(out, t, b, v, nu);
 }
+
+// the largest argument list of the pairs kernels (k_pairs_place) stays within 4 KiB
+static_assert(sizeof(ImgRuns) + sizeof(PriorImgs) + sizeof(OpTable) + sizeof(SlotArg) +
+                  sizeof(ModelArg) + sizeof(CsrOut) + 16 * 8 <= 4096,
+              "pairs kernel arguments");
 
 static size_t pairs_ws(int64_t R, int nterms, int nops) {
   int64_t m = (int64_t)nterms * R;
@@ -1517,18 +1525,29 @@ static int prior_arg(const OpTable& T, const paces_prior_images_t* pr, PriorImgs
   A->nup = nullptr;
   A->IR.n = T.n;
   for (int k = 0; k <= kMaxOps; ++k) A->IR.off[k] = 0;
+  for (int k = 0; k < kMaxOps; ++k) A->base[k] = 0;
   if (!pr) return PACES_OK;
   if (!pr->run_offsets || pr->nprev < 0) return PACES_E_INVALID;
-  int rc = img_runs_arg(T, pr->run_offsets, &A->IR);
+  ImgRuns full;
+  int rc = img_runs_arg(T, pr->run_offsets, &full);
   if (rc) return rc;
-  if (A->IR.off[0] != 0) return PACES_E_INVALID;
-  A->n = A->IR.off[A->IR.n];
+  if (full.off[0] != 0) return PACES_E_INVALID;
+  // the positive-direction runs only (the other ops' prior images give no entry)
+  int64_t acc = 0;
+  for (int k = 0; k < T.n; ++k) {
+    A->base[k] = full.off[k];
+    A->IR.off[k] = acc;
+    if (T.dir[k] > 0) acc += full.off[k + 1] - full.off[k];
+  }
+  for (int k = T.n; k <= kMaxOps; ++k) A->IR.off[k] = acc;
+  for (int k = T.n; k < kMaxOps; ++k) A->base[k] = full.off[T.n];
+  A->n = acc;
   A->nprev = pr->nprev;
   A->cpos = pr->cand_pos;
   A->src = pr->img_src;
   A->nup = pr->img_nup;
   if (A->n > 0 && (!A->cpos || !A->src || !A->nup)) return PACES_E_INVALID;
-  if (A->nprev + A->n >= 2147483647LL) return PACES_E_INVALID;
+  if (A->nprev + full.off[T.n] >= 2147483647LL) return PACES_E_INVALID;
   return PACES_OK;
 }
 
