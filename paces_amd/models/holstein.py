@@ -27,6 +27,21 @@ from ..core.time_evolution import TimeEvolutionFramework
 _lib = _hip.load()
 
 
+# Outputs sized by an upper bound (so that their kernels are launched before the exact size
+# reaches the host) are only so sized up to this many bytes; beyond it the host waits for the size.
+_BOUND_BYTES = 8 << 30
+
+
+def _to_host_async(t):
+    """A pinned host copy of a small device tensor, queued on the current stream without a
+    sync: (host tensor, event); read the host tensor after event.synchronize()."""
+    host = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    host.copy_(t, non_blocking=True)
+    ev = torch.cuda.Event()
+    ev.record(torch.cuda.current_stream())
+    return host, ev
+
+
 class Hamiltonian(HamiltonianFramework):
     """Holstein Hamiltonian H = sum_terms + diag (holstein.py:22-287).
 
@@ -116,10 +131,23 @@ class Hamiltonian(HamiltonianFramework):
         self.add_n_at_site(arr, -1, -1)
 
     # ---- device expansion ---------------------------------------------------------------------------
-    def _images(self, basis, hstruct=None, want_src=False, skip=None):
+    def _pooled(self, key, n, dtype):
+        """A device tensor of n elements from a grow-only buffer kept under key across calls
+        (for outputs whose contents are dead by the next call with the same key): the output of
+        a size that changes every step then needs no new device allocation."""
+        pool = self.__dict__.setdefault("_pool", {})
+        t = pool.get(key)
+        if t is None or t.numel() < n or t.dtype != dtype:
+            t = torch.empty(int(n * 1.25) + 1, dtype=dtype, device=self.device)
+            pool[key] = t
+        return t[:n]
+
+    def _images(self, basis, hstruct=None, want_src=False, skip=None, slot=None):
         """Image runs of a sorted unique basis: (images tensor, run offsets list) and, with
         want_src, the source basis index and vib upper phonon count of every image. With skip
-        (uint8 per row), only the rows with skip == 0 give images."""
+        (uint8 per row), only the rows with skip == 0 give images. With slot, the outputs live in
+        buffers kept under that name (_pooled): the next call with the same slot overwrites
+        them."""
         h = self._hs if hstruct is None else hstruct
         n = basis.shape[0]
         nruns = _lib.paces_holstein_num_runs(C.byref(h))
@@ -134,13 +162,27 @@ class Hamiltonian(HamiltonianFramework):
             check(_lib.paces_holstein_images_count(ptr(basis) if n else None, n,
                                                    C.byref(self._layout), C.byref(h), ptr(offs),
                                                    wsp, wsn, stream()), "holstein_images_count")
-        offs_h = offs.tolist()
-        imgs = torch.empty((max(offs_h[-1], 1), self.totwordwidth), dtype=self.dtype,
-                           device=self.device)
+        # The emit is launched before the offsets reach the host: a row gives at most one image
+        # per run, so n * nruns rows bound the output, and the host reads the offsets while the
+        # emit runs.
+        offs_host, ev = _to_host_async(offs)
+        cap = max(n * nruns, 1)
+        if cap * (8 * self.totwordwidth + 5) > _BOUND_BYTES:
+            ev.synchronize()
+            cap = max(int(offs_host[-1]), 1)
+        W = self.totwordwidth
+        if slot is None:
+            imgs = torch.empty((cap, W), dtype=self.dtype, device=self.device)
+        else:
+            imgs = self._pooled(slot + ".rows", cap * W, self.dtype).view(cap, W)
         src = nup = None
         if want_src:
-            src = torch.empty(max(offs_h[-1], 1), dtype=torch.int32, device=self.device)
-            nup = torch.empty(max(offs_h[-1], 1), dtype=torch.uint8, device=self.device)
+            if slot is None:
+                src = torch.empty(cap, dtype=torch.int32, device=self.device)
+                nup = torch.empty(cap, dtype=torch.uint8, device=self.device)
+            else:
+                src = self._pooled(slot + ".src", cap, torch.int32)
+                nup = self._pooled(slot + ".nup", cap, torch.uint8)
         if skip is not None:
             check(_lib.paces_holstein_images_emit_skip(
                 ptr(basis) if n else None, n, ptr(skip) if n else None, C.byref(self._layout),
@@ -151,9 +193,12 @@ class Hamiltonian(HamiltonianFramework):
                                                   C.byref(self._layout), C.byref(h), ptr(imgs),
                                                   ptr(src), ptr(nup), wsp, wsn, stream()),
                   "holstein_images_emit")
+        ev.synchronize()
+        offs_h = offs_host.tolist()
+        m = max(offs_h[-1], 1)
         if want_src:
-            return imgs, offs_h, src, nup
-        return imgs, offs_h
+            return imgs[:m], offs_h, src[:m], nup[:m]
+        return imgs[:m], offs_h
 
     def _expand(self, basis, want_from0=False):
         imgs, offs = self._images(basis)
@@ -193,7 +238,7 @@ class Hamiltonian(HamiltonianFramework):
         last = None  # the last expansion's images, with their union positions
         prior = None
         for _ in range(enlarge_steps):
-            imgs, offs, src, nup = self._images(basis, want_src=True)
+            imgs, offs, src, nup = self._images(basis, want_src=True, slot="expand")
             runs = [(basis, 0, basis.shape[0])]
             runs += [(imgs, offs[k], offs[k + 1] - offs[k]) for k in range(len(offs) - 1)]
             rows, from0, cpos = engine.merge_unique(runs, self.totwordwidth, self._ws,
@@ -212,7 +257,8 @@ class Hamiltonian(HamiltonianFramework):
             # the pairs still take every image of the expanded basis.
             n_prev, offs_a, src_a, nup_a, cpos_a, from0_a = last
             prior = (n_prev, offs_a, cpos_a, src_a, nup_a)
-            imgs, offs, src, nup = self._images(basis, want_src=True, skip=from0_a)
+            imgs, offs, src, nup = self._images(basis, want_src=True, skip=from0_a,
+                                                slot="expand2")
             runs = [(basis, 0, basis.shape[0])]
             runs += [(imgs, offs[k], offs[k + 1] - offs[k]) for k in range(len(offs) - 1)]
             new_inds, in_basis, cand_pos = engine.merge_unique(
@@ -239,7 +285,9 @@ class Hamiltonian(HamiltonianFramework):
                                 ptr(src_a), ptr(nup_a))
 
     def _pairs_count(self, R, nbasis, run_off, cand_pos, in_basis, src, pr):
-        """Per-term nnz (host list) and the workspace the fill call reuses."""
+        """Per-term nnz on its way to the host ((pinned tensor, event): the fill is launched
+        before it arrives), an upper bound of every term's nnz (two entries per image) and the
+        workspace the fill call reuses."""
         h = self._hs
         nnz = torch.empty(max(h.nterms, 1), dtype=torch.int32, device=self.device)
         wsp, wsn = self._ws.get(_lib.paces_holstein_pairs_workspace_bytes(R, C.byref(h)))
@@ -252,7 +300,13 @@ class Hamiltonian(HamiltonianFramework):
                                                         ptr(in_basis), ptr(src), C.byref(pr),
                                                         C.byref(h), ptr(nnz), wsp, wsn,
                                                         stream()), "holstein_pairs_count_split")
-        return nnz.tolist(), wsp, wsn
+        nimg = run_off[len(run_off) - 1] + (pr.run_offsets[len(run_off) - 1] if pr else 0)
+        nnz_host, ev = _to_host_async(nnz)
+        cap = max(2 * nimg, 1)
+        if cap * 12 * h.nterms > _BOUND_BYTES:
+            ev.synchronize()
+            cap = max(max(nnz_host.tolist()), 1)
+        return (nnz_host, ev), cap, wsp, wsn
 
     def _pairs_csr(self, R, nbasis, offs, cand_pos, in_basis, src, nup, basis=None,
                    value_dtype=torch.float64, rows=None, prior=None):
@@ -268,15 +322,16 @@ class Hamiltonian(HamiltonianFramework):
         h = self._hs
         run_off = (C.c_int64 * len(offs))(*offs)
         pr = self._prior_arg(prior)
-        nnz_h, wsp, wsn = self._pairs_count(R, nbasis, run_off, cand_pos, in_basis, src, pr)
-        mats = {}
+        (nnz_host, ev), cap, wsp, wsn = self._pairs_count(R, nbasis, run_off, cand_pos, in_basis,
+                                                          src, pr)
+        parts = []
         arrs = [(C.c_void_p * _hip.MAX_TERMS)() for _ in range(3)]
         for t, name in enumerate(self.offdiag_terms):
             ip = torch.empty(R + 1, dtype=torch.int32, device=self.device)
-            ix = torch.empty(max(nnz_h[t], 1), dtype=torch.int32, device=self.device)
-            dv = torch.empty(max(nnz_h[t], 1), dtype=value_dtype, device=self.device)
+            ix = torch.empty(cap, dtype=torch.int32, device=self.device)
+            dv = torch.empty(cap, dtype=value_dtype, device=self.device)
             arrs[0][t], arrs[1][t], arrs[2][t] = ip.data_ptr(), ix.data_ptr(), dv.data_ptr()
-            mats[name] = CsrMatrix(ip, ix[:nnz_h[t]], dv[:nnz_h[t]], (R, R))
+            parts.append((name, ip, ix, dv))
         if pr is not None:
             f64 = value_dtype == torch.float64
             check(_lib.paces_holstein_pairs_fill_split(
@@ -284,13 +339,16 @@ class Hamiltonian(HamiltonianFramework):
                 C.byref(pr), ptr(basis), C.byref(self._layout), C.byref(h), arrs[0], arrs[1],
                 arrs[2] if f64 else None, None if f64 else arrs[2], None, wsp, wsn, stream()),
                 "holstein_pairs_fill_split")
-            return mats
-        fill = _lib.paces_holstein_pairs_fill if value_dtype == torch.float64 else \
-            _lib.paces_holstein_pairs_fill32
-        check(fill(R, nbasis, run_off, ptr(cand_pos), ptr(in_basis), ptr(src), ptr(nup),
-                   ptr(basis), C.byref(self._layout), C.byref(h), arrs[0], arrs[1], arrs[2], wsp,
-                   wsn, stream()), "holstein_pairs_fill")
-        return mats
+        else:
+            fill = _lib.paces_holstein_pairs_fill if value_dtype == torch.float64 else \
+                _lib.paces_holstein_pairs_fill32
+            check(fill(R, nbasis, run_off, ptr(cand_pos), ptr(in_basis), ptr(src), ptr(nup),
+                       ptr(basis), C.byref(self._layout), C.byref(h), arrs[0], arrs[1], arrs[2],
+                       wsp, wsn, stream()), "holstein_pairs_fill")
+        ev.synchronize()
+        nnz_h = nnz_host.tolist()
+        return {name: CsrMatrix(ip, ix[:nnz_h[t]], dv[:nnz_h[t]], (R, R))
+                for t, (name, ip, ix, dv) in enumerate(parts)}
 
     def _pairs_csr_coded(self, R, nbasis, offs, cand_pos, in_basis, src, nup, rows, prior=None):
         """Coded per-term CSR (paces_holstein_pairs_fill_coded): a code byte per vib entry."""
@@ -299,31 +357,34 @@ class Hamiltonian(HamiltonianFramework):
             raise NotImplementedError("Coded CSR values are for batched trajectories.")
         run_off = (C.c_int64 * len(offs))(*offs)
         pr = self._prior_arg(prior)
-        nnz_h, wsp, wsn = self._pairs_count(R, nbasis, run_off, cand_pos, in_basis, src, pr)
-        mats = {}
+        (nnz_host, ev), cap, wsp, wsn = self._pairs_count(R, nbasis, run_off, cand_pos, in_basis,
+                                                          src, pr)
+        parts = []
         arrs = [(C.c_void_p * _hip.MAX_TERMS)() for _ in range(3)]
         for t, name in enumerate(self.offdiag_terms):
             ip = torch.empty(R + 1, dtype=torch.int32, device=self.device)
-            ix = torch.empty(max(nnz_h[t], 1), dtype=torch.int32, device=self.device)
+            ix = torch.empty(cap, dtype=torch.int32, device=self.device)
             cd = None
             if name == "vib_coupling":
-                cd = torch.empty(max(nnz_h[t], 1), dtype=torch.uint8, device=self.device)
+                cd = torch.empty(cap, dtype=torch.uint8, device=self.device)
                 arrs[2][t] = cd.data_ptr()
             arrs[0][t], arrs[1][t] = ip.data_ptr(), ix.data_ptr()
-            mats[name] = CodedCsrMatrix(ip, ix[:nnz_h[t]], cd[:nnz_h[t]] if cd is not None
-                                        else None, (R, R), t,
-                                        lambda m, rows=rows: self._coded_values(m, rows))
+            parts.append((name, ip, ix, cd))
         if pr is not None:
             check(_lib.paces_holstein_pairs_fill_split(
                 R, nbasis, run_off, ptr(cand_pos), ptr(in_basis), ptr(src), ptr(nup),
                 C.byref(pr), None, None, C.byref(h), arrs[0], arrs[1], None, None, arrs[2], wsp,
                 wsn, stream()), "holstein_pairs_fill_split")
-            return mats
-        check(_lib.paces_holstein_pairs_fill_coded(R, nbasis, run_off, ptr(cand_pos),
-                                                   ptr(in_basis), ptr(src), ptr(nup), C.byref(h),
-                                                   arrs[0], arrs[1], arrs[2], wsp, wsn, stream()),
-              "holstein_pairs_fill_coded")
-        return mats
+        else:
+            check(_lib.paces_holstein_pairs_fill_coded(R, nbasis, run_off, ptr(cand_pos),
+                                                       ptr(in_basis), ptr(src), ptr(nup),
+                                                       C.byref(h), arrs[0], arrs[1], arrs[2], wsp,
+                                                       wsn, stream()), "holstein_pairs_fill_coded")
+        ev.synchronize()
+        nnz_h = nnz_host.tolist()
+        return {name: CodedCsrMatrix(ip, ix[:nnz_h[t]], cd[:nnz_h[t]] if cd is not None else None,
+                                     (R, R), t, lambda m, rows=rows: self._coded_values(m, rows))
+                for t, (name, ip, ix, cd) in enumerate(parts)}
 
     def _coded_values(self, m, rows):
         """float64 values of a coded CSR: the term value of each entry's row's trajectory, times
