@@ -53,6 +53,26 @@ def test_ctypes_binding_covers_header_and_host_only_calls():
     # argument validation runs on the host and returns a status, no device work
     assert lib.paces_lower_bound(None, -1, None, 0, 1, 1, None, None, None, None, 0, None) == -1
     assert lib.paces_merge_unique(None, 1, None, None, None, None, None, 0, None) == -1
+    # the split pairs entry points need the prior part, a valid one, and one value form
+    h.periodic, h.geo_dim, h.lattice_l, h.nchain = 0, 1, 16, 16
+    offs = (C.c_int64 * 5)(0, 1, 2, 3, 4)
+    ws = C.c_void_p(256)
+    nnz = C.c_void_p(256)
+    assert lib.paces_holstein_pairs_count_split(8, 4, offs, None, None, None, None, C.byref(h),
+                                                nnz, ws, 1 << 20, None) == -1
+    bad = _hip.PriorImages(-1, offs, None, None, None)
+    assert lib.paces_holstein_pairs_count_split(8, 4, offs, None, None, None, C.byref(bad),
+                                                C.byref(h), nnz, ws, 1 << 20, None) == -1
+    unsorted = _hip.PriorImages(2, (C.c_int64 * 5)(0, 3, 2, 3, 4), None, None, None)
+    assert lib.paces_holstein_pairs_count_split(8, 4, offs, None, None, None, C.byref(unsorted),
+                                                C.byref(h), nnz, ws, 1 << 20, None) == -1
+    arr = (C.c_void_p * _hip.MAX_TERMS)(256, 256)
+    good = _hip.PriorImages(2, offs, 256, 256, 256)
+    for vals in ((arr, arr, None), (None, None, None)):
+        assert lib.paces_holstein_pairs_fill_split(8, 4, offs, None, None, None, None,
+                                                   C.byref(good), None, None, C.byref(h), arr,
+                                                   arr, vals[0], vals[1], vals[2], ws, 1 << 20,
+                                                   None) == -1
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
