@@ -273,6 +273,16 @@ int paces_holstein_diag(const uint64_t* rows, int64_t R, const paces_layout_t* l
 /* The same diagonal rounded to FP32 (for the mixed-precision series; no reference counterpart). */
 int paces_holstein_diag32(const uint64_t* rows, int64_t R, const paces_layout_t* lay,
                           const paces_holstein_t* h, float* diag, void* stream);
+/* paces_merge_unique that also writes the diagonal of every output row, as
+ * paces_holstein_diag (diag, FP64) or paces_holstein_diag32 (diag32, FP32) would: exactly one of
+ * them, each with room for sum(len) values (the output count is not known before the call). The
+ * new basis's rows and diagonal come from one pass (no reference counterpart: the reference
+ * forms the rows with cupy_unique, hamiltonian.py:434-452, then the diagonal,
+ * holstein.py:280-287). Same workspace as paces_merge_unique; lay->W must equal W. */
+int paces_merge_unique_diag(const paces_runs_t* runs, int32_t W, uint64_t* out_rows,
+                            int32_t* out_count, uint8_t* out_from_run0, int32_t* cand_pos,
+                            const paces_layout_t* lay, const paces_holstein_t* h, double* diag,
+                            float* diag32, void* ws, size_t ws_bytes, void* stream);
 
 /* ---- H1: observables n_exc[N], n_pho[N], sum |psi|^2 (calculate_n_exc / calculate_n_pho,
  * paces/models/holstein.py:311-322; calc_bath_n_b_kernel, _holstein_kernels.py:215-255).

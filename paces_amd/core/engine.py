@@ -128,9 +128,11 @@ def lower_bound(book, q, want_hit=True, count_misses=False, q_sorted=False, ws=N
     return idx, hit, miss
 
 
-def merge_unique(runs, W, ws, want_from0=False, want_cand_pos=False):
+def merge_unique(runs, W, ws, want_from0=False, want_cand_pos=False, diag=None):
     """Sorted union of sorted unique row runs; returns (rows, from_run0 flags or None) and, with
-    want_cand_pos, the output index of every input row as a third element."""
+    want_cand_pos, the output index of every input row as a third element. With diag =
+    (layout, holstein struct, torch.float64 or torch.float32), the Holstein diagonal of the rows
+    comes from the same pass (paces_merge_unique_diag) as a last element."""
     if len(runs) > _hip.MAX_RUNS:
         raise ValueError("too many runs")
     R = _hip.Runs()
@@ -147,12 +149,25 @@ def merge_unique(runs, W, ws, want_from0=False, want_cand_pos=False):
     from0 = torch.empty(max(total, 1), dtype=torch.uint8, device=dev) if want_from0 else None
     cpos = torch.empty(max(total, 1), dtype=torch.int32, device=dev) if want_cand_pos else None
     wsp, wsn = ws.get(lib.paces_merge_unique_workspace_bytes(total, R.K, W))
-    check(lib.paces_merge_unique(C.byref(R), W, ptr(out), ptr(cnt), ptr(from0), ptr(cpos), wsp,
-                                 wsn, stream()), "merge_unique")
+    d = None
+    if diag is None:
+        check(lib.paces_merge_unique(C.byref(R), W, ptr(out), ptr(cnt), ptr(from0), ptr(cpos),
+                                     wsp, wsn, stream()), "merge_unique")
+    else:
+        lay, h, ddt = diag
+        d = torch.empty(max(total, 1), dtype=ddt, device=dev)
+        f64 = ddt == torch.float64
+        check(lib.paces_merge_unique_diag(C.byref(R), W, ptr(out), ptr(cnt), ptr(from0),
+                                          ptr(cpos), C.byref(lay), C.byref(h),
+                                          ptr(d) if f64 else None, None if f64 else ptr(d), wsp,
+                                          wsn, stream()), "merge_unique_diag")
     n = int(cnt.item())
+    res = [out[:n], (from0[:n] if want_from0 else None)]
     if want_cand_pos:
-        return out[:n], (from0[:n] if want_from0 else None), cpos
-    return out[:n], (from0[:n] if want_from0 else None)
+        res.append(cpos)
+    if diag is not None:
+        res.append(d[:n])
+    return tuple(res)
 
 
 def sort_rows(rows, ws, unique=True, want_perm=False):

@@ -426,13 +426,38 @@ __global__ void __launch_bounds__(kBlock) k_mu_rank(RunsArg R, const int64_t* __
 // repeat's index is the prefix - 1.
 constexpr uint32_t kMuNoCode = 0xFFFFFFFEu;  // flag 0, index 2^31 - 1: above every candidate
 
+// Optional epilogue of the emit (DG): the Holstein diagonal of every output row, written with
+// the row: eps + hw * (sum of the phonon fields, a weighted popcount) + de * field pf, in FP64
+// or rounded to FP32 -- the expression and order of paces_holstein_diag.
+struct RowDiag {
+  const u64* sum_mask;
+  const int32_t* off;
+  const int32_t* width;
+  int sum_bits, pf;
+  double eps, hw, de;
+  double* d64;
+  float* d32;
+};
+
 template <int W>
+__device__ __forceinline__ double row_diag(const Row<W>& r, const RowDiag& D) {
+  long long s = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w)
+    for (int k = 0; k < D.sum_bits; ++k)
+      s += (long long)__popcll(r.w[w] & D.sum_mask[w * D.sum_bits + k]) << k;
+  double d = D.eps + D.hw * (double)s;
+  if (D.de != 0.0) d = d + D.de * (double)get_field<W>(r, D.off[D.pf], D.width[D.pf]);
+  return d;
+}
+
+template <int W, bool DG>
 __global__ void __launch_bounds__(kBlock) k_mu_emit(RunsArg R, const uint32_t* __restrict__ code,
                                                     const int32_t* __restrict__ offs,
                                                     u64* __restrict__ out,
                                                     uint8_t* __restrict__ from0,
                                                     int32_t* __restrict__ cand_pos,
-                                                    int32_t* __restrict__ out_count) {
+                                                    int32_t* __restrict__ out_count, RowDiag D) {
   constexpr int kSteps = kScanTile / kBlock;
   constexpr int kWaves = kBlock / 64;
   __shared__ int wpre[kSteps * kWaves];
@@ -483,6 +508,13 @@ __global__ void __launch_bounds__(kBlock) k_mu_emit(RunsArg R, const uint32_t* _
     if (f[k]) {
       store_row<W>(out, pos, x[k]);
       if (from0) from0[pos] = (rr[k] == 0) ? 1 : 0;
+      if (DG) {
+        const double d = row_diag<W>(x[k], D);
+        if (D.d64)
+          D.d64[pos] = d;
+        else
+          D.d32[pos] = (float)d;  // the FP64 value rounded to nearest
+      }
     }
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *out_count = offs[gridDim.x];
@@ -516,7 +548,8 @@ size_t mu_ws_bytes(int64_t C, int K) {
 
 template <int W>
 static int run_mu(const RunsArg& R, uint64_t* out, int32_t* out_count, uint8_t* from0,
-                  int32_t* cand_pos, void* ws, size_t ws_bytes, hipStream_t st) {
+                  int32_t* cand_pos, const RowDiag& D, void* ws, size_t ws_bytes,
+                  hipStream_t st) {
   int64_t C = R.off[R.K];
   if (ws_bytes < mu_ws_bytes(C, R.K)) return PACES_E_WORKSPACE;
   if (C == 0) {
@@ -563,9 +596,14 @@ static int run_mu(const RunsArg& R, uint64_t* out, int32_t* out_count, uint8_t* 
   if (rc) return rc;
   rc = scan_tile_offsets_bit0(code, C, offs, p, scan_tile_offsets_ws_bytes(C), st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_mu_emit<W>, dim3((unsigned)nst), dim3(kBlock), 0, st, R,
-                     (const uint32_t*)code, (const int32_t*)offs, (u64*)out,
-                     from0, cand_pos, out_count);
+  if (D.d64 || D.d32)
+    hipLaunchKernelGGL((k_mu_emit<W, true>), dim3((unsigned)nst), dim3(kBlock), 0, st, R,
+                       (const uint32_t*)code, (const int32_t*)offs, (u64*)out, from0, cand_pos,
+                       out_count, D);
+  else
+    hipLaunchKernelGGL((k_mu_emit<W, false>), dim3((unsigned)nst), dim3(kBlock), 0, st, R,
+                       (const uint32_t*)code, (const int32_t*)offs, (u64*)out, from0, cand_pos,
+                       out_count, D);
   return launch_status();
 }
 
@@ -760,9 +798,9 @@ size_t paces_merge_unique_workspace_bytes(int64_t total, int32_t K, int32_t W) {
   return mu_ws_bytes(total, K);
 }
 
-int paces_merge_unique(const paces_runs_t* runs, int32_t W, uint64_t* out_rows,
-                       int32_t* out_count, uint8_t* out_from_run0, int32_t* cand_pos, void* ws,
-                       size_t ws_bytes, void* stream) {
+static int merge_unique_impl(const paces_runs_t* runs, int32_t W, uint64_t* out_rows,
+                             int32_t* out_count, uint8_t* out_from_run0, int32_t* cand_pos,
+                             const RowDiag& D, void* ws, size_t ws_bytes, void* stream) {
   if (!runs || runs->K < 1 || runs->K > PACES_MAX_RUNS || !out_count) return PACES_E_INVALID;
   RunsArg R;
   R.K = runs->K;
@@ -782,8 +820,39 @@ int paces_merge_unique(const paces_runs_t* runs, int32_t W, uint64_t* out_rows,
   if (R.off[R.K] >= 2147483647LL) return PACES_E_INVALID;
   if (R.off[R.K] > 0 && !out_rows) return PACES_E_INVALID;
   if (W < 1 || W > PACES_MAX_ROW_WORDS) return PACES_E_UNSUPPORTED;
-  PACES_DISPATCH_W(W, run_mu, R, out_rows, out_count, out_from_run0, cand_pos, ws, ws_bytes,
+  PACES_DISPATCH_W(W, run_mu, R, out_rows, out_count, out_from_run0, cand_pos, D, ws, ws_bytes,
                    (hipStream_t)stream);
+}
+
+int paces_merge_unique(const paces_runs_t* runs, int32_t W, uint64_t* out_rows,
+                       int32_t* out_count, uint8_t* out_from_run0, int32_t* cand_pos, void* ws,
+                       size_t ws_bytes, void* stream) {
+  RowDiag D = {};
+  return merge_unique_impl(runs, W, out_rows, out_count, out_from_run0, cand_pos, D, ws, ws_bytes,
+                           stream);
+}
+
+int paces_merge_unique_diag(const paces_runs_t* runs, int32_t W, uint64_t* out_rows,
+                            int32_t* out_count, uint8_t* out_from_run0, int32_t* cand_pos,
+                            const paces_layout_t* lay, const paces_holstein_t* h, double* diag,
+                            float* diag32, void* ws, size_t ws_bytes, void* stream) {
+  if (!lay || !h || lay->W != W || (!diag) == (!diag32)) return PACES_E_INVALID;
+  if (!lay->d_sum_mask || lay->sum_bits < 1 || lay->sum_bits > 32) return PACES_E_INVALID;
+  if (h->delta_eps != 0.0 && (!lay->d_off || !lay->d_width || lay->nfields < 2))
+    return PACES_E_INVALID;
+  RowDiag D;
+  D.sum_mask = (const u64*)lay->d_sum_mask;
+  D.off = lay->d_off;
+  D.width = lay->d_width;
+  D.sum_bits = lay->sum_bits;
+  D.pf = h->batched ? 1 : 0;
+  D.eps = h->eps_sys;
+  D.hw = h->hbar_omega;
+  D.de = h->delta_eps;
+  D.d64 = diag;
+  D.d32 = diag32;
+  return merge_unique_impl(runs, W, out_rows, out_count, out_from_run0, cand_pos, D, ws, ws_bytes,
+                           stream);
 }
 
 size_t paces_reassign_workspace_bytes(int64_t R) { return lb_ws_bytes(R); }

@@ -249,8 +249,9 @@ class Hamiltonian(HamiltonianFramework):
             imgs, offs, src, nup = self._images(basis, want_src=True)
             runs = [(basis, 0, basis.shape[0])]
             runs += [(imgs, offs[k], offs[k + 1] - offs[k]) for k in range(len(offs) - 1)]
-            new_inds, in_basis, cand_pos = engine.merge_unique(
-                runs, self.totwordwidth, self._ws, want_from0=True, want_cand_pos=True)
+            new_inds, in_basis, cand_pos, diag = engine.merge_unique(
+                runs, self.totwordwidth, self._ws, want_from0=True, want_cand_pos=True,
+                diag=(self._layout, self._hs, diag_dtype))
         else:
             # The expanded basis already holds the images of the rows it was expanded from, so
             # the new rows are its union with the images of the rows the expansion added only;
@@ -261,10 +262,10 @@ class Hamiltonian(HamiltonianFramework):
                                                 slot="expand2")
             runs = [(basis, 0, basis.shape[0])]
             runs += [(imgs, offs[k], offs[k + 1] - offs[k]) for k in range(len(offs) - 1)]
-            new_inds, in_basis, cand_pos = engine.merge_unique(
-                runs, self.totwordwidth, self._ws, want_from0=True, want_cand_pos=True)
+            new_inds, in_basis, cand_pos, diag = engine.merge_unique(
+                runs, self.totwordwidth, self._ws, want_from0=True, want_cand_pos=True,
+                diag=(self._layout, self._hs, diag_dtype))
         self._check_basis_size(new_inds.shape[0])
-        diag = self.generate_mel_diag(new_inds, diag_dtype)
         mats = self._pairs_csr(new_inds.shape[0], basis.shape[0], offs, cand_pos, in_basis, src,
                                nup, basis, value_dtype, rows=new_inds, prior=prior)
         dbg = {}

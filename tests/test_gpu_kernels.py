@@ -327,8 +327,9 @@ def test_expand_csr_diag_taylor_observables(env, case):
 def test_generate_mel_two_expansions_match_oracle(env, case):
     """Two expansions before the pairs: the second union takes the images of the rows the last
     expansion added, and the pairs take the earlier images as a separate part
-    (paces_holstein_pairs_*_split). Rows, expanded basis, diagonal, CSR entries and values (FP64
-    and FP32) and the vib ceiling count must equal the oracle's full-image-list result."""
+    (paces_holstein_pairs_*_split). Rows, expanded basis, diagonal (FP64 and FP32, written by
+    the union's pass), CSR entries and values (FP64 and FP32) and the vib ceiling count must
+    equal the oracle's full-image-list result."""
     import sys
     import os
     sys.path.insert(0, os.path.dirname(__file__))
@@ -347,7 +348,10 @@ def test_generate_mel_two_expansions_match_oracle(env, case):
     np.testing.assert_array_equal(_np(env, ham.last_basis_e), basis_e)
     np.testing.assert_array_equal(_np(env, gnew), new_inds)
     np.testing.assert_array_equal(gdiag.cpu().numpy(), diag)
-    _, mats32, _ = ham.generate_mel(_rows(env, sel), enlarge_steps=2, value_dtype=torch.float32)
+    (gd32, _), mats32, _ = ham.generate_mel(_rows(env, sel), enlarge_steps=2,
+                                            value_dtype=torch.float32, diag_dtype=torch.float32)
+    assert gd32.dtype == torch.float32
+    np.testing.assert_array_equal(gd32.cpu().numpy(), diag.astype(np.float32))
     for name in oham.offdiag_terms:
         ip, ix, dv = po.coo_to_csr(*coo[name], R)
         for m, vals in ((mats[name], dv), (mats32[name], dv.astype(np.float32))):
