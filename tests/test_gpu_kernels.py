@@ -360,6 +360,19 @@ def test_generate_mel_two_expansions_match_oracle(env, case):
             np.testing.assert_array_equal(m.data.cpu().numpy(), vals)
     if "vib_coupling" in oham.offdiag_terms:
         assert gdbg["vib_coupling"] == dbg["vib_coupling"]
+    # with no room for bounded outputs the host waits for the exact sizes: the same result
+    holstein._BOUND_BYTES, keep = 0, holstein._BOUND_BYTES
+    try:
+        (gd2, gn2), mats2, _ = ham.generate_mel(_rows(env, sel), enlarge_steps=2)
+    finally:
+        holstein._BOUND_BYTES = keep
+    np.testing.assert_array_equal(_np(env, gn2), new_inds)
+    np.testing.assert_array_equal(gd2.cpu().numpy(), diag)
+    for name in oham.offdiag_terms:
+        ip, ix, dv = po.coo_to_csr(*coo[name], R)
+        np.testing.assert_array_equal(mats2[name].indptr.cpu().numpy(), ip)
+        np.testing.assert_array_equal(mats2[name].indices.cpu().numpy(), ix)
+        np.testing.assert_array_equal(mats2[name].data.cpu().numpy(), dv)
 
 
 @pytest.mark.parametrize("layout_case", ["ring64_cap15", "torus16_cap7"])
