@@ -718,8 +718,8 @@ class TimeEvolutionOracle:
         if self.renormalize:
             res = res / np.linalg.norm(res)
         elif np.linalg.norm(res) > self.explosion_cutoff:
-            raise RuntimeError("Norm has exceeded preset explosion cutoff value"
-                               f" ({self.explosion_cutoff})!")
+            raise RuntimeError(f"norm {np.linalg.norm(res)} above explosion_cutoff "
+                               f"{self.explosion_cutoff}")
         self.vector = res
         return dbg
 

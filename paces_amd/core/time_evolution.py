@@ -39,13 +39,17 @@ class CoeffSaveParams:
     max_i: int = 0
 
     def save_necessary(self, i: int):
-        if self.max_i == 0:
-            raise ValueError("max_i should be set before calling this method.")
-        necessary = (self.save_first and i == 0)
-        necessary |= (self.save_last and i == self.max_i)
-        if self.save_every != 0:
-            necessary |= (self.save_every is not None and i % self.save_every == 0)
-        return necessary
+        """True when step i's coefficients are due: step 0 with save_first, step max_i with
+        save_last, or any multiple of a nonzero save_every. max_i must have been set (the
+        timeline sets it to the last step index)."""
+        if not self.max_i:
+            raise ValueError("CoeffSaveParams.max_i is 0: set it (generate_timeline does) "
+                             "before asking which steps to save.")
+        if i == 0 and self.save_first:
+            return True
+        if i == self.max_i and self.save_last:
+            return True
+        return bool(self.save_every) and i % self.save_every == 0
 
 
 @dataclass
@@ -95,25 +99,28 @@ class TimeEvolutionFramework:
         self.te_params = te_params
         self.expm_params = expm_params
         if expm_params.lanczos_order and expm_params.use_scaling:
-            raise ValueError("Scaling and squaring cannot be used in conjunction with Lanczos.")
+            raise ValueError("ExpmParams: lanczos_order > 0 and use_scaling are exclusive "
+                             "propagators; choose one.")
         if expm_params.use_scaling:
             raise NotImplementedError("The scaling-and-squaring expm_multiply branch is not part "
                                       "of this engine; use the Taylor series or Lanczos.")
         if te_params.use_two_streams:
-            raise NotImplementedError("Using two streams does not appear to confer a benefit.")
+            raise NotImplementedError("use_two_streams (the reference's concurrent basis on a "
+                                      "second device) is not part of this one-GPU engine.")
         if te_params.tau != 0:
             raise NotImplementedError("Weighting functions (tau != 0) are not supported.")
         if te_params.garbage_tol >= 0:
             raise NotImplementedError("garbage_tol is not supported.")
         self.weighting_func = None
         if dirname is not None and not os.path.exists(dirname):
-            raise FileNotFoundError(f"The results directory {dirname} does not exist.")
+            raise FileNotFoundError(f"results directory {dirname!r} is missing; create it "
+                                    "before building the TimeEvolution.")
 
         real_valued = all(all(numpy.imag(v) == 0 for v in pd.values())
                           for pd in hamobj.use_terms.values())
         if not real_valued:
-            raise NotImplementedError("Not all functions have been adapted to complex-valued"
-                                      " off-diagonal Hamiltonian matrix elements.")
+            raise NotImplementedError("complex Hamiltonian parameters are not supported: "
+                                      "the engine stores H as real CSR values.")
 
         self.obsobj = ObsObj(self, obs_list)
         self.ham_mat = None
@@ -141,44 +148,52 @@ class TimeEvolutionFramework:
         self.initialize_params_file()
 
     # ---- params file (217-264) ----------------------------------------------------------------------
+    def _params_sections(self):
+        """(heading, text) blocks of the params log after its version banner and the device
+        description; the headings and the pprint layouts are the log's format."""
+        def origin(obj):
+            return f"{obj.__class__.__module__}.{obj.__class__.__name__}"
+        return [
+            (f"### Hilbert space constructed from {origin(self.hamobj)} with the following "
+             "parameters:", pprint.pformat(self.hamobj.input_args, compact=True)),
+            ("\n### Hamiltonian terms and their parameters:",
+             pprint.pformat(self.hamobj.use_terms, width=1)),
+            (f"\n### Observables constructed from {origin(self.obsobj)} with the following "
+             "observables:", pprint.pformat(list(self.obsobj.obs_dict))),
+            (f"\n### TimeEvolution constructed from {origin(self)} with the following "
+             "parameters:", pprint.pformat(self.te_params) + "\n"
+             + pprint.pformat(self.expm_params)),
+        ]
+
     def initialize_params_file(self):
+        """Start the params log: a banner naming the version, framed by lines of '#' as long as
+        the banner, a blank line, the device description, then one block per section and the
+        closing marker."""
         if self.params_file is None:
             return
-        ver = f"### paces_amd version {__version__} ###"
-        hashes = "#" * len(ver) + "\n"
+        banner = f"### paces_amd version {__version__} ###"
+        frame = "#" * len(banner)
+        body = []
+        for heading, text in self._params_sections():
+            body += [heading, text]
+        body += ["\n### End of initialization parameter list", "#" * 65]
         with open(self.params_file, "w", encoding="utf-8") as pf:
-            pf.write(hashes + ver + "\n" + hashes + "\n")
-            pf.write(devices.desc_str)
-            pf.write("### Hilbert space constructed from"
-                     f" {self.hamobj.__class__.__module__}.{self.hamobj.__class__.__name__}"
-                     " with the following parameters:\n")
-            pf.write(pprint.pformat(self.hamobj.input_args, compact=True) + "\n")
-            pf.write("\n### Hamiltonian terms and their parameters:\n")
-            pf.write(pprint.pformat(self.hamobj.use_terms, width=1) + "\n")
-            pf.write("\n### Observables constructed from"
-                     f" {self.obsobj.__class__.__module__}.{self.obsobj.__class__.__name__}"
-                     " with the following observables:\n")
-            pf.write(pprint.pformat([*self.obsobj.obs_dict]) + "\n")
-            pf.write("\n### TimeEvolution constructed from"
-                     f" {self.__class__.__module__}.{self.__class__.__name__}"
-                     " with the following parameters:\n")
-            pf.write(pprint.pformat(self.te_params) + "\n")
-            pf.write(pprint.pformat(self.expm_params) + "\n")
-            pf.write("\n### End of initialization parameter list\n"
-                     "#################################################################\n")
+            pf.write("\n".join([frame, banner, frame, "", ""]) + devices.desc_str
+                     + "\n".join(body) + "\n")
 
     def write_current_params_to_file(self, frame, start_time=None):
+        """Append a record of a set-up call to the params log: when it was made (epoch seconds
+        and local time) and every argument except self as name=value, one per line."""
         if self.params_file is None:
             return
-        start_time = time.time() if start_time is None else start_time
-        localtime = time.asctime(time.localtime(start_time))
-        args, _, _, values = inspect.getargvalues(frame)
-        arg_list = [f"{a}={values[a]}" for a in args if a != "self"]
-        mod_obj = f"{self.__class__.__module__}.{self.__class__.__name__}"
-        header = f"\nFunction call in {mod_obj} at {start_time} ({localtime} local):\n"
-        header += f"  {frame.f_code.co_name}(\n    " + ",\n    ".join(arg_list) + ")\n"
-        with open(self.params_file, "a", encoding="utf-8") as f:
-            f.write(header)
+        when = time.time() if start_time is None else start_time
+        info = inspect.getargvalues(frame)
+        given = [f"{name}={info.locals[name]}" for name in info.args if name != "self"]
+        owner = f"{self.__class__.__module__}.{self.__class__.__name__}"
+        record = [f"\nFunction call in {owner} at {when} ({time.asctime(time.localtime(when))}"
+                  " local):", f"  {frame.f_code.co_name}(", "    " + ",\n    ".join(given) + ")"]
+        with open(self.params_file, "a", encoding="utf-8") as pf:
+            pf.write("\n".join(record) + "\n")
 
     # ---- initial basis (271-359) ----------------------------------------------------------------------
     def load_basis_from_file(self, loadfile):
@@ -186,47 +201,56 @@ class TimeEvolutionFramework:
         self.write_current_params_to_file(inspect.currentframe())
         arr = numpy.load(loadfile, allow_pickle=False)
         if arr.ndim != 2 or arr.shape[1] != self.hamobj.totwordwidth:
-            raise ValueError("Loaded basis set doesn't match total number of bits given by hamobj.")
+            raise ValueError(f"{loadfile}: rows of {arr.shape[-1]} words, the layout needs "
+                             f"{self.hamobj.totwordwidth}.")
         if arr.dtype not in (numpy.uint64, numpy.int64):
-            raise ValueError("Loaded basis set does not match the uint64 word format.")
+            raise ValueError(f"{loadfile}: {arr.dtype} words, expected uint64.")
         self.whoami = torch.from_numpy(arr.view(numpy.int64).copy()).to(self.device)
         self.numstates = self.whoami.shape[0]
 
     def grow_optimal_basis(self, n_max=numpy.inf, fillfac=1.0):
-        """Enlarge the initial basis while it holds at most maxstates*fillfac states; keep the
-        last one that did (time_evolution.py:290-319, including its n_max behaviour)."""
+        """Grow the initial basis by whole expansions through H (time_evolution.py:290-319).
+
+        Expansions continue while the basis holds at most maxstates * fillfac rows and fewer
+        than n_max have been made; the basis before the last expansion is kept, so the result is
+        the largest grown basis within the limit, and with n_max binding it holds n_max - 1
+        expansions. An initial basis already above the limit raises UnboundLocalError, as the
+        reference's loop does. Returns the number of expansions made."""
         self.write_current_params_to_file(inspect.currentframe())
-        initsize = len(self.whoami)
-        whoami = self.whoami
-        n = 0
-        while len(whoami) <= self.te_params.maxstates * fillfac:
-            if n >= n_max:
-                break
-            n += 1
-            previous_whoami = whoami
-            whoami = self.hamobj.enlarge_basis_set(previous_whoami)
-        self.whoami = previous_whoami  # noqa: F821 -- UnboundLocalError as in the reference
-        self.numstates = self.whoami.shape[0]
+        limit = self.te_params.maxstates * fillfac
+        start = int(self.whoami.shape[0])
+        current, kept, made = self.whoami, None, 0
+        while current.shape[0] <= limit and made < n_max:
+            kept = current
+            current = self.hamobj.enlarge_basis_set(kept)
+            made += 1
+        if kept is None:
+            raise UnboundLocalError(f"the initial basis ({start} rows) already exceeds "
+                                    f"maxstates * fillfac = {limit}")
+        self.whoami = kept
+        self.numstates = int(kept.shape[0])
         if self.debug_verb > INIT_VERBOSITY_LEVEL:
-            print(f"Done! Performed {n} enlargements (grew from {initsize} to"
-                  f" {len(self.whoami)} states).")
-        return n
+            print(f"grow_optimal_basis: {made} expansions made, basis of {start} rows grown to "
+                  f"{self.numstates}.")
+        return made
 
     def create_initial_vector(self, vector_coeffs, vector_coo, auto_normalize=True):
-        """Place coefficients on the listed basis states (time_evolution.py:325-359)."""
+        """The state with the given coefficients on the listed basis states, zero elsewhere
+        (time_evolution.py:325-359). vector_coo holds one field tuple per row; every tuple must
+        be a row of the basis."""
         coo = numpy.asarray(vector_coo.cpu() if torch.is_tensor(vector_coo) else vector_coo)
         if coo.ndim != 2:
-            raise ValueError("The coo list must be 2D"
-                             " (even if it contains only the entry for one basis state).")
-        if numpy.unique(coo, axis=0).shape != coo.shape:
-            raise ValueError("There are duplicates in the specified vector coordinates.")
+            raise ValueError(f"vector_coo must be two-dimensional, one field tuple per row "
+                             f"(got {coo.ndim} dimensions)")
+        if len(numpy.unique(coo, axis=0)) != len(coo):
+            raise ValueError("vector_coo lists a basis state more than once")
         coeffs = numpy.asarray(vector_coeffs, dtype=numpy.complex128)
-        if coo.shape[0] != len(coeffs):
-            raise ValueError("The number of coefficients does not match the number of coordinates.")
+        if len(coeffs) != len(coo):
+            raise ValueError(f"{len(coeffs)} coefficients for {len(coo)} basis states")
         if auto_normalize:
             coeffs = coeffs / numpy.linalg.norm(coeffs)
-        rows = self.hamobj.compress_ind_arr(coo)
-        idx = self.hamobj.searchsorted(self.whoami, rows, allow_escapes=False)
+        idx = self.hamobj.searchsorted(self.whoami, self.hamobj.compress_ind_arr(coo),
+                                       allow_escapes=False)
         self.vector = torch.zeros(len(self.whoami), dtype=torch.complex128, device=self.device)
         self.vector[idx.long()] = torch.from_numpy(coeffs).to(self.device)
 
@@ -327,18 +351,25 @@ class TimeEvolutionFramework:
             self.vector, expm_dbg = self._lanczos(delta_t)
         else:
             self.vector, expm_dbg = self._simple_taylor(delta_t)
-        norm = self._reducer.norm(self.vector)
-        if self.expm_params.renormalize:
-            engine.axpby(1.0 / norm, self.vector, 0.0, self.vector)
-            norm = self._reducer.norm(self.vector)
-        elif norm > self.expm_params.explosion_cutoff:
-            raise RuntimeError("Norm has exceeded preset explosion cutoff value"
-                               f" ({self.expm_params.explosion_cutoff})!")
+        norm = self._norm_after_propagation()
         self.last_expm_dbg = expm_dbg
         self._postprocess(t_curr, norm, i)
         self._save_diagnostic_data(t_curr, norm, post, expm_dbg, dbg_dict)
         self._step_index = i
         return None
+
+    def _norm_after_propagation(self):
+        """The state's norm after a propagation: rescaled to 1 with renormalize, else checked
+        against explosion_cutoff (time_evolution.py:585-589)."""
+        norm = self._reducer.norm(self.vector)
+        cutoff = self.expm_params.explosion_cutoff
+        if self.expm_params.renormalize:
+            engine.axpby(1.0 / norm, self.vector, 0.0, self.vector)
+            return self._reducer.norm(self.vector)
+        if norm > cutoff:
+            raise RuntimeError(f"the propagated state's norm {norm} is above explosion_cutoff "
+                               f"{cutoff}: the series has diverged")
+        return norm
 
     def _perform_first_step(self, t_curr):
         dbg_dict, post = self._generate_new_hilbert_space(self.te_params.enlarge_steps)
@@ -356,24 +387,32 @@ class TimeEvolutionFramework:
             numpy.save(self.wf_name_list[i].replace("wf_file_", "whoami_"),
                        self.whoami.cpu().numpy().view(numpy.uint64))
 
+    @staticmethod
+    def wf_time_format(t_array):
+        """The fixed-point format of the times in wavefunction file names: enough decimals to
+        tell the smallest nonzero |t| and the smallest spacing of |t| apart (one more than the
+        order of magnitude of the smaller of the two), and enough width for the largest |t|,
+        zero-padded. A single time, or times that are all zero, use one integer digit and the
+        decimals of the time itself (the reference cannot name these)."""
+        mags = numpy.abs(numpy.asarray(t_array, dtype=numpy.float64))
+        small = [float(mags.min())] if mags.min() > 0 else []
+        if mags.size > 1:
+            small.append(float(numpy.diff(mags).min()))
+        decimals = int(abs(math.floor(math.log10(min(small))))) + 1 if small else 1
+        top = float(mags.max())
+        int_digits = int(abs(math.ceil(math.log10(top)))) if top > 0 else 1
+        return "{0:0%d.%df}" % (int_digits + decimals + 1, decimals)
+
     def _compute_wf_name_list(self, t_array):
+        """Names of the wavefunction files of each step, dirname/wf_coeffs/wf_file_<t> with t in
+        wf_time_format; the folder is created when missing."""
         if self.dirname is None:
             self.wf_name_list = None
             return
-        t_array = numpy.asarray(t_array)
-        smallest_val = numpy.min(numpy.abs(t_array))
-        smallest_diff = numpy.diff(numpy.abs(t_array)).min() if len(t_array) > 1 else 1.0
-        if smallest_val == 0:
-            decnum = numpy.abs(numpy.floor(numpy.log10(smallest_diff))) + 1
-        else:
-            decnum = numpy.abs(numpy.floor(numpy.log10(min(smallest_val, smallest_diff)))) + 1
-        largest_val = numpy.max(numpy.abs(t_array))
-        intnum = numpy.abs(numpy.ceil(numpy.log10(largest_val))) if largest_val > 0 else 1
-        f_str = "wf_file_{0:0%i.%if}" % (intnum + decnum + 1, decnum)
-        wf_dir = os.path.join(self.dirname, "wf_coeffs")
-        if not os.path.exists(wf_dir):
-            os.mkdir(wf_dir)
-        self.wf_name_list = [os.path.join(wf_dir, f_str.format(t)) for t in t_array]
+        folder = os.path.join(self.dirname, "wf_coeffs")
+        os.makedirs(folder, exist_ok=True)
+        fmt = self.wf_time_format(t_array)
+        self.wf_name_list = [os.path.join(folder, "wf_file_" + fmt.format(t)) for t in t_array]
 
     def _save_diagnostic_data(self, t_curr, norm, norm_energy, expm_dbg, dbg_dict):
         self.last_dbg_dict = dbg_dict

@@ -53,6 +53,12 @@ static int tbits(int ntraj) {
   return b;
 }
 
+// trajectory id of a batched row from its first word: the top trajbits bits (a one-trajectory
+// batch has a zero-width field, where a shift by 64 would be undefined)
+__device__ __forceinline__ int traj_of(u64 w0, int trajbits) {
+  return trajbits ? (int)(w0 >> (64 - trajbits)) : 0;
+}
+
 static ModelArg marg(const paces_holstein_t* h) {
   ModelArg M;
   M.batched = h->batched ? 1 : 0;
@@ -604,7 +610,7 @@ __global__ void __launch_bounds__(kBlock) k_csr_fill(int64_t R, OpTable T, Model
                                                      const int32_t* __restrict__ G, CsrOut out) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= R) return;
-  const int traj = M.batched ? (int)(rows[i * W] >> (64 - trajbits)) : 0;
+  const int traj = M.batched ? traj_of(rows[i * W], trajbits) : 0;
   for (int t = 0; t < nterms; ++t) {
     const int32_t base0 = G[(int64_t)t * R];
     const int32_t base = G[(int64_t)t * R + i] - base0;
@@ -803,7 +809,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs_fill(int64_t R, int64_t nbasis
   const int t = T.term[k];
   double v = 0.0;
   if (!out.coded) {
-    const int traj = M.batched ? (int)(basis[(int64_t)s * W] >> (64 - trajbits)) : 0;
+    const int traj = M.batched ? traj_of(basis[(int64_t)s * W], trajbits) : 0;
     v = op_value(T, k, M, traj, sqrt_tab, nu);
   }
   const int32_t a = atomicAdd(&cursor[(int64_t)t * R + q], 1);
@@ -1061,7 +1067,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs_place(int64_t R, int64_t nbasi
   const int32_t sp = out.indptr[t][pc], sq = out.indptr[t][qc];
   if (p < 0 || q < 0) return;
   if (T.dir[k] < 0 && pin) return;
-  const int traj = M.batched ? (int)(w0 >> (64 - trajbits)) : 0;
+  const int traj = M.batched ? traj_of(w0, trajbits) : 0;
   const double v = out.coded ? 0.0 : op_value(T, k, M, traj, sqrt_tab, nu);
   int32_t a = sq, b = sp;
   const int rk = SL.rank[k], rinv = SL.rank[k ^ 1];

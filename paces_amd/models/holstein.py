@@ -35,7 +35,7 @@ _BOUND_BYTES = 8 << 30
 def _to_host_async(t):
     """A pinned host copy of a small device tensor, queued on the current stream without a
     sync: (host tensor, event); read the host tensor after event.synchronize()."""
-    host = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    host = torch.empty(t.shape, dtype=t.dtype, device="cpu", pin_memory=True)
     host.copy_(t, non_blocking=True)
     ev = torch.cuda.Event()
     ev.record(torch.cuda.current_stream())
@@ -781,13 +781,7 @@ class LearnedTimeEvolution(TimeEvolution):
         self._full = None
         self._sample = None
         self.vector = vec
-        norm = self._reducer.norm(self.vector)
-        if self.expm_params.renormalize:
-            engine.axpby(1.0 / norm, self.vector, 0.0, self.vector)
-            norm = self._reducer.norm(self.vector)
-        elif norm > self.expm_params.explosion_cutoff:
-            raise RuntimeError("Norm has exceeded preset explosion cutoff value"
-                               f" ({self.expm_params.explosion_cutoff})!")
+        norm = self._norm_after_propagation()
         self.last_expm_dbg = expm_dbg
         self._postprocess(t_curr, norm, i)
         self._save_diagnostic_data(t_curr, norm, post, expm_dbg, dbg_dict)
@@ -987,7 +981,7 @@ class BatchedTimeEvolution:
         """The segment offsets, with a pinned host copy made without a stream sync (the next
         select sizes its output from it: sum over trajectories of min(maxstates, rows))."""
         self.seg_off = seg_off
-        host = torch.empty(seg_off.shape, dtype=seg_off.dtype, pin_memory=True)
+        host = torch.empty(seg_off.shape, dtype=seg_off.dtype, device="cpu", pin_memory=True)
         host.copy_(seg_off, non_blocking=True)
         ev = torch.cuda.Event()
         ev.record(torch.cuda.current_stream())
@@ -1145,7 +1139,7 @@ class BatchedTimeEvolution:
         self.vector = res
         # the per-trajectory series states go to pinned host memory without a stream sync; they
         # are parsed when first read (last_expm, or the next step's graph size)
-        host = torch.empty(ts.shape, dtype=ts.dtype, pin_memory=True)
+        host = torch.empty(ts.shape, dtype=ts.dtype, device="cpu", pin_memory=True)
         host.copy_(ts, non_blocking=True)
         ev = torch.cuda.Event()
         ev.record(torch.cuda.current_stream())
