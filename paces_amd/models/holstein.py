@@ -856,19 +856,31 @@ class BatchedHamiltonian(Hamiltonian):
               "batch_segments")
         return seg
 
-    def site_observables_batched(self, rows, vector, seg_off):
-        """(n_exc [ntraj, N], n_pho [ntraj, N], sum|psi|^2 [ntraj]) per trajectory."""
+    def site_observables_batched_async(self, rows, vector, seg_off, out=None):
+        """Launches the per-trajectory observables into out (device float64, ntraj * (2N + 1):
+        per trajectory n_exc[N], n_pho[N], sum|psi|^2) and returns out; no host sync."""
         R = rows.shape[0]
         N = self.nchain
-        out = torch.empty(self.ntraj * (2 * N + 1), dtype=torch.float64, device=self.device)
+        if out is None:
+            out = torch.empty(self.ntraj * (2 * N + 1), dtype=torch.float64, device=self.device)
         wsp, wsn = self._ws.get(
             _lib.paces_holstein_observables_batched_workspace_bytes(R, self.ntraj, N))
         check(_lib.paces_holstein_observables_batched(
             ptr(rows) if R else None, ptr(torch.view_as_real(vector)) if R else None, R,
             ptr(seg_off), C.byref(self._layout), C.byref(self._hs), ptr(out), wsp, wsn,
             stream()), "holstein_observables_batched")
-        h = out.cpu().numpy().reshape(self.ntraj, 2 * N + 1)
+        return out
+
+    def split_observables(self, flat):
+        """(n_exc [ntraj, N], n_pho [ntraj, N], sum|psi|^2 [ntraj]) from the flat host array."""
+        N = self.nchain
+        h = numpy.asarray(flat).reshape(self.ntraj, 2 * N + 1)
         return h[:, :N], h[:, N:2 * N], h[:, 2 * N]
+
+    def site_observables_batched(self, rows, vector, seg_off):
+        """(n_exc [ntraj, N], n_pho [ntraj, N], sum|psi|^2 [ntraj]) per trajectory."""
+        out = self.site_observables_batched_async(rows, vector, seg_off)
+        return self.split_observables(out.cpu().numpy())
 
 
 class BatchedTimeEvolution:
@@ -881,8 +893,20 @@ class BatchedTimeEvolution:
 
     def __init__(self, hamobj: BatchedHamiltonian, maxstates, enlarge_steps=1,
                  precision="mixed", m_star=100, taylor_tol=None, autotune=False, graph=False,
-                 launch_shape=None, csr_values="coded"):
+                 launch_shape=None, csr_values="coded", explosion_cutoff=2.0,
+                 on_observables=None):
         self.hamobj = hamobj
+        # every step ends with the per-trajectory norm and site observables (as the reference's
+        # step does, time_evolution.py:585-605); they travel to pinned host memory without a
+        # stream sync and are read once the next step's adaptation has waited on the device
+        # (or by observables()): a norm above explosion_cutoff then raises, and
+        # on_observables(i, n_exc, n_pho, norm2) receives each step's values
+        self.explosion_cutoff = float(explosion_cutoff)
+        self.on_observables = on_observables
+        self._obs_dev = None
+        self._obs_host = None
+        self._obs_pending = None
+        self._obs_last = None
         self.device = hamobj.device
         ms = numpy.asarray(maxstates, dtype=numpy.int64)
         if ms.shape != (hamobj.ntraj,) or numpy.any(ms < 1):
@@ -974,6 +998,7 @@ class BatchedTimeEvolution:
         self.whoami = new_inds
         self.diag_vals = diag
         self.sparse_mats_dict = mats
+        self._obs_last = None
         self._set_segments(self.hamobj.segments(new_inds))
         return dbg
 
@@ -1137,6 +1162,7 @@ class BatchedTimeEvolution:
                                              st), "batch_taylor_finish")
         del keep
         self.vector = res
+        self._obs_last = None
         # the per-trajectory series states go to pinned host memory without a stream sync; they
         # are parsed when first read (last_expm, or the next step's graph size)
         host = torch.empty(ts.shape, dtype=ts.dtype, device="cpu", pin_memory=True)
@@ -1193,11 +1219,50 @@ class BatchedTimeEvolution:
         return autotune.launch_cfg(tuner.pick(key, autotune.SHAPES, run))
 
     def step(self, i, delta_t):
-        """Step 0 adapts only; later steps adapt then propagate (time_evolution.py:426-458)."""
+        """Step 0 adapts only; later steps adapt then propagate (time_evolution.py:426-458).
+        Every step ends with each trajectory's norm, n_exc and n_pho (time_evolution.py:585-605;
+        holstein.py:311-322)."""
         dbg = self.adapt()
+        self._resolve_observables()  # the last step's: the adaptation has waited on the device
         if i > 0:
             self.propagate(delta_t)
+        self._launch_observables(i)
         return dbg
 
+    def _launch_observables(self, i):
+        h = self.hamobj
+        n = h.ntraj * (2 * h.nchain + 1)
+        if self._obs_dev is None or self._obs_dev.numel() != n:
+            self._obs_dev = torch.empty(n, dtype=torch.float64, device=self.device)
+            self._obs_host = torch.empty(n, dtype=torch.float64, device="cpu", pin_memory=True)
+        h.site_observables_batched_async(self.whoami, self.vector, self.seg_off, self._obs_dev)
+        self._obs_host.copy_(self._obs_dev, non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream())
+        self._obs_pending = (i, ev)
+
+    def _resolve_observables(self):
+        """Reads the pending step's observables: checks the norms against explosion_cutoff and
+        hands the values to on_observables."""
+        if self._obs_pending is None:
+            return
+        i, ev = self._obs_pending
+        self._obs_pending = None
+        ev.synchronize()
+        n_exc, n_pho, nrm2 = self.hamobj.split_observables(self._obs_host.numpy().copy())
+        self._obs_last = (n_exc, n_pho, nrm2)
+        bad = numpy.nonzero(numpy.sqrt(nrm2) > self.explosion_cutoff)[0]
+        if bad.size:
+            raise RuntimeError(f"step {i}: the norm of trajectories {bad.tolist()} is above "
+                               f"explosion_cutoff {self.explosion_cutoff}: the series diverged")
+        if self.on_observables is not None:
+            self.on_observables(i, n_exc, n_pho, nrm2)
+
     def observables(self):
+        """(n_exc [ntraj, N], n_pho [ntraj, N], norm^2 [ntraj]) of the current state."""
+        if self._obs_pending is not None:
+            self._resolve_observables()
+            return self._obs_last
+        if self._obs_last is not None:
+            return self._obs_last
         return self.hamobj.site_observables_batched(self.whoami, self.vector, self.seg_off)

@@ -57,9 +57,10 @@ def test_batched_trajectories_match_per_trajectory_oracle(precision):
                                         precision=precision)
     bte.create_single_exciton_states([d[0] // 2 for d in draws])
     worst = 0.0
+    t_arr = dt * np.arange(steps + 1)
     for i in range(steps + 1):
         print(f"[batched {precision}] step {i}", flush=True)
-        bte.step(i, dt)
+        bte.step(i, t_arr[i] - t_arr[i - 1] if i else 0.0)
         n_exc, n_pho, nrm2 = bte.observables()
         raw = bham.decompress_ind_arr(bte.whoami).cpu().numpy()
         vec = bte.vector.cpu().numpy()
@@ -73,7 +74,8 @@ def test_batched_trajectories_match_per_trajectory_oracle(precision):
                 mine = raw[sel][:, 1:2 + N]
                 np.testing.assert_array_equal(mine, oham.layout.unpack(rec.whoami))
                 assert np.all(raw[sel][:, 2 + N:] == 0)
-                assert np.max(np.abs(vec[sel] - rec.vec), initial=0) <= 1e-10, (i, t)
+                # the same per-step dt (t[i] - t[i-1]) as the oracle: bit-identical amplitudes
+                assert np.array_equal(vec[sel], rec.vec), (i, t)
                 assert err <= 1e-10, (i, t, err)
                 assert np.max(np.abs(n_pho[t, :N] - rec.obs["n_pho"])) <= 1e-10
                 assert abs(np.sqrt(nrm2[t]) - rec.obs["norm"]) <= 1e-10
@@ -103,9 +105,10 @@ def test_batched_periodic_rings_match_per_trajectory_oracle():
     bte = holstein.BatchedTimeEvolution(bham, [d[3] for d in draws], enlarge_steps=1,
                                         precision="fp64")
     bte.create_single_exciton_states([d[0] // 2 for d in draws])
+    t_arr = dt * np.arange(steps + 1)
     for i in range(steps + 1):
         print(f"[batched periodic] step {i}", flush=True)
-        bte.step(i, dt)
+        bte.step(i, t_arr[i] - t_arr[i - 1] if i else 0.0)
         n_exc, n_pho, nrm2 = bte.observables()
         raw = bham.decompress_ind_arr(bte.whoami).cpu().numpy()
         vec = bte.vector.cpu().numpy()
@@ -189,3 +192,35 @@ def test_coded_values_equal_explicit_values(precision, graph):
         assert (mb.code is not None) == (name == "vib_coupling")
         assert mb.data.dtype == torch.float64
         assert torch.equal(mb.data.to(ma.data.dtype), ma.data), name
+
+
+@pytest.mark.parametrize("csr_values", ["explicit", "coded"])
+def test_single_trajectory_batch_matches_oracle(csr_values):
+    """A batch of one trajectory (a zero-width trajectory field): basis, amplitudes and
+    observables equal the oracle's, with explicit and coded CSR values."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from paces_amd.config import devices
+    from paces_amd.models import holstein
+    if not devices.configure_called:
+        devices.configure()
+    N, J, g, ms = 12, 0.9, 1.3, 800
+    dim, steps, dt = 8, 8, 0.05
+    oham, recs = _oracle(N, J, g, ms, dim, steps, dt)
+    bham = holstein.BatchedHamiltonian([N], dim, [-J], [g])
+    bte = holstein.BatchedTimeEvolution(bham, [ms], enlarge_steps=1, precision="fp64",
+                                        csr_values=csr_values)
+    bte.create_single_exciton_states([N // 2])
+    t_arr = dt * np.arange(steps + 1)
+    for i in range(steps + 1):
+        print(f"[single trajectory {csr_values}] step {i}", flush=True)
+        bte.step(i, t_arr[i] - t_arr[i - 1] if i else 0.0)
+        n_exc, n_pho, _ = bte.observables()
+        raw = bham.decompress_ind_arr(bte.whoami).cpu().numpy()
+        rec = recs[i]
+        assert np.all(raw[:, 0] == 0)
+        np.testing.assert_array_equal(raw[:, 1:2 + N], oham.layout.unpack(rec.whoami))
+        assert np.array_equal(bte.vector.cpu().numpy(), rec.vec), i
+        assert np.max(np.abs(n_exc[0, :N] - rec.obs["n_exc"])) <= 1e-10, i
+        assert np.max(np.abs(n_pho[0, :N] - rec.obs["n_pho"])) <= 1e-10, i
