@@ -3,15 +3,21 @@
 Default workload (traj256): 256 independent 1D Holstein chains (open), N in {16,24,32,48},
 J ~ U[0.5,1.5], w0=1, g ~ U[0.5,2], phonon cap 7, maxstates ~ 10^U[3,5.3], exciton at N//2,
 dt=0.05, enlarge_steps=1, all advanced together by the batched form (complex64 Taylor terms,
-complex128 state). The same line carries the plain FP64 form's throughput on the same
-trajectories (one engine per trajectory), the batched form's maximum site-density error against
-the batched FP64 form, the Taylor term's HBM roofline and the oracle's CPU throughput.
+complex128 state). A step is the whole step of the reference's timeline: select, expansions,
+CSR, reassignment, Taylor series, then every trajectory's norm (checked against the explosion
+cutoff) and site observables. Before the timed run the same run is made once, untimed, so that
+the launch-shape tuner and the series graphs have seen every size the timed steps meet.
+
+The line also carries: the plain FP64 form's throughput on the same trajectories over the same
+steps (one engine per trajectory); the batched form's largest site-density error against the
+FP64 oracle at every 10th step and the last, on the trajectories the oracle runs in seconds;
+the Taylor term's HBM roofline; and the oracle's throughput on this host's cores.
 
 ``--workload ring64|chain16`` runs one plain-form trajectory instead (64-site periodic ring,
-cap 15, maxstates 2e6, dt 0.02; or 16-site open chain, cap 7, maxstates 2e4, dt 0.05).
+cap 15, maxstates 2e6, dt 0.02; or 16-site open chain, cap 7, maxstates 2e4, dt 0.05);
+``--workload learned`` the scorer-pruned form on the 16-site chain at maxstates 2e6.
 
-One step = select + expansion + CSR + reassignment + Taylor propagation, with every buffer in
-HBM. Prints one JSON line. ``--gpus`` above 1 exits: this engine runs on one GPU.
+Prints one JSON line. ``--gpus`` above 1 exits: this engine runs on one GPU.
 """
 
 from __future__ import annotations
@@ -224,40 +230,65 @@ def measure_batched_term(bte, reps=20):
 
 
 def batched_leg(draws, precision, warmup, steps, label, graph=False, autotune=False,
-                launch_shape=None, csr_values="coded"):
-    """Batched run: step 0 and ``warmup`` steps untimed, then ``steps`` timed steps. Returns
-    (state-steps, seconds, the evolution object)."""
+                launch_shape=None, csr_values="coded", record_steps=(), graph_pool=None,
+                verbose=True):
+    """Batched run over t = TRAJ_DT * arange(warmup + steps + 1) (each step's dt is the
+    difference of successive times, as in the reference's timeline): step 0 and ``warmup``
+    steps untimed, then ``steps`` timed whole steps (select, expansions, CSR, reassignment,
+    Taylor series, per-trajectory norm and site observables). Returns (state-steps, seconds,
+    the evolution object, {step: n_exc} for the steps in record_steps)."""
     import torch
     from paces_amd.models import holstein
+    recorded = {}
+    record_steps = set(record_steps)
+
+    def keep(i, n_exc, n_pho, nrm2):
+        if i in record_steps:
+            recorded[i] = n_exc.copy()
+
     bham = holstein.BatchedHamiltonian([d[0] for d in draws], 8, [-d[1] for d in draws],
                                        [d[2] for d in draws])
     bte = holstein.BatchedTimeEvolution(bham, [d[3] for d in draws], enlarge_steps=1,
                                         precision=precision, graph=graph, autotune=autotune,
-                                        launch_shape=launch_shape, csr_values=csr_values)
+                                        launch_shape=launch_shape, csr_values=csr_values,
+                                        on_observables=keep, graph_pool=graph_pool)
     bte.create_single_exciton_states([d[0] // 2 for d in draws])
-    bte.step(0, TRAJ_DT)
+    t_arr = TRAJ_DT * np.arange(warmup + steps + 1)
+    bte.step(0, 0.0)
     for i in range(1, warmup + 1):
-        bte.step(i, TRAJ_DT)
-        print(f"[{label}] warmup step {i}: basis {bte.numstates}", file=sys.stderr, flush=True)
+        bte.step(i, t_arr[i] - t_arr[i - 1])
+        if verbose:
+            print(f"[{label}] warmup step {i}: basis {bte.numstates}", file=sys.stderr,
+                  flush=True)
     torch.cuda.synchronize()
     prevs = []  # each step's incoming segment offsets, counted after the timed region
+    w0 = time.monotonic_ns()
     t0 = time.perf_counter()
     for i in range(warmup + 1, warmup + 1 + steps):
         prevs.append(bte.seg_off)
-        bte.step(i, TRAJ_DT)
+        bte.step(i, t_arr[i] - t_arr[i - 1])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    if verbose:
+        # the timed window on CLOCK_MONOTONIC, to cut a kernel trace to the timed steps
+        print(f"[{label}] timed window monotonic_ns {w0} {time.monotonic_ns()}",
+              file=sys.stderr, flush=True)
+    bte.observables()  # the last step's norms are checked and recorded
     units = sum(int(np.minimum(np.diff(p.cpu().numpy()), bte.maxstates).sum()) for p in prevs)
-    print(f"[{label}] timed {steps} steps in {elapsed:.3f} s, basis {bte.numstates}",
-          file=sys.stderr, flush=True)
-    return units, elapsed, bte
+    if verbose:
+        print(f"[{label}] timed {steps} steps in {elapsed:.3f} s, basis {bte.numstates}",
+              file=sys.stderr, flush=True)
+    return units, elapsed, bte, recorded
 
 
 def plain_leg(draws, warmup, steps):
-    """Every trajectory stepped by its own plain FP64 engine, one after the other."""
+    """Every trajectory stepped by its own plain FP64 engine, one after the other, over the
+    same steps as the batched leg (untimed step 0 and warmup steps, then ``steps`` timed whole
+    steps with norm and observables)."""
     import torch
     from paces_amd.core.time_evolution import ExpmParams, TimeEvoParams
     from paces_amd.models import holstein
+    t_arr = TRAJ_DT * np.arange(warmup + steps + 1)
     tes = []
     for (N, J, g, ms) in draws:
         ham = holstein.Hamiltonian(N, [8] * N, use_terms=use_terms(J, g))
@@ -273,7 +304,7 @@ def plain_leg(draws, warmup, steps):
         tes.append(te)
     for i in range(1, warmup + 1):
         for te in tes:
-            te._evolve_single_step(i, TRAJ_DT * i, TRAJ_DT)
+            te._evolve_single_step(i, t_arr[i], t_arr[i] - t_arr[i - 1])
         print(f"[plain] warmup step {i}: basis {sum(te.numstates for te in tes)}",
               file=sys.stderr, flush=True)
     torch.cuda.synchronize()
@@ -282,7 +313,7 @@ def plain_leg(draws, warmup, steps):
     for i in range(warmup + 1, warmup + 1 + steps):
         for te, d in zip(tes, draws):
             prev = te.numstates
-            te._evolve_single_step(i, TRAJ_DT * i, TRAJ_DT)
+            te._evolve_single_step(i, t_arr[i], t_arr[i] - t_arr[i - 1])
             units += min(prev, d[3])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -290,58 +321,69 @@ def plain_leg(draws, warmup, steps):
     return units, elapsed
 
 
-def cpu_baseline_traj(draws, budget_s=25.0, max_ms=20000):
-    """The oracle on this host, one trajectory at a time, in draw order over the draws with
-    maxstates <= max_ms: each is grown untimed until its basis holds maxstates rows (at most 12
-    steps) plus one step, so that the selection saturates, then two whole steps are timed; stops
-    when the timed budget is used."""
-    from oracle import paces_oracle as po
-    units, elapsed, ntraj, nsteps = 0, 0.0, 0, 0
-    wall0 = time.perf_counter()
-    for (N, J, g, ms) in draws:
-        if ms > max_ms:
-            continue
-        if elapsed >= budget_s or time.perf_counter() - wall0 > 3 * budget_s:
-            break
-        ham = po.HolsteinOracle(N, [8] * N, use_terms=use_terms(J, g))
-        te = po.TimeEvolutionOracle(ham, ms, 1)
-        te.create_single_state_basis(N // 2)
-        raw = np.zeros((1, N + 1), dtype=np.int64)
-        raw[0, 0] = N // 2
-        te.create_initial_vector([1.0], raw)
-        te.step(0, 0.0, 0.0)
-        i = 0
-        while i < 12 and te.whoami.shape[0] < ms:
-            i += 1
-            te.step(i, TRAJ_DT * i, TRAJ_DT)
-        i += 1  # one more: the selection now saturates maxstates
-        te.step(i, TRAJ_DT * i, TRAJ_DT)
-        for _ in range(2):
-            prev = te.whoami.shape[0]
-            i += 1
-            t0 = time.perf_counter()
-            te.step(i, TRAJ_DT * i, TRAJ_DT)
-            elapsed += time.perf_counter() - t0
-            units += min(prev, ms)
-            nsteps += 1
-        ntraj += 1
-    return {"value": units / elapsed, "unit": "state-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (NumPy, one thread) on the first {ntraj} draws with maxstates <= "
-                      f"{max_ms}, each grown untimed to steady state, then {nsteps} timed whole "
-                      f"steps: {units} retained state-steps in {elapsed:.1f} s"}
+def host_cpu():
+    """(CPU model name, cores this process may run on)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo", encoding="utf-8") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def oracle_runs(draws, nsteps, record, max_ms, workers):
+    """The oracle on every draw with maxstates <= max_ms, one process per worker (spawned, so
+    nothing of this process's GPU state is inherited). Returns ({draw index: {step: n_exc}},
+    retained state-steps, wall seconds of the runs, worker count)."""
+    import multiprocessing as mp
+    from oracle import pool as opool
+    idx = [k for k, d in enumerate(draws) if d[3] <= max_ms]
+    # the largest first, so the pool ends evenly
+    idx.sort(key=lambda k: -draws[k][3])
+    jobs = [(*draws[k], TRAJ_DT, nsteps, tuple(record)) for k in idx]
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(workers) as pool:
+        pool.map(opool.ready, range(workers))  # the workers are up before the clock starts
+        t0 = time.perf_counter()
+        res = pool.map(opool.run_chain, jobs, chunksize=1)
+        wall = time.perf_counter() - t0
+    return ({k: r[0] for k, r in zip(idx, res)}, sum(r[1] for r in res), wall, len(jobs))
 
 
 def run_traj256(a):
-    """Headline line: the batched form (default mixed precision) of the 256-trajectory workload,
-    with the plain FP64 form's throughput on the same trajectories, the batched form's density
-    error against the batched FP64 form (which matches the oracle to 1e-10), the Taylor term's
-    roofline and the oracle's CPU throughput."""
+    """Headline line: the batched form (default complex64 terms) of the 256-trajectory workload,
+    beside the plain FP64 form on the same trajectories over the same steps, the batched form's
+    site-density error against the FP64 oracle, the Taylor term's roofline and the oracle's CPU
+    throughput."""
     import torch
+    from paces_amd.models import holstein
     draws = traj256_draws(a.ntraj)
     prec = a.precision_batched
-    units, elapsed, bte = batched_leg(draws, prec, a.warmup, a.steps, "batched-" + prec,
-                                      graph=a.graph, autotune=a.autotune,
-                                      launch_shape=a.term_shape, csr_values=a.csr_values)
+    last = a.warmup + a.steps
+    record = sorted(set(range(10, last + 1, 10)) | {last})
+    pool = None
+    if a.graph or a.autotune:
+        # untimed preparation: the same run once through, so that the tuner has chosen the
+        # Taylor term's launch shape for every size the run meets and the series graphs of
+        # those sizes are captured (the timed run replays them)
+        dev = torch.device("cuda", torch.cuda.current_device())
+        pool = holstein.SeriesGraphPool(dev)
+        t0 = time.perf_counter()
+        *_, prep, _ = batched_leg(draws, prec, a.warmup, a.steps, "prepare", graph=a.graph,
+                                  autotune=a.autotune, launch_shape=a.term_shape,
+                                  csr_values=a.csr_values, graph_pool=pool, verbose=False)
+        del prep
+        torch.cuda.empty_cache()
+        print(f"[prepare] tuned and captured in {time.perf_counter() - t0:.2f} s",
+              file=sys.stderr, flush=True)
+    units, elapsed, bte, recorded = batched_leg(
+        draws, prec, a.warmup, a.steps, "batched-" + prec, graph=a.graph, autotune=a.autotune,
+        launch_shape=a.term_shape, csr_values=a.csr_values, record_steps=record,
+        graph_pool=pool)
     rows = bte.numstates
     value = units / elapsed
     bte_csr_values = bte.csr_values
@@ -358,29 +400,46 @@ def run_traj256(a):
                          "coded" if bte.csr_values == "coded" else "tuned", bte.csr_values),
             "avg_us": round(sec * 1e6, 2), "bytes_per_launch": int(nbytes), "rows": int(rows),
             "launches_timed": reps}
-    n_exc_b = bte.observables()[0]
-    del bte
+    del bte, pool
     torch.cuda.empty_cache()
     extra = {}
-    if not a.no_check and prec != "fp64":
-        _, _, ref = batched_leg(draws, "fp64", a.warmup, a.steps, "batched-fp64")
-        n_exc_r = ref.observables()[0]
-        extra["density_err_vs_fp64"] = float(np.max(np.abs(n_exc_b - n_exc_r)))
-        del ref
-        torch.cuda.empty_cache()
     if not a.no_plain:
-        pu, pe = plain_leg(draws, a.warmup, a.plain_steps)
+        pu, pe = plain_leg(draws, a.warmup, a.steps)
         extra["plain_gpu_state_steps_per_s"] = pu / pe
-        extra["plain_ms_per_step"] = 1e3 * pe / a.plain_steps
+        extra["plain_ms_per_step"] = 1e3 * pe / a.steps
         extra["speedup_vs_plain"] = value / (pu / pe)
         torch.cuda.empty_cache()
-    cpu = None if a.no_cpu_baseline else cpu_baseline_traj(draws, a.cpu_budget)
+    cpu = None
+    if not (a.no_check and a.no_cpu_baseline):
+        model, host_cores = host_cpu()
+        workers = max(1, min(16, host_cores))
+        ref, ounits, owall, njobs = oracle_runs(draws, last, record, a.oracle_max_ms, workers)
+        if not a.no_check:
+            err = 0.0
+            for k, per_step in ref.items():
+                N = draws[k][0]
+                for i, n_exc in per_step.items():
+                    err = max(err, float(np.max(np.abs(recorded[i][k, :N] - n_exc))))
+            extra["density_err_vs_fp64_oracle"] = err
+            extra["density_err_checked"] = (f"max over sites of |n_exc - n_exc(oracle)| at steps "
+                                            f"{record}, trajectories with maxstates <= "
+                                            f"{a.oracle_max_ms} ({njobs} of {len(draws)})")
+        if not a.no_cpu_baseline:
+            cpu = {"value": ounits / owall, "unit": "state-steps/s", "cores": workers,
+                   "kind": "port", "cpu_model": model, "host_cores": host_cores,
+                   "sample": f"oracle (NumPy) on the {njobs} draws with maxstates <= "
+                             f"{a.oracle_max_ms}, steps 1..{last} from the initial state, one "
+                             f"process per core: {ounits} retained state-steps in {owall:.1f} s "
+                             f"wall"}
     dtype = "c64 terms / f64 state" if prec == "mixed" else "f64"
     line = {"metric": METRIC, "value": value, "unit": "state-steps/s", "n_gpus": 1,
             "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * elapsed / a.steps,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
             "data": "synthetic",
-            "config": dict({"workload": TRAJ_DESC.format(n=a.ntraj), "form": "batched-" + prec,
+            "config": dict({"workload": TRAJ_DESC.format(n=a.ntraj),
+                            "form": "batched-" + prec + " (no scorer)",
+                            "step": "select, expansions, CSR, reassignment, Taylor series, "
+                                    "per-trajectory norm check and n_exc/n_pho",
                             "series_graph": bool(a.graph), "autotune": bool(a.autotune),
                             "term_shape": list(a.term_shape) if a.term_shape else "tuned",
                             "csr_values": bte_csr_values,
@@ -569,11 +628,12 @@ def main(argv=None):
                     help="traj256 only: 'plain' times the per-trajectory engines alone")
     ap.add_argument("--ntraj", type=int, default=256)
     ap.add_argument("--precision-batched", default="mixed", choices=["fp64", "mixed"])
-    ap.add_argument("--plain-steps", type=int, default=2,
-                    help="timed steps of the plain form beside the batched line")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-check", action="store_true",
-                    help="skip the batched FP64 run that gives the density error")
+                    help="skip the site-density error against the oracle")
+    ap.add_argument("--oracle-max-ms", type=int, default=5000,
+                    help="traj256: trajectories with maxstates up to this run on the oracle "
+                         "(density error and CPU baseline)")
     ap.add_argument("--graph", type=int, default=1,
                     help="traj256: run each step's Taylor series as one replayed HIP graph")
     ap.add_argument("--term-shape", default=None,

@@ -284,6 +284,18 @@ int paces_merge_unique_diag(const paces_runs_t* runs, int32_t W, uint64_t* out_r
                             const paces_layout_t* lay, const paces_holstein_t* h, double* diag,
                             float* diag32, void* ws, size_t ws_bytes, void* stream);
 
+/* ---- Per-position sums behind V_hop and V_vib_coupl (calculate_hopping / calculate_coupling,
+ * paces/models/holstein.py:325-365, which apply _calculate_partitioned_sum, 380-396, to
+ * conj(psi) * (M psi)): out[2s], out[2s+1] = sum over rows i in [seg_off[s], seg_off[s+1]) of
+ * conj(x_i) * y_i, complex128 interleaved. One workgroup per segment, fixed reduction order. */
+int paces_segment_vdot(const double* x, const double* y, const int32_t* seg_off, int32_t nseg,
+                       double* out, void* stream);
+/* <mu(t) mu(0)> (calculate_mu_to_0, paces/models/holstein.py:366-377): out[0..1] = the sum of
+ * x over the rows whose bits after the first lead_bits (the position field) are all zero; with
+ * rows sorted, that is the first row of a position's segment when it is the phonon vacuum. */
+int paces_vacuum_sum(const uint64_t* rows, int32_t W, int32_t lead_bits, const double* x,
+                     const int32_t* seg_off, int32_t nseg, double* out, void* stream);
+
 /* ---- H1: observables n_exc[N], n_pho[N], sum |psi|^2 (calculate_n_exc / calculate_n_pho,
  * paces/models/holstein.py:311-322; calc_bath_n_b_kernel, _holstein_kernels.py:215-255).
  * out has 2N+1 doubles. Deterministic (fixed reduction order). */

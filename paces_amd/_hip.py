@@ -197,6 +197,8 @@ _SIGNATURES = {
     "paces_taylor_term_mixed": (C.c_int, [_I64, _P, C.POINTER(CsrSet32), _P, _P, _P, C.c_float,
                                           _I32, _D, _P, _P, _SZ, _P]),
     "paces_batch_segments": (C.c_int, [_P, _I64, _I32, _I32, _P, _P]),
+    "paces_segment_vdot": (C.c_int, [_P, _P, _P, _I32, _P, _P]),
+    "paces_vacuum_sum": (C.c_int, [_P, _I32, _I32, _P, _P, _I32, _P, _P]),
     "paces_batch_select_workspace_bytes": (_SZ, [_I64, _I32]),
     "paces_batch_select_topk": (C.c_int, [_P, _P, _I64, _I32, _P, _I32, _P, _P, _P, _P, _SZ, _P]),
     "paces_holstein_observables_batched_workspace_bytes": (_SZ, [_I64, _I32, _I32]),

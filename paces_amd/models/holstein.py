@@ -883,6 +883,20 @@ class BatchedHamiltonian(Hamiltonian):
         return self.split_observables(out.cpu().numpy())
 
 
+class SeriesGraphPool:
+    """Captured series graphs and the device descriptor their kernels read (N1).
+
+    A series graph depends only on the row capacity, the trajectory count, the term count, the
+    precision and the launch shape: everything else comes from the descriptor, rewritten before
+    each replay. Engines that pass the same pool (on the same device and stream) reuse each
+    other's captures, so a run prepared once replays without capturing again."""
+
+    def __init__(self, device):
+        self.device = device
+        self.desc = torch.empty(C.sizeof(_hip.BatchSeries), dtype=torch.uint8, device=device)
+        self.graphs = {}
+
+
 class BatchedTimeEvolution:
     """Adaptive-basis evolution of every trajectory of a BatchedHamiltonian in one set of
     launches per step: per-trajectory select, one shared expansion/union/CSR/reassignment, and a
@@ -894,7 +908,7 @@ class BatchedTimeEvolution:
     def __init__(self, hamobj: BatchedHamiltonian, maxstates, enlarge_steps=1,
                  precision="mixed", m_star=100, taylor_tol=None, autotune=False, graph=False,
                  launch_shape=None, csr_values="coded", explosion_cutoff=2.0,
-                 on_observables=None):
+                 on_observables=None, graph_pool=None):
         self.hamobj = hamobj
         # every step ends with the per-trajectory norm and site observables (as the reference's
         # step does, time_evolution.py:585-605); they travel to pinned host memory without a
@@ -942,8 +956,7 @@ class BatchedTimeEvolution:
         # graph=True: the Taylor series runs as one HIP graph per step (captured once per row
         # capacity and term count, replayed with a rewritten device descriptor)
         self.graph = bool(graph)
-        self._graphs = {}
-        self._desc_dev = None
+        self.graph_pool = graph_pool if graph_pool is not None else SeriesGraphPool(self.device)
         self._graph_terms = 8 if precision == "mixed" else 16
         self.last_launch_shape = None
         self.whoami = None
@@ -1092,21 +1105,19 @@ class BatchedTimeEvolution:
         d.tol = self.tol
         for j in range(_hip.SERIES_MAX_TERMS):
             d.c[j] = engine.taylor_coeff(delta_t, j)
-        if self._desc_dev is None:
-            self._desc_dev = torch.empty(C.sizeof(_hip.BatchSeries), dtype=torch.uint8,
-                                         device=self.device)
-        self._desc_dev.copy_(torch.frombuffer(bytearray(bytes(d)), dtype=torch.uint8))
+        pool = self.graph_pool
+        pool.desc.copy_(torch.frombuffer(bytearray(bytes(d)), dtype=torch.uint8))
         shape = (cfg.block, cfg.rows_per_thread) if cfg is not None else None
-        key = (cap, nterms, mixed, shape)
-        g = self._graphs.get(key)
+        key = (cap, h.ntraj, nterms, mixed, shape)
+        g = pool.graphs.get(key)
         if g is None:
             g = torch.cuda.CUDAGraph()
             cref = C.byref(cfg) if cfg is not None else None
             with torch.cuda.graph(g):
-                check(_lib.paces_batch_series_launch(ptr(self._desc_dev), cap, h.ntraj,
+                check(_lib.paces_batch_series_launch(ptr(pool.desc), cap, h.ntraj,
                                                      1 if mixed else 0, nterms, cref, stream()),
                       "batch_series_launch")
-            self._graphs[key] = g
+            pool.graphs[key] = g
         g.replay()
         return nterms
 
