@@ -619,6 +619,46 @@ def observables(ham: HolsteinOracle, rows, vec):
     return {"n_exc": n_exc, "n_pho": n_pho, "norm": float(np.sqrt(np.sum(w)))}
 
 
+def per_position_products(ham: HolsteinOracle, rows, vec, csr):
+    """_calculate_partitioned_sum (holstein.py:380-396) of conj(psi) * (M psi) for one term's
+    CSR: per exciton position p, the sum over its rows.
+    Pinned by: dense <psi| P_p M |psi> on tiny lattices (test_oracle.py)."""
+    ip, ix, dv = csr
+    tr, ti = csr_matvec_seq(ip, ix, dv, np.ascontiguousarray(vec.real),
+                            np.ascontiguousarray(vec.imag))
+    prod = np.conj(vec) * _cplx(tr, ti)
+    pos = ham.get_pos(rows)
+    out = np.zeros(ham.nchain, dtype=np.complex128)
+    np.add.at(out, pos, prod)
+    return out
+
+
+def hopping_bonds(ham: HolsteinOracle, rows, vec, hop_csr):
+    """calculate_hopping (holstein.py:325-359): bond energies from the per-position products V
+    of the total hopping, bond i = sum_j (1 - B_ij) V_j - B_ij conj(V_j), B the board of
+    holstein.py:348-352 (1 at j > i with i + j even and at j < i with i + j odd). Open chains
+    return the first N - 1 entries, odd rings all N.
+    Pinned by: brute-force bond energies <psi| T_(i,i+1) |psi> from the dense hopping matrix."""
+    N = ham.nchain
+    if ham.periodic and N % 2 == 0:
+        raise NotImplementedError("even ring")
+    v = per_position_products(ham, rows, vec, hop_csr)
+    out = np.zeros(N, dtype=np.complex128)
+    for i in range(N):
+        for j in range(N):
+            on_board = (j > i and (i + j) % 2 == 0) or (j < i and (i + j) % 2 == 1)
+            out[i] += -np.conj(v[j]) if on_board else v[j]
+    return out if ham.periodic else out[:-1]
+
+
+def vacuum_amplitude_sum(ham: HolsteinOracle, rows, vec):
+    """calculate_mu_to_0 (holstein.py:366-377): sum of psi over rows with every phonon zero.
+    Pinned by: a per-row check of the unpacked fields."""
+    raw = ham.layout.unpack(rows)
+    mask = np.all(raw[:, 1:] == 0, axis=1)
+    return complex(np.sum(vec[mask]))
+
+
 def energy(diag, csrs, vec):
     """calculate_total_energy (observables.py:95-99): vdot(psi, H psi)."""
     return complex(np.vdot(vec, total_ham(diag, csrs, vec)))
@@ -738,6 +778,56 @@ class TimeEvolutionOracle:
             rec = self.step(i, float(t), dt)
             if record is not None:
                 record(rec)
+
+
+# ------------------------------------------------------------------------------------------------
+# Initial bases and states (holstein.py:406-569)
+# ------------------------------------------------------------------------------------------------
+
+def nonuniform_basis_tuples(nchain, truncate, lowest=None, minpos=0, maxpos=None):
+    """create_nonuniform_basis (holstein.py:427-476): exciton at minpos..maxpos-1; a site with
+    truncate[j] > 1 takes lowest[j] .. lowest[j] + truncate[j] - 1 phonons, every other site 0
+    (its lowest value is not used). Lexicographic order.
+    Pinned by: itertools enumeration of hand-written examples (test_oracle.py)."""
+    import itertools
+    maxpos = nchain if maxpos is None else maxpos
+    lowest = [0] * nchain if lowest is None else list(lowest)
+    ranges = [range(lowest[j], lowest[j] + truncate[j]) if truncate[j] > 1 else range(0, 1)
+              for j in range(nchain)]
+    out = [(p, *occ) for p in range(minpos, maxpos) for occ in itertools.product(*ranges)]
+    return np.array(out, dtype=np.int64).reshape(-1, nchain + 1)
+
+
+def moving_gaussian_tuples(nchain, super_mu, super_sigma, sigma, maxval):
+    """create_moving_gaussian_obc_basis (holstein.py:479-526) without its bounds checks:
+    position i, site j takes 0 .. ceil((1 + maxval g(i; super_mu, super_sigma)) g(j; i, sigma))
+    - 1 phonons with g(x; m, s) = exp(-(x - m)^2 / (2 s^2)); positions ascending.
+    Pinned by: hand-computed ranges and counts (test_oracle.py)."""
+    import itertools
+    rows = []
+    for i in range(nchain):
+        height = 1.0 + maxval * math.exp(-0.5 * ((i - super_mu) / super_sigma) ** 2)
+        tops = [math.ceil(height * math.exp(-0.5 * ((j - i) / sigma) ** 2)) for j in range(nchain)]
+        rows += [(i, *occ) for occ in itertools.product(*[range(t) for t in tops])]
+    return np.array(rows, dtype=np.int64).reshape(-1, nchain + 1)
+
+
+def tensor_state(nchain, bstates, coeffs, excsite, sites="all"):
+    """create_tensor_init_state (holstein.py:528-569): (coefficients, tuples) of
+    |excsite> (x) prod over the chosen sites of sum_j c_j |b_j>, the other sites in vacuum.
+    Pinned by: the Kronecker product of the one-site states in the full space (test_oracle.py)."""
+    import itertools
+    chosen = list(range(nchain)) if isinstance(sites, str) else sorted(set(sites))
+    coef, rows = [], []
+    for combo in itertools.product(range(len(bstates)), repeat=len(chosen)):
+        occ = [0] * nchain
+        c = 1.0 + 0j
+        for site, k in zip(chosen, combo):
+            occ[site] = int(bstates[k])
+            c *= complex(coeffs[k])
+        coef.append(c)
+        rows.append((excsite, *occ))
+    return np.array(coef, dtype=np.complex128), np.array(rows, dtype=np.int64)
 
 
 # ------------------------------------------------------------------------------------------------

@@ -490,6 +490,46 @@ class Hamiltonian(HamiltonianFramework):
         return h[:N], h[N:2 * N], float(h[2 * N])
 
 
+    def position_segments(self, rows):
+        """seg_off (device int32, nchain + 1): the rows of exciton position p are
+        [seg_off[p], seg_off[p + 1]) (the position is the leading field of a sorted basis)."""
+        seg = torch.empty(self.nchain + 1, dtype=torch.int32, device=self.device)
+        R = rows.shape[0]
+        check(_lib.paces_batch_segments(ptr(rows) if R else None, R, self.totwordwidth,
+                                        self.nchain, ptr(seg), stream()), "batch_segments")
+        return seg
+
+    def position_vdot(self, seg, x, y):
+        """Per exciton position p: sum of conj(x) * y over its rows (complex128[N], host)."""
+        out = torch.empty(2 * self.nchain, dtype=torch.float64, device=self.device)
+        if x.shape[0] == 0:
+            return numpy.zeros(self.nchain, dtype=numpy.complex128)
+        check(_lib.paces_segment_vdot(ptr(torch.view_as_real(x)), ptr(torch.view_as_real(y)),
+                                      ptr(seg), self.nchain, ptr(out), stream()),
+              "segment_vdot")
+        h = out.cpu().numpy()
+        return h[0::2] + 1j * h[1::2]
+
+    def vacuum_sum(self, rows, seg, x):
+        """Sum of x over the rows with every phonon field zero (complex)."""
+        if x.shape[0] == 0:
+            return 0j
+        out = torch.empty(2, dtype=torch.float64, device=self.device)
+        check(_lib.paces_vacuum_sum(ptr(rows), self.totwordwidth, self.posbitwidth,
+                                    ptr(torch.view_as_real(x)), ptr(seg), self.nchain, ptr(out),
+                                    stream()), "vacuum_sum")
+        h = out.tolist()
+        return complex(h[0], h[1])
+
+
+def bond_board(n):
+    """The n x n sign board of calculate_hopping (holstein.py:348-352): 1 where the position
+    sum of row i's bond runs through the conjugate of V_j, i.e. at (i, j) with j > i and i + j
+    even, or j < i and i + j odd."""
+    i, j = numpy.meshgrid(numpy.arange(n), numpy.arange(n), indexing="ij")
+    return (((j > i) & ((i + j) % 2 == 0)) | ((j < i) & ((i + j) % 2 == 1))).astype(numpy.int64)
+
+
 class Observables(ObservablesFramework):
     """Holstein observables (holstein.py:291-396)."""
 
@@ -500,6 +540,7 @@ class Observables(ObservablesFramework):
 
     def primer_hook(self):
         self._cache_key = None
+        self._seg_key = None
 
     def _site_obs(self, vector):
         key = (vector.data_ptr(), vector._version, self.teobj.whoami.data_ptr())
@@ -517,6 +558,55 @@ class Observables(ObservablesFramework):
     def calculate_n_pho(self, vector):
         """Phonon number per site: sum of |psi|^2 * n_site."""
         return self._site_obs(vector)[1]
+
+    def _position_segments(self):
+        whoami = self.teobj.whoami
+        key = (whoami.data_ptr(), whoami.shape[0])
+        if getattr(self, "_seg_key", None) != key:
+            self._seg = self.hamobj.position_segments(whoami)
+            self._seg_key = key
+        return self._seg
+
+    def _term_per_position(self, term, vector):
+        """V_p = sum over rows at exciton position p of conj(psi) * (M_term psi)
+        (_calculate_partitioned_sum of holstein.py:380-396 applied to those products)."""
+        mat = self.teobj.sparse_mats_dict[term]
+        return self.hamobj.position_vdot(self._position_segments(), vector, mat.dot(vector))
+
+    @obs_attrs(fname="V_hop", header=" ".join(f"V_{i}" for i in range(3)) + " etc.")
+    def calculate_hopping(self, vector):
+        """Hopping energy of each bond (holstein.py:325-359): from the per-position products
+        V_p = <psi| P_p T |psi> of the total hopping T, bond i's energy is
+        sum_j (1 - B_ij) V_j - B_ij conj(V_j) with B = bond_board(N). An open chain gives its
+        N - 1 bonds (the formula's last entry is zero), a ring of odd length its N bonds; rings
+        of even length are not covered by the formula."""
+        h = self.hamobj
+        if h.periodic and h.nchain % 2 == 0:
+            raise NotImplementedError("the bond formula needs an odd ring length (or open "
+                                      "boundaries)")
+        v = self._term_per_position("hopping", vector)
+        board = bond_board(h.nchain)
+        bonds = ((1 - board) * v[None, :] - board * numpy.conj(v)[None, :]).sum(axis=1)
+        if h.periodic:
+            return bonds
+        if abs(bonds[-1]) >= 1e-12:
+            raise RuntimeError(f"open chain: the bond past the last site has energy "
+                               f"{bonds[-1]} (zero expected)")
+        return bonds[:-1]
+
+    @obs_attrs(fname="V_vib_coupl", header=" ".join(f"V_{i}" for i in range(3)) + " etc.")
+    def calculate_coupling(self, vector):
+        """Exciton-phonon coupling energy per site: sum over rows at exciton position p of
+        conj(psi) * (M_vib psi) (holstein.py:360-365). The reference reads the matrix of a
+        term named "coupling", which does not exist (its term is "vib_coupling", so it raises
+        KeyError); here the vib_coupling matrix is used."""
+        return self._term_per_position("vib_coupling", vector)
+
+    @obs_attrs(fname="mu_to_0", header=r"<mu(t) mu(0)>")
+    def calculate_mu_to_0(self, vector):
+        """<mu(t) mu(0)> for a start in the global vacuum (holstein.py:366-377): the sum of the
+        amplitudes of the rows with no phonons, at any exciton position."""
+        return self.hamobj.vacuum_sum(self.teobj.whoami, self._position_segments(), vector)
 
 
 class TimeEvolution(TimeEvolutionFramework):
@@ -563,6 +653,91 @@ class TimeEvolution(TimeEvolutionFramework):
         raw = cartesian_product(numpy.arange(N), *([numpy.arange(truncate_d)] * N))
         self.whoami = self.hamobj.compress_ind_arr(raw)
         self.numstates = self.whoami.shape[0]
+
+    @staticmethod
+    def moving_gaussian_tuples(nchain, super_mu, super_sigma, sigma, maxval, maxstates):
+        """The field tuples of create_moving_gaussian_obc_basis (holstein.py:479-526), in
+        position order. With w(x, m, s) = exp(-(x - m)^2 / (2 s^2)): for exciton position i,
+        site j holds 0 .. ceil((1 + maxval w(i, super_mu, super_sigma)) w(j, i, sigma)) - 1
+        phonons (an empty range leaves position i without states), every combination of the
+        sites' ranges once, site 0 varying slowest. Raises ValueError once the count passes
+        maxstates."""
+        sites = numpy.arange(nchain, dtype=numpy.float64)
+
+        def w(x, m, s):
+            return numpy.exp(-0.5 * ((x - m) / s) ** 2)
+
+        height = 1.0 + maxval * w(sites, super_mu, super_sigma)
+        blocks, total = [], 0
+        for i in range(nchain):
+            tops = numpy.ceil(height[i] * w(sites, i, sigma)).astype(numpy.int64)
+            count = int(numpy.prod(numpy.maximum(tops, 0).astype(object)))
+            total += count
+            if total > maxstates:
+                raise ValueError(f"the Gaussian basis has more than maxstates={maxstates} "
+                                 f"states (position {i} brings it to {total})")
+            if count == 0:
+                continue
+            block = numpy.zeros((count, nchain + 1), dtype=numpy.int64)
+            block[:, 0] = i
+            block[:, 1:] = cartesian_product(*[numpy.arange(t, dtype=numpy.int64) for t in tops])
+            blocks.append(block)
+        if not blocks:
+            return numpy.zeros((0, nchain + 1), dtype=numpy.int64)
+        return numpy.concatenate(blocks)
+
+    def create_moving_gaussian_obc_basis(self, super_mu, super_sigma, sigma, maxval):
+        """Initial basis of Gaussian phonon ranges around each exciton position, the heights
+        peaked at super_mu (holstein.py:479-526); see moving_gaussian_tuples.
+
+        The reference ends with a bounds check whose operands have N + 1 and N entries, which
+        NumPy and CuPy refuse to broadcast, so it raises on every call; here the phonon
+        columns are checked against max_ho_dims, the check it intends."""
+        h = self.hamobj
+        if maxval >= int(h.max_ho_dims.max()):
+            raise ValueError(f"maxval={maxval} reaches the largest phonon dimension "
+                             f"{int(h.max_ho_dims.max())}")
+        self.write_current_params_to_file(inspect.currentframe())
+        raw = self.moving_gaussian_tuples(h.nchain, super_mu, super_sigma, sigma, maxval,
+                                          self.te_params.maxstates)
+        if raw.shape[0] and numpy.any(raw[:, 1:].max(axis=0) >= h.max_ho_dims):
+            raise ValueError("a Gaussian phonon range exceeds max_ho_dims")
+        self.whoami = self.hamobj.compress_ind_arr(raw)
+        self.numstates = int(self.whoami.shape[0])
+
+    @staticmethod
+    def tensor_state_tuples(nchain, bstates, coeffs, excsite, sites="all"):
+        """(coefficients, field tuples) of create_tensor_init_state (holstein.py:528-569): the
+        exciton at excsite times, on each chosen site, sum_j coeffs[j] |bstates[j]>, other sites
+        in their vacuum; one entry per combination of the chosen sites' Fock states (first
+        chosen site slowest), with the product of their coefficients."""
+        bstates = numpy.asarray(bstates)
+        coeffs = numpy.asarray(coeffs, dtype=numpy.complex128)
+        if not (isinstance(sites, str) and sites == "all") and len(sites) > nchain:
+            raise ValueError(f"{len(sites)} sites chosen on a lattice of {nchain}")
+        if len(bstates) != len(coeffs):
+            raise ValueError(f"{len(coeffs)} coefficients for {len(bstates)} Fock states")
+        if isinstance(sites, str) and sites == "all":
+            chosen = numpy.arange(nchain)
+        else:
+            chosen = numpy.array([i for i in range(nchain) if i in sites], dtype=numpy.int64)
+        k = len(chosen)
+        vec = cartesian_product(*([coeffs] * k)).prod(axis=1) if k else numpy.ones(1, complex)
+        raw = numpy.zeros((vec.shape[0], nchain + 1), dtype=numpy.int64)
+        raw[:, 0] = excsite
+        if k:
+            raw[:, 1 + chosen] = cartesian_product(*([bstates.astype(numpy.int64)] * k))
+        return vec, raw
+
+    def create_tensor_init_state(self, bstates, coeffs, excsite, sites="all"):
+        """Initial vector |excsite> (x) prod over the chosen sites of sum_j c_j |j>, other sites
+        in their vacuum (holstein.py:528-569), on the current basis, which must hold every
+        product state; see tensor_state_tuples. The state must be normalised to 1e-12."""
+        vec, raw = self.tensor_state_tuples(self.hamobj.nchain, bstates, coeffs, excsite, sites)
+        norm = float(numpy.linalg.norm(vec))
+        if abs(1.0 - norm) > 1e-12:
+            raise ValueError(f"the tensor-product state has norm {norm}, not 1")
+        self.create_initial_vector(vec, raw, auto_normalize=True)
 
 
 # ------------------------------------------------------------------------------------------------

@@ -366,3 +366,108 @@ def test_explosion_cutoff_raises():
     te.create_initial_vector([1.0], raw)
     with pytest.raises(RuntimeError):
         te.generate_timeline(np.array([0.0, 0.05]))
+
+
+def _full_space_csrs(ham):
+    """Every state of a tiny lattice, its packed rows and the per-term CSR of the whole space
+    (the full space expanded once is itself)."""
+    raw = po.all_states(ham)
+    rows = ham.layout.pack(raw)
+    diag, new_inds, coo, _, _ = ham.generate_mel(rows, 0)
+    np.testing.assert_array_equal(new_inds, rows)
+    R = rows.shape[0]
+    return raw, rows, {t: po.coo_to_csr(*coo[t], R) for t in ham.offdiag_terms}
+
+
+def _bond_matrix(ham, raw, a, b):
+    """Dense hopping restricted to the bond {a, b}: H entries between states that differ only
+    by the exciton moving between sites a and b."""
+    H = po.dense_hamiltonian(ham, raw)
+    n = raw.shape[0]
+    M = np.zeros((n, n))
+    for k in range(n):
+        for j in np.nonzero(H[:, k])[0]:
+            if j != k and {raw[j, 0], raw[k, 0]} == {a, b} and np.array_equal(raw[j, 1:],
+                                                                               raw[k, 1:]):
+                M[j, k] = H[j, k]
+    return M
+
+
+@pytest.mark.parametrize("N,periodic", [(5, False), (4, False), (5, True), (3, True)])
+def test_hopping_bonds_equal_brute_force_bond_energies(N, periodic):
+    """calculate_hopping's board formula gives <psi| T_bond |psi> for each bond (brute force
+    from the dense hopping matrix)."""
+    ham = po.HolsteinOracle(N, [2] * N, periodic=periodic, use_terms={
+        "diag": {"eps_sys": 0.0, "hbar_omega": 1.0, "delta_eps": 0.0},
+        "hopping": {"J": -0.7}, "vib_coupling": {"g": 0.9}})
+    raw, rows, csrs = _full_space_csrs(ham)
+    hop = po.HolsteinOracle(N, [2] * N, periodic=periodic, use_terms={
+        "diag": {"eps_sys": 0.0, "hbar_omega": 0.0, "delta_eps": 0.0}, "hopping": {"J": -0.7}})
+    rng = np.random.default_rng(N)
+    psi = rng.normal(size=rows.shape[0]) + 1j * rng.normal(size=rows.shape[0])
+    bonds = po.hopping_bonds(ham, rows, psi, csrs["hopping"])
+    nb = N if periodic else N - 1
+    want = np.array([np.vdot(psi, _bond_matrix(hop, raw, b, (b + 1) % N) @ psi)
+                     for b in range(nb)])
+    np.testing.assert_allclose(bonds, want, rtol=1e-12, atol=1e-12)
+    v = po.per_position_products(ham, rows, psi, csrs["hopping"])
+    assert abs(np.sum(bonds) - np.sum(v)) < 1e-9  # the bonds add up to <psi|T|psi>
+
+
+def test_per_position_coupling_and_vacuum_sum_brute_force():
+    """Per-position vib products equal <psi| P_p V |psi> from the dense matrices; the vacuum
+    amplitude sum equals a sum over the unpacked all-zero phonon tuples."""
+    ham = _tiny_ham(N=3, dim=3)
+    raw, rows, csrs = _full_space_csrs(ham)
+    vib = po.HolsteinOracle(3, [3] * 3, use_terms={
+        "diag": {"eps_sys": 0.0, "hbar_omega": 0.0, "delta_eps": 0.0}, "vib_coupling": {"g": 0.9}})
+    V = po.dense_hamiltonian(vib, raw)
+    rng = np.random.default_rng(4)
+    psi = rng.normal(size=rows.shape[0]) + 1j * rng.normal(size=rows.shape[0])
+    got = po.per_position_products(ham, rows, psi, csrs["vib_coupling"])
+    for p in range(3):
+        P = np.diag((raw[:, 0] == p).astype(float))
+        assert abs(got[p] - np.vdot(psi, P @ V @ psi)) < 1e-12
+    want = sum(psi[k] for k in range(raw.shape[0]) if not raw[k, 1:].any())
+    assert abs(po.vacuum_amplitude_sum(ham, rows, psi) - want) < 1e-14
+
+
+def test_initial_basis_generators_hand_examples():
+    """The initial-basis generators on examples worked by hand."""
+    raw = po.nonuniform_basis_tuples(3, [2, 1, 3], lowest=[1, 4, 0], minpos=1, maxpos=3)
+    want = [(p, a, 0, b) for p in (1, 2) for a in (1, 2) for b in (0, 1, 2)]
+    assert [tuple(r) for r in raw.tolist()] == want  # truncate 1 leaves the site at 0
+    # maxval 2, super_sigma huge (height 3 everywhere), sigma 1 on 3 sites:
+    # position 0: ceil(3 * [1, e^-.5, e^-2]) = [3, 2, 1]; position 1: [2, 3, 2]; 2: [1, 2, 3]
+    raw = po.moving_gaussian_tuples(3, 1.0, 1e9, 1.0, 2)
+    counts = [int(np.sum(raw[:, 0] == p)) for p in range(3)]
+    assert counts == [6, 12, 6]
+    assert raw[:, 1:].max(axis=0).tolist() == [2, 2, 2]
+    blk = raw[raw[:, 0] == 1][:, 1:]
+    assert sorted(map(tuple, blk.tolist())) == sorted(
+        (a, b, c) for a in range(2) for b in range(3) for c in range(2))
+    assert [tuple(r) for r in raw.tolist()] == sorted(map(tuple, raw.tolist()))
+
+
+def test_tensor_state_equals_kronecker_product():
+    """The tensor-product state's coefficients on its tuples equal the Kronecker product of the
+    one-site states |b> = sum_j c_j |b_j> (vacuum off the chosen sites) in the full space."""
+    N, dim = 3, 3
+    ham = po.HolsteinOracle(N, [dim] * N, use_terms={
+        "diag": {"eps_sys": 0.0, "hbar_omega": 1.0, "delta_eps": 0.0}, "hopping": {"J": 1.0}})
+    bst, cf = [0, 2], np.array([0.6, 0.8j])
+    coef, raw = po.tensor_state(N, bst, cf, excsite=1, sites=[0, 2])
+    local = np.zeros(dim, dtype=complex)
+    local[bst] = cf
+    vac = np.zeros(dim)
+    vac[0] = 1.0
+    exc = np.zeros(N)
+    exc[1] = 1.0
+    full = np.kron(exc, np.kron(local, np.kron(vac, local)))
+    allraw = po.all_states(ham)
+    index = {tuple(r): k for k, r in enumerate(allraw.tolist())}
+    got = np.zeros(allraw.shape[0], dtype=complex)
+    for c, r in zip(coef, raw.tolist()):
+        got[index[tuple(r)]] += c
+    np.testing.assert_allclose(got, full, rtol=0, atol=1e-15)
+    assert abs(np.linalg.norm(coef) - 1.0) < 1e-15
