@@ -321,6 +321,47 @@ typedef struct {
   const double* data[PACES_MAX_TERMS];
 } paces_csr_set_t;
 
+/* ---- The scaling-and-squaring propagator (expm_multiply_simple, paces/aux/_scipy/
+ * _expm_multiply.py:106-202, 395-450; onenormest, paces/aux/_scipy/_onenormest.py:11-103,
+ * 202-345). The series itself runs on paces_taylor_term with a shifted diagonal; these are the
+ * shift, the exact 1-norm and the block 1-norm estimator's vector operations. A "pair" is two
+ * real columns held as one complex128 vector (column 0 = real parts, column 1 = imaginary
+ * parts). Reductions have a fixed order: results depend only on the inputs. */
+size_t paces_pair_workspace_bytes(void);
+/* out[0] = sum of x (mu = trace(A) / n, _expm_multiply.py:150-151) */
+int paces_sum_f64(const double* x, int64_t n, double* out, void* ws, size_t ws_bytes,
+                  void* stream);
+/* out_i = x_i - m (the diagonal of A - mu I, _expm_multiply.py:152) */
+int paces_shift_f64(const double* x, int64_t n, double m, double* out, void* stream);
+/* out[0] = max_i |diag_i - m| + sum over terms of |row i's values|: the exact 1-norm of the
+ * symmetric H - m I (_exact_1_norm, _expm_multiply.py:26-33) */
+int paces_csr_shifted_onenorm(int64_t n, const double* diag, double m, const paces_csr_set_t* h,
+                              double* out, void* ws, size_t ws_bytes, void* stream);
+/* out[0..1] = sum |column 0|, sum |column 1| (_sum_abs_axis0, _onenormest.py:149-158) */
+int paces_pair_abs_sums(const double* y, int64_t n, double* out, void* ws, size_t ws_bytes,
+                        void* stream);
+/* out[0..3] = a0.b0, a0.b1, a1.b0, a1.b1 (vectors_are_parallel, _onenormest.py:167-174) */
+int paces_pair_dots(const double* a, const double* b, int64_t n, double* out, void* ws,
+                    size_t ws_bytes, void* stream);
+/* s = sign(y) per column, sign(0) = 1 (sign_round_up, _onenormest.py:128-142, real case) */
+int paces_pair_signs(const double* y, int64_t n, double* s, void* stream);
+/* h_i = max(|z_i0|, |z_i1|) (_max_abs_axis1, _onenormest.py:145-146) */
+int paces_pair_rowmax(const double* z, int64_t n, double* h, void* stream);
+/* *out = the index of the first entry of h after (after_value, after_index) in the order of
+ * decreasing value, then decreasing index (all entries when after_index < 0), or -1: repeated,
+ * it lists h as argsort(h)[::-1] does (_onenormest.py:312) */
+int paces_argmax_after(const double* h, int64_t n, double after_value, int64_t after_index,
+                       int64_t* out, void* ws, size_t ws_bytes, void* stream);
+/* column col of x = +-scale from a hash of (seed, i) (resample_column, _onenormest.py:198-199;
+ * the reference draws from CuPy's generator) */
+int paces_pair_random_column(double* x, int64_t n, int32_t col, uint64_t seed, double scale,
+                             void* stream);
+/* x = (re, im) everywhere, then x[i0] column 0 and x[i1] column 1 = 1 when i0 / i1 >= 0
+ * (starting matrix and unit vectors, _onenormest.py:259-280, 324-325) */
+int paces_pair_fill(double* x, int64_t n, double re, double im, int64_t i0, int64_t i1,
+                    void* stream);
+
+
 int paces_spmv(int64_t R, const double* diag, const paces_csr_set_t* H, const double* x,
                double* y, void* stream);
 

@@ -8,7 +8,7 @@ the loaded HIP library are shared whichever name a caller uses:
     from paces.config import devices                  # paces_amd.config
     from paces.models import holstein                 # paces_amd.models.holstein
     from paces.core.time_evolution import ExpmParams  # paces_amd.core.time_evolution
-    from paces.aux import expm_multiply_simple        # paces_amd.aux.expm
+    from paces.aux import expm_multiply_simple        # paces_amd.aux.expm_multiply_simple
 """
 
 import importlib
@@ -33,6 +33,7 @@ _MODULES = {
     "aux": "paces_amd.aux",
     "aux.helpers": "paces_amd.aux.helpers",
     "aux.cupy_search": "paces_amd.aux.search",
+    "aux.expm": "paces_amd.aux.expm",
 }
 
 for _name, _target in _MODULES.items():

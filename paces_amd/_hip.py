@@ -198,6 +198,17 @@ _SIGNATURES = {
                                           _I32, _D, _P, _P, _SZ, _P]),
     "paces_batch_segments": (C.c_int, [_P, _I64, _I32, _I32, _P, _P]),
     "paces_segment_vdot": (C.c_int, [_P, _P, _P, _I32, _P, _P]),
+    "paces_pair_workspace_bytes": (_SZ, []),
+    "paces_sum_f64": (C.c_int, [_P, _I64, _P, _P, _SZ, _P]),
+    "paces_shift_f64": (C.c_int, [_P, _I64, _D, _P, _P]),
+    "paces_csr_shifted_onenorm": (C.c_int, [_I64, _P, _D, C.POINTER(CsrSet), _P, _P, _SZ, _P]),
+    "paces_pair_abs_sums": (C.c_int, [_P, _I64, _P, _P, _SZ, _P]),
+    "paces_pair_dots": (C.c_int, [_P, _P, _I64, _P, _P, _SZ, _P]),
+    "paces_pair_signs": (C.c_int, [_P, _I64, _P, _P]),
+    "paces_pair_rowmax": (C.c_int, [_P, _I64, _P, _P]),
+    "paces_argmax_after": (C.c_int, [_P, _I64, _D, _I64, _P, _P, _SZ, _P]),
+    "paces_pair_random_column": (C.c_int, [_P, _I64, _I32, C.c_uint64, _D, _P]),
+    "paces_pair_fill": (C.c_int, [_P, _I64, _D, _D, _I64, _I64, _P]),
     "paces_vacuum_sum": (C.c_int, [_P, _I32, _I32, _P, _P, _I32, _P, _P]),
     "paces_batch_select_workspace_bytes": (_SZ, [_I64, _I32]),
     "paces_batch_select_topk": (C.c_int, [_P, _P, _I64, _I32, _P, _I32, _P, _P, _P, _P, _SZ, _P]),

@@ -295,10 +295,15 @@ def _tuned_shape(tuner, kind, R, nterms, run):
 
 
 def simple_taylor(diag, mats, v, delta_t, m_star=100, tol=TAYLOR_TOL, chunk=6, ws=None,
-                  tuner=None):
+                  tuner=None, coeff=None):
     """res = sum_j (-i dt H)^j v / j! until (|t_j| + |t_{j+1}|)_inf / |res|_inf <= tol
     (paces/core/time_evolution.py:482-512). Returns (res, [converged, j_ind, c1+c2, ratio]).
-    With a tuner, the term kernel's launch shape is the tuned one (identical results)."""
+    With a tuner, the term kernel's launch shape is the tuned one (identical results).
+    coeff(j), when given, is term j's factor c in t_j = -i c H t_(j-1) (default: the reference's
+    dt / (j + 1))."""
+    if coeff is None:
+        def coeff(j):
+            return taylor_coeff(delta_t, j)
     R = v.shape[0]
     res = torch.empty_like(v)
     bufs = (torch.empty_like(v), torch.empty_like(v))
@@ -333,7 +338,7 @@ def simple_taylor(diag, mats, v, delta_t, m_star=100, tol=TAYLOR_TOL, chunk=6, w
                                               ptr(torch.view_as_real(x_in)),
                                               ptr(torch.view_as_real(x_out)),
                                               ptr(torch.view_as_real(res)),
-                                              taylor_coeff(delta_t, j), j, tol, ptr(state), wsp,
+                                              coeff(j), j, tol, ptr(state), wsp,
                                               wsn, C.byref(cfg) if cfg else None, st),
                   "taylor_term")
             x_in = x_out

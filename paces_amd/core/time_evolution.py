@@ -101,9 +101,6 @@ class TimeEvolutionFramework:
         if expm_params.lanczos_order and expm_params.use_scaling:
             raise ValueError("ExpmParams: lanczos_order > 0 and use_scaling are exclusive "
                              "propagators; choose one.")
-        if expm_params.use_scaling:
-            raise NotImplementedError("The scaling-and-squaring expm_multiply branch is not part "
-                                      "of this engine; use the Taylor series or Lanczos.")
         if te_params.use_two_streams:
             raise NotImplementedError("use_two_streams (the reference's concurrent basis on a "
                                       "second device) is not part of this one-GPU engine.")
@@ -258,6 +255,14 @@ class TimeEvolutionFramework:
     def create_matrices(self, diag_params, mats_dict):
         self.sparse_mats_dict = mats_dict
         self.diag_vals = diag_params
+        if self.expm_params.use_scaling:
+            self.create_ham_mat()
+
+    def create_ham_mat(self):
+        """The total Hamiltonian as one operator (time_evolution.py:375-379): the diagonal and
+        the term CSRs, applied together."""
+        from ..aux.expm import HamiltonianOperator
+        self.ham_mat = HamiltonianOperator(self.diag_vals, self._mats(), scale=1.0)
 
     def _mats(self):
         return [self.sparse_mats_dict[t] for t in self.hamobj.offdiag_terms]
@@ -299,6 +304,17 @@ class TimeEvolutionFramework:
             raise ValueError(f"Unknown precision {ep.precision!r} (use 'fp64' or 'mixed').")
         if self.debug_verb > EXPM_EVO_LEVEL:
             print(f"         Taylor series: converged={dbg[0]} after j={dbg[1]}.")
+        return res, dbg
+
+    def _scaling_and_squaring(self, delta_t):
+        """exp(-i dt H) psi by the shifted scaling-and-squaring Taylor method (the reference's
+        use_scaling branch, time_evolution.py:580-581, with the time direction of its other
+        propagators: it passes 1j * H, which evolves backwards)."""
+        from ..aux.expm import HamiltonianOperator, expm_multiply_simple
+        op = HamiltonianOperator(self.diag_vals, self._mats(), scale=-1j)
+        res, dbg = expm_multiply_simple(op, self.vector, t=delta_t, return_dbg=True,
+                                        seed=self._step_index + 1)
+        self.last_expm_choice = op.last_choice
         return res, dbg
 
     def _tuner(self):
@@ -349,6 +365,8 @@ class TimeEvolutionFramework:
         dbg_dict, post = self._generate_new_hilbert_space(self.te_params.enlarge_steps)
         if self.expm_params.lanczos_order > 0:
             self.vector, expm_dbg = self._lanczos(delta_t)
+        elif self.expm_params.use_scaling:
+            self.vector, expm_dbg = self._scaling_and_squaring(delta_t)
         else:
             self.vector, expm_dbg = self._simple_taylor(delta_t)
         norm = self._norm_after_propagation()
