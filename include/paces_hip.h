@@ -28,6 +28,7 @@ extern "C" {
 #define PACES_E_UNSUPPORTED -2 /* row width or term count beyond the compiled limits */
 #define PACES_E_HIP -3         /* a HIP launch or runtime call failed */
 #define PACES_E_WORKSPACE -4   /* workspace smaller than *_workspace_bytes() */
+#define PACES_E_SCRATCH -5     /* size beyond the fixed scratch of a no-allocation entry point */
 
 #define PACES_MAX_ROW_WORDS 16
 #define PACES_MAX_RUNS 32
@@ -310,6 +311,19 @@ size_t paces_reassign_workspace_bytes(int64_t R);
 int paces_reassign(const uint64_t* old_rows, int64_t M, const double* old_psi,
                    const uint64_t* new_rows, int64_t R, int32_t W, double* new_psi, void* ws,
                    size_t ws_bytes, void* stream);
+/* No counterpart in the reference: paces_reassign for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It takes
+ * no workspace: its per-tile bounds live in a static device array of
+ * PACES_REASSIGN_NOALLOC_SCRATCH_BYTES (16 bytes per 256 new rows, so R up to
+ * PACES_REASSIGN_NOALLOC_MAX_ROWS), held by each GPU the library runs on until the process ends.
+ * R beyond that returns PACES_E_SCRATCH. It calls no hipMalloc or hipFree. Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (different streams, or host threads)
+ * race and must be serialised by the caller. Results equal paces_reassign's bit for bit. */
+#define PACES_REASSIGN_NOALLOC_SCRATCH_BYTES (4u << 20)
+#define PACES_REASSIGN_NOALLOC_MAX_ROWS (((int64_t)PACES_REASSIGN_NOALLOC_SCRATCH_BYTES / 16 - 1) * 256)
+int paces_reassign_noalloc(const uint64_t* old_rows, int64_t M, const double* old_psi,
+                           const uint64_t* new_rows, int64_t R, int32_t W, double* new_psi,
+                           void* stream);
 
 /* ---- E1/E2: H.x and the Taylor series (total_ham / _simple_taylor,
  * paces/core/time_evolution.py:382-400, 482-512). FP64 parity order: per row

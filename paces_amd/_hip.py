@@ -187,6 +187,7 @@ _SIGNATURES = {
                                              C.POINTER(Holstein), _P, _P, _SZ, _P]),
     "paces_reassign_workspace_bytes": (_SZ, [_I64]),
     "paces_reassign": (C.c_int, [_P, _I64, _P, _P, _I64, _I32, _P, _P, _SZ, _P]),
+    "paces_reassign_noalloc": (C.c_int, [_P, _I64, _P, _P, _I64, _I32, _P, _P]),
     "paces_spmv": (C.c_int, [_I64, _P, C.POINTER(CsrSet), _P, _P, _P]),
     "paces_taylor_workspace_bytes": (_SZ, [_I64]),
     "paces_taylor_begin": (C.c_int, [_I64, _P, _P, _P, _P, _SZ, _P]),

@@ -1,5 +1,7 @@
 // Packed-row primitives: pack / field read / field add (A2-A5), lexicographic lower_bound (B1),
 // union of sorted unique runs (A7) and amplitude reassignment (G1).
+#include <mutex>
+
 #include "common.hpp"
 
 namespace paces {
@@ -753,6 +755,29 @@ static int run_compact(const uint8_t* keep, int64_t n, const uint64_t* rows, con
   return launch_status();
 }
 
+// ---- paces_reassign without allocation: the tile bounds in a static device array. Host code may
+// not use the array's name as an address (it names a host-side stand-in): each device's address
+// comes from hipGetSymbolAddress, fetched once per device with that device current.
+static __device__ __align__(256) unsigned char scratch[PACES_REASSIGN_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static void* scratch_ptr[kMaxDevices];
+static std::mutex scratch_mu;
+
+static int scratch_for_current_device(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return PACES_E_HIP;
+  if (dev < 0 || dev >= kMaxDevices) return PACES_E_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return PACES_E_HIP;
+    scratch_ptr[dev] = p;
+  }
+  *out = scratch_ptr[dev];
+  return PACES_OK;
+}
+
 }  // namespace paces
 
 using namespace paces;
@@ -864,6 +889,23 @@ int paces_reassign(const uint64_t* old_rows, int64_t M, const double* old_psi,
   if (R > 0 && (!new_rows || !new_psi || !ws || (M > 0 && (!old_rows || !old_psi))))
     return PACES_E_INVALID;
   if (ws_bytes < lb_ws_bytes(R)) return PACES_E_WORKSPACE;
+  PACES_DISPATCH_W(W, run_reassign, old_rows, M, old_psi, new_rows, R, new_psi, ws,
+                   (hipStream_t)stream);
+}
+
+int paces_reassign_noalloc(const uint64_t* old_rows, int64_t M, const double* old_psi,
+                           const uint64_t* new_rows, int64_t R, int32_t W, double* new_psi,
+                           void* stream) {
+  if (M < 0 || R < 0 || W < 1) return PACES_E_INVALID;
+  if (R > PACES_REASSIGN_NOALLOC_MAX_ROWS) return PACES_E_SCRATCH;
+  if (R > 0 && (!new_rows || !new_psi || (M > 0 && (!old_rows || !old_psi))))
+    return PACES_E_INVALID;
+  if (R == 0) return PACES_OK;
+  if (lb_ws_bytes(R) > PACES_REASSIGN_NOALLOC_SCRATCH_BYTES) return PACES_E_SCRATCH;
+  void* ws = nullptr;
+  const int rc = scratch_for_current_device(&ws);
+  if (rc != PACES_OK) return rc;
+  // the tile bounds are written in full before they are read: nothing to clear
   PACES_DISPATCH_W(W, run_reassign, old_rows, M, old_psi, new_rows, R, new_psi, ws,
                    (hipStream_t)stream);
 }

@@ -679,6 +679,7 @@ const char* paces_strerror(int code) {
     case PACES_E_UNSUPPORTED: return "unsupported size (row width, term or run count)";
     case PACES_E_HIP: return "HIP launch or runtime error";
     case PACES_E_WORKSPACE: return "workspace too small";
+    case PACES_E_SCRATCH: return "size beyond the fixed scratch of a no-allocation entry point";
     default: return "unknown paces status";
   }
 }
