@@ -470,7 +470,8 @@ constexpr int kObsTileB = 256;  // the smallest tile (sizes the workspace)
 
 template <int W>
 constexpr int obs_tile() {
-  return W <= 6 ? 512 : kObsTileB;
+  // more rows per lane spread each site's wave reduction over more rows
+  return W <= 3 ? 1024 : (W <= 6 ? 512 : kObsTileB);
 }
 
 // Partial sums of one tile (rows of one trajectory): the sum of weights, the weight at each
@@ -485,6 +486,7 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
                                                          const int64_t* __restrict__ boff,
                                                          int ntraj, const int32_t* __restrict__ off,
                                                          const int32_t* __restrict__ width, int N,
+                                                         const int32_t* __restrict__ traj_L, int geo,
                                                          double* __restrict__ part) {
   constexpr int TB = obs_tile<W>(), RPL = TB / 64, PER = TB * W / kBlock;
   __shared__ u64 srow[TB * W];
@@ -497,31 +499,38 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
   }
   const int S = 2 * N + 1;
   const int nr = (int)(T.r1 - T.r0);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double w[RPL];
   {
+    // the rows (to LDS) and this lane's amplitudes are loaded together, before the barrier
     u64 tmp[PER];
+    double2 v[RPL];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
       const int e = (int)threadIdx.x + j * kBlock;
       tmp[j] = e < nr * W ? rows[T.r0 * W + e] : 0ull;
     }
 #pragma unroll
+    for (int j = 0; j < RPL; ++j) {
+      const int r = lane + 64 * j;
+      v[j] = psi[T.r0 + (r < nr ? r : 0)];
+    }
+#pragma unroll
     for (int j = 0; j < PER; ++j) {
       const int e = (int)threadIdx.x + j * kBlock;
       if (e < nr * W) srow[e] = tmp[j];
     }
+#pragma unroll
+    for (int j = 0; j < RPL; ++j)
+      w[j] = (lane + 64 * j < nr) ? v[j].x * v[j].x + v[j].y * v[j].y : 0.0;
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double w[RPL];
   int pos[RPL];
 #pragma unroll
   for (int j = 0; j < RPL; ++j) {
     const int r = lane + 64 * j;
-    w[j] = 0.0;
     pos[j] = -1;
     if (r < nr) {
-      const double2 v = psi[T.r0 + r];
-      w[j] = v.x * v.x + v.y * v.y;
       Row<W> rr;
 #pragma unroll
       for (int k = 0; k < W; ++k) rr.w[k] = srow[r * W + k];
@@ -551,7 +560,17 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
   }
   __syncthreads();
   const int pmin = spr[0], pmax = spr[1];
+  // sites past the tile's own lattice hold no phonons and no exciton: their partials are zero
+  const int L = traj_L[T.t];
+  const int Nt = geo == 1 ? L : (geo == 2 ? L * L : L * L * L);
   for (int s = wave; s < N; s += kBlock / 64) {
+    if (s >= Nt) {
+      if (lane == 0) {
+        part[(int64_t)blockIdx.x * S + s] = 0.0;
+        part[(int64_t)blockIdx.x * S + N + s] = 0.0;
+      }
+      continue;
+    }
     const int fo = off[s + 2], b = width[s + 2];
     const int k = fo >> 6, o = fo & 63;
     double ap = 0.0;
@@ -612,8 +631,8 @@ size_t bobs_ws_bytes(int64_t R, int ntraj, int N) {
 
 template <int W>
 static int run_bobs(const uint64_t* rows, const double* psi, int64_t R, const int32_t* seg_off,
-                    int ntraj, const paces_layout_t* lay, int N, double* out, void* ws,
-                    hipStream_t st) {
+                    int ntraj, const paces_layout_t* lay, int N, const int32_t* traj_L, int geo,
+                    double* out, void* ws, hipStream_t st) {
   char* p = (char*)ws;
   int64_t* boff = (int64_t*)p;
   p += a256((size_t)(ntraj + 1) * 8);
@@ -623,7 +642,7 @@ static int run_bobs(const uint64_t* rows, const double* psi, int64_t R, const in
   hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(1), 0, st, seg_off, ntraj, TB, boff);
   hipLaunchKernelGGL(k_bobs_partial<W>, dim3(seg_grid(R, ntraj, TB)), dim3(kBlock), 0, st,
                      (const u64*)rows, (const double2*)psi, seg_off, (const int64_t*)boff, ntraj,
-                     lay->d_off, lay->d_width, N, part);
+                     lay->d_off, lay->d_width, N, traj_L, geo, part);
   hipLaunchKernelGGL(k_bobs_final, dim3(ntraj * S), dim3(64), 0, st, (const double*)part,
                      (const int64_t*)boff, S, out);
   return launch_status();
@@ -1188,9 +1207,10 @@ int paces_holstein_observables_batched(const uint64_t* rows, const double* psi, 
   if (!lay || !h || !h->batched || !seg_off || !out || !ws || R < 0) return PACES_E_INVALID;
   if (R > 0 && (!rows || !psi)) return PACES_E_INVALID;
   if (lay->nfields != h->nchain + 2) return PACES_E_INVALID;
+  if (!h->d_traj_L || h->geo_dim < 1 || h->geo_dim > 3) return PACES_E_INVALID;
   if (ws_bytes < bobs_ws_bytes(R, h->ntraj, h->nchain)) return PACES_E_WORKSPACE;
-  PACES_DISPATCH_W(lay->W, run_bobs, rows, psi, R, seg_off, h->ntraj, lay, h->nchain, out, ws,
-                   (hipStream_t)stream);
+  PACES_DISPATCH_W(lay->W, run_bobs, rows, psi, R, seg_off, h->ntraj, lay, h->nchain,
+                   h->d_traj_L, h->geo_dim, out, ws, (hipStream_t)stream);
 }
 
 size_t paces_batch_taylor_workspace_bytes(int64_t R, int32_t ntraj) {

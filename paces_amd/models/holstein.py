@@ -301,13 +301,34 @@ class Hamiltonian(HamiltonianFramework):
                                                         ptr(in_basis), ptr(src), C.byref(pr),
                                                         C.byref(h), ptr(nnz), wsp, wsn,
                                                         stream()), "holstein_pairs_count_split")
-        nimg = run_off[len(run_off) - 1] + (pr.run_offsets[len(run_off) - 1] if pr else 0)
+        # a term's entries come from its own ops' images, two per image (the pair and its
+        # transpose): each term's arrays are bounded by its runs' images, earlier part included
+        caps = []
+        runs = self._term_runs()
+        if runs[-1][1] != len(run_off) - 1:
+            raise RuntimeError("image runs do not match the terms' ops")
+        for b, e in runs:
+            nimg = run_off[e] - run_off[b]
+            if pr:
+                nimg += pr.run_offsets[e] - pr.run_offsets[b]
+            caps.append(max(2 * nimg, 1))
         nnz_host, ev = _to_host_async(nnz)
-        cap = max(2 * nimg, 1)
-        if cap * 12 * h.nterms > _BOUND_BYTES:
+        if sum(caps) * 12 > _BOUND_BYTES:
             ev.synchronize()
-            cap = max(max(nnz_host.tolist()), 1)
-        return (nnz_host, ev), cap, wsp, wsn
+            caps = [max(int(c), 1) for c in nnz_host.tolist()[:h.nterms]]
+        return (nnz_host, ev), caps, wsp, wsn
+
+    def _term_runs(self):
+        """(first run, one past the last run) of each off-diagonal term, in term order: the
+        image runs are the ops of the terms in turn, a hopping term's two directions per
+        lattice dimension (and the wrap-around ones when periodic), a vib term's two."""
+        out, k = [], 0
+        for name in self.offdiag_terms:
+            n = 2 * self.geo_dim * (2 if self.periodic else 1) if name in ("hopping",
+                                                                           "hopping2") else 2
+            out.append((k, k + n))
+            k += n
+        return out
 
     def _pairs_csr(self, R, nbasis, offs, cand_pos, in_basis, src, nup, basis=None,
                    value_dtype=torch.float64, rows=None, prior=None):
@@ -323,14 +344,14 @@ class Hamiltonian(HamiltonianFramework):
         h = self._hs
         run_off = (C.c_int64 * len(offs))(*offs)
         pr = self._prior_arg(prior)
-        (nnz_host, ev), cap, wsp, wsn = self._pairs_count(R, nbasis, run_off, cand_pos, in_basis,
-                                                          src, pr)
+        (nnz_host, ev), caps, wsp, wsn = self._pairs_count(R, nbasis, run_off, cand_pos, in_basis,
+                                                           src, pr)
         parts = []
         arrs = [(C.c_void_p * _hip.MAX_TERMS)() for _ in range(3)]
         for t, name in enumerate(self.offdiag_terms):
             ip = torch.empty(R + 1, dtype=torch.int32, device=self.device)
-            ix = torch.empty(cap, dtype=torch.int32, device=self.device)
-            dv = torch.empty(cap, dtype=value_dtype, device=self.device)
+            ix = torch.empty(caps[t], dtype=torch.int32, device=self.device)
+            dv = torch.empty(caps[t], dtype=value_dtype, device=self.device)
             arrs[0][t], arrs[1][t], arrs[2][t] = ip.data_ptr(), ix.data_ptr(), dv.data_ptr()
             parts.append((name, ip, ix, dv))
         if pr is not None:
@@ -358,16 +379,16 @@ class Hamiltonian(HamiltonianFramework):
             raise NotImplementedError("Coded CSR values are for batched trajectories.")
         run_off = (C.c_int64 * len(offs))(*offs)
         pr = self._prior_arg(prior)
-        (nnz_host, ev), cap, wsp, wsn = self._pairs_count(R, nbasis, run_off, cand_pos, in_basis,
-                                                          src, pr)
+        (nnz_host, ev), caps, wsp, wsn = self._pairs_count(R, nbasis, run_off, cand_pos, in_basis,
+                                                           src, pr)
         parts = []
         arrs = [(C.c_void_p * _hip.MAX_TERMS)() for _ in range(3)]
         for t, name in enumerate(self.offdiag_terms):
             ip = torch.empty(R + 1, dtype=torch.int32, device=self.device)
-            ix = torch.empty(cap, dtype=torch.int32, device=self.device)
+            ix = torch.empty(caps[t], dtype=torch.int32, device=self.device)
             cd = None
             if name == "vib_coupling":
-                cd = torch.empty(cap, dtype=torch.uint8, device=self.device)
+                cd = torch.empty(caps[t], dtype=torch.uint8, device=self.device)
                 arrs[2][t] = cd.data_ptr()
             arrs[0][t], arrs[1][t] = ip.data_ptr(), ix.data_ptr()
             parts.append((name, ip, ix, cd))
