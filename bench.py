@@ -506,52 +506,59 @@ def measure_scorer(scorer, X, reps=10):
     return e0.elapsed_time(e1) / reps * 1e-3
 
 
-def run_learned(a):
-    """The learned form (scorer pruning, complex64 Taylor terms) against the plain FP64 form on
-    the same trajectory, after logging labels from plain runs and training the scorer on them."""
+def learned_leg(ms, labels, log_steps, epochs, warmup, steps, precision="mixed",
+                verbose=True):
+    """The learned form (scorer pruning before the second expansion, complex64 Taylor terms by
+    default) against the plain FP64 form on the same trajectory, after logging labels from plain
+    runs at three couplings and training the scorer on them; both timed over the same steps.
+    Returns a dict of the measurements (the plain FP64 form equals the FP64 oracle bit for bit,
+    so its densities are the reference of the error)."""
     import torch
     from paces_amd.core.scorer import CandidateScorer
     from paces_amd.models import holstein
-    ms = a.learned_ms
     gs = learned_draws()
     dev = torch.device("cuda", torch.cuda.current_device())
-    log = holstein.LabelLog(dev, a.labels)
+    log = holstein.LabelLog(dev, labels)
     t0 = time.perf_counter()
+    t_arr = 0.05 * np.arange(max(log_steps, warmup + steps) + 1)
     for g in gs[:3]:
         te = chain_te(holstein.LearnedTimeEvolution, g, ms, label_log=log)
-        for i in range(1, a.log_steps + 1):
-            te._evolve_single_step(i, 0.05 * i, 0.05)
+        for i in range(1, log_steps + 1):
+            te._evolve_single_step(i, t_arr[i], t_arr[i] - t_arr[i - 1])
             if log.full:
                 break
-        print(f"[learned] logged g={g:.3f}: {log.n} labels", file=sys.stderr, flush=True)
+        if verbose:
+            print(f"[learned] logged g={g:.3f}: {log.n} labels", file=sys.stderr, flush=True)
         del te
     torch.cuda.synchronize()
     log_s = time.perf_counter() - t0
     X, y = log.data()
     scorer = CandidateScorer(dev, seed=1)
     t0 = time.perf_counter()
-    hist = scorer.train(X, y, epochs=a.epochs, batch=16384, lr=1e-3)
+    hist = scorer.train(X, y, epochs=epochs, batch=16384, lr=1e-3)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
-    print(f"[learned] trained on {log.n} labels in {train_s:.1f} s, loss {hist}",
-          file=sys.stderr, flush=True)
+    if verbose:
+        print(f"[learned] trained on {log.n} labels in {train_s:.1f} s, loss {hist}",
+              file=sys.stderr, flush=True)
+    nlabels = log.n
     del X, y, log
     torch.cuda.empty_cache()
     g = gs[3]
     plain = chain_te(holstein.TimeEvolution, g, ms)
-    learned = chain_te(holstein.LearnedTimeEvolution, g, ms, precision=a.learned_precision,
+    learned = chain_te(holstein.LearnedTimeEvolution, g, ms, precision=precision,
                        scorer=scorer)
     tp = tl = 0.0
     up = ul = 0
     err = 0.0
-    n_total = a.warmup + a.steps
-    for i in range(1, n_total + 1):
-        timed = i > a.warmup
+    rows_ratio = []
+    for i in range(1, warmup + steps + 1):
+        timed = i > warmup
         for name, te in (("plain", plain), ("learned", learned)):
             prev = te.numstates
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            te._evolve_single_step(i, 0.05 * i, 0.05)
+            te._evolve_single_step(i, t_arr[i], t_arr[i] - t_arr[i - 1])
             torch.cuda.synchronize()
             dt_ = time.perf_counter() - t0
             if timed:
@@ -565,38 +572,50 @@ def run_learned(a):
                                 - plain.obsobj.last_results["n_exc"])))
         err = max(err, e)
         s = learned.stats[-1]
-        print(f"[learned] step {i}: rows {s['rows_full']} kept {s['rows_kept']} fallback "
-              f"{s['fallback']} density err {e:.2e}", file=sys.stderr, flush=True)
-    st = learned.stats[a.warmup + 1:]
-    # scorer roofline on the last union's candidate features
-    ham = learned.hamobj
-    R = learned.stats[-1]["rows_full"]
-    Xb = torch.zeros((R, 64), dtype=torch.int16, device=dev)
+        if timed:
+            rows_ratio.append(s["rows"] / plain.numstates)
+        if verbose:
+            print(f"[learned] step {i}: first expansion {s['rows_first']}, scored {s['scored']}, "
+                  f"kept {s['kept']}, rows {s['rows']} (plain {plain.numstates}), fallback "
+                  f"{s['fallback']}, density err {e:.2e}", file=sys.stderr, flush=True)
+    st = learned.stats[warmup:]
+    # the scorer's roofline on as many rows as the last step scored
+    nscore = max(int(learned.stats[-1]["scored"]), 1)
+    Xb = torch.zeros((nscore, 64), dtype=torch.int16, device=dev)
     sec = measure_scorer(scorer, Xb)
-    achieved = MLP_FLOPS_PER_ROW * R / sec / 1e12
-    value = ul / tl
-    line = {"metric": METRIC, "value": value, "unit": "state-steps/s", "n_gpus": 1,
-            "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * tl / a.steps,
+    achieved = MLP_FLOPS_PER_ROW * nscore / sec / 1e12
+    return {"value": ul / tl, "ms_per_step": 1e3 * tl / steps,
+            "plain_gpu_state_steps_per_s": up / tp, "plain_ms_per_step": 1e3 * tp / steps,
+            "speedup_vs_plain": (ul / tl) / (up / tp), "density_err_vs_plain_fp64": err,
+            "mean_kept_fraction_of_scored": float(np.mean([x["kept"] / max(x["scored"], 1)
+                                                           for x in st])),
+            "mean_rows_vs_plain": float(np.mean(rows_ratio)),
+            "fallbacks": int(sum(x["fallback"] for x in st)), "labels": int(nlabels),
+            "label_log_s": log_s, "train_s": train_s, "train_epochs": epochs,
+            "precision": precision,
+            "scorer_roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": 2500.0,
+                                "unit": "TFLOP/s", "frac": round(achieved / 2500.0, 4),
+                                "traffic": None,
+                                "kernel": "k_mlp_infer_s (paces_mlp_infer, bf16 MFMA, 256-row "
+                                          "tiles)" if scorer.tile_rows in (0, 256) else
+                                          f"paces_mlp_infer, {scorer.tile_rows}-row tiles",
+                                "avg_us": round(sec * 1e6, 1), "rows": int(nscore)}}
+
+
+def run_learned(a):
+    """The learned form's own line (config-4 physics): value = its state-steps/s."""
+    r = learned_leg(a.learned_ms, a.labels, a.log_steps, a.epochs, a.warmup, a.steps,
+                    a.learned_precision)
+    roof = r.pop("scorer_roofline")
+    line = {"metric": METRIC, "value": r.pop("value"), "unit": "state-steps/s", "n_gpus": 1,
+            "steps": a.steps, "warmup": a.warmup, "ms_per_step": r.pop("ms_per_step"),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "c64 terms / f64 state + bf16 scorer" if a.learned_precision == "mixed"
             else "f64 + bf16 scorer",
             "data": "synthetic",
-            "config": {"workload": LEARNED_DESC.format(ms=ms), "form": "learned-"
-                       + a.learned_precision,
-                       "plain_gpu_state_steps_per_s": up / tp,
-                       "plain_ms_per_step": 1e3 * tp / a.steps,
-                       "speedup_vs_plain": value / (up / tp),
-                       "density_err_vs_plain_fp64": err,
-                       "mean_kept_fraction": float(np.mean([x["rows_kept"] / x["rows_full"]
-                                                            for x in st])),
-                       "fallbacks": int(sum(x["fallback"] for x in st)),
-                       "labels": int(a.labels), "label_log_s": log_s, "train_s": train_s,
-                       "train_epochs": a.epochs},
-            "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": 2500.0,
-                         "unit": "TFLOP/s", "frac": round(achieved / 2500.0, 4), "traffic": None,
-                         "kernel": "k_mlp_infer (paces_mlp_infer, bf16 MFMA)",
-                         "avg_us": round(sec * 1e6, 1), "rows": int(R)},
-            "cpu_baseline": None}
+            "config": dict({"workload": LEARNED_DESC.format(ms=a.learned_ms),
+                            "form": "learned-" + a.learned_precision}, **r),
+            "roofline": roof, "cpu_baseline": None}
     print(json.dumps(line), flush=True)
     return 0
 

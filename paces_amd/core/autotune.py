@@ -24,7 +24,7 @@ from .. import _hip
 
 SHAPES = [(b, r) for b in (128, 256, 512, 1024) for r in (1, 2, 4)]
 DEFAULT_SHAPE = (256, 1)
-MLP_TILES = (64, 128)
+MLP_TILES = (64, 128, 256)
 
 
 def bucket(n):

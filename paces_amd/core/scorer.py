@@ -65,7 +65,7 @@ class CandidateScorer:
         self.grads = torch.zeros(NPARAM, dtype=torch.float32, device=device)
         self.loss = torch.zeros(1, dtype=torch.float32, device=device)
         self.step = 0
-        self.tile_rows = 0  # 0: the library default; the autotuner may set 64 or 128
+        self.tile_rows = 0  # 0: the library default; the autotuner may set 64, 128 or 256
         self._ws = _hip.Workspace(device)
         self._lib = _hip.load()
         self.refresh_bf16()
@@ -106,7 +106,7 @@ class CandidateScorer:
               "mlp_adam")
 
     def infer(self, X, want_logit=False, tile_rows=None):
-        """Retain probabilities (or logits) of the rows of X. tile_rows (64 or 128) overrides
+        """Retain probabilities (or logits) of the rows of X. tile_rows (64, 128 or 256) overrides
         the workgroup tile; by default the autotuned choice, else the library default."""
         n = int(X.shape[0])
         out = torch.empty(max(n, 1), dtype=torch.float32, device=self.device)

@@ -744,8 +744,9 @@ __device__ __forceinline__ void image_ends(int64_t e, const PriorImgs& A, const 
     const int32_t ca = A.cpos[A.nprev + ea];
     s = A.cpos[A.src[ea]];
     nu = A.nup[ea];
-    p = cand_pos[ca];
-    q = cand_pos[s];
+    // an image pruned from the expanded basis has position -1 (and so has no pair)
+    p = ca >= 0 ? cand_pos[ca] : -1;
+    q = s >= 0 ? cand_pos[s] : -1;
   } else {
     const int64_t e2 = e - A.n;
     k = img_run_of(IR, e2);
@@ -1055,7 +1056,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs_place(int64_t R, int64_t nbasi
   image_ends(e, A, IR, nbasis, cand_pos, src, nup, k, p, q, s, nu);
   const int t = T.term[k], S = SL.S[t];
   const int32_t pc = p < 0 ? 0 : p, qc = q < 0 ? 0 : q;
-  const u64 w0 = (M.batched && !out.coded) ? basis[(int64_t)s * W] : 0;
+  const u64 w0 = (M.batched && !out.coded) ? basis[(int64_t)(s < 0 ? 0 : s) * W] : 0;
   const bool pin = inB[pc] != 0;
   const uint8_t* f = flags + R * SL.foff[t];
   uint8_t fp[8], fq[8];

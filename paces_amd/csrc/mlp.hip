@@ -15,6 +15,8 @@
 //   k_colsum / k_sum_parts   deterministic bias-gradient and split-K reductions
 //   k_adam          Adam update of the FP32 master and refresh of the BF16 copies
 //   k_mlp_infer<RB> all layers fused for a tile of 64 RB rows, activations kept in LDS
+//   k_mlp_infer_w<RB> the same with 64 x 128 wave tiles (two waves per 64 rows)
+//   k_mlp_infer_s   256 rows, 64 x 128 wave tiles, weight chunks staged once per workgroup in LDS
 
 #include "common.hpp"
 
@@ -40,27 +42,31 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// erf by Abramowitz & Stegun, Handbook of Mathematical Functions, formula 7.1.26:
-// erf(x) = 1 - (a1 t + a2 t^2 + a3 t^3 + a4 t^4 + a5 t^5) exp(-x^2), t = 1 / (1 + p x), x >= 0,
-// |error| <= 1.5e-7 -- far below the bf16 resolution of the activations it feeds.
-__device__ __forceinline__ float erf_as(float x) {
-  const float ax = fabsf(x);
-  const float t = __frcp_rn(1.0f + 0.3275911f * ax);
-  const float poly =
-      t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f +
-                                                                      t * 1.061405429f))));
-  const float y = 1.0f - poly * __expf(-ax * ax);
-  return copysignf(y, x);
+// GELU(a) = a Phi(a) with Phi the standard normal CDF, in a form without transcendental functions
+// (the epilogue of every layer evaluates it on each activation, and v_exp / v_rcp run at a quarter
+// of the VALU rate): Phi(a) = 0.5 + a P(a^2) for |a| <= 3.5 with P an even polynomial of degree 10
+// fitted to minimise the largest error of a Phi(a); above 3.5 GELU(a) = a, below -3.5 it is 0.
+// Largest absolute error against the erf form: 8.2e-4 (at |a| = 3.5), below half a bf16 unit of
+// the outputs it feeds. GELU(a) - GELU(-a) = a holds exactly in real arithmetic (P is even).
+constexpr float kGeluCut = 3.5f;
+constexpr float kG0 = 0.39749979972839355f, kG1 = -0.0638996884226799f,
+                kG2 = 0.008380419574677944f, kG3 = -0.0007098362548276782f,
+                kG4 = 3.406826363061555e-05f, kG5 = -6.95484573043359e-07f;
+
+__device__ __forceinline__ float gelu_f(float a) {
+  const float u = a * a;
+  const float p = kG0 + u * (kG1 + u * (kG2 + u * (kG3 + u * (kG4 + u * kG5))));
+  const float g = a * (0.5f + a * p);
+  return a > kGeluCut ? a : (a < -kGeluCut ? 0.0f : g);
 }
 
-// GELU(a) = a * Phi(a) = 0.5 a (1 + erf(a / sqrt 2)) and its derivative Phi(a) + a phi(a)
-__device__ __forceinline__ float gelu_f(float a) {
-  return 0.5f * a * (1.0f + erf_as(a * 0.70710678118654752f));
-}
+// GELU(a) = 0.5 a + sum c_k a^(2k+2), so GELU'(a) = 0.5 + a sum (2k + 2) c_k a^(2k)
 __device__ __forceinline__ float gelu_grad_f(float a) {
-  const float cdf = 0.5f * (1.0f + erf_as(a * 0.70710678118654752f));
-  const float pdf = __expf(-0.5f * a * a) * 0.39894228040143268f;
-  return cdf + a * pdf;
+  const float u = a * a;
+  const float q = 2.0f * kG0 + u * (4.0f * kG1 + u * (6.0f * kG2 + u * (8.0f * kG3 +
+                  u * (10.0f * kG4 + u * (12.0f * kG5)))));
+  const float d = 0.5f + a * q;
+  return a > kGeluCut ? 1.0f : (a < -kGeluCut ? 0.0f : d);
 }
 
 __device__ __forceinline__ bf16x8 lds_frag(const uint16_t* p) {
@@ -471,11 +477,277 @@ __global__ void __launch_bounds__(256 * RB) k_mlp_infer(const uint16_t* __restri
   }
 }
 
+// ---- fused inference, wide form: 2 RB waves over 64 RB rows ----------------------------------
+// Wave w takes rows [64 (w / 2), +64) and columns [128 (w % 2), +128) of each layer, so each weight
+// fragment feeds four row blocks and each activation fragment eight column blocks: per row, half
+// the weight traffic of the form above at equal rows per workgroup. The features sit in the first
+// IN columns of the activation buffer until the first layer's outputs replace them, so a 128-row
+// tile takes 68 KB of LDS (two workgroups per CU) and a 256-row tile 139 KB. Every sum is formed
+// in the same order as in k_mlp_infer, so the outputs are identical.
+template <int K>
+__device__ __forceinline__ void infer_layer_w(const uint16_t* src,
+                                              const uint16_t* __restrict__ Wt,
+                                              f32x4 (&acc)[4][8]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int rb = w >> 1, ch = w & 1;
+  const int lr = lane & 15, lk = 8 * (lane >> 4);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const uint16_t* wrow = Wt + (int64_t)(128 * ch + lr) * K + lk;
+  const uint16_t* arow = src + (64 * rb + lr) * kHs + lk;
+  bf16x8 b[8], bn[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) b[j] = glb_frag(wrow + (int64_t)16 * j * K);
+#pragma unroll
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    if (k0 + 32 < K) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) bn[j] = glb_frag(wrow + (int64_t)16 * j * K + k0 + 32);
+    }
+    bf16x8 a[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = lds_frag(arow + 16 * i * kHs + k0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+    if (k0 + 32 < K) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) b[j] = bn[j];
+    }
+  }
+}
+
+__device__ __forceinline__ void infer_store_w(f32x4 (&acc)[4][8], const float* __restrict__ bias,
+                                              uint16_t* Hs) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int rb = w >> 1, ch = w & 1;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int col = 128 * ch + 16 * j + (lane & 15);
+    const float bn = bias[col];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 64 * rb + 16 * i + 4 * (lane >> 4) + r;
+        Hs[row * kHs + col] = f2bf(gelu_f(acc[i][j][r] + bn));
+      }
+  }
+}
+
+template <int RB>
+__global__ void __launch_bounds__(128 * RB, 2) k_mlp_infer_w(const uint16_t* __restrict__ X,
+                                                          int64_t n,
+                                                          const float* __restrict__ params,
+                                                          const uint16_t* __restrict__ wbf,
+                                                          float* __restrict__ prob,
+                                                          float* __restrict__ logit) {
+  constexpr int ROWS = 64 * RB, NT = 128 * RB;
+  __shared__ __attribute__((aligned(16))) uint16_t Hs[ROWS * kHs];
+  __shared__ float zp[4][ROWS];  // per 64-column block, as in k_mlp_infer
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int rb = w >> 1, ch = w & 1;
+  const int64_t m0 = (int64_t)blockIdx.x * ROWS;
+  for (int e = t; e < ROWS * IN / 8; e += NT) {
+    const int r = e / (IN / 8), c8 = (e % (IN / 8)) * 8;
+    u32x4 v = {0u, 0u, 0u, 0u};
+    if (m0 + r < n) v = *(const u32x4*)(X + (m0 + r) * IN + c8);
+    *(u32x4*)&Hs[r * kHs + c8] = v;
+  }
+  __syncthreads();
+  f32x4 acc[4][8];
+  infer_layer_w<IN>(Hs, wbf + BOFF_W1, acc);
+  __syncthreads();  // every wave has read the features before the outputs replace them
+  infer_store_w(acc, params + OFF_B1, Hs);
+  __syncthreads();
+  infer_layer_w<HID>(Hs, wbf + BOFF_W2, acc);
+  __syncthreads();
+  infer_store_w(acc, params + OFF_B2, Hs);
+  __syncthreads();
+  infer_layer_w<HID>(Hs, wbf + BOFF_W3, acc);
+  const float* b3 = params + OFF_B3;
+  const float* w4 = params + OFF_W4;
+  // z over each 64-column block (blocks 2 ch and 2 ch + 1), in k_mlp_infer's order
+  float part[2][4][4];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) part[h][i][r] = 0.f;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int col = 128 * ch + 16 * j + (lane & 15);
+    const float bn = b3[col], wn = w4[col];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        part[j >> 2][i][r] += bf2f(f2bf(gelu_f(acc[i][j][r] + bn))) * wn;
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float v = part[h][i][r];
+        v += __shfl_xor(v, 1, 64);
+        v += __shfl_xor(v, 2, 64);
+        v += __shfl_xor(v, 4, 64);
+        v += __shfl_xor(v, 8, 64);
+        if ((lane & 15) == 0) zp[2 * ch + h][64 * rb + 16 * i + 4 * (lane >> 4) + r] = v;
+      }
+  __syncthreads();
+  for (int r = t; r < ROWS; r += NT) {
+    if (m0 + r >= n) continue;
+    const float z = zp[0][r] + zp[1][r] + zp[2][r] + zp[3][r] + params[OFF_B4];
+    if (logit) logit[m0 + r] = z;
+    if (prob) prob[m0 + r] = 1.0f / (1.0f + __expf(-z));
+  }
+}
+
+// ---- fused inference, staged form: 256 rows, weights through LDS --------------------------------
+// Eight waves over 256 rows (wave w: rows [64 (w / 2), +64), columns [128 (w % 2), +128), as in
+// k_mlp_infer_w). Each 32-k chunk of a layer's weights (256 outputs x 32 k) is loaded once per
+// workgroup with coalesced 16-byte loads, stored to LDS and read there by all eight waves, so a
+// weight byte crosses the CU's load path once per 256 rows; the next chunk is loaded into
+// registers while the current one feeds the matrix cores. LDS: activations 135 KB + one weight
+// chunk 20 KB (the head's partial sums reuse the chunk's space). Same sums, same order: outputs
+// identical to k_mlp_infer.
+constexpr int kWs = 40;  // LDS row of a weight chunk: 32 k + padding (80 bytes, 16-byte aligned)
+
+template <int K>
+__device__ __forceinline__ void stage_chunk_load(const uint16_t* __restrict__ Wt, int c,
+                                                 u32x4 (&v)[2]) {
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int p = (int)threadIdx.x + 512 * q, n = p >> 2, kk = (p & 3) * 8;
+    v[q] = *(const u32x4*)(Wt + (int64_t)n * K + 32 * c + kk);
+  }
+}
+
+__device__ __forceinline__ void stage_chunk_store(uint16_t* Wb, const u32x4 (&v)[2]) {
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int p = (int)threadIdx.x + 512 * q, n = p >> 2, kk = (p & 3) * 8;
+    *(u32x4*)&Wb[n * kWs + kk] = v[q];
+  }
+}
+
+template <int K>
+__device__ __forceinline__ void infer_layer_s(const uint16_t* Hs, uint16_t* Wb,
+                                              const uint16_t* __restrict__ Wt,
+                                              f32x4 (&acc)[4][8]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int rb = w >> 1, ch = w & 1;
+  const int lr = lane & 15, lk = 8 * (lane >> 4);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  u32x4 nxt[2];
+  stage_chunk_load<K>(Wt, 0, nxt);
+  stage_chunk_store(Wb, nxt);
+  __syncthreads();
+  const uint16_t* arow = Hs + (64 * rb + lr) * kHs + lk;
+  const uint16_t* brow = Wb + (128 * ch + lr) * kWs + lk;
+#pragma unroll
+  for (int c = 0; c < K / 32; ++c) {
+    if (c + 1 < K / 32) stage_chunk_load<K>(Wt, c + 1, nxt);
+    bf16x8 a[4], b[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = lds_frag(arow + 16 * i * kHs + 32 * c);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) b[j] = lds_frag(brow + 16 * j * kWs);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+    __syncthreads();  // every wave has read this chunk
+    if (c + 1 < K / 32) {
+      stage_chunk_store(Wb, nxt);
+      __syncthreads();
+    }
+  }
+}
+
+__global__ void __launch_bounds__(512, 2) k_mlp_infer_s(const uint16_t* __restrict__ X,
+                                                        int64_t n,
+                                                        const float* __restrict__ params,
+                                                        const uint16_t* __restrict__ wbf,
+                                                        float* __restrict__ prob,
+                                                        float* __restrict__ logit) {
+  constexpr int ROWS = 256;
+  __shared__ __attribute__((aligned(16))) uint16_t Hs[ROWS * kHs];
+  __shared__ __attribute__((aligned(16))) uint16_t Wb[256 * kWs];
+  float* zp = (float*)Wb;  // [4][ROWS] after the last layer (4 KB of the chunk's 20 KB)
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int rb = w >> 1, ch = w & 1;
+  const int64_t m0 = (int64_t)blockIdx.x * ROWS;
+  for (int e = t; e < ROWS * IN / 8; e += 512) {
+    const int r = e / (IN / 8), c8 = (e % (IN / 8)) * 8;
+    u32x4 v = {0u, 0u, 0u, 0u};
+    if (m0 + r < n) v = *(const u32x4*)(X + (m0 + r) * IN + c8);
+    *(u32x4*)&Hs[r * kHs + c8] = v;
+  }
+  f32x4 acc[4][8];
+  infer_layer_s<IN>(Hs, Wb, wbf + BOFF_W1, acc);  // its first barrier also orders the features
+  infer_store_w(acc, params + OFF_B1, Hs);
+  __syncthreads();
+  infer_layer_s<HID>(Hs, Wb, wbf + BOFF_W2, acc);
+  infer_store_w(acc, params + OFF_B2, Hs);
+  __syncthreads();
+  infer_layer_s<HID>(Hs, Wb, wbf + BOFF_W3, acc);
+  const float* b3 = params + OFF_B3;
+  const float* w4 = params + OFF_W4;
+  float part[2][4][4];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) part[h][i][r] = 0.f;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int col = 128 * ch + 16 * j + (lane & 15);
+    const float bn = b3[col], wn = w4[col];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        part[j >> 2][i][r] += bf2f(f2bf(gelu_f(acc[i][j][r] + bn))) * wn;
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float v = part[h][i][r];
+        v += __shfl_xor(v, 1, 64);
+        v += __shfl_xor(v, 2, 64);
+        v += __shfl_xor(v, 4, 64);
+        v += __shfl_xor(v, 8, 64);
+        if ((lane & 15) == 0) zp[(2 * ch + h) * ROWS + 64 * rb + 16 * i + 4 * (lane >> 4) + r] = v;
+      }
+  __syncthreads();
+  for (int r = t; r < ROWS; r += 512) {
+    if (m0 + r >= n) continue;
+    const float z = zp[r] + zp[ROWS + r] + zp[2 * ROWS + r] + zp[3 * ROWS + r] + params[OFF_B4];
+    if (logit) logit[m0 + r] = z;
+    if (prob) prob[m0 + r] = 1.0f / (1.0f + __expf(-z));
+  }
+}
+
 // ---- host side ---------------------------------------------------------------------------------
 static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 constexpr int64_t kSplitK = 512;  // batch rows per weight-gradient partial
-constexpr int kInferTileDefault = 64;  // rows per workgroup of the fused inference
+constexpr int kInferTileDefault = 256;  // rows per workgroup of the fused inference
 
 struct TrainWs {
   float *a1, *a2, *a3;
@@ -699,12 +971,15 @@ int paces_mlp_infer_tiled(const uint16_t* X, int64_t n, const float* params,
                           void* stream) {
   if (n < 0 || !params || !wbf16 || (!prob && !logit)) return PACES_E_INVALID;
   if (tile_rows == 0) tile_rows = kInferTileDefault;
-  if (tile_rows != 64 && tile_rows != 128) return PACES_E_INVALID;
+  if (tile_rows != 64 && tile_rows != 128 && tile_rows != 256) return PACES_E_INVALID;
   if (n == 0) return PACES_OK;
   if (!X) return PACES_E_INVALID;
   hipStream_t st = (hipStream_t)stream;
-  if (tile_rows == 128)
-    hipLaunchKernelGGL(k_mlp_infer<2>, dim3(grid_for(n, 128)), dim3(512), 0, st, X, n, params,
+  if (tile_rows == 256)  // weights staged through LDS
+    hipLaunchKernelGGL(k_mlp_infer_s, dim3(grid_for(n, 256)), dim3(512), 0, st, X, n, params,
+                       wbf16, prob, logit);
+  else if (tile_rows == 128)
+    hipLaunchKernelGGL(k_mlp_infer_w<2>, dim3(grid_for(n, 128)), dim3(256), 0, st, X, n, params,
                        wbf16, prob, logit);
   else
     hipLaunchKernelGGL(k_mlp_infer<1>, dim3(grid_for(n, 64)), dim3(256), 0, st, X, n, params,

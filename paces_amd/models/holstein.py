@@ -223,7 +223,7 @@ class Hamiltonian(HamiltonianFramework):
         return None
 
     def generate_mel(self, basis_states, enlarge_steps=0, log_ind=1.2,
-                     value_dtype=torch.float64, diag_dtype=torch.float64):
+                     value_dtype=torch.float64, diag_dtype=torch.float64, first=None):
         """((diag, new_inds), {term: CsrMatrix}, dbg) for the basis (hamiltonian.py:383-492).
 
         new_inds = the basis (after ``enlarge_steps`` expansions) plus its neighbours; each term's
@@ -231,12 +231,19 @@ class Hamiltonian(HamiltonianFramework):
         gives the CSR values rounded to FP32 (what the mixed-precision series reads);
         value_dtype="coded" (batched trajectories) gives coded CSRs (engine.CodedCsrMatrix);
         diag_dtype=torch.float32 the diagonal rounded to FP32.
+
+        first=(expanded basis, n_prev, run offsets, src, nup, union positions, from0): an
+        expansion already made by the caller (the learned step's, with the rows it pruned
+        removed: their union positions are -1), continued here with enlarge_steps more.
         """
         if basis_states.shape[1] != self.totwordwidth:
             raise ValueError("Shape of basis states does not match the specified number of bits.")
         basis = basis_states
         last = None  # the last expansion's images, with their union positions
         prior = None
+        if first is not None:
+            basis, n_prev, offs_f, src_f, nup_f, cpos_f, from0_f = first
+            last = (n_prev, offs_f, src_f, nup_f, cpos_f, from0_f)
         for _ in range(enlarge_steps):
             imgs, offs, src, nup = self._images(basis, want_src=True, slot="expand")
             runs = [(basis, 0, basis.shape[0])]
@@ -815,24 +822,30 @@ class LabelLog:
 class LearnedTimeEvolution(TimeEvolution):
     """The adaptive-basis step with the candidate scorer (no counterpart in the reference).
 
-    Per step:
-    * select, expand, and build the images and their union, as the plain step does;
-    * carry the amplitudes over onto the union and score every union row outside the selection;
-    * drop rows whose predicted retain probability is below ``threshold`` before the CSR build;
-    * keep a hashed sample of 2^-sample_bits of the dropped rows in the basis, so the weight
-      that all dropped rows would have received is measured: the sample's weight after
-      propagation times (dropped / sampled);
-    * when that estimate exceeds ``fallback_tol``, redo the step with the full union, which is
-      the plain step.
+    Per step (enlarge_steps >= 1):
+    * select, then expand the selection once: the basis plus its images (the first shell);
+    * carry the amplitudes over onto that basis and score every row the expansion added;
+    * drop the added rows whose predicted retain probability is below ``threshold``, keeping a
+      hashed sample of 2^-sample_bits of them, so rows unlikely to survive the next selection,
+      and the rows they would have reached, are never materialised;
+    * run the remaining expansions, the union, the CSR and the propagation on the rest;
+    * the weight that all dropped rows would have kept is estimated from the sample: its weight
+      in the next selection times (dropped / sampled); when that exceeds ``fallback_tol`` the
+      step is redone from the whole first expansion, which is the plain step.
 
-    With ``scorer=None`` every row is kept (the plain step through the same code). A
-    ``label_log`` then records training labels. ``stats`` holds one dict per step: full and kept
-    rows, sampled rows, estimated dropped weight and whether the step fell back.
+    With ``scorer=None`` nothing is dropped (the plain step through the same code). A
+    ``label_log`` then records training labels: the added rows of each first expansion with
+    their features, labelled by the next selection. ``stats`` holds one dict per step: rows of
+    the first expansion, rows scored and kept, sampled rows, final rows, the estimated dropped
+    weight and whether the step fell back.
     """
 
     def __init__(self, *args, scorer=None, threshold=0.01, sample_bits=6, fallback_tol=1e-6,
                  label_log=None, **kwargs):
         super().__init__(*args, **kwargs)
+        if self.te_params.enlarge_steps < 1:
+            raise ValueError("the learned step prunes the first of enlarge_steps + 1 expansions: "
+                             "it needs enlarge_steps >= 1")
         self.scorer = scorer
         self.threshold = float(threshold)
         self.sample_bits = int(sample_bits)
@@ -851,14 +864,16 @@ class LearnedTimeEvolution(TimeEvolution):
             return sel
         return super()._determine_select_whoami(vector, use_weight_function, tau, garbage_tol)
 
-    def _install(self, rows, in_basis, cand_pos, psi, diag, basis, offs, src, nup):
+    def _build(self, first, enlarge_steps):
+        """The rows, CSR and carried amplitudes of the step from its first expansion (see
+        Hamiltonian.generate_mel's first); returns the debug columns."""
         ham = self.hamobj
-        mats = ham._pairs_csr(rows.shape[0], basis.shape[0], offs, cand_pos, in_basis, src, nup,
-                              basis)
-        self.numstates = rows.shape[0]
+        (diag, new_inds), mats, dbg = ham.generate_mel(first[0], enlarge_steps, first=first)
+        self.numstates = new_inds.shape[0]
         self.create_matrices(diag, mats)
-        self.vector = psi
-        self.whoami = rows
+        self.vector = engine.reassign(self.whoami, self.vector, new_inds, self._ws)
+        self.whoami = new_inds
+        return dbg
 
     def _generate_new_hilbert_space(self, enlarge_steps):
         ham = self.hamobj
@@ -867,78 +882,92 @@ class LearnedTimeEvolution(TimeEvolution):
         self.last_select = select
         if self.label_log is not None:
             self.label_log.resolve(select)
-        basis = select
-        for _ in range(enlarge_steps):
-            basis = ham.enlarge_basis_set(basis)
-        imgs, offs, src, nup = ham._images(basis, want_src=True)
-        runs = [(basis, 0, basis.shape[0])]
+        # first expansion: the selection and its images
+        imgs, offs, src, nup = ham._images(select, want_src=True, slot="learned")
+        runs = [(select, 0, select.shape[0])]
         runs += [(imgs, offs[k], offs[k + 1] - offs[k]) for k in range(len(offs) - 1)]
-        U, in_basis, cand_pos = engine.merge_unique(runs, W, self._ws, want_from0=True,
-                                                    want_cand_pos=True)
-        ham._check_basis_size(U.shape[0])
-        psiU = engine.reassign(self.whoami, self.vector, U, self._ws)
-        diagU = ham.generate_mel_diag(U)
-        R = U.shape[0]
+        B, in_sel, cand_pos = engine.merge_unique(runs, W, self._ws, want_from0=True,
+                                                  want_cand_pos=True)
+        nB = B.shape[0]
+        nsel = select.shape[0]
+        full = (B, nsel, offs, src, nup, cand_pos, in_sel)
+        want_x = self.scorer is not None or (self.label_log is not None and
+                                             not self.label_log.full)
         X = None
-        if self.scorer is not None or (self.label_log is not None and not self.label_log.full):
-            X = ham.features(U, psiU, diagU, basis.shape[0], offs, cand_pos, src)
+        if want_x:
+            psiB = engine.reassign(self.whoami, self.vector, B, self._ws)
+            X = ham.features(B, psiB, ham.generate_mel_diag(B), select.shape[0], offs, cand_pos,
+                             src)
+            del psiB
         if self.label_log is not None and X is not None:
-            self.label_log.stage(U, X, select)
-        st = {"rows_full": R, "rows_kept": R, "sampled": 0, "dropped_weight": 0.0,
-              "dropped_retained_weight": 0.0, "fallback": False}
+            self.label_log.stage(B, X, select)
+        st = {"rows_first": nB, "scored": 0, "kept": 0, "sampled": 0, "rows": 0,
+              "dropped_weight": 0.0, "dropped_retained_weight": 0.0, "fallback": False}
         self._sample = None
         self._full = None
         if self.scorer is None:
-            self._install(U, in_basis, cand_pos, psiU, diagU, basis, offs, src, nup)
+            dbg = self._build(full, enlarge_steps - 1)
         else:
-            tuner = self._tuner()
-            if tuner is not None and R:
-                from ..core import autotune
-                self.scorer.tile_rows = tuner.pick(
-                    tuner.key("mlp", R), autotune.MLP_TILES,
-                    lambda t: (lambda: self.scorer.infer(X, tile_rows=t)))
-            prob = self.scorer.infer(X)
-            forced, _, _ = engine.lower_bound(U, select, want_hit=False, q_sorted=True)
-            keep = torch.empty(max(R, 1), dtype=torch.uint8, device=self.device)
-            # sample 2^-bits of the dropped rows, with bits lowered on small unions so the sample
+            added = torch.nonzero(in_sel == 0).flatten()
+            prob = torch.ones(max(nB, 1), dtype=torch.float32, device=self.device)
+            if added.numel():
+                Xa = X.index_select(0, added)
+                tuner = self._tuner()
+                if tuner is not None:
+                    from ..core import autotune
+                    self.scorer.tile_rows = tuner.pick(
+                        tuner.key("mlp", Xa.shape[0]), autotune.MLP_TILES,
+                        lambda t: (lambda: self.scorer.infer(Xa, tile_rows=t)))
+                prob.index_copy_(0, added, self.scorer.infer(Xa))
+                del Xa
+            del X
+            forced = torch.nonzero(in_sel).flatten().to(torch.int32)
+            keep = torch.empty(max(nB, 1), dtype=torch.uint8, device=self.device)
+            # sample 2^-bits of the dropped rows, bits lowered on small expansions so the sample
             # holds a few hundred rows
-            bits = max(0, min(self.sample_bits, int(math.log2(max(R, 1) / 4096.0))
-                              if R >= 4096 else 0))
-            check(_lib.paces_prune_plan(ptr(prob), R, self.threshold, ptr(forced),
-                                        select.shape[0], bits,
+            bits = max(0, min(self.sample_bits, int(math.log2(max(nB, 1) / 4096.0))
+                              if nB >= 4096 else 0))
+            check(_lib.paces_prune_plan(ptr(prob), nB, self.threshold,
+                                        ptr(forced) if forced.numel() else None, forced.numel(),
+                                        bits,
                                         (self._step_index + 1) * 0x9E3779B97F4A7C15 % (1 << 64),
                                         ptr(keep), stream()), "prune_plan")
+            # the kept rows in order, the selection's flags and the union positions renumbered
+            # (-1 for a pruned row: its pairs are not formed)
             nC = cand_pos.shape[0]
-            rows_k = torch.empty((max(R, 1), W), dtype=U.dtype, device=self.device)
-            psi_k = torch.empty(max(R, 1), dtype=torch.complex128, device=self.device)
-            inb_k = torch.empty(max(R, 1), dtype=torch.uint8, device=self.device)
-            smp_k = torch.empty(max(R, 1), dtype=torch.uint8, device=self.device)
+            rows_k = torch.empty((max(nB, 1), W), dtype=B.dtype, device=self.device)
+            inb_k = torch.empty(max(nB, 1), dtype=torch.uint8, device=self.device)
+            smp_k = torch.empty(max(nB, 1), dtype=torch.uint8, device=self.device)
             pos_k = torch.empty(max(nC, 1), dtype=torch.int32, device=self.device)
             cnt = torch.empty(1, dtype=torch.int32, device=self.device)
-            wsp, wsn = self._ws.get(_lib.paces_compact_workspace_bytes(R))
-            check(_lib.paces_compact_kept(ptr(keep), R, ptr(U), W,
-                                          ptr(torch.view_as_real(psiU)), ptr(in_basis),
-                                          ptr(cand_pos), nC, ptr(rows_k),
-                                          ptr(torch.view_as_real(psi_k)), ptr(inb_k), ptr(smp_k),
-                                          ptr(pos_k), ptr(cnt), wsp, wsn, stream()),
+            wsp, wsn = self._ws.get(_lib.paces_compact_workspace_bytes(nB))
+            check(_lib.paces_compact_kept(ptr(keep), nB, ptr(B), W, None, ptr(in_sel),
+                                          ptr(cand_pos), nC, ptr(rows_k), None, ptr(inb_k),
+                                          ptr(smp_k), ptr(pos_k), ptr(cnt), wsp, wsn, stream()),
                   "compact_kept")
-            Rk = int(cnt.item())
-            rows_k, psi_k, inb_k, smp_k = rows_k[:Rk], psi_k[:Rk], inb_k[:Rk], smp_k[:Rk]
-            nsample = int(smp_k.sum().item()) if Rk else 0
-            st.update(rows_kept=Rk, sampled=nsample)
-            self._sample = (smp_k, nsample, R - Rk + nsample)
-            self._full = (U, in_basis, cand_pos, psiU, diagU, basis, offs, src, nup)
-            self._install(rows_k, inb_k, pos_k, psi_k, ham.generate_mel_diag(rows_k), basis,
-                          offs, src, nup)
+            nK = int(cnt.item())
+            rows_k, inb_k, smp_k = rows_k[:nK], inb_k[:nK], smp_k[:nK]
+            sampled = rows_k[smp_k.bool()] if nK else rows_k
+            st.update(scored=nB - nsel, kept=nK - nsel, sampled=int(sampled.shape[0]))
+            self._full = (full, enlarge_steps - 1, self.whoami, self.vector)
+            dbg = self._build((rows_k, nsel, offs, src, nup, pos_k, inb_k), enlarge_steps - 1)
+            # the sampled rows' positions in the new basis
+            mark = torch.zeros(max(self.numstates, 1), dtype=torch.uint8, device=self.device)
+            if sampled.shape[0]:
+                idx, _, _ = engine.lower_bound(self.whoami, sampled, want_hit=False,
+                                               q_sorted=True)
+                # every kept row is a row of the new basis
+                check(_lib.paces_mark_indices(ptr(idx), idx.shape[0], None, ptr(mark),
+                                              stream()), "mark_indices")
+            self._sample = (mark, int(sampled.shape[0]), nB - nK + int(sampled.shape[0]))
+        st["rows"] = self.numstates
         self.stats.append(st)
-        dbg = {}
-        if "vib_coupling" in ham.offdiag_terms:
-            dbg["vib_coupling"] = [basis.shape[0] - ham._vib_plus_count(offs)]
         return dbg, self._post_assignment_diagnostics()
 
     def _check_dropped(self, vec, expm_dbg, delta_t):
         """Estimate, from the sampled dropped rows, the weight of dropped rows that the next
-        selection would keep; fall back to the full union when it exceeds fallback_tol."""
+        selection would keep; fall back to the whole first expansion when it exceeds
+        fallback_tol."""
         smp, nsample, ndropped = self._sample
         st = self.stats[-1]
         if ndropped == 0:
@@ -958,7 +987,10 @@ class LearnedTimeEvolution(TimeEvolution):
         st["dropped_retained_weight"] = est
         if est > self.fallback_tol:
             st["fallback"] = True
-            self._install(*self._full)
+            full, steps, old_rows, old_vec = self._full
+            self.whoami, self.vector = old_rows, old_vec
+            self._build(full, steps)
+            st["rows"] = self.numstates
             return self._propagate(delta_t)
         if not self.expm_params.renormalize:
             self._next_select = sel

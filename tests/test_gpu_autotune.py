@@ -5,7 +5,7 @@
   mixed precision and batched (FP64 and mixed): partial sums, terms and series state.
 * A run with autotuning equals the run without it, bit for bit, and the tuner records a choice
   among the candidates, with a timing for each.
-* The scorer's two inference tiles give identical outputs.
+* The scorer's inference tiles (64, 128 and 256 rows) give identical outputs.
 """
 
 import ctypes as C
@@ -151,4 +151,8 @@ def test_scorer_tiles_identical(env):
     from paces_amd.core import scorer
     s = scorer.CandidateScorer(env["dev"], seed=4)
     X = (torch.rand((5000, 64), device=env["dev"]) * 2 - 1).to(torch.bfloat16).view(torch.int16)
-    assert torch.equal(s.infer(X, tile_rows=64), s.infer(X, tile_rows=128))
+    ref = s.infer(X, tile_rows=64)
+    for tile in (128, 256):
+        assert torch.equal(ref, s.infer(X, tile_rows=tile)), tile
+        assert torch.equal(s.infer(X, want_logit=True, tile_rows=64),
+                           s.infer(X, want_logit=True, tile_rows=tile)), tile

@@ -7,10 +7,11 @@
 * Compaction (paces_prune_plan, paces_compact_kept) is checked against NumPy boolean indexing.
 * With no scorer the learned step keeps every row and equals the plain step bit for bit.
 * Ablation on the 16-site chain (cap 7, maxstates 2e4):
-  * labels are logged from plain runs at three couplings, and the scorer is trained on the
-    device;
-  * at a fourth coupling the trained scorer drops a share of the union rows;
-  * the maximum site-density error against the plain step stays within 1e-4, with no more
+  * labels are logged from plain runs at three couplings (the rows each first expansion adds,
+    labelled by the next selection), and the scorer is trained on the device;
+  * at a fourth coupling the trained scorer drops a share of the added rows before the second
+    expansion, so the final basis is smaller than the plain step's;
+  * the maximum site-density error against the FP64 oracle stays within 1e-4, with no more
     than one fallback;
   * an untrained network keeps everything, so it gains nothing over the method without it.
 """
@@ -212,28 +213,43 @@ def test_trained_scorer_gains_over_untrained_at_1e4_density_error(env):
     print(f"[learned] training loss by epoch {['%.4f' % v for v in hist]}", flush=True)
     assert hist[-1] < hist[0]
     g = gs[3]
+    # the FP64 oracle over the same times (the plain GPU step equals it bit for bit)
+    oham = po.HolsteinOracle(16, [8] * 16, use_terms=_terms(1.0, g))
+    ote = po.TimeEvolutionOracle(oham, 20000, 1)
+    ote.create_single_state_basis(8)
+    raw = np.zeros((1, 17), dtype=np.int64)
+    raw[0, 0] = 8
+    ote.create_initial_vector([1.0], raw)
+    recs = []
+    ote.generate_timeline(DT * np.arange(41), recs.append)
     plain = _make(env, h.TimeEvolution, 16, 8, g, 20000)
     runs = {"trained": _make(env, h.LearnedTimeEvolution, 16, 8, g, 20000, scorer=trained),
             "untrained": _make(env, h.LearnedTimeEvolution, 16, 8, g, 20000,
                                scorer=sc.CandidateScorer(env["dev"], seed=2))}
     err = {k: 0.0 for k in runs}
+    rows_plain = []
+    t = DT * np.arange(41)
     for i in range(1, 41):
-        plain._evolve_single_step(i, DT * i, DT)
-        ref = plain.obsobj.last_results["n_exc"]
+        plain._evolve_single_step(i, t[i], t[i] - t[i - 1])
+        rows_plain.append(plain.numstates)
+        ref = recs[i].obs["n_exc"]
         for k, te in runs.items():
-            te._evolve_single_step(i, DT * i, DT)
+            te._evolve_single_step(i, t[i], t[i] - t[i - 1])
             err[k] = max(err[k], float(np.max(np.abs(te.obsobj.last_results["n_exc"] - ref))))
     summary = {}
     for k, te in runs.items():
         st = te.stats[10:]  # the basis has saturated maxstates
-        kept = float(np.mean([s["rows_kept"] / s["rows_full"] for s in st]))
+        kept = float(np.mean([s["kept"] / max(s["scored"], 1) for s in st]))
+        rows = float(np.mean([s["rows"] / r for s, r in zip(st, rows_plain[10:])]))
         fb = sum(s["fallback"] for s in te.stats)
-        summary[k] = (kept, fb, err[k])
-        print(f"[learned] {k}: mean kept fraction {kept:.3f}, fallbacks {fb}, max density "
-              f"error {err[k]:.2e}", flush=True)
-    kept_t, fb_t, err_t = summary["trained"]
-    kept_u, _, _ = summary["untrained"]
+        summary[k] = (kept, fb, err[k], rows)
+        print(f"[learned] {k}: mean kept fraction of the added rows {kept:.3f}, final rows "
+              f"{rows:.3f} of the plain step's, fallbacks {fb}, max density error vs the oracle "
+              f"{err[k]:.2e}", flush=True)
+    kept_t, fb_t, err_t, rows_t = summary["trained"]
+    kept_u, _, _, rows_u = summary["untrained"]
     assert err_t <= 1e-4
     assert fb_t <= 1
-    assert kept_t <= 0.8
-    assert kept_t < kept_u - 0.1
+    assert kept_t <= 0.8 and rows_t < 0.95
+    # an untrained network passes (nearly) every row: no smaller basis than the plain step's
+    assert kept_u >= 0.99 and abs(rows_u - 1.0) < 0.01 and kept_t < kept_u - 0.1

@@ -42,7 +42,7 @@ def _theta(seed):
     return th.astype(np.float32)
 
 
-@pytest.mark.parametrize("tile", [64, 128])
+@pytest.mark.parametrize("tile", [64, 128, 256])
 @pytest.mark.parametrize("n", [1, 64, 1000, 4099])
 def test_infer_matches_oracle(env, n, tile):
     sc = env["scorer"]

@@ -6,8 +6,9 @@
   bias correction), and later steps follow Kingma & Ba's recurrences written out by hand.
 * bf16 rounding agrees with torch's float32 -> bfloat16 conversion (round to nearest even),
   including ties, infinities and NaN.
-* GELU is the erf form: GELU(0) = 0, GELU(x) - GELU(-x) = x, and its derivative matches a
-  central difference.
+* GELU is within 8.2e-4 of the erf form a (1 + erf(a / sqrt 2)) / 2 (math.erf) everywhere,
+  GELU(0) = 0, GELU(x) - GELU(-x) = x, and its derivative matches a central difference away
+  from the cut at |a| = 3.5.
 """
 
 import math
@@ -87,14 +88,16 @@ def test_bf16_rounding_matches_torch():
     assert mo.bf16(np.array([1 + 2 ** -8], dtype=np.float32))[0] == 1.0  # tie to even
 
 
-def test_gelu_erf_form():
-    x = np.linspace(-4, 4, 81)
+def test_gelu_polynomial_form():
+    x = np.linspace(-8, 8, 16001)
+    erf_form = np.array([0.5 * v * (1 + math.erf(v / math.sqrt(2))) for v in x])
+    assert np.max(np.abs(mo.gelu(x) - erf_form)) <= 8.2e-4
     assert mo.gelu(np.array([0.0]))[0] == 0.0
-    np.testing.assert_allclose(mo.gelu(x) - mo.gelu(-x), x, atol=1e-14)
+    np.testing.assert_allclose(mo.gelu(x) - mo.gelu(-x), x, atol=1e-13)
+    xs = x[np.abs(np.abs(x) - 3.5) > 1e-3]
     h = 1e-6
-    fd = (mo.gelu(x + h) - mo.gelu(x - h)) / (2 * h)
-    np.testing.assert_allclose(mo.gelu_grad(x), fd, atol=1e-8)
-    assert abs(mo.gelu(np.array([1.0]))[0] - 0.5 * (1 + math.erf(1 / math.sqrt(2)))) < 1e-15
+    fd = (mo.gelu(xs + h) - mo.gelu(xs - h)) / (2 * h)
+    np.testing.assert_allclose(mo.gelu_grad(xs), fd, atol=1e-7)
 
 
 def test_parameter_layout_is_the_kernels():
