@@ -507,7 +507,7 @@ def measure_scorer(scorer, X, reps=10):
 
 
 def learned_leg(ms, labels, log_steps, epochs, warmup, steps, precision="mixed",
-                verbose=True):
+                verbose=True, lr=2e-3):
     """The learned form (scorer pruning before the second expansion, complex64 Taylor terms by
     default) against the plain FP64 form on the same trajectory, after logging labels from plain
     runs at three couplings and training the scorer on them; both timed over the same steps.
@@ -535,7 +535,7 @@ def learned_leg(ms, labels, log_steps, epochs, warmup, steps, precision="mixed",
     X, y = log.data()
     scorer = CandidateScorer(dev, seed=1)
     t0 = time.perf_counter()
-    hist = scorer.train(X, y, epochs=epochs, batch=16384, lr=1e-3)
+    hist = scorer.train(X, y, epochs=epochs, batch=16384, lr=lr)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     if verbose:
@@ -552,15 +552,19 @@ def learned_leg(ms, labels, log_steps, epochs, warmup, steps, precision="mixed",
     up = ul = 0
     err = 0.0
     rows_ratio = []
+    windows = []
     for i in range(1, warmup + steps + 1):
         timed = i > warmup
         for name, te in (("plain", plain), ("learned", learned)):
             prev = te.numstates
             torch.cuda.synchronize()
+            w0 = time.monotonic_ns()
             t0 = time.perf_counter()
             te._evolve_single_step(i, t_arr[i], t_arr[i] - t_arr[i - 1])
             torch.cuda.synchronize()
             dt_ = time.perf_counter() - t0
+            if timed:
+                windows.append((name, i, w0, time.monotonic_ns()))
             if timed:
                 if name == "plain":
                     tp += dt_
@@ -578,11 +582,17 @@ def learned_leg(ms, labels, log_steps, epochs, warmup, steps, precision="mixed",
             print(f"[learned] step {i}: first expansion {s['rows_first']}, scored {s['scored']}, "
                   f"kept {s['kept']}, rows {s['rows']} (plain {plain.numstates}), fallback "
                   f"{s['fallback']}, density err {e:.2e}", file=sys.stderr, flush=True)
+    if verbose:
+        # per-step windows, for cutting a kernel trace of this run by step and form
+        for name, i, w0, w1 in windows:
+            print(f"[learned] step window monotonic_ns {name} {i} {w0} {w1}", file=sys.stderr,
+                  flush=True)
     st = learned.stats[warmup:]
     # the scorer's roofline on as many rows as the last step scored
     nscore = max(int(learned.stats[-1]["scored"]), 1)
     Xb = torch.zeros((nscore, 64), dtype=torch.int16, device=dev)
     sec = measure_scorer(scorer, Xb)
+    tile = scorer.tile_rows or 128  # 0: the library default, 128 rows
     achieved = MLP_FLOPS_PER_ROW * nscore / sec / 1e12
     return {"value": ul / tl, "ms_per_step": 1e3 * tl / steps,
             "plain_gpu_state_steps_per_s": up / tp, "plain_ms_per_step": 1e3 * tp / steps,
@@ -591,21 +601,20 @@ def learned_leg(ms, labels, log_steps, epochs, warmup, steps, precision="mixed",
                                                            for x in st])),
             "mean_rows_vs_plain": float(np.mean(rows_ratio)),
             "fallbacks": int(sum(x["fallback"] for x in st)), "labels": int(nlabels),
-            "label_log_s": log_s, "train_s": train_s, "train_epochs": epochs,
+            "label_log_s": log_s, "train_s": train_s, "train_epochs": epochs, "train_lr": lr,
             "precision": precision,
             "scorer_roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": 2500.0,
                                 "unit": "TFLOP/s", "frac": round(achieved / 2500.0, 4),
                                 "traffic": None,
-                                "kernel": "k_mlp_infer_s (paces_mlp_infer, bf16 MFMA, 256-row "
-                                          "tiles)" if scorer.tile_rows in (0, 256) else
-                                          f"paces_mlp_infer, {scorer.tile_rows}-row tiles",
+                                "kernel": "k_mlp_infer<%d> (paces_mlp_infer, bf16 MFMA, %d-row "
+                                          "tiles)" % (tile // 64, tile),
                                 "avg_us": round(sec * 1e6, 1), "rows": int(nscore)}}
 
 
 def run_learned(a):
     """The learned form's own line (config-4 physics): value = its state-steps/s."""
     r = learned_leg(a.learned_ms, a.labels, a.log_steps, a.epochs, a.warmup, a.steps,
-                    a.learned_precision)
+                    a.learned_precision, lr=a.lr)
     roof = r.pop("scorer_roofline")
     line = {"metric": METRIC, "value": r.pop("value"), "unit": "state-steps/s", "n_gpus": 1,
             "steps": a.steps, "warmup": a.warmup, "ms_per_step": r.pop("ms_per_step"),
@@ -667,8 +676,10 @@ def main(argv=None):
     ap.add_argument("--learned-precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--labels", type=int, default=100_000_000,
                     help="labelled candidates logged for training (learned workload)")
-    ap.add_argument("--log-steps", type=int, default=20)
-    ap.add_argument("--epochs", type=int, default=2)
+    ap.add_argument("--log-steps", type=int, default=35,
+                    help="steps per training coupling to log labels from (until --labels)")
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=2e-3, help="scorer training: Adam step size")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 1 GiB device copy (counter calibration)")
     a = ap.parse_args(argv)

@@ -613,11 +613,33 @@ int paces_holstein_features(const uint64_t* rows, int64_t R, const double* psi,
                             const int32_t* cand_pos, const int32_t* img_src,
                             const paces_layout_t* lay, const paces_holstein_t* h, uint16_t* X,
                             void* ws, size_t ws_bytes, void* stream);
+/* The same features for the added rows of a union only: the rows with in_basis[i] == 0 (in_basis
+ * the 0/1 flags of the rows the expansion started from), in order. X holds (R - sum in_basis)
+ * rows and row_index[j] = the union row of X's row j. The diagonal is computed from each row.
+ * Replaces a full-union paces_holstein_features followed by a gather of the added rows. */
+size_t paces_holstein_features_added_workspace_bytes(int64_t R);
+int paces_holstein_features_added(const uint64_t* rows, int64_t R, const double* psi,
+                                  int64_t nbasis, const int64_t* run_offsets,
+                                  const int32_t* cand_pos, const int32_t* img_src,
+                                  const uint8_t* in_basis, const paces_layout_t* lay,
+                                  const paces_holstein_t* h, uint16_t* X, int32_t* row_index,
+                                  void* ws, size_t ws_bytes, void* stream);
 /* keep[i] = 1 when prob[i] >= threshold or i is one of forced_idx; else 2 when sample_bits >= 0
  * and the low sample_bits bits of a hash of (i, seed) are zero (a sample of the pruned rows kept
  * to measure their weight); else 0. */
 int paces_prune_plan(const float* prob, int64_t n, float threshold, const int32_t* forced_idx,
                      int64_t nforced, int32_t sample_bits, uint64_t seed, uint8_t* keep,
+                     void* stream);
+/* The plan of paces_prune_plan from the scores of the added rows only (prob[j] for the union
+ * row row_index[j], j < m, as paces_holstein_features_added writes them): keep = in_basis
+ * (0/1) for the n union rows, then the decision of each scored row at its union position, with
+ * the same sample hash of (union row, seed). Equals paces_prune_plan on the scores scattered to
+ * the union with the in_basis rows forced. */
+int paces_prune_plan_rows(const float* prob, const int32_t* row_index, int64_t m,
+                          const uint8_t* in_basis, int64_t n, float threshold,
+                          int32_t sample_bits, uint64_t seed, uint8_t* keep, void* stream);
+/* out[idx[j]] = src[j] for j < m; other entries are untouched. */
+int paces_scatter_u8(const int32_t* idx, int64_t m, const uint8_t* src, uint8_t* out,
                      void* stream);
 /* out[idx[j]] = src[idx[j]] (or 1 when src is null) for j < m; other entries are untouched. */
 int paces_mark_indices(const int32_t* idx, int64_t m, const uint8_t* src, uint8_t* out,

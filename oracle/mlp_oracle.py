@@ -72,22 +72,23 @@ def bf16(x):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-# GELU(a) = a Phi(a), Phi(a) = 0.5 + a P(a^2) on |a| <= 3.5 (P even, the float32 coefficients the
-# kernels use), a above 3.5 and 0 below -3.5; evaluated in float64 here
+# GELU(a) = a / 2 + h(a), h even: h = u P(u) (u = a^2) on |a| <= 3.5 and |a| / 2 beyond, with P
+# the float32 coefficients the kernels use (fitted with 3.5^2 P(3.5^2) = 3.5 / 2, so the pieces
+# meet); the kernels' branch-free arrangement, evaluated in float64 here
 GELU_CUT = 3.5
-GELU_COEF = [float(np.float32(c)) for c in (0.39749979972839355, -0.0638996884226799,
-                                             0.008380419574677944, -0.0007098362548276782,
-                                             3.406826363061555e-05, -6.95484573043359e-07)]
+GELU_COEF = [float(np.float32(c)) for c in (0.39518553018569946, -0.061023153364658356,
+                                             0.0072402348741889, -0.0005149735952727497,
+                                             1.9077817341894843e-05, -2.691017755296343e-07)]
 
 
 def gelu(a):
     a = np.asarray(a, dtype=np.float64)
-    u = a * a
+    ac = np.clip(a, -GELU_CUT, GELU_CUT)
+    u = ac * ac
     p = np.zeros_like(a)
     for c in reversed(GELU_COEF):
         p = p * u + c
-    body = a * (0.5 + a * p)
-    return np.where(a > GELU_CUT, a, np.where(a < -GELU_CUT, 0.0, body))
+    return 0.5 * (np.abs(a) - np.abs(ac)) + (0.5 * a + p * u)
 
 
 def gelu_grad(a):

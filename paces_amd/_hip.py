@@ -255,7 +255,12 @@ _SIGNATURES = {
     "paces_holstein_features_workspace_bytes": (_SZ, [_I64]),
     "paces_holstein_features": (C.c_int, [_P, _I64, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P,
                                           _SZ, _P]),
+    "paces_holstein_features_added_workspace_bytes": (_SZ, [_I64]),
+    "paces_holstein_features_added": (C.c_int, [_P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _P,
+                                                _P, _P, _SZ, _P]),
     "paces_prune_plan": (C.c_int, [_P, _I64, _F, _P, _I64, _I32, C.c_uint64, _P, _P]),
+    "paces_prune_plan_rows": (C.c_int, [_P, _P, _I64, _P, _I64, _F, _I32, C.c_uint64, _P, _P]),
+    "paces_scatter_u8": (C.c_int, [_P, _I64, _P, _P, _P]),
     "paces_norm2_masked": (C.c_int, [_P, _P, _I64, _P, _P, _SZ, _P]),
     "paces_mark_indices": (C.c_int, [_P, _I64, _P, _P, _P]),
     "paces_compact_workspace_bytes": (_SZ, [_I64]),

@@ -255,6 +255,17 @@ class Reducer:
                                      stream()), "norm2_masked")
         return float(self.out[0].item())
 
+    def norm2_masked_pair(self, x, mask_a, mask_b):
+        """The sums of |x_i|^2 over mask_a's and over mask_b's rows, read in one transfer."""
+        R = x.shape[0]
+        wsp, wsn = self.ws.get(lib.paces_reduce_workspace_bytes(R))
+        for k, mask in enumerate((mask_a, mask_b)):
+            check(lib.paces_norm2_masked(ptr(torch.view_as_real(x)) if R else None,
+                                         ptr(mask) if R else None, R, ptr(self.out[k:]), wsp,
+                                         wsn, stream()), "norm2_masked")
+        h = self.out.tolist()
+        return h[0], h[1]
+
     def vdot(self, x, y):
         R = x.shape[0]
         wsp, wsn = self.ws.get(lib.paces_reduce_workspace_bytes(R))

@@ -1108,13 +1108,9 @@ static uint8_t* pairs_flags(void* ws, int64_t R, int nterms) {
 // sum is exact in FP64, so the result is the reference's rounding of the same expression.
 // ------------------------------------------------------------------------------------------------
 
-template <int W, typename D>
-__global__ void __launch_bounds__(kBlock) k_diag(const u64* __restrict__ rows, int64_t R,
-                                                 LayoutArg L, int pf, double eps, double hw,
-                                                 double de, D* __restrict__ diag) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= R) return;
-  const Row<W> r = load_row<W>(rows, i);
+template <int W>
+__device__ __forceinline__ double row_diag(const Row<W>& r, const LayoutArg& L, int pf,
+                                           double eps, double hw, double de) {
   // sum over all phonon fields as a weighted popcount of the bits of each significance
   long long s = 0;
 #pragma unroll
@@ -1123,7 +1119,16 @@ __global__ void __launch_bounds__(kBlock) k_diag(const u64* __restrict__ rows, i
       s += (long long)__popcll(r.w[w] & L.sum_mask[w * L.sum_bits + k]) << k;
   double d = eps + hw * (double)s;
   if (de != 0.0) d = d + de * (double)get_field<W>(r, L.off[pf], L.width[pf]);
-  diag[i] = (D)d;  // FP32: the FP64 value rounded to nearest
+  return d;
+}
+
+template <int W, typename D>
+__global__ void __launch_bounds__(kBlock) k_diag(const u64* __restrict__ rows, int64_t R,
+                                                 LayoutArg L, int pf, double eps, double hw,
+                                                 double de, D* __restrict__ diag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  diag[i] = (D)row_diag<W>(load_row<W>(rows, i), L, pf, eps, hw, de);  // FP32: rounded to nearest
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1361,20 +1366,25 @@ __global__ void __launch_bounds__(kBlock) k_zero_u64(u64* __restrict__ p, int64_
 }
 
 // par[i] = max |psi_j|^2 over the pairs {i, j} of the images (bit patterns of non-negative
-// doubles order like the values)
+// doubles order like the values). With image_side_only, only the image end of each pair (p) is
+// updated: the rows of the expanded basis (q) get no value (the added-rows form never reads them).
 __global__ void __launch_bounds__(kBlock) k_parent_amp(int64_t nbasis, int64_t nimg,
                                                        const int32_t* __restrict__ cand_pos,
                                                        const int32_t* __restrict__ src,
                                                        const double2* __restrict__ psi,
+                                                       int image_side_only,
                                                        u64* __restrict__ par) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nimg) return;
   const int32_t p = cand_pos[nbasis + e];
   const int32_t q = cand_pos[src[e]];
   if (p < 0 || q < 0) return;
-  const double2 a = psi[q], b = psi[p];
+  const double2 a = psi[q];
   atomicMax(&par[p], (u64)__double_as_longlong(a.x * a.x + a.y * a.y));
-  atomicMax(&par[q], (u64)__double_as_longlong(b.x * b.x + b.y * b.y));
+  if (!image_side_only) {
+    const double2 b = psi[p];
+    atomicMax(&par[q], (u64)__double_as_longlong(b.x * b.x + b.y * b.y));
+  }
 }
 
 __device__ __forceinline__ uint16_t feat_bf16(float f) {
@@ -1388,17 +1398,28 @@ __device__ __forceinline__ float feat_log_amp(double w2) {
   return (float)(l / 10.0);
 }
 
-template <int W>
+// ADDED: only the rows with in_basis[i] == 0, written at i - G[i] (G the exclusive prefix count of
+// in_basis), with row_index[i - G[i]] = i, and the diagonal computed from the row (hw, de, pf as
+// for k_diag) instead of read
+template <int W, bool ADDED>
 __global__ void __launch_bounds__(kBlock) k_features(const u64* __restrict__ rows, int64_t R,
                                                      const double2* __restrict__ psi,
                                                      const double* __restrict__ diag,
                                                      const u64* __restrict__ par, LayoutArg L,
                                                      int N, int periodic, double eps,
                                                      double inv_scale, float gfeat, float jfeat,
+                                                     const uint8_t* __restrict__ in_basis,
+                                                     const int32_t* __restrict__ G, int pf,
+                                                     double hw, double de,
+                                                     int32_t* __restrict__ row_index,
                                                      uint16_t* __restrict__ X) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= R) return;
+  if (ADDED && in_basis[i]) return;
   const Row<W> r = load_row<W>(rows, i);
+  const int64_t o = ADDED ? i - G[i] : i;
+  if (ADDED) row_index[o] = (int32_t)i;
+  const double dg = ADDED ? row_diag<W>(r, L, pf, eps, hw, de) : diag[i];
   const int pos = (int)get_field<W>(r, L.off[0], L.width[0]);
   const double2 a = psi[i];
   uint32_t packed[PACES_MLP_IN / 2];
@@ -1425,7 +1446,7 @@ __global__ void __launch_bounds__(kBlock) k_features(const u64* __restrict__ row
     } else if (c == 33) {
       v = feat_log_amp(__longlong_as_double((long long)par[i]));
     } else if (c == 34) {
-      v = (float)((diag[i] - eps) * inv_scale);
+      v = (float)((dg - eps) * inv_scale);
     } else if (c == 35) {
       v = feat_log_amp(a.x * a.x + a.y * a.y);
     } else if (c == 36) {
@@ -1437,7 +1458,7 @@ __global__ void __launch_bounds__(kBlock) k_features(const u64* __restrict__ row
     }
     packed[c >> 1] |= (uint32_t)feat_bf16(v) << (16 * (c & 1));
   }
-  uint4* dst = (uint4*)(X + i * PACES_MLP_IN);
+  uint4* dst = (uint4*)(X + o * PACES_MLP_IN);
 #pragma unroll
   for (int q = 0; q < PACES_MLP_IN / 8; ++q)
     dst[q] = make_uint4(packed[4 * q], packed[4 * q + 1], packed[4 * q + 2], packed[4 * q + 3]);
@@ -1446,10 +1467,17 @@ __global__ void __launch_bounds__(kBlock) k_features(const u64* __restrict__ row
 template <int W>
 static int run_features(const uint64_t* rows, int64_t R, const double* psi, const double* diag,
                         const u64* par, LayoutArg L, int N, int periodic, double eps,
-                        double inv_scale, float gfeat, float jfeat, uint16_t* X, hipStream_t st) {
-  hipLaunchKernelGGL(k_features<W>, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
-                     (const u64*)rows, R, (const double2*)psi, diag, par, L, N, periodic, eps,
-                     inv_scale, gfeat, jfeat, X);
+                        double inv_scale, float gfeat, float jfeat, const uint8_t* in_basis,
+                        const int32_t* G, int pf, double hw, double de, int32_t* row_index,
+                        uint16_t* X, hipStream_t st) {
+  if (in_basis)
+    hipLaunchKernelGGL((k_features<W, true>), dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
+                       (const u64*)rows, R, (const double2*)psi, diag, par, L, N, periodic, eps,
+                       inv_scale, gfeat, jfeat, in_basis, G, pf, hw, de, row_index, X);
+  else
+    hipLaunchKernelGGL((k_features<W, false>), dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
+                       (const u64*)rows, R, (const double2*)psi, diag, par, L, N, periodic, eps,
+                       inv_scale, gfeat, jfeat, in_basis, G, pf, hw, de, row_index, X);
   return launch_status();
 }
 
@@ -1873,6 +1901,45 @@ int paces_holstein_observables(const uint64_t* rows, const double* psi, int64_t 
 
 size_t paces_holstein_features_workspace_bytes(int64_t R) { return a256((size_t)(R > 0 ? R : 1) * 8); }
 
+static int features_common(const uint64_t* rows, int64_t R, const double* psi, const double* diag,
+                           int64_t nbasis, const int64_t* run_offsets, const int32_t* cand_pos,
+                           const int32_t* img_src, const uint8_t* in_basis,
+                           const paces_layout_t* lay, const paces_holstein_t* h, uint16_t* X,
+                           int32_t* row_index, void* ws, hipStream_t st) {
+  OpTable T;
+  int rc = build_image_ops(h, &T);
+  if (rc) return rc;
+  const int64_t nimg = run_offsets[T.n] - run_offsets[0];
+  if (nimg < 0) return PACES_E_INVALID;
+  if (nimg > 0 && (!cand_pos || !img_src)) return PACES_E_INVALID;
+  char* p = (char*)ws;
+  u64* par = (u64*)p;
+  p += a256((size_t)R * 8);
+  int32_t* G = nullptr;
+  if (in_basis) {
+    G = (int32_t*)p;
+    p += a256((size_t)(R + 1) * 4);
+    rc = scan_u8(in_basis, R, G, p, scan_ws_bytes(R), st);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st, par, R);
+  if (nimg > 0)
+    hipLaunchKernelGGL(k_parent_amp, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, nbasis,
+                       nimg, cand_pos, img_src, (const double2*)psi, in_basis ? 1 : 0, par);
+  rc = launch_status();
+  if (rc) return rc;
+  float gfeat = 0.f, jfeat = 0.f;
+  for (int t = h->nterms - 1; t >= 0; --t) {
+    if (h->terms[t].kind == PACES_TERM_VIB) gfeat = (float)h->terms[t].value;
+    if (h->terms[t].kind == PACES_TERM_HOP && h->terms[t].order == 1)
+      jfeat = (float)h->terms[t].value;
+  }
+  const double inv_scale = 1.0 / (h->hbar_omega * (double)h->nchain);
+  PACES_DISPATCH_W(lay->W, run_features, rows, R, psi, diag, (const u64*)par, larg(lay),
+                   h->nchain, h->periodic ? 1 : 0, h->eps_sys, inv_scale, gfeat, jfeat, in_basis,
+                   (const int32_t*)G, 0, h->hbar_omega, h->delta_eps, row_index, X, st);
+}
+
 int paces_holstein_features(const uint64_t* rows, int64_t R, const double* psi,
                             const double* diag, int64_t nbasis, const int64_t* run_offsets,
                             const int32_t* cand_pos, const int32_t* img_src,
@@ -1885,29 +1952,32 @@ int paces_holstein_features(const uint64_t* rows, int64_t R, const double* psi,
   if (R == 0) return PACES_OK;
   if (!rows || !psi || !diag || !X) return PACES_E_INVALID;
   if (ws_bytes < paces_holstein_features_workspace_bytes(R)) return PACES_E_WORKSPACE;
-  OpTable T;
-  rc = build_image_ops(h, &T);
+  return features_common(rows, R, psi, diag, nbasis, run_offsets, cand_pos, img_src, nullptr, lay,
+                         h, X, nullptr, ws, (hipStream_t)stream);
+}
+
+size_t paces_holstein_features_added_workspace_bytes(int64_t R) {
+  const int64_t n = R > 0 ? R : 1;
+  return a256((size_t)n * 8) + a256((size_t)(n + 1) * 4) + a256(scan_ws_bytes(n));
+}
+
+int paces_holstein_features_added(const uint64_t* rows, int64_t R, const double* psi,
+                                  int64_t nbasis, const int64_t* run_offsets,
+                                  const int32_t* cand_pos, const int32_t* img_src,
+                                  const uint8_t* in_basis, const paces_layout_t* lay,
+                                  const paces_holstein_t* h, uint16_t* X, int32_t* row_index,
+                                  void* ws, size_t ws_bytes, void* stream) {
+  int rc = check_h(lay, h);
   if (rc) return rc;
-  const int64_t nimg = run_offsets[T.n] - run_offsets[0];
-  if (nimg < 0) return PACES_E_INVALID;
-  if (nimg > 0 && (!cand_pos || !img_src)) return PACES_E_INVALID;
-  hipStream_t st = (hipStream_t)stream;
-  u64* par = (u64*)ws;
-  hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st, par, R);
-  if (nimg > 0)
-    hipLaunchKernelGGL(k_parent_amp, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, nbasis,
-                       nimg, cand_pos, img_src, (const double2*)psi, par);
-  rc = launch_status();
-  if (rc) return rc;
-  float gfeat = 0.f, jfeat = 0.f;
-  for (int t = h->nterms - 1; t >= 0; --t) {
-    if (h->terms[t].kind == PACES_TERM_VIB) gfeat = (float)h->terms[t].value;
-    if (h->terms[t].kind == PACES_TERM_HOP && h->terms[t].order == 1)
-      jfeat = (float)h->terms[t].value;
-  }
-  const double inv_scale = 1.0 / (h->hbar_omega * (double)h->nchain);
-  PACES_DISPATCH_W(lay->W, run_features, rows, R, psi, diag, (const u64*)par, larg(lay),
-                   h->nchain, h->periodic ? 1 : 0, h->eps_sys, inv_scale, gfeat, jfeat, X, st);
+  if (h->batched) return PACES_E_UNSUPPORTED;
+  if (R < 0 || nbasis < 0 || !run_offsets || !ws) return PACES_E_INVALID;
+  if (R >= 2147483647LL) return PACES_E_INVALID;
+  if (R == 0) return PACES_OK;
+  if (!rows || !psi || !in_basis || !X || !row_index) return PACES_E_INVALID;
+  if (!lay->d_sum_mask || lay->sum_bits < 1 || lay->sum_bits > 32) return PACES_E_INVALID;
+  if (ws_bytes < paces_holstein_features_added_workspace_bytes(R)) return PACES_E_WORKSPACE;
+  return features_common(rows, R, psi, nullptr, nbasis, run_offsets, cand_pos, img_src, in_basis,
+                         lay, h, X, row_index, ws, (hipStream_t)stream);
 }
 
 }  // extern "C"

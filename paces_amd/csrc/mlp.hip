@@ -14,9 +14,8 @@
 //   k_head_train    output layer, loss, and the gradient into the last hidden layer
 //   k_colsum / k_sum_parts   deterministic bias-gradient and split-K reductions
 //   k_adam          Adam update of the FP32 master and refresh of the BF16 copies
-//   k_mlp_infer<RB> all layers fused for a tile of 64 RB rows, activations kept in LDS
-//   k_mlp_infer_w<RB> the same with 64 x 128 wave tiles (two waves per 64 rows)
-//   k_mlp_infer_s   256 rows, 64 x 128 wave tiles, weight chunks staged once per workgroup in LDS
+//   k_mlp_infer<RB> all layers fused for a tile of 64 RB rows: activations kept in LDS, weight
+//                   chunks staged once per workgroup in LDS, transposed products
 
 #include "common.hpp"
 
@@ -41,23 +40,37 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
   return __builtin_bit_cast(uint16_t, (__bf16)f);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// two floats -> packed bf16 bits (lo = a), one v_cvt_pk_bf16_f32
+typedef __attribute__((ext_vector_type(2))) float f32x2;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+__device__ __forceinline__ uint32_t bf16_pair(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
 
 // GELU(a) = a Phi(a) with Phi the standard normal CDF, in a form without transcendental functions
-// (the epilogue of every layer evaluates it on each activation, and v_exp / v_rcp run at a quarter
-// of the VALU rate): Phi(a) = 0.5 + a P(a^2) for |a| <= 3.5 with P an even polynomial of degree 10
-// fitted to minimise the largest error of a Phi(a); above 3.5 GELU(a) = a, below -3.5 it is 0.
-// Largest absolute error against the erf form: 8.2e-4 (at |a| = 3.5), below half a bf16 unit of
-// the outputs it feeds. GELU(a) - GELU(-a) = a holds exactly in real arithmetic (P is even).
+// or branches (the epilogue of every layer evaluates it on each activation, and v_exp / v_rcp run
+// at a quarter of the VALU rate): GELU(a) = a / 2 + h(a) with h even, h = u P(u) (u = a^2) for
+// |a| <= 3.5 and h = |a| / 2 beyond. P has degree 5, fitted to minimise the largest error of GELU
+// under the constraint 3.5^2 P(3.5^2) = 3.5 / 2, so the pieces meet at |a| = 3.5 (a above, 0
+// below). With c = a clamped to [-3.5, 3.5] and u = c^2:
+//   GELU(a) = fma(1/2, |a| - |c|, fma(1/2, a, u P(u)))
+// in 11 operations (|a| - |c| is 0 inside and |a| - 3.5 beyond; no max, whose IEEE form would
+// add NaN-quieting operations). Largest absolute error against the erf form: 8.2e-4 (well below a bf16 unit of
+// the outputs it feeds); GELU(a) - GELU(-a) = a holds exactly in real arithmetic (P is even).
 constexpr float kGeluCut = 3.5f;
-constexpr float kG0 = 0.39749979972839355f, kG1 = -0.0638996884226799f,
-                kG2 = 0.008380419574677944f, kG3 = -0.0007098362548276782f,
-                kG4 = 3.406826363061555e-05f, kG5 = -6.95484573043359e-07f;
+constexpr float kG0 = 0.39518553018569946f, kG1 = -0.061023153364658356f,
+                kG2 = 0.0072402348741889f, kG3 = -0.0005149735952727497f,
+                kG4 = 1.9077817341894843e-05f, kG5 = -2.691017755296343e-07f;
 
 __device__ __forceinline__ float gelu_f(float a) {
-  const float u = a * a;
-  const float p = kG0 + u * (kG1 + u * (kG2 + u * (kG3 + u * (kG4 + u * kG5))));
-  const float g = a * (0.5f + a * p);
-  return a > kGeluCut ? a : (a < -kGeluCut ? 0.0f : g);
+  const float c = __builtin_amdgcn_fmed3f(a, -kGeluCut, kGeluCut);
+  const float u = c * c;
+  float p = fmaf(kG5, u, kG4);
+  p = fmaf(p, u, kG3);
+  p = fmaf(p, u, kG2);
+  p = fmaf(p, u, kG1);
+  p = fmaf(p, u, kG0);
+  return fmaf(0.5f, fabsf(a) - fabsf(c), fmaf(0.5f, a, p * u));
 }
 
 // GELU(a) = 0.5 a + sum c_k a^(2k+2), so GELU'(a) = 0.5 + a sum (2k + 2) c_k a^(2k)
@@ -353,201 +366,129 @@ __global__ void __launch_bounds__(256) k_adam(float* __restrict__ params,
   if (q >= 0) wbf[q] = f2bf(p);
 }
 
-// ---- fused inference: RB x 64 rows per workgroup of 4 RB waves, activations in LDS ----------
-// Wave w computes rows [64 (w / 4), +64) x columns [64 (w % 4), +64) of each layer; the two (RB = 2)
-// waves that share a column block read the same weight fragments, so L2 traffic per row halves.
-constexpr int kXs = IN + 8, kHs = HID + 8;  // padded LDS row lengths (16-byte aligned rows)
+// ---- fused inference: all layers for a tile of 64 RB rows, 2 RB waves ---------------------------
+// Wave w takes rows [64 (w / 2), +64) and outputs [128 (w % 2), +128) of each layer. Each 32-k
+// chunk of a layer's weights (256 outputs x 32 k, 16 KB) is loaded once per workgroup with
+// coalesced 16-byte loads into LDS and read there by every wave; the next chunk is loaded into
+// registers while the current one feeds the matrix cores. The activations of the tile stay in LDS
+// (row-major, the features in the first IN columns until the first layer replaces them):
+// 64 RB x 512 bytes, plus the 16 KB chunk.
+//
+// Each product is formed transposed, D[n][m] = sum_k W[n][k] H[m][k] (the weight fragment as the
+// matrix core's A operand), so a lane's four accumulator values are four consecutive outputs of
+// one row: the epilogue writes them as one 8-byte store. The accumulators start at the bias. The
+// head sums, per row, over j (16-output blocks) then r within a lane, then across the four lane
+// groups, then the two output halves: the same order for every RB, so the tiles give identical
+// outputs.
+//
+// LDS layouts without padding, swizzled at 16-byte granularity so that the eight lanes a
+// ds_read_b128 serves together (eight consecutive rows, one 16-byte column) fall in distinct banks:
+//   activations: row m of 256 values, 16-byte column q stored at column q ^ (m % 8)
+//   weight chunk: row n of 32 k values, 16-byte column q stored at column q ^ ((n / 2) % 4)
+// Unpadded, a 128-row tile and its weight chunk take 80 KB, so two such workgroups share a CU.
+__device__ __forceinline__ int hs_at(int m, int k) {
+  return m * HID + ((((k >> 3) ^ (m & 7))) << 3) + (k & 7);
+}
+__device__ __forceinline__ int wb_at(int n, int k) {
+  return n * 32 + ((((k >> 3) ^ ((n >> 1) & 3))) << 3) + (k & 7);
+}
 
-template <int K>
-__device__ __forceinline__ void infer_layer(const uint16_t* src, int ld,
-                                            const uint16_t* __restrict__ Wt, f32x4 (&acc)[4][4]) {
+template <int K, int NT>
+__device__ __forceinline__ void stage_chunk_load(const uint16_t* __restrict__ Wt, int c,
+                                                 u32x4 (&v)[1024 / NT]) {
+#pragma unroll
+  for (int q = 0; q < 1024 / NT; ++q) {
+    const int p = (int)threadIdx.x + NT * q, n = p >> 2, kk = (p & 3) * 8;
+    v[q] = *(const u32x4*)(Wt + (int64_t)n * K + 32 * c + kk);
+  }
+}
+
+template <int NT>
+__device__ __forceinline__ void stage_chunk_store(uint16_t* Wb, const u32x4 (&v)[1024 / NT]) {
+#pragma unroll
+  for (int q = 0; q < 1024 / NT; ++q) {
+    const int p = (int)threadIdx.x + NT * q, n = p >> 2, kk = (p & 3) * 8;
+    *(u32x4*)&Wb[wb_at(n, kk)] = v[q];
+  }
+}
+
+// acc[i][j] += W[128 ch + 16 j + .][:] . H[64 rb + 16 i + .][:] over K, weights through LDS
+template <int K, int NT>
+__device__ __forceinline__ void infer_layer(const uint16_t* Hs, uint16_t* Wb,
+                                            const uint16_t* __restrict__ Wt,
+                                            f32x4 (&acc)[4][8]) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int rb = w >> 2, cb = w & 3;
+  const int rb = w >> 1, ch = w & 1;
   const int lr = lane & 15, lk = 8 * (lane >> 4);
+  u32x4 nxt[1024 / NT];
+  stage_chunk_load<K, NT>(Wt, 0, nxt);
+  stage_chunk_store<NT>(Wb, nxt);
+  __syncthreads();
+  // hs_at(64 rb + 16 i + lr, 32 c + lk) = row base + 8 ((g ^ lr) % 4) + 32 (c ^ (lr % 8 / 4))
+  // + 4096 i (g = lane / 16): two per-lane bases (even and odd c) plus immediate offsets
+  const int m = 64 * rb + lr, g = lane >> 4, s2 = (lr >> 2) & 1;
+  const uint16_t* a_even = Hs + m * HID + 8 * ((g ^ lr) & 3) + 32 * s2;
+  const uint16_t* a_odd = Hs + m * HID + 8 * ((g ^ lr) & 3) - 32 * s2;
+  const uint16_t* bw = Wb + wb_at(128 * ch + lr, lk);  // + 512 j: rows 16 j further
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int c = 0; c < K / 32; ++c) {
+    if (c + 1 < K / 32) stage_chunk_load<K, NT>(Wt, c + 1, nxt);
+    bf16x8 a[4], b[8];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const uint16_t* wrow = Wt + (int64_t)(64 * cb + lr) * K + lk;
-  const uint16_t* arow = src + (64 * rb + lr) * ld + lk;
-  bf16x8 b[4], bn[4];
+    for (int i = 0; i < 4; ++i) a[i] = lds_frag((c & 1 ? a_odd : a_even) + 16 * i * HID + 32 * c);
 #pragma unroll
-  for (int j = 0; j < 4; ++j) b[j] = glb_frag(wrow + (int64_t)16 * j * K);
-#pragma unroll
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    if (k0 + 32 < K) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) bn[j] = glb_frag(wrow + (int64_t)16 * j * K + k0 + 32);
-    }
-    bf16x8 a[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) a[i] = lds_frag(arow + 16 * i * ld + k0);
+    for (int j = 0; j < 8; ++j) b[j] = lds_frag(bw + 16 * j * 32);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
-    if (k0 + 32 < K) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = bn[j];
+      for (int j = 0; j < 8; ++j) acc[i][j] = mfma(b[j], a[i], acc[i][j]);
+    __syncthreads();  // every wave has read this chunk (and, after the last, the activations)
+    if (c + 1 < K / 32) {
+      stage_chunk_store<NT>(Wb, nxt);
+      __syncthreads();
     }
   }
 }
 
-__device__ __forceinline__ void infer_store(f32x4 (&acc)[4][4], const float* __restrict__ bias,
-                                            uint16_t* Hs) {
+// accumulators := the bias of the outputs they hold (lane: outputs 16 j + 4 (lane / 16) + 0..3)
+__device__ __forceinline__ void infer_bias(const float* __restrict__ bias, f32x4 (&acc)[4][8]) {
+  const int lane = threadIdx.x & 63, ch = (threadIdx.x >> 6) & 1;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const f32x4 b = *(const f32x4*)(bias + 128 * ch + 16 * j + 4 * (lane >> 4));
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i][j] = b;
+  }
+}
+
+// H[m][n..n+3] = bf16(GELU(acc)), one 8-byte store per accumulator
+__device__ __forceinline__ void infer_store(const f32x4 (&acc)[4][8], uint16_t* Hs) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int rb = w >> 2, cb = w & 3;
+  const int rb = w >> 1, ch = w & 1;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int n = 64 * cb + 16 * j + (lane & 15);
-    const float bn = bias[n];
+  for (int i = 0; i < 4; ++i) {
+    const int m = 64 * rb + 16 * i + (lane & 15);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = 64 * rb + 16 * i + 4 * (lane >> 4) + r;
-        Hs[row * kHs + n] = f2bf(gelu_f(acc[i][j][r] + bn));
-      }
+    for (int j = 0; j < 8; ++j) {
+      const int n = 128 * ch + 16 * j + 4 * (lane >> 4);
+      uint2 v;
+      v.x = bf16_pair(gelu_f(acc[i][j][0]), gelu_f(acc[i][j][1]));
+      v.y = bf16_pair(gelu_f(acc[i][j][2]), gelu_f(acc[i][j][3]));
+      *(uint2*)&Hs[hs_at(m, n)] = v;
+    }
   }
 }
 
 template <int RB>
-__global__ void __launch_bounds__(256 * RB) k_mlp_infer(const uint16_t* __restrict__ X, int64_t n,
+__global__ void __launch_bounds__(128 * RB, 2) k_mlp_infer(const uint16_t* __restrict__ X, int64_t n,
                                                         const float* __restrict__ params,
                                                         const uint16_t* __restrict__ wbf,
                                                         float* __restrict__ prob,
                                                         float* __restrict__ logit) {
-  constexpr int ROWS = 64 * RB;
-  __shared__ __attribute__((aligned(16))) uint16_t Xs[ROWS * kXs];
-  __shared__ __attribute__((aligned(16))) uint16_t Hs[ROWS * kHs];
-  __shared__ float zp[4][ROWS];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int rb = w >> 2, cb = w & 3;
-  const int64_t m0 = (int64_t)blockIdx.x * ROWS;
-  for (int e = t; e < ROWS * IN / 8; e += 256 * RB) {
-    const int r = e / (IN / 8), c8 = (e % (IN / 8)) * 8;
-    u32x4 v = {0u, 0u, 0u, 0u};
-    if (m0 + r < n) v = *(const u32x4*)(X + (m0 + r) * IN + c8);
-    *(u32x4*)&Xs[r * kXs + c8] = v;
-  }
-  __syncthreads();
-  f32x4 acc[4][4];
-  infer_layer<IN>(Xs, kXs, wbf + BOFF_W1, acc);
-  infer_store(acc, params + OFF_B1, Hs);
-  __syncthreads();
-  infer_layer<HID>(Hs, kHs, wbf + BOFF_W2, acc);
-  __syncthreads();
-  infer_store(acc, params + OFF_B2, Hs);
-  __syncthreads();
-  infer_layer<HID>(Hs, kHs, wbf + BOFF_W3, acc);
-  // head: h3 = bf16(GELU(a3)); z = h3 . w4 + b4, summed over this wave's 64 columns
-  const float* b3 = params + OFF_B3;
-  const float* w4 = params + OFF_W4;
-  float part[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) part[i][r] = 0.f;
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int col = 64 * cb + 16 * j + (lane & 15);
-    const float bn = b3[col], wn = w4[col];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) part[i][r] += bf2f(f2bf(gelu_f(acc[i][j][r] + bn))) * wn;
-  }
-  // sum over the 16 lanes that share rows (lane bits 0-3)
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      float v = part[i][r];
-      v += __shfl_xor(v, 1, 64);
-      v += __shfl_xor(v, 2, 64);
-      v += __shfl_xor(v, 4, 64);
-      v += __shfl_xor(v, 8, 64);
-      if ((lane & 15) == 0) zp[cb][64 * rb + 16 * i + 4 * (lane >> 4) + r] = v;
-    }
-  __syncthreads();
-  if (t < ROWS && m0 + t < n) {
-    const float z = zp[0][t] + zp[1][t] + zp[2][t] + zp[3][t] + params[OFF_B4];
-    if (logit) logit[m0 + t] = z;
-    if (prob) prob[m0 + t] = 1.0f / (1.0f + __expf(-z));
-  }
-}
-
-// ---- fused inference, wide form: 2 RB waves over 64 RB rows ----------------------------------
-// Wave w takes rows [64 (w / 2), +64) and columns [128 (w % 2), +128) of each layer, so each weight
-// fragment feeds four row blocks and each activation fragment eight column blocks: per row, half
-// the weight traffic of the form above at equal rows per workgroup. The features sit in the first
-// IN columns of the activation buffer until the first layer's outputs replace them, so a 128-row
-// tile takes 68 KB of LDS (two workgroups per CU) and a 256-row tile 139 KB. Every sum is formed
-// in the same order as in k_mlp_infer, so the outputs are identical.
-template <int K>
-__device__ __forceinline__ void infer_layer_w(const uint16_t* src,
-                                              const uint16_t* __restrict__ Wt,
-                                              f32x4 (&acc)[4][8]) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int rb = w >> 1, ch = w & 1;
-  const int lr = lane & 15, lk = 8 * (lane >> 4);
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 8; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const uint16_t* wrow = Wt + (int64_t)(128 * ch + lr) * K + lk;
-  const uint16_t* arow = src + (64 * rb + lr) * kHs + lk;
-  bf16x8 b[8], bn[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) b[j] = glb_frag(wrow + (int64_t)16 * j * K);
-#pragma unroll
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    if (k0 + 32 < K) {
-#pragma unroll
-      for (int j = 0; j < 8; ++j) bn[j] = glb_frag(wrow + (int64_t)16 * j * K + k0 + 32);
-    }
-    bf16x8 a[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) a[i] = lds_frag(arow + 16 * i * kHs + k0);
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 8; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
-    if (k0 + 32 < K) {
-#pragma unroll
-      for (int j = 0; j < 8; ++j) b[j] = bn[j];
-    }
-  }
-}
-
-__device__ __forceinline__ void infer_store_w(f32x4 (&acc)[4][8], const float* __restrict__ bias,
-                                              uint16_t* Hs) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int rb = w >> 1, ch = w & 1;
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int col = 128 * ch + 16 * j + (lane & 15);
-    const float bn = bias[col];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = 64 * rb + 16 * i + 4 * (lane >> 4) + r;
-        Hs[row * kHs + col] = f2bf(gelu_f(acc[i][j][r] + bn));
-      }
-  }
-}
-
-template <int RB>
-__global__ void __launch_bounds__(128 * RB, 2) k_mlp_infer_w(const uint16_t* __restrict__ X,
-                                                          int64_t n,
-                                                          const float* __restrict__ params,
-                                                          const uint16_t* __restrict__ wbf,
-                                                          float* __restrict__ prob,
-                                                          float* __restrict__ logit) {
   constexpr int ROWS = 64 * RB, NT = 128 * RB;
-  __shared__ __attribute__((aligned(16))) uint16_t Hs[ROWS * kHs];
-  __shared__ float zp[4][ROWS];  // per 64-column block, as in k_mlp_infer
+  __shared__ __attribute__((aligned(16))) uint16_t Hs[ROWS * HID];
+  __shared__ __attribute__((aligned(16))) uint16_t Wb[256 * 32];
+  float* zp = (float*)Wb;  // [2][ROWS] after the last layer (at most 2 KB of the chunk's 16 KB)
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int rb = w >> 1, ch = w & 1;
   const int64_t m0 = (int64_t)blockIdx.x * ROWS;
@@ -555,189 +496,41 @@ __global__ void __launch_bounds__(128 * RB, 2) k_mlp_infer_w(const uint16_t* __r
     const int r = e / (IN / 8), c8 = (e % (IN / 8)) * 8;
     u32x4 v = {0u, 0u, 0u, 0u};
     if (m0 + r < n) v = *(const u32x4*)(X + (m0 + r) * IN + c8);
-    *(u32x4*)&Hs[r * kHs + c8] = v;
+    *(u32x4*)&Hs[hs_at(r, c8)] = v;
   }
-  __syncthreads();
   f32x4 acc[4][8];
-  infer_layer_w<IN>(Hs, wbf + BOFF_W1, acc);
-  __syncthreads();  // every wave has read the features before the outputs replace them
-  infer_store_w(acc, params + OFF_B1, Hs);
+  infer_bias(params + OFF_B1, acc);
+  infer_layer<IN, NT>(Hs, Wb, wbf + BOFF_W1, acc);  // its first barrier also orders the features
+  infer_store(acc, Hs);
   __syncthreads();
-  infer_layer_w<HID>(Hs, wbf + BOFF_W2, acc);
+  infer_bias(params + OFF_B2, acc);
+  infer_layer<HID, NT>(Hs, Wb, wbf + BOFF_W2, acc);
+  infer_store(acc, Hs);
   __syncthreads();
-  infer_store_w(acc, params + OFF_B2, Hs);
-  __syncthreads();
-  infer_layer_w<HID>(Hs, wbf + BOFF_W3, acc);
-  const float* b3 = params + OFF_B3;
+  infer_bias(params + OFF_B3, acc);
+  infer_layer<HID, NT>(Hs, Wb, wbf + BOFF_W3, acc);
+  // head: z = bf16(GELU(a3)) . w4 + b4
   const float* w4 = params + OFF_W4;
-  // z over each 64-column block (blocks 2 ch and 2 ch + 1), in k_mlp_infer's order
-  float part[2][4][4];
-#pragma unroll
-  for (int h = 0; h < 2; ++h)
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) part[h][i][r] = 0.f;
+  float part[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    const int col = 128 * ch + 16 * j + (lane & 15);
-    const float bn = b3[col], wn = w4[col];
+    const f32x4 wn = *(const f32x4*)(w4 + 128 * ch + 16 * j + 4 * (lane >> 4));
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        part[j >> 2][i][r] += bf2f(f2bf(gelu_f(acc[i][j][r] + bn))) * wn;
+      for (int r = 0; r < 4; ++r) part[i] += bf2f(f2bf(gelu_f(acc[i][j][r]))) * wn[r];
   }
 #pragma unroll
-  for (int h = 0; h < 2; ++h)
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        float v = part[h][i][r];
-        v += __shfl_xor(v, 1, 64);
-        v += __shfl_xor(v, 2, 64);
-        v += __shfl_xor(v, 4, 64);
-        v += __shfl_xor(v, 8, 64);
-        if ((lane & 15) == 0) zp[2 * ch + h][64 * rb + 16 * i + 4 * (lane >> 4) + r] = v;
-      }
+  for (int i = 0; i < 4; ++i) {
+    float v = part[i];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    if (lane < 16) zp[ch * ROWS + 64 * rb + 16 * i + lane] = v;
+  }
   __syncthreads();
   for (int r = t; r < ROWS; r += NT) {
     if (m0 + r >= n) continue;
-    const float z = zp[0][r] + zp[1][r] + zp[2][r] + zp[3][r] + params[OFF_B4];
-    if (logit) logit[m0 + r] = z;
-    if (prob) prob[m0 + r] = 1.0f / (1.0f + __expf(-z));
-  }
-}
-
-// ---- fused inference, staged form: 256 rows, weights through LDS --------------------------------
-// Eight waves over 256 rows (wave w: rows [64 (w / 2), +64), columns [128 (w % 2), +128), as in
-// k_mlp_infer_w). Each 32-k chunk of a layer's weights (256 outputs x 32 k) is loaded once per
-// workgroup with coalesced 16-byte loads, stored to LDS and read there by all eight waves, so a
-// weight byte crosses the CU's load path once per 256 rows; the next chunk is loaded into
-// registers while the current one feeds the matrix cores. LDS: activations 135 KB + one weight
-// chunk 20 KB (the head's partial sums reuse the chunk's space). Same sums, same order: outputs
-// identical to k_mlp_infer.
-constexpr int kWs = 40;  // LDS row of a weight chunk: 32 k + padding (80 bytes, 16-byte aligned)
-
-template <int K>
-__device__ __forceinline__ void stage_chunk_load(const uint16_t* __restrict__ Wt, int c,
-                                                 u32x4 (&v)[2]) {
-#pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int p = (int)threadIdx.x + 512 * q, n = p >> 2, kk = (p & 3) * 8;
-    v[q] = *(const u32x4*)(Wt + (int64_t)n * K + 32 * c + kk);
-  }
-}
-
-__device__ __forceinline__ void stage_chunk_store(uint16_t* Wb, const u32x4 (&v)[2]) {
-#pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int p = (int)threadIdx.x + 512 * q, n = p >> 2, kk = (p & 3) * 8;
-    *(u32x4*)&Wb[n * kWs + kk] = v[q];
-  }
-}
-
-template <int K>
-__device__ __forceinline__ void infer_layer_s(const uint16_t* Hs, uint16_t* Wb,
-                                              const uint16_t* __restrict__ Wt,
-                                              f32x4 (&acc)[4][8]) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int rb = w >> 1, ch = w & 1;
-  const int lr = lane & 15, lk = 8 * (lane >> 4);
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 8; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  u32x4 nxt[2];
-  stage_chunk_load<K>(Wt, 0, nxt);
-  stage_chunk_store(Wb, nxt);
-  __syncthreads();
-  const uint16_t* arow = Hs + (64 * rb + lr) * kHs + lk;
-  const uint16_t* brow = Wb + (128 * ch + lr) * kWs + lk;
-#pragma unroll
-  for (int c = 0; c < K / 32; ++c) {
-    if (c + 1 < K / 32) stage_chunk_load<K>(Wt, c + 1, nxt);
-    bf16x8 a[4], b[8];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) a[i] = lds_frag(arow + 16 * i * kHs + 32 * c);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) b[j] = lds_frag(brow + 16 * j * kWs);
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 8; ++j) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
-    __syncthreads();  // every wave has read this chunk
-    if (c + 1 < K / 32) {
-      stage_chunk_store(Wb, nxt);
-      __syncthreads();
-    }
-  }
-}
-
-__global__ void __launch_bounds__(512, 2) k_mlp_infer_s(const uint16_t* __restrict__ X,
-                                                        int64_t n,
-                                                        const float* __restrict__ params,
-                                                        const uint16_t* __restrict__ wbf,
-                                                        float* __restrict__ prob,
-                                                        float* __restrict__ logit) {
-  constexpr int ROWS = 256;
-  __shared__ __attribute__((aligned(16))) uint16_t Hs[ROWS * kHs];
-  __shared__ __attribute__((aligned(16))) uint16_t Wb[256 * kWs];
-  float* zp = (float*)Wb;  // [4][ROWS] after the last layer (4 KB of the chunk's 20 KB)
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int rb = w >> 1, ch = w & 1;
-  const int64_t m0 = (int64_t)blockIdx.x * ROWS;
-  for (int e = t; e < ROWS * IN / 8; e += 512) {
-    const int r = e / (IN / 8), c8 = (e % (IN / 8)) * 8;
-    u32x4 v = {0u, 0u, 0u, 0u};
-    if (m0 + r < n) v = *(const u32x4*)(X + (m0 + r) * IN + c8);
-    *(u32x4*)&Hs[r * kHs + c8] = v;
-  }
-  f32x4 acc[4][8];
-  infer_layer_s<IN>(Hs, Wb, wbf + BOFF_W1, acc);  // its first barrier also orders the features
-  infer_store_w(acc, params + OFF_B1, Hs);
-  __syncthreads();
-  infer_layer_s<HID>(Hs, Wb, wbf + BOFF_W2, acc);
-  infer_store_w(acc, params + OFF_B2, Hs);
-  __syncthreads();
-  infer_layer_s<HID>(Hs, Wb, wbf + BOFF_W3, acc);
-  const float* b3 = params + OFF_B3;
-  const float* w4 = params + OFF_W4;
-  float part[2][4][4];
-#pragma unroll
-  for (int h = 0; h < 2; ++h)
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) part[h][i][r] = 0.f;
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int col = 128 * ch + 16 * j + (lane & 15);
-    const float bn = b3[col], wn = w4[col];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        part[j >> 2][i][r] += bf2f(f2bf(gelu_f(acc[i][j][r] + bn))) * wn;
-  }
-#pragma unroll
-  for (int h = 0; h < 2; ++h)
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        float v = part[h][i][r];
-        v += __shfl_xor(v, 1, 64);
-        v += __shfl_xor(v, 2, 64);
-        v += __shfl_xor(v, 4, 64);
-        v += __shfl_xor(v, 8, 64);
-        if ((lane & 15) == 0) zp[(2 * ch + h) * ROWS + 64 * rb + 16 * i + 4 * (lane >> 4) + r] = v;
-      }
-  __syncthreads();
-  for (int r = t; r < ROWS; r += 512) {
-    if (m0 + r >= n) continue;
-    const float z = zp[r] + zp[ROWS + r] + zp[2 * ROWS + r] + zp[3 * ROWS + r] + params[OFF_B4];
+    const float z = zp[r] + zp[ROWS + r] + params[OFF_B4];
     if (logit) logit[m0 + r] = z;
     if (prob) prob[m0 + r] = 1.0f / (1.0f + __expf(-z));
   }
@@ -747,7 +540,7 @@ __global__ void __launch_bounds__(512, 2) k_mlp_infer_s(const uint16_t* __restri
 static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 constexpr int64_t kSplitK = 512;  // batch rows per weight-gradient partial
-constexpr int kInferTileDefault = 256;  // rows per workgroup of the fused inference
+constexpr int kInferTileDefault = 128;  // rows per workgroup of the fused inference (two per CU)
 
 struct TrainWs {
   float *a1, *a2, *a3;
@@ -975,14 +768,14 @@ int paces_mlp_infer_tiled(const uint16_t* X, int64_t n, const float* params,
   if (n == 0) return PACES_OK;
   if (!X) return PACES_E_INVALID;
   hipStream_t st = (hipStream_t)stream;
-  if (tile_rows == 256)  // weights staged through LDS
-    hipLaunchKernelGGL(k_mlp_infer_s, dim3(grid_for(n, 256)), dim3(512), 0, st, X, n, params,
+  if (tile_rows == 256)
+    hipLaunchKernelGGL(k_mlp_infer<4>, dim3(grid_for(n, 256)), dim3(512), 0, st, X, n, params,
                        wbf16, prob, logit);
   else if (tile_rows == 128)
-    hipLaunchKernelGGL(k_mlp_infer_w<2>, dim3(grid_for(n, 128)), dim3(256), 0, st, X, n, params,
+    hipLaunchKernelGGL(k_mlp_infer<2>, dim3(grid_for(n, 128)), dim3(256), 0, st, X, n, params,
                        wbf16, prob, logit);
   else
-    hipLaunchKernelGGL(k_mlp_infer<1>, dim3(grid_for(n, 64)), dim3(256), 0, st, X, n, params,
+    hipLaunchKernelGGL(k_mlp_infer<1>, dim3(grid_for(n, 64)), dim3(128), 0, st, X, n, params,
                        wbf16, prob, logit);
   return launch_status();
 }

@@ -674,6 +674,20 @@ __global__ void __launch_bounds__(kBlock) k_prune_plan(const float* __restrict__
   keep[i] = k;
 }
 
+// the added rows' decisions scattered to their union positions (the rest keep in_basis's 1)
+__global__ void __launch_bounds__(kBlock) k_prune_rows(const float* __restrict__ prob,
+                                                       const int32_t* __restrict__ row_index,
+                                                       int64_t m, float thr, int sample,
+                                                       uint64_t smask, uint64_t seed,
+                                                       uint8_t* __restrict__ keep) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const int64_t i = row_index[j];
+  uint8_t k = prob[j] >= thr ? 1 : 0;
+  if (!k && sample && (fmix64((uint64_t)i ^ seed) & smask) == 0) k = 2;
+  keep[i] = k;
+}
+
 __global__ void __launch_bounds__(kBlock) k_force_keep(const int32_t* __restrict__ idx, int64_t m,
                                                        uint8_t* __restrict__ keep) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -724,6 +738,13 @@ __global__ void __launch_bounds__(kBlock) k_mark_indices(const int32_t* __restri
   if (j >= m) return;
   const int32_t i = idx[j];
   out[i] = src ? src[i] : 1;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scatter_u8(const int32_t* __restrict__ idx, int64_t m,
+                                                       const uint8_t* __restrict__ src,
+                                                       uint8_t* __restrict__ out) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m) out[idx[j]] = src[j];
 }
 
 size_t compact_ws_bytes(int64_t n) {
@@ -924,6 +945,32 @@ int paces_prune_plan(const float* prob, int64_t n, float threshold, const int32_
   if (nforced > 0)
     hipLaunchKernelGGL(k_force_keep, dim3(grid_for(nforced, kBlock)), dim3(kBlock), 0, st,
                        forced_idx, nforced, keep);
+  return launch_status();
+}
+
+int paces_prune_plan_rows(const float* prob, const int32_t* row_index, int64_t m,
+                          const uint8_t* in_basis, int64_t n, float threshold,
+                          int32_t sample_bits, uint64_t seed, uint8_t* keep, void* stream) {
+  if (n < 0 || m < 0 || m > n || sample_bits > 62) return PACES_E_INVALID;
+  if (n > 0 && (!in_basis || !keep)) return PACES_E_INVALID;
+  if (m > 0 && (!prob || !row_index)) return PACES_E_INVALID;
+  if (n == 0) return PACES_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (hipMemcpyAsync(keep, in_basis, (size_t)n, hipMemcpyDeviceToDevice, st) != hipSuccess)
+    return PACES_E_HIP;
+  if (m == 0) return PACES_OK;
+  const uint64_t smask = sample_bits > 0 ? ((1ULL << sample_bits) - 1ULL) : 0ULL;
+  hipLaunchKernelGGL(k_prune_rows, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, st, prob,
+                     row_index, m, threshold, sample_bits >= 0 ? 1 : 0, smask, seed, keep);
+  return launch_status();
+}
+
+int paces_scatter_u8(const int32_t* idx, int64_t m, const uint8_t* src, uint8_t* out,
+                     void* stream) {
+  if (m < 0 || (m > 0 && (!idx || !src || !out))) return PACES_E_INVALID;
+  if (m == 0) return PACES_OK;
+  hipLaunchKernelGGL(k_scatter_u8, dim3(grid_for(m, kBlock)), dim3(kBlock), 0, (hipStream_t)stream,
+                     idx, m, src, out);
   return launch_status();
 }
 

@@ -223,7 +223,8 @@ class Hamiltonian(HamiltonianFramework):
         return None
 
     def generate_mel(self, basis_states, enlarge_steps=0, log_ind=1.2,
-                     value_dtype=torch.float64, diag_dtype=torch.float64, first=None):
+                     value_dtype=torch.float64, diag_dtype=torch.float64, first=None,
+                     want_start_pos=False):
         """((diag, new_inds), {term: CsrMatrix}, dbg) for the basis (hamiltonian.py:383-492).
 
         new_inds = the basis (after ``enlarge_steps`` expansions) plus its neighbours; each term's
@@ -235,6 +236,8 @@ class Hamiltonian(HamiltonianFramework):
         first=(expanded basis, n_prev, run offsets, src, nup, union positions, from0): an
         expansion already made by the caller (the learned step's, with the rows it pruned
         removed: their union positions are -1), continued here with enlarge_steps more.
+        want_start_pos: set self.last_start_pos, the position in new_inds of each row of the basis
+        the expansions started from (int32, device).
         """
         if basis_states.shape[1] != self.totwordwidth:
             raise ValueError("Shape of basis states does not match the specified number of bits.")
@@ -244,6 +247,12 @@ class Hamiltonian(HamiltonianFramework):
         if first is not None:
             basis, n_prev, offs_f, src_f, nup_f, cpos_f, from0_f = first
             last = (n_prev, offs_f, src_f, nup_f, cpos_f, from0_f)
+        start_pos = None  # positions of the starting rows in the current basis (None: identity)
+
+        def follow(cpos, nb):
+            # run 0 of a merge is the current basis: its rows' union positions lead the cand_pos
+            return cpos[:nb] if start_pos is None else cpos.index_select(0, start_pos.long())
+
         for _ in range(enlarge_steps):
             imgs, offs, src, nup = self._images(basis, want_src=True, slot="expand")
             runs = [(basis, 0, basis.shape[0])]
@@ -251,6 +260,8 @@ class Hamiltonian(HamiltonianFramework):
             rows, from0, cpos = engine.merge_unique(runs, self.totwordwidth, self._ws,
                                                     want_from0=True, want_cand_pos=True)
             last = (basis.shape[0], offs, src, nup, cpos, from0)
+            if want_start_pos:
+                start_pos = follow(cpos, basis.shape[0])
             basis = rows
         if last is None:
             imgs, offs, src, nup = self._images(basis, want_src=True)
@@ -273,6 +284,8 @@ class Hamiltonian(HamiltonianFramework):
                 runs, self.totwordwidth, self._ws, want_from0=True, want_cand_pos=True,
                 diag=(self._layout, self._hs, diag_dtype))
         self._check_basis_size(new_inds.shape[0])
+        if want_start_pos:
+            self.last_start_pos = follow(cand_pos, basis.shape[0])
         mats = self._pairs_csr(new_inds.shape[0], basis.shape[0], offs, cand_pos, in_basis, src,
                                nup, basis, value_dtype, rows=new_inds, prior=prior)
         dbg = {}
@@ -479,6 +492,23 @@ class Hamiltonian(HamiltonianFramework):
                                            C.byref(self._hs), ptr(X), wsp, wsn, stream()),
               "holstein_features")
         return X[:R]
+
+    def features_added(self, rows, psi, nbasis, offs, cand_pos, src, in_basis, nadd):
+        """Scorer inputs of the union rows outside the expanded basis only
+        (paces_holstein_features_added): ((nadd, 64) bf16 bits in int16, int32 union row of each).
+        in_basis: the union's 0/1 flags of the basis rows, nadd = rows - sum(in_basis) (known to
+        the caller); the diagonal is computed from the rows."""
+        R = rows.shape[0]
+        X = torch.empty((max(nadd, 1), 64), dtype=torch.int16, device=self.device)
+        idx = torch.empty(max(nadd, 1), dtype=torch.int32, device=self.device)
+        run_off = (C.c_int64 * len(offs))(*offs)
+        wsp, wsn = self._ws.get(_lib.paces_holstein_features_added_workspace_bytes(R))
+        check(_lib.paces_holstein_features_added(
+            ptr(rows) if R else None, R, ptr(torch.view_as_real(psi)) if R else None, nbasis,
+            run_off, ptr(cand_pos), ptr(src), ptr(in_basis) if R else None,
+            C.byref(self._layout), C.byref(self._hs), ptr(X), ptr(idx), wsp, wsn, stream()),
+            "holstein_features_added")
+        return X[:nadd], idx[:nadd]
 
     # ---- plugin methods with the reference's contract (holstein.py:151-277) ----------------------
     def _term_images(self, basis, name):
@@ -778,9 +808,9 @@ class TimeEvolution(TimeEvolutionFramework):
 class LabelLog:
     """Retained/discarded labels of candidate rows, logged from steps that keep every row.
 
-    At step i the union rows outside the selection are staged with their scorer features; at
-    step i+1 each staged row is labelled 1 when the new selection kept it, else 0. Labels and
-    features accumulate on the device up to ``capacity`` rows."""
+    At step i the union rows outside the selection (sorted) are staged with their scorer
+    features; at step i+1 each staged row is labelled 1 when the new selection kept it, else 0.
+    Labels and features accumulate on the device up to ``capacity`` rows."""
 
     def __init__(self, device, capacity):
         self.device = device
@@ -794,13 +824,9 @@ class LabelLog:
     def full(self):
         return self.n >= self.capacity
 
-    def stage(self, union_rows, X, select):
-        if self.full:
-            self._pending = None
-            return
-        _, hit, _ = engine.lower_bound(select, union_rows, want_hit=True)
-        cand = torch.nonzero(hit == 0).flatten()
-        self._pending = (union_rows.index_select(0, cand), X.index_select(0, cand))
+    def stage(self, rows, X):
+        """Stage candidate rows (sorted, outside the selection) with their features."""
+        self._pending = None if self.full else (rows, X)
 
     def resolve(self, select):
         if self._pending is None:
@@ -864,11 +890,12 @@ class LearnedTimeEvolution(TimeEvolution):
             return sel
         return super()._determine_select_whoami(vector, use_weight_function, tau, garbage_tol)
 
-    def _build(self, first, enlarge_steps):
+    def _build(self, first, enlarge_steps, want_start_pos=False):
         """The rows, CSR and carried amplitudes of the step from its first expansion (see
         Hamiltonian.generate_mel's first); returns the debug columns."""
         ham = self.hamobj
-        (diag, new_inds), mats, dbg = ham.generate_mel(first[0], enlarge_steps, first=first)
+        (diag, new_inds), mats, dbg = ham.generate_mel(first[0], enlarge_steps, first=first,
+                                                       want_start_pos=want_start_pos)
         self.numstates = new_inds.shape[0]
         self.create_matrices(diag, mats)
         self.vector = engine.reassign(self.whoami, self.vector, new_inds, self._ws)
@@ -890,17 +917,18 @@ class LearnedTimeEvolution(TimeEvolution):
                                                   want_cand_pos=True)
         nB = B.shape[0]
         nsel = select.shape[0]
+        nadd = nB - nsel  # every selected row is a union row
         full = (B, nsel, offs, src, nup, cand_pos, in_sel)
         want_x = self.scorer is not None or (self.label_log is not None and
                                              not self.label_log.full)
-        X = None
-        if want_x:
+        X = idx = None
+        if want_x and nadd:
+            # features of the added rows only, in union order, with their union positions
             psiB = engine.reassign(self.whoami, self.vector, B, self._ws)
-            X = ham.features(B, psiB, ham.generate_mel_diag(B), select.shape[0], offs, cand_pos,
-                             src)
+            X, idx = ham.features_added(B, psiB, nsel, offs, cand_pos, src, in_sel, nadd)
             del psiB
         if self.label_log is not None and X is not None:
-            self.label_log.stage(B, X, select)
+            self.label_log.stage(B.index_select(0, idx), X)
         st = {"rows_first": nB, "scored": 0, "kept": 0, "sampled": 0, "rows": 0,
               "dropped_weight": 0.0, "dropped_retained_weight": 0.0, "fallback": False}
         self._sample = None
@@ -908,30 +936,27 @@ class LearnedTimeEvolution(TimeEvolution):
         if self.scorer is None:
             dbg = self._build(full, enlarge_steps - 1)
         else:
-            added = torch.nonzero(in_sel == 0).flatten()
-            prob = torch.ones(max(nB, 1), dtype=torch.float32, device=self.device)
-            if added.numel():
-                Xa = X.index_select(0, added)
+            prob = None
+            if nadd:
                 tuner = self._tuner()
                 if tuner is not None:
                     from ..core import autotune
                     self.scorer.tile_rows = tuner.pick(
-                        tuner.key("mlp", Xa.shape[0]), autotune.MLP_TILES,
-                        lambda t: (lambda: self.scorer.infer(Xa, tile_rows=t)))
-                prob.index_copy_(0, added, self.scorer.infer(Xa))
-                del Xa
+                        tuner.key("mlp", nadd), autotune.MLP_TILES,
+                        lambda t: (lambda: self.scorer.infer(X, tile_rows=t)))
+                prob = self.scorer.infer(X)
             del X
-            forced = torch.nonzero(in_sel).flatten().to(torch.int32)
             keep = torch.empty(max(nB, 1), dtype=torch.uint8, device=self.device)
             # sample 2^-bits of the dropped rows, bits lowered on small expansions so the sample
             # holds a few hundred rows
             bits = max(0, min(self.sample_bits, int(math.log2(max(nB, 1) / 4096.0))
                               if nB >= 4096 else 0))
-            check(_lib.paces_prune_plan(ptr(prob), nB, self.threshold,
-                                        ptr(forced) if forced.numel() else None, forced.numel(),
-                                        bits,
-                                        (self._step_index + 1) * 0x9E3779B97F4A7C15 % (1 << 64),
-                                        ptr(keep), stream()), "prune_plan")
+            check(_lib.paces_prune_plan_rows(ptr(prob) if nadd else None,
+                                             ptr(idx) if nadd else None, nadd, ptr(in_sel), nB,
+                                             self.threshold, bits,
+                                             (self._step_index + 1) * 0x9E3779B97F4A7C15
+                                             % (1 << 64), ptr(keep), stream()), "prune_plan_rows")
+            del prob, idx
             # the kept rows in order, the selection's flags and the union positions renumbered
             # (-1 for a pruned row: its pairs are not formed)
             nC = cand_pos.shape[0]
@@ -947,19 +972,19 @@ class LearnedTimeEvolution(TimeEvolution):
                   "compact_kept")
             nK = int(cnt.item())
             rows_k, inb_k, smp_k = rows_k[:nK], inb_k[:nK], smp_k[:nK]
-            sampled = rows_k[smp_k.bool()] if nK else rows_k
-            st.update(scored=nB - nsel, kept=nK - nsel, sampled=int(sampled.shape[0]))
+            # the number of sampled rows reaches the host while the step runs on
+            nsample = _to_host_async(smp_k.sum()) if nK else None
+            st.update(scored=nadd, kept=nK - nsel)
             self._full = (full, enlarge_steps - 1, self.whoami, self.vector)
-            dbg = self._build((rows_k, nsel, offs, src, nup, pos_k, inb_k), enlarge_steps - 1)
-            # the sampled rows' positions in the new basis
+            dbg = self._build((rows_k, nsel, offs, src, nup, pos_k, inb_k), enlarge_steps - 1,
+                              want_start_pos=True)
+            # the sampled rows' flags at their positions in the new basis (every kept row is a
+            # row of it)
             mark = torch.zeros(max(self.numstates, 1), dtype=torch.uint8, device=self.device)
-            if sampled.shape[0]:
-                idx, _, _ = engine.lower_bound(self.whoami, sampled, want_hit=False,
-                                               q_sorted=True)
-                # every kept row is a row of the new basis
-                check(_lib.paces_mark_indices(ptr(idx), idx.shape[0], None, ptr(mark),
-                                              stream()), "mark_indices")
-            self._sample = (mark, int(sampled.shape[0]), nB - nK + int(sampled.shape[0]))
+            if nK:
+                check(_lib.paces_scatter_u8(ptr(ham.last_start_pos), nK, ptr(smp_k), ptr(mark),
+                                            stream()), "scatter_u8")
+            self._sample = (mark, nsample, nB - nK)
         st["rows"] = self.numstates
         self.stats.append(st)
         return dbg, self._post_assignment_diagnostics()
@@ -968,8 +993,15 @@ class LearnedTimeEvolution(TimeEvolution):
         """Estimate, from the sampled dropped rows, the weight of dropped rows that the next
         selection would keep; fall back to the whole first expansion when it exceeds
         fallback_tol."""
-        smp, nsample, ndropped = self._sample
+        smp, nsample_h, nd_unsampled = self._sample
+        nsample = 0
+        if nsample_h is not None:
+            host, ev = nsample_h
+            ev.synchronize()
+            nsample = int(host.item())
+        ndropped = nd_unsampled + nsample  # the pruned rows and the sampled ones
         st = self.stats[-1]
+        st["sampled"] = nsample
         if ndropped == 0:
             return vec, expm_dbg
         R = self.whoami.shape[0]
@@ -980,8 +1012,9 @@ class LearnedTimeEvolution(TimeEvolution):
             mark = torch.zeros(R, dtype=torch.uint8, device=self.device)
             check(_lib.paces_mark_indices(ptr(sidx), sidx.shape[0], ptr(smp), ptr(mark),
                                           stream()), "mark_indices")
-            st["dropped_weight"] = self._reducer.norm2_masked(vec, smp) * scale
-            est = self._reducer.norm2_masked(vec, mark) * scale
+            w_smp, w_kept = self._reducer.norm2_masked_pair(vec, smp, mark)
+            st["dropped_weight"] = w_smp * scale
+            est = w_kept * scale
         else:
             est = float("inf")  # rows were dropped but none sampled: no estimate
         st["dropped_retained_weight"] = est

@@ -3,7 +3,8 @@
 * Features (paces_holstein_features) are restated in NumPy from the unpacked union rows, the
   carried amplitudes and the per-term CSR pattern. Integer-derived columns are bit-exact; the
   log-amplitude columns are within one bf16 ulp, since device and NumPy log10 may differ in the
-  last double ulp.
+  last double ulp. The added-rows form (what the step scores) equals them at the added rows.
+* The plan from the added rows' scores equals the union plan with the basis rows forced.
 * Compaction (paces_prune_plan, paces_compact_kept) is checked against NumPy boolean indexing.
 * With no scorer the learned step keeps every row and equals the plain step bit for bit.
 * Ablation on the 16-site chain (cap 7, maxstates 2e4):
@@ -129,6 +130,69 @@ def test_features_match_numpy(env, case):
         tol = np.abs(exp[:, c]) * 2.0 ** -7
         assert np.all(np.abs(got[:, c] - exp[:, c]) <= tol), c
     assert np.any(par > 0) and np.any(exp[:, 33] > -2.0)
+
+
+@pytest.mark.parametrize("case", ["chain16", "ring24"])
+def test_features_of_added_rows_equal_full_features(env, case):
+    """paces_holstein_features_added = the full-union features at the rows outside the basis,
+    bit for bit (an added row is never the basis end of an image pair, and the inline diagonal is
+    k_diag's), with their union positions in order."""
+    torch = env["torch"]
+    N, dim, periodic, g = (16, 8, False, 1.3) if case == "chain16" else (24, 16, True, 0.9)
+    te = _make(env, env["holstein"].TimeEvolution, N, dim, g, 3000, periodic=periodic)
+    for i in range(1, 6):
+        te._evolve_single_step(i, DT * i, DT)
+    ham, U, psi, diag, basis, offs, cpos, src, mats = _union(env, te)
+    full = ham.features(U, psi, diag, basis.shape[0], offs, cpos, src)
+    # the union's basis flags as the learned step has them
+    inb = torch.zeros(U.shape[0], dtype=torch.uint8, device=U.device)
+    inb[cpos[:basis.shape[0]].long()] = 1
+    nadd = U.shape[0] - basis.shape[0]
+    X, idx = ham.features_added(U, psi, basis.shape[0], offs, cpos, src, inb, nadd)
+    want_idx = torch.nonzero(inb == 0).flatten()
+    assert nadd > 1000
+    np.testing.assert_array_equal(idx.cpu().numpy(), want_idx.cpu().numpy())
+    assert torch.equal(X, full.index_select(0, want_idx))
+
+
+def test_prune_plan_of_added_rows_equals_union_plan(env):
+    """paces_prune_plan_rows (scores of the added rows at their union positions) equals
+    paces_prune_plan on the scores scattered to the union with the basis rows forced;
+    paces_scatter_u8 writes out[idx[j]] = src[j]."""
+    torch, hip = env["torch"], env["hip"]
+    lib = hip.load()
+    rng = np.random.default_rng(9)
+    d = env["dev"]
+    n = 40_011
+    inb = (rng.uniform(size=n) < 0.4).astype(np.uint8)
+    added = np.nonzero(inb == 0)[0].astype(np.int32)
+    m = added.shape[0]
+    prob_a = rng.uniform(size=m).astype(np.float32)
+    prob_u = np.ones(n, dtype=np.float32)
+    prob_u[added] = prob_a
+    forced = np.nonzero(inb)[0].astype(np.int32)
+    k1 = torch.empty(n, dtype=torch.uint8, device=d)
+    k2 = torch.empty(n, dtype=torch.uint8, device=d)
+    t = {k: torch.from_numpy(v).to(d) for k, v in
+         dict(pa=prob_a, pu=prob_u, idx=added, inb=inb, forced=forced).items()}
+    for bits in (0, 3, 6):
+        hip.check(lib.paces_prune_plan(hip.ptr(t["pu"]), n, 0.35, hip.ptr(t["forced"]),
+                                       forced.shape[0], bits, 1234567, hip.ptr(k1), hip.stream()))
+        hip.check(lib.paces_prune_plan_rows(hip.ptr(t["pa"]), hip.ptr(t["idx"]), m,
+                                            hip.ptr(t["inb"]), n, 0.35, bits, 1234567,
+                                            hip.ptr(k2), hip.stream()))
+        assert torch.equal(k1, k2), bits
+        k = k2.cpu().numpy()
+        assert np.all(k[inb == 1] == 1)
+        low = k[added[prob_a < 0.35]]
+        assert np.all(low == 2) if bits == 0 else (np.any(low == 0) and np.any(low == 2))
+    src = torch.from_numpy(rng.integers(0, 3, m).astype(np.uint8)).to(d)
+    out = torch.full((n,), 7, dtype=torch.uint8, device=d)
+    hip.check(lib.paces_scatter_u8(hip.ptr(t["idx"]), m, hip.ptr(src), hip.ptr(out),
+                                   hip.stream()))
+    want = np.full(n, 7, dtype=np.uint8)
+    want[added] = src.cpu().numpy()
+    np.testing.assert_array_equal(out.cpu().numpy(), want)
 
 
 def test_compaction_matches_numpy(env):

@@ -94,6 +94,9 @@ def test_gelu_polynomial_form():
     assert np.max(np.abs(mo.gelu(x) - erf_form)) <= 8.2e-4
     assert mo.gelu(np.array([0.0]))[0] == 0.0
     np.testing.assert_allclose(mo.gelu(x) - mo.gelu(-x), x, atol=1e-13)
+    # the fitted piece meets the tails: a above 3.5, 0 below -3.5 (to the float32 coefficients)
+    far = np.abs(x) >= 3.5
+    np.testing.assert_allclose(mo.gelu(x[far]), np.maximum(x[far], 0.0), atol=2e-6)
     xs = x[np.abs(np.abs(x) - 3.5) > 1e-3]
     h = 1e-6
     fd = (mo.gelu(xs + h) - mo.gelu(xs - h)) / (2 * h)
