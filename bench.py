@@ -11,7 +11,10 @@ the launch-shape tuner and the series graphs have seen every size the timed step
 The line also carries: the plain FP64 form's throughput on the same trajectories over the same
 steps (one engine per trajectory); the batched form's largest site-density error against the
 FP64 oracle at every 10th step and the last, on the trajectories the oracle runs in seconds;
-the Taylor term's HBM roofline; and the oracle's throughput on this host's cores.
+the Taylor term's HBM roofline; the oracle's throughput on this host's cores; and, under
+"learned", the scorer-pruned form on its own workload (one 16-site chain at maxstates 2e6, the
+scorer trained untimed on 1e8 labels logged from plain runs at three other couplings) against
+the plain FP64 form over the same steps, with the scorer's MFMA roofline.
 
 ``--workload ring64|chain16`` runs one plain-form trajectory instead (64-site periodic ring,
 cap 15, maxstates 2e6, dt 0.02; or 16-site open chain, cap 7, maxstates 2e4, dt 0.05);
@@ -431,6 +434,17 @@ def run_traj256(a):
                              f"{a.oracle_max_ms}, steps 1..{last} from the initial state, one "
                              f"process per core: {ounits} retained state-steps in {owall:.1f} s "
                              f"wall"}
+    if a.learned_leg:
+        # the learned form on its own workload (one chain at maxstates 2e6), reported beside
+        # the headline: its throughput against the plain FP64 form over the same steps
+        torch.cuda.empty_cache()
+        r = learned_leg(a.learned_ms, a.labels, a.log_steps, a.epochs, a.learned_warmup,
+                        a.learned_steps, a.learned_precision, verbose=False, lr=a.lr)
+        extra["learned"] = dict({"workload": LEARNED_DESC.format(ms=a.learned_ms),
+                                 "form": "learned-" + a.learned_precision,
+                                 "steps": a.learned_steps, "warmup": a.learned_warmup,
+                                 "state_steps_per_s": r.pop("value")}, **r)
+        torch.cuda.empty_cache()
     dtype = "c64 terms / f64 state" if prec == "mixed" else "f64"
     line = {"metric": METRIC, "value": value, "unit": "state-steps/s", "n_gpus": 1,
             "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * elapsed / a.steps,
@@ -672,6 +686,13 @@ def main(argv=None):
                          "vib entry) or explicit FP32/FP64 values (identical results)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="traj256: tune the Taylor-term launch shape")
+    ap.add_argument("--learned-leg", type=int, default=1,
+                    help="traj256: add the learned form's own workload to the line (label "
+                         "logging and scorer training untimed, about half a minute)")
+    ap.add_argument("--learned-warmup", type=int, default=12,
+                    help="traj256: untimed steps of the learned leg")
+    ap.add_argument("--learned-steps", type=int, default=10,
+                    help="traj256: timed steps of the learned leg")
     ap.add_argument("--learned-ms", type=int, default=2_000_000)
     ap.add_argument("--learned-precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--labels", type=int, default=100_000_000,
