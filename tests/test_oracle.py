@@ -471,3 +471,24 @@ def test_tensor_state_equals_kronecker_product():
         got[index[tuple(r)]] += c
     np.testing.assert_allclose(got, full, rtol=0, atol=1e-15)
     assert abs(np.linalg.norm(coef) - 1.0) < 1e-15
+
+
+def test_pool_chain_worker_equals_timeline():
+    """oracle.pool.run_chain (the CPU baseline's worker) runs the oracle's own timeline: the
+    same site densities at the recorded steps and min(maxstates, R_prev) state-steps per step."""
+    from oracle import pool
+    N, J, g, ms, dt, nsteps = 8, 1.0, 0.8, 150, 0.05, 6
+    out, units, secs = pool.run_chain((N, J, g, ms, dt, nsteps, [3, 6]))
+    ham = po.HolsteinOracle(N, [8] * N, use_terms=pool.chain_terms(J, g))
+    te = po.TimeEvolutionOracle(ham, ms, 1)
+    te.create_single_state_basis(N // 2)
+    raw = np.zeros((1, N + 1), dtype=np.int64)
+    raw[0, 0] = N // 2
+    te.create_initial_vector([1.0], raw)
+    recs = []
+    te.generate_timeline(dt * np.arange(nsteps + 1), recs.append)
+    assert sorted(out) == [3, 6]
+    for i in (3, 6):
+        np.testing.assert_array_equal(out[i], recs[i].obs["n_exc"])
+    assert units == sum(min(recs[i - 1].whoami.shape[0], ms) for i in range(1, nsteps + 1))
+    assert secs > 0.0
