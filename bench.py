@@ -157,29 +157,27 @@ def btay_term_bytes(R, mats, mixed, coded=False):
     """Algorithmic HBM bytes of one batched Taylor term over R rows, averaged over an even and an
     odd term (the roofline loop times both alike): diagonal (4 or 8 per row), each term's indptr
     (4(R+1)), indices + values (4 + 4 or 8 per nonzero; coded: 4 + 1 per vib nonzero and 4 per
-    hopping nonzero), x_in and x_out (8 or 16 per row each), the complex128 partial sum read on
-    every term and written on odd terms (16 + 8 per row)."""
+    hopping nonzero), x_in and x_out (8 or 16 per row each), and the complex128 partial sum:
+    FP64 form, read on every term and written on odd terms (16 + 8 per row); mixed form, read
+    and written on odd terms only (16 per row)."""
     f = 4 if mixed else 8
     nnz = sum(m.nnz for m in mats)
     if coded:
         vals = sum(m.nnz for m in mats if getattr(m, "code", None) is not None)
     else:
         vals = f * nnz
-    return f * R + 4 * (R + 1) * len(mats) + 4 * nnz + vals + 4 * f * R + 24 * R
+    res = 16 if mixed else 24
+    return f * R + 4 * (R + 1) * len(mats) + 4 * nnz + vals + 4 * f * R + res * R
 
 
-# HBM bytes per row of one k_btay_term<float, B, RR> launch, per launch shape (the tuner picks
-# among shapes within a few percent of each other) and CSR value form (coded is the default; the
-# explicit-value figures are from before it), from FETCH_SIZE and WRITE_SIZE counter passes
-# over the roofline loop below with the shape fixed (--term-shape), each scaled by the counter's
-# reading for a 1 GiB device copy (profiles/k_btay_term_traj256_hbm_traffic.txt). Only the
-# mixed-precision kernel is measured; other shapes report null.
-BTAY_TRAFFIC_BYTES_PER_ROW = {("mixed", (256, 2), "coded"): 84.57,
-                              ("mixed", (256, 4), "coded"): 85.16,
-                              ("mixed", (512, 4), "coded"): 83.91,
-                              ("mixed", (256, 2), "explicit"): 89.72,
-                              ("mixed", (256, 4), "explicit"): 88.19,
-                              ("mixed", (512, 4), "explicit"): 87.42}
+# HBM bytes per row of one k_btay_term<float, B, RR> launch with coded CSR values, per launch
+# shape (the tuner picks among shapes within a few percent of each other), from FETCH_SIZE and
+# WRITE_SIZE counter passes over the roofline loop below with the shape fixed (--term-shape),
+# each scaled by the counter's reading for a 1 GiB device copy
+# (profiles/k_btay_term_traj256_hbm_traffic.txt). Other forms and shapes report null.
+BTAY_TRAFFIC_BYTES_PER_ROW = {("mixed", (256, 2), "coded"): 75.27,
+                              ("mixed", (256, 4), "coded"): 76.14,
+                              ("mixed", (512, 4), "coded"): 74.77}
 
 
 def measure_batched_term(bte, reps=20):

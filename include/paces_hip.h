@@ -448,6 +448,9 @@ int paces_taylor_term_mixed_tuned(int64_t R, const float* diag, const paces_csr3
 typedef struct {
   double c1, c1_plus_c2, ratio;
   int32_t done, j_ind;
+  double rn; /* the last exact ||res||_inf (begin: ||v||_inf; then every term of the FP64 form
+              * and every odd term of the mixed form); the mixed form's even terms do not read
+              * res and divide by it */
 } paces_batch_taylor_t;
 typedef struct {
   int32_t notdone; /* trajectories still running after the last term */

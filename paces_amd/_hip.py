@@ -93,7 +93,7 @@ class PriorImages(C.Structure):
 
 class BatchTaylor(C.Structure):
     _fields_ = [("c1", C.c_double), ("c1_plus_c2", C.c_double), ("ratio", C.c_double),
-                ("done", C.c_int32), ("j_ind", C.c_int32)]
+                ("done", C.c_int32), ("j_ind", C.c_int32), ("rn", C.c_double)]
 
 
 class LaunchCfg(C.Structure):

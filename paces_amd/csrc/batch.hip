@@ -777,11 +777,13 @@ __device__ __forceinline__ void btay_begin_body(const double2* __restrict__ v,
 }
 
 // per-trajectory reduction of the partials; fin == 1: initialise c1, else the stopping test
+// use_prev: the term read no partial sum (a mixed-form even term): divide by the last exact
+// ||res||_inf instead of this term's
 __device__ __forceinline__ void btay_reduce_body(const u64* __restrict__ part,
                                                  const int64_t* __restrict__ boff,
                                                  paces_batch_taylor_t* __restrict__ ts,
                                                  paces_batch_taylor_hdr_t* __restrict__ hdr,
-                                                 int fin, int j, double tol) {
+                                                 int fin, int j, double tol, int use_prev) {
   const int t = blockIdx.x;
   if (!fin && ts[t].done) return;
   u64 a = 0, b = 0;
@@ -796,13 +798,15 @@ __device__ __forceinline__ void btay_reduce_body(const u64* __restrict__ part,
   const double m0 = sqrt(__longlong_as_double((long long)a));
   if (fin) {
     ts[t].c1 = m0;
+    ts[t].rn = m0;  // res = v
     ts[t].c1_plus_c2 = 0.0;
     ts[t].ratio = 0.0;
     ts[t].done = (boff[t + 1] == boff[t]) ? 1 : 0;  // an empty trajectory has nothing to do
     ts[t].j_ind = 0;
     return;
   }
-  const double rn = sqrt(__longlong_as_double((long long)b));
+  const double rn = use_prev ? ts[t].rn : sqrt(__longlong_as_double((long long)b));
+  if (!use_prev) ts[t].rn = rn;
   const double c1pc2 = ts[t].c1 + m0;
   const double ratio = c1pc2 / rn;
   ts[t].c1_plus_c2 = c1pc2;
@@ -817,16 +821,24 @@ __device__ __forceinline__ void btay_reduce_body(const u64* __restrict__ part,
 }
 
 // The term's output for one row: t = -i c (H.x)_i, the partial sum (res, plus x_i on odd terms,
-// plus t; stored on odd terms) and the running maxima of |t|^2 and |res|^2.
+// plus t; stored on odd terms) and the running maxima of |t|^2 and |res|^2. Without with_res (a
+// mixed-form even term) the partial sum is neither read nor formed.
 template <typename F>
 __device__ __forceinline__ void btay_row_out(int64_t i, typename Vec2<F>::T h,
                                              typename Vec2<F>::T xi, double2 r, F c, bool odd,
+                                             bool with_res,
                                              typename Vec2<F>::T* __restrict__ xout,
                                              double2* __restrict__ res, u64& mt, u64& mr) {
   typename Vec2<F>::T t;
   t.x = h.y * c;
   t.y = -(h.x * c);
   xout[i] = t;
+  if (!with_res) {
+    const double tx = t.x, ty = t.y;
+    const u64 a = (u64)__double_as_longlong(tx * tx + ty * ty);
+    mt = a > mt ? a : mt;
+    return;
+  }
   if (odd) {
     r.x = r.x + (double)xi.x;
     r.y = r.y + (double)xi.y;
@@ -843,7 +855,7 @@ __device__ __forceinline__ void btay_row_out(int64_t i, typename Vec2<F>::T h,
 
 // The batched term's rows of one thread: the partial sums are loaded first (independent of H.x),
 // then H.x in load phases (hx_rows_phased), then each row's output.
-template <typename F, int BS, int RPT, int NT, typename VL>
+template <typename F, int BS, int RPT, int NT, bool WITH_RES, typename VL>
 __device__ __forceinline__ void btay_rows_phased(const F* __restrict__ diag, const CsrSetB64& H,
                                                  const typename Vec2<F>::T* __restrict__ xin,
                                                  typename Vec2<F>::T* __restrict__ xout,
@@ -858,17 +870,19 @@ __device__ __forceinline__ void btay_rows_phased(const F* __restrict__ diag, con
   for (int q = 0; q < RPT; ++q) {
     ii[q] = T.r0 + q * BS + threadIdx.x;
     vv[q] = ii[q] < T.r1;
-    rr[q] = res[vv[q] ? ii[q] : T.r0];
+    if (WITH_RES) rr[q] = res[vv[q] ? ii[q] : T.r0];
   }
   V h[RPT], xi[RPT];
   hx_rows_phased<F, V, RPT, NT>(ii, vv, T.r0, diag, H, xin, h, xi, vl);
 #pragma unroll
   for (int q = 0; q < RPT; ++q)
-    if (vv[q]) btay_row_out<F>(ii[q], h[q], xi[q], rr[q], c, odd, xout, res, mt, mr);
+    if (vv[q]) btay_row_out<F>(ii[q], h[q], xi[q], rr[q], c, odd, WITH_RES, xout, res, mt, mr);
 }
 
 // A tile's rows of one term: the phased form for one or two Hamiltonian terms, else row by row.
-template <typename F, int BS, int RPT, typename VL>
+// WITH_RES false (a mixed-form even term): the partial sum is neither read nor formed; the
+// stopping test divides by the last exact ||res||_inf (btay_reduce_body's use_prev).
+template <typename F, int BS, int RPT, bool WITH_RES, typename VL>
 __device__ __forceinline__ void btay_rows_all(const F* __restrict__ diag, const CsrSetB64& H,
                                               const typename Vec2<F>::T* __restrict__ xin,
                                               typename Vec2<F>::T* __restrict__ xout,
@@ -877,9 +891,9 @@ __device__ __forceinline__ void btay_rows_all(const F* __restrict__ diag, const 
   // the phased form needs about 40 VGPRs per FP64 row: not at 1024 threads x 4 rows (128 max)
   constexpr bool phased = !(sizeof(F) == 8 && BS * RPT > 2048);
   if (phased && H.nterms == 2) {
-    btay_rows_phased<F, BS, RPT, 2>(diag, H, xin, xout, res, c, odd, T, mt, mr, vl);
+    btay_rows_phased<F, BS, RPT, 2, WITH_RES>(diag, H, xin, xout, res, c, odd, T, mt, mr, vl);
   } else if (phased && H.nterms == 1) {
-    btay_rows_phased<F, BS, RPT, 1>(diag, H, xin, xout, res, c, odd, T, mt, mr, vl);
+    btay_rows_phased<F, BS, RPT, 1, WITH_RES>(diag, H, xin, xout, res, c, odd, T, mt, mr, vl);
   } else {
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
@@ -887,10 +901,24 @@ __device__ __forceinline__ void btay_rows_all(const F* __restrict__ diag, const 
       if (i < T.r1) {
         typename Vec2<F>::T xi;
         const typename Vec2<F>::T h = hx_row_t<F>(i, diag, H, xin, xi, vl);
-        btay_row_out<F>(i, h, xi, res[i], c, odd, xout, res, mt, mr);
+        double2 r = {0.0, 0.0};
+        if (WITH_RES) r = res[i];
+        btay_row_out<F>(i, h, xi, r, c, odd, WITH_RES, xout, res, mt, mr);
       }
     }
   }
+}
+
+template <typename F, int BS, int RPT, typename VL>
+__device__ __forceinline__ void btay_rows_par(const F* __restrict__ diag, const CsrSetB64& H,
+                                              const typename Vec2<F>::T* __restrict__ xin,
+                                              typename Vec2<F>::T* __restrict__ xout,
+                                              double2* __restrict__ res, F c, bool odd,
+                                              const SegTile& T, u64& mt, u64& mr, const VL& vl) {
+  if (sizeof(F) == 8 || odd)
+    btay_rows_all<F, BS, RPT, true>(diag, H, xin, xout, res, c, odd, T, mt, mr, vl);
+  else
+    btay_rows_all<F, BS, RPT, false>(diag, H, xin, xout, res, c, odd, T, mt, mr, vl);
 }
 
 // The partial sum res is stored on odd terms only. On an even term j the sum res + t_j is formed
@@ -926,9 +954,9 @@ __device__ __forceinline__ void btay_term_body(
             (F)(H.code[t] ? (m < H.sqrt_len ? H.sqrt_table[m] * v : 0.0) : v);
     }
     __syncthreads();
-    btay_rows_all<F, BS, RPT>(diag, H, xin, xout, res, c, odd, T, mt, mr, CodedVals<F>{vt});
+    btay_rows_par<F, BS, RPT>(diag, H, xin, xout, res, c, odd, T, mt, mr, CodedVals<F>{vt});
   } else {
-    btay_rows_all<F, BS, RPT>(diag, H, xin, xout, res, c, odd, T, mt, mr, PlainVals<F>{});
+    btay_rows_par<F, BS, RPT>(diag, H, xin, xout, res, c, odd, T, mt, mr, PlainVals<F>{});
   }
   blk_max2<BS>(mt, mr, part);
 }
@@ -949,8 +977,8 @@ __global__ void __launch_bounds__(64) k_btay_reduce(const u64* __restrict__ part
                                                     const int64_t* __restrict__ boff,
                                                     paces_batch_taylor_t* __restrict__ ts,
                                                     paces_batch_taylor_hdr_t* __restrict__ hdr,
-                                                    int fin, int j, double tol) {
-  btay_reduce_body(part, boff, ts, hdr, fin, j, tol);
+                                                    int fin, int j, double tol, int use_prev) {
+  btay_reduce_body(part, boff, ts, hdr, fin, j, tol, use_prev);
 }
 
 template <typename F, int BS, int RPT>
@@ -1059,9 +1087,10 @@ __global__ void __launch_bounds__(BS) k_bser_begin(const paces_batch_series_t* _
 }
 
 __global__ void __launch_bounds__(64) k_bser_reduce(const paces_batch_series_t* __restrict__ desc,
-                                                    int ntraj, int fin, int j) {
+                                                    int ntraj, int fin, int j, int use_prev) {
   const SeriesDev* d = (const SeriesDev*)desc;
-  btay_reduce_body(ser_part(d, ntraj), ser_boff(d), gen(d->ts), gen(d->hdr), fin, j, d->tol);
+  btay_reduce_body(ser_part(d, ntraj), ser_boff(d), gen(d->ts), gen(d->hdr), fin, j, d->tol,
+                   use_prev);
 }
 
 template <typename F, int BS, int RPT>
@@ -1252,7 +1281,7 @@ int paces_batch_taylor_begin_tuned(int64_t R, const double* v, double* res, void
   PACES_CFG_SWITCH(B, RR, BB_LAUNCH)
 #undef BB_LAUNCH
   hipLaunchKernelGGL(k_btay_reduce, dim3(ntraj), dim3(64), 0, st, (const u64*)part,
-                     (const int64_t*)boff, ts, (paces_batch_taylor_hdr_t*)nullptr, 1, 0, 0.0);
+                     (const int64_t*)boff, ts, (paces_batch_taylor_hdr_t*)nullptr, 1, 0, 0.0, 0);
   return launch_status();
 }
 
@@ -1276,7 +1305,7 @@ int paces_batch_series_launch(const paces_batch_series_t* desc, int64_t R_cap, i
     hipLaunchKernelGGL((k_bser_begin<double, BB, QQ>), dim3(gs), dim3(BB), 0, st, desc, ntraj)
   PACES_CFG_SWITCH(B, RR, SB_LAUNCH)
 #undef SB_LAUNCH
-  hipLaunchKernelGGL(k_bser_reduce, dim3(ntraj), dim3(64), 0, st, desc, ntraj, 1, 0);
+  hipLaunchKernelGGL(k_bser_reduce, dim3(ntraj), dim3(64), 0, st, desc, ntraj, 1, 0, 0);
   for (int j = 0; j < nterms; ++j) {
 #define ST_LAUNCH(BB, QQ)                                                                      \
   if (mixed)                                                                                   \
@@ -1287,7 +1316,8 @@ int paces_batch_series_launch(const paces_batch_series_t* desc, int64_t R_cap, i
                        j)
     PACES_CFG_SWITCH(B, RR, ST_LAUNCH)
 #undef ST_LAUNCH
-    hipLaunchKernelGGL(k_bser_reduce, dim3(ntraj), dim3(64), 0, st, desc, ntraj, 0, j);
+    hipLaunchKernelGGL(k_bser_reduce, dim3(ntraj), dim3(64), 0, st, desc, ntraj, 0, j,
+                       (mixed && !(j & 1)) ? 1 : 0);
   }
   return launch_status();
 }
@@ -1360,7 +1390,7 @@ static int btay_term_launch(int64_t R, const void* diag, int mixed, const CsrSet
   PACES_CFG_SWITCH(B, RR, BT_LAUNCH)
 #undef BT_LAUNCH
   hipLaunchKernelGGL(k_btay_reduce, dim3(ntraj), dim3(64), 0, st, (const u64*)part, boff, ts, hdr,
-                     0, (int)j, tol);
+                     0, (int)j, tol, (mixed && !(j & 1)) ? 1 : 0);
   return launch_status();
 }
 

@@ -1445,7 +1445,7 @@ class BatchedTimeEvolution:
         self._expm_info = None
 
     _EXPM_DTYPE = numpy.dtype([("c1", "f8"), ("c1pc2", "f8"), ("ratio", "f8"), ("done", "i4"),
-                               ("j", "i4")])
+                               ("j", "i4"), ("rn", "f8")])
 
     @property
     def last_expm(self):

@@ -82,5 +82,5 @@ def test_struct_layouts_match_the_library():
     from paces_amd import _hip
     lib = _hip.load()
     assert lib.paces_batch_series_desc_bytes() == C.sizeof(_hip.BatchSeries)
-    assert C.sizeof(_hip.BatchTaylor) == 32
+    assert C.sizeof(_hip.BatchTaylor) == 40  # 3 doubles, 2 int32, the last exact norm
     assert C.sizeof(_hip.TaylorState) == 64
