@@ -154,6 +154,30 @@ class Hamiltonian(HamiltonianFramework):
         check(min(nruns, 0), "holstein_num_runs")
         offs = torch.empty(nruns + 1, dtype=torch.int32, device=self.device)
         wsp, wsn = self._ws.get(_lib.paces_holstein_images_workspace_bytes(n, C.byref(h)))
+        # The outputs are sized by a bound (a row gives at most one image per run): the count is
+        # launched first, the outputs are allocated while it runs, then the offsets' copy to the
+        # host and the emit are queued back to back, so the host reads the offsets while the
+        # emit runs (the emit places images by the scanned tile counts in the workspace, not by
+        # the host's copy). Only a bound beyond _BOUND_BYTES waits for the exact size first.
+        W = self.totwordwidth
+        cap = max(n * nruns, 1)
+        bounded = cap * (8 * W + 5) <= _BOUND_BYTES
+
+        def outputs(cap):
+            if slot is None:
+                imgs = torch.empty((cap, W), dtype=self.dtype, device=self.device)
+            else:
+                imgs = self._pooled(slot + ".rows", cap * W, self.dtype).view(cap, W)
+            src = nup = None
+            if want_src:
+                if slot is None:
+                    src = torch.empty(cap, dtype=torch.int32, device=self.device)
+                    nup = torch.empty(cap, dtype=torch.uint8, device=self.device)
+                else:
+                    src = self._pooled(slot + ".src", cap, torch.int32)
+                    nup = self._pooled(slot + ".nup", cap, torch.uint8)
+            return imgs, src, nup
+
         if skip is not None:
             check(_lib.paces_holstein_images_count_skip(
                 ptr(basis) if n else None, n, ptr(skip) if n else None, C.byref(self._layout),
@@ -162,27 +186,12 @@ class Hamiltonian(HamiltonianFramework):
             check(_lib.paces_holstein_images_count(ptr(basis) if n else None, n,
                                                    C.byref(self._layout), C.byref(h), ptr(offs),
                                                    wsp, wsn, stream()), "holstein_images_count")
-        # The emit is launched before the offsets reach the host: a row gives at most one image
-        # per run, so n * nruns rows bound the output, and the host reads the offsets while the
-        # emit runs.
+        if bounded:
+            imgs, src, nup = outputs(cap)
         offs_host, ev = _to_host_async(offs)
-        cap = max(n * nruns, 1)
-        if cap * (8 * self.totwordwidth + 5) > _BOUND_BYTES:
+        if not bounded:
             ev.synchronize()
-            cap = max(int(offs_host[-1]), 1)
-        W = self.totwordwidth
-        if slot is None:
-            imgs = torch.empty((cap, W), dtype=self.dtype, device=self.device)
-        else:
-            imgs = self._pooled(slot + ".rows", cap * W, self.dtype).view(cap, W)
-        src = nup = None
-        if want_src:
-            if slot is None:
-                src = torch.empty(cap, dtype=torch.int32, device=self.device)
-                nup = torch.empty(cap, dtype=torch.uint8, device=self.device)
-            else:
-                src = self._pooled(slot + ".src", cap, torch.int32)
-                nup = self._pooled(slot + ".nup", cap, torch.uint8)
+            imgs, src, nup = outputs(max(int(offs_host[-1]), 1))
         if skip is not None:
             check(_lib.paces_holstein_images_emit_skip(
                 ptr(basis) if n else None, n, ptr(skip) if n else None, C.byref(self._layout),
