@@ -388,7 +388,8 @@ typedef struct {
   double ratio;
   int32_t done;
   int32_t j_ind;
-  int32_t pad[4];
+  double rn; /* the last exact ||res||_inf (the mixed series' even terms divide by it) */
+  int32_t pad[2];
 } paces_taylor_state_t;
 
 /* Workspace of per-workgroup partial maxima shared by _begin and _term. */
@@ -430,7 +431,11 @@ int paces_convert_f64_to_f32(int64_t n, const double* in, float* out, void* stre
 /* res = v; x0 = (complex64) v; state.c1 = ||v||_inf */
 int paces_taylor_begin_mixed(int64_t R, const double* v, double* res, float* x0,
                              paces_taylor_state_t* state, void* ws, size_t ws_bytes, void* stream);
-/* x_out = (-i*c) * (H32 x_in) in complex64; res += x_out in complex128; stopping test as above */
+/* x_out = (-i*c) * (H32 x_in) in complex64, summed into res in complex128 on odd terms j only:
+ * an odd term adds its input and its output, (res + x_in) + x_out, the same roundings in the same
+ * order as adding every term. An even term neither reads nor writes res, and its stopping test
+ * divides by the last exact ||res||_inf (state.rn) instead of ||res + x_out||_inf. Even terms must
+ * write x_out to one buffer (x_even); after the last term call paces_taylor_finish_mixed. */
 int paces_taylor_term_mixed(int64_t R, const float* diag, const paces_csr32_set_t* H,
                             const float* x_in, float* x_out, double* res, float c, int32_t j,
                             double tol, paces_taylor_state_t* state, void* ws, size_t ws_bytes,
@@ -439,6 +444,9 @@ int paces_taylor_term_mixed_tuned(int64_t R, const float* diag, const paces_csr3
                                   const float* x_in, float* x_out, double* res, float c,
                                   int32_t j, double tol, paces_taylor_state_t* state, void* ws,
                                   size_t ws_bytes, const paces_launch_cfg_t* cfg, void* stream);
+/* res += x_even when the series' last term (state.j_ind, read on the device) was even */
+int paces_taylor_finish_mixed(int64_t R, const float* x_even, double* res,
+                              const paces_taylor_state_t* state, void* stream);
 
 /* ---- N1 batched trajectories (no reference counterpart). Field 0 of every row is the
  * trajectory id (ceil(log2 ntraj) bits), so trajectory t owns the contiguous segment

@@ -117,7 +117,7 @@ class BatchSeries(C.Structure):
 class TaylorState(C.Structure):
     _fields_ = [("max_term2", C.c_uint64), ("max_res2", C.c_uint64), ("c1", C.c_double),
                 ("c1_plus_c2", C.c_double), ("ratio", C.c_double), ("done", C.c_int32),
-                ("j_ind", C.c_int32), ("pad", C.c_int32 * 4)]
+                ("j_ind", C.c_int32), ("rn", C.c_double), ("pad", C.c_int32 * 2)]
 
 
 _P, _I64, _I32, _D, _SZ, _F = (C.c_void_p, C.c_int64, C.c_int32, C.c_double, C.c_size_t,
@@ -226,6 +226,7 @@ _SIGNATURES = {
     "paces_axpby": (C.c_int, [_I64, _D, _D, _P, _D, _D, _P, _P]),
     "paces_taylor_term_tuned": (C.c_int, [_I64, _P, C.POINTER(CsrSet), _P, _P, _P, _D, _I32,
                                           _D, _P, _P, _SZ, C.POINTER(LaunchCfg), _P]),
+    "paces_taylor_finish_mixed": (C.c_int, [_I64, _P, _P, _P, _P]),
     "paces_taylor_term_mixed_tuned": (C.c_int, [_I64, _P, C.POINTER(CsrSet32), _P, _P, _P, _F,
                                                 _I32, _D, _P, _P, _SZ, C.POINTER(LaunchCfg),
                                                 _P]),

@@ -437,4 +437,9 @@ def simple_taylor_mixed(mm: MixedMats, v, delta_t, m_star=100, tol=TAYLOR_TOL_MI
         host = _hip.TaylorState.from_buffer_copy(bytes(state.cpu().numpy()))
         if host.done:
             break
+    # the sum is stored on odd terms: a series that stopped on an even term (its output in
+    # bufs[0]) gets that term added here
+    check(lib.paces_taylor_finish_mixed(R, ptr(torch.view_as_real(bufs[0])),
+                                        ptr(torch.view_as_real(res)), ptr(state), st),
+          "taylor_finish_mixed")
     return res, [bool(host.done), int(host.j_ind), float(host.c1_plus_c2), float(host.ratio)]

@@ -176,7 +176,10 @@ __global__ void __launch_bounds__(1024) k_tay_begin_fin(paces_taylor_state_t* s,
                                                         const u64* __restrict__ part, int nblk) {
   u64 m0 = 0, m1 = 0;
   reduce_partials(part, nblk, &m0, &m1);
-  if (threadIdx.x == 0) s->c1 = sqrt(__longlong_as_double((long long)m0));
+  if (threadIdx.x == 0) {
+    s->c1 = sqrt(__longlong_as_double((long long)m0));
+    s->rn = s->c1;  // res = v (the mixed form's first even term divides by it)
+  }
 }
 
 // One Taylor term j: x_out = (-i c) H x_in, res += x_out, per-workgroup maxima of |x_out|^2 and
@@ -248,15 +251,18 @@ __global__ void __launch_bounds__(BS) k_tay_term_t(int64_t R, const double* __re
   block_max2_store_t<BS>(mt, mr, part);
 }
 
+// use_prev: the term formed no partial sum (a mixed-form even term): divide by the last exact
+// ||res||_inf (s->rn) instead of this term's, which is then kept in s->rn
 __global__ void __launch_bounds__(1024) k_tay_check(paces_taylor_state_t* s,
                                                     const u64* __restrict__ part, int nblk, int j,
-                                                    double tol) {
+                                                    double tol, int use_prev) {
   if (s->done) return;
   u64 m0 = 0, m1 = 0;
   reduce_partials(part, nblk, &m0, &m1);
   if (threadIdx.x != 0) return;
   const double c2 = sqrt(__longlong_as_double((long long)m0));
-  const double rn = sqrt(__longlong_as_double((long long)m1));
+  const double rn = use_prev ? s->rn : sqrt(__longlong_as_double((long long)m1));
+  if (!use_prev) s->rn = rn;
   const double c1pc2 = s->c1 + c2;
   const double ratio = c1pc2 / rn;
   s->c1_plus_c2 = c1pc2;
@@ -314,67 +320,91 @@ __global__ void __launch_bounds__(kBlock) k_tay_begin_mixed(int64_t R,
   block_max2_store(m, 0, part);
 }
 
-template <int BS, int RPT>
-__global__ void __launch_bounds__(BS) k_tay_term_mixed_t(
-    int64_t R, const float* __restrict__ diag, CsrSet32 H, const float2* __restrict__ xin,
-    float2* __restrict__ xout, double2* __restrict__ res, float c,
-    const paces_taylor_state_t* __restrict__ s, u64* __restrict__ part) {
-  if (s->done) return;
+// One mixed term j. The complex128 partial sum is formed and stored on odd terms only: an odd
+// term adds its input (t_(j-1), the even term's output) and then its own output,
+// (res + t_(j-1)) + t_j, so a series that stops on an even term needs
+// paces_taylor_finish_mixed to add that term. An even term neither reads nor writes res; its
+// stopping test divides by the last exact ||res||_inf (k_tay_check's use_prev).
+template <int BS, int RPT, bool ODD>
+__device__ __forceinline__ void tay_term_mixed_body(int64_t R, const float* __restrict__ diag,
+                                                    const CsrSet32& H,
+                                                    const float2* __restrict__ xin,
+                                                    float2* __restrict__ xout,
+                                                    double2* __restrict__ res, float c,
+                                                    u64* __restrict__ part) {
   const int64_t base = (int64_t)blockIdx.x * (BS * RPT) + threadIdx.x;
   u64 mt = 0, mr = 0;
-  if (H.nterms == 1 || H.nterms == 2) {  // rows in load phases (hx_rows_phased)
-    int64_t ii[RPT];
-    bool vv[RPT];
-    double2 rr[RPT];
-    float2 h[RPT], xi[RPT];
-    const int64_t spare = (int64_t)blockIdx.x * (BS * RPT);  // the block's first row (< R)
+  int64_t ii[RPT];
+  bool vv[RPT];
+  double2 rr[RPT];
+  float2 h[RPT], xi[RPT];
+  const int64_t spare = (int64_t)blockIdx.x * (BS * RPT);  // the block's first row (< R)
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    ii[q] = base + (int64_t)q * BS;
+    vv[q] = ii[q] < R;
+    if (ODD) rr[q] = res[vv[q] ? ii[q] : spare];
+  }
+  if (H.nterms == 2) {  // rows in load phases (hx_rows_phased)
+    hx_rows_phased<float, float2, RPT, 2>(ii, vv, spare, diag, H, xin, h, xi);
+  } else if (H.nterms == 1) {
+    hx_rows_phased<float, float2, RPT, 1>(ii, vv, spare, diag, H, xin, h, xi);
+  } else {
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-      ii[q] = base + (int64_t)q * BS;
-      vv[q] = ii[q] < R;
-      rr[q] = res[vv[q] ? ii[q] : spare];
+      const int64_t i = vv[q] ? ii[q] : spare;
+      h[q] = hx_row32(i, diag, H, xin);
+      xi[q] = xin[i];
     }
-    if (H.nterms == 2)
-      hx_rows_phased<float, float2, RPT, 2>(ii, vv, spare, diag, H, xin, h, xi);
-    else
-      hx_rows_phased<float, float2, RPT, 1>(ii, vv, spare, diag, H, xin, h, xi);
-#pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-      if (!vv[q]) continue;
-      const float2 t = make_float2(h[q].y * c, -(h[q].x * c));
-      xout[ii[q]] = t;
-      double2 r = rr[q];
-      r.x = r.x + (double)t.x;
-      r.y = r.y + (double)t.y;
-      res[ii[q]] = r;
-      const double tx = t.x, ty = t.y;
-      const u64 a = (u64)__double_as_longlong(tx * tx + ty * ty);
-      const u64 b = (u64)__double_as_longlong(r.x * r.x + r.y * r.y);
-      mt = a > mt ? a : mt;
-      mr = b > mr ? b : mr;
-    }
-    block_max2_store_t<BS>(mt, mr, part);
-    return;
   }
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
-    const int64_t i = base + (int64_t)q * BS;
-    if (i < R) {
-      const float2 h = hx_row32(i, diag, H, xin);
-      const float2 t = make_float2(h.y * c, -(h.x * c));
-      xout[i] = t;
-      double2 r = res[i];
+    if (!vv[q]) continue;
+    const float2 t = make_float2(h[q].y * c, -(h[q].x * c));
+    xout[ii[q]] = t;
+    const double tx = t.x, ty = t.y;
+    const u64 a = (u64)__double_as_longlong(tx * tx + ty * ty);
+    mt = a > mt ? a : mt;
+    if (ODD) {
+      double2 r = rr[q];
+      r.x = r.x + (double)xi[q].x;
+      r.y = r.y + (double)xi[q].y;
       r.x = r.x + (double)t.x;
       r.y = r.y + (double)t.y;
-      res[i] = r;
-      const double tx = t.x, ty = t.y;
-      const u64 a = (u64)__double_as_longlong(tx * tx + ty * ty);
+      res[ii[q]] = r;
       const u64 b = (u64)__double_as_longlong(r.x * r.x + r.y * r.y);
-      mt = a > mt ? a : mt;
       mr = b > mr ? b : mr;
     }
   }
   block_max2_store_t<BS>(mt, mr, part);
+}
+
+template <int BS, int RPT>
+__global__ void __launch_bounds__(BS) k_tay_term_mixed_t(
+    int64_t R, const float* __restrict__ diag, CsrSet32 H, const float2* __restrict__ xin,
+    float2* __restrict__ xout, double2* __restrict__ res, float c, int odd,
+    const paces_taylor_state_t* __restrict__ s, u64* __restrict__ part) {
+  if (s->done) return;
+  if (odd)
+    tay_term_mixed_body<BS, RPT, true>(R, diag, H, xin, xout, res, c, part);
+  else
+    tay_term_mixed_body<BS, RPT, false>(R, diag, H, xin, xout, res, c, part);
+}
+
+// after the series: a series that stopped on an even term j has res one term behind; that term
+// is still in x_even (even terms write there, and later terms skip a stopped series)
+__global__ void __launch_bounds__(kBlock) k_tay_finish_mixed(int64_t R,
+                                                             const float2* __restrict__ x_even,
+                                                             double2* __restrict__ res,
+                                                             const paces_taylor_state_t* s) {
+  if ((s->j_ind & 1) != 0) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= R) return;
+  const float2 x = x_even[i];
+  double2 r = res[i];
+  r.x = r.x + (double)x.x;
+  r.y = r.y + (double)x.y;
+  res[i] = r;
 }
 
 __global__ void __launch_bounds__(kBlock) k_f64_to_f32(int64_t n, const double* __restrict__ in,
@@ -560,7 +590,7 @@ int paces_taylor_term_tuned(int64_t R, const double* diag, const paces_csr_set_t
 #undef TAY_LAUNCH
   }
   hipLaunchKernelGGL(k_tay_check, dim3(1), dim3(1024), 0, st, state, (const u64*)ws, nblk,
-                     (int)j, tol);
+                     (int)j, tol, 0);
   return launch_status();
 }
 
@@ -613,13 +643,22 @@ int paces_taylor_term_mixed_tuned(int64_t R, const float* diag, const paces_csr3
     nblk = (int)grid_for(R, B * RR);
 #define TAYM_LAUNCH(BB, QQ)                                                                 \
   hipLaunchKernelGGL((k_tay_term_mixed_t<BB, QQ>), dim3(nblk), dim3(BB), 0, st, R, diag, H, \
-                     (const float2*)x_in, (float2*)x_out, (double2*)res, c,                \
+                     (const float2*)x_in, (float2*)x_out, (double2*)res, c, (int)(j & 1),  \
                      (const paces_taylor_state_t*)state, (u64*)ws)
     PACES_CFG_SWITCH(B, RR, TAYM_LAUNCH)
 #undef TAYM_LAUNCH
   }
   hipLaunchKernelGGL(k_tay_check, dim3(1), dim3(1024), 0, st, state, (const u64*)ws, nblk,
-                     (int)j, tol);
+                     (int)j, tol, (j & 1) ? 0 : 1);
+  return launch_status();
+}
+
+int paces_taylor_finish_mixed(int64_t R, const float* x_even, double* res,
+                              const paces_taylor_state_t* state, void* stream) {
+  if (R < 0 || !state || (R > 0 && (!x_even || !res))) return PACES_E_INVALID;
+  if (R == 0) return PACES_OK;
+  hipLaunchKernelGGL(k_tay_finish_mixed, dim3(grid_for(R, kBlock)), dim3(kBlock), 0,
+                     (hipStream_t)stream, R, (const float2*)x_even, (double2*)res, state);
   return launch_status();
 }
 
