@@ -129,6 +129,13 @@ size_t paces_merge_unique_workspace_bytes(int64_t total, int32_t K, int32_t W);
 int paces_merge_unique(const paces_runs_t* runs, int32_t W, uint64_t* out_rows,
                        int32_t* out_count, uint8_t* out_from_run0, int32_t* cand_pos, void* ws,
                        size_t ws_bytes, void* stream);
+/* The same union; every output row also gets out_code[row] = run_codes[the run of its kept copy]
+ * (run_codes: K host bytes). With the codes of paces_holstein_inverse_skip_codes this is the skip
+ * input of the next expansion (paces_holstein_images_*_skip). No counterpart in the reference. */
+int paces_merge_unique_coded(const paces_runs_t* runs, int32_t W, uint64_t* out_rows,
+                             int32_t* out_count, uint8_t* out_from_run0, int32_t* cand_pos,
+                             const uint8_t* run_codes, uint8_t* out_code, void* ws,
+                             size_t ws_bytes, void* stream);
 
 /* ---- A7 general sort + dedup (cupy_unique, paces/aux/helpers.py:63-75, as called by
  * enlarge_basis_set, paces/core/hamiltonian.py:550-577), for rows in no particular order:
@@ -169,9 +176,10 @@ int paces_holstein_images_count(const uint64_t* basis, int64_t n, const paces_la
 int paces_holstein_images_emit(const uint64_t* basis, int64_t n, const paces_layout_t* lay,
                                const paces_holstein_t* h, uint64_t* out_images, int32_t* out_src,
                                uint8_t* out_nup, void* ws, size_t ws_bytes, void* stream);
-/* The same over the rows i with skip[i] == 0 only (the images of the rows an expansion added: the
- * other rows' images are already in the expanded basis; no reference counterpart). out_src is
- * the row's index in basis. Same workspace. */
+/* The same with a skip code per row (the images of the rows an expansion added: the other rows'
+ * images are already in the expanded basis; no reference counterpart): skip[i] == 0 gives every
+ * image of row i, 1 none, 2 + k all but op k's (a duplicate of a basis row; see
+ * paces_holstein_inverse_skip_codes). out_src is the row's index in basis. Same workspace. */
 int paces_holstein_images_count_skip(const uint64_t* basis, int64_t n, const uint8_t* skip,
                                      const paces_layout_t* lay, const paces_holstein_t* h,
                                      int32_t* run_offsets, void* ws, size_t ws_bytes,
@@ -180,6 +188,11 @@ int paces_holstein_images_emit_skip(const uint64_t* basis, int64_t n, const uint
                                     const paces_layout_t* lay, const paces_holstein_t* h,
                                     uint64_t* out_images, int32_t* out_src, uint8_t* out_nup,
                                     void* ws, size_t ws_bytes, void* stream);
+/* codes[0..K) for the union of [basis, op 0's images, op 1's images, ...] (K = number of ops + 1,
+ * returned): 1 for the basis run, 2 + (k ^ 1) for the run of a positive-direction op k, else 0.
+ * Ops come in inverse pairs (k, k ^ 1): a row added as op k's image of a basis row s has s as its
+ * op k ^ 1 image; for positive k that image adds neither a row nor a CSR entry. */
+int paces_holstein_inverse_skip_codes(const paces_holstein_t* h, uint8_t* codes, int32_t cap);
 
 /* ---- D1/D2 from the union (no lookups): given the union of a basis (run 0, nbasis rows) and its
  * image runs (paces_merge_unique with cand_pos), the per-term CSR on the R union rows. Each image

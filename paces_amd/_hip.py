@@ -136,6 +136,9 @@ _SIGNATURES = {
     "paces_lower_bound": (C.c_int, [_P, _I64, _P, _I64, _I32, _I32, _P, _P, _P, _P, _SZ, _P]),
     "paces_merge_unique_workspace_bytes": (_SZ, [_I64, _I32, _I32]),
     "paces_merge_unique": (C.c_int, [C.POINTER(Runs), _I32, _P, _P, _P, _P, _P, _SZ, _P]),
+    "paces_merge_unique_coded": (C.c_int, [C.POINTER(Runs), _I32, _P, _P, _P, _P, _P, _P, _P,
+                                           _SZ, _P]),
+    "paces_holstein_inverse_skip_codes": (C.c_int, [C.POINTER(Holstein), _P, _I32]),
     "paces_merge_unique_diag": (C.c_int, [C.POINTER(Runs), _I32, _P, _P, _P, _P,
                                           C.POINTER(Layout), C.POINTER(Holstein), _P, _P, _P,
                                           _SZ, _P]),

@@ -128,11 +128,15 @@ def lower_bound(book, q, want_hit=True, count_misses=False, q_sorted=False, ws=N
     return idx, hit, miss
 
 
-def merge_unique(runs, W, ws, want_from0=False, want_cand_pos=False, diag=None):
+def merge_unique(runs, W, ws, want_from0=False, want_cand_pos=False, diag=None, run_codes=None):
     """Sorted union of sorted unique row runs; returns (rows, from_run0 flags or None) and, with
     want_cand_pos, the output index of every input row as a third element. With diag =
     (layout, holstein struct, torch.float64 or torch.float32), the Holstein diagonal of the rows
-    comes from the same pass (paces_merge_unique_diag) as a last element."""
+    comes from the same pass (paces_merge_unique_diag) as a last element. With run_codes (one
+    byte per run), each row's code (the code of the run its kept copy came from,
+    paces_merge_unique_coded) is the last element instead; diag and run_codes do not combine."""
+    if diag is not None and run_codes is not None:
+        raise ValueError("merge_unique takes diag or run_codes, not both")
     if len(runs) > _hip.MAX_RUNS:
         raise ValueError("too many runs")
     R = _hip.Runs()
@@ -150,7 +154,15 @@ def merge_unique(runs, W, ws, want_from0=False, want_cand_pos=False, diag=None):
     cpos = torch.empty(max(total, 1), dtype=torch.int32, device=dev) if want_cand_pos else None
     wsp, wsn = ws.get(lib.paces_merge_unique_workspace_bytes(total, R.K, W))
     d = None
-    if diag is None:
+    if run_codes is not None:
+        if len(run_codes) != len(runs):
+            raise ValueError("one code per run")
+        codes = (C.c_uint8 * len(run_codes))(*run_codes)
+        d = torch.empty(max(total, 1), dtype=torch.uint8, device=dev)
+        check(lib.paces_merge_unique_coded(C.byref(R), W, ptr(out), ptr(cnt), ptr(from0),
+                                           ptr(cpos), codes, ptr(d), wsp, wsn, stream()),
+              "merge_unique_coded")
+    elif diag is None:
         check(lib.paces_merge_unique(C.byref(R), W, ptr(out), ptr(cnt), ptr(from0), ptr(cpos),
                                      wsp, wsn, stream()), "merge_unique")
     else:
@@ -165,7 +177,7 @@ def merge_unique(runs, W, ws, want_from0=False, want_cand_pos=False, diag=None):
     res = [out[:n], (from0[:n] if want_from0 else None)]
     if want_cand_pos:
         res.append(cpos)
-    if diag is not None:
+    if d is not None:
         res.append(d[:n])
     return tuple(res)
 

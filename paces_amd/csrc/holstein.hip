@@ -381,12 +381,15 @@ __global__ void __launch_bounds__(kBlock) k_img_tcount(const u64* __restrict__ b
   const RowFields F = row_fields(L, M);
   RowCtx cx[kImgSteps];
   bool vv[kImgSteps];
+  int so[kImgSteps];  // the one op a row skips (skip code 2 + k), else -1
   if (W <= kImgHoldWords) {  // every row's load in flight before any is used
     Row<W> rr[kImgSteps];
 #pragma unroll
     for (int q = 0; q < kImgSteps; ++q) {
       const int64_t i = wt * kImgTile + q * 64 + lane;
-      vv[q] = i < n && !(skip && skip[i]);
+      const int sc = (skip && i < n) ? skip[i] : 0;
+      vv[q] = i < n && sc != 1;
+      so[q] = sc >= 2 ? sc - 2 : -1;
       rr[q] = load_row<W>(basis, i < n ? i : 0);
     }
     row_ctx_n<W, kImgSteps>(T, L, M, F, rr, cx);
@@ -394,7 +397,9 @@ __global__ void __launch_bounds__(kBlock) k_img_tcount(const u64* __restrict__ b
 #pragma unroll
     for (int q = 0; q < kImgSteps; ++q) {
       const int64_t i = wt * kImgTile + q * 64 + lane;
-      vv[q] = i < n && !(skip && skip[i]);
+      const int sc = (skip && i < n) ? skip[i] : 0;
+      vv[q] = i < n && sc != 1;
+      so[q] = sc >= 2 ? sc - 2 : -1;
       cx[q] = row_ctx<W>(T, L, M, F, load_row<W>(basis, i < n ? i : 0));
     }
   }
@@ -402,7 +407,8 @@ __global__ void __launch_bounds__(kBlock) k_img_tcount(const u64* __restrict__ b
     const OpView o = op_view(T, k, F);
     int c = 0;
 #pragma unroll
-    for (int q = 0; q < kImgSteps; ++q) c += __popcll(__ballot(op_exists(o, M, cx[q]) & vv[q]));
+    for (int q = 0; q < kImgSteps; ++q)
+      c += __popcll(__ballot(op_exists(o, M, cx[q]) & vv[q] & (so[q] != k)));
     if (lane == 0) tcnt[(int64_t)k * ntl + wt] = c;
   }
 }
@@ -446,12 +452,14 @@ __global__ void __launch_bounds__(kBlock) k_img_temit(const u64* __restrict__ ba
 #pragma unroll
   for (int q = 0; q < kImgSteps; ++q) {
     const int64_t i = wt * kImgTile + q * 64 + lane;
-    const bool v = i < n && !(skip && skip[i]);
+    const int sc = (skip && i < n) ? skip[i] : 0;
+    const bool v = i < n && sc != 1;
+    const int so = sc >= 2 ? sc - 2 : -1;  // the one op this row skips
     const Row<W> r0 = kHold ? rh[kHold ? q : 0] : load_row<W>(basis, i < n ? i : 0);
     const RowCtx cx = kHold ? ch[kHold ? q : 0] : row_ctx<W>(T, L, M, F, r0);
     for (int k = 0; k < T.n; ++k) {
       const OpView o = op_view(T, k, F);
-      const bool ok = op_exists(o, M, cx) & v;
+      const bool ok = op_exists(o, M, cx) & v & (so != k);
       const u64 bl = __ballot(ok);
       if (bl == 0) continue;  // wave-uniform
       const int cnt = __popcll(bl);
@@ -1488,6 +1496,23 @@ int paces_holstein_num_runs(const paces_holstein_t* h) {
   if (!h) return PACES_E_INVALID;
   int rc = build_image_ops(h, &T);
   return rc ? rc : T.n;
+}
+
+// Ops come in inverse pairs (2m of positive direction, 2m + 1 its inverse). A row added to a
+// basis as op k's image of a row s has op k^1's image s, already in the basis; when k is of
+// positive direction that image is a pure duplicate: its pair with s is the one op k's image
+// marked from s's side (a negative-direction image whose other end is a basis row gives no CSR
+// entry). Codes for the union of [basis, op 0 images, op 1 images, ...]: 1 for the basis run
+// (skip the row), 2 + (k ^ 1) for a positive-direction op k's run (skip that op only), else 0.
+int paces_holstein_inverse_skip_codes(const paces_holstein_t* h, uint8_t* codes, int32_t cap) {
+  OpTable T;
+  if (!h || !codes) return PACES_E_INVALID;
+  int rc = build_image_ops(h, &T);
+  if (rc) return rc;
+  if (cap < T.n + 1) return PACES_E_INVALID;
+  codes[0] = 1;
+  for (int k = 0; k < T.n; ++k) codes[1 + k] = T.dir[k] > 0 ? (uint8_t)(2 + (k ^ 1)) : 0;
+  return T.n + 1;
 }
 
 size_t paces_holstein_images_workspace_bytes(int64_t n, const paces_holstein_t* h) {

@@ -188,6 +188,7 @@ struct RunsArg {
   int64_t off[PACES_MAX_RUNS + 1];
   int64_t boff[PACES_MAX_RUNS + 1];  // first workgroup of each run in k_mu_rank's grid
   int K;
+  uint8_t code[PACES_MAX_RUNS];  // out_code[row] = code[run of the row's kept copy]
 };
 
 __device__ __forceinline__ int run_of(const RunsArg& R, int64_t g) {
@@ -459,6 +460,7 @@ __global__ void __launch_bounds__(kBlock) k_mu_emit(RunsArg R, const uint32_t* _
                                                     u64* __restrict__ out,
                                                     uint8_t* __restrict__ from0,
                                                     int32_t* __restrict__ cand_pos,
+                                                    uint8_t* __restrict__ out_code,
                                                     int32_t* __restrict__ out_count, RowDiag D) {
   constexpr int kSteps = kScanTile / kBlock;
   constexpr int kWaves = kBlock / 64;
@@ -510,6 +512,7 @@ __global__ void __launch_bounds__(kBlock) k_mu_emit(RunsArg R, const uint32_t* _
     if (f[k]) {
       store_row<W>(out, pos, x[k]);
       if (from0) from0[pos] = (rr[k] == 0) ? 1 : 0;
+      if (out_code) out_code[pos] = R.code[rr[k]];
       if (DG) {
         const double d = row_diag<W>(x[k], D);
         if (D.d64)
@@ -550,8 +553,8 @@ size_t mu_ws_bytes(int64_t C, int K) {
 
 template <int W>
 static int run_mu(const RunsArg& R, uint64_t* out, int32_t* out_count, uint8_t* from0,
-                  int32_t* cand_pos, const RowDiag& D, void* ws, size_t ws_bytes,
-                  hipStream_t st) {
+                  int32_t* cand_pos, uint8_t* out_code, const RowDiag& D, void* ws,
+                  size_t ws_bytes, hipStream_t st) {
   int64_t C = R.off[R.K];
   if (ws_bytes < mu_ws_bytes(C, R.K)) return PACES_E_WORKSPACE;
   if (C == 0) {
@@ -601,11 +604,11 @@ static int run_mu(const RunsArg& R, uint64_t* out, int32_t* out_count, uint8_t* 
   if (D.d64 || D.d32)
     hipLaunchKernelGGL((k_mu_emit<W, true>), dim3((unsigned)nst), dim3(kBlock), 0, st, R,
                        (const uint32_t*)code, (const int32_t*)offs, (u64*)out, from0, cand_pos,
-                       out_count, D);
+                       out_code, out_count, D);
   else
     hipLaunchKernelGGL((k_mu_emit<W, false>), dim3((unsigned)nst), dim3(kBlock), 0, st, R,
                        (const uint32_t*)code, (const int32_t*)offs, (u64*)out, from0, cand_pos,
-                       out_count, D);
+                       out_code, out_count, D);
   return launch_status();
 }
 
@@ -846,10 +849,13 @@ size_t paces_merge_unique_workspace_bytes(int64_t total, int32_t K, int32_t W) {
 
 static int merge_unique_impl(const paces_runs_t* runs, int32_t W, uint64_t* out_rows,
                              int32_t* out_count, uint8_t* out_from_run0, int32_t* cand_pos,
-                             const RowDiag& D, void* ws, size_t ws_bytes, void* stream) {
+                             const uint8_t* run_codes, uint8_t* out_code, const RowDiag& D,
+                             void* ws, size_t ws_bytes, void* stream) {
   if (!runs || runs->K < 1 || runs->K > PACES_MAX_RUNS || !out_count) return PACES_E_INVALID;
+  if (out_code && !run_codes) return PACES_E_INVALID;
   RunsArg R;
   R.K = runs->K;
+  for (int s = 0; s < PACES_MAX_RUNS; ++s) R.code[s] = (run_codes && s < R.K) ? run_codes[s] : 0;
   R.off[0] = 0;
   for (int s = 0; s < R.K; ++s) {
     if (runs->len[s] < 0 || (runs->len[s] > 0 && !runs->ptr[s])) return PACES_E_INVALID;
@@ -866,16 +872,26 @@ static int merge_unique_impl(const paces_runs_t* runs, int32_t W, uint64_t* out_
   if (R.off[R.K] >= 2147483647LL) return PACES_E_INVALID;
   if (R.off[R.K] > 0 && !out_rows) return PACES_E_INVALID;
   if (W < 1 || W > PACES_MAX_ROW_WORDS) return PACES_E_UNSUPPORTED;
-  PACES_DISPATCH_W(W, run_mu, R, out_rows, out_count, out_from_run0, cand_pos, D, ws, ws_bytes,
-                   (hipStream_t)stream);
+  PACES_DISPATCH_W(W, run_mu, R, out_rows, out_count, out_from_run0, cand_pos, out_code, D, ws,
+                   ws_bytes, (hipStream_t)stream);
 }
 
 int paces_merge_unique(const paces_runs_t* runs, int32_t W, uint64_t* out_rows,
                        int32_t* out_count, uint8_t* out_from_run0, int32_t* cand_pos, void* ws,
                        size_t ws_bytes, void* stream) {
   RowDiag D = {};
-  return merge_unique_impl(runs, W, out_rows, out_count, out_from_run0, cand_pos, D, ws, ws_bytes,
-                           stream);
+  return merge_unique_impl(runs, W, out_rows, out_count, out_from_run0, cand_pos, nullptr, nullptr,
+                           D, ws, ws_bytes, stream);
+}
+
+int paces_merge_unique_coded(const paces_runs_t* runs, int32_t W, uint64_t* out_rows,
+                             int32_t* out_count, uint8_t* out_from_run0, int32_t* cand_pos,
+                             const uint8_t* run_codes, uint8_t* out_code, void* ws,
+                             size_t ws_bytes, void* stream) {
+  if (!out_code || !run_codes) return PACES_E_INVALID;
+  RowDiag D = {};
+  return merge_unique_impl(runs, W, out_rows, out_count, out_from_run0, cand_pos, run_codes,
+                           out_code, D, ws, ws_bytes, stream);
 }
 
 int paces_merge_unique_diag(const paces_runs_t* runs, int32_t W, uint64_t* out_rows,
@@ -897,8 +913,8 @@ int paces_merge_unique_diag(const paces_runs_t* runs, int32_t W, uint64_t* out_r
   D.de = h->delta_eps;
   D.d64 = diag;
   D.d32 = diag32;
-  return merge_unique_impl(runs, W, out_rows, out_count, out_from_run0, cand_pos, D, ws, ws_bytes,
-                           stream);
+  return merge_unique_impl(runs, W, out_rows, out_count, out_from_run0, cand_pos, nullptr, nullptr,
+                           D, ws, ws_bytes, stream);
 }
 
 size_t paces_reassign_workspace_bytes(int64_t R) { return lb_ws_bytes(R); }

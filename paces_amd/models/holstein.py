@@ -266,9 +266,13 @@ class Hamiltonian(HamiltonianFramework):
             imgs, offs, src, nup = self._images(basis, want_src=True, slot="expand")
             runs = [(basis, 0, basis.shape[0])]
             runs += [(imgs, offs[k], offs[k + 1] - offs[k]) for k in range(len(offs) - 1)]
-            rows, from0, cpos = engine.merge_unique(runs, self.totwordwidth, self._ws,
-                                                    want_from0=True, want_cand_pos=True)
-            last = (basis.shape[0], offs, src, nup, cpos, from0)
+            # each row's skip code for the next expansion: the basis rows give no images (theirs
+            # are in already), and a row added by a positive-direction op skips the inverse op,
+            # whose image is the row it came from
+            rows, _, cpos, skipc = engine.merge_unique(runs, self.totwordwidth, self._ws,
+                                                       want_cand_pos=True,
+                                                       run_codes=self._inverse_skip_codes())
+            last = (basis.shape[0], offs, src, nup, cpos, skipc)
             if want_start_pos:
                 start_pos = follow(cpos, basis.shape[0])
             basis = rows
@@ -283,9 +287,9 @@ class Hamiltonian(HamiltonianFramework):
             # The expanded basis already holds the images of the rows it was expanded from, so
             # the new rows are its union with the images of the rows the expansion added only;
             # the pairs still take every image of the expanded basis.
-            n_prev, offs_a, src_a, nup_a, cpos_a, from0_a = last
+            n_prev, offs_a, src_a, nup_a, cpos_a, skip_a = last
             prior = (n_prev, offs_a, cpos_a, src_a, nup_a)
-            imgs, offs, src, nup = self._images(basis, want_src=True, skip=from0_a,
+            imgs, offs, src, nup = self._images(basis, want_src=True, skip=skip_a,
                                                 slot="expand2")
             runs = [(basis, 0, basis.shape[0])]
             runs += [(imgs, offs[k], offs[k + 1] - offs[k]) for k in range(len(offs) - 1)]
@@ -303,6 +307,14 @@ class Hamiltonian(HamiltonianFramework):
             dbg["vib_coupling"] = [basis.shape[0] - self._vib_plus_count(offs_all)]
         self.last_basis_e = basis
         return (diag, new_inds), mats, dbg
+
+    def _inverse_skip_codes(self):
+        """paces_holstein_inverse_skip_codes: the union-run codes that make each expanded row's
+        skip code for the next expansion."""
+        codes = (C.c_uint8 * (_hip.MAX_RUNS + 1))()
+        k = _lib.paces_holstein_inverse_skip_codes(C.byref(self._hs), codes, _hip.MAX_RUNS + 1)
+        check(min(k, 0), "holstein_inverse_skip_codes")
+        return list(codes[:k])
 
     @staticmethod
     def _prior_arg(prior):
