@@ -561,7 +561,9 @@ int paces_batch_taylor_finish(int64_t R, const void* x_even, int32_t mixed, doub
  * j == 0, else x[(j-1) % 2], and writes x[j % 2], so x_even = x[0]); terms of stopped
  * trajectories do nothing, and
  * a series not finished after nterms continues with paces_batch_taylor_term_tuned from term
- * nterms with the same state. */
+ * nterms with the same state. Unlike paces_batch_taylor_begin, the launch does not copy v into
+ * res: terms 0 and 1 read v as the partial sum and term 1 stores res. tol must be below 1 (no
+ * series then stops at term 0) and nterms at least 2. */
 #define PACES_SERIES_MAX_TERMS 64
 typedef struct {
   int64_t R;                              /* rows this step (informational; grids use R_cap) */

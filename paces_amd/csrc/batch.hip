@@ -749,7 +749,9 @@ __device__ __forceinline__ void blk_max2(u64 a, u64 b, u64* __restrict__ part) {
 
 // Launch shape (tuned): BS threads per workgroup, RPT rows per thread (t, t + BS, ...), a
 // segment-aligned tile of BS * RPT rows; begin and every term of one series use the same shape.
-template <typename F, int BS, int RPT>
+// WRITE_RES false (the graph's series): res is left unwritten; the first odd term reads v in its
+// place (res_src of btay_term_body)
+template <typename F, int BS, int RPT, bool WRITE_RES = true>
 __device__ __forceinline__ void btay_begin_body(const double2* __restrict__ v,
                                                 double2* __restrict__ res,
                                                 typename Vec2<F>::T* __restrict__ x0,
@@ -764,7 +766,7 @@ __device__ __forceinline__ void btay_begin_body(const double2* __restrict__ v,
     const int64_t i = T.r0 + q * BS + threadIdx.x;
     if (i < T.r1) {
       double2 a = v[i];
-      res[i] = a;
+      if (WRITE_RES) res[i] = a;
       typename Vec2<F>::T c;
       c.x = (F)a.x;
       c.y = (F)a.y;
@@ -859,8 +861,9 @@ template <typename F, int BS, int RPT, int NT, bool WITH_RES, typename VL>
 __device__ __forceinline__ void btay_rows_phased(const F* __restrict__ diag, const CsrSetB64& H,
                                                  const typename Vec2<F>::T* __restrict__ xin,
                                                  typename Vec2<F>::T* __restrict__ xout,
-                                                 double2* __restrict__ res, F c, bool odd,
-                                                 const SegTile& T, u64& mt, u64& mr,
+                                                 double2* __restrict__ res,
+                                                 const double2* __restrict__ res_src, F c,
+                                                 bool odd, const SegTile& T, u64& mt, u64& mr,
                                                  const VL& vl) {
   typedef typename Vec2<F>::T V;
   int64_t ii[RPT];
@@ -870,7 +873,7 @@ __device__ __forceinline__ void btay_rows_phased(const F* __restrict__ diag, con
   for (int q = 0; q < RPT; ++q) {
     ii[q] = T.r0 + q * BS + threadIdx.x;
     vv[q] = ii[q] < T.r1;
-    if (WITH_RES) rr[q] = res[vv[q] ? ii[q] : T.r0];
+    if (WITH_RES) rr[q] = res_src[vv[q] ? ii[q] : T.r0];
   }
   V h[RPT], xi[RPT];
   hx_rows_phased<F, V, RPT, NT>(ii, vv, T.r0, diag, H, xin, h, xi, vl);
@@ -886,14 +889,17 @@ template <typename F, int BS, int RPT, bool WITH_RES, typename VL>
 __device__ __forceinline__ void btay_rows_all(const F* __restrict__ diag, const CsrSetB64& H,
                                               const typename Vec2<F>::T* __restrict__ xin,
                                               typename Vec2<F>::T* __restrict__ xout,
-                                              double2* __restrict__ res, F c, bool odd,
+                                              double2* __restrict__ res,
+                                              const double2* __restrict__ res_src, F c, bool odd,
                                               const SegTile& T, u64& mt, u64& mr, const VL& vl) {
   // the phased form needs about 40 VGPRs per FP64 row: not at 1024 threads x 4 rows (128 max)
   constexpr bool phased = !(sizeof(F) == 8 && BS * RPT > 2048);
   if (phased && H.nterms == 2) {
-    btay_rows_phased<F, BS, RPT, 2, WITH_RES>(diag, H, xin, xout, res, c, odd, T, mt, mr, vl);
+    btay_rows_phased<F, BS, RPT, 2, WITH_RES>(diag, H, xin, xout, res, res_src, c, odd, T, mt,
+                                              mr, vl);
   } else if (phased && H.nterms == 1) {
-    btay_rows_phased<F, BS, RPT, 1, WITH_RES>(diag, H, xin, xout, res, c, odd, T, mt, mr, vl);
+    btay_rows_phased<F, BS, RPT, 1, WITH_RES>(diag, H, xin, xout, res, res_src, c, odd, T, mt,
+                                              mr, vl);
   } else {
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
@@ -902,7 +908,7 @@ __device__ __forceinline__ void btay_rows_all(const F* __restrict__ diag, const 
         typename Vec2<F>::T xi;
         const typename Vec2<F>::T h = hx_row_t<F>(i, diag, H, xin, xi, vl);
         double2 r = {0.0, 0.0};
-        if (WITH_RES) r = res[i];
+        if (WITH_RES) r = res_src[i];
         btay_row_out<F>(i, h, xi, r, c, odd, WITH_RES, xout, res, mt, mr);
       }
     }
@@ -913,12 +919,13 @@ template <typename F, int BS, int RPT, typename VL>
 __device__ __forceinline__ void btay_rows_par(const F* __restrict__ diag, const CsrSetB64& H,
                                               const typename Vec2<F>::T* __restrict__ xin,
                                               typename Vec2<F>::T* __restrict__ xout,
-                                              double2* __restrict__ res, F c, bool odd,
+                                              double2* __restrict__ res,
+                                              const double2* __restrict__ res_src, F c, bool odd,
                                               const SegTile& T, u64& mt, u64& mr, const VL& vl) {
   if (sizeof(F) == 8 || odd)
-    btay_rows_all<F, BS, RPT, true>(diag, H, xin, xout, res, c, odd, T, mt, mr, vl);
+    btay_rows_all<F, BS, RPT, true>(diag, H, xin, xout, res, res_src, c, odd, T, mt, mr, vl);
   else
-    btay_rows_all<F, BS, RPT, false>(diag, H, xin, xout, res, c, odd, T, mt, mr, vl);
+    btay_rows_all<F, BS, RPT, false>(diag, H, xin, xout, res, res_src, c, odd, T, mt, mr, vl);
 }
 
 // The partial sum res is stored on odd terms only. On an even term j the sum res + t_j is formed
@@ -928,7 +935,8 @@ __device__ __forceinline__ void btay_rows_par(const F* __restrict__ diag, const 
 template <typename F, int BS, int RPT>
 __device__ __forceinline__ void btay_term_body(
     const F* __restrict__ diag, const CsrSetB64& H, const typename Vec2<F>::T* __restrict__ xin,
-    typename Vec2<F>::T* __restrict__ xout, double2* __restrict__ res, F c, int j,
+    typename Vec2<F>::T* __restrict__ xout, double2* __restrict__ res,
+    const double2* __restrict__ res_src, F c, int j,
     const int32_t* __restrict__ seg_off, const int64_t* __restrict__ boff, int ntraj,
     const paces_batch_taylor_t* __restrict__ ts, paces_batch_taylor_hdr_t* __restrict__ hdr,
     u64* __restrict__ part) {
@@ -954,9 +962,11 @@ __device__ __forceinline__ void btay_term_body(
             (F)(H.code[t] ? (m < H.sqrt_len ? H.sqrt_table[m] * v : 0.0) : v);
     }
     __syncthreads();
-    btay_rows_par<F, BS, RPT>(diag, H, xin, xout, res, c, odd, T, mt, mr, CodedVals<F>{vt});
+    btay_rows_par<F, BS, RPT>(diag, H, xin, xout, res, res_src, c, odd, T, mt, mr,
+                              CodedVals<F>{vt});
   } else {
-    btay_rows_par<F, BS, RPT>(diag, H, xin, xout, res, c, odd, T, mt, mr, PlainVals<F>{});
+    btay_rows_par<F, BS, RPT>(diag, H, xin, xout, res, res_src, c, odd, T, mt, mr,
+                              PlainVals<F>{});
   }
   blk_max2<BS>(mt, mr, part);
 }
@@ -988,7 +998,8 @@ __global__ void __launch_bounds__(BS) k_btay_term(
     const int32_t* __restrict__ seg_off, const int64_t* __restrict__ boff, int ntraj,
     const paces_batch_taylor_t* __restrict__ ts, paces_batch_taylor_hdr_t* __restrict__ hdr,
     u64* __restrict__ part) {
-  btay_term_body<F, BS, RPT>(diag, H, xin, xout, res, c, j, seg_off, boff, ntraj, ts, hdr, part);
+  btay_term_body<F, BS, RPT>(diag, H, xin, xout, res, res, c, j, seg_off, boff, ntraj, ts, hdr,
+                             part);
 }
 
 // After the series: a trajectory whose last term j is even has res one term behind; its last
@@ -1081,9 +1092,10 @@ __global__ void __launch_bounds__(BS) k_bser_begin(const paces_batch_series_t* _
                                                    int ntraj) {
   const SeriesDev* d = (const SeriesDev*)desc;
   typedef typename Vec2<F>::T V;
-  btay_begin_body<F, BS, RPT>(gen((PACES_GPTR(const double2))d->v),
-                              gen((PACES_GPTR(double2))d->res), gen((PACES_GPTR(V))d->x[2]),
-                              gen(d->seg_off), ser_boff(d), ntraj, ser_part(d, ntraj));
+  btay_begin_body<F, BS, RPT, false>(gen((PACES_GPTR(const double2))d->v),
+                                     gen((PACES_GPTR(double2))d->res),
+                                     gen((PACES_GPTR(V))d->x[2]), gen(d->seg_off), ser_boff(d),
+                                     ntraj, ser_part(d, ntraj));
 }
 
 __global__ void __launch_bounds__(64) k_bser_reduce(const paces_batch_series_t* __restrict__ desc,
@@ -1113,9 +1125,15 @@ __global__ void __launch_bounds__(BS) k_bser_term(const paces_batch_series_t* __
   typedef typename Vec2<F>::T V;
   const V* xin = gen((PACES_GPTR(const V))(j == 0 ? d->x[2] : d->x[(j - 1) & 1]));
   V* xout = gen((PACES_GPTR(V))d->x[j & 1]);
-  btay_term_body<F, BS, RPT>(gen((PACES_GPTR(const F))d->diag), H, xin, xout,
-                             gen((PACES_GPTR(double2))d->res), (F)d->c[j], j, gen(d->seg_off),
-                             ser_boff(d), ntraj, gen(d->ts), gen(d->hdr), ser_part(d, ntraj));
+  // the begin kernel left res unwritten: until term 1 has stored it, the partial sum is v
+  // (term 0 reads it for the FP64 stopping test, term 1 adds to it); with a tolerance below 1
+  // no series stops at term 0 ((||v|| + ||t_0||) / ||v + t_0|| is about 1), so the finish pass
+  // never adds to an unwritten res
+  double2* res = gen((PACES_GPTR(double2))d->res);
+  const double2* res_src = j <= 1 ? gen((PACES_GPTR(const double2))d->v) : res;
+  btay_term_body<F, BS, RPT>(gen((PACES_GPTR(const F))d->diag), H, xin, xout, res, res_src,
+                             (F)d->c[j], j, gen(d->seg_off), ser_boff(d), ntraj, gen(d->ts),
+                             gen(d->hdr), ser_part(d, ntraj));
 }
 
 size_t btay_ws_bytes(int64_t R, int ntraj) {
@@ -1290,7 +1308,7 @@ size_t paces_batch_series_desc_bytes(void) { return sizeof(paces_batch_series_t)
 int paces_batch_series_launch(const paces_batch_series_t* desc, int64_t R_cap, int32_t ntraj,
                               int32_t mixed, int32_t nterms, const paces_launch_cfg_t* cfg,
                               void* stream) {
-  if (!desc || R_cap < 0 || ntraj < 1 || nterms < 0 || nterms > PACES_SERIES_MAX_TERMS ||
+  if (!desc || R_cap < 0 || ntraj < 1 || nterms < 2 || nterms > PACES_SERIES_MAX_TERMS ||
       !cfg_ok(cfg))
     return PACES_E_INVALID;
   hipStream_t st = (hipStream_t)stream;

@@ -1236,8 +1236,11 @@ class BatchedTimeEvolution:
             csr_values = "explicit"  # codes index a table of at most CODE_TABLE entries
         self.csr_values = csr_values
         # graph=True: the Taylor series runs as one HIP graph per step (captured once per row
-        # capacity and term count, replayed with a rewritten device descriptor)
+        # capacity and term count, replayed with a rewritten device descriptor); it runs at least
+        # two terms, and a series with a tolerance of 1 or more could stop at term 0
         self.graph = bool(graph)
+        if self.graph and (self.m_star < 2 or not self.tol < 1.0):
+            raise ValueError("the graph-replayed series needs m_star >= 2 and taylor_tol < 1")
         self.graph_pool = graph_pool if graph_pool is not None else SeriesGraphPool(self.device)
         self._graph_terms = 8 if precision == "mixed" else 16
         self.last_launch_shape = None
@@ -1370,7 +1373,8 @@ class BatchedTimeEvolution:
         if info is not None and info.size:
             # terms 0 .. j of the slowest trajectory; a longer series continues eagerly
             self._graph_terms = int(info["j"].max()) + 1
-        nterms = max(1, min(self.m_star, self._graph_terms, _hip.SERIES_MAX_TERMS))
+        # at least two terms: the graph's series reads v as the partial sum until term 1 stores it
+        nterms = max(2, min(self.m_star, self._graph_terms, _hip.SERIES_MAX_TERMS))
         wsp, wsn = self._tws.get(_lib.paces_batch_taylor_workspace_bytes(cap, h.ntraj))
         d = _hip.BatchSeries()
         d.R = R
