@@ -153,20 +153,23 @@ TRAJ_DESC = ("traj256: {n} independent 1D Holstein chains (open), N in {{16,24,3
              "N//2, dt=0.05, enlarge_steps=1 (draws seeded 0)")
 
 
-def btay_term_bytes(R, mats, mixed, coded=False):
+def btay_term_bytes(R, mats, mixed, form="explicit"):
     """Algorithmic HBM bytes of one batched Taylor term over R rows, averaged over an even and an
-    odd term (the roofline loop times both alike): diagonal (4 or 8 per row), each term's indptr
-    (4(R+1)), indices + values (4 + 4 or 8 per nonzero; coded: 4 + 1 per vib nonzero and 4 per
-    hopping nonzero), x_in and x_out (8 or 16 per row each), and the complex128 partial sum:
-    FP64 form, read on every term and written on odd terms (16 + 8 per row); mixed form, read
-    and written on odd terms only (16 per row)."""
+    odd term (the roofline loop times both alike): diagonal (4 or 8 per row), the matrix, x_in
+    and x_out (8 or 16 per row each), and the complex128 partial sum: FP64 form, read on every
+    term and written on odd terms (16 + 8 per row); mixed form, read and written on odd terms
+    only (16 per row). The matrix: slot form, 4 bytes per slot (nslots per row, every term);
+    CSR forms, each term's indptr (4(R+1)) and indices + values (4 + 4 or 8 per nonzero; coded:
+    4 + 1 per vib nonzero and 4 per hopping nonzero)."""
     f = 4 if mixed else 8
+    res = 16 if mixed else 24
+    if form == "slots":
+        return f * R + 4 * mats[0].slotmat.layout.nslots * R + 4 * f * R + res * R
     nnz = sum(m.nnz for m in mats)
-    if coded:
+    if form == "coded":
         vals = sum(m.nnz for m in mats if getattr(m, "code", None) is not None)
     else:
         vals = f * nnz
-    res = 16 if mixed else 24
     return f * R + 4 * (R + 1) * len(mats) + 4 * nnz + vals + 4 * f * R + res * R
 
 
@@ -227,11 +230,11 @@ def measure_batched_term(bte, reps=20):
     e1.synchronize()
     del keep, diag
     return (e0.elapsed_time(e1) / reps * 1e-3,
-            btay_term_bytes(R, mats, mixed, bte.csr_values == "coded"), reps, shape)
+            btay_term_bytes(R, mats, mixed, bte.csr_values), reps, shape)
 
 
 def batched_leg(draws, precision, warmup, steps, label, graph=False, autotune=False,
-                launch_shape=None, csr_values="coded", record_steps=(), graph_pool=None,
+                launch_shape=None, csr_values="slots", record_steps=(), graph_pool=None,
                 verbose=True):
     """Batched run over t = TRAJ_DT * arange(warmup + steps + 1) (each step's dt is the
     difference of successive times, as in the reference's timeline): step 0 and ``warmup``
@@ -388,6 +391,8 @@ def run_traj256(a):
     rows = bte.numstates
     value = units / elapsed
     bte_csr_values = bte.csr_values
+    nslots = next(iter(bte.sparse_mats_dict.values())).slotmat.layout.nslots \
+        if bte.csr_values == "slots" else 0
     sec, nbytes, reps, shape = measure_batched_term(bte)
     kshape = "%d, %d" % shape if shape else "256, 1"
     achieved = nbytes / sec / 1e9
@@ -395,10 +400,12 @@ def run_traj256(a):
             "frac": round(achieved / 8000.0, 4),
             "traffic": (int(BTAY_TRAFFIC_BYTES_PER_ROW[(prec, shape, bte.csr_values)] * rows)
                         if (prec, shape, bte.csr_values) in BTAY_TRAFFIC_BYTES_PER_ROW else None),
-            "kernel": "k_btay_term<%s, %s> (paces_batch_taylor_term_%s, %s CSR values, the timed "
+            "kernel": "k_btay_term<%s, %s, %d> (paces_batch_taylor_term_%s, %s, the timed "
                       "steps' launch shape, stopping test off)"
                       % ("float" if prec == "mixed" else "double", kshape,
-                         "coded" if bte.csr_values == "coded" else "tuned", bte.csr_values),
+                         nslots, {"slots": "slots", "coded": "coded"}.get(bte.csr_values, "tuned"),
+                         {"slots": "slot matrix", "coded": "coded CSR values",
+                          "explicit": "explicit CSR values"}[bte.csr_values]),
             "avg_us": round(sec * 1e6, 2), "bytes_per_launch": int(nbytes), "rows": int(rows),
             "launches_timed": reps}
     del bte, pool
@@ -679,9 +686,10 @@ def main(argv=None):
     ap.add_argument("--term-shape", default=None,
                     help="traj256: fix the Taylor term's launch shape, e.g. 256,4 (threads, rows "
                          "per thread), instead of autotuning it (for reproducible profiles)")
-    ap.add_argument("--csr-values", default="coded", choices=["coded", "explicit"],
-                    help="traj256: the batched series reads coded CSR values (a code byte per "
-                         "vib entry) or explicit FP32/FP64 values (identical results)")
+    ap.add_argument("--csr-values", default="slots", choices=["slots", "coded", "explicit"],
+                    help="traj256: the batched series reads one slot matrix (a uint32 per row "
+                         "and op), coded CSR values (a code byte per vib entry) or explicit "
+                         "FP32/FP64 values (identical results)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="traj256: tune the Taylor-term launch shape")
     ap.add_argument("--learned-leg", type=int, default=1,

@@ -265,6 +265,46 @@ int paces_holstein_pairs_fill_split(int64_t R, int64_t nbasis, const int64_t* ru
                                     uint8_t* const* code, void* ws, size_t ws_bytes,
                                     void* stream);
 
+/* ---- Slot matrices for batched trajectories (no reference counterpart: the reference keeps one
+ * CSR per term, time_evolution.py:365-372). Every off-diagonal entry sits in one array of
+ * nslots x R uint32 slots (slot m of row i at slots[m * stride + i]), one slot per (row, op):
+ * an op moves one key field of a row by a fixed
+ * amount (a hop the exciton position, a vib op the phonon count at the exciton), so the column
+ * order of a row's neighbours under a term's ops is the same for every row of an open lattice.
+ * Slot slot_off[t] + r of row i holds row i's entry of term t whose column has rank r among the
+ * term's ops: the column in the low col_bits bits and, for a vib term, the pair's upper phonon
+ * count m above them (hopping: 0); PACES_SLOT_EMPTY where row i has no such entry. Read in slot
+ * order, a row's slots of term t are its CSR row of term t (paces_holstein_pairs_fill_coded),
+ * columns ascending, and the entry values are the coded ones (paces_csr_coded_t). */
+#define PACES_SLOT_EMPTY 0xffffffffu
+#define PACES_MAX_SLOTS 8
+typedef struct {
+  int32_t nterms;
+  int32_t nslots;                         /* slots per row: 4 or 8 (the ops, padded) */
+  int32_t col_bits;                       /* column bits of a slot; the code sits above them */
+  int32_t sqrt_len;                       /* entries of sqrt_table */
+  int32_t slot_off[PACES_MAX_TERMS + 1];  /* term t's slots: [slot_off[t], slot_off[t + 1]) */
+  int32_t vib[PACES_MAX_TERMS];           /* 1: term t's slots carry codes (a vib term) */
+  int32_t stride;                         /* rows: slot m of row i is slots[m * stride + i] */
+  const uint32_t* slots;                  /* device, nslots x stride */
+  const double* term_value;               /* device, ntraj x nterms */
+  const double* sqrt_table;               /* device, sqrt_len */
+} paces_slots_t;
+/* The host fields of S (every field but the three pointers) for model h and R rows (stride R):
+ * the ops' slots, padded with slots that stay empty to 4 or 8 per row. Returns
+ * PACES_E_UNSUPPORTED when h has no slot form: not batched, a periodic lattice (its wrap-around
+ * hops give row-dependent column orders), more than PACES_MAX_SLOTS ops, more than
+ * PACES_CODE_TABLE codes, or R beyond what col_bits can index. */
+int paces_holstein_slot_layout(const paces_holstein_t* h, int64_t R, paces_slots_t* S);
+/* The slot matrix of the pairs: arguments as paces_holstein_pairs_fill_split, with prior null for
+ * one image list (paces_holstein_pairs_fill_coded). Writes all nslots x R slots (stride R, nslots
+ * from paces_holstein_slot_layout); no count pass, no workspace. */
+int paces_holstein_pairs_slots(int64_t R, int64_t nbasis, const int64_t* run_offsets,
+                               const int32_t* cand_pos, const uint8_t* in_basis,
+                               const int32_t* img_src, const uint8_t* img_nup,
+                               const paces_prior_images_t* prior, const paces_holstein_t* h,
+                               uint32_t* slots, void* stream);
+
 /* ---- D1/D2: per-term canonical CSR of the Holstein H on the new basis (generate_mel +
  * _add_hc_and_generate_coo + coo_matrix().tocsr(): hamiltonian.py:383-547,
  * time_evolution.py:365-372). An entry (i, j) of term t exists when rows i, j are linked by t and
@@ -545,6 +585,14 @@ int paces_batch_taylor_term_coded(int64_t R, const void* diag, int32_t mixed,
                                   const int32_t* seg_off, int32_t ntraj, paces_batch_taylor_t* ts,
                                   paces_batch_taylor_hdr_t* hdr, void* ws, size_t ws_bytes,
                                   const paces_launch_cfg_t* cfg, void* stream);
+/* paces_batch_taylor_term_coded with the slot matrix H (paces_slots_t): each row's slots are
+ * loaded in one pass and its neighbours gathered in the next; the same results bit for bit */
+int paces_batch_taylor_term_slots(int64_t R, const void* diag, int32_t mixed,
+                                  const paces_slots_t* H, const void* x_in, void* x_out,
+                                  double* res, double c, int32_t j, double tol,
+                                  const int32_t* seg_off, int32_t ntraj, paces_batch_taylor_t* ts,
+                                  paces_batch_taylor_hdr_t* hdr, void* ws, size_t ws_bytes,
+                                  const paces_launch_cfg_t* cfg, void* stream);
 /* after the series (terms and/or a replayed paces_batch_series_launch): adds to res the last
  * term of every trajectory whose last term index ts[t].j_ind is even, from x_even (the buffer
  * every even term wrote); same launch shape and workspace as the series */
@@ -588,11 +636,19 @@ typedef struct {
   const uint8_t* code[PACES_MAX_TERMS];
   const double* term_value;
   const double* sqrt_table;
+  /* slots != null (with coded != 0): the Hamiltonian is the slot matrix described by the fields
+   * below, sqrt_len, term_value and sqrt_table (paces_slots_t); indptr, indices and code unused */
+  const uint32_t* slots;
+  int32_t nslots;
+  int32_t col_bits;
+  int32_t slot_off[PACES_MAX_TERMS + 1];
+  int32_t slot_vib[PACES_MAX_TERMS];
+  int32_t slot_stride;
 } paces_batch_series_t;
 size_t paces_batch_series_desc_bytes(void); /* sizeof(paces_batch_series_t), for bindings */
 int paces_batch_series_launch(const paces_batch_series_t* desc, int64_t R_cap, int32_t ntraj,
-                              int32_t mixed, int32_t nterms, const paces_launch_cfg_t* cfg,
-                              void* stream);
+                              int32_t mixed, int32_t nslots, int32_t nterms,
+                              const paces_launch_cfg_t* cfg, void* stream);
 
 /* ---- N3 candidate scorer (no reference counterpart): 64 inputs -> 3 x 256 GELU -> 1 logit,
  * BF16 matrix operands with FP32 accumulation on the matrix cores. params is the FP32 master

@@ -26,6 +26,9 @@ TERM_VIB = 1
 _lib = None
 
 
+E_UNSUPPORTED = -2  # PACES_E_UNSUPPORTED
+
+
 class PacesError(RuntimeError):
     pass
 
@@ -85,6 +88,18 @@ class CsrCoded(C.Structure):
                 ("sqrt_table", C.c_void_p)]
 
 
+SLOT_EMPTY = 0xFFFFFFFF
+MAX_SLOTS = 8
+
+
+class Slots(C.Structure):
+    """paces_slots_t: every off-diagonal entry in one uint32 slot per (row, op)."""
+    _fields_ = [("nterms", C.c_int32), ("nslots", C.c_int32), ("col_bits", C.c_int32),
+                ("sqrt_len", C.c_int32), ("slot_off", C.c_int32 * (MAX_TERMS + 1)),
+                ("vib", C.c_int32 * MAX_TERMS), ("stride", C.c_int32), ("slots", C.c_void_p),
+                ("term_value", C.c_void_p), ("sqrt_table", C.c_void_p)]
+
+
 class PriorImages(C.Structure):
     """paces_prior_images_t: the previous expansion's part of the image list."""
     _fields_ = [("nprev", C.c_int64), ("run_offsets", C.POINTER(C.c_int64)),
@@ -111,7 +126,10 @@ class BatchSeries(C.Structure):
                 ("seg_off", C.c_void_p), ("ts", C.c_void_p), ("hdr", C.c_void_p),
                 ("ws", C.c_void_p), ("tol", C.c_double), ("c", C.c_double * SERIES_MAX_TERMS),
                 ("coded", C.c_int32), ("sqrt_len", C.c_int32), ("code", C.c_void_p * MAX_TERMS),
-                ("term_value", C.c_void_p), ("sqrt_table", C.c_void_p)]
+                ("term_value", C.c_void_p), ("sqrt_table", C.c_void_p),
+                ("slots", C.c_void_p), ("nslots", C.c_int32), ("col_bits", C.c_int32),
+                ("slot_off", C.c_int32 * (MAX_TERMS + 1)), ("slot_vib", C.c_int32 * MAX_TERMS),
+                ("slot_stride", C.c_int32)]
 
 
 class TaylorState(C.Structure):
@@ -176,6 +194,10 @@ _SIGNATURES = {
                                                   C.POINTER(Layout), C.POINTER(Holstein),
                                                   C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
                                                   C.POINTER(_P), C.POINTER(_P), _P, _SZ, _P]),
+    "paces_holstein_slot_layout": (C.c_int, [C.POINTER(Holstein), _I64, C.POINTER(Slots)]),
+    "paces_holstein_pairs_slots": (C.c_int, [_I64, _I64, C.POINTER(C.c_int64), _P, _P, _P, _P,
+                                             C.POINTER(PriorImages), C.POINTER(Holstein), _P,
+                                             _P]),
     "paces_holstein_csr_workspace_bytes": (_SZ, [_I64, C.POINTER(Holstein)]),
     "paces_holstein_csr_count": (C.c_int, [_P, _I64, _P, C.POINTER(Layout), C.POINTER(Holstein),
                                            _P, _P, _SZ, _P]),
@@ -242,14 +264,17 @@ _SIGNATURES = {
     "paces_batch_taylor_term_coded": (C.c_int, [_I64, _P, _I32, C.POINTER(CsrCoded), _P, _P,
                                                 _P, _D, _I32, _D, _P, _I32, _P, _P, _P, _SZ,
                                                 C.POINTER(LaunchCfg), _P]),
+    "paces_batch_taylor_term_slots": (C.c_int, [_I64, _P, _I32, C.POINTER(Slots), _P, _P,
+                                                _P, _D, _I32, _D, _P, _I32, _P, _P, _P, _SZ,
+                                                C.POINTER(LaunchCfg), _P]),
     "paces_batch_taylor_finish": (C.c_int, [_I64, _P, _I32, _P, _P, _I32, _P, _P, _SZ,
                                             C.POINTER(LaunchCfg), _P]),
     "paces_rows_varying_bits": (C.c_int, [_P, _I64, _I32, _P, _P]),
     "paces_sort_rows_workspace_bytes": (_SZ, [_I64, _I32]),
     "paces_sort_rows": (C.c_int, [_P, _I64, _I32, _P, _I32, _P, _P, _P, _P, _SZ, _P]),
     "paces_batch_series_desc_bytes": (_SZ, []),
-    "paces_batch_series_launch": (C.c_int, [_P, _I64, _I32, _I32, _I32, C.POINTER(LaunchCfg),
-                                            _P]),
+    "paces_batch_series_launch": (C.c_int, [_P, _I64, _I32, _I32, _I32, _I32,
+                                            C.POINTER(LaunchCfg), _P]),
     "paces_mlp_weights_bf16": (C.c_int, [_P, _P, _P]),
     "paces_mlp_train_workspace_bytes": (_SZ, [_I64]),
     "paces_mlp_grad": (C.c_int, [_P, _P, _P, _I64, _F, _P, _P, _P, _P, _P, _SZ, _P]),

@@ -76,6 +76,64 @@ class CodedCsrMatrix:
         return CsrMatrix(self.indptr, self.indices, self.data, self.shape).to_scipy()
 
 
+class SlotMatrix:
+    """Every off-diagonal term of a batched Hamiltonian in one (nslots, R) array of uint32 slots
+    (paces_slots_t, held as torch.int32, slot-major): slot layout.slot_off[t] + r of row i is row
+    i's entry of term t whose column has rank r among the term's ops -- its column in the low
+    col_bits bits and, for a vib term, the pair's upper phonon count above them -- or SLOT_EMPTY
+    (-1 as int32). ``term(t, value_fn)`` views term t as a coded CSR."""
+
+    def __init__(self, slots, layout, shape):
+        self.slots = slots
+        self.layout = layout
+        self.shape = shape
+        layout.slots = slots.data_ptr()
+
+    def term(self, t, value_fn):
+        return SlotTermMatrix(self, t, value_fn)
+
+
+class SlotTermMatrix(CodedCsrMatrix):
+    """Term t of a SlotMatrix with the CodedCsrMatrix interface. indptr, indices and code are
+    formed from the slots on first use (a row's occupied slots of the term, in slot order, are
+    its CSR row: columns ascending); the batched series reads the slots themselves."""
+
+    def __init__(self, sm, t, value_fn):
+        self.slotmat = sm
+        self.shape = sm.shape
+        self.term = t
+        self._value_fn = value_fn
+        self._data = None
+        self._csr = None
+
+    def _form(self):
+        if self._csr is None:
+            lay = self.slotmat.layout
+            a, b = lay.slot_off[self.term], lay.slot_off[self.term + 1]
+            sub = self.slotmat.slots[a:b].t()  # (R, the term's slots)
+            occ = sub != -1
+            counts = occ.sum(dim=1, dtype=torch.int32)
+            indptr = torch.zeros(sub.shape[0] + 1, dtype=torch.int32, device=sub.device)
+            torch.cumsum(counts, 0, out=indptr[1:])
+            words = sub[occ].long() & 0xFFFFFFFF
+            indices = (words & ((1 << lay.col_bits) - 1)).to(torch.int32)
+            code = (words >> lay.col_bits).to(torch.uint8) if lay.vib[self.term] else None
+            self._csr = (indptr, indices, code)
+        return self._csr
+
+    @property
+    def indptr(self):
+        return self._form()[0]
+
+    @property
+    def indices(self):
+        return self._form()[1]
+
+    @property
+    def code(self):
+        return self._form()[2]
+
+
 def csr_coded(mats, sqrt_len, term_value_ptr, sqrt_table_ptr):
     """paces_csr_coded_t of a list of CodedCsrMatrix (term order)."""
     s = _hip.CsrCoded()

@@ -661,7 +661,19 @@ struct CsrSetB64 {
   const uint8_t* code[PACES_MAX_TERMS];
   const double* term_value;
   const double* sqrt_table;
+  // slot matrix (paces_slots_t, coded values): indptr, indices and code unused
+  const uint32_t* slots;  // slot m of row i at slots[m * slot_stride + i]
+  int64_t slot_stride;
+  int nslots;
+  int col_bits;
+  int slot_begin;  // bit m set: slot m >= 1 is the first slot of a term
+  int slot_vib[PACES_MAX_TERMS];
 };
+
+// term t takes coded values of the form sqrt_table[m] * v (a vib term)
+__device__ __forceinline__ bool coded_vib(const CsrSetB64& H, int t) {
+  return H.slots ? H.slot_vib[t] != 0 : H.code[t] != nullptr;
+}
 
 // Coded entry values: a vib entry's raw load is its code byte (an absent entry reads a byte of
 // the term's indptr; the mask keeps any byte inside the table), a hopping entry loads nothing;
@@ -928,11 +940,104 @@ __device__ __forceinline__ void btay_rows_par(const F* __restrict__ diag, const 
     btay_rows_all<F, BS, RPT, false>(diag, H, xin, xout, res, res_src, c, odd, T, mt, mr, vl);
 }
 
+// ---- the term over a slot matrix: a row's slots come in one load with its own x and diagonal,
+// then every occupied slot's x is gathered (an empty slot gathers a row of the tile and the value
+// goes unused), then the sums: the diagonal, then each term's occupied slots in slot order (its
+// CSR row, columns ascending), as hx_rows_phased sums a CSR row.
+template <int NS>
+__device__ __forceinline__ void load_slots(const uint32_t* __restrict__ p, int64_t stride,
+                                           int64_t i, uint32_t (&s)[NS]) {
+  static_assert(NS == 4 || NS == 8, "slots per row");
+#pragma unroll
+  for (int m = 0; m < NS; ++m) s[m] = p[m * stride + i];
+}
+
+// rows c0 .. c0 + RC - 1 of the thread's RPT rows (t, t + BS, ...)
+template <typename F, int BS, int RC, int NS, bool WITH_RES>
+__device__ __forceinline__ void btay_rows_slots_chunk(
+    int c0, const F* __restrict__ diag, const CsrSetB64& H,
+    const typename Vec2<F>::T* __restrict__ xin, typename Vec2<F>::T* __restrict__ xout,
+    double2* __restrict__ res, const double2* __restrict__ res_src, F c, bool odd,
+    const SegTile& T, u64& mt, u64& mr, const F* vt) {
+  typedef typename Vec2<F>::T V;
+  const uint32_t colmask = (1u << H.col_bits) - 1u;
+  int64_t ii[RC];
+  bool vv[RC];
+  double2 rr[RC];
+  V xi[RC];
+  F d[RC];
+  uint32_t sw[RC][NS];
+#pragma unroll
+  for (int q = 0; q < RC; ++q) {
+    ii[q] = T.r0 + (c0 + q) * BS + threadIdx.x;
+    vv[q] = ii[q] < T.r1;
+    const int64_t i = vv[q] ? ii[q] : T.r0;
+    if (WITH_RES) rr[q] = res_src[i];
+    xi[q] = xin[i];
+    d[q] = diag[i];
+    load_slots<NS>(H.slots, H.slot_stride, i, sw[q]);
+  }
+  V xj[RC][NS];
+#pragma unroll
+  for (int q = 0; q < RC; ++q)
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+      const bool ok = sw[q][m] != PACES_SLOT_EMPTY;
+      xj[q][m] = xin[ok ? (int64_t)(sw[q][m] & colmask) : T.r0];
+    }
+#pragma unroll
+  for (int q = 0; q < RC; ++q) {
+    if (!vv[q]) continue;
+    F yr = d[q] * xi[q].x, yi = d[q] * xi[q].y;
+    F tr = 0, ti = 0;
+    int t = 0;
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+      if ((H.slot_begin >> m) & 1) {  // the next term's slots begin
+        yr = yr + tr;
+        yi = yi + ti;
+        tr = 0;
+        ti = 0;
+        ++t;
+      }
+      if (sw[q][m] != PACES_SLOT_EMPTY) {
+        const F av = vt[t * PACES_CODE_TABLE + (int)(sw[q][m] >> H.col_bits)];
+        tr = tr + av * xj[q][m].x;
+        ti = ti + av * xj[q][m].y;
+      }
+    }
+    yr = yr + tr;
+    yi = yi + ti;
+    V h;
+    h.x = yr;
+    h.y = yi;
+    btay_row_out<F>(ii[q], h, xi[q], rr[q], c, odd, WITH_RES, xout, res, mt, mr);
+  }
+}
+
+template <typename F, int BS, int RPT, int NS, bool WITH_RES>
+__device__ __forceinline__ void btay_rows_slots(const F* __restrict__ diag, const CsrSetB64& H,
+                                                const typename Vec2<F>::T* __restrict__ xin,
+                                                typename Vec2<F>::T* __restrict__ xout,
+                                                double2* __restrict__ res,
+                                                const double2* __restrict__ res_src, F c,
+                                                bool odd, const SegTile& T, u64& mt, u64& mr,
+                                                const F* vt) {
+  // rows per load phase: the gathered x of a phase within 64 registers
+  constexpr int kFit = 256 / (NS * (int)sizeof(typename Vec2<F>::T));
+  constexpr int RC = kFit < 1 ? 1 : (kFit < RPT ? kFit : RPT);
+  static_assert(RPT % RC == 0, "row chunks");
+#pragma unroll
+  for (int c0 = 0; c0 < RPT; c0 += RC)
+    btay_rows_slots_chunk<F, BS, RC, NS, WITH_RES>(c0, diag, H, xin, xout, res, res_src, c, odd, T,
+                                                   mt, mr, vt);
+}
+
 // The partial sum res is stored on odd terms only. On an even term j the sum res + t_j is formed
 // in registers for the stopping test; the next (odd) term adds its own input (t_j) and then its
 // output, (res + t_j) + t_(j+1), the same roundings in the same order as storing every term. A
 // trajectory whose series ends on an even term gets its last term added by k_btay_finish.
-template <typename F, int BS, int RPT>
+template <typename F, int BS, int RPT, int NS>
 __device__ __forceinline__ void btay_term_body(
     const F* __restrict__ diag, const CsrSetB64& H, const typename Vec2<F>::T* __restrict__ xin,
     typename Vec2<F>::T* __restrict__ xout, double2* __restrict__ res,
@@ -959,12 +1064,21 @@ __device__ __forceinline__ void btay_term_body(
       const int m = threadIdx.x;
       if (m < PACES_CODE_TABLE)
         vt[t * PACES_CODE_TABLE + m] =
-            (F)(H.code[t] ? (m < H.sqrt_len ? H.sqrt_table[m] * v : 0.0) : v);
+            (F)(coded_vib(H, t) ? (m < H.sqrt_len ? H.sqrt_table[m] * v : 0.0) : v);
     }
     __syncthreads();
-    btay_rows_par<F, BS, RPT>(diag, H, xin, xout, res, res_src, c, odd, T, mt, mr,
-                              CodedVals<F>{vt});
-  } else {
+    if constexpr (NS > 0) {
+      if (sizeof(F) == 8 || odd)
+        btay_rows_slots<F, BS, RPT, NS, true>(diag, H, xin, xout, res, res_src, c, odd, T, mt, mr,
+                                              vt);
+      else
+        btay_rows_slots<F, BS, RPT, NS, false>(diag, H, xin, xout, res, res_src, c, odd, T, mt,
+                                               mr, vt);
+    } else {
+      btay_rows_par<F, BS, RPT>(diag, H, xin, xout, res, res_src, c, odd, T, mt, mr,
+                                CodedVals<F>{vt});
+    }
+  } else if constexpr (NS == 0) {
     btay_rows_par<F, BS, RPT>(diag, H, xin, xout, res, res_src, c, odd, T, mt, mr,
                               PlainVals<F>{});
   }
@@ -991,15 +1105,15 @@ __global__ void __launch_bounds__(64) k_btay_reduce(const u64* __restrict__ part
   btay_reduce_body(part, boff, ts, hdr, fin, j, tol, use_prev);
 }
 
-template <typename F, int BS, int RPT>
+template <typename F, int BS, int RPT, int NS>
 __global__ void __launch_bounds__(BS) k_btay_term(
     const F* __restrict__ diag, CsrSetB64 H, const typename Vec2<F>::T* __restrict__ xin,
     typename Vec2<F>::T* __restrict__ xout, double2* __restrict__ res, F c, int j,
     const int32_t* __restrict__ seg_off, const int64_t* __restrict__ boff, int ntraj,
     const paces_batch_taylor_t* __restrict__ ts, paces_batch_taylor_hdr_t* __restrict__ hdr,
     u64* __restrict__ part) {
-  btay_term_body<F, BS, RPT>(diag, H, xin, xout, res, res, c, j, seg_off, boff, ntraj, ts, hdr,
-                             part);
+  btay_term_body<F, BS, RPT, NS>(diag, H, xin, xout, res, res, c, j, seg_off, boff, ntraj, ts,
+                                 hdr, part);
 }
 
 // After the series: a trajectory whose last term j is even has res one term behind; its last
@@ -1053,8 +1167,18 @@ struct SeriesDev {
   PACES_GPTR(const uint8_t) code[PACES_MAX_TERMS];
   PACES_GPTR(const double) term_value;
   PACES_GPTR(const double) sqrt_table;
+  PACES_GPTR(const uint32_t) slots;
+  int32_t nslots;
+  int32_t col_bits;
+  int32_t slot_off[PACES_MAX_TERMS + 1];
+  int32_t slot_vib[PACES_MAX_TERMS];
+  int32_t slot_stride;
 };
 static_assert(sizeof(SeriesDev) == sizeof(paces_batch_series_t), "descriptor layout");
+static_assert(offsetof(SeriesDev, slots) == offsetof(paces_batch_series_t, slots),
+              "descriptor layout");
+static_assert(offsetof(SeriesDev, slot_vib) == offsetof(paces_batch_series_t, slot_vib),
+              "descriptor layout");
 static_assert(offsetof(SeriesDev, code) == offsetof(paces_batch_series_t, code),
               "descriptor layout");
 static_assert(offsetof(SeriesDev, c) == offsetof(paces_batch_series_t, c), "descriptor layout");
@@ -1105,7 +1229,7 @@ __global__ void __launch_bounds__(64) k_bser_reduce(const paces_batch_series_t* 
                    use_prev);
 }
 
-template <typename F, int BS, int RPT>
+template <typename F, int BS, int RPT, int NS>
 __global__ void __launch_bounds__(BS) k_bser_term(const paces_batch_series_t* __restrict__ desc,
                                                   int ntraj, int j) {
   const SeriesDev* d = (const SeriesDev*)desc;
@@ -1122,6 +1246,16 @@ __global__ void __launch_bounds__(BS) k_bser_term(const paces_batch_series_t* __
     H.data[t] = gen(d->data[t]);
     H.code[t] = gen(d->code[t]);
   }
+  H.slots = gen(d->slots);
+  H.slot_stride = d->slot_stride;
+  H.nslots = d->nslots;
+  H.col_bits = d->col_bits;
+  H.slot_begin = 0;
+#pragma unroll
+  for (int t = 0; t < PACES_MAX_TERMS; ++t) {
+    H.slot_vib[t] = d->slot_vib[t];
+    if (t >= 1 && t < H.nterms) H.slot_begin |= 1 << d->slot_off[t];
+  }
   typedef typename Vec2<F>::T V;
   const V* xin = gen((PACES_GPTR(const V))(j == 0 ? d->x[2] : d->x[(j - 1) & 1]));
   V* xout = gen((PACES_GPTR(V))d->x[j & 1]);
@@ -1131,14 +1265,62 @@ __global__ void __launch_bounds__(BS) k_bser_term(const paces_batch_series_t* __
   // never adds to an unwritten res
   double2* res = gen((PACES_GPTR(double2))d->res);
   const double2* res_src = j <= 1 ? gen((PACES_GPTR(const double2))d->v) : res;
-  btay_term_body<F, BS, RPT>(gen((PACES_GPTR(const F))d->diag), H, xin, xout, res, res_src,
-                             (F)d->c[j], j, gen(d->seg_off), ser_boff(d), ntraj, gen(d->ts),
-                             gen(d->hdr), ser_part(d, ntraj));
+  btay_term_body<F, BS, RPT, NS>(gen((PACES_GPTR(const F))d->diag), H, xin, xout, res,
+                                 res_src, (F)d->c[j], j, gen(d->seg_off), ser_boff(d), ntraj,
+                                 gen(d->ts), gen(d->hdr), ser_part(d, ntraj));
 }
 
 size_t btay_ws_bytes(int64_t R, int ntraj) {
   int64_t nblk = (R + 127) / 128 + ntraj + 1;  // the smallest tile of any launch shape
   return a256((size_t)(ntraj + 1) * 8) + a256((size_t)nblk * 2 * 8);
+}
+
+static void no_slots(CsrSetB64* H) {
+  H->slots = nullptr;
+  H->slot_stride = 0;
+  H->nslots = 0;
+  H->col_bits = 0;
+  H->slot_begin = 0;
+  for (int t = 0; t < PACES_MAX_TERMS; ++t) H->slot_vib[t] = 0;
+}
+
+// the host fields of a slot matrix, checked (its pointers are the caller's)
+static bool slots_ok(const paces_slots_t* S, int64_t R) {
+  if (!S || S->nterms < 1 || S->nterms > PACES_MAX_TERMS) return false;
+  if (S->nslots != 4 && S->nslots != 8) return false;
+  if (S->col_bits < 1 || S->col_bits > 31 || R > ((int64_t)1 << S->col_bits) - 1) return false;
+  if (S->sqrt_len < 1 || S->sqrt_len > PACES_CODE_TABLE) return false;
+  if ((int64_t)S->sqrt_len > ((int64_t)1 << (32 - S->col_bits))) return false;
+  if (S->slot_off[0] != 0 || S->slot_off[S->nterms] > S->nslots) return false;
+  for (int t = 0; t < S->nterms; ++t)
+    if (S->slot_off[t + 1] < S->slot_off[t] + 2) return false;  // every term has slots
+  if (!S->term_value || !S->sqrt_table || (R > 0 && !S->slots) || S->stride < R) return false;
+  return true;
+}
+
+static bool csr_slots(const paces_slots_t* S, int64_t R, CsrSetB64* H) {
+  if (!slots_ok(S, R)) return false;
+  H->nterms = S->nterms;
+  H->coded = 1;
+  H->sqrt_len = S->sqrt_len;
+  H->term_value = S->term_value;
+  H->sqrt_table = S->sqrt_table;
+  for (int t = 0; t < PACES_MAX_TERMS; ++t) {
+    H->indptr[t] = nullptr;
+    H->indices[t] = nullptr;
+    H->data[t] = nullptr;
+    H->code[t] = nullptr;
+  }
+  H->slots = S->slots;
+  H->slot_stride = S->stride;
+  H->nslots = S->nslots;
+  H->col_bits = S->col_bits;
+  H->slot_begin = 0;
+  for (int t = 0; t < PACES_MAX_TERMS; ++t) {
+    H->slot_vib[t] = t < S->nterms ? (S->vib[t] ? 1 : 0) : 0;
+    if (t >= 1 && t < S->nterms) H->slot_begin |= 1 << S->slot_off[t];
+  }
+  return true;
 }
 
 static bool csr_b(const paces_csr_set_t* h64, const paces_csr32_set_t* h32, CsrSetB64* H) {
@@ -1149,6 +1331,7 @@ static bool csr_b(const paces_csr_set_t* h64, const paces_csr32_set_t* h32, CsrS
   H->sqrt_len = 0;
   H->term_value = nullptr;
   H->sqrt_table = nullptr;
+  no_slots(H);
   for (int t = 0; t < PACES_MAX_TERMS; ++t) H->code[t] = nullptr;
   for (int t = 0; t < PACES_MAX_TERMS; ++t) {
     H->indptr[t] = t < n ? (h64 ? h64->indptr[t] : h32->indptr[t]) : nullptr;
@@ -1165,6 +1348,7 @@ static bool csr_coded(const paces_csr_coded_t* hc, CsrSetB64* H) {
   if (hc->nterms > 0 && !hc->term_value) return false;
   H->nterms = hc->nterms;
   H->coded = 1;
+  no_slots(H);
   H->sqrt_len = hc->sqrt_len;
   H->term_value = hc->term_value;
   H->sqrt_table = hc->sqrt_table;
@@ -1306,10 +1490,10 @@ int paces_batch_taylor_begin_tuned(int64_t R, const double* v, double* res, void
 size_t paces_batch_series_desc_bytes(void) { return sizeof(paces_batch_series_t); }
 
 int paces_batch_series_launch(const paces_batch_series_t* desc, int64_t R_cap, int32_t ntraj,
-                              int32_t mixed, int32_t nterms, const paces_launch_cfg_t* cfg,
-                              void* stream) {
+                              int32_t mixed, int32_t nslots, int32_t nterms,
+                              const paces_launch_cfg_t* cfg, void* stream) {
   if (!desc || R_cap < 0 || ntraj < 1 || nterms < 2 || nterms > PACES_SERIES_MAX_TERMS ||
-      !cfg_ok(cfg))
+      !cfg_ok(cfg) || (nslots != 0 && nslots != 4 && nslots != 8))
     return PACES_E_INVALID;
   hipStream_t st = (hipStream_t)stream;
   int B, RR;
@@ -1325,15 +1509,24 @@ int paces_batch_series_launch(const paces_batch_series_t* desc, int64_t R_cap, i
 #undef SB_LAUNCH
   hipLaunchKernelGGL(k_bser_reduce, dim3(ntraj), dim3(64), 0, st, desc, ntraj, 1, 0, 0);
   for (int j = 0; j < nterms; ++j) {
-#define ST_LAUNCH(BB, QQ)                                                                      \
+#define ST_LAUNCH_NS(BB, QQ, NS)                                                               \
   if (mixed)                                                                                   \
-    hipLaunchKernelGGL((k_bser_term<float, BB, QQ>), dim3(gs), dim3(BB), 0, st, desc, ntraj,   \
-                       j);                                                                     \
+    hipLaunchKernelGGL((k_bser_term<float, BB, QQ, NS>), dim3(gs), dim3(BB), 0, st, desc,      \
+                       ntraj, j);                                                              \
   else                                                                                         \
-    hipLaunchKernelGGL((k_bser_term<double, BB, QQ>), dim3(gs), dim3(BB), 0, st, desc, ntraj,  \
-                       j)
+    hipLaunchKernelGGL((k_bser_term<double, BB, QQ, NS>), dim3(gs), dim3(BB), 0, st, desc,     \
+                       ntraj, j)
+#define ST_LAUNCH(BB, QQ)          \
+  if (nslots == 4) {               \
+    ST_LAUNCH_NS(BB, QQ, 4);       \
+  } else if (nslots == 8) {        \
+    ST_LAUNCH_NS(BB, QQ, 8);       \
+  } else {                         \
+    ST_LAUNCH_NS(BB, QQ, 0);       \
+  }
     PACES_CFG_SWITCH(B, RR, ST_LAUNCH)
 #undef ST_LAUNCH
+#undef ST_LAUNCH_NS
     hipLaunchKernelGGL(k_bser_reduce, dim3(ntraj), dim3(64), 0, st, desc, ntraj, 0, j,
                        (mixed && !(j & 1)) ? 1 : 0);
   }
@@ -1375,6 +1568,18 @@ int paces_batch_taylor_term_coded(int64_t R, const void* diag, int32_t mixed,
                           ts, hdr, ws, ws_bytes, cfg, stream);
 }
 
+int paces_batch_taylor_term_slots(int64_t R, const void* diag, int32_t mixed,
+                                  const paces_slots_t* Hs, const void* x_in, void* x_out,
+                                  double* res, double c, int32_t j, double tol,
+                                  const int32_t* seg_off, int32_t ntraj, paces_batch_taylor_t* ts,
+                                  paces_batch_taylor_hdr_t* hdr, void* ws, size_t ws_bytes,
+                                  const paces_launch_cfg_t* cfg, void* stream) {
+  CsrSetB64 H;
+  if (!csr_slots(Hs, R, &H)) return PACES_E_INVALID;
+  return btay_term_launch(R, diag, mixed ? 1 : 0, H, x_in, x_out, res, c, j, tol, seg_off, ntraj,
+                          ts, hdr, ws, ws_bytes, cfg, stream);
+}
+
 }  // extern "C"
 
 static int btay_term_launch(int64_t R, const void* diag, int mixed, const CsrSetB64& H,
@@ -1394,19 +1599,29 @@ static int btay_term_launch(int64_t R, const void* diag, int mixed, const CsrSet
   int B, RR;
   cfg_get(cfg, &B, &RR);
   const unsigned gs = seg_grid(R, ntraj, B * RR);
-#define BT_LAUNCH(BB, QQ)                                                                      \
+  const int nslots = H.slots ? H.nslots : 0;
+#define BT_LAUNCH_NS(BB, QQ, NS)                                                               \
   if (mixed)                                                                                   \
-    hipLaunchKernelGGL((k_btay_term<float, BB, QQ>), dim3(gs), dim3(BB), 0, st,                \
+    hipLaunchKernelGGL((k_btay_term<float, BB, QQ, NS>), dim3(gs), dim3(BB), 0, st,            \
                        (const float*)diag, H, (const float2*)x_in, (float2*)x_out,             \
                        (double2*)res, (float)c, j, seg_off, boff, ntraj,                       \
                        (const paces_batch_taylor_t*)ts, hdr, part);                            \
   else                                                                                         \
-    hipLaunchKernelGGL((k_btay_term<double, BB, QQ>), dim3(gs), dim3(BB), 0, st,               \
+    hipLaunchKernelGGL((k_btay_term<double, BB, QQ, NS>), dim3(gs), dim3(BB), 0, st,           \
                        (const double*)diag, H, (const double2*)x_in, (double2*)x_out,          \
                        (double2*)res, c, j, seg_off, boff, ntraj,                              \
                        (const paces_batch_taylor_t*)ts, hdr, part)
+#define BT_LAUNCH(BB, QQ)          \
+  if (nslots == 4) {               \
+    BT_LAUNCH_NS(BB, QQ, 4);       \
+  } else if (nslots == 8) {        \
+    BT_LAUNCH_NS(BB, QQ, 8);       \
+  } else {                         \
+    BT_LAUNCH_NS(BB, QQ, 0);       \
+  }
   PACES_CFG_SWITCH(B, RR, BT_LAUNCH)
 #undef BT_LAUNCH
+#undef BT_LAUNCH_NS
   hipLaunchKernelGGL(k_btay_reduce, dim3(ntraj), dim3(64), 0, st, (const u64*)part, boff, ts, hdr,
                      0, (int)j, tol, (mixed && !(j & 1)) ? 1 : 0);
   return launch_status();

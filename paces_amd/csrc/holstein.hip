@@ -1094,6 +1094,46 @@ __global__ void __launch_bounds__(kBlock) k_pairs_place(int64_t R, int64_t nbasi
   put_entry(out, t, b, v, nu);
 }
 
+// Slot matrices (paces_slots_t): the slot of (row, op) is fixed, so a pair writes its two
+// entries straight into their slots; an image pair {b, e} of op k fills slot rank[k] of row
+// q = pos(b) and slot rank[k ^ 1] of row p = pos(e) within their term's slots. Each slot has one
+// writer at most (as the flags of k_pairs_mark), and k_slots_clear has set every slot empty first.
+// Slots are stored slot-major: an image run is monotone in both q and p, so its writes sweep two
+// slot columns in order and each cache line of a column is completed within a short window (a
+// row-major layout had every line written piecemeal by four runs far apart in time).
+// (n slots from a 16-byte aligned base: a 16-byte store per thread, the tail one slot at a time)
+__global__ void __launch_bounds__(kBlock) k_slots_clear(uint32_t* __restrict__ p, int64_t n) {
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (i + 4 <= n) {
+    *(uint4*)(p + i) = make_uint4(PACES_SLOT_EMPTY, PACES_SLOT_EMPTY, PACES_SLOT_EMPTY,
+                                  PACES_SLOT_EMPTY);
+  } else {
+    for (int64_t j = i; j < n; ++j) p[j] = PACES_SLOT_EMPTY;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_pairs_slot_place(int64_t R, int64_t nbasis, ImgRuns IR,
+                                                             PriorImgs A, OpTable T, SlotArg SL,
+                                                             int col_bits,
+                                                             const int32_t* __restrict__ cand_pos,
+                                                             const uint8_t* __restrict__ inB,
+                                                             const int32_t* __restrict__ src,
+                                                             const uint8_t* __restrict__ nup,
+                                                             uint32_t* __restrict__ slots) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= A.n + IR.off[IR.n]) return;
+  int k, nu;
+  int32_t p, q, s;
+  image_ends(e, A, IR, nbasis, cand_pos, src, nup, k, p, q, s, nu);
+  const bool pin = inB[p < 0 ? 0 : p] != 0;
+  if (p < 0 || q < 0) return;
+  if (T.dir[k] < 0 && pin) return;
+  const int t = T.term[k];
+  const uint32_t code = T.kind[k] == PACES_TERM_VIB ? (uint32_t)nu << col_bits : 0u;
+  slots[(int64_t)(SL.foff[t] + SL.rank[k]) * R + q] = (uint32_t)p | code;
+  slots[(int64_t)(SL.foff[t] + SL.rank[k ^ 1]) * R + p] = (uint32_t)q | code;
+}
+
 // the largest argument list of the pairs kernels (k_pairs_place) stays within 4 KiB
 static_assert(sizeof(ImgRuns) + sizeof(PriorImgs) + sizeof(OpTable) + sizeof(SlotArg) +
                   sizeof(ModelArg) + sizeof(CsrOut) + 16 * 8 <= 4096,
@@ -1833,6 +1873,76 @@ int paces_holstein_pairs_fill_split(int64_t R, int64_t nbasis, const int64_t* ru
   return pairs_fill_impl(R, nbasis, run_offsets, cand_pos, in_basis, img_src, img_nup, prior,
                          code ? nullptr : basis, h, code ? nullptr : lay, indptr, indices, data,
                          data32, code, ws, ws_bytes, stream);
+}
+
+// the slot table of h (SL) and the host fields of S, or a nonzero status
+static int slot_layout(const paces_holstein_t* h, int64_t R, OpTable* T, SlotArg* SL,
+                       paces_slots_t* S) {
+  if (!h || !S || R < 0) return PACES_E_INVALID;
+  if (!h->batched || h->periodic) return PACES_E_UNSUPPORTED;
+  if (h->nterms < 1 || h->nterms > PACES_MAX_TERMS) return PACES_E_UNSUPPORTED;
+  if (h->sqrt_table_len < 1 || h->sqrt_table_len > PACES_CODE_TABLE) return PACES_E_UNSUPPORTED;
+  int rc = build_image_ops(h, T);
+  if (rc) return rc;
+  if (T->n < 2 || T->n > PACES_MAX_SLOTS || (T->n & 1)) return PACES_E_UNSUPPORTED;
+  if (!build_slots(*T, h, SL)) return PACES_E_UNSUPPORTED;
+  for (int t = 0; t < h->nterms; ++t)
+    if (SL->S[t] < 2) return PACES_E_UNSUPPORTED;  // every term starts a run of slots
+  // the codes (upper phonon counts 0 .. sqrt_len - 1) take the top bits; a column is at most
+  // R - 1 <= 2^col_bits - 2, so no entry is all ones (PACES_SLOT_EMPTY)
+  int cb = 1;
+  while ((1 << cb) < h->sqrt_table_len) ++cb;
+  const int col_bits = 32 - cb;
+  if (R > ((int64_t)1 << col_bits) - 1 || R > 2147483647LL) return PACES_E_UNSUPPORTED;
+  S->nterms = h->nterms;
+  S->nslots = T->n <= 4 ? 4 : 8;
+  S->col_bits = col_bits;
+  S->sqrt_len = h->sqrt_table_len;
+  for (int t = 0; t <= PACES_MAX_TERMS; ++t)
+    S->slot_off[t] = t < h->nterms ? SL->foff[t] : T->n;
+  for (int t = 0; t < PACES_MAX_TERMS; ++t)
+    S->vib[t] = (t < h->nterms && h->terms[t].kind == PACES_TERM_VIB) ? 1 : 0;
+  S->stride = (int32_t)R;
+  return PACES_OK;
+}
+
+int paces_holstein_slot_layout(const paces_holstein_t* h, int64_t R, paces_slots_t* S) {
+  OpTable T;
+  SlotArg SL;
+  return slot_layout(h, R, &T, &SL, S);
+}
+
+int paces_holstein_pairs_slots(int64_t R, int64_t nbasis, const int64_t* run_offsets,
+                               const int32_t* cand_pos, const uint8_t* in_basis,
+                               const int32_t* img_src, const uint8_t* img_nup,
+                               const paces_prior_images_t* prior, const paces_holstein_t* h,
+                               uint32_t* slots, void* stream) {
+  OpTable T;
+  SlotArg SL;
+  paces_slots_t S;
+  if (!run_offsets || nbasis < 0) return PACES_E_INVALID;
+  int rc = slot_layout(h, R, &T, &SL, &S);
+  if (rc) return rc;
+  if (R > 0 && (!slots || ((uintptr_t)slots & 15))) return PACES_E_INVALID;
+  ImgRuns IR;
+  rc = img_runs_arg(T, run_offsets, &IR);
+  if (rc) return rc;
+  PriorImgs A;
+  rc = prior_arg(T, prior, &A);
+  if (rc) return rc;
+  const int64_t nimg = A.n + IR.off[IR.n];
+  if (nimg > 0 && (!cand_pos || !in_basis || (IR.off[IR.n] > 0 && (!img_src || !img_nup))))
+    return PACES_E_INVALID;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = R * S.nslots;
+  if (n > 0)
+    hipLaunchKernelGGL(k_slots_clear, dim3(grid_for((n + 3) / 4, kBlock)), dim3(kBlock), 0, st,
+                       slots, n);
+  if (nimg > 0 && R > 0)
+    hipLaunchKernelGGL(k_pairs_slot_place, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, R,
+                       nbasis, IR, A, T, SL, S.col_bits, cand_pos, in_basis, img_src,
+                       img_nup, slots);
+  return launch_status();
 }
 
 size_t paces_holstein_csr_workspace_bytes(int64_t R, const paces_holstein_t* h) {

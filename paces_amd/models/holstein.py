@@ -380,6 +380,8 @@ class Hamiltonian(HamiltonianFramework):
         if value_dtype == "coded":
             return self._pairs_csr_coded(R, nbasis, offs, cand_pos, in_basis, src, nup, rows,
                                          prior)
+        if value_dtype == "slots":
+            return self._pairs_slots(R, nbasis, offs, cand_pos, in_basis, src, nup, rows, prior)
         if value_dtype not in (torch.float64, torch.float32):
             raise ValueError("CSR values are float64, float32 or coded.")
         h = self._hs
@@ -448,6 +450,36 @@ class Hamiltonian(HamiltonianFramework):
         return {name: CodedCsrMatrix(ip, ix[:nnz_h[t]], cd[:nnz_h[t]] if cd is not None else None,
                                      (R, R), t, lambda m, rows=rows: self._coded_values(m, rows))
                 for t, (name, ip, ix, cd) in enumerate(parts)}
+
+    def slot_layout(self, R):
+        """paces_slots_t host fields for R rows (paces_holstein_slot_layout), or None when this
+        model has no slot form (not batched, periodic, too many ops or rows)."""
+        lay = _hip.Slots()
+        rc = _lib.paces_holstein_slot_layout(C.byref(self._hs), R, C.byref(lay))
+        if rc == _hip.E_UNSUPPORTED:
+            return None
+        check(rc, "holstein_slot_layout")
+        lay.term_value = self._hs.d_traj_value
+        lay.sqrt_table = self._hs.d_sqrt_table
+        return lay
+
+    def _pairs_slots(self, R, nbasis, offs, cand_pos, in_basis, src, nup, rows, prior=None):
+        """Every term's entries in one slot matrix (paces_holstein_pairs_slots, batched open
+        lattices): no count pass and no host wait; each term is a SlotTermMatrix view."""
+        lay = self.slot_layout(R)
+        if lay is None:
+            raise NotImplementedError("This model has no slot form (batched open lattices only).")
+        slots = torch.empty((lay.nslots, R), dtype=torch.int32, device=self.device)
+        run_off = (C.c_int64 * len(offs))(*offs)
+        pr = self._prior_arg(prior)
+        check(_lib.paces_holstein_pairs_slots(R, nbasis, run_off, ptr(cand_pos), ptr(in_basis),
+                                              ptr(src), ptr(nup),
+                                              C.byref(pr) if pr is not None else None,
+                                              C.byref(self._hs), ptr(slots), stream()),
+              "holstein_pairs_slots")
+        sm = engine.SlotMatrix(slots, lay, (R, R))
+        return {name: sm.term(t, lambda m, rows=rows: self._coded_values(m, rows))
+                for t, name in enumerate(self.offdiag_terms)}
 
     def _coded_values(self, m, rows):
         """float64 values of a coded CSR: the term value of each entry's row's trajectory, times
@@ -1189,7 +1221,7 @@ class BatchedTimeEvolution:
 
     def __init__(self, hamobj: BatchedHamiltonian, maxstates, enlarge_steps=1,
                  precision="mixed", m_star=100, taylor_tol=None, autotune=False, graph=False,
-                 launch_shape=None, csr_values="coded", explosion_cutoff=2.0,
+                 launch_shape=None, csr_values="slots", explosion_cutoff=2.0,
                  on_observables=None, graph_pool=None):
         self.hamobj = hamobj
         # every step ends with the per-trajectory norm and site observables (as the reference's
@@ -1227,11 +1259,15 @@ class BatchedTimeEvolution:
             if launch_shape not in _at.SHAPES:
                 raise ValueError(f"launch_shape must be one of {_at.SHAPES}")
         self.launch_shape = launch_shape
-        # csr_values="coded": the series reads each vib entry's code byte and forms the values
-        # per trajectory in LDS (paces_csr_coded_t); "explicit": FP32 (mixed) or FP64 values.
-        # Both give identical results.
-        if csr_values not in ("coded", "explicit"):
-            raise ValueError("csr_values must be 'coded' or 'explicit'.")
+        # csr_values="slots": one slot matrix for every term (paces_slots_t: a uint32 per
+        # (row, op) holding the column and a vib entry's code; open lattices), the values formed
+        # per trajectory in LDS; "coded": a CSR per term with a code byte per vib entry
+        # (paces_csr_coded_t); "explicit": a CSR per term with FP32 (mixed) or FP64 values.
+        # All give identical results.
+        if csr_values not in ("slots", "coded", "explicit"):
+            raise ValueError("csr_values must be 'slots', 'coded' or 'explicit'.")
+        if csr_values == "slots" and hamobj.slot_layout(0) is None:
+            csr_values = "coded"  # periodic lattices: wrap-around hops order columns per row
         if csr_values == "coded" and hamobj._hs.sqrt_table_len > _hip.CODE_TABLE:
             csr_values = "explicit"  # codes index a table of at most CODE_TABLE entries
         self.csr_values = csr_values
@@ -1288,8 +1324,8 @@ class BatchedTimeEvolution:
         # the mixed-precision series reads FP32 values only: the CSR is built with them
         vdt = torch.float32 if self.precision == "mixed" else torch.float64
         ddt = vdt  # the diagonal too (diag_vals is FP32 in the mixed form)
-        if self.csr_values == "coded":
-            vdt = "coded"
+        if self.csr_values in ("coded", "slots"):
+            vdt = self.csr_values
         (diag, new_inds), mats, dbg = self.hamobj.generate_mel(select, self.enlarge_steps,
                                                                value_dtype=vdt, diag_dtype=ddt)
         self.vector = engine.reassign(self.whoami, self.vector, new_inds, self._ws)
@@ -1315,6 +1351,32 @@ class BatchedTimeEvolution:
         objects, call(R, x_in, x_out, res, c, j, tol, ts, hdr, wsp, wsn, cfgref, st) launching one
         term, fill(desc) writing the Hamiltonian into a series descriptor)."""
         h = self.hamobj
+        if self.csr_values == "slots":
+            diag = engine.to_f32(self.diag_vals) if mixed else self.diag_vals
+            sm = mats[0].slotmat
+            lay = sm.layout
+
+            def call(R, x_in, x_out, res, c, j, tol, ts, hdr, wsp, wsn, cref, st):
+                check(_lib.paces_batch_taylor_term_slots(
+                    R, ptr(diag), 1 if mixed else 0, C.byref(lay), x_in, x_out, res, c, j, tol,
+                    ptr(self.seg_off), h.ntraj, ts, hdr, wsp, wsn, cref, st),
+                    "batch_taylor_term_slots")
+
+            def fill(d):
+                d.nterms_h = lay.nterms
+                d.coded = 1
+                d.sqrt_len = lay.sqrt_len
+                d.term_value = lay.term_value
+                d.sqrt_table = lay.sqrt_table
+                d.slots = lay.slots
+                d.slot_stride = lay.stride
+                d.nslots = lay.nslots
+                d.col_bits = lay.col_bits
+                for t in range(_hip.MAX_TERMS + 1):
+                    d.slot_off[t] = lay.slot_off[t]
+                for t in range(_hip.MAX_TERMS):
+                    d.slot_vib[t] = lay.vib[t]
+            return diag, (sm,), call, fill
         if self.csr_values == "coded":
             diag = engine.to_f32(self.diag_vals) if mixed else self.diag_vals
             hc = engine.csr_coded(mats, h._hs.sqrt_table_len, h._hs.d_traj_value,
@@ -1394,15 +1456,16 @@ class BatchedTimeEvolution:
         pool = self.graph_pool
         pool.desc.copy_(torch.frombuffer(bytearray(bytes(d)), dtype=torch.uint8))
         shape = (cfg.block, cfg.rows_per_thread) if cfg is not None else None
-        key = (cap, h.ntraj, nterms, mixed, shape)
+        nslots = d.nslots if d.slots else 0  # the term kernel instance: CSR (0) or slots
+        key = (cap, h.ntraj, nterms, mixed, shape, nslots)
         g = pool.graphs.get(key)
         if g is None:
             g = torch.cuda.CUDAGraph()
             cref = C.byref(cfg) if cfg is not None else None
             with torch.cuda.graph(g):
                 check(_lib.paces_batch_series_launch(ptr(pool.desc), cap, h.ntraj,
-                                                     1 if mixed else 0, nterms, cref, stream()),
-                      "batch_series_launch")
+                                                     1 if mixed else 0, nslots, nterms, cref,
+                                                     stream()), "batch_series_launch")
             pool.graphs[key] = g
         g.replay()
         return nterms

@@ -84,3 +84,40 @@ def test_struct_layouts_match_the_library():
     assert lib.paces_batch_series_desc_bytes() == C.sizeof(_hip.BatchSeries)
     assert C.sizeof(_hip.BatchTaylor) == 40  # 3 doubles, 2 int32, the last exact norm
     assert C.sizeof(_hip.TaylorState) == 64
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_slot_layout_host_checks():
+    """paces_holstein_slot_layout: a batched open chain gets four slots per row (the two hops,
+    then the two vib ops, each term's slots in column order), a code field wide enough for the
+    phonon counts and a column field for the rows; periodic or unbatched models and row counts
+    beyond the column field are refused on the host."""
+    import ctypes as C
+    from paces_amd import _hip
+    lib = _hip.load()
+    h = _hip.Holstein()
+    h.nchain, h.lattice_l, h.geo_dim, h.periodic, h.nterms = 16, 16, 1, 0, 2
+    h.terms[0] = _hip.Term(_hip.TERM_HOP, 1, -1.0)
+    h.terms[1] = _hip.Term(_hip.TERM_VIB, 0, 1.0)
+    h.sqrt_table_len = 8
+    h.batched, h.ntraj = 1, 4
+    s = _hip.Slots()
+    assert lib.paces_holstein_slot_layout(C.byref(h), 1000, C.byref(s)) == 0
+    assert (s.nterms, s.nslots, s.col_bits, s.sqrt_len, s.stride) == (2, 4, 29, 8, 1000)
+    assert list(s.slot_off)[:3] == [0, 2, 4] and list(s.vib)[:2] == [0, 1]
+    assert lib.paces_holstein_slot_layout(C.byref(h), (1 << 29) - 1, C.byref(s)) == 0
+    assert lib.paces_holstein_slot_layout(C.byref(h), 1 << 29, C.byref(s)) == _hip.E_UNSUPPORTED
+    h.sqrt_table_len = 16  # cap 15: four code bits
+    assert lib.paces_holstein_slot_layout(C.byref(h), 10, C.byref(s)) == 0
+    assert s.col_bits == 28
+    h.periodic = 1
+    assert lib.paces_holstein_slot_layout(C.byref(h), 10, C.byref(s)) == _hip.E_UNSUPPORTED
+    h.periodic, h.batched = 0, 0
+    assert lib.paces_holstein_slot_layout(C.byref(h), 10, C.byref(s)) == _hip.E_UNSUPPORTED
+    # the pairs entry point checks its arguments before any launch
+    h.batched = 1
+    offs = (C.c_int64 * 5)(0, 1, 2, 3, 4)
+    assert lib.paces_holstein_pairs_slots(8, 4, offs, None, None, None, None, None, C.byref(h),
+                                          None, None) == -1
+    assert lib.paces_holstein_pairs_slots(8, 4, offs, None, None, None, None, None, C.byref(h),
+                                          C.c_void_p(4096 + 8), None) == -1  # 16-byte aligned

@@ -157,9 +157,10 @@ def test_graph_series_equals_eager(precision):
 @pytest.mark.parametrize("graph", [False, True])
 def test_coded_values_equal_explicit_values(precision, graph):
     """The series over coded CSR values (a code byte per vib entry, each trajectory's values
-    formed in LDS) gives the explicit-value series' basis, amplitudes and stopping states bit for
-    bit at every step; the coded matrices' values, formed on demand, equal the FP64 fill's (and,
-    in FP32, the FP32 fill's) bit for bit."""
+    formed in LDS) and over the slot matrix (a uint32 per row and op: column and code) gives the
+    explicit-value series' basis, amplitudes and stopping states bit for bit at every step; the
+    coded matrices' values, formed on demand, equal the FP64 fill's (and, in FP32, the FP32
+    fill's) bit for bit, and each term of the slot matrix read as a CSR is the coded CSR."""
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
@@ -169,35 +170,41 @@ def test_coded_values_equal_explicit_values(precision, graph):
         devices.configure()
     draws = _draws(6, seed=9)
     runs = []
-    for csr in ("explicit", "coded"):
+    for csr in ("explicit", "coded", "slots"):
         bham = holstein.BatchedHamiltonian([d[0] for d in draws], 8, [-d[1] for d in draws],
                                            [d[2] for d in draws])
         bte = holstein.BatchedTimeEvolution(bham, [d[3] for d in draws], enlarge_steps=1,
                                             precision=precision, graph=graph, csr_values=csr)
         bte.create_single_exciton_states([d[0] // 2 for d in draws])
         runs.append(bte)
-    assert runs[1].csr_values == "coded"
+    assert [r.csr_values for r in runs] == ["explicit", "coded", "slots"]
     for i in range(10):
         for bte in runs:
             bte.step(i, 0.05)
-        a, b = runs
-        assert torch.equal(a.whoami, b.whoami), i
-        assert torch.equal(a.vector, b.vector), i
-        if i > 0:
-            assert a.last_expm.tobytes() == b.last_expm.tobytes(), i
-    a, b = runs
+        a = runs[0]
+        for b in runs[1:]:
+            assert torch.equal(a.whoami, b.whoami), (b.csr_values, i)
+            assert torch.equal(a.vector, b.vector), (b.csr_values, i)
+            if i > 0:
+                assert a.last_expm.tobytes() == b.last_expm.tobytes(), (b.csr_values, i)
+    a, b, c = runs
     for name in a.hamobj.offdiag_terms:
-        ma, mb = a.sparse_mats_dict[name], b.sparse_mats_dict[name]
+        ma, mb, mc = a.sparse_mats_dict[name], b.sparse_mats_dict[name], c.sparse_mats_dict[name]
         assert torch.equal(ma.indptr, mb.indptr) and torch.equal(ma.indices, mb.indices), name
         assert (mb.code is not None) == (name == "vib_coupling")
         assert mb.data.dtype == torch.float64
         assert torch.equal(mb.data.to(ma.data.dtype), ma.data), name
+        assert torch.equal(mc.indptr, mb.indptr) and torch.equal(mc.indices, mb.indices), name
+        assert (mc.code is None) == (mb.code is None), name
+        if mb.code is not None:
+            assert torch.equal(mc.code, mb.code), name
+        assert torch.equal(mc.data, mb.data), name
 
 
-@pytest.mark.parametrize("csr_values", ["explicit", "coded"])
+@pytest.mark.parametrize("csr_values", ["explicit", "coded", "slots"])
 def test_single_trajectory_batch_matches_oracle(csr_values):
     """A batch of one trajectory (a zero-width trajectory field): basis, amplitudes and
-    observables equal the oracle's, with explicit and coded CSR values."""
+    observables equal the oracle's, with explicit and coded CSR values and the slot matrix."""
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")

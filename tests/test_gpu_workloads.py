@@ -112,8 +112,8 @@ def test_chain_workload_layout_matches_oracle(precision):
     assert bham.totwordwidth == 3
     bte = holstein.BatchedTimeEvolution(bham, [d[3] for d in draws], enlarge_steps=1,
                                         precision=precision, graph=True, autotune=True,
-                                        csr_values="coded")
-    assert bte.csr_values == "coded"
+                                        csr_values="slots")
+    assert bte.csr_values == "slots"
     bte.create_single_exciton_states([d[0] // 2 for d in draws])
     worst = _run_against_oracle(bham, bte, draws, oracles, t_arr, precision == "fp64",
                                 f"chains {precision}", every=10 if precision == "mixed" else 1)
