@@ -173,12 +173,13 @@ def btay_term_bytes(R, mats, mixed, form="explicit"):
     return f * R + 4 * (R + 1) * len(mats) + 4 * nnz + vals + 4 * f * R + res * R
 
 
-# HBM bytes per row of one k_btay_term<float, B, RR> launch with coded CSR values, per launch
+# HBM bytes per row of one k_btay_term<float, B, RR, NS> launch, per matrix form and launch
 # shape (the tuner picks among shapes within a few percent of each other), from FETCH_SIZE and
 # WRITE_SIZE counter passes over the roofline loop below with the shape fixed (--term-shape),
 # each scaled by the counter's reading for a 1 GiB device copy
 # (profiles/k_btay_term_traj256_hbm_traffic.txt). Other forms and shapes report null.
-BTAY_TRAFFIC_BYTES_PER_ROW = {("mixed", (256, 2), "coded"): 75.27,
+BTAY_TRAFFIC_BYTES_PER_ROW = {("mixed", (256, 4), "slots"): 69.28,
+                              ("mixed", (256, 2), "coded"): 75.27,
                               ("mixed", (256, 4), "coded"): 76.14,
                               ("mixed", (512, 4), "coded"): 74.77}
 
