@@ -305,6 +305,68 @@ int paces_holstein_pairs_slots(int64_t R, int64_t nbasis, const int64_t* run_off
                                const paces_prior_images_t* prior, const paces_holstein_t* h,
                                uint32_t* slots, void* stream);
 
+/* Pairs of explicit images into a slot matrix of R rows (stride R): image e of run k (op k; run
+ * offsets on the host, ops + 1 entries, the first 0) of source img_src[e], where src_pos[s] is
+ * the position of source s among the R rows and img_pos[img_map[e]] (img_pos[e] when img_map is
+ * null) that of image e; writes slot rank[k] of the source's row and slot rank[k ^ 1] of the
+ * image's row as paces_holstein_pairs_slots does. A position outside [0, R) skips the pair. */
+int paces_holstein_slots_place_explicit(int64_t R, const int64_t* run_offsets,
+                                        const int32_t* src_pos, const int32_t* img_pos,
+                                        const int32_t* img_map, const int32_t* img_src,
+                                        const uint8_t* img_nup, const paces_holstein_t* h,
+                                        uint32_t* slots, void* stream);
+
+/* ---- Incremental adaptation of the batched step (no reference counterpart: the reference
+ * rebuilds each step's basis by expansions and a sort of every candidate, hamiltonian.py:383-492,
+ * time_evolution.py:735-807). A row of the previous expanded basis B_e has every neighbour in the
+ * previous basis and lists it in the previous slot matrix, so the new basis is found by marking
+ * flag bytes over the previous rows through their slots; only marked rows outside the previous
+ * B_e (in_be = 0) need their images formed (paces_holstein_images_*), which may be rows of
+ * neither basis. The entry points below are the steps of that construction; the result equals
+ * the full one (rows, slots, carried-over amplitudes) bit for bit. */
+/* Flag bytes over the previous rows hold bit 0x80 for a row of the previous B_e
+ * (paces_batch_select_flags writes it). Row i is a source when flags[i] has a bit of src_mask
+ * and none of skip_mask: every row a source with 0x80 names in its slots gets mark_bit; a source
+ * without 0x80 gets expl[i] = 1 (expl[i] = 0 for every other row) and is counted into
+ * *expl_count (device, added to). mark_bit must not overlap src_mask, skip_mask or 0x80. */
+int paces_slots_mark(const uint32_t* slots, int64_t stride, int32_t nslots, int32_t col_bits,
+                     int64_t R, uint8_t* flags, int32_t src_mask, int32_t skip_mask,
+                     int32_t mark_bit, uint8_t* expl, int32_t* expl_count, void* stream);
+/* out_idx = the indices i < n with flags[i] & mask, ascending; *out_count (device) = their
+ * number */
+size_t paces_flag_indices_workspace_bytes(int64_t n);
+int paces_flag_indices(const uint8_t* flags, int64_t n, int32_t mask, int32_t* out_idx,
+                       int32_t* out_count, void* ws, size_t ws_bytes, void* stream);
+/* out[j] = rows[idx[j]] (W words per row) for j < m */
+int paces_gather_rows(const uint64_t* rows, int32_t W, const int32_t* idx, int64_t m,
+                      uint64_t* out, void* stream);
+/* flags[idx[j]] |= bit where hit[j]; miss[j] = !hit[j] (j < m) */
+int paces_mark_hits(const int32_t* idx, const uint8_t* hit, int64_t m, uint8_t* flags,
+                    int32_t bit, uint8_t* miss, void* stream);
+/* Positions in the new basis: the rows i < R with flags[i] & 0x7f, in order, with the nY new rows
+ * merged in at ins[j] = their lower_bound among the R rows (ascending). NP[i] (i <= R) = the
+ * flagged rows before i + #{j : ins[j] <= i}, row i's position when flagged; ypos[j] = new row
+ * j's position; *out_total (device) = the new basis's rows. */
+size_t paces_warm_positions_workspace_bytes(int64_t R);
+int paces_warm_positions(const uint8_t* flags, int64_t R, const int32_t* ins, int64_t nY,
+                         int32_t* NP, int32_t* ypos, int32_t* out_total, void* ws,
+                         size_t ws_bytes, void* stream);
+/* Every flagged row i < R (flags & 0x7f) of the previous basis to position NP[i]: its row,
+ * amplitude (the reassignment of time_evolution.py:781-797), diagonal (float when diag_f32,
+ * else double), out_be = (flags & be_mask) != 0, and its slots: an entry whose pair has an
+ * endpoint with flags & be_mask is kept with its column renumbered through NP, any other slot
+ * is empty. */
+int paces_warm_emit(const uint64_t* rows, int32_t W, const double* psi, const void* diag,
+                    int32_t diag_f32, int64_t R, const uint8_t* flags, int32_t be_mask,
+                    const int32_t* NP, const uint32_t* slots, int64_t stride_old, int32_t nslots,
+                    int32_t col_bits, uint64_t* out_rows, double* out_psi, void* out_diag,
+                    uint8_t* out_be, uint32_t* out_slots, int64_t stride_new, void* stream);
+/* The nY new rows Y to positions ypos: amplitude 0, diagonal ydiag, out_be = ybe, empty slots */
+int paces_warm_emit_new(const uint64_t* Y, int32_t W, int64_t nY, const int32_t* ypos,
+                        const void* ydiag, int32_t diag_f32, const uint8_t* ybe, int32_t nslots,
+                        uint64_t* out_rows, double* out_psi, void* out_diag, uint8_t* out_be,
+                        uint32_t* out_slots, int64_t stride_new, void* stream);
+
 /* ---- D1/D2: per-term canonical CSR of the Holstein H on the new basis (generate_mel +
  * _add_hc_and_generate_coo + coo_matrix().tocsr(): hamiltonian.py:383-547,
  * time_evolution.py:365-372). An entry (i, j) of term t exists when rows i, j are linked by t and
@@ -527,6 +589,12 @@ int paces_batch_select_topk(const double* psi, const uint64_t* rows, int64_t R, 
                             const int32_t* seg_off, int32_t ntraj, const int64_t* k_per_traj,
                             uint64_t* out_rows, int32_t* out_count, void* ws, size_t ws_bytes,
                             void* stream);
+/* The same selection as one flag byte per row: keep[i] = 1 for a selected row, else 0 (every
+ * row written), plus 0x80 where in_be (optional) is nonzero; *out_count (device) = the selected
+ * rows. Same workspace. */
+int paces_batch_select_flags(const double* psi, int64_t R, const int32_t* seg_off, int32_t ntraj,
+                             const int64_t* k_per_traj, const uint8_t* in_be, uint8_t* keep,
+                             int32_t* out_count, void* ws, size_t ws_bytes, void* stream);
 /* out: ntraj blocks of 2*nchain+1 doubles (n_exc, n_pho, sum |psi|^2); h->batched must be set */
 size_t paces_holstein_observables_batched_workspace_bytes(int64_t R, int32_t ntraj,
                                                           int32_t nsites);

@@ -198,6 +198,20 @@ _SIGNATURES = {
     "paces_holstein_pairs_slots": (C.c_int, [_I64, _I64, C.POINTER(C.c_int64), _P, _P, _P, _P,
                                              C.POINTER(PriorImages), C.POINTER(Holstein), _P,
                                              _P]),
+    "paces_holstein_slots_place_explicit": (C.c_int, [_I64, C.POINTER(C.c_int64), _P, _P, _P,
+                                                      _P, _P, C.POINTER(Holstein), _P, _P]),
+    "paces_slots_mark": (C.c_int, [_P, _I64, _I32, _I32, _I64, _P, _I32, _I32, _I32, _P, _P,
+                                   _P]),
+    "paces_flag_indices_workspace_bytes": (_SZ, [_I64]),
+    "paces_flag_indices": (C.c_int, [_P, _I64, _I32, _P, _P, _P, _SZ, _P]),
+    "paces_gather_rows": (C.c_int, [_P, _I32, _P, _I64, _P, _P]),
+    "paces_mark_hits": (C.c_int, [_P, _P, _I64, _P, _I32, _P, _P]),
+    "paces_warm_positions_workspace_bytes": (_SZ, [_I64]),
+    "paces_warm_positions": (C.c_int, [_P, _I64, _P, _I64, _P, _P, _P, _P, _SZ, _P]),
+    "paces_warm_emit": (C.c_int, [_P, _I32, _P, _P, _I32, _I64, _P, _I32, _P, _P, _I64, _I32,
+                                  _I32, _P, _P, _P, _P, _P, _I64, _P]),
+    "paces_warm_emit_new": (C.c_int, [_P, _I32, _I64, _P, _P, _I32, _P, _I32, _P, _P, _P, _P,
+                                      _P, _I64, _P]),
     "paces_holstein_csr_workspace_bytes": (_SZ, [_I64, C.POINTER(Holstein)]),
     "paces_holstein_csr_count": (C.c_int, [_P, _I64, _P, C.POINTER(Layout), C.POINTER(Holstein),
                                            _P, _P, _SZ, _P]),
@@ -264,6 +278,7 @@ _SIGNATURES = {
     "paces_batch_taylor_term_coded": (C.c_int, [_I64, _P, _I32, C.POINTER(CsrCoded), _P, _P,
                                                 _P, _D, _I32, _D, _P, _I32, _P, _P, _P, _SZ,
                                                 C.POINTER(LaunchCfg), _P]),
+    "paces_batch_select_flags": (C.c_int, [_P, _I64, _P, _I32, _P, _P, _P, _P, _P, _SZ, _P]),
     "paces_batch_taylor_term_slots": (C.c_int, [_I64, _P, _I32, C.POINTER(Slots), _P, _P,
                                                 _P, _D, _I32, _D, _P, _I32, _P, _P, _P, _SZ,
                                                 C.POINTER(LaunchCfg), _P]),

@@ -324,6 +324,47 @@ __global__ void __launch_bounds__(256) k_bsel_tiles(const int64_t* __restrict__ 
   }
 }
 
+// The keep decision of each of the thread's rows of tile T (keys k[], valid vv[]): above the
+// threshold key, or a tie whose rank in row order reaches the trajectory's tie threshold. wt is
+// the workgroup's kSelRpt x waves scratch; every thread of the workgroup must call this.
+__device__ __forceinline__ void bsel_keep(const u64 (&k)[kSelRpt], const bool (&vv)[kSelRpt],
+                                          const SegTile& T, const SegSel& s,
+                                          const int32_t* __restrict__ TO,
+                                          const long long* __restrict__ thr, int* wt,
+                                          bool (&keep)[kSelRpt]) {
+  constexpr int kWaves = kBlock / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // tie ranks: exclusive prefix of the tie flags in row order, plus the tile's tie offset
+  int pre[kSelRpt];
+  bool tie[kSelRpt];
+#pragma unroll
+  for (int q = 0; q < kSelRpt; ++q) {
+    tie[q] = vv[q] && !s.all && k[q] == s.prefix;
+    const u64 bl = __ballot(tie[q]);
+    pre[q] = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bl >> 32),
+                                            __builtin_amdgcn_mbcnt_lo((unsigned)bl, 0u));
+    if (lane == 0) wt[q * kWaves + wave] = __popcll(bl);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int u = 0; u < kSelRpt * kWaves; ++u) {
+      const int c = wt[u];
+      wt[u] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  const long long th = thr[T.t];
+  const int to = TO[blockIdx.x];
+#pragma unroll
+  for (int q = 0; q < kSelRpt; ++q) {
+    const long long rank = (long long)to + wt[q * kWaves + wave] + pre[q];
+    keep[q] = vv[q] && (s.all || k[q] > s.prefix || (tie[q] && rank >= th));
+  }
+  __syncthreads();  // the tie prefixes are read; wt may be reused
+}
+
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_bsel_emit(const u64* __restrict__ keys,
                                                       const u64* __restrict__ rows,
@@ -353,36 +394,9 @@ __global__ void __launch_bounds__(kBlock) k_bsel_emit(const u64* __restrict__ ke
     k[q] = keys[ic];
     x[q] = load_row<W>(rows, ic);
   }
-  // tie ranks: exclusive prefix of the tie flags in row order, plus the tile's tie offset
-  int pre[kSelRpt];
-  bool tie[kSelRpt];
-#pragma unroll
-  for (int q = 0; q < kSelRpt; ++q) {
-    tie[q] = vv[q] && !s.all && k[q] == s.prefix;
-    const u64 bl = __ballot(tie[q]);
-    pre[q] = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bl >> 32),
-                                            __builtin_amdgcn_mbcnt_lo((unsigned)bl, 0u));
-    if (lane == 0) wt[q * kWaves + wave] = __popcll(bl);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int run = 0;
-    for (int u = 0; u < kSelRpt * kWaves; ++u) {
-      const int c = wt[u];
-      wt[u] = run;
-      run += c;
-    }
-  }
-  __syncthreads();
-  const long long th = thr[T.t];
-  const int to = TO[blockIdx.x];
   bool keep[kSelRpt];
-#pragma unroll
-  for (int q = 0; q < kSelRpt; ++q) {
-    const long long rank = (long long)to + wt[q * kWaves + wave] + pre[q];
-    keep[q] = vv[q] && (s.all || k[q] > s.prefix || (tie[q] && rank >= th));
-  }
-  __syncthreads();  // the tie prefixes are read; wt is reused for the kept counts
+  bsel_keep(k, vv, T, s, TO, thr, wt, keep);
+  int pre[kSelRpt];
 #pragma unroll
   for (int q = 0; q < kSelRpt; ++q) {
     const u64 bl = __ballot(keep[q]);
@@ -406,6 +420,41 @@ __global__ void __launch_bounds__(kBlock) k_bsel_emit(const u64* __restrict__ ke
     if (keep[q]) store_row<W>(out, ko + wt[q * kWaves + wave] + pre[q], x[q]);
 }
 
+// the same decisions as k_bsel_emit, written as one flag byte per row (1 kept, 0 not), with bit
+// 7 set where in_be (optional) is nonzero
+__global__ void __launch_bounds__(kBlock) k_bsel_flags(const u64* __restrict__ keys,
+                                                       const int32_t* __restrict__ seg_off,
+                                                       const int64_t* __restrict__ boff, int ntraj,
+                                                       const SegSel* __restrict__ st,
+                                                       const int32_t* __restrict__ TO,
+                                                       const long long* __restrict__ thr,
+                                                       const int32_t* __restrict__ KO, int ntiles,
+                                                       const uint8_t* __restrict__ in_be,
+                                                       uint8_t* __restrict__ flags,
+                                                       int32_t* __restrict__ out_count) {
+  constexpr int kWaves = kBlock / 64;
+  __shared__ int wt[kSelRpt * kWaves];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *out_count = KO[ntiles];
+  const SegTile T = seg_tile(seg_off, boff, ntraj, kSelTile);
+  if (T.t < 0) return;
+  const SegSel s = st[T.t];
+  u64 k[kSelRpt];
+  bool vv[kSelRpt];
+#pragma unroll
+  for (int q = 0; q < kSelRpt; ++q) {
+    const int64_t i = T.r0 + q * kBlock + threadIdx.x;
+    vv[q] = i < T.r1;
+    k[q] = keys[vv[q] ? i : T.r0];
+  }
+  bool keep[kSelRpt];
+  bsel_keep(k, vv, T, s, TO, thr, wt, keep);
+#pragma unroll
+  for (int q = 0; q < kSelRpt; ++q) {
+    const int64_t i = T.r0 + q * kBlock + threadIdx.x;
+    if (vv[q]) flags[i] = (uint8_t)((keep[q] ? 1 : 0) | ((in_be && in_be[i]) ? 0x80 : 0));
+  }
+}
+
 size_t bsel_ws_bytes(int64_t R, int ntraj) {
   const int64_t nt = seg_grid(R, ntraj, kSelTile);
   return a256((size_t)ntraj * sizeof(SegSel)) + a256((size_t)ntraj * kSelBins * 4) +
@@ -414,10 +463,12 @@ size_t bsel_ws_bytes(int64_t R, int ntraj) {
          a256(scan_ws_bytes(nt));
 }
 
+// W == 0: the decisions as flag bytes (out_flags), no rows read or written
 template <int W>
 static int run_bselect(const double* psi, const uint64_t* rows, int64_t R,
                        const int32_t* seg_off, int ntraj, const int64_t* kpt, uint64_t* out_rows,
-                       int32_t* out_count, void* ws, hipStream_t st) {
+                       int32_t* out_count, void* ws, hipStream_t st, uint8_t* out_flags = nullptr,
+                       const uint8_t* in_be = nullptr) {
   const unsigned gsh = seg_grid(R, ntraj, kSelTile);
   char* p = (char*)ws;
   SegSel* s = (SegSel*)p;
@@ -457,10 +508,17 @@ static int run_bselect(const double* psi, const uint64_t* rows, int64_t R,
   if (rc) return rc;
   rc = scan_i32(kc, gsh, KO, p, scan_ws_bytes(gsh), st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_bsel_emit<W>, dim3(gsh), dim3(kBlock), 0, st, (const u64*)keys,
-                     (const u64*)rows, seg_off, (const int64_t*)boffh, ntraj, (const SegSel*)s,
-                     (const int32_t*)TO, (const long long*)thr, (const int32_t*)KO, (int)gsh,
-                     (u64*)out_rows, out_count);
+  if constexpr (W == 0) {
+    hipLaunchKernelGGL(k_bsel_flags, dim3(gsh), dim3(kBlock), 0, st, (const u64*)keys, seg_off,
+                       (const int64_t*)boffh, ntraj, (const SegSel*)s, (const int32_t*)TO,
+                       (const long long*)thr, (const int32_t*)KO, (int)gsh, in_be, out_flags,
+                       out_count);
+  } else {
+    hipLaunchKernelGGL(k_bsel_emit<W>, dim3(gsh), dim3(kBlock), 0, st, (const u64*)keys,
+                       (const u64*)rows, seg_off, (const int64_t*)boffh, ntraj,
+                       (const SegSel*)s, (const int32_t*)TO, (const long long*)thr,
+                       (const int32_t*)KO, (int)gsh, (u64*)out_rows, out_count);
+  }
   return launch_status();
 }
 
@@ -1424,6 +1482,17 @@ int paces_batch_select_topk(const double* psi, const uint64_t* rows, int64_t R, 
   if (ws_bytes < bsel_ws_bytes(R, ntraj)) return PACES_E_WORKSPACE;
   PACES_DISPATCH_W(W, run_bselect, psi, rows, R, seg_off, ntraj, k_per_traj, out_rows, out_count,
                    ws, (hipStream_t)stream);
+}
+
+int paces_batch_select_flags(const double* psi, int64_t R, const int32_t* seg_off, int32_t ntraj,
+                             const int64_t* k_per_traj, const uint8_t* in_be, uint8_t* keep,
+                             int32_t* out_count, void* ws, size_t ws_bytes, void* stream) {
+  if (R < 0 || ntraj < 1 || !seg_off || !k_per_traj || !out_count || !ws) return PACES_E_INVALID;
+  if (R > 0 && (!psi || !keep)) return PACES_E_INVALID;
+  if (R >= 2147483647LL) return PACES_E_INVALID;
+  if (ws_bytes < bsel_ws_bytes(R, ntraj)) return PACES_E_WORKSPACE;
+  return run_bselect<0>(psi, nullptr, R, seg_off, ntraj, k_per_traj, nullptr, out_count, ws,
+                        (hipStream_t)stream, keep, in_be);
 }
 
 size_t paces_holstein_observables_batched_workspace_bytes(int64_t R, int32_t ntraj,

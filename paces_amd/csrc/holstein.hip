@@ -1134,6 +1134,26 @@ __global__ void __launch_bounds__(kBlock) k_pairs_slot_place(int64_t R, int64_t 
   slots[(int64_t)(SL.foff[t] + SL.rank[k ^ 1]) * R + p] = (uint32_t)q | code;
 }
 
+// Explicit pairs of a slot matrix: images e of source rows (img_src[e] indexes the source list),
+// run k of the image list from op k, with src_pos / img_pos the positions of the sources and the
+// images in the slot matrix's rows: slot rank[k] of the source's row and slot rank[k ^ 1] of the
+// image's row, as k_pairs_slot_place writes them.
+__global__ void __launch_bounds__(kBlock) k_slots_place_explicit(
+    int64_t R, ImgRuns IR, OpTable T, SlotArg SL, int col_bits,
+    const int32_t* __restrict__ src_pos, const int32_t* __restrict__ img_pos,
+    const int32_t* __restrict__ img_map, const int32_t* __restrict__ src,
+    const uint8_t* __restrict__ nup, uint32_t* __restrict__ slots) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= IR.off[IR.n]) return;
+  const int k = img_run_of(IR, e);
+  const int32_t q = src_pos[src[e]], p = img_pos[img_map ? img_map[e] : e];
+  if (p < 0 || q < 0 || p >= R || q >= R) return;  // a row not among the R rows has no slots
+  const int t = T.term[k];
+  const uint32_t code = T.kind[k] == PACES_TERM_VIB ? (uint32_t)nup[e] << col_bits : 0u;
+  slots[(int64_t)(SL.foff[t] + SL.rank[k]) * R + q] = (uint32_t)p | code;
+  slots[(int64_t)(SL.foff[t] + SL.rank[k ^ 1]) * R + p] = (uint32_t)q | code;
+}
+
 // the largest argument list of the pairs kernels (k_pairs_place) stays within 4 KiB
 static_assert(sizeof(ImgRuns) + sizeof(PriorImgs) + sizeof(OpTable) + sizeof(SlotArg) +
                   sizeof(ModelArg) + sizeof(CsrOut) + 16 * 8 <= 4096,
@@ -1942,6 +1962,31 @@ int paces_holstein_pairs_slots(int64_t R, int64_t nbasis, const int64_t* run_off
     hipLaunchKernelGGL(k_pairs_slot_place, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0, st, R,
                        nbasis, IR, A, T, SL, S.col_bits, cand_pos, in_basis, img_src,
                        img_nup, slots);
+  return launch_status();
+}
+
+int paces_holstein_slots_place_explicit(int64_t R, const int64_t* run_offsets,
+                                        const int32_t* src_pos, const int32_t* img_pos,
+                                        const int32_t* img_map, const int32_t* img_src,
+                                        const uint8_t* img_nup, const paces_holstein_t* h,
+                                        uint32_t* slots, void* stream) {
+  OpTable T;
+  SlotArg SL;
+  paces_slots_t S;
+  if (!run_offsets) return PACES_E_INVALID;
+  int rc = slot_layout(h, R, &T, &SL, &S);
+  if (rc) return rc;
+  ImgRuns IR;
+  rc = img_runs_arg(T, run_offsets, &IR);
+  if (rc) return rc;
+  const int64_t nimg = IR.off[IR.n];
+  if (IR.off[0] != 0) return PACES_E_INVALID;
+  if (nimg > 0 && (!src_pos || !img_pos || !img_src || !img_nup || !slots || R <= 0))
+    return PACES_E_INVALID;
+  if (nimg > 0)
+    hipLaunchKernelGGL(k_slots_place_explicit, dim3(grid_for(nimg, kBlock)), dim3(kBlock), 0,
+                       (hipStream_t)stream, R, IR, T, SL, S.col_bits, src_pos, img_pos, img_map,
+                       img_src, img_nup, slots);
   return launch_status();
 }
 

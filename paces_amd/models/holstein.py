@@ -306,6 +306,7 @@ class Hamiltonian(HamiltonianFramework):
             offs_all = offs if prior is None else [a + b for a, b in zip(prior[1], offs)]
             dbg["vib_coupling"] = [basis.shape[0] - self._vib_plus_count(offs_all)]
         self.last_basis_e = basis
+        self.last_in_basis = in_basis  # the rows of new_inds that are rows of basis
         return (diag, new_inds), mats, dbg
 
     def _inverse_skip_codes(self):
@@ -1222,7 +1223,7 @@ class BatchedTimeEvolution:
     def __init__(self, hamobj: BatchedHamiltonian, maxstates, enlarge_steps=1,
                  precision="mixed", m_star=100, taylor_tol=None, autotune=False, graph=False,
                  launch_shape=None, csr_values="slots", explosion_cutoff=2.0,
-                 on_observables=None, graph_pool=None):
+                 on_observables=None, graph_pool=None, incremental=True, incremental_max=0.05):
         self.hamobj = hamobj
         # every step ends with the per-trajectory norm and site observables (as the reference's
         # step does, time_evolution.py:585-605); they travel to pinned host memory without a
@@ -1278,6 +1279,16 @@ class BatchedTimeEvolution:
         if self.graph and (self.m_star < 2 or not self.tol < 1.0):
             raise ValueError("the graph-replayed series needs m_star >= 2 and taylor_tol < 1")
         self.graph_pool = graph_pool if graph_pool is not None else SeriesGraphPool(self.device)
+        # incremental=True: with the slot matrix and one expansion, a step after the first finds
+        # its basis by marking the previous basis through the previous slots (_adapt_incremental;
+        # the same rows, matrix and amplitudes as the full construction). A step whose marked
+        # rows outside the previous expanded basis exceed incremental_max of the previous rows
+        # (the growth from a single state) takes the full construction instead.
+        self.incremental = bool(incremental) and self.csr_values == "slots" and \
+            self.enlarge_steps == 1
+        self.incremental_max = float(incremental_max)
+        self._prev_be = None  # the expanded basis's rows among whoami (uint8), for the next step
+        self.last_adapt = None
         self._graph_terms = 8 if precision == "mixed" else 16
         self.last_launch_shape = None
         self.whoami = None
@@ -1320,7 +1331,13 @@ class BatchedTimeEvolution:
         return out[:n]
 
     def adapt(self):
-        select = self._select()
+        if self.incremental and self._prev_be is not None and self.numstates > 0:
+            done = self._adapt_incremental()
+            if done is not None:
+                return done
+            select = self._selected_rows
+        else:
+            select = self._select()
         # the mixed-precision series reads FP32 values only: the CSR is built with them
         vdt = torch.float32 if self.precision == "mixed" else torch.float64
         ddt = vdt  # the diagonal too (diag_vals is FP32 in the mixed form)
@@ -1332,9 +1349,175 @@ class BatchedTimeEvolution:
         self.whoami = new_inds
         self.diag_vals = diag
         self.sparse_mats_dict = mats
+        self._prev_be = self.hamobj.last_in_basis if self.incremental else None
         self._obs_last = None
         self._set_segments(self.hamobj.segments(new_inds))
+        self.last_adapt = "full"
         return dbg
+
+    def _flag_indices(self, flags, n, mask):
+        idx = torch.empty(max(n, 1), dtype=torch.int32, device=self.device)
+        cnt = torch.empty(1, dtype=torch.int32, device=self.device)
+        wsp, wsn = self._ws.get(_lib.paces_flag_indices_workspace_bytes(n))
+        check(_lib.paces_flag_indices(ptr(flags) if n else None, n, mask, ptr(idx), ptr(cnt),
+                                      wsp, wsn, stream()), "flag_indices")
+        return idx, cnt
+
+    def _gather(self, rows, idx, m):
+        W = rows.shape[1]
+        out = torch.empty((max(m, 1), W), dtype=rows.dtype, device=self.device)
+        check(_lib.paces_gather_rows(ptr(rows), W, ptr(idx), m, ptr(out), stream()),
+              "gather_rows")
+        return out[:m]
+
+    def _union(self, runs):
+        """Sorted union of sorted unique row runs [(tensor, offset, length)], with the union
+        index of every input row (paces_merge_unique)."""
+        rows, _, cpos = engine.merge_unique(runs, self.hamobj.totwordwidth, self._ws,
+                                            want_cand_pos=True)
+        return rows, cpos
+
+    def _explicit(self, X, flags, bit):
+        """Images of the sorted unique rows X (rows outside the previous expanded basis): their
+        union, with every image's index in it; the union rows found among the previous rows set
+        bit in their flags, the others (sorted) are returned as new rows."""
+        rows0 = self.whoami
+        imgs, offs, src, nup = self.hamobj._images(X, want_src=True)
+        nimg = offs[-1]
+        if not nimg:
+            return rows0[:0], None
+        U, cpos = self._union([(imgs, offs[k], offs[k + 1] - offs[k])
+                               for k in range(len(offs) - 1)])
+        nU = U.shape[0]
+        idx, hit, _ = engine.lower_bound(rows0, U, want_hit=True, q_sorted=True, ws=self._ws)
+        miss = torch.empty(nU, dtype=torch.uint8, device=self.device)
+        check(_lib.paces_mark_hits(ptr(idx), ptr(hit), nU, ptr(flags), bit, ptr(miss),
+                                   stream()), "mark_hits")
+        midx, mcnt = self._flag_indices(miss, nU, 1)
+        new = self._gather(U, midx, int(mcnt.item()))
+        return new, (X, offs, src[:nimg], nup[:nimg], U, cpos)
+
+    def _adapt_incremental(self):
+        """The new basis from the previous one (paces_slots_mark / _warm_*): the selection as
+        flags over the previous rows; the first and second expansions as marks through the
+        previous slot matrix (every row of the previous expanded basis lists all its neighbours
+        there); explicit images for the marked rows outside it, whose new neighbours join as new
+        rows; then positions, the rows, amplitudes, diagonal and slots moved in one pass, and the
+        explicit pairs placed. Returns None after leaving the selection's rows in
+        _selected_rows when the explicit rows exceed incremental_max of the previous rows."""
+        h = self.hamobj
+        rows0, psi0, diag0 = self.whoami, self.vector, self.diag_vals
+        R0, W = rows0.shape
+        mats0 = self.sparse_mats_dict
+        sm0 = next(iter(mats0.values())).slotmat
+        lay = sm0.layout
+        dev = self.device
+        flags = torch.empty(R0, dtype=torch.uint8, device=dev)
+        cnt = torch.empty(1, dtype=torch.int32, device=dev)
+        wsp, wsn = self._ws.get(_lib.paces_batch_select_workspace_bytes(R0, h.ntraj))
+        check(_lib.paces_batch_select_flags(ptr(torch.view_as_real(psi0)), R0, ptr(self.seg_off),
+                                            h.ntraj, ptr(self._d_ms), ptr(self._prev_be),
+                                            ptr(flags), ptr(cnt), wsp, wsn, stream()),
+              "batch_select_flags")
+        ecnt = torch.zeros(2, dtype=torch.int32, device=dev)
+        e1 = torch.empty(R0, dtype=torch.uint8, device=dev)
+        e2 = torch.empty(R0, dtype=torch.uint8, device=dev)
+        slots0 = ptr(sm0.slots)
+
+        def mark(src, skip, bit, expl, counter):
+            check(_lib.paces_slots_mark(slots0, R0, lay.nslots, lay.col_bits, R0, ptr(flags), src,
+                                        skip, bit, ptr(expl), counter, stream()), "slots_mark")
+
+        # bit 0 selected; bit 1 the first expansion (neighbours of selected rows); bit 2 the
+        # second (neighbours of the rows the first one added, bit 1 without bit 0); bit 7 (from
+        # the select) rows of the previous expanded basis
+        mark(1, 0, 2, e1, ptr(ecnt))
+        mark(2, 1, 4, e2, ecnt.data_ptr() + 4)
+        n1, n2 = ecnt.tolist()
+        if n1 + n2 > self.incremental_max * R0:
+            sidx, scnt = self._flag_indices(flags, R0, 1)
+            self._selected_rows = self._gather(rows0, sidx, int(scnt.item()))
+            return None
+        pairs = []
+        y1 = rows0[:0]
+        if n1:
+            idx1, _ = self._flag_indices(e1, R0, 1)
+            y1, pr = self._explicit(self._gather(rows0, idx1, n1), flags, 2)
+            if pr is not None:
+                pairs.append(pr)
+            # the rows found by the first explicit images are first-expansion rows too
+            e2c = torch.zeros(1, dtype=torch.int32, device=dev)
+            mark(2, 1, 4, e2, ptr(e2c))
+            n2 = int(e2c.item())
+        x2 = None
+        if n2:
+            idx2, _ = self._flag_indices(e2, R0, 1)
+            x2 = self._gather(rows0, idx2, n2)
+        if y1.shape[0]:
+            # the new first-expansion rows are not previous rows: a disjoint sorted union
+            x2 = y1 if x2 is None else self._union([(x2, 0, x2.shape[0]),
+                                                    (y1, 0, y1.shape[0])])[0]
+        Y = y1
+        if x2 is not None:
+            y2, pr = self._explicit(x2, flags, 4)
+            if pr is not None:
+                pairs.append(pr)
+            if y2.shape[0]:
+                Y = y2 if not y1.shape[0] else self._union([(y1, 0, y1.shape[0]),
+                                                            (y2, 0, y2.shape[0])])[0]
+        nY = Y.shape[0]
+        ybe = None
+        ins = None
+        if nY:
+            ins, _, _ = engine.lower_bound(rows0, Y, want_hit=False, q_sorted=True, ws=self._ws)
+            if y1.shape[0]:
+                _, ybe, _ = engine.lower_bound(y1, Y, want_hit=True, q_sorted=True, ws=self._ws)
+            else:
+                ybe = torch.zeros(nY, dtype=torch.uint8, device=dev)
+        NP = torch.empty(R0 + 1, dtype=torch.int32, device=dev)
+        ypos = torch.empty(max(nY, 1), dtype=torch.int32, device=dev)
+        tot = torch.empty(1, dtype=torch.int32, device=dev)
+        wsp, wsn = self._ws.get(_lib.paces_warm_positions_workspace_bytes(R0))
+        check(_lib.paces_warm_positions(ptr(flags), R0, ptr(ins) if nY else None, nY, ptr(NP),
+                                        ptr(ypos), ptr(tot), wsp, wsn, stream()),
+              "warm_positions")
+        R1 = int(tot.item())
+        lay1 = h.slot_layout(R1)
+        if lay1 is None or lay1.col_bits != lay.col_bits or lay1.nslots != lay.nslots:
+            raise RuntimeError("the slot layout changed between steps")
+        f32 = 1 if diag0.dtype == torch.float32 else 0
+        rows1 = torch.empty((R1, W), dtype=rows0.dtype, device=dev)
+        psi1 = torch.empty(R1, dtype=torch.complex128, device=dev)
+        diag1 = torch.empty(R1, dtype=diag0.dtype, device=dev)
+        be1 = torch.empty(R1, dtype=torch.uint8, device=dev)
+        slots1 = torch.empty((lay.nslots, R1), dtype=torch.int32, device=dev)
+        check(_lib.paces_warm_emit(ptr(rows0), W, ptr(torch.view_as_real(psi0)), ptr(diag0), f32,
+                                   R0, ptr(flags), 3, ptr(NP), slots0, R0, lay.nslots,
+                                   lay.col_bits, ptr(rows1), ptr(torch.view_as_real(psi1)),
+                                   ptr(diag1), ptr(be1), ptr(slots1), R1, stream()), "warm_emit")
+        if nY:
+            ydiag = h.generate_mel_diag(Y, dtype=diag0.dtype)
+            check(_lib.paces_warm_emit_new(ptr(Y), W, nY, ptr(ypos), ptr(ydiag), f32, ptr(ybe),
+                                           lay.nslots, ptr(rows1),
+                                           ptr(torch.view_as_real(psi1)), ptr(diag1), ptr(be1),
+                                           ptr(slots1), R1, stream()), "warm_emit_new")
+        for X, offs, src, nup, U, cpos in pairs:
+            sp, _, _ = engine.lower_bound(rows1, X, want_hit=False, q_sorted=True, ws=self._ws)
+            up, _, _ = engine.lower_bound(rows1, U, want_hit=False, q_sorted=True, ws=self._ws)
+            run_off = (C.c_int64 * len(offs))(*offs)
+            check(_lib.paces_holstein_slots_place_explicit(R1, run_off, ptr(sp), ptr(up),
+                                                           ptr(cpos), ptr(src), ptr(nup),
+                                                           C.byref(h._hs), ptr(slots1), stream()),
+                  "holstein_slots_place_explicit")
+        sm1 = engine.SlotMatrix(slots1, lay1, (R1, R1))
+        self.whoami, self.vector, self.diag_vals = rows1, psi1, diag1
+        self.sparse_mats_dict = {name: sm1.term(t, lambda m, rows=rows1: h._coded_values(m, rows))
+                                 for t, name in enumerate(h.offdiag_terms)}
+        self._prev_be = be1
+        self._obs_last = None
+        self._set_segments(h.segments(rows1))
+        self.last_adapt = ("incremental", n1, n2, nY)
+        return {}
 
     def _set_segments(self, seg_off):
         """The segment offsets, with a pinned host copy made without a stream sync (the next
