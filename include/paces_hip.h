@@ -589,12 +589,19 @@ int paces_batch_select_topk(const double* psi, const uint64_t* rows, int64_t R, 
                             const int32_t* seg_off, int32_t ntraj, const int64_t* k_per_traj,
                             uint64_t* out_rows, int32_t* out_count, void* ws, size_t ws_bytes,
                             void* stream);
+/* The same with the select keys already formed (keys: the bits of |psi|^2 per row, as
+ * paces_holstein_observables_batched_finish writes them; psi then unread, may be null) */
+int paces_batch_select_topk_keys(const double* psi, const uint64_t* keys, const uint64_t* rows,
+                                 int64_t R, int32_t W, const int32_t* seg_off, int32_t ntraj,
+                                 const int64_t* k_per_traj, uint64_t* out_rows,
+                                 int32_t* out_count, void* ws, size_t ws_bytes, void* stream);
 /* The same selection as one flag byte per row: keep[i] = 1 for a selected row, else 0 (every
  * row written), plus 0x80 where in_be (optional) is nonzero; *out_count (device) = the selected
- * rows. Same workspace. */
-int paces_batch_select_flags(const double* psi, int64_t R, const int32_t* seg_off, int32_t ntraj,
-                             const int64_t* k_per_traj, const uint8_t* in_be, uint8_t* keep,
-                             int32_t* out_count, void* ws, size_t ws_bytes, void* stream);
+ * rows. Same workspace; keys as above (optional: null forms them from psi). */
+int paces_batch_select_flags(const double* psi, const uint64_t* keys, int64_t R,
+                             const int32_t* seg_off, int32_t ntraj, const int64_t* k_per_traj,
+                             const uint8_t* in_be, uint8_t* keep, int32_t* out_count, void* ws,
+                             size_t ws_bytes, void* stream);
 /* out: ntraj blocks of 2*nchain+1 doubles (n_exc, n_pho, sum |psi|^2); h->batched must be set */
 size_t paces_holstein_observables_batched_workspace_bytes(int64_t R, int32_t ntraj,
                                                           int32_t nsites);
@@ -602,6 +609,17 @@ int paces_holstein_observables_batched(const uint64_t* rows, const double* psi, 
                                        const int32_t* seg_off, const paces_layout_t* lay,
                                        const paces_holstein_t* h, double* out, void* ws,
                                        size_t ws_bytes, void* stream);
+/* The same at the end of a series (no reference counterpart): with ts (the series' per-trajectory
+ * states), a trajectory whose last term index is even first gets x_even (complex64 when mixed,
+ * else complex128; the buffer its even terms wrote) added to its amplitudes in psi, the work of
+ * paces_batch_taylor_finish; keys (optional, R entries) receives each row's select key (the bits
+ * of |psi|^2) for paces_batch_select_flags / paces_batch_select_topk_keys. */
+int paces_holstein_observables_batched_finish(uint64_t const* rows, double* psi, int64_t R,
+                                              const int32_t* seg_off, const paces_layout_t* lay,
+                                              const paces_holstein_t* h, const void* x_even,
+                                              int32_t mixed, const paces_batch_taylor_t* ts,
+                                              uint64_t* keys, double* out, void* ws,
+                                              size_t ws_bytes, void* stream);
 size_t paces_batch_taylor_workspace_bytes(int64_t R, int32_t ntraj);
 /* res = v; x0 = v as complex64 (mixed != 0) or complex128; ts[t].c1 = ||v_t||_inf */
 int paces_batch_taylor_begin(int64_t R, const double* v, double* res, void* x0, int32_t mixed,

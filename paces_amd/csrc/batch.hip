@@ -468,7 +468,7 @@ template <int W>
 static int run_bselect(const double* psi, const uint64_t* rows, int64_t R,
                        const int32_t* seg_off, int ntraj, const int64_t* kpt, uint64_t* out_rows,
                        int32_t* out_count, void* ws, hipStream_t st, uint8_t* out_flags = nullptr,
-                       const uint8_t* in_be = nullptr) {
+                       const uint8_t* in_be = nullptr, const uint64_t* keys_in = nullptr) {
   const unsigned gsh = seg_grid(R, ntraj, kSelTile);
   char* p = (char*)ws;
   SegSel* s = (SegSel*)p;
@@ -490,7 +490,9 @@ static int run_bselect(const double* psi, const uint64_t* rows, int64_t R,
   long long* thr = (long long*)p;
   p += a256((size_t)ntraj * 8);
   hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(1), 0, st, seg_off, ntraj, kSelTile, boffh);
-  if (R > 0)
+  if (keys_in)
+    keys = (u64*)keys_in;  // the keys of these amplitudes, formed by the observables pass
+  else if (R > 0)
     hipLaunchKernelGGL(k_bsel_keys, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
                        (const double2*)psi, R, keys);
   hipLaunchKernelGGL(k_bsel_init, dim3(ntraj), dim3(256), 0, st, seg_off, kpt, ntraj, s, hist);
@@ -537,9 +539,21 @@ constexpr int obs_tile() {
 // site's field offset and width are uniform across the wave; lane l holds rows l, l + 64, ...
 // with their weights and exciton positions in registers, and a site's sum ends in a wave
 // reduction. Only sites inside the tile's range of exciton positions can hold exciton weight.
+// With fin (the step's series state ts, its even-term buffer x_even in complex64 or complex128):
+// a trajectory whose series ended on an even term first gets that term added to its amplitudes
+// (psi updated in place: the finish of the series, as k_btay_finish does it). keys (optional)
+// receives each row's select key, the bits of |psi|^2 (k_bsel_keys' expression), for the next
+// select.
+struct ObsFinish {
+  const void* x_even;
+  int mixed;
+  const paces_batch_taylor_t* ts;
+  u64* keys;
+};
+
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__ rows,
-                                                         const double2* __restrict__ psi,
+                                                         double2* __restrict__ psi, ObsFinish fin,
                                                          const int32_t* __restrict__ seg_off,
                                                          const int64_t* __restrict__ boff,
                                                          int ntraj, const int32_t* __restrict__ off,
@@ -547,7 +561,9 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
                                                          const int32_t* __restrict__ traj_L, int geo,
                                                          double* __restrict__ part) {
   constexpr int TB = obs_tile<W>(), RPL = TB / 64, PER = TB * W / kBlock;
+  static_assert(TB % kBlock == 0, "whole rows per thread");
   __shared__ u64 srow[TB * W];
+  __shared__ double sw[TB];
   __shared__ int spr[2];
   const SegTile T = seg_tile(seg_off, boff, ntraj, TB);
   if (T.t < 0) return;
@@ -560,29 +576,62 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double w[RPL];
   {
-    // the rows (to LDS) and this lane's amplitudes are loaded together, before the barrier
+    // the rows to LDS
     u64 tmp[PER];
-    double2 v[RPL];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
       const int e = (int)threadIdx.x + j * kBlock;
       tmp[j] = e < nr * W ? rows[T.r0 * W + e] : 0ull;
     }
 #pragma unroll
-    for (int j = 0; j < RPL; ++j) {
-      const int r = lane + 64 * j;
-      v[j] = psi[T.r0 + (r < nr ? r : 0)];
-    }
-#pragma unroll
     for (int j = 0; j < PER; ++j) {
       const int e = (int)threadIdx.x + j * kBlock;
       if (e < nr * W) srow[e] = tmp[j];
     }
+  }
+  {
+    // the amplitudes, each row by one thread (rows t, t + kBlock, ...): a trajectory whose
+    // series ended on an even term gets that term added first (each row read and written once);
+    // the weights go through LDS to the lanes that hold their rows in the site loop below
+    constexpr int TPR = TB / kBlock;
+    const bool add = fin.ts && (fin.ts[T.t].j_ind & 1) == 0;
+    double2 v[TPR];
+    int64_t ii[TPR];
 #pragma unroll
-    for (int j = 0; j < RPL; ++j)
-      w[j] = (lane + 64 * j < nr) ? v[j].x * v[j].x + v[j].y * v[j].y : 0.0;
+    for (int q = 0; q < TPR; ++q) {
+      const int r = (int)threadIdx.x + kBlock * q;
+      ii[q] = T.r0 + (r < nr ? r : 0);
+      v[q] = psi[ii[q]];
+    }
+    if (add) {
+#pragma unroll
+      for (int q = 0; q < TPR; ++q) {
+        double xr, xi;
+        if (fin.mixed) {
+          const float2 x = ((const float2*)fin.x_even)[ii[q]];
+          xr = x.x;
+          xi = x.y;
+        } else {
+          const double2 x = ((const double2*)fin.x_even)[ii[q]];
+          xr = x.x;
+          xi = x.y;
+        }
+        v[q].x = v[q].x + xr;
+        v[q].y = v[q].y + xi;
+        if ((int)threadIdx.x + kBlock * q < nr) psi[ii[q]] = v[q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < TPR; ++q) {
+      const int r = (int)threadIdx.x + kBlock * q;
+      const double k = v[q].x * v[q].x + v[q].y * v[q].y;
+      if (fin.keys && r < nr) fin.keys[ii[q]] = (u64)__double_as_longlong(k);
+      sw[r] = r < nr ? k : 0.0;
+    }
   }
   __syncthreads();
+#pragma unroll
+  for (int j = 0; j < RPL; ++j) w[j] = sw[lane + 64 * j];
   int pos[RPL];
 #pragma unroll
   for (int j = 0; j < RPL; ++j) {
@@ -643,7 +692,19 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
       }
     } else if (b > 0) {
       const u64 mask = (b >= 64) ? ~0ull : ((1ull << b) - 1ull);
-      if (b <= 32) {
+      if (o + b <= 32 || o >= 32) {
+        // the field lies in one 32-bit half of its word: a 32-bit LDS read and a bit-field
+        // extract per row, one FMA (the sum is compared within 1e-10, not bit for bit)
+        const int half = o >= 32 ? 0 : 1;  // little endian: the high half is at byte 4
+        const int sh = (half ? 32 : 64) - o - b;
+        const uint32_t* s32 = (const uint32_t*)srow;
+#pragma unroll
+        for (int j = 0; j < RPL; ++j) {
+          const int r = lane + 64 * j;
+          const uint32_t n = (s32[(r * W + k) * 2 + half] >> sh) & (uint32_t)mask;
+          ap = fma(w[j], (double)n, ap);
+        }
+      } else if (b <= 32) {
 #pragma unroll
         for (int j = 0; j < RPL; ++j) {
           const int r = lane + 64 * j;
@@ -688,9 +749,9 @@ size_t bobs_ws_bytes(int64_t R, int ntraj, int N) {
 }
 
 template <int W>
-static int run_bobs(const uint64_t* rows, const double* psi, int64_t R, const int32_t* seg_off,
+static int run_bobs(const uint64_t* rows, double* psi, int64_t R, const int32_t* seg_off,
                     int ntraj, const paces_layout_t* lay, int N, const int32_t* traj_L, int geo,
-                    double* out, void* ws, hipStream_t st) {
+                    double* out, void* ws, hipStream_t st, ObsFinish fin) {
   char* p = (char*)ws;
   int64_t* boff = (int64_t*)p;
   p += a256((size_t)(ntraj + 1) * 8);
@@ -699,7 +760,7 @@ static int run_bobs(const uint64_t* rows, const double* psi, int64_t R, const in
   constexpr int TB = obs_tile<W>();
   hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(1), 0, st, seg_off, ntraj, TB, boff);
   hipLaunchKernelGGL(k_bobs_partial<W>, dim3(seg_grid(R, ntraj, TB)), dim3(kBlock), 0, st,
-                     (const u64*)rows, (const double2*)psi, seg_off, (const int64_t*)boff, ntraj,
+                     (const u64*)rows, (double2*)psi, fin, seg_off, (const int64_t*)boff, ntraj,
                      lay->d_off, lay->d_width, N, traj_L, geo, part);
   hipLaunchKernelGGL(k_bobs_final, dim3(ntraj * S), dim3(64), 0, st, (const double*)part,
                      (const int64_t*)boff, S, out);
@@ -1476,23 +1537,32 @@ int paces_batch_select_topk(const double* psi, const uint64_t* rows, int64_t R, 
                             const int32_t* seg_off, int32_t ntraj, const int64_t* k_per_traj,
                             uint64_t* out_rows, int32_t* out_count, void* ws, size_t ws_bytes,
                             void* stream) {
+  return paces_batch_select_topk_keys(psi, nullptr, rows, R, W, seg_off, ntraj, k_per_traj,
+                                      out_rows, out_count, ws, ws_bytes, stream);
+}
+
+int paces_batch_select_topk_keys(const double* psi, const uint64_t* keys, const uint64_t* rows,
+                                 int64_t R, int32_t W, const int32_t* seg_off, int32_t ntraj,
+                                 const int64_t* k_per_traj, uint64_t* out_rows,
+                                 int32_t* out_count, void* ws, size_t ws_bytes, void* stream) {
   if (R < 0 || ntraj < 1 || !seg_off || !k_per_traj || !out_count || !ws) return PACES_E_INVALID;
-  if (R > 0 && (!psi || !rows || !out_rows)) return PACES_E_INVALID;
+  if (R > 0 && ((!psi && !keys) || !rows || !out_rows)) return PACES_E_INVALID;
   if (R >= 2147483647LL) return PACES_E_INVALID;
   if (ws_bytes < bsel_ws_bytes(R, ntraj)) return PACES_E_WORKSPACE;
   PACES_DISPATCH_W(W, run_bselect, psi, rows, R, seg_off, ntraj, k_per_traj, out_rows, out_count,
-                   ws, (hipStream_t)stream);
+                   ws, (hipStream_t)stream, nullptr, nullptr, keys);
 }
 
-int paces_batch_select_flags(const double* psi, int64_t R, const int32_t* seg_off, int32_t ntraj,
-                             const int64_t* k_per_traj, const uint8_t* in_be, uint8_t* keep,
-                             int32_t* out_count, void* ws, size_t ws_bytes, void* stream) {
+int paces_batch_select_flags(const double* psi, const uint64_t* keys, int64_t R,
+                             const int32_t* seg_off, int32_t ntraj, const int64_t* k_per_traj,
+                             const uint8_t* in_be, uint8_t* keep, int32_t* out_count, void* ws,
+                             size_t ws_bytes, void* stream) {
   if (R < 0 || ntraj < 1 || !seg_off || !k_per_traj || !out_count || !ws) return PACES_E_INVALID;
-  if (R > 0 && (!psi || !keep)) return PACES_E_INVALID;
+  if (R > 0 && ((!psi && !keys) || !keep)) return PACES_E_INVALID;
   if (R >= 2147483647LL) return PACES_E_INVALID;
   if (ws_bytes < bsel_ws_bytes(R, ntraj)) return PACES_E_WORKSPACE;
   return run_bselect<0>(psi, nullptr, R, seg_off, ntraj, k_per_traj, nullptr, out_count, ws,
-                        (hipStream_t)stream, keep, in_be);
+                        (hipStream_t)stream, keep, in_be, keys);
 }
 
 size_t paces_holstein_observables_batched_workspace_bytes(int64_t R, int32_t ntraj,
@@ -1509,8 +1579,25 @@ int paces_holstein_observables_batched(const uint64_t* rows, const double* psi, 
   if (lay->nfields != h->nchain + 2) return PACES_E_INVALID;
   if (!h->d_traj_L || h->geo_dim < 1 || h->geo_dim > 3) return PACES_E_INVALID;
   if (ws_bytes < bobs_ws_bytes(R, h->ntraj, h->nchain)) return PACES_E_WORKSPACE;
+  const ObsFinish fin = {nullptr, 0, nullptr, nullptr};
+  PACES_DISPATCH_W(lay->W, run_bobs, rows, const_cast<double*>(psi), R, seg_off, h->ntraj, lay,
+                   h->nchain, h->d_traj_L, h->geo_dim, out, ws, (hipStream_t)stream, fin);
+}
+
+int paces_holstein_observables_batched_finish(uint64_t const* rows, double* psi, int64_t R,
+                                              const int32_t* seg_off, const paces_layout_t* lay,
+                                              const paces_holstein_t* h, const void* x_even,
+                                              int32_t mixed, const paces_batch_taylor_t* ts,
+                                              uint64_t* keys, double* out, void* ws,
+                                              size_t ws_bytes, void* stream) {
+  if (!lay || !h || !h->batched || !seg_off || !out || !ws || R < 0) return PACES_E_INVALID;
+  if (R > 0 && (!rows || !psi || (ts && !x_even))) return PACES_E_INVALID;
+  if (lay->nfields != h->nchain + 2) return PACES_E_INVALID;
+  if (!h->d_traj_L || h->geo_dim < 1 || h->geo_dim > 3) return PACES_E_INVALID;
+  if (ws_bytes < bobs_ws_bytes(R, h->ntraj, h->nchain)) return PACES_E_WORKSPACE;
+  const ObsFinish fin = {x_even, mixed ? 1 : 0, ts, (u64*)keys};
   PACES_DISPATCH_W(lay->W, run_bobs, rows, psi, R, seg_off, h->ntraj, lay, h->nchain,
-                   h->d_traj_L, h->geo_dim, out, ws, (hipStream_t)stream);
+                   h->d_traj_L, h->geo_dim, out, ws, (hipStream_t)stream, fin);
 }
 
 size_t paces_batch_taylor_workspace_bytes(int64_t R, int32_t ntraj) {

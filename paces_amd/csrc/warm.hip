@@ -33,34 +33,52 @@ constexpr int kWTile = kBlock * kWRpt;  // rows per workgroup
 // stored in both rows), so each row reads its own slots and marks itself when one names a source
 // in the old B_e: every row's flag byte is written by its own thread only, and the bits the
 // others read (src_mask, skip_mask, bit 7) are not among those it writes.
+// A thread takes kMarkRpt rows (t, t + kBlock, ...): every slot of its rows is loaded, then every
+// named row's flags, then the rows are decided, so the dependent loads of all rows overlap.
+constexpr int kMarkRpt = 4;
+
+template <int NS>
 __global__ void __launch_bounds__(kBlock) k_slots_mark(const uint32_t* __restrict__ slots,
-                                                       int64_t stride, int nslots, uint32_t colmask,
+                                                       int64_t stride, uint32_t colmask,
                                                        int64_t R, uint8_t* flags, int src_mask,
                                                        int skip_mask, int mark_bit,
                                                        uint8_t* __restrict__ expl,
                                                        int32_t* __restrict__ count) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int e = 0;
-  if (i < R) {
-    uint32_t nb[PACES_MAX_SLOTS];
+  const int64_t i0 = (int64_t)blockIdx.x * (kBlock * kMarkRpt) + threadIdx.x;
+  uint32_t nb[kMarkRpt][NS];
+  int f[kMarkRpt];
 #pragma unroll
-    for (int m = 0; m < PACES_MAX_SLOTS; ++m)
-      nb[m] = m < nslots ? slots[(int64_t)m * stride + i] : PACES_SLOT_EMPTY;
-    const int f = flags[i];
-    int g[PACES_MAX_SLOTS];
+  for (int q = 0; q < kMarkRpt; ++q) {
+    const int64_t i = i0 + q * kBlock;
+    const int64_t ic = i < R ? i : 0;
 #pragma unroll
-    for (int m = 0; m < PACES_MAX_SLOTS; ++m)
-      g[m] = nb[m] != PACES_SLOT_EMPTY ? flags[nb[m] & colmask] : 0;
+    for (int m = 0; m < NS; ++m) nb[q][m] = i < R ? slots[(int64_t)m * stride + ic] : PACES_SLOT_EMPTY;
+    f[q] = flags[ic];
+  }
+  int g[kMarkRpt][NS];
+#pragma unroll
+  for (int q = 0; q < kMarkRpt; ++q)
+#pragma unroll
+    for (int m = 0; m < NS; ++m)
+      g[q][m] = nb[q][m] != PACES_SLOT_EMPTY ? flags[nb[q][m] & colmask] : 0;
+  int ne = 0;
+#pragma unroll
+  for (int q = 0; q < kMarkRpt; ++q) {
+    const int64_t i = i0 + q * kBlock;
     bool hit = false;
 #pragma unroll
-    for (int m = 0; m < PACES_MAX_SLOTS; ++m)
-      hit = hit || ((g[m] & 0x80) && (g[m] & src_mask) && !(g[m] & skip_mask));
-    if (hit && !(f & mark_bit)) flags[i] = (uint8_t)(f | mark_bit);
-    e = ((f & src_mask) && !(f & skip_mask) && !(f & 0x80)) ? 1 : 0;
-    expl[i] = (uint8_t)e;
+    for (int m = 0; m < NS; ++m)
+      hit = hit || ((g[q][m] & 0x80) && (g[q][m] & src_mask) && !(g[q][m] & skip_mask));
+    const bool e = i < R && (f[q] & src_mask) && !(f[q] & skip_mask) && !(f[q] & 0x80);
+    if (i < R) {
+      if (hit && !(f[q] & mark_bit)) flags[i] = (uint8_t)(f[q] | mark_bit);
+      expl[i] = e ? 1 : 0;
+    }
+    ne += e ? 1 : 0;
   }
-  const u64 bl = __ballot(e);
-  if ((threadIdx.x & 63) == 0 && bl) atomicAdd(count, (int32_t)__popcll(bl));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) ne += __shfl_xor(ne, o, 64);
+  if ((threadIdx.x & 63) == 0 && ne) atomicAdd(count, ne);
 }
 
 // ---- stable compaction of flagged indices ------------------------------------------------------
@@ -342,10 +360,18 @@ int paces_slots_mark(const uint32_t* slots, int64_t stride, int32_t nslots, int3
   const int tested = src_mask | skip_mask | 0x80;
   if (mark_bit < 1 || mark_bit > 0x7f || (mark_bit & tested) || ((src_mask | skip_mask) & 0x80))
     return PACES_E_INVALID;
-  if (R > 0)
-    hipLaunchKernelGGL(k_slots_mark, dim3(grid_for(R, kBlock)), dim3(kBlock), 0,
-                       (hipStream_t)stream, slots, stride, (int)nslots, cm, R, flags,
-                       (int)src_mask, (int)skip_mask, (int)mark_bit, expl, expl_count);
+  if (nslots != 4 && nslots != 8) return PACES_E_INVALID;
+  if (R > 0) {
+    const dim3 g(grid_for(R, kBlock * kMarkRpt));
+    if (nslots == 4)
+      hipLaunchKernelGGL(k_slots_mark<4>, g, dim3(kBlock), 0, (hipStream_t)stream, slots, stride,
+                         cm, R, flags, (int)src_mask, (int)skip_mask, (int)mark_bit, expl,
+                         expl_count);
+    else
+      hipLaunchKernelGGL(k_slots_mark<8>, g, dim3(kBlock), 0, (hipStream_t)stream, slots, stride,
+                         cm, R, flags, (int)src_mask, (int)skip_mask, (int)mark_bit, expl,
+                         expl_count);
+  }
   return launch_status();
 }
 

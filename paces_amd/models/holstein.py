@@ -1171,19 +1171,26 @@ class BatchedHamiltonian(Hamiltonian):
               "batch_segments")
         return seg
 
-    def site_observables_batched_async(self, rows, vector, seg_off, out=None):
+    def site_observables_batched_async(self, rows, vector, seg_off, out=None, finish=None,
+                                       keys=None):
         """Launches the per-trajectory observables into out (device float64, ntraj * (2N + 1):
-        per trajectory n_exc[N], n_pho[N], sum|psi|^2) and returns out; no host sync."""
+        per trajectory n_exc[N], n_pho[N], sum|psi|^2) and returns out; no host sync. finish =
+        (x_even, series states, mixed): the series' last even terms are first added to vector
+        in place (paces_holstein_observables_batched_finish); keys (int64, R) receives the
+        rows' select keys."""
         R = rows.shape[0]
         N = self.nchain
         if out is None:
             out = torch.empty(self.ntraj * (2 * N + 1), dtype=torch.float64, device=self.device)
         wsp, wsn = self._ws.get(
             _lib.paces_holstein_observables_batched_workspace_bytes(R, self.ntraj, N))
-        check(_lib.paces_holstein_observables_batched(
+        x_even, ts, mixed = finish if finish is not None else (None, None, 0)
+        check(_lib.paces_holstein_observables_batched_finish(
             ptr(rows) if R else None, ptr(torch.view_as_real(vector)) if R else None, R,
-            ptr(seg_off), C.byref(self._layout), C.byref(self._hs), ptr(out), wsp, wsn,
-            stream()), "holstein_observables_batched")
+            ptr(seg_off), C.byref(self._layout), C.byref(self._hs),
+            ptr(torch.view_as_real(x_even)) if x_even is not None else None, 1 if mixed else 0,
+            ptr(ts) if ts is not None else None, ptr(keys) if keys is not None and R else None,
+            ptr(out), wsp, wsn, stream()), "holstein_observables_batched_finish")
         return out
 
     def split_observables(self, flat):
@@ -1289,6 +1296,10 @@ class BatchedTimeEvolution:
         self.incremental_max = float(incremental_max)
         self._prev_be = None  # the expanded basis's rows among whoami (uint8), for the next step
         self.last_adapt = None
+        # the last series' finish (x_even, states, mixed), done by the observables pass, and the
+        # select keys that pass formed for the current amplitudes
+        self._finish = None
+        self._keys = None
         self._graph_terms = 8 if precision == "mixed" else 16
         self.last_launch_shape = None
         self.whoami = None
@@ -1314,6 +1325,8 @@ class BatchedTimeEvolution:
         raw[:, 1] = pos
         self.whoami = h.compress_ind_arr(raw)
         self.vector = torch.ones(h.ntraj, dtype=torch.complex128, device=self.device)
+        self._keys = None
+        self._finish = None
         self._set_segments(h.segments(self.whoami))
 
     def _select(self):
@@ -1321,10 +1334,12 @@ class BatchedTimeEvolution:
         out = torch.empty((max(R, 1), W), dtype=torch.int64, device=self.device)
         cnt = torch.empty(1, dtype=torch.int32, device=self.device)
         wsp, wsn = self._ws.get(_lib.paces_batch_select_workspace_bytes(R, self.hamobj.ntraj))
-        check(_lib.paces_batch_select_topk(ptr(torch.view_as_real(self.vector)), ptr(self.whoami),
-                                           R, W, ptr(self.seg_off), self.hamobj.ntraj,
-                                           ptr(self._d_ms), ptr(out), ptr(cnt), wsp, wsn,
-                                           stream()), "batch_select_topk")
+        check(_lib.paces_batch_select_topk_keys(ptr(torch.view_as_real(self.vector)),
+                                                ptr(self._keys) if self._keys is not None else
+                                                None, ptr(self.whoami), R, W, ptr(self.seg_off),
+                                                self.hamobj.ntraj, ptr(self._d_ms), ptr(out),
+                                                ptr(cnt), wsp, wsn, stream()),
+              "batch_select_topk_keys")
         host, ev = self._seg_host
         ev.synchronize()  # recorded a step ago: long complete
         n = int(numpy.minimum(numpy.diff(host.numpy()), self.maxstates).sum())
@@ -1350,6 +1365,7 @@ class BatchedTimeEvolution:
         self.diag_vals = diag
         self.sparse_mats_dict = mats
         self._prev_be = self.hamobj.last_in_basis if self.incremental else None
+        self._keys = None
         self._obs_last = None
         self._set_segments(self.hamobj.segments(new_inds))
         self.last_adapt = "full"
@@ -1415,10 +1431,11 @@ class BatchedTimeEvolution:
         flags = torch.empty(R0, dtype=torch.uint8, device=dev)
         cnt = torch.empty(1, dtype=torch.int32, device=dev)
         wsp, wsn = self._ws.get(_lib.paces_batch_select_workspace_bytes(R0, h.ntraj))
-        check(_lib.paces_batch_select_flags(ptr(torch.view_as_real(psi0)), R0, ptr(self.seg_off),
-                                            h.ntraj, ptr(self._d_ms), ptr(self._prev_be),
-                                            ptr(flags), ptr(cnt), wsp, wsn, stream()),
-              "batch_select_flags")
+        check(_lib.paces_batch_select_flags(ptr(torch.view_as_real(psi0)),
+                                            ptr(self._keys) if self._keys is not None else None,
+                                            R0, ptr(self.seg_off), h.ntraj, ptr(self._d_ms),
+                                            ptr(self._prev_be), ptr(flags), ptr(cnt), wsp, wsn,
+                                            stream()), "batch_select_flags")
         ecnt = torch.zeros(2, dtype=torch.int32, device=dev)
         e1 = torch.empty(R0, dtype=torch.uint8, device=dev)
         e2 = torch.empty(R0, dtype=torch.uint8, device=dev)
@@ -1514,6 +1531,7 @@ class BatchedTimeEvolution:
         self.sparse_mats_dict = {name: sm1.term(t, lambda m, rows=rows1: h._coded_values(m, rows))
                                  for t, name in enumerate(h.offdiag_terms)}
         self._prev_be = be1
+        self._keys = None
         self._obs_last = None
         self._set_segments(h.segments(rows1))
         self.last_adapt = ("incremental", n1, n2, nY)
@@ -1653,7 +1671,7 @@ class BatchedTimeEvolution:
         g.replay()
         return nterms
 
-    def propagate(self, delta_t):
+    def propagate(self, delta_t, fuse_finish=False):
         h = self.hamobj
         R = self.numstates
         mats = [self.sparse_mats_dict[t] for t in h.offdiag_terms]
@@ -1698,13 +1716,19 @@ class BatchedTimeEvolution:
             if int(hdr[:4].view(torch.int32).item()) == 0:
                 break
         # trajectories whose series ended on an even term get that term added (the sum is
-        # stored on odd terms); even terms wrote bufs[0]
-        check(_lib.paces_batch_taylor_finish(R, ptr(torch.view_as_real(bufs[0])),
-                                             1 if mixed else 0, ptr(torch.view_as_real(res)),
-                                             ptr(self.seg_off), h.ntraj, ptr(ts), wsp, wsn, cref,
-                                             st), "batch_taylor_finish")
+        # stored on odd terms); even terms wrote bufs[0]. With fuse_finish the step's
+        # observables pass adds it (step() launches that pass next and nothing reads the
+        # amplitudes in between).
+        if fuse_finish:
+            self._finish = (bufs[0], ts, mixed)
+        else:
+            check(_lib.paces_batch_taylor_finish(R, ptr(torch.view_as_real(bufs[0])),
+                                                 1 if mixed else 0, ptr(torch.view_as_real(res)),
+                                                 ptr(self.seg_off), h.ntraj, ptr(ts), wsp, wsn,
+                                                 cref, st), "batch_taylor_finish")
         del keep
         self.vector = res
+        self._keys = None
         self._obs_last = None
         # the per-trajectory series states go to pinned host memory without a stream sync; they
         # are parsed when first read (last_expm, or the next step's graph size)
@@ -1768,7 +1792,7 @@ class BatchedTimeEvolution:
         dbg = self.adapt()
         self._resolve_observables()  # the last step's: the adaptation has waited on the device
         if i > 0:
-            self.propagate(delta_t)
+            self.propagate(delta_t, fuse_finish=True)
         self._launch_observables(i)
         return dbg
 
@@ -1778,7 +1802,11 @@ class BatchedTimeEvolution:
         if self._obs_dev is None or self._obs_dev.numel() != n:
             self._obs_dev = torch.empty(n, dtype=torch.float64, device=self.device)
             self._obs_host = torch.empty(n, dtype=torch.float64, device="cpu", pin_memory=True)
-        h.site_observables_batched_async(self.whoami, self.vector, self.seg_off, self._obs_dev)
+        keys = torch.empty(self.numstates, dtype=torch.int64, device=self.device)
+        h.site_observables_batched_async(self.whoami, self.vector, self.seg_off, self._obs_dev,
+                                         finish=self._finish, keys=keys)
+        self._finish = None
+        self._keys = keys
         self._obs_host.copy_(self._obs_dev, non_blocking=True)
         ev = torch.cuda.Event()
         ev.record(torch.cuda.current_stream())

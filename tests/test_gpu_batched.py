@@ -231,3 +231,43 @@ def test_single_trajectory_batch_matches_oracle(csr_values):
         assert np.array_equal(bte.vector.cpu().numpy(), rec.vec), i
         assert np.max(np.abs(n_exc[0, :N] - rec.obs["n_exc"])) <= 1e-10, i
         assert np.max(np.abs(n_pho[0, :N] - rec.obs["n_pho"])) <= 1e-10, i
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fused_finish_equals_separate_finish(precision):
+    """The step's observables pass also finishes the series (adds the last even term of the
+    trajectories that stopped on one) and forms the next select's keys. Every amplitude must be
+    finished exactly once: the step equals the one with the separate finish and key passes bit
+    for bit, at every step, twice over (a workgroup's waves share each row's amplitude, so a
+    wave re-reading a finished amplitude would show here)."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from paces_amd.config import devices
+    from paces_amd.models import holstein
+    if not devices.configure_called:
+        devices.configure()
+    draws = _draws(7, seed=5)
+    for rep in range(2):
+        runs = []
+        for graph in (False, True):
+            bham = holstein.BatchedHamiltonian([d[0] for d in draws], 8, [-d[1] for d in draws],
+                                               [d[2] for d in draws])
+            bte = holstein.BatchedTimeEvolution(bham, [d[3] for d in draws], enlarge_steps=1,
+                                                precision=precision, graph=graph)
+            bte.create_single_exciton_states([d[0] // 2 for d in draws])
+            runs.append(bte)
+        fused, separate = runs
+        for i in range(12):
+            print(f"[fused finish {precision}] rep {rep} step {i}", flush=True)
+            fused.step(i, 0.05)
+            separate.adapt()
+            separate._resolve_observables()
+            if i > 0:
+                separate.propagate(0.05, fuse_finish=False)
+            separate._launch_observables(i)
+            separate._keys = None  # the separate form's next select forms its own keys
+            assert torch.equal(fused.whoami, separate.whoami), (rep, i)
+            assert torch.equal(fused.vector, separate.vector), (rep, i)
+            a, b = fused.observables(), separate.observables()
+            assert all(np.array_equal(x, y) for x, y in zip(a, b)), (rep, i)
