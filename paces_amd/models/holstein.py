@@ -1655,7 +1655,10 @@ class BatchedTimeEvolution:
         for j in range(_hip.SERIES_MAX_TERMS):
             d.c[j] = engine.taylor_coeff(delta_t, j)
         pool = self.graph_pool
-        pool.desc.copy_(torch.frombuffer(bytearray(bytes(d)), dtype=torch.uint8))
+        # from pinned memory without a host wait (a pageable copy would wait for the stream to
+        # drain); the caching host allocator keeps the block until the copy has run
+        pool.desc.copy_(torch.frombuffer(bytearray(bytes(d)), dtype=torch.uint8).pin_memory(),
+                        non_blocking=True)
         shape = (cfg.block, cfg.rows_per_thread) if cfg is not None else None
         nslots = d.nslots if d.slots else 0  # the term kernel instance: CSR (0) or slots
         key = (cap, h.ntraj, nterms, mixed, shape, nslots)
