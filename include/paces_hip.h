@@ -340,6 +340,12 @@ int paces_flag_indices(const uint8_t* flags, int64_t n, int32_t mask, int32_t* o
 /* out[j] = rows[idx[j]] (W words per row) for j < m */
 int paces_gather_rows(const uint64_t* rows, int32_t W, const int32_t* idx, int64_t m,
                       uint64_t* out, void* stream);
+/* dst[dmap ? dmap[j] : j] = src[smap ? smap[j] : j] for j < m (no reference counterpart: the
+ * incremental adaptation composes row positions through it instead of searching for them) */
+int paces_move_i32(const int32_t* src, const int32_t* smap, const int32_t* dmap, int64_t m,
+                   int32_t* dst, void* stream);
+/* dst[dmap[j]] = v (0..255) for j < m (no reference counterpart) */
+int paces_fill_u8_at(uint8_t* dst, const int32_t* dmap, int64_t m, int32_t v, void* stream);
 /* flags[idx[j]] |= bit where hit[j]; miss[j] = !hit[j] (j < m) */
 int paces_mark_hits(const int32_t* idx, const uint8_t* hit, int64_t m, uint8_t* flags,
                     int32_t bit, uint8_t* miss, void* stream);

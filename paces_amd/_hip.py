@@ -206,6 +206,8 @@ _SIGNATURES = {
     "paces_flag_indices": (C.c_int, [_P, _I64, _I32, _P, _P, _P, _SZ, _P]),
     "paces_gather_rows": (C.c_int, [_P, _I32, _P, _I64, _P, _P]),
     "paces_mark_hits": (C.c_int, [_P, _P, _I64, _P, _I32, _P, _P]),
+    "paces_move_i32": (C.c_int, [_P, _P, _P, _I64, _P, _P]),
+    "paces_fill_u8_at": (C.c_int, [_P, _P, _I64, _I32, _P]),
     "paces_warm_positions_workspace_bytes": (_SZ, [_I64]),
     "paces_warm_positions": (C.c_int, [_P, _I64, _P, _I64, _P, _P, _P, _P, _SZ, _P]),
     "paces_warm_emit": (C.c_int, [_P, _I32, _P, _P, _I32, _I64, _P, _I32, _P, _P, _I64, _I32,

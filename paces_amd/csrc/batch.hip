@@ -560,11 +560,14 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
                                                          const int32_t* __restrict__ width, int N,
                                                          const int32_t* __restrict__ traj_L, int geo,
                                                          double* __restrict__ part) {
-  constexpr int TB = obs_tile<W>(), RPL = TB / 64, PER = TB * W / kBlock;
+  constexpr int TB = obs_tile<W>(), RPL = TB / 64, PER = TB * W / kBlock, TPR = TB / kBlock;
+  constexpr int NW = kBlock / 64;
   static_assert(TB % kBlock == 0, "whole rows per thread");
   __shared__ u64 srow[TB * W];
   __shared__ double sw[TB];
+  __shared__ short spos[TB];  // positions < PACES_MAX_SITES; -1 past the tile's rows
   __shared__ int spr[2];
+  __shared__ double snorm[NW];
   const SegTile T = seg_tile(seg_off, boff, ntraj, TB);
   if (T.t < 0) return;
   if (threadIdx.x == 0) {
@@ -573,8 +576,9 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
   }
   const int S = 2 * N + 1;
   const int nr = (int)(T.r1 - T.r0);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double w[RPL];
+  // the wave index is uniform: the site loop's field offsets then come from scalar loads and
+  // its branches are scalar
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   {
     // the rows to LDS
     u64 tmp[PER];
@@ -593,7 +597,6 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
     // the amplitudes, each row by one thread (rows t, t + kBlock, ...): a trajectory whose
     // series ended on an even term gets that term added first (each row read and written once);
     // the weights go through LDS to the lanes that hold their rows in the site loop below
-    constexpr int TPR = TB / kBlock;
     const bool add = fin.ts && (fin.ts[T.t].j_ind & 1) == 0;
     double2 v[TPR];
     int64_t ii[TPR];
@@ -621,56 +624,71 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
         if ((int)threadIdx.x + kBlock * q < nr) psi[ii[q]] = v[q];
       }
     }
+    double a = 0.0;
 #pragma unroll
     for (int q = 0; q < TPR; ++q) {
       const int r = (int)threadIdx.x + kBlock * q;
       const double k = v[q].x * v[q].x + v[q].y * v[q].y;
       if (fin.keys && r < nr) fin.keys[ii[q]] = (u64)__double_as_longlong(k);
       sw[r] = r < nr ? k : 0.0;
+      a += r < nr ? k : 0.0;
     }
+    a = wave_sum(a);
+    if (lane == 0) snorm[wave] = a;
   }
   __syncthreads();
-#pragma unroll
-  for (int j = 0; j < RPL; ++j) w[j] = sw[lane + 64 * j];
-  int pos[RPL];
-#pragma unroll
-  for (int j = 0; j < RPL; ++j) {
-    const int r = lane + 64 * j;
-    pos[j] = -1;
-    if (r < nr) {
-      Row<W> rr;
-#pragma unroll
-      for (int k = 0; k < W; ++k) rr.w[k] = srow[r * W + k];
-      pos[j] = (int)get_field<W>(rr, off[1], width[1]);
-    }
-  }
   {
+    // each row's exciton position (field 1, uniform offset), once per row, and the tile's range
+    const int po = __builtin_amdgcn_readfirstlane(off[1]);
+    const int pb = __builtin_amdgcn_readfirstlane(width[1]);
+    const int pk = po >> 6, pend = (po & 63) + pb;
+    const u64 pmask = (pb >= 64) ? ~0ull : ((1ull << pb) - 1ull);
     int lo = 0x7fffffff, hi = -1;
 #pragma unroll
-    for (int j = 0; j < RPL; ++j) {
-      if (pos[j] >= 0) {
-        lo = pos[j] < lo ? pos[j] : lo;
-        hi = pos[j] > hi ? pos[j] : hi;
+    for (int q = 0; q < TPR; ++q) {
+      const int r = (int)threadIdx.x + kBlock * q;
+      int p = -1;
+      if (r < nr) {
+        const u64 w0 = srow[r * W + pk];
+        u64 x;
+        if (pend <= 64) {
+          x = w0 >> (64 - pend);
+        } else {  // straddles words pk and pk + 1 (the layout keeps pk + 1 < W)
+          const u64 w1 = srow[r * W + pk + 1];
+          x = (w0 << (pend - 64)) | (w1 >> (128 - pend));
+        }
+        p = (int)(x & pmask);
+        lo = p < lo ? p : lo;
+        hi = p > hi ? p : hi;
       }
+      spos[r] = (short)p;
     }
-    if (hi >= 0) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const int l2 = __shfl_xor(lo, o, 64), h2 = __shfl_xor(hi, o, 64);
+      lo = l2 < lo ? l2 : lo;
+      hi = h2 > hi ? h2 : hi;
+    }
+    if (lane == 0 && hi >= 0) {
       atomicMin(&spr[0], lo);
       atomicMax(&spr[1], hi);
     }
   }
-  if (wave == 0) {
+  __syncthreads();
+  if (wave == 0 && lane == 0) {
     double a = 0.0;
 #pragma unroll
-    for (int j = 0; j < RPL; ++j) a += w[j];
-    a = wave_sum(a);
-    if (lane == 0) part[(int64_t)blockIdx.x * S + 2 * N] = a;
+    for (int k = 0; k < NW; ++k) a += snorm[k];
+    part[(int64_t)blockIdx.x * S + 2 * N] = a;
   }
-  __syncthreads();
+  double w[RPL];
+#pragma unroll
+  for (int j = 0; j < RPL; ++j) w[j] = sw[lane + 64 * j];
   const int pmin = spr[0], pmax = spr[1];
   // sites past the tile's own lattice hold no phonons and no exciton: their partials are zero
   const int L = traj_L[T.t];
   const int Nt = geo == 1 ? L : (geo == 2 ? L * L : L * L * L);
-  for (int s = wave; s < N; s += kBlock / 64) {
+  for (int s = wave; s < N; s += NW) {
     if (s >= Nt) {
       if (lane == 0) {
         part[(int64_t)blockIdx.x * S + s] = 0.0;
@@ -688,41 +706,42 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
         const int r = lane + 64 * j;
         const u64 n = ((srow[r * W + k] << (o + b - 64)) | (srow[r * W + k + 1] >> (128 - o - b))) &
                       mask;
-        ap += w[j] * (double)n;
+        ap = fma(w[j], (double)n, ap);
       }
-    } else if (b > 0) {
-      const u64 mask = (b >= 64) ? ~0ull : ((1ull << b) - 1ull);
+    } else if (b > 0 && b <= 32) {
+      const uint32_t mask = (b >= 32) ? ~0u : ((1u << b) - 1u);
+      const uint32_t* s32 = (const uint32_t*)srow;
       if (o + b <= 32 || o >= 32) {
         // the field lies in one 32-bit half of its word: a 32-bit LDS read and a bit-field
-        // extract per row, one FMA (the sum is compared within 1e-10, not bit for bit)
+        // extract per row, one FMA (the sums are compared within 1e-10, not bit for bit)
         const int half = o >= 32 ? 0 : 1;  // little endian: the high half is at byte 4
         const int sh = (half ? 32 : 64) - o - b;
-        const uint32_t* s32 = (const uint32_t*)srow;
 #pragma unroll
         for (int j = 0; j < RPL; ++j) {
           const int r = lane + 64 * j;
-          const uint32_t n = (s32[(r * W + k) * 2 + half] >> sh) & (uint32_t)mask;
+          const uint32_t n = (s32[(r * W + k) * 2 + half] >> sh) & mask;
           ap = fma(w[j], (double)n, ap);
-        }
-      } else if (b <= 32) {
-#pragma unroll
-        for (int j = 0; j < RPL; ++j) {
-          const int r = lane + 64 * j;
-          ap += w[j] * (double)(uint32_t)((srow[r * W + k] >> (64 - o - b)) & mask);
         }
       } else {
 #pragma unroll
         for (int j = 0; j < RPL; ++j) {
           const int r = lane + 64 * j;
-          ap += w[j] * (double)((srow[r * W + k] >> (64 - o - b)) & mask);
+          ap = fma(w[j], (double)(uint32_t)((srow[r * W + k] >> (64 - o - b)) & mask), ap);
         }
+      }
+    } else if (b > 0) {
+      const u64 mask = (b >= 64) ? ~0ull : ((1ull << b) - 1ull);
+#pragma unroll
+      for (int j = 0; j < RPL; ++j) {
+        const int r = lane + 64 * j;
+        ap = fma(w[j], (double)((srow[r * W + k] >> (64 - o - b)) & mask), ap);
       }
     }
     ap = wave_sum(ap);
     double ae = 0.0;
     if (s >= pmin && s <= pmax) {
 #pragma unroll
-      for (int j = 0; j < RPL; ++j) ae += (pos[j] == s) ? w[j] : 0.0;
+      for (int j = 0; j < RPL; ++j) ae += (spos[lane + 64 * j] == s) ? w[j] : 0.0;
       ae = wave_sum(ae);
     }
     if (lane == 0) {

@@ -1247,7 +1247,8 @@ __global__ void __launch_bounds__(kBlock) k_obs_partial(const u64* __restrict__ 
     }
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // a uniform wave index: the site loop's field offsets are scalar and its branches scalar
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   double w[RPL];
   int pos[RPL];
   int lo = 0x7fffffff, hi = -1;

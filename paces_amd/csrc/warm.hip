@@ -35,7 +35,7 @@ constexpr int kWTile = kBlock * kWRpt;  // rows per workgroup
 // others read (src_mask, skip_mask, bit 7) are not among those it writes.
 // A thread takes kMarkRpt rows (t, t + kBlock, ...): every slot of its rows is loaded, then every
 // named row's flags, then the rows are decided, so the dependent loads of all rows overlap.
-constexpr int kMarkRpt = 4;
+constexpr int kMarkRpt = 8;
 
 template <int NS>
 __global__ void __launch_bounds__(kBlock) k_slots_mark(const uint32_t* __restrict__ slots,
@@ -82,9 +82,20 @@ __global__ void __launch_bounds__(kBlock) k_slots_mark(const uint32_t* __restric
 }
 
 // ---- stable compaction of flagged indices ------------------------------------------------------
+static_assert(kWRpt == 8, "a thread's flags are one 8-byte load");
 __device__ __forceinline__ int tile_count(const uint8_t* __restrict__ flags, int64_t n, int mask,
                                           int64_t r0, int (&c)[kWRpt]) {
   int tot = 0;
+  if (r0 + kWRpt <= n && ((uintptr_t)flags & 7) == 0) {
+    // r0 is a multiple of 8: the thread's 8 flags in one aligned load
+    const u64 f8 = *(const u64*)(flags + r0);
+#pragma unroll
+    for (int j = 0; j < kWRpt; ++j) {
+      c[j] = ((int)(f8 >> (8 * j)) & mask) != 0 ? 1 : 0;
+      tot += c[j];
+    }
+    return tot;
+  }
 #pragma unroll
   for (int j = 0; j < kWRpt; ++j) {
     c[j] = (r0 + j < n && (flags[r0 + j] & mask) != 0) ? 1 : 0;
@@ -155,6 +166,24 @@ __global__ void __launch_bounds__(kBlock) k_gather_rows(const u64* __restrict__ 
   store_row<W>(out, j, load_row<W>(rows, idx[j]));
 }
 
+// dst[dmap ? dmap[j] : j] = src[smap ? smap[j] : j] (index bookkeeping of the explicit images)
+__global__ void __launch_bounds__(kBlock) k_move_i32(const int32_t* __restrict__ src,
+                                                     const int32_t* __restrict__ smap,
+                                                     const int32_t* __restrict__ dmap, int64_t m,
+                                                     int32_t* __restrict__ dst) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  dst[dmap ? (int64_t)dmap[j] : j] = src[smap ? (int64_t)smap[j] : j];
+}
+
+// dst[dmap[j]] = v
+__global__ void __launch_bounds__(kBlock) k_fill_u8_at(uint8_t* __restrict__ dst,
+                                                       const int32_t* __restrict__ dmap, int64_t m,
+                                                       uint8_t v) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m) dst[dmap[j]] = v;
+}
+
 // flags[idx[j]] |= bit where hit[j]; miss[j] = !hit[j]
 __global__ void __launch_bounds__(kBlock) k_mark_hits(const int32_t* __restrict__ idx,
                                                       const uint8_t* __restrict__ hit, int64_t m,
@@ -189,10 +218,34 @@ __device__ __forceinline__ int count_le(const int32_t* __restrict__ ins, int n, 
   return lo;
 }
 
+// number of entries of the ascending ins[lo, hi) that are <= v, plus lo
+__device__ __forceinline__ int count_le_in(const int32_t* __restrict__ ins, int lo, int hi,
+                                           int64_t v) {
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (ins[mid] <= v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// tfirst[t] = the entries of ins below tile t's first row (t <= ntiles), one search per tile, so
+// that the rows of a tile search only the tile's own insertion points
+__global__ void __launch_bounds__(kBlock) k_ins_tile_first(const int32_t* __restrict__ ins, int nY,
+                                                           int ntiles,
+                                                           int32_t* __restrict__ tfirst) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t > ntiles) return;
+  tfirst[t] = count_le(ins, nY, (int64_t)t * kWTile - 1);
+}
+
 __global__ void __launch_bounds__(kBlock) k_warm_newpos(const uint8_t* __restrict__ flags,
                                                         int64_t R, const int32_t* __restrict__ TO,
                                                         int ntiles,
                                                         const int32_t* __restrict__ ins, int nY,
+                                                        const int32_t* __restrict__ tfirst,
                                                         int32_t* __restrict__ NP,
                                                         int32_t* __restrict__ out_total) {
   const int64_t r0 = (int64_t)blockIdx.x * kWTile + (int64_t)threadIdx.x * kWRpt;
@@ -201,15 +254,28 @@ __global__ void __launch_bounds__(kBlock) k_warm_newpos(const uint8_t* __restric
   int all;
   int o = TO[blockIdx.x] + block_prefix(tot, &all);
   if (r0 >= R + 1) return;
-  int sh = nY ? count_le(ins, nY, r0) : 0;
+  int sh = nY ? count_le_in(ins, tfirst[blockIdx.x], tfirst[blockIdx.x + 1], r0) : 0;
+  // the next insertion point past r0 - 1 in a register: loads only as insertions are passed
+  int64_t nxt = sh < nY ? ins[sh] : INT64_MAX;
+  int v[kWRpt];
 #pragma unroll
   for (int j = 0; j < kWRpt; ++j) {
     const int64_t r = r0 + j;
-    if (r <= R) {
-      while (sh < nY && ins[sh] <= r) ++sh;
-      NP[r] = o + sh;
+    while (nxt <= r) {
+      ++sh;
+      nxt = sh < nY ? ins[sh] : INT64_MAX;
     }
+    v[j] = o + sh;
     o += c[j];
+  }
+  if (r0 + kWRpt <= R + 1 && ((uintptr_t)NP & 15) == 0) {
+    // r0 is a multiple of 8: two aligned 16-byte stores
+    ((int4*)(NP + r0))[0] = make_int4(v[0], v[1], v[2], v[3]);
+    ((int4*)(NP + r0))[1] = make_int4(v[4], v[5], v[6], v[7]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < kWRpt; ++j)
+      if (r0 + j <= R) NP[r0 + j] = v[j];
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) *out_total = TO[ntiles] + nY;
 }
@@ -422,9 +488,26 @@ int paces_mark_hits(const int32_t* idx, const uint8_t* hit, int64_t m, uint8_t* 
   return launch_status();
 }
 
+int paces_move_i32(const int32_t* src, const int32_t* smap, const int32_t* dmap, int64_t m,
+                   int32_t* dst, void* stream) {
+  if (m < 0 || (m > 0 && (!src || !dst))) return PACES_E_INVALID;
+  if (m > 0)
+    hipLaunchKernelGGL(k_move_i32, dim3(grid_for(m, kBlock)), dim3(kBlock), 0,
+                       (hipStream_t)stream, src, smap, dmap, m, dst);
+  return launch_status();
+}
+
+int paces_fill_u8_at(uint8_t* dst, const int32_t* dmap, int64_t m, int32_t v, void* stream) {
+  if (m < 0 || v < 0 || v > 255 || (m > 0 && (!dst || !dmap))) return PACES_E_INVALID;
+  if (m > 0)
+    hipLaunchKernelGGL(k_fill_u8_at, dim3(grid_for(m, kBlock)), dim3(kBlock), 0,
+                       (hipStream_t)stream, dst, dmap, m, (uint8_t)v);
+  return launch_status();
+}
+
 size_t paces_warm_positions_workspace_bytes(int64_t R) {
   const int64_t nt = (R + 1 + kWTile - 1) / kWTile;
-  return a256w((size_t)(nt + 1) * 4) * 2 + a256w(scan_ws_bytes(nt + 1));
+  return a256w((size_t)(nt + 1) * 4) * 3 + a256w(scan_ws_bytes(nt + 1));
 }
 
 int paces_warm_positions(const uint8_t* flags, int64_t R, const int32_t* ins, int64_t nY,
@@ -441,13 +524,18 @@ int paces_warm_positions(const uint8_t* flags, int64_t R, const int32_t* ins, in
   p += a256w((size_t)(nt + 1) * 4);
   int32_t* TO = (int32_t*)p;
   p += a256w((size_t)(nt + 1) * 4);
+  int32_t* tfirst = (int32_t*)p;
+  p += a256w((size_t)(nt + 1) * 4);
   hipLaunchKernelGGL(k_flag_tilecount, dim3(nt), dim3(kBlock), 0, st, flags, R, 0x7f, ts);
   int rc = launch_status();
   if (rc) return rc;
+  if (nY > 0)
+    hipLaunchKernelGGL(k_ins_tile_first, dim3(grid_for(nt + 1, kBlock)), dim3(kBlock), 0, st, ins,
+                       (int)nY, (int)nt, tfirst);
   rc = scan_i32(ts, nt, TO, p, scan_ws_bytes(nt + 1), st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_warm_newpos, dim3(nt), dim3(kBlock), 0, st, flags, R, (const int32_t*)TO,
-                     (int)nt, ins, (int)nY, NP, out_total);
+                     (int)nt, ins, (int)nY, (const int32_t*)tfirst, NP, out_total);
   if (nY > 0)
     hipLaunchKernelGGL(k_warm_ypos, dim3(grid_for(nY, kBlock)), dim3(kBlock), 0, st,
                        (const int32_t*)NP, ins, (int)nY, ypos);

@@ -1395,8 +1395,11 @@ class BatchedTimeEvolution:
 
     def _explicit(self, X, flags, bit):
         """Images of the sorted unique rows X (rows outside the previous expanded basis): their
-        union, with every image's index in it; the union rows found among the previous rows set
-        bit in their flags, the others (sorted) are returned as new rows."""
+        union U, with every image's index in it; the rows of U found among the previous rows set
+        bit in their flags, the others (sorted) are returned as new rows. The second element
+        keeps what the pairs and positions need: (X, run offsets, sources, phonon counts, U, the
+        images' indices in U, each U row's lower bound among the previous rows, the indices in
+        U of the new rows), or None without images."""
         rows0 = self.whoami
         imgs, offs, src, nup = self.hamobj._images(X, want_src=True)
         nimg = offs[-1]
@@ -1410,8 +1413,17 @@ class BatchedTimeEvolution:
         check(_lib.paces_mark_hits(ptr(idx), ptr(hit), nU, ptr(flags), bit, ptr(miss),
                                    stream()), "mark_hits")
         midx, mcnt = self._flag_indices(miss, nU, 1)
-        new = self._gather(U, midx, int(mcnt.item()))
-        return new, (X, offs, src[:nimg], nup[:nimg], U, cpos)
+        m = int(mcnt.item())
+        new = self._gather(U, midx, m)
+        return new, (X, offs, src[:nimg], nup[:nimg], U, cpos, idx, midx[:m])
+
+    @staticmethod
+    def _move(dst, src, smap, dmap, m):
+        """dst[dmap[j]] = src[smap[j]] for j < m (a None map is the identity)."""
+        if m:
+            check(_lib.paces_move_i32(ptr(src), ptr(smap) if smap is not None else None,
+                                      ptr(dmap) if dmap is not None else None, m, ptr(dst),
+                                      stream()), "move_i32")
 
     def _adapt_incremental(self):
         """The new basis from the previous one (paces_slots_mark / _warm_*): the selection as
@@ -1455,42 +1467,74 @@ class BatchedTimeEvolution:
             sidx, scnt = self._flag_indices(flags, R0, 1)
             self._selected_rows = self._gather(rows0, sidx, int(scnt.item()))
             return None
-        pairs = []
+        # Each explicit pass p leaves (its sources X_p, where their positions in the new basis
+        # come from, its image record); the new rows' insertion points, their expanded-basis
+        # bytes and every position the explicit pairs need are composed from the lookups of the
+        # passes' image unions (no further searches).
+        passes = []
         y1 = rows0[:0]
         if n1:
             idx1, _ = self._flag_indices(e1, R0, 1)
             y1, pr = self._explicit(self._gather(rows0, idx1, n1), flags, 2)
             if pr is not None:
-                pairs.append(pr)
+                passes.append((pr, [(idx1, None, n1)]))
             # the rows found by the first explicit images are first-expansion rows too
             e2c = torch.zeros(1, dtype=torch.int32, device=dev)
             mark(2, 1, 4, e2, ptr(e2c))
             n2 = int(e2c.item())
+        ny1 = y1.shape[0]
         x2 = None
+        x2_src = []  # (indices into the previous rows or None for the rows of y1, their places)
         if n2:
             idx2, _ = self._flag_indices(e2, R0, 1)
             x2 = self._gather(rows0, idx2, n2)
-        if y1.shape[0]:
+            x2_src.append((idx2, None, n2))
+        if ny1:
             # the new first-expansion rows are not previous rows: a disjoint sorted union
-            x2 = y1 if x2 is None else self._union([(x2, 0, x2.shape[0]),
-                                                    (y1, 0, y1.shape[0])])[0]
-        Y = y1
+            if x2 is None:
+                x2 = y1
+                x2_src.append((None, None, ny1))
+            else:
+                x2, cx2 = self._union([(x2, 0, n2), (y1, 0, ny1)])
+                x2_src = [(idx2, cx2[:n2], n2), (None, cx2[n2:n2 + ny1], ny1)]
+        Y, cY = y1, None
+        y2pr = None
         if x2 is not None:
             y2, pr = self._explicit(x2, flags, 4)
             if pr is not None:
-                pairs.append(pr)
+                passes.append((pr, x2_src))
+                y2pr = pr
             if y2.shape[0]:
-                Y = y2 if not y1.shape[0] else self._union([(y1, 0, y1.shape[0]),
-                                                            (y2, 0, y2.shape[0])])[0]
+                if not ny1:
+                    Y = y2
+                else:
+                    Y, cY = self._union([(y1, 0, ny1), (y2, 0, y2.shape[0])])
         nY = Y.shape[0]
         ybe = None
         ins = None
+        yidx = {}  # each pass's new rows: their indices in Y (None: the identity)
         if nY:
-            ins, _, _ = engine.lower_bound(rows0, Y, want_hit=False, q_sorted=True, ws=self._ws)
-            if y1.shape[0]:
-                _, ybe, _ = engine.lower_bound(y1, Y, want_hit=True, q_sorted=True, ws=self._ws)
-            else:
-                ybe = torch.zeros(nY, dtype=torch.uint8, device=dev)
+            ins = torch.empty(nY, dtype=torch.int32, device=dev)
+            ybe = torch.zeros(nY, dtype=torch.uint8, device=dev)
+            for pr, _ in passes:
+                ym = pr[7].shape[0]
+                if not ym:
+                    continue
+                if cY is None:
+                    yi = None
+                elif pr is y2pr:
+                    yi = cY[ny1:ny1 + ym]
+                else:
+                    yi = cY[:ny1]
+                yidx[id(pr)] = yi
+                # a new row's insertion point is its lower bound found by its pass's lookup
+                self._move(ins, pr[6], pr[7], yi, ym)
+            if ny1:
+                if cY is None:
+                    ybe.fill_(1)
+                else:
+                    check(_lib.paces_fill_u8_at(ptr(ybe), ptr(cY), ny1, 1, stream()),
+                          "fill_u8_at")
         NP = torch.empty(R0 + 1, dtype=torch.int32, device=dev)
         ypos = torch.empty(max(nY, 1), dtype=torch.int32, device=dev)
         tot = torch.empty(1, dtype=torch.int32, device=dev)
@@ -1518,9 +1562,29 @@ class BatchedTimeEvolution:
                                            lay.nslots, ptr(rows1),
                                            ptr(torch.view_as_real(psi1)), ptr(diag1), ptr(be1),
                                            ptr(slots1), R1, stream()), "warm_emit_new")
-        for X, offs, src, nup, U, cpos in pairs:
-            sp, _, _ = engine.lower_bound(rows1, X, want_hit=False, q_sorted=True, ws=self._ws)
-            up, _, _ = engine.lower_bound(rows1, U, want_hit=False, q_sorted=True, ws=self._ws)
+        y1pos = None
+        if ny1:
+            # the rows of y1 in the new basis (they are the first pass's new rows)
+            y1pos = torch.empty(ny1, dtype=torch.int32, device=dev)
+            self._move(y1pos, ypos, cY[:ny1] if cY is not None else None, None, ny1)
+        for pr, xsrc in passes:
+            X, offs, src, nup, U, cpos, uidx, umiss = pr
+            # the sources' positions: a previous row's through NP, a row of y1's through ypos
+            sp = torch.empty(X.shape[0], dtype=torch.int32, device=dev)
+            for idx, place, m in xsrc:
+                if idx is not None:
+                    self._move(sp, NP, idx, place, m)
+                else:
+                    self._move(sp, y1pos, None, place, m)
+            # the images' rows: a previous row's through NP at its lower bound, a new row's
+            # through ypos
+            nU = U.shape[0]
+            up = torch.empty(nU, dtype=torch.int32, device=dev)
+            self._move(up, NP, uidx, None, nU)
+            ym = umiss.shape[0]
+            if ym:
+                yi = yidx[id(pr)]
+                self._move(up, ypos, yi, umiss, ym)
             run_off = (C.c_int64 * len(offs))(*offs)
             check(_lib.paces_holstein_slots_place_explicit(R1, run_off, ptr(sp), ptr(up),
                                                            ptr(cpos), ptr(src), ptr(nup),
