@@ -154,15 +154,15 @@ TRAJ_DESC = ("traj256: {n} independent 1D Holstein chains (open), N in {{16,24,3
 
 
 def btay_term_bytes(R, mats, mixed, form="explicit"):
-    """Algorithmic HBM bytes of one batched Taylor term over R rows, averaged over an even and an
-    odd term (the roofline loop times both alike): diagonal (4 or 8 per row), the matrix, x_in
-    and x_out (8 or 16 per row each), and the complex128 partial sum: FP64 form, read on every
-    term and written on odd terms (16 + 8 per row); mixed form, read and written on odd terms
-    only (16 per row). The matrix: slot form, 4 bytes per slot (nslots per row, every term);
-    CSR forms, each term's indptr (4(R+1)) and indices + values (4 + 4 or 8 per nonzero; coded:
-    4 + 1 per vib nonzero and 4 per hopping nonzero)."""
+    """Algorithmic HBM bytes of one batched Taylor term over R rows: diagonal (4 or 8 per row),
+    the matrix, x_in and x_out (8 or 16 per row each), and in the FP64 form the complex128
+    partial sum, read on every term and written on odd terms (16 + 8 per row, the mean of an even
+    and an odd term: the roofline loop times both alike); the mixed form's terms neither read nor
+    write it. The matrix: slot form, 4 bytes per slot (nslots per row, every term); CSR forms,
+    each term's indptr (4(R+1)) and indices + values (4 + 4 or 8 per nonzero; coded: 4 + 1 per
+    vib nonzero and 4 per hopping nonzero)."""
     f = 4 if mixed else 8
-    res = 16 if mixed else 24
+    res = 0 if mixed else 24
     if form == "slots":
         return f * R + 4 * mats[0].slotmat.layout.nslots * R + 4 * f * R + res * R
     nnz = sum(m.nnz for m in mats)
@@ -178,10 +178,7 @@ def btay_term_bytes(R, mats, mixed, form="explicit"):
 # WRITE_SIZE counter passes over the roofline loop below with the shape fixed (--term-shape),
 # each scaled by the counter's reading for a 1 GiB device copy
 # (profiles/k_btay_term_traj256_hbm_traffic.txt). Other forms and shapes report null.
-BTAY_TRAFFIC_BYTES_PER_ROW = {("mixed", (256, 4), "slots"): 69.28,
-                              ("mixed", (256, 2), "coded"): 75.27,
-                              ("mixed", (256, 4), "coded"): 76.14,
-                              ("mixed", (512, 4), "coded"): 74.77}
+BTAY_TRAFFIC_BYTES_PER_ROW = {}
 
 
 def measure_batched_term(bte, reps=20):

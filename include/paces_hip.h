@@ -616,18 +616,21 @@ int paces_holstein_observables_batched(const uint64_t* rows, const double* psi, 
                                        const paces_holstein_t* h, double* out, void* ws,
                                        size_t ws_bytes, void* stream);
 /* The same at the end of a series (no reference counterpart): with ts (the series' per-trajectory
- * states), a trajectory whose last term index is even first gets x_even (complex64 when mixed,
- * else complex128; the buffer its even terms wrote) added to its amplitudes in psi, the work of
- * paces_batch_taylor_finish; keys (optional, R entries) receives each row's select key (the bits
- * of |psi|^2) for paces_batch_select_flags / paces_batch_select_topk_keys. */
+ * states) and terms (a host array of nterms device buffers, as paces_batch_taylor_finish takes
+ * them), psi first gets the series' finish in place, the work of paces_batch_taylor_finish
+ * (mixed: psi holds v and gets t_0 .. t_j added; FP64: psi holds res and gets terms[0] where j
+ * is even); keys (optional, R entries) receives each row's select key (the bits of |psi|^2) for
+ * paces_batch_select_flags / paces_batch_select_topk_keys. */
 int paces_holstein_observables_batched_finish(uint64_t const* rows, double* psi, int64_t R,
                                               const int32_t* seg_off, const paces_layout_t* lay,
-                                              const paces_holstein_t* h, const void* x_even,
+                                              const paces_holstein_t* h,
+                                              const void* const* terms, int32_t nterms,
                                               int32_t mixed, const paces_batch_taylor_t* ts,
                                               uint64_t* keys, double* out, void* ws,
                                               size_t ws_bytes, void* stream);
 size_t paces_batch_taylor_workspace_bytes(int64_t R, int32_t ntraj);
-/* res = v; x0 = v as complex64 (mixed != 0) or complex128; ts[t].c1 = ||v_t||_inf */
+/* res = v (not for mixed != 0: res is unused and may be null); x0 = v as complex64 (mixed) or
+ * complex128; ts[t].c1 = ||v_t||_inf */
 int paces_batch_taylor_begin(int64_t R, const double* v, double* res, void* x0, int32_t mixed,
                              const int32_t* seg_off, int32_t ntraj, paces_batch_taylor_t* ts,
                              void* ws, size_t ws_bytes, void* stream);
@@ -639,9 +642,11 @@ int paces_batch_taylor_begin_tuned(int64_t R, const double* v, double* res, void
                                    const paces_launch_cfg_t* cfg, void* stream);
 /* one term for the trajectories still running; exactly one of H64 (FP64 values, complex128
  * terms, double diag) and H32 (FP32 values, complex64 terms, float diag) is non-null.
- * res is stored on odd terms j only (an even term's sum is formed for the stopping test and
- * completed by the next term, with the same roundings); after the last term call
- * paces_batch_taylor_finish. Even terms must write their x_out to one buffer (x_even). */
+ * FP64: res is stored on odd terms j only (an even term's sum is formed for the stopping test
+ * and completed by the next term, with the same roundings); even terms must write their x_out
+ * to one buffer (x_even). Mixed (complex64 terms): no term reads or writes res (it may be
+ * null), each term j writes x_out to a buffer of its own (term j + 1 reads it as x_in), and the
+ * stopping test divides by ||v||_inf. After the last term call paces_batch_taylor_finish. */
 int paces_batch_taylor_term(int64_t R, const void* diag, const paces_csr_set_t* H64,
                             const paces_csr32_set_t* H32, const void* x_in, void* x_out,
                             double* res, double c, int32_t j, double tol, const int32_t* seg_off,
@@ -685,11 +690,14 @@ int paces_batch_taylor_term_slots(int64_t R, const void* diag, int32_t mixed,
                                   const int32_t* seg_off, int32_t ntraj, paces_batch_taylor_t* ts,
                                   paces_batch_taylor_hdr_t* hdr, void* ws, size_t ws_bytes,
                                   const paces_launch_cfg_t* cfg, void* stream);
-/* after the series (terms and/or a replayed paces_batch_series_launch): adds to res the last
- * term of every trajectory whose last term index ts[t].j_ind is even, from x_even (the buffer
- * every even term wrote); same launch shape and workspace as the series */
-int paces_batch_taylor_finish(int64_t R, const void* x_even, int32_t mixed, double* res,
-                              const int32_t* seg_off, int32_t ntraj,
+/* after the series (terms and/or a replayed paces_batch_series_launch), with terms a host
+ * array of nterms (1 .. PACES_SERIES_MAX_TERMS) device buffers. FP64: adds to res the last term
+ * of every trajectory whose last term index ts[t].j_ind is even, from terms[0] (the buffer every
+ * even term wrote). Mixed: res holds v and becomes v + t_0 + ... + t_j, j = ts[t].j_ind, added
+ * in that order (terms[k] is term k's buffer; nterms must cover every trajectory's last term).
+ * Same launch shape and workspace as the series. */
+int paces_batch_taylor_finish(int64_t R, const void* const* terms, int32_t nterms, int32_t mixed,
+                              double* res, const int32_t* seg_off, int32_t ntraj,
                               const paces_batch_taylor_t* ts, void* ws, size_t ws_bytes,
                               const paces_launch_cfg_t* cfg, void* stream);
 
@@ -697,8 +705,9 @@ int paces_batch_taylor_finish(int64_t R, const void* x_even, int32_t mixed, doub
  * counterpart). Every argument lives in a device-resident descriptor that the caller rewrites
  * before each step, so a HIP graph captured once from paces_batch_series_launch is replayed at
  * every step. Grids are sized for the capacity R_cap (>= the step's rows); workgroups past the
- * step's segments exit. The launch runs begin and terms 0 .. nterms-1 (term j reads x[2] for
- * j == 0, else x[(j-1) % 2], and writes x[j % 2], so x_even = x[0]); terms of stopped
+ * step's segments exit. The launch runs begin and terms 0 .. nterms-1 (FP64: term j reads x[2]
+ * for j == 0, else x[(j-1) % 2], and writes x[j % 2], so x_even = x[0]; mixed: term j reads x[2]
+ * for j == 0, else xt[j-1], and writes xt[j], and res is unused); terms of stopped
  * trajectories do nothing, and
  * a series not finished after nterms continues with paces_batch_taylor_term_tuned from term
  * nterms with the same state. Unlike paces_batch_taylor_begin, the launch does not copy v into
@@ -736,6 +745,7 @@ typedef struct {
   int32_t slot_off[PACES_MAX_TERMS + 1];
   int32_t slot_vib[PACES_MAX_TERMS];
   int32_t slot_stride;
+  void* xt[PACES_SERIES_MAX_TERMS];       /* mixed: term j's output buffer (complex64) */
 } paces_batch_series_t;
 size_t paces_batch_series_desc_bytes(void); /* sizeof(paces_batch_series_t), for bindings */
 int paces_batch_series_launch(const paces_batch_series_t* desc, int64_t R_cap, int32_t ntraj,

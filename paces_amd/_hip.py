@@ -129,7 +129,7 @@ class BatchSeries(C.Structure):
                 ("term_value", C.c_void_p), ("sqrt_table", C.c_void_p),
                 ("slots", C.c_void_p), ("nslots", C.c_int32), ("col_bits", C.c_int32),
                 ("slot_off", C.c_int32 * (MAX_TERMS + 1)), ("slot_vib", C.c_int32 * MAX_TERMS),
-                ("slot_stride", C.c_int32)]
+                ("slot_stride", C.c_int32), ("xt", C.c_void_p * SERIES_MAX_TERMS)]
 
 
 class TaylorState(C.Structure):
@@ -285,12 +285,12 @@ _SIGNATURES = {
     "paces_batch_select_topk_keys": (C.c_int, [_P, _P, _P, _I64, _I32, _P, _I32, _P, _P, _P, _P,
                                                _SZ, _P]),
     "paces_holstein_observables_batched_finish": (C.c_int, [_P, _P, _I64, _P, C.POINTER(Layout),
-                                                            C.POINTER(Holstein), _P, _I32, _P,
-                                                            _P, _P, _P, _SZ, _P]),
+                                                            C.POINTER(Holstein), _P, _I32, _I32,
+                                                            _P, _P, _P, _P, _SZ, _P]),
     "paces_batch_taylor_term_slots": (C.c_int, [_I64, _P, _I32, C.POINTER(Slots), _P, _P,
                                                 _P, _D, _I32, _D, _P, _I32, _P, _P, _P, _SZ,
                                                 C.POINTER(LaunchCfg), _P]),
-    "paces_batch_taylor_finish": (C.c_int, [_I64, _P, _I32, _P, _P, _I32, _P, _P, _SZ,
+    "paces_batch_taylor_finish": (C.c_int, [_I64, _P, _I32, _I32, _P, _P, _I32, _P, _P, _SZ,
                                             C.POINTER(LaunchCfg), _P]),
     "paces_rows_varying_bits": (C.c_int, [_P, _I64, _I32, _P, _P]),
     "paces_sort_rows_workspace_bytes": (_SZ, [_I64, _I32]),

@@ -539,17 +539,34 @@ constexpr int obs_tile() {
 // site's field offset and width are uniform across the wave; lane l holds rows l, l + 64, ...
 // with their weights and exciton positions in registers, and a site's sum ends in a wave
 // reduction. Only sites inside the tile's range of exciton positions can hold exciton weight.
-// With fin (the step's series state ts, its even-term buffer x_even in complex64 or complex128):
-// a trajectory whose series ended on an even term first gets that term added to its amplitudes
-// (psi updated in place: the finish of the series, as k_btay_finish does it). keys (optional)
-// receives each row's select key, the bits of |psi|^2 (k_bsel_keys' expression), for the next
-// select.
+// With fin (the step's series state ts and its term buffers): the finish of the series, as
+// k_btay_finish does it, with psi updated in place: mixed, psi (holding v) gets every term of
+// its trajectory added in order; FP64, a trajectory whose series ended on an even term gets that
+// term added. keys (optional) receives each row's select key, the bits of |psi|^2 (k_bsel_keys'
+// expression), for the next select.
+// The buffers of a series' terms, for its finish: FP64, t[0] is the even-term buffer; mixed,
+// t[k] is term k's buffer (n of them)
+struct SeriesTerms {
+  const void* t[PACES_SERIES_MAX_TERMS];
+  int n;
+};
+
 struct ObsFinish {
-  const void* x_even;
+  SeriesTerms terms;
   int mixed;
   const paces_batch_taylor_t* ts;
   u64* keys;
 };
+
+static bool series_terms(const void* const* terms, int32_t nterms, SeriesTerms* st) {
+  if (!terms || nterms < 1 || nterms > PACES_SERIES_MAX_TERMS) return false;
+  for (int k = 0; k < PACES_SERIES_MAX_TERMS; ++k) {
+    st->t[k] = k < nterms ? terms[k] : nullptr;
+    if (k < nterms && !terms[k]) return false;
+  }
+  st->n = nterms;
+  return true;
+}
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__ rows,
@@ -594,10 +611,11 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
     }
   }
   {
-    // the amplitudes, each row by one thread (rows t, t + kBlock, ...): a trajectory whose
-    // series ended on an even term gets that term added first (each row read and written once);
-    // the weights go through LDS to the lanes that hold their rows in the site loop below
-    const bool add = fin.ts && (fin.ts[T.t].j_ind & 1) == 0;
+    // the amplitudes, each row by one thread (rows t, t + kBlock, ...), with the series'
+    // finish first (each row read and written once); the weights go through LDS to the lanes
+    // that hold their rows in the site loop below
+    const int jt = fin.ts ? fin.ts[T.t].j_ind : 0;
+    const bool add = fin.ts && (fin.mixed || (jt & 1) == 0);
     double2 v[TPR];
     int64_t ii[TPR];
 #pragma unroll
@@ -607,22 +625,33 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
       v[q] = psi[ii[q]];
     }
     if (add) {
+      if (fin.mixed) {
+        // psi holds v: t_0, ..., t_jt added in order
+        const int nk = jt + 1 < fin.terms.n ? jt + 1 : fin.terms.n;
+#pragma unroll 4
+        for (int k = 0; k < nk; ++k) {
+          const float2* x = (const float2*)fin.terms.t[k];
+          float2 xv[TPR];
 #pragma unroll
-      for (int q = 0; q < TPR; ++q) {
-        double xr, xi;
-        if (fin.mixed) {
-          const float2 x = ((const float2*)fin.x_even)[ii[q]];
-          xr = x.x;
-          xi = x.y;
-        } else {
-          const double2 x = ((const double2*)fin.x_even)[ii[q]];
-          xr = x.x;
-          xi = x.y;
+          for (int q = 0; q < TPR; ++q) xv[q] = x[ii[q]];
+#pragma unroll
+          for (int q = 0; q < TPR; ++q) {
+            v[q].x = v[q].x + (double)xv[q].x;
+            v[q].y = v[q].y + (double)xv[q].y;
+          }
         }
-        v[q].x = v[q].x + xr;
-        v[q].y = v[q].y + xi;
-        if ((int)threadIdx.x + kBlock * q < nr) psi[ii[q]] = v[q];
+      } else {
+        const double2* x = (const double2*)fin.terms.t[0];
+#pragma unroll
+        for (int q = 0; q < TPR; ++q) {
+          const double2 xv = x[ii[q]];
+          v[q].x = v[q].x + xv.x;
+          v[q].y = v[q].y + xv.y;
+        }
       }
+#pragma unroll
+      for (int q = 0; q < TPR; ++q)
+        if ((int)threadIdx.x + kBlock * q < nr) psi[ii[q]] = v[q];
     }
     double a = 0.0;
 #pragma unroll
@@ -929,8 +958,8 @@ __device__ __forceinline__ void btay_begin_body(const double2* __restrict__ v,
 }
 
 // per-trajectory reduction of the partials; fin == 1: initialise c1, else the stopping test
-// use_prev: the term read no partial sum (a mixed-form even term): divide by the last exact
-// ||res||_inf instead of this term's
+// use_prev: the term formed no partial sum (a mixed-form term): divide by ||v||_inf (rn as the
+// begin set it) instead of this term's ||res||_inf
 __device__ __forceinline__ void btay_reduce_body(const u64* __restrict__ part,
                                                  const int64_t* __restrict__ boff,
                                                  paces_batch_taylor_t* __restrict__ ts,
@@ -1072,7 +1101,7 @@ __device__ __forceinline__ void btay_rows_par(const F* __restrict__ diag, const 
                                               double2* __restrict__ res,
                                               const double2* __restrict__ res_src, F c, bool odd,
                                               const SegTile& T, u64& mt, u64& mr, const VL& vl) {
-  if (sizeof(F) == 8 || odd)
+  if (sizeof(F) == 8)
     btay_rows_all<F, BS, RPT, true>(diag, H, xin, xout, res, res_src, c, odd, T, mt, mr, vl);
   else
     btay_rows_all<F, BS, RPT, false>(diag, H, xin, xout, res, res_src, c, odd, T, mt, mr, vl);
@@ -1171,10 +1200,13 @@ __device__ __forceinline__ void btay_rows_slots(const F* __restrict__ diag, cons
                                                    mt, mr, vt);
 }
 
-// The partial sum res is stored on odd terms only. On an even term j the sum res + t_j is formed
-// in registers for the stopping test; the next (odd) term adds its own input (t_j) and then its
-// output, (res + t_j) + t_(j+1), the same roundings in the same order as storing every term. A
-// trajectory whose series ends on an even term gets its last term added by k_btay_finish.
+// FP64 (complex128 terms): the partial sum res is stored on odd terms only. On an even term j
+// the sum res + t_j is formed in registers for the stopping test; the next (odd) term adds its
+// own input (t_j) and then its output, (res + t_j) + t_(j+1), the same roundings in the same
+// order as storing every term. A trajectory whose series ends on an even term gets its last term
+// added by the finish. Mixed (complex64 terms): no term reads or forms the partial sum; every
+// term writes a buffer of its own, the stopping test divides by ||v||_inf, and the finish forms
+// v + t_0 + ... + t_j in that order (the roundings of a running sum).
 template <typename F, int BS, int RPT, int NS>
 __device__ __forceinline__ void btay_term_body(
     const F* __restrict__ diag, const CsrSetB64& H, const typename Vec2<F>::T* __restrict__ xin,
@@ -1206,7 +1238,7 @@ __device__ __forceinline__ void btay_term_body(
     }
     __syncthreads();
     if constexpr (NS > 0) {
-      if (sizeof(F) == 8 || odd)
+      if (sizeof(F) == 8)
         btay_rows_slots<F, BS, RPT, NS, true>(diag, H, xin, xout, res, res_src, c, odd, T, mt, mr,
                                               vt);
       else
@@ -1232,7 +1264,7 @@ __global__ void __launch_bounds__(BS) k_btay_begin(const double2* __restrict__ v
                                                    const int32_t* __restrict__ seg_off,
                                                    const int64_t* __restrict__ boff, int ntraj,
                                                    u64* __restrict__ part) {
-  btay_begin_body<F, BS, RPT>(v, res, x0, seg_off, boff, ntraj, part);
+  btay_begin_body<F, BS, RPT, sizeof(F) == 8>(v, res, x0, seg_off, boff, ntraj, part);
 }
 
 __global__ void __launch_bounds__(64) k_btay_reduce(const u64* __restrict__ part,
@@ -1254,27 +1286,43 @@ __global__ void __launch_bounds__(BS) k_btay_term(
                                  hdr, part);
 }
 
-// After the series: a trajectory whose last term j is even has res one term behind; its last
-// term is still in x_even (even terms write there, and later terms skip stopped trajectories).
+// After the series. FP64: a trajectory whose last term j is even has res one term behind; its
+// last term is still in terms.t[0] (the even-term buffer: later terms skip stopped
+// trajectories). Mixed: res holds v, and gets t_0, ..., t_j added in order (terms.t[k] is term
+// k's buffer).
 template <typename F, int BS, int RPT>
-__global__ void __launch_bounds__(BS) k_btay_finish(const typename Vec2<F>::T* __restrict__ x_even,
-                                                    double2* __restrict__ res,
+__global__ void __launch_bounds__(BS) k_btay_finish(SeriesTerms terms, double2* __restrict__ res,
                                                     const int32_t* __restrict__ seg_off,
                                                     const int64_t* __restrict__ boff, int ntraj,
                                                     const paces_batch_taylor_t* __restrict__ ts) {
+  typedef typename Vec2<F>::T V;
   const SegTile T = seg_tile(seg_off, boff, ntraj, BS * RPT);
-  if (T.t < 0 || (ts[T.t].j_ind & 1) != 0) return;
+  if (T.t < 0) return;
+  const int jt = ts[T.t].j_ind;
+  if (sizeof(F) == 8 && (jt & 1) != 0) return;
+  const int nk = sizeof(F) == 8 ? 1 : (jt + 1 < terms.n ? jt + 1 : terms.n);
+  int64_t ii[RPT];
+  double2 r[RPT];
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
     const int64_t i = T.r0 + q * BS + threadIdx.x;
-    if (i < T.r1) {
-      const typename Vec2<F>::T x = x_even[i];
-      double2 r = res[i];
-      r.x = r.x + (double)x.x;
-      r.y = r.y + (double)x.y;
-      res[i] = r;
+    ii[q] = i < T.r1 ? i : T.r0;
+    r[q] = res[ii[q]];
+  }
+  for (int k = 0; k < nk; ++k) {
+    const V* x = (const V*)terms.t[k];
+    V xv[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) xv[q] = x[ii[q]];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      r[q].x = r[q].x + (double)xv[q].x;
+      r[q].y = r[q].y + (double)xv[q].y;
     }
   }
+#pragma unroll
+  for (int q = 0; q < RPT; ++q)
+    if (T.r0 + q * BS + threadIdx.x < T.r1) res[ii[q]] = r[q];
 }
 
 // ---- the series from a device descriptor (N1: captured once into a HIP graph, replayed every
@@ -1311,8 +1359,10 @@ struct SeriesDev {
   int32_t slot_off[PACES_MAX_TERMS + 1];
   int32_t slot_vib[PACES_MAX_TERMS];
   int32_t slot_stride;
+  PACES_GPTR(void) xt[PACES_SERIES_MAX_TERMS];
 };
 static_assert(sizeof(SeriesDev) == sizeof(paces_batch_series_t), "descriptor layout");
+static_assert(offsetof(SeriesDev, xt) == offsetof(paces_batch_series_t, xt), "descriptor layout");
 static_assert(offsetof(SeriesDev, slots) == offsetof(paces_batch_series_t, slots),
               "descriptor layout");
 static_assert(offsetof(SeriesDev, slot_vib) == offsetof(paces_batch_series_t, slot_vib),
@@ -1395,8 +1445,11 @@ __global__ void __launch_bounds__(BS) k_bser_term(const paces_batch_series_t* __
     if (t >= 1 && t < H.nterms) H.slot_begin |= 1 << d->slot_off[t];
   }
   typedef typename Vec2<F>::T V;
-  const V* xin = gen((PACES_GPTR(const V))(j == 0 ? d->x[2] : d->x[(j - 1) & 1]));
-  V* xout = gen((PACES_GPTR(V))d->x[j & 1]);
+  // mixed: term j writes xt[j] and reads xt[j - 1] (x[2] for term 0); FP64: x[j % 2] and
+  // x[(j - 1) % 2]
+  const V* xin = gen((PACES_GPTR(const V))(
+      j == 0 ? d->x[2] : (sizeof(F) == 4 ? d->xt[j - 1] : d->x[(j - 1) & 1])));
+  V* xout = gen((PACES_GPTR(V))(sizeof(F) == 4 ? d->xt[j] : d->x[j & 1]));
   // the begin kernel left res unwritten: until term 1 has stored it, the partial sum is v
   // (term 0 reads it for the FP64 stopping test, term 1 adds to it); with a tolerance below 1
   // no series stops at term 0 ((||v|| + ||t_0||) / ||v + t_0|| is about 1), so the finish pass
@@ -1598,23 +1651,28 @@ int paces_holstein_observables_batched(const uint64_t* rows, const double* psi, 
   if (lay->nfields != h->nchain + 2) return PACES_E_INVALID;
   if (!h->d_traj_L || h->geo_dim < 1 || h->geo_dim > 3) return PACES_E_INVALID;
   if (ws_bytes < bobs_ws_bytes(R, h->ntraj, h->nchain)) return PACES_E_WORKSPACE;
-  const ObsFinish fin = {nullptr, 0, nullptr, nullptr};
+  ObsFinish fin = {};
   PACES_DISPATCH_W(lay->W, run_bobs, rows, const_cast<double*>(psi), R, seg_off, h->ntraj, lay,
                    h->nchain, h->d_traj_L, h->geo_dim, out, ws, (hipStream_t)stream, fin);
 }
 
 int paces_holstein_observables_batched_finish(uint64_t const* rows, double* psi, int64_t R,
                                               const int32_t* seg_off, const paces_layout_t* lay,
-                                              const paces_holstein_t* h, const void* x_even,
+                                              const paces_holstein_t* h,
+                                              const void* const* terms, int32_t nterms,
                                               int32_t mixed, const paces_batch_taylor_t* ts,
                                               uint64_t* keys, double* out, void* ws,
                                               size_t ws_bytes, void* stream) {
   if (!lay || !h || !h->batched || !seg_off || !out || !ws || R < 0) return PACES_E_INVALID;
-  if (R > 0 && (!rows || !psi || (ts && !x_even))) return PACES_E_INVALID;
+  if (R > 0 && (!rows || !psi)) return PACES_E_INVALID;
   if (lay->nfields != h->nchain + 2) return PACES_E_INVALID;
   if (!h->d_traj_L || h->geo_dim < 1 || h->geo_dim > 3) return PACES_E_INVALID;
   if (ws_bytes < bobs_ws_bytes(R, h->ntraj, h->nchain)) return PACES_E_WORKSPACE;
-  const ObsFinish fin = {x_even, mixed ? 1 : 0, ts, (u64*)keys};
+  ObsFinish fin = {};
+  if (ts && R > 0 && !series_terms(terms, nterms, &fin.terms)) return PACES_E_INVALID;
+  fin.mixed = mixed ? 1 : 0;
+  fin.ts = ts;
+  fin.keys = (u64*)keys;
   PACES_DISPATCH_W(lay->W, run_bobs, rows, psi, R, seg_off, h->ntraj, lay, h->nchain,
                    h->d_traj_L, h->geo_dim, out, ws, (hipStream_t)stream, fin);
 }
@@ -1635,7 +1693,7 @@ int paces_batch_taylor_begin_tuned(int64_t R, const double* v, double* res, void
                                    paces_batch_taylor_t* ts, void* ws, size_t ws_bytes,
                                    const paces_launch_cfg_t* cfg, void* stream) {
   if (R < 0 || ntraj < 1 || !seg_off || !ts || !ws || !cfg_ok(cfg)) return PACES_E_INVALID;
-  if (R > 0 && (!v || !res || !x0)) return PACES_E_INVALID;
+  if (R > 0 && (!v || (!mixed && !res) || !x0)) return PACES_E_INVALID;  // mixed: res unused
   if (ws_bytes < btay_ws_bytes(R, ntraj)) return PACES_E_WORKSPACE;
   hipStream_t st = (hipStream_t)stream;
   char* p = (char*)ws;
@@ -1703,7 +1761,7 @@ int paces_batch_series_launch(const paces_batch_series_t* desc, int64_t R_cap, i
 #undef ST_LAUNCH
 #undef ST_LAUNCH_NS
     hipLaunchKernelGGL(k_bser_reduce, dim3(ntraj), dim3(64), 0, st, desc, ntraj, 0, j,
-                       (mixed && !(j & 1)) ? 1 : 0);
+                       mixed ? 1 : 0);
   }
   return launch_status();
 }
@@ -1764,7 +1822,7 @@ static int btay_term_launch(int64_t R, const void* diag, int mixed, const CsrSet
                             size_t ws_bytes, const paces_launch_cfg_t* cfg, void* stream) {
   if (R < 0 || ntraj < 1 || !seg_off || !ts || !hdr || !ws || !cfg_ok(cfg))
     return PACES_E_INVALID;
-  if (R > 0 && (!diag || !x_in || !x_out || !res)) return PACES_E_INVALID;
+  if (R > 0 && (!diag || !x_in || !x_out || (!mixed && !res))) return PACES_E_INVALID;
   if (ws_bytes < btay_ws_bytes(R, ntraj)) return PACES_E_WORKSPACE;
   hipStream_t st = (hipStream_t)stream;
   char* p = (char*)ws;
@@ -1798,18 +1856,19 @@ static int btay_term_launch(int64_t R, const void* diag, int mixed, const CsrSet
 #undef BT_LAUNCH
 #undef BT_LAUNCH_NS
   hipLaunchKernelGGL(k_btay_reduce, dim3(ntraj), dim3(64), 0, st, (const u64*)part, boff, ts, hdr,
-                     0, (int)j, tol, (mixed && !(j & 1)) ? 1 : 0);
+                     0, (int)j, tol, mixed ? 1 : 0);
   return launch_status();
 }
 
 extern "C" {
 
-int paces_batch_taylor_finish(int64_t R, const void* x_even, int32_t mixed, double* res,
-                              const int32_t* seg_off, int32_t ntraj,
+int paces_batch_taylor_finish(int64_t R, const void* const* terms, int32_t nterms, int32_t mixed,
+                              double* res, const int32_t* seg_off, int32_t ntraj,
                               const paces_batch_taylor_t* ts, void* ws, size_t ws_bytes,
                               const paces_launch_cfg_t* cfg, void* stream) {
   if (R < 0 || ntraj < 1 || !seg_off || !ts || !ws || !cfg_ok(cfg)) return PACES_E_INVALID;
-  if (R > 0 && (!x_even || !res)) return PACES_E_INVALID;
+  SeriesTerms tt = {};
+  if (R > 0 && (!res || !series_terms(terms, nterms, &tt))) return PACES_E_INVALID;
   if (ws_bytes < btay_ws_bytes(R, ntraj)) return PACES_E_WORKSPACE;
   hipStream_t st = (hipStream_t)stream;
   const int64_t* boff = (const int64_t*)ws;  // the series' tile offsets (same shape)
@@ -1818,11 +1877,11 @@ int paces_batch_taylor_finish(int64_t R, const void* x_even, int32_t mixed, doub
   const unsigned gs = seg_grid(R, ntraj, B * RR);
 #define BF_LAUNCH(BB, QQ)                                                                      \
   if (mixed)                                                                                   \
-    hipLaunchKernelGGL((k_btay_finish<float, BB, QQ>), dim3(gs), dim3(BB), 0, st,              \
-                       (const float2*)x_even, (double2*)res, seg_off, boff, ntraj, ts);        \
+    hipLaunchKernelGGL((k_btay_finish<float, BB, QQ>), dim3(gs), dim3(BB), 0, st, tt,          \
+                       (double2*)res, seg_off, boff, ntraj, ts);                               \
   else                                                                                         \
-    hipLaunchKernelGGL((k_btay_finish<double, BB, QQ>), dim3(gs), dim3(BB), 0, st,             \
-                       (const double2*)x_even, (double2*)res, seg_off, boff, ntraj, ts)
+    hipLaunchKernelGGL((k_btay_finish<double, BB, QQ>), dim3(gs), dim3(BB), 0, st, tt,         \
+                       (double2*)res, seg_off, boff, ntraj, ts)
   PACES_CFG_SWITCH(B, RR, BF_LAUNCH)
 #undef BF_LAUNCH
   return launch_status();

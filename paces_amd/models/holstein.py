@@ -1175,22 +1175,23 @@ class BatchedHamiltonian(Hamiltonian):
                                        keys=None):
         """Launches the per-trajectory observables into out (device float64, ntraj * (2N + 1):
         per trajectory n_exc[N], n_pho[N], sum|psi|^2) and returns out; no host sync. finish =
-        (x_even, series states, mixed): the series' last even terms are first added to vector
-        in place (paces_holstein_observables_batched_finish); keys (int64, R) receives the
-        rows' select keys."""
+        (term buffers, series states, mixed): the series' finish is first done on vector in
+        place (paces_holstein_observables_batched_finish: mixed, every term added to v; FP64,
+        the last even terms added); keys (int64, R) receives the rows' select keys."""
         R = rows.shape[0]
         N = self.nchain
         if out is None:
             out = torch.empty(self.ntraj * (2 * N + 1), dtype=torch.float64, device=self.device)
         wsp, wsn = self._ws.get(
             _lib.paces_holstein_observables_batched_workspace_bytes(R, self.ntraj, N))
-        x_even, ts, mixed = finish if finish is not None else (None, None, 0)
+        terms, ts, mixed = finish if finish is not None else ([], None, 0)
+        tp = (C.c_void_p * max(len(terms), 1))(*[ptr(torch.view_as_real(t)) for t in terms])
         check(_lib.paces_holstein_observables_batched_finish(
             ptr(rows) if R else None, ptr(torch.view_as_real(vector)) if R else None, R,
-            ptr(seg_off), C.byref(self._layout), C.byref(self._hs),
-            ptr(torch.view_as_real(x_even)) if x_even is not None else None, 1 if mixed else 0,
-            ptr(ts) if ts is not None else None, ptr(keys) if keys is not None and R else None,
-            ptr(out), wsp, wsn, stream()), "holstein_observables_batched_finish")
+            ptr(seg_off), C.byref(self._layout), C.byref(self._hs), tp, len(terms),
+            1 if mixed else 0, ptr(ts) if ts is not None else None,
+            ptr(keys) if keys is not None and R else None, ptr(out), wsp, wsn, stream()),
+            "holstein_observables_batched_finish")
         return out
 
     def split_observables(self, flat):
@@ -1688,9 +1689,10 @@ class BatchedTimeEvolution:
             d.coded = 0
         return diag, keep, call, fill
 
-    def _propagate_graph(self, delta_t, fill, mixed, res, bufs, ts, hdr, diag, cfg, st):
+    def _propagate_graph(self, delta_t, fill, mixed, res, bufs, tbuf, ts, hdr, diag, cfg, st):
         """Terms 0 .. n-1 by replaying a captured graph; returns n (the eager loop continues
-        from there if some trajectory has not stopped)."""
+        from there if some trajectory has not stopped). bufs: x[0..2] of the descriptor; tbuf(k):
+        the mixed form's buffer of term k."""
         h = self.hamobj
         R = self.numstates
         # capacity: R rounded up in sixteenths of its octave (at most 6.25% spare rows)
@@ -1710,7 +1712,11 @@ class BatchedTimeEvolution:
         d.v = ptr(torch.view_as_real(self.vector))
         d.res = ptr(torch.view_as_real(res))
         for k in range(3):
-            d.x[k] = ptr(torch.view_as_real(bufs[k]))
+            if bufs[k] is not None:
+                d.x[k] = ptr(torch.view_as_real(bufs[k]))
+        if mixed:
+            for k in range(nterms):
+                d.xt[k] = ptr(torch.view_as_real(tbuf(k)))
         d.seg_off = ptr(self.seg_off)
         d.ts = ptr(ts)
         d.hdr = ptr(hdr)
@@ -1739,13 +1745,31 @@ class BatchedTimeEvolution:
         return nterms
 
     def propagate(self, delta_t, fuse_finish=False):
+        """The Taylor series of every trajectory. FP64: complex128 terms in two alternating
+        buffers, the partial sum stored on odd terms. Mixed: complex64 terms, each in a buffer of
+        its own; no term forms the partial sum, and the finish adds t_0 .. t_j to v in place (at
+        most SERIES_MAX_TERMS terms). With fuse_finish the step's observables pass does the
+        finish (step() launches that pass next and nothing reads the amplitudes in between)."""
         h = self.hamobj
         R = self.numstates
         mats = [self.sparse_mats_dict[t] for t in h.offdiag_terms]
         mixed = self.precision == "mixed"
         vdt = torch.complex64 if mixed else torch.complex128
-        res = torch.empty_like(self.vector)
-        bufs = [torch.empty(R, dtype=vdt, device=self.device) for _ in range(3)]
+        tb = []
+
+        def tbuf(k):
+            while len(tb) <= k:
+                tb.append(torch.empty(R, dtype=vdt, device=self.device))
+            return tb[k]
+
+        if mixed:
+            res = self.vector  # the finish forms v + t_0 + ... + t_j in place
+            bufs = [None, None, torch.empty(R, dtype=vdt, device=self.device)]
+            m_star = min(self.m_star, _hip.SERIES_MAX_TERMS)
+        else:
+            res = torch.empty_like(self.vector)
+            bufs = [torch.empty(R, dtype=vdt, device=self.device) for _ in range(3)]
+            m_star = self.m_star
         ts = torch.zeros(h.ntraj * C.sizeof(_hip.BatchTaylor), dtype=torch.uint8,
                          device=self.device)
         hdr = torch.zeros(16, dtype=torch.uint8, device=self.device)
@@ -1757,40 +1781,43 @@ class BatchedTimeEvolution:
         cref = C.byref(cfg) if cfg is not None else None
         j = 0
         if self.graph:
-            j = self._propagate_graph(delta_t, fill, mixed, res, bufs, ts, hdr, diag, cfg, st)
+            j = self._propagate_graph(delta_t, fill, mixed, res, bufs, tbuf, ts, hdr, diag, cfg,
+                                      st)
             wsp, wsn = self._tws.get(0)  # the graph's workspace (same buffer, grown if needed)
-            x_in = bufs[(j - 1) % 2]
+            x_in = tb[j - 1] if mixed else bufs[(j - 1) % 2]
         else:
             check(_lib.paces_batch_taylor_begin_tuned(R, ptr(torch.view_as_real(self.vector)),
+                                                      None if mixed else
                                                       ptr(torch.view_as_real(res)),
                                                       ptr(torch.view_as_real(bufs[2])),
                                                       1 if mixed else 0, ptr(self.seg_off),
                                                       h.ntraj, ptr(ts), wsp, wsn, cref, st),
                   "batch_taylor_begin")
             x_in = bufs[2]
-        while j < self.m_star:
+        while j < m_star:
             if self.graph and int(hdr[:4].view(torch.int32).item()) == 0:
                 break
             for _ in range(4):
-                if j >= self.m_star:
+                if j >= m_star:
                     break
-                x_out = bufs[j % 2]
+                x_out = tbuf(j) if mixed else bufs[j % 2]
                 term(R, ptr(torch.view_as_real(x_in)), ptr(torch.view_as_real(x_out)),
-                     ptr(torch.view_as_real(res)), engine.taylor_coeff(delta_t, j), j, self.tol,
-                     ptr(ts), ptr(hdr), wsp, wsn, cref, st)
+                     None if mixed else ptr(torch.view_as_real(res)),
+                     engine.taylor_coeff(delta_t, j), j, self.tol, ptr(ts), ptr(hdr), wsp, wsn,
+                     cref, st)
                 x_in = x_out
                 j += 1
             if int(hdr[:4].view(torch.int32).item()) == 0:
                 break
-        # trajectories whose series ended on an even term get that term added (the sum is
-        # stored on odd terms); even terms wrote bufs[0]. With fuse_finish the step's
-        # observables pass adds it (step() launches that pass next and nothing reads the
-        # amplitudes in between).
+        # the finish: mixed, v + t_0 + ... + t_j (the j terms run); FP64, the last term of the
+        # trajectories whose series ended on an even term (even terms wrote bufs[0])
+        terms = tb[:j] if mixed else [bufs[0]]
         if fuse_finish:
-            self._finish = (bufs[0], ts, mixed)
+            self._finish = (terms, ts, mixed)
         else:
-            check(_lib.paces_batch_taylor_finish(R, ptr(torch.view_as_real(bufs[0])),
-                                                 1 if mixed else 0, ptr(torch.view_as_real(res)),
+            tp = (C.c_void_p * len(terms))(*[ptr(torch.view_as_real(t)) for t in terms])
+            check(_lib.paces_batch_taylor_finish(R, tp, len(terms), 1 if mixed else 0,
+                                                 ptr(torch.view_as_real(res)),
                                                  ptr(self.seg_off), h.ntraj, ptr(ts), wsp, wsn,
                                                  cref, st), "batch_taylor_finish")
         del keep
