@@ -55,12 +55,35 @@ struct SegTile {
   int t;        // trajectory, -1 if the workgroup is past the last tile
   int64_t r0;   // first row of the tile
   int64_t r1;   // one past the last row
+  int64_t b;    // the tile's index (the workgroup's, unless remapped)
 };
 
+// kXcdChunk > 0: workgroup b runs tile (k / C) * 8C + (b % 8) * C + k % C, k = b / 8, C =
+// kXcdChunk: each XCD (workgroup b runs on XCD b % 8) takes runs of C consecutive tiles, so that
+// tiles whose rows neighbour each other share an L2 (the grid is padded to a multiple of 8C)
+#ifndef PACES_XCD_CHUNK
+#define PACES_XCD_CHUNK 64
+#endif
+constexpr int kXcdChunk = PACES_XCD_CHUNK;
+
+__device__ __forceinline__ int64_t xcd_tile(int64_t b) {
+  if (kXcdChunk == 0) return b;
+  const int64_t k = b >> 3;
+  return (k / kXcdChunk) * (8 * kXcdChunk) + (b & 7) * kXcdChunk + k % kXcdChunk;
+}
+
+static unsigned xcd_grid(unsigned g) {
+  if (kXcdChunk == 0) return g;
+  const unsigned q = 8u * kXcdChunk;
+  return (g + q - 1) / q * q;
+}
+
+template <bool XCD = false>
 __device__ __forceinline__ SegTile seg_tile(const int32_t* __restrict__ seg_off,
                                             const int64_t* __restrict__ boff, int ntraj, int tile) {
   SegTile st;
-  const int64_t b = blockIdx.x;
+  const int64_t b = XCD ? xcd_tile(blockIdx.x) : (int64_t)blockIdx.x;
+  st.b = b;
   if (b >= boff[ntraj]) {
     st.t = -1;
     st.r0 = st.r1 = 0;
@@ -905,7 +928,7 @@ __device__ __forceinline__ typename Vec2<F>::T hx_row_t(int64_t i, const F* __re
 }
 
 template <int BS>
-__device__ __forceinline__ void blk_max2(u64 a, u64 b, u64* __restrict__ part) {
+__device__ __forceinline__ void blk_max2(u64 a, u64 b, u64* __restrict__ part, int64_t tile) {
   __shared__ u64 sa[BS / 64], sb[BS / 64];
   a = wave_max_u64(a);
   b = wave_max_u64(b);
@@ -921,8 +944,8 @@ __device__ __forceinline__ void blk_max2(u64 a, u64 b, u64* __restrict__ part) {
       ma = sa[q] > ma ? sa[q] : ma;
       mb = sb[q] > mb ? sb[q] : mb;
     }
-    part[2 * blockIdx.x] = ma;
-    part[2 * blockIdx.x + 1] = mb;
+    part[2 * tile] = ma;
+    part[2 * tile + 1] = mb;
   }
 }
 
@@ -954,7 +977,7 @@ __device__ __forceinline__ void btay_begin_body(const double2* __restrict__ v,
       m = w > m ? w : m;
     }
   }
-  blk_max2<BS>(m, 0, part);
+  blk_max2<BS>(m, 0, part, T.b);
 }
 
 // per-trajectory reduction of the partials; fin == 1: initialise c1, else the stopping test
@@ -1216,7 +1239,7 @@ __device__ __forceinline__ void btay_term_body(
     const paces_batch_taylor_t* __restrict__ ts, paces_batch_taylor_hdr_t* __restrict__ hdr,
     u64* __restrict__ part) {
   if (blockIdx.x == 0 && threadIdx.x == 0) hdr->notdone = 0;
-  const SegTile T = seg_tile(seg_off, boff, ntraj, BS * RPT);
+  const SegTile T = seg_tile<true>(seg_off, boff, ntraj, BS * RPT);
   if (T.t < 0 || ts[T.t].done) return;
   const bool odd = (j & 1) != 0;
   u64 mt = 0, mr = 0;
@@ -1252,7 +1275,7 @@ __device__ __forceinline__ void btay_term_body(
     btay_rows_par<F, BS, RPT>(diag, H, xin, xout, res, res_src, c, odd, T, mt, mr,
                               PlainVals<F>{});
   }
-  blk_max2<BS>(mt, mr, part);
+  blk_max2<BS>(mt, mr, part, T.b);
 }
 
 // Launch shape (tuned): BS threads per workgroup, RPT rows per thread (t, t + BS, ...), a
@@ -1732,6 +1755,7 @@ int paces_batch_series_launch(const paces_batch_series_t* desc, int64_t R_cap, i
   int B, RR;
   cfg_get(cfg, &B, &RR);
   const unsigned gs = seg_grid(R_cap, ntraj, B * RR);
+  const unsigned gst = xcd_grid(gs);  // the terms' grid
   hipLaunchKernelGGL(k_bser_segblocks, dim3(1), dim3(1), 0, st, desc, ntraj, B * RR);
 #define SB_LAUNCH(BB, QQ)                                                                      \
   if (mixed)                                                                                   \
@@ -1744,10 +1768,10 @@ int paces_batch_series_launch(const paces_batch_series_t* desc, int64_t R_cap, i
   for (int j = 0; j < nterms; ++j) {
 #define ST_LAUNCH_NS(BB, QQ, NS)                                                               \
   if (mixed)                                                                                   \
-    hipLaunchKernelGGL((k_bser_term<float, BB, QQ, NS>), dim3(gs), dim3(BB), 0, st, desc,      \
+    hipLaunchKernelGGL((k_bser_term<float, BB, QQ, NS>), dim3(gst), dim3(BB), 0, st, desc,     \
                        ntraj, j);                                                              \
   else                                                                                         \
-    hipLaunchKernelGGL((k_bser_term<double, BB, QQ, NS>), dim3(gs), dim3(BB), 0, st, desc,     \
+    hipLaunchKernelGGL((k_bser_term<double, BB, QQ, NS>), dim3(gst), dim3(BB), 0, st, desc,    \
                        ntraj, j)
 #define ST_LAUNCH(BB, QQ)          \
   if (nslots == 4) {               \
@@ -1831,7 +1855,7 @@ static int btay_term_launch(int64_t R, const void* diag, int mixed, const CsrSet
   u64* part = (u64*)p;
   int B, RR;
   cfg_get(cfg, &B, &RR);
-  const unsigned gs = seg_grid(R, ntraj, B * RR);
+  const unsigned gs = xcd_grid(seg_grid(R, ntraj, B * RR));
   const int nslots = H.slots ? H.nslots : 0;
 #define BT_LAUNCH_NS(BB, QQ, NS)                                                               \
   if (mixed)                                                                                   \
