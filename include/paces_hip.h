@@ -618,16 +618,19 @@ int paces_holstein_observables_batched(const uint64_t* rows, const double* psi, 
 /* The same at the end of a series (no reference counterpart): with ts (the series' per-trajectory
  * states) and terms (a host array of nterms device buffers, as paces_batch_taylor_finish takes
  * them), psi first gets the series' finish in place, the work of paces_batch_taylor_finish
- * (mixed: psi holds v and gets t_0 .. t_j added; FP64: psi holds res and gets terms[0] where j
- * is even); keys (optional, R entries) receives each row's select key (the bits of |psi|^2) for
+ * (mixed: psi holds v + t_0 + ... + t_(k0-1) and gets t_k0 .. t_j added, j = ts[t].j_ind, so a
+ * finish launched while some series still run is completed by a second call with k0 = the terms
+ * the first one saw; FP64 (k0 = 0): psi holds res and gets terms[0] where j is even); keys
+ * (optional, R entries) receives each row's select key (the bits of |psi|^2) for
  * paces_batch_select_flags / paces_batch_select_topk_keys. */
 int paces_holstein_observables_batched_finish(uint64_t const* rows, double* psi, int64_t R,
                                               const int32_t* seg_off, const paces_layout_t* lay,
                                               const paces_holstein_t* h,
                                               const void* const* terms, int32_t nterms,
-                                              int32_t mixed, const paces_batch_taylor_t* ts,
-                                              uint64_t* keys, double* out, void* ws,
-                                              size_t ws_bytes, void* stream);
+                                              int32_t k0, int32_t mixed,
+                                              const paces_batch_taylor_t* ts, uint64_t* keys,
+                                              double* out, void* ws, size_t ws_bytes,
+                                              void* stream);
 size_t paces_batch_taylor_workspace_bytes(int64_t R, int32_t ntraj);
 /* res = v (not for mixed != 0: res is unused and may be null); x0 = v as complex64 (mixed) or
  * complex128; ts[t].c1 = ||v_t||_inf */

@@ -286,7 +286,7 @@ _SIGNATURES = {
                                                _SZ, _P]),
     "paces_holstein_observables_batched_finish": (C.c_int, [_P, _P, _I64, _P, C.POINTER(Layout),
                                                             C.POINTER(Holstein), _P, _I32, _I32,
-                                                            _P, _P, _P, _P, _SZ, _P]),
+                                                            _I32, _P, _P, _P, _P, _SZ, _P]),
     "paces_batch_taylor_term_slots": (C.c_int, [_I64, _P, _I32, C.POINTER(Slots), _P, _P,
                                                 _P, _D, _I32, _D, _P, _I32, _P, _P, _P, _SZ,
                                                 C.POINTER(LaunchCfg), _P]),

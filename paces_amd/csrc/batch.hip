@@ -576,6 +576,7 @@ struct SeriesTerms {
 
 struct ObsFinish {
   SeriesTerms terms;
+  int k0;  // mixed: the first term to add (a later k0 completes an earlier finish of psi)
   int mixed;
   const paces_batch_taylor_t* ts;
   u64* keys;
@@ -638,7 +639,7 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
     // finish first (each row read and written once); the weights go through LDS to the lanes
     // that hold their rows in the site loop below
     const int jt = fin.ts ? fin.ts[T.t].j_ind : 0;
-    const bool add = fin.ts && (fin.mixed || (jt & 1) == 0);
+    const bool add = fin.ts && (fin.mixed ? jt >= fin.k0 : (jt & 1) == 0);
     double2 v[TPR];
     int64_t ii[TPR];
 #pragma unroll
@@ -649,10 +650,10 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
     }
     if (add) {
       if (fin.mixed) {
-        // psi holds v: t_0, ..., t_jt added in order
+        // psi holds v + t_0 + ... + t_(k0-1): t_k0, ..., t_jt added in order
         const int nk = jt + 1 < fin.terms.n ? jt + 1 : fin.terms.n;
 #pragma unroll 4
-        for (int k = 0; k < nk; ++k) {
+        for (int k = fin.k0; k < nk; ++k) {
           const float2* x = (const float2*)fin.terms.t[k];
           float2 xv[TPR];
 #pragma unroll
@@ -1683,16 +1684,19 @@ int paces_holstein_observables_batched_finish(uint64_t const* rows, double* psi,
                                               const int32_t* seg_off, const paces_layout_t* lay,
                                               const paces_holstein_t* h,
                                               const void* const* terms, int32_t nterms,
-                                              int32_t mixed, const paces_batch_taylor_t* ts,
-                                              uint64_t* keys, double* out, void* ws,
-                                              size_t ws_bytes, void* stream) {
+                                              int32_t k0, int32_t mixed,
+                                              const paces_batch_taylor_t* ts, uint64_t* keys,
+                                              double* out, void* ws, size_t ws_bytes,
+                                              void* stream) {
   if (!lay || !h || !h->batched || !seg_off || !out || !ws || R < 0) return PACES_E_INVALID;
+  if (k0 < 0 || (k0 > 0 && !mixed)) return PACES_E_INVALID;
   if (R > 0 && (!rows || !psi)) return PACES_E_INVALID;
   if (lay->nfields != h->nchain + 2) return PACES_E_INVALID;
   if (!h->d_traj_L || h->geo_dim < 1 || h->geo_dim > 3) return PACES_E_INVALID;
   if (ws_bytes < bobs_ws_bytes(R, h->ntraj, h->nchain)) return PACES_E_WORKSPACE;
   ObsFinish fin = {};
   if (ts && R > 0 && !series_terms(terms, nterms, &fin.terms)) return PACES_E_INVALID;
+  fin.k0 = k0;
   fin.mixed = mixed ? 1 : 0;
   fin.ts = ts;
   fin.keys = (u64*)keys;

@@ -1175,8 +1175,8 @@ class BatchedHamiltonian(Hamiltonian):
                                        keys=None):
         """Launches the per-trajectory observables into out (device float64, ntraj * (2N + 1):
         per trajectory n_exc[N], n_pho[N], sum|psi|^2) and returns out; no host sync. finish =
-        (term buffers, series states, mixed): the series' finish is first done on vector in
-        place (paces_holstein_observables_batched_finish: mixed, every term added to v; FP64,
+        (term buffers, series states, mixed, k0): the series' finish is first done on vector in
+        place (paces_holstein_observables_batched_finish: mixed, terms k0 .. j added to v; FP64,
         the last even terms added); keys (int64, R) receives the rows' select keys."""
         R = rows.shape[0]
         N = self.nchain
@@ -1184,11 +1184,11 @@ class BatchedHamiltonian(Hamiltonian):
             out = torch.empty(self.ntraj * (2 * N + 1), dtype=torch.float64, device=self.device)
         wsp, wsn = self._ws.get(
             _lib.paces_holstein_observables_batched_workspace_bytes(R, self.ntraj, N))
-        terms, ts, mixed = finish if finish is not None else ([], None, 0)
+        terms, ts, mixed, k0 = finish if finish is not None else ([], None, 0, 0)
         tp = (C.c_void_p * max(len(terms), 1))(*[ptr(torch.view_as_real(t)) for t in terms])
         check(_lib.paces_holstein_observables_batched_finish(
             ptr(rows) if R else None, ptr(torch.view_as_real(vector)) if R else None, R,
-            ptr(seg_off), C.byref(self._layout), C.byref(self._hs), tp, len(terms),
+            ptr(seg_off), C.byref(self._layout), C.byref(self._hs), tp, len(terms), k0,
             1 if mixed else 0, ptr(ts) if ts is not None else None,
             ptr(keys) if keys is not None and R else None, ptr(out), wsp, wsn, stream()),
             "holstein_observables_batched_finish")
@@ -1302,6 +1302,9 @@ class BatchedTimeEvolution:
         self._finish = None
         self._keys = None
         self._graph_terms = 8 if precision == "mixed" else 16
+        # at most this many terms in the graph (None: no limit); a smaller limit makes more
+        # series continue past the graph
+        self.graph_terms_max = None
         self.last_launch_shape = None
         self.whoami = None
         self.vector = None
@@ -1703,7 +1706,8 @@ class BatchedTimeEvolution:
             # terms 0 .. j of the slowest trajectory; a longer series continues eagerly
             self._graph_terms = int(info["j"].max()) + 1
         # at least two terms: the graph's series reads v as the partial sum until term 1 stores it
-        nterms = max(2, min(self.m_star, self._graph_terms, _hip.SERIES_MAX_TERMS))
+        nterms = max(2, min(self.m_star, self._graph_terms, _hip.SERIES_MAX_TERMS,
+                            self.graph_terms_max or _hip.SERIES_MAX_TERMS))
         wsp, wsn = self._tws.get(_lib.paces_batch_taylor_workspace_bytes(cap, h.ntraj))
         d = _hip.BatchSeries()
         d.R = R
@@ -1744,12 +1748,15 @@ class BatchedTimeEvolution:
         g.replay()
         return nterms
 
-    def propagate(self, delta_t, fuse_finish=False):
+    def propagate(self, delta_t, fuse_finish=False, obs_step=None):
         """The Taylor series of every trajectory. FP64: complex128 terms in two alternating
         buffers, the partial sum stored on odd terms. Mixed: complex64 terms, each in a buffer of
         its own; no term forms the partial sum, and the finish adds t_0 .. t_j to v in place (at
-        most SERIES_MAX_TERMS terms). With fuse_finish the step's observables pass does the
-        finish (step() launches that pass next and nothing reads the amplitudes in between)."""
+        most SERIES_MAX_TERMS terms). With fuse_finish the step's observables pass (for step
+        obs_step) does the finish; in the mixed graph form it is launched right after the graph,
+        before the host learns whether every series has stopped, and a series that continues
+        past the graph gets its remaining terms added by a second pass (terms are added in
+        order, so the sum is the same)."""
         h = self.hamobj
         R = self.numstates
         mats = [self.sparse_mats_dict[t] for t in h.offdiag_terms]
@@ -1780,11 +1787,19 @@ class BatchedTimeEvolution:
         self.last_launch_shape = (cfg.block, cfg.rows_per_thread) if cfg is not None else None
         cref = C.byref(cfg) if cfg is not None else None
         j = 0
+        spec = 0  # terms the speculative observables pass finished
         if self.graph:
             j = self._propagate_graph(delta_t, fill, mixed, res, bufs, tbuf, ts, hdr, diag, cfg,
                                       st)
             wsp, wsn = self._tws.get(0)  # the graph's workspace (same buffer, grown if needed)
             x_in = tb[j - 1] if mixed else bufs[(j - 1) % 2]
+            if fuse_finish and mixed:
+                # res is v itself: the observables pass finishes it in place
+                self._keys = None
+                self._obs_last = None
+                self._finish = (tb[:j], ts, mixed, 0)
+                self._launch_observables(obs_step)
+                spec = j
         else:
             check(_lib.paces_batch_taylor_begin_tuned(R, ptr(torch.view_as_real(self.vector)),
                                                       None if mixed else
@@ -1812,9 +1827,7 @@ class BatchedTimeEvolution:
         # the finish: mixed, v + t_0 + ... + t_j (the j terms run); FP64, the last term of the
         # trajectories whose series ended on an even term (even terms wrote bufs[0])
         terms = tb[:j] if mixed else [bufs[0]]
-        if fuse_finish:
-            self._finish = (terms, ts, mixed)
-        else:
+        if not fuse_finish:
             tp = (C.c_void_p * len(terms))(*[ptr(torch.view_as_real(t)) for t in terms])
             check(_lib.paces_batch_taylor_finish(R, tp, len(terms), 1 if mixed else 0,
                                                  ptr(torch.view_as_real(res)),
@@ -1822,8 +1835,12 @@ class BatchedTimeEvolution:
                                                  cref, st), "batch_taylor_finish")
         del keep
         self.vector = res
-        self._keys = None
-        self._obs_last = None
+        if j > spec:  # (no second pass when the speculative one saw every term)
+            self._keys = None
+            self._obs_last = None
+            if fuse_finish:
+                self._finish = (terms, ts, mixed, spec)
+                self._launch_observables(obs_step)
         # the per-trajectory series states go to pinned host memory without a stream sync; they
         # are parsed when first read (last_expm, or the next step's graph size)
         host = torch.empty(ts.shape, dtype=ts.dtype, device="cpu", pin_memory=True)
@@ -1886,8 +1903,9 @@ class BatchedTimeEvolution:
         dbg = self.adapt()
         self._resolve_observables()  # the last step's: the adaptation has waited on the device
         if i > 0:
-            self.propagate(delta_t, fuse_finish=True)
-        self._launch_observables(i)
+            self.propagate(delta_t, fuse_finish=True, obs_step=i)  # launches the observables
+        else:
+            self._launch_observables(i)
         return dbg
 
     def _launch_observables(self, i):

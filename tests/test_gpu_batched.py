@@ -154,6 +154,45 @@ def test_graph_series_equals_eager(precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_series_continued_past_the_graph(precision):
+    """With at most 2 (then 3) terms in the graph every series continues eagerly past it; in the
+    mixed form the step's observables pass is launched right after the graph (before the host
+    reads whether every series stopped) and a second pass adds the remaining terms. The state,
+    the stopping states and the observables equal the eager run's bit for bit at every step."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from paces_amd.config import devices
+    from paces_amd.models import holstein
+    if not devices.configure_called:
+        devices.configure()
+    draws = _draws(7, seed=5)
+    runs = []
+    for graph in (False, True):
+        bham = holstein.BatchedHamiltonian([d[0] for d in draws], 8, [-d[1] for d in draws],
+                                           [d[2] for d in draws])
+        bte = holstein.BatchedTimeEvolution(bham, [d[3] for d in draws], enlarge_steps=1,
+                                            precision=precision, graph=graph)
+        bte.create_single_exciton_states([d[0] // 2 for d in draws])
+        runs.append(bte)
+    runs[1].graph_terms_max = 2
+    for i in range(12):
+        if i == 6:
+            runs[1].graph_terms_max = 3
+        print(f"[continued {precision}] step {i}", flush=True)
+        for bte in runs:
+            bte.step(i, 0.05)
+        a, b = runs
+        assert torch.equal(a.whoami, b.whoami), i
+        assert torch.equal(a.vector, b.vector), i
+        if i > 0:
+            assert a.last_expm.tobytes() == b.last_expm.tobytes(), i
+            assert int(b.last_expm["j"].max()) >= 3, i  # every step continued past the graph
+        x, y = a.observables(), b.observables()
+        assert all(np.array_equal(p, q) for p, q in zip(x, y)), i
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("graph", [False, True])
 def test_coded_values_equal_explicit_values(precision, graph):
     """The series over coded CSR values (a code byte per vib entry, each trajectory's values
