@@ -136,11 +136,14 @@ __host__ __device__ constexpr int sel_shift(int pass) { return pass < 5 ? 53 - 1
 __host__ __device__ constexpr int sel_width(int pass) { return pass < 5 ? 11 : 9; }
 
 __global__ void k_bsel_init(const int32_t* __restrict__ seg_off, const int64_t* __restrict__ kpt,
-                            int ntraj, SegSel* __restrict__ st, unsigned* __restrict__ hist) {
+                            int ntraj, SegSel* __restrict__ st, unsigned* __restrict__ hist,
+                            int32_t* __restrict__ ccount, int32_t* __restrict__ out_count) {
   const int t = blockIdx.x;
   if (t >= ntraj) return;
   for (int d = threadIdx.x; d < kSelBins; d += blockDim.x) hist[(int64_t)t * kSelBins + d] = 0;
+  if (t == 0 && threadIdx.x == 0) *out_count = 0;
   if (threadIdx.x == 0) {
+    ccount[t] = 0;
     const long long Rt = seg_off[t + 1] - seg_off[t];
     SegSel s;
     s.prefix = 0;
@@ -238,154 +241,239 @@ __global__ void __launch_bounds__(256) k_bsel_pick(SegSel* __restrict__ st,
 
 // ---- the selection's output: the keys above each trajectory's threshold key, and the last krem
 // of the keys equal to it (among equal keys the later rows are kept: the stable order of the
-// reference's sort). Three passes over kSelTile-row tiles of one trajectory each:
-// 1. k_bsel_count: the tile's numbers of keys above and equal to the threshold;
-// 2. k_bsel_tiles (one workgroup per trajectory): each tile's tie offset within its trajectory
-//    and its kept count; a scan of the kept counts then gives each tile's output offset;
-// 3. k_bsel_emit: each tile writes its kept rows in row order (in-tile prefixes from ballots).
-// A thread takes rows r0 + q kBlock + x (q < kSelRpt), so row order is step-major.
+// reference's sort). After the first kSelFull digit passes over every row, the rows whose keys
+// match the prefix fixed so far (a bucket of relative width 2^-11 around the threshold) are the
+// candidates: k_bsel_cand gathers their row indices into their trajectory's stretch of a
+// workspace array, and k_bsel_final (one workgroup per trajectory) runs the remaining digits over
+// them, then finds the cut: the row index of the first kept tie (ties ranked in row order), so a
+// row is kept iff its key is above the threshold, or equal to it at a row index >= the cut.
+constexpr int kSelFull = 2;
 
-__global__ void __launch_bounds__(kBlock) k_bsel_count(const u64* __restrict__ keys,
-                                                       const int32_t* __restrict__ seg_off,
-                                                       const int64_t* __restrict__ boff, int ntraj,
-                                                       const SegSel* __restrict__ st,
-                                                       int2* __restrict__ tc,
-                                                       int32_t* __restrict__ kc) {
-  __shared__ int sg[kBlock / 64], se[kBlock / 64];
+__global__ void __launch_bounds__(kBlock) k_bsel_cand(const u64* __restrict__ keys,
+                                                      const int32_t* __restrict__ seg_off,
+                                                      const int64_t* __restrict__ boff, int ntraj,
+                                                      const SegSel* __restrict__ st,
+                                                      int32_t* __restrict__ ci,
+                                                      int32_t* __restrict__ ccount) {
   const SegTile T = seg_tile(seg_off, boff, ntraj, kSelTile);
-  if (T.t < 0) {
-    if (threadIdx.x == 0) kc[blockIdx.x] = 0;  // tiles past the last trajectory keep nothing
-    return;
-  }
-  const SegSel s = st[T.t];
+  if (T.t < 0 || st[T.t].all || st[T.t].krem == 0) return;
+  const u64 prefix = st[T.t].prefix, mask = st[T.t].mask;
+  const int lane = threadIdx.x & 63;
   u64 k[kSelRpt];
 #pragma unroll
   for (int q = 0; q < kSelRpt; ++q) {
     const int64_t i = T.r0 + q * kBlock + threadIdx.x;
     k[q] = keys[i < T.r1 ? i : T.r0];
   }
-  int ng = 0, ne = 0;
+  int32_t* out = ci + seg_off[T.t];  // the trajectory's stretch: at most R_t candidates
 #pragma unroll
   for (int q = 0; q < kSelRpt; ++q) {
     const int64_t i = T.r0 + q * kBlock + threadIdx.x;
-    if (i < T.r1) {
-      ng += (s.all || k[q] > s.prefix) ? 1 : 0;
-      ne += (!s.all && k[q] == s.prefix) ? 1 : 0;
-    }
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    ng += __shfl_xor(ng, o, 64);
-    ne += __shfl_xor(ne, o, 64);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    sg[threadIdx.x >> 6] = ng;
-    se[threadIdx.x >> 6] = ne;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int a = 0, b = 0;
-#pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) {
-      a += sg[w];
-      b += se[w];
-    }
-    tc[blockIdx.x] = make_int2(a, b);
+    const bool m = i < T.r1 && (k[q] & mask) == prefix;
+    const u64 bl = __ballot(m);
+    if (bl == 0) continue;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(&ccount[T.t], (int)__popcll(bl));
+    base = __shfl(base, 0, 64);
+    const int pre = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bl >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((unsigned)bl, 0u));
+    if (m) out[base + pre] = (int32_t)i;
   }
 }
 
-// one workgroup per trajectory: tie offsets (exclusive, within the trajectory) and kept counts of
-// its tiles, and the tie rank from which ties are kept (thr = ties - krem)
-__global__ void __launch_bounds__(256) k_bsel_tiles(const int64_t* __restrict__ boff,
-                                                    const SegSel* __restrict__ st,
-                                                    const int2* __restrict__ tc,
-                                                    int32_t* __restrict__ TO,
-                                                    int32_t* __restrict__ kc,
-                                                    long long* __restrict__ thr) {
-  __shared__ int wsum[4];
-  __shared__ long long carry_s;
+// In one 256-thread workgroup: the bin of h[0, kSelBins) holding the r-th key (r >= 1) counted
+// from the top (desc) or the r-th (r >= 0, 0-based) counted from the bottom (asc); returns the
+// bin and, in *before, the keys counted past before it. Every thread must call it.
+__device__ __forceinline__ int blk_pick_bin(const unsigned* h, long long r, bool desc,
+                                            long long* before) {
+  constexpr int kPer = kSelBins / 256;
+  __shared__ long long acc[257];
+  __shared__ int s_pick;
+  __shared__ long long s_before;
+  const int d0 = threadIdx.x * kPer;
+  unsigned c[kPer];
+  long long tot = 0;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    c[j] = h[d0 + j];
+    tot += c[j];
+  }
+  acc[threadIdx.x] = tot;
+  if (threadIdx.x == 0) {
+    acc[256] = 0;
+    s_pick = -1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (desc) {  // acc[u] = keys in the bins of threads u .. 255
+      for (int u = 255; u >= 0; --u) acc[u] += acc[u + 1];
+    } else {  // acc[u] = keys in the bins of threads 0 .. u - 1
+      long long run = 0;
+      for (int u = 0; u < 256; ++u) {
+        const long long v = acc[u];
+        acc[u] = run;
+        run += v;
+      }
+    }
+  }
+  __syncthreads();
+  if (desc) {
+    long long above = acc[threadIdx.x + 1];
+#pragma unroll
+    for (int j = kPer - 1; j >= 0; --j) {
+      const long long with = above + c[j];
+      if (with >= r && above < r) {
+        s_pick = d0 + j;
+        s_before = above;
+      }
+      above = with;
+    }
+  } else {
+    long long below = acc[threadIdx.x];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const long long with = below + c[j];
+      if (below <= r && r < with) {
+        s_pick = d0 + j;
+        s_before = below;
+      }
+      below = with;
+    }
+  }
+  __syncthreads();
+  const int pick = s_pick;
+  *before = s_before;
+  __syncthreads();
+  return pick;
+}
+
+// One workgroup per trajectory: digit passes kSelFull .. kSelPasses - 1 over the candidates, then
+// the cut (the (ties - krem)-th smallest row index among the keys equal to the threshold, by
+// digits of the row index), and the kept count added to *out_count. A trajectory that keeps
+// nothing gets the threshold ~0 (no key reaches it: keys are the bits of non-negative doubles).
+__global__ void __launch_bounds__(256) k_bsel_final(SegSel* __restrict__ st,
+                                                    const u64* __restrict__ keys,
+                                                    const int32_t* __restrict__ ci,
+                                                    const int32_t* __restrict__ ccount,
+                                                    const int32_t* __restrict__ seg_off,
+                                                    const int64_t* __restrict__ kpt,
+                                                    int32_t* __restrict__ cut,
+                                                    int32_t* __restrict__ out_count) {
+  __shared__ unsigned h[kSelBins];
+  __shared__ int s_n;
   const int t = blockIdx.x;
-  const int64_t b0 = boff[t], b1 = boff[t + 1];
-  const SegSel s = st[t];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (threadIdx.x == 0) carry_s = 0;
-  __syncthreads();
-  for (int64_t c0 = b0; c0 < b1; c0 += 256) {  // exclusive scan of the tie counts
-    const int64_t b = c0 + threadIdx.x;
-    const int v = b < b1 ? tc[b].y : 0;
-    int inc = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int u = __shfl_up(inc, o, 64);
-      if (lane >= o) inc += u;
-    }
-    if (lane == 63) wsum[wave] = inc;
-    __syncthreads();
-    int before = 0, total = 0;
-#pragma unroll
-    for (int w = 0; w < 4; ++w) {
-      before += w < wave ? wsum[w] : 0;
-      total += wsum[w];
-    }
-    const long long carry = carry_s;
-    if (b < b1) TO[b] = (int32_t)(carry + before + inc - v);
-    __syncthreads();
-    if (threadIdx.x == 0) carry_s = carry + total;
-    __syncthreads();
+  SegSel s = st[t];
+  const long long Rt = seg_off[t + 1] - seg_off[t];
+  if (threadIdx.x == 0) {
+    const long long kept = s.all ? Rt : (kpt[t] < Rt ? kpt[t] : Rt);
+    atomicAdd(out_count, (int)kept);
   }
-  const long long th = s.all ? 0 : carry_s - s.krem;
-  if (threadIdx.x == 0) thr[t] = th;
-  for (int64_t b = b0 + threadIdx.x; b < b1; b += 256) {
-    const int2 c = tc[b];
-    int keep = c.x;
-    if (!s.all) {
-      const long long lo = TO[b], hi = lo + c.y;  // this tile's tie ranks [lo, hi)
-      const long long from = lo > th ? lo : th;
-      keep += (int)(hi > from ? hi - from : 0);
+  if (s.all) {
+    if (threadIdx.x == 0) cut[t] = 0;
+    return;
+  }
+  if (s.krem == 0) {
+    if (threadIdx.x == 0) {
+      st[t].prefix = ~0ull;
+      st[t].mask = ~0ull;
+      cut[t] = 0x7fffffff;
     }
-    kc[b] = keep;
+    return;
+  }
+  const int n = ccount[t];
+  const int32_t* c = ci + seg_off[t];
+  for (int pass = kSelFull; pass < kSelPasses; ++pass) {
+    const int shift = sel_shift(pass), width = sel_width(pass);
+    const u64 dm = (1ull << width) - 1ull;
+    for (int d = threadIdx.x; d < kSelBins; d += 256) h[d] = 0;
+    __syncthreads();
+    for (int q = threadIdx.x; q < n; q += 256) {
+      const u64 k = keys[c[q]];
+      if ((k & s.mask) == s.prefix) atomicAdd(&h[(k >> shift) & dm], 1u);
+    }
+    __syncthreads();
+    long long above;
+    const int pick = blk_pick_bin(h, s.krem, true, &above);
+    s.prefix |= ((u64)pick) << shift;
+    s.mask |= dm << shift;
+    s.krem -= above;
+  }
+  // ties: the keys equal to the threshold; the first kept one has tie rank th = ties - krem
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  int nt = 0;
+  for (int q = threadIdx.x; q < n; q += 256) nt += keys[c[q]] == s.prefix ? 1 : 0;
+  atomicAdd(&s_n, nt);
+  __syncthreads();
+  long long r = s_n - s.krem;  // 0 <= r < ties
+  // the r-th smallest row index among the ties, 11 + 11 + 9 bits (row indices < 2^31)
+  uint32_t ip = 0, im = 0;
+  for (int pass = 0; pass < 3; ++pass) {
+    const int shift = pass == 0 ? 20 : (pass == 1 ? 9 : 0);
+    const int width = pass == 2 ? 9 : 11;
+    const uint32_t dm = (1u << width) - 1u;
+    for (int d = threadIdx.x; d < kSelBins; d += 256) h[d] = 0;
+    __syncthreads();
+    for (int q = threadIdx.x; q < n; q += 256) {
+      const uint32_t i = (uint32_t)c[q];
+      if (keys[i] == s.prefix && (i & im) == ip) atomicAdd(&h[(i >> shift) & dm], 1u);
+    }
+    __syncthreads();
+    long long below;
+    const int pick = blk_pick_bin(h, r, false, &below);
+    ip |= (uint32_t)pick << shift;
+    im |= dm << shift;
+    r -= below;
+  }
+  if (threadIdx.x == 0) {
+    st[t].prefix = s.prefix;
+    st[t].mask = s.mask;
+    st[t].krem = s.krem;
+    cut[t] = (int32_t)ip;
   }
 }
 
-// The keep decision of each of the thread's rows of tile T (keys k[], valid vv[]): above the
-// threshold key, or a tie whose rank in row order reaches the trajectory's tie threshold. wt is
-// the workgroup's kSelRpt x waves scratch; every thread of the workgroup must call this.
-__device__ __forceinline__ void bsel_keep(const u64 (&k)[kSelRpt], const bool (&vv)[kSelRpt],
-                                          const SegTile& T, const SegSel& s,
-                                          const int32_t* __restrict__ TO,
-                                          const long long* __restrict__ thr, int* wt,
-                                          bool (&keep)[kSelRpt]) {
-  constexpr int kWaves = kBlock / 64;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  // tie ranks: exclusive prefix of the tie flags in row order, plus the tile's tie offset
-  int pre[kSelRpt];
-  bool tie[kSelRpt];
+__device__ __forceinline__ bool bsel_kept(u64 k, int64_t i, const SegSel& s, int32_t cut) {
+  return s.all || k > s.prefix || (k == s.prefix && i >= (int64_t)cut);
+}
+
+// per tile: the rows kept (tiles past the last trajectory keep nothing)
+__global__ void __launch_bounds__(kBlock) k_bsel_count(const u64* __restrict__ keys,
+                                                       const int32_t* __restrict__ seg_off,
+                                                       const int64_t* __restrict__ boff, int ntraj,
+                                                       const SegSel* __restrict__ st,
+                                                       const int32_t* __restrict__ cut,
+                                                       int32_t* __restrict__ kc) {
+  __shared__ int sg[kBlock / 64];
+  const SegTile T = seg_tile(seg_off, boff, ntraj, kSelTile);
+  if (T.t < 0) {
+    if (threadIdx.x == 0) kc[blockIdx.x] = 0;
+    return;
+  }
+  const SegSel s = st[T.t];
+  const int32_t cu = cut[T.t];
+  u64 k[kSelRpt];
 #pragma unroll
   for (int q = 0; q < kSelRpt; ++q) {
-    tie[q] = vv[q] && !s.all && k[q] == s.prefix;
-    const u64 bl = __ballot(tie[q]);
-    pre[q] = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bl >> 32),
-                                            __builtin_amdgcn_mbcnt_lo((unsigned)bl, 0u));
-    if (lane == 0) wt[q * kWaves + wave] = __popcll(bl);
+    const int64_t i = T.r0 + q * kBlock + threadIdx.x;
+    k[q] = keys[i < T.r1 ? i : T.r0];
   }
+  int ng = 0;
+#pragma unroll
+  for (int q = 0; q < kSelRpt; ++q) {
+    const int64_t i = T.r0 + q * kBlock + threadIdx.x;
+    ng += (i < T.r1 && bsel_kept(k[q], i, s, cu)) ? 1 : 0;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) ng += __shfl_xor(ng, o, 64);
+  if ((threadIdx.x & 63) == 0) sg[threadIdx.x >> 6] = ng;
   __syncthreads();
   if (threadIdx.x == 0) {
-    int run = 0;
-    for (int u = 0; u < kSelRpt * kWaves; ++u) {
-      const int c = wt[u];
-      wt[u] = run;
-      run += c;
-    }
-  }
-  __syncthreads();
-  const long long th = thr[T.t];
-  const int to = TO[blockIdx.x];
+    int a = 0;
 #pragma unroll
-  for (int q = 0; q < kSelRpt; ++q) {
-    const long long rank = (long long)to + wt[q * kWaves + wave] + pre[q];
-    keep[q] = vv[q] && (s.all || k[q] > s.prefix || (tie[q] && rank >= th));
+    for (int w = 0; w < kBlock / 64; ++w) a += sg[w];
+    kc[blockIdx.x] = a;
   }
-  __syncthreads();  // the tie prefixes are read; wt may be reused
 }
 
 template <int W>
@@ -394,34 +482,31 @@ __global__ void __launch_bounds__(kBlock) k_bsel_emit(const u64* __restrict__ ke
                                                       const int32_t* __restrict__ seg_off,
                                                       const int64_t* __restrict__ boff, int ntraj,
                                                       const SegSel* __restrict__ st,
-                                                      const int32_t* __restrict__ TO,
-                                                      const long long* __restrict__ thr,
-                                                      const int32_t* __restrict__ KO, int ntiles,
-                                                      u64* __restrict__ out,
-                                                      int32_t* __restrict__ out_count) {
+                                                      const int32_t* __restrict__ cut,
+                                                      const int32_t* __restrict__ KO,
+                                                      u64* __restrict__ out) {
   constexpr int kWaves = kBlock / 64;
   __shared__ int wt[kSelRpt * kWaves];
-  if (blockIdx.x == 0 && threadIdx.x == 0) *out_count = KO[ntiles];
   const SegTile T = seg_tile(seg_off, boff, ntraj, kSelTile);
   if (T.t < 0) return;
   const SegSel s = st[T.t];
+  const int32_t cu = cut[T.t];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   u64 k[kSelRpt];
   Row<W> x[kSelRpt];
-  bool vv[kSelRpt];
+  bool keep[kSelRpt];
 #pragma unroll
   for (int q = 0; q < kSelRpt; ++q) {
     const int64_t i = T.r0 + q * kBlock + threadIdx.x;
-    vv[q] = i < T.r1;
-    const int64_t ic = vv[q] ? i : T.r0;
+    const int64_t ic = i < T.r1 ? i : T.r0;
     k[q] = keys[ic];
     x[q] = load_row<W>(rows, ic);
   }
-  bool keep[kSelRpt];
-  bsel_keep(k, vv, T, s, TO, thr, wt, keep);
   int pre[kSelRpt];
 #pragma unroll
   for (int q = 0; q < kSelRpt; ++q) {
+    const int64_t i = T.r0 + q * kBlock + threadIdx.x;
+    keep[q] = i < T.r1 && bsel_kept(k[q], i, s, cu);
     const u64 bl = __ballot(keep[q]);
     pre[q] = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bl >> 32),
                                             __builtin_amdgcn_mbcnt_lo((unsigned)bl, 0u));
@@ -449,40 +534,33 @@ __global__ void __launch_bounds__(kBlock) k_bsel_flags(const u64* __restrict__ k
                                                        const int32_t* __restrict__ seg_off,
                                                        const int64_t* __restrict__ boff, int ntraj,
                                                        const SegSel* __restrict__ st,
-                                                       const int32_t* __restrict__ TO,
-                                                       const long long* __restrict__ thr,
-                                                       const int32_t* __restrict__ KO, int ntiles,
+                                                       const int32_t* __restrict__ cut,
                                                        const uint8_t* __restrict__ in_be,
-                                                       uint8_t* __restrict__ flags,
-                                                       int32_t* __restrict__ out_count) {
-  constexpr int kWaves = kBlock / 64;
-  __shared__ int wt[kSelRpt * kWaves];
-  if (blockIdx.x == 0 && threadIdx.x == 0) *out_count = KO[ntiles];
+                                                       uint8_t* __restrict__ flags) {
   const SegTile T = seg_tile(seg_off, boff, ntraj, kSelTile);
   if (T.t < 0) return;
   const SegSel s = st[T.t];
+  const int32_t cu = cut[T.t];
   u64 k[kSelRpt];
-  bool vv[kSelRpt];
 #pragma unroll
   for (int q = 0; q < kSelRpt; ++q) {
     const int64_t i = T.r0 + q * kBlock + threadIdx.x;
-    vv[q] = i < T.r1;
-    k[q] = keys[vv[q] ? i : T.r0];
+    k[q] = keys[i < T.r1 ? i : T.r0];
   }
-  bool keep[kSelRpt];
-  bsel_keep(k, vv, T, s, TO, thr, wt, keep);
 #pragma unroll
   for (int q = 0; q < kSelRpt; ++q) {
     const int64_t i = T.r0 + q * kBlock + threadIdx.x;
-    if (vv[q]) flags[i] = (uint8_t)((keep[q] ? 1 : 0) | ((in_be && in_be[i]) ? 0x80 : 0));
+    if (i < T.r1)
+      flags[i] = (uint8_t)((bsel_kept(k[q], i, s, cu) ? 1 : 0) |
+                           ((in_be && in_be[i]) ? 0x80 : 0));
   }
 }
 
 size_t bsel_ws_bytes(int64_t R, int ntraj) {
   const int64_t nt = seg_grid(R, ntraj, kSelTile);
   return a256((size_t)ntraj * sizeof(SegSel)) + a256((size_t)ntraj * kSelBins * 4) +
-         a256((size_t)(ntraj + 1) * 8) + a256((size_t)R * 8) + a256((size_t)nt * 8) +
-         2 * a256((size_t)nt * 4) + a256((size_t)(nt + 1) * 4) + a256((size_t)ntraj * 8) +
+         a256((size_t)(ntraj + 1) * 8) + a256((size_t)R * 8) + a256((size_t)R * 4) +
+         2 * a256((size_t)ntraj * 4) + a256((size_t)nt * 4) + a256((size_t)(nt + 1) * 4) +
          a256(scan_ws_bytes(nt));
 }
 
@@ -502,47 +580,48 @@ static int run_bselect(const double* psi, const uint64_t* rows, int64_t R,
   p += a256((size_t)(ntraj + 1) * 8);
   u64* keys = (u64*)p;
   p += a256((size_t)R * 8);
-  int2* tc = (int2*)p;
-  p += a256((size_t)gsh * 8);
-  int32_t* TO = (int32_t*)p;
-  p += a256((size_t)gsh * 4);
+  int32_t* ci = (int32_t*)p;  // candidates' row indices, trajectory t's at seg_off[t]
+  p += a256((size_t)R * 4);
+  int32_t* ccount = (int32_t*)p;
+  p += a256((size_t)ntraj * 4);
+  int32_t* cut = (int32_t*)p;
+  p += a256((size_t)ntraj * 4);
   int32_t* kc = (int32_t*)p;
   p += a256((size_t)gsh * 4);
   int32_t* KO = (int32_t*)p;
   p += a256((size_t)(gsh + 1) * 4);
-  long long* thr = (long long*)p;
-  p += a256((size_t)ntraj * 8);
   hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(1), 0, st, seg_off, ntraj, kSelTile, boffh);
   if (keys_in)
     keys = (u64*)keys_in;  // the keys of these amplitudes, formed by the observables pass
   else if (R > 0)
     hipLaunchKernelGGL(k_bsel_keys, dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
                        (const double2*)psi, R, keys);
-  hipLaunchKernelGGL(k_bsel_init, dim3(ntraj), dim3(256), 0, st, seg_off, kpt, ntraj, s, hist);
-  for (int pass = 0; pass < kSelPasses; ++pass) {
+  hipLaunchKernelGGL(k_bsel_init, dim3(ntraj), dim3(256), 0, st, seg_off, kpt, ntraj, s, hist,
+                     ccount, out_count);
+  for (int pass = 0; pass < kSelFull; ++pass) {
     const int shift = sel_shift(pass), width = sel_width(pass);
     hipLaunchKernelGGL(k_bsel_hist, dim3(gsh), dim3(kBlock), 0, st, (const u64*)keys, seg_off,
                        (const int64_t*)boffh, ntraj, (const SegSel*)s, shift, width, hist);
     hipLaunchKernelGGL(k_bsel_pick, dim3(ntraj), dim3(256), 0, st, s, hist, shift, width);
   }
-  hipLaunchKernelGGL(k_bsel_count, dim3(gsh), dim3(kBlock), 0, st, (const u64*)keys, seg_off,
-                     (const int64_t*)boffh, ntraj, (const SegSel*)s, tc, kc);
-  hipLaunchKernelGGL(k_bsel_tiles, dim3(ntraj), dim3(256), 0, st, (const int64_t*)boffh,
-                     (const SegSel*)s, (const int2*)tc, TO, kc, thr);
-  int rc = launch_status();
-  if (rc) return rc;
-  rc = scan_i32(kc, gsh, KO, p, scan_ws_bytes(gsh), st);
-  if (rc) return rc;
+  hipLaunchKernelGGL(k_bsel_cand, dim3(gsh), dim3(kBlock), 0, st, (const u64*)keys, seg_off,
+                     (const int64_t*)boffh, ntraj, (const SegSel*)s, ci, ccount);
+  hipLaunchKernelGGL(k_bsel_final, dim3(ntraj), dim3(256), 0, st, s, (const u64*)keys,
+                     (const int32_t*)ci, (const int32_t*)ccount, seg_off, kpt, cut, out_count);
   if constexpr (W == 0) {
     hipLaunchKernelGGL(k_bsel_flags, dim3(gsh), dim3(kBlock), 0, st, (const u64*)keys, seg_off,
-                       (const int64_t*)boffh, ntraj, (const SegSel*)s, (const int32_t*)TO,
-                       (const long long*)thr, (const int32_t*)KO, (int)gsh, in_be, out_flags,
-                       out_count);
+                       (const int64_t*)boffh, ntraj, (const SegSel*)s, (const int32_t*)cut,
+                       in_be, out_flags);
   } else {
+    hipLaunchKernelGGL(k_bsel_count, dim3(gsh), dim3(kBlock), 0, st, (const u64*)keys, seg_off,
+                       (const int64_t*)boffh, ntraj, (const SegSel*)s, (const int32_t*)cut, kc);
+    int rc = launch_status();
+    if (rc) return rc;
+    rc = scan_i32(kc, gsh, KO, p, scan_ws_bytes(gsh), st);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_bsel_emit<W>, dim3(gsh), dim3(kBlock), 0, st, (const u64*)keys,
                        (const u64*)rows, seg_off, (const int64_t*)boffh, ntraj,
-                       (const SegSel*)s, (const int32_t*)TO, (const long long*)thr,
-                       (const int32_t*)KO, (int)gsh, (u64*)out_rows, out_count);
+                       (const SegSel*)s, (const int32_t*)cut, (const int32_t*)KO, (u64*)out_rows);
   }
   return launch_status();
 }
