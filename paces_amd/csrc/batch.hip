@@ -40,15 +40,38 @@ __global__ void k_seg_offsets(const u64* __restrict__ rows, int64_t R, int trajb
 }
 
 // boff[t]: first workgroup of trajectory t; boff[ntraj] = number of workgroups
-__global__ void k_seg_blocks(const int32_t* __restrict__ seg_off, int ntraj, int tile,
-                             int64_t* __restrict__ boff) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  int64_t b = 0;
-  for (int t = 0; t < ntraj; ++t) {
+// boff[t] = the tiles of `tile` rows before trajectory t's (boff[ntraj]: all of them); one
+// workgroup of kSegThreads threads, each taking a run of trajectories, with a scan across them
+constexpr int kSegThreads = 256;
+
+__device__ __forceinline__ void seg_blocks_body(const int32_t* __restrict__ seg_off, int ntraj,
+                                                int tile, int64_t* __restrict__ boff) {
+  __shared__ int64_t sc[kSegThreads];
+  const int per = (ntraj + kSegThreads - 1) / kSegThreads;
+  const int t0 = (int)threadIdx.x * per;
+  const int t1 = t0 + per < ntraj ? t0 + per : ntraj;
+  int64_t loc = 0;
+  for (int t = t0; t < t1; ++t) loc += (seg_off[t + 1] - seg_off[t] + tile - 1) / tile;
+  sc[threadIdx.x] = loc;
+  __syncthreads();
+  for (int o = 1; o < kSegThreads; o <<= 1) {  // inclusive scan
+    const int64_t v = (int)threadIdx.x >= o ? sc[threadIdx.x - o] : 0;
+    __syncthreads();
+    sc[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int64_t b = sc[threadIdx.x] - loc;
+  for (int t = t0; t < t1; ++t) {
     boff[t] = b;
     b += (seg_off[t + 1] - seg_off[t] + tile - 1) / tile;
   }
-  boff[ntraj] = b;
+  if (threadIdx.x == kSegThreads - 1) boff[ntraj] = sc[kSegThreads - 1];
+}
+
+__global__ void __launch_bounds__(kSegThreads) k_seg_blocks(const int32_t* __restrict__ seg_off,
+                                                            int ntraj, int tile,
+                                                            int64_t* __restrict__ boff) {
+  seg_blocks_body(seg_off, ntraj, tile, boff);
 }
 
 struct SegTile {
@@ -382,13 +405,28 @@ __global__ void __launch_bounds__(256) k_bsel_final(SegSel* __restrict__ st,
   }
   const int n = ccount[t];
   const int32_t* c = ci + seg_off[t];
+  // up to kCandLds candidates (the usual case) are read once into LDS, with their keys
+  constexpr int kCandLds = 2048;
+  __shared__ u64 lk[kCandLds];
+  __shared__ int32_t li[kCandLds];
+  const bool inl = n <= kCandLds;
+  if (inl) {
+    for (int q = threadIdx.x; q < n; q += 256) {
+      const int32_t i = c[q];
+      li[q] = i;
+      lk[q] = keys[i];
+    }
+    __syncthreads();
+  }
+  auto key_at = [&](int q) -> u64 { return inl ? lk[q] : keys[c[q]]; };
+  auto idx_at = [&](int q) -> uint32_t { return inl ? (uint32_t)li[q] : (uint32_t)c[q]; };
   for (int pass = kSelFull; pass < kSelPasses; ++pass) {
     const int shift = sel_shift(pass), width = sel_width(pass);
     const u64 dm = (1ull << width) - 1ull;
     for (int d = threadIdx.x; d < kSelBins; d += 256) h[d] = 0;
     __syncthreads();
     for (int q = threadIdx.x; q < n; q += 256) {
-      const u64 k = keys[c[q]];
+      const u64 k = key_at(q);
       if ((k & s.mask) == s.prefix) atomicAdd(&h[(k >> shift) & dm], 1u);
     }
     __syncthreads();
@@ -402,7 +440,7 @@ __global__ void __launch_bounds__(256) k_bsel_final(SegSel* __restrict__ st,
   if (threadIdx.x == 0) s_n = 0;
   __syncthreads();
   int nt = 0;
-  for (int q = threadIdx.x; q < n; q += 256) nt += keys[c[q]] == s.prefix ? 1 : 0;
+  for (int q = threadIdx.x; q < n; q += 256) nt += key_at(q) == s.prefix ? 1 : 0;
   atomicAdd(&s_n, nt);
   __syncthreads();
   long long r = s_n - s.krem;  // 0 <= r < ties
@@ -415,8 +453,8 @@ __global__ void __launch_bounds__(256) k_bsel_final(SegSel* __restrict__ st,
     for (int d = threadIdx.x; d < kSelBins; d += 256) h[d] = 0;
     __syncthreads();
     for (int q = threadIdx.x; q < n; q += 256) {
-      const uint32_t i = (uint32_t)c[q];
-      if (keys[i] == s.prefix && (i & im) == ip) atomicAdd(&h[(i >> shift) & dm], 1u);
+      const uint32_t i = idx_at(q);
+      if (key_at(q) == s.prefix && (i & im) == ip) atomicAdd(&h[(i >> shift) & dm], 1u);
     }
     __syncthreads();
     long long below;
@@ -590,7 +628,8 @@ static int run_bselect(const double* psi, const uint64_t* rows, int64_t R,
   p += a256((size_t)gsh * 4);
   int32_t* KO = (int32_t*)p;
   p += a256((size_t)(gsh + 1) * 4);
-  hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(1), 0, st, seg_off, ntraj, kSelTile, boffh);
+  hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(kSegThreads), 0, st, seg_off, ntraj, kSelTile,
+                     boffh);
   if (keys_in)
     keys = (u64*)keys_in;  // the keys of these amplitudes, formed by the observables pass
   else if (R > 0)
@@ -909,7 +948,7 @@ static int run_bobs(const uint64_t* rows, double* psi, int64_t R, const int32_t*
   double* part = (double*)p;
   const int S = 2 * N + 1;
   constexpr int TB = obs_tile<W>();
-  hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(1), 0, st, seg_off, ntraj, TB, boff);
+  hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(kSegThreads), 0, st, seg_off, ntraj, TB, boff);
   hipLaunchKernelGGL(k_bobs_partial<W>, dim3(seg_grid(R, ntraj, TB)), dim3(kBlock), 0, st,
                      (const u64*)rows, (double2*)psi, fin, seg_off, (const int64_t*)boff, ntraj,
                      lay->d_off, lay->d_width, N, traj_L, geo, part);
@@ -1488,18 +1527,10 @@ __device__ __forceinline__ u64* ser_part(const SeriesDev* d, int ntraj) {
                                (((size_t)(ntraj + 1) * 8 + 255) & ~(size_t)255)));
 }
 
-__global__ void k_bser_segblocks(const paces_batch_series_t* __restrict__ desc, int ntraj,
-                                 int tile) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+__global__ void __launch_bounds__(kSegThreads) k_bser_segblocks(
+    const paces_batch_series_t* __restrict__ desc, int ntraj, int tile) {
   const SeriesDev* d = (const SeriesDev*)desc;
-  int64_t* boff = gen((PACES_GPTR(int64_t))d->ws);
-  const int32_t* seg = gen(d->seg_off);
-  int64_t b = 0;
-  for (int t = 0; t < ntraj; ++t) {
-    boff[t] = b;
-    b += (seg[t + 1] - seg[t] + tile - 1) / tile;
-  }
-  boff[ntraj] = b;
+  seg_blocks_body(gen(d->seg_off), ntraj, tile, gen((PACES_GPTR(int64_t))d->ws));
 }
 
 template <typename F, int BS, int RPT>
@@ -1808,7 +1839,7 @@ int paces_batch_taylor_begin_tuned(int64_t R, const double* v, double* res, void
   u64* part = (u64*)p;
   int B, RR;
   cfg_get(cfg, &B, &RR);
-  hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(1), 0, st, seg_off, ntraj, B * RR, boff);
+  hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(kSegThreads), 0, st, seg_off, ntraj, B * RR, boff);
   const unsigned gs = seg_grid(R, ntraj, B * RR);
 #define BB_LAUNCH(BB, QQ)                                                                      \
   if (mixed)                                                                                   \
@@ -1839,7 +1870,7 @@ int paces_batch_series_launch(const paces_batch_series_t* desc, int64_t R_cap, i
   cfg_get(cfg, &B, &RR);
   const unsigned gs = seg_grid(R_cap, ntraj, B * RR);
   const unsigned gst = xcd_grid(gs);  // the terms' grid
-  hipLaunchKernelGGL(k_bser_segblocks, dim3(1), dim3(1), 0, st, desc, ntraj, B * RR);
+  hipLaunchKernelGGL(k_bser_segblocks, dim3(1), dim3(kSegThreads), 0, st, desc, ntraj, B * RR);
 #define SB_LAUNCH(BB, QQ)                                                                      \
   if (mixed)                                                                                   \
     hipLaunchKernelGGL((k_bser_begin<float, BB, QQ>), dim3(gs), dim3(BB), 0, st, desc, ntraj); \

@@ -42,8 +42,7 @@ __global__ void __launch_bounds__(kBlock) k_slots_mark(const uint32_t* __restric
                                                        int64_t stride, uint32_t colmask,
                                                        int64_t R, uint8_t* flags, int src_mask,
                                                        int skip_mask, int mark_bit,
-                                                       uint8_t* __restrict__ expl,
-                                                       int32_t* __restrict__ count) {
+                                                       uint8_t* __restrict__ expl) {
   const int64_t i0 = (int64_t)blockIdx.x * (kBlock * kMarkRpt) + threadIdx.x;
   uint32_t nb[kMarkRpt][NS];
   int f[kMarkRpt];
@@ -61,7 +60,6 @@ __global__ void __launch_bounds__(kBlock) k_slots_mark(const uint32_t* __restric
 #pragma unroll
     for (int m = 0; m < NS; ++m)
       g[q][m] = nb[q][m] != PACES_SLOT_EMPTY ? flags[nb[q][m] & colmask] : 0;
-  int ne = 0;
 #pragma unroll
   for (int q = 0; q < kMarkRpt; ++q) {
     const int64_t i = i0 + q * kBlock;
@@ -71,14 +69,47 @@ __global__ void __launch_bounds__(kBlock) k_slots_mark(const uint32_t* __restric
       hit = hit || ((g[q][m] & 0x80) && (g[q][m] & src_mask) && !(g[q][m] & skip_mask));
     const bool e = i < R && (f[q] & src_mask) && !(f[q] & skip_mask) && !(f[q] & 0x80);
     if (i < R) {
-      if (hit && !(f[q] & mark_bit)) flags[i] = (uint8_t)(f[q] | mark_bit);
+      // every row's byte is written (unmarked rows unchanged), so the stores fill whole lines;
+      // a concurrent reader of this byte tests only src_mask, skip_mask and 0x80
+      flags[i] = (uint8_t)(f[q] | (hit ? mark_bit : 0));
       expl[i] = e ? 1 : 0;
     }
-    ne += e ? 1 : 0;
   }
+}
+
+// *count += the nonzero bytes of a[0, n): 8 bytes per load, grid-stride, one atomic per
+// workgroup (a fixed grid of kCountBlocks; an atomic per wave had every wave with an explicit
+// row serialise on one address)
+constexpr int kCountBlocks = 2048;
+__global__ void __launch_bounds__(kBlock) k_count_u8(const uint8_t* __restrict__ a, int64_t n,
+                                                     int32_t* __restrict__ count) {
+  __shared__ int ws[kBlock / 64];
+  int c = 0;
+  const int64_t n8 = n >> 3;
+  const bool al = ((uintptr_t)a & 7) == 0;
+  for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n8;
+       v += (int64_t)gridDim.x * kBlock) {
+    if (al) {
+      const u64 x = ((const u64*)a)[v];
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) ne += __shfl_xor(ne, o, 64);
-  if ((threadIdx.x & 63) == 0 && ne) atomicAdd(count, ne);
+      for (int j = 0; j < 8; ++j) c += ((x >> (8 * j)) & 0xff) ? 1 : 0;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) c += a[v * 8 + j] ? 1 : 0;
+    }
+  }
+  if (blockIdx.x == 0)
+    for (int64_t i = (n8 << 3) + threadIdx.x; i < n; i += kBlock) c += a[i] ? 1 : 0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) t += ws[w];
+    if (t) atomicAdd(count, t);
+  }
 }
 
 // ---- stable compaction of flagged indices ------------------------------------------------------
@@ -431,12 +462,12 @@ int paces_slots_mark(const uint32_t* slots, int64_t stride, int32_t nslots, int3
     const dim3 g(grid_for(R, kBlock * kMarkRpt));
     if (nslots == 4)
       hipLaunchKernelGGL(k_slots_mark<4>, g, dim3(kBlock), 0, (hipStream_t)stream, slots, stride,
-                         cm, R, flags, (int)src_mask, (int)skip_mask, (int)mark_bit, expl,
-                         expl_count);
+                         cm, R, flags, (int)src_mask, (int)skip_mask, (int)mark_bit, expl);
     else
       hipLaunchKernelGGL(k_slots_mark<8>, g, dim3(kBlock), 0, (hipStream_t)stream, slots, stride,
-                         cm, R, flags, (int)src_mask, (int)skip_mask, (int)mark_bit, expl,
-                         expl_count);
+                         cm, R, flags, (int)src_mask, (int)skip_mask, (int)mark_bit, expl);
+    hipLaunchKernelGGL(k_count_u8, dim3(kCountBlocks), dim3(kBlock), 0, (hipStream_t)stream,
+                       (const uint8_t*)expl, R, expl_count);
   }
   return launch_status();
 }
