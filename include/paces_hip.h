@@ -361,17 +361,20 @@ int paces_warm_positions(const uint8_t* flags, int64_t R, const int32_t* ins, in
  * amplitude (the reassignment of time_evolution.py:781-797), diagonal (float when diag_f32,
  * else double), out_be = (flags & be_mask) != 0, and its slots: an entry whose pair has an
  * endpoint with flags & be_mask is kept with its column renumbered through NP, any other slot
- * is empty. */
+ * is empty. out_x0 (optional): the amplitude rounded to complex64 (a mixed series' first term
+ * input, for paces_batch_series_launch's prebegun mode). */
 int paces_warm_emit(const uint64_t* rows, int32_t W, const double* psi, const void* diag,
                     int32_t diag_f32, int64_t R, const uint8_t* flags, int32_t be_mask,
                     const int32_t* NP, const uint32_t* slots, int64_t stride_old, int32_t nslots,
                     int32_t col_bits, uint64_t* out_rows, double* out_psi, void* out_diag,
-                    uint8_t* out_be, uint32_t* out_slots, int64_t stride_new, void* stream);
-/* The nY new rows Y to positions ypos: amplitude 0, diagonal ydiag, out_be = ybe, empty slots */
+                    uint8_t* out_be, uint32_t* out_slots, int64_t stride_new, void* out_x0,
+                    void* stream);
+/* The nY new rows Y to positions ypos: amplitude 0 (and 0 in out_x0, optional), diagonal ydiag,
+ * out_be = ybe, empty slots */
 int paces_warm_emit_new(const uint64_t* Y, int32_t W, int64_t nY, const int32_t* ypos,
                         const void* ydiag, int32_t diag_f32, const uint8_t* ybe, int32_t nslots,
                         uint64_t* out_rows, double* out_psi, void* out_diag, uint8_t* out_be,
-                        uint32_t* out_slots, int64_t stride_new, void* stream);
+                        uint32_t* out_slots, int64_t stride_new, void* out_x0, void* stream);
 
 /* ---- D1/D2: per-term canonical CSR of the Holstein H on the new basis (generate_mel +
  * _add_hc_and_generate_coo + coo_matrix().tocsr(): hamiltonian.py:383-547,
@@ -622,15 +625,16 @@ int paces_holstein_observables_batched(const uint64_t* rows, const double* psi, 
  * finish launched while some series still run is completed by a second call with k0 = the terms
  * the first one saw; FP64 (k0 = 0): psi holds res and gets terms[0] where j is even); keys
  * (optional, R entries) receives each row's select key (the bits of |psi|^2) for
- * paces_batch_select_flags / paces_batch_select_topk_keys. */
+ * paces_batch_select_flags / paces_batch_select_topk_keys; vmax (optional, ntraj entries) each
+ * trajectory's largest key (0 for an empty one). */
 int paces_holstein_observables_batched_finish(uint64_t const* rows, double* psi, int64_t R,
                                               const int32_t* seg_off, const paces_layout_t* lay,
                                               const paces_holstein_t* h,
                                               const void* const* terms, int32_t nterms,
                                               int32_t k0, int32_t mixed,
                                               const paces_batch_taylor_t* ts, uint64_t* keys,
-                                              double* out, void* ws, size_t ws_bytes,
-                                              void* stream);
+                                              uint64_t* vmax, double* out, void* ws,
+                                              size_t ws_bytes, void* stream);
 size_t paces_batch_taylor_workspace_bytes(int64_t R, int32_t ntraj);
 /* res = v (not for mixed != 0: res is unused and may be null); x0 = v as complex64 (mixed) or
  * complex128; ts[t].c1 = ||v_t||_inf */
@@ -749,10 +753,16 @@ typedef struct {
   int32_t slot_vib[PACES_MAX_TERMS];
   int32_t slot_stride;
   void* xt[PACES_SERIES_MAX_TERMS];       /* mixed: term j's output buffer (complex64) */
+  /* prebegun launches: each trajectory's largest |v|^2 as the bits of a double (ntraj), as
+   * paces_holstein_observables_batched_finish's vmax gives it */
+  const uint64_t* vmax;
 } paces_batch_series_t;
 size_t paces_batch_series_desc_bytes(void); /* sizeof(paces_batch_series_t), for bindings */
+/* prebegun != 0: x[2] already holds v as the series' term input (complex64 or complex128) and
+ * vmax its per-trajectory largest |v|^2: the begin pass over v is replaced by the stopping
+ * state's initialisation from vmax (the same state) */
 int paces_batch_series_launch(const paces_batch_series_t* desc, int64_t R_cap, int32_t ntraj,
-                              int32_t mixed, int32_t nslots, int32_t nterms,
+                              int32_t mixed, int32_t nslots, int32_t nterms, int32_t prebegun,
                               const paces_launch_cfg_t* cfg, void* stream);
 
 /* ---- N3 candidate scorer (no reference counterpart): 64 inputs -> 3 x 256 GELU -> 1 logit,

@@ -129,7 +129,8 @@ class BatchSeries(C.Structure):
                 ("term_value", C.c_void_p), ("sqrt_table", C.c_void_p),
                 ("slots", C.c_void_p), ("nslots", C.c_int32), ("col_bits", C.c_int32),
                 ("slot_off", C.c_int32 * (MAX_TERMS + 1)), ("slot_vib", C.c_int32 * MAX_TERMS),
-                ("slot_stride", C.c_int32), ("xt", C.c_void_p * SERIES_MAX_TERMS)]
+                ("slot_stride", C.c_int32), ("xt", C.c_void_p * SERIES_MAX_TERMS),
+                ("vmax", C.c_void_p)]
 
 
 class TaylorState(C.Structure):
@@ -211,9 +212,9 @@ _SIGNATURES = {
     "paces_warm_positions_workspace_bytes": (_SZ, [_I64]),
     "paces_warm_positions": (C.c_int, [_P, _I64, _P, _I64, _P, _P, _P, _P, _SZ, _P]),
     "paces_warm_emit": (C.c_int, [_P, _I32, _P, _P, _I32, _I64, _P, _I32, _P, _P, _I64, _I32,
-                                  _I32, _P, _P, _P, _P, _P, _I64, _P]),
+                                  _I32, _P, _P, _P, _P, _P, _I64, _P, _P]),
     "paces_warm_emit_new": (C.c_int, [_P, _I32, _I64, _P, _P, _I32, _P, _I32, _P, _P, _P, _P,
-                                      _P, _I64, _P]),
+                                      _P, _I64, _P, _P]),
     "paces_holstein_csr_workspace_bytes": (_SZ, [_I64, C.POINTER(Holstein)]),
     "paces_holstein_csr_count": (C.c_int, [_P, _I64, _P, C.POINTER(Layout), C.POINTER(Holstein),
                                            _P, _P, _SZ, _P]),
@@ -286,7 +287,7 @@ _SIGNATURES = {
                                                _SZ, _P]),
     "paces_holstein_observables_batched_finish": (C.c_int, [_P, _P, _I64, _P, C.POINTER(Layout),
                                                             C.POINTER(Holstein), _P, _I32, _I32,
-                                                            _I32, _P, _P, _P, _P, _SZ, _P]),
+                                                            _I32, _P, _P, _P, _P, _P, _SZ, _P]),
     "paces_batch_taylor_term_slots": (C.c_int, [_I64, _P, _I32, C.POINTER(Slots), _P, _P,
                                                 _P, _D, _I32, _D, _P, _I32, _P, _P, _P, _SZ,
                                                 C.POINTER(LaunchCfg), _P]),
@@ -296,7 +297,7 @@ _SIGNATURES = {
     "paces_sort_rows_workspace_bytes": (_SZ, [_I64, _I32]),
     "paces_sort_rows": (C.c_int, [_P, _I64, _I32, _P, _I32, _P, _P, _P, _P, _SZ, _P]),
     "paces_batch_series_desc_bytes": (_SZ, []),
-    "paces_batch_series_launch": (C.c_int, [_P, _I64, _I32, _I32, _I32, _I32,
+    "paces_batch_series_launch": (C.c_int, [_P, _I64, _I32, _I32, _I32, _I32, _I32,
                                             C.POINTER(LaunchCfg), _P]),
     "paces_mlp_weights_bf16": (C.c_int, [_P, _P, _P]),
     "paces_mlp_train_workspace_bytes": (_SZ, [_I64]),

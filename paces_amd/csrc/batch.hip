@@ -698,6 +698,8 @@ struct ObsFinish {
   int mixed;
   const paces_batch_taylor_t* ts;
   u64* keys;
+  u64* pmax;  // with vmax: each tile's largest key (workspace), reduced per trajectory into vmax
+  u64* vmax;
 };
 
 static bool series_terms(const void* const* terms, int32_t nterms, SeriesTerms* st) {
@@ -727,6 +729,7 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
   __shared__ short spos[TB];  // positions < PACES_MAX_SITES; -1 past the tile's rows
   __shared__ int spr[2];
   __shared__ double snorm[NW];
+  __shared__ u64 smax[NW];
   const SegTile T = seg_tile(seg_off, boff, ntraj, TB);
   if (T.t < 0) return;
   if (threadIdx.x == 0) {
@@ -796,16 +799,23 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
         if ((int)threadIdx.x + kBlock * q < nr) psi[ii[q]] = v[q];
     }
     double a = 0.0;
+    u64 mx = 0;
 #pragma unroll
     for (int q = 0; q < TPR; ++q) {
       const int r = (int)threadIdx.x + kBlock * q;
       const double k = v[q].x * v[q].x + v[q].y * v[q].y;
-      if (fin.keys && r < nr) fin.keys[ii[q]] = (u64)__double_as_longlong(k);
+      const u64 kb = (u64)__double_as_longlong(k);
+      if (fin.keys && r < nr) fin.keys[ii[q]] = kb;
       sw[r] = r < nr ? k : 0.0;
       a += r < nr ? k : 0.0;
+      mx = (r < nr && kb > mx) ? kb : mx;  // keys of non-negative doubles order as the values
     }
     a = wave_sum(a);
-    if (lane == 0) snorm[wave] = a;
+    mx = wave_max_u64(mx);
+    if (lane == 0) {
+      snorm[wave] = a;
+      smax[wave] = mx;
+    }
   }
   __syncthreads();
   {
@@ -848,9 +858,14 @@ __global__ void __launch_bounds__(kBlock) k_bobs_partial(const u64* __restrict__
   __syncthreads();
   if (wave == 0 && lane == 0) {
     double a = 0.0;
+    u64 mx = 0;
 #pragma unroll
-    for (int k = 0; k < NW; ++k) a += snorm[k];
+    for (int k = 0; k < NW; ++k) {
+      a += snorm[k];
+      mx = smax[k] > mx ? smax[k] : mx;
+    }
     part[(int64_t)blockIdx.x * S + 2 * N] = a;
+    if (fin.vmax) fin.pmax[blockIdx.x] = mx;
   }
   double w[RPL];
 #pragma unroll
@@ -935,7 +950,19 @@ __global__ void __launch_bounds__(64) k_bobs_final(const double* __restrict__ pa
 
 size_t bobs_ws_bytes(int64_t R, int ntraj, int N) {
   int64_t nblk = (R + kObsTileB - 1) / kObsTileB + ntraj + 1;
-  return a256((size_t)(ntraj + 1) * 8) + a256((size_t)nblk * (2 * N + 1) * 8);
+  return a256((size_t)(ntraj + 1) * 8) + a256((size_t)nblk * (2 * N + 1) * 8) +
+         a256((size_t)nblk * 8);
+}
+
+// one wavefront per trajectory: the largest key of its tiles (0 for an empty trajectory)
+__global__ void __launch_bounds__(64) k_bobs_vmax(const u64* __restrict__ pmax,
+                                                  const int64_t* __restrict__ boff,
+                                                  u64* __restrict__ vmax) {
+  const int t = blockIdx.x;
+  u64 m = 0;
+  for (int64_t b = boff[t] + threadIdx.x; b < boff[t + 1]; b += 64) m = pmax[b] > m ? pmax[b] : m;
+  m = wave_max_u64(m);
+  if (threadIdx.x == 0) vmax[t] = m;
 }
 
 template <int W>
@@ -948,12 +975,18 @@ static int run_bobs(const uint64_t* rows, double* psi, int64_t R, const int32_t*
   double* part = (double*)p;
   const int S = 2 * N + 1;
   constexpr int TB = obs_tile<W>();
+  const int64_t nblk = (R + kObsTileB - 1) / kObsTileB + ntraj + 1;
+  p += a256((size_t)nblk * S * 8);
+  fin.pmax = (u64*)p;
   hipLaunchKernelGGL(k_seg_blocks, dim3(1), dim3(kSegThreads), 0, st, seg_off, ntraj, TB, boff);
   hipLaunchKernelGGL(k_bobs_partial<W>, dim3(seg_grid(R, ntraj, TB)), dim3(kBlock), 0, st,
                      (const u64*)rows, (double2*)psi, fin, seg_off, (const int64_t*)boff, ntraj,
                      lay->d_off, lay->d_width, N, traj_L, geo, part);
   hipLaunchKernelGGL(k_bobs_final, dim3(ntraj * S), dim3(64), 0, st, (const double*)part,
                      (const int64_t*)boff, S, out);
+  if (fin.vmax)
+    hipLaunchKernelGGL(k_bobs_vmax, dim3(ntraj), dim3(64), 0, st, (const u64*)fin.pmax,
+                       (const int64_t*)boff, fin.vmax);
   return launch_status();
 }
 
@@ -1502,9 +1535,12 @@ struct SeriesDev {
   int32_t slot_vib[PACES_MAX_TERMS];
   int32_t slot_stride;
   PACES_GPTR(void) xt[PACES_SERIES_MAX_TERMS];
+  PACES_GPTR(const u64) vmax;
 };
 static_assert(sizeof(SeriesDev) == sizeof(paces_batch_series_t), "descriptor layout");
 static_assert(offsetof(SeriesDev, xt) == offsetof(paces_batch_series_t, xt), "descriptor layout");
+static_assert(offsetof(SeriesDev, vmax) == offsetof(paces_batch_series_t, vmax),
+              "descriptor layout");
 static_assert(offsetof(SeriesDev, slots) == offsetof(paces_batch_series_t, slots),
               "descriptor layout");
 static_assert(offsetof(SeriesDev, slot_vib) == offsetof(paces_batch_series_t, slot_vib),
@@ -1542,6 +1578,24 @@ __global__ void __launch_bounds__(BS) k_bser_begin(const paces_batch_series_t* _
                                      gen((PACES_GPTR(double2))d->res),
                                      gen((PACES_GPTR(V))d->x[2]), gen(d->seg_off), ser_boff(d),
                                      ntraj, ser_part(d, ntraj));
+}
+
+// the begin's state from the largest key of each trajectory's v (vmax), when x[2] already holds
+// v as complex64: what btay_reduce_body's fin pass sets from the begin's maxima
+__global__ void __launch_bounds__(256) k_bser_init(const paces_batch_series_t* __restrict__ desc,
+                                                   int ntraj) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntraj) return;
+  const SeriesDev* d = (const SeriesDev*)desc;
+  paces_batch_taylor_t* ts = gen(d->ts);
+  const int32_t* seg = gen(d->seg_off);
+  const double m0 = sqrt(__longlong_as_double((long long)gen(d->vmax)[t]));
+  ts[t].c1 = m0;
+  ts[t].rn = m0;
+  ts[t].c1_plus_c2 = 0.0;
+  ts[t].ratio = 0.0;
+  ts[t].done = (seg[t + 1] == seg[t]) ? 1 : 0;
+  ts[t].j_ind = 0;
 }
 
 __global__ void __launch_bounds__(64) k_bser_reduce(const paces_batch_series_t* __restrict__ desc,
@@ -1796,8 +1850,8 @@ int paces_holstein_observables_batched_finish(uint64_t const* rows, double* psi,
                                               const void* const* terms, int32_t nterms,
                                               int32_t k0, int32_t mixed,
                                               const paces_batch_taylor_t* ts, uint64_t* keys,
-                                              double* out, void* ws, size_t ws_bytes,
-                                              void* stream) {
+                                              uint64_t* vmax, double* out, void* ws,
+                                              size_t ws_bytes, void* stream) {
   if (!lay || !h || !h->batched || !seg_off || !out || !ws || R < 0) return PACES_E_INVALID;
   if (k0 < 0 || (k0 > 0 && !mixed)) return PACES_E_INVALID;
   if (R > 0 && (!rows || !psi)) return PACES_E_INVALID;
@@ -1810,6 +1864,7 @@ int paces_holstein_observables_batched_finish(uint64_t const* rows, double* psi,
   fin.mixed = mixed ? 1 : 0;
   fin.ts = ts;
   fin.keys = (u64*)keys;
+  fin.vmax = (u64*)vmax;
   PACES_DISPATCH_W(lay->W, run_bobs, rows, psi, R, seg_off, h->ntraj, lay, h->nchain,
                    h->d_traj_L, h->geo_dim, out, ws, (hipStream_t)stream, fin);
 }
@@ -1860,7 +1915,7 @@ int paces_batch_taylor_begin_tuned(int64_t R, const double* v, double* res, void
 size_t paces_batch_series_desc_bytes(void) { return sizeof(paces_batch_series_t); }
 
 int paces_batch_series_launch(const paces_batch_series_t* desc, int64_t R_cap, int32_t ntraj,
-                              int32_t mixed, int32_t nslots, int32_t nterms,
+                              int32_t mixed, int32_t nslots, int32_t nterms, int32_t prebegun,
                               const paces_launch_cfg_t* cfg, void* stream) {
   if (!desc || R_cap < 0 || ntraj < 1 || nterms < 2 || nterms > PACES_SERIES_MAX_TERMS ||
       !cfg_ok(cfg) || (nslots != 0 && nslots != 4 && nslots != 8))
@@ -1871,14 +1926,18 @@ int paces_batch_series_launch(const paces_batch_series_t* desc, int64_t R_cap, i
   const unsigned gs = seg_grid(R_cap, ntraj, B * RR);
   const unsigned gst = xcd_grid(gs);  // the terms' grid
   hipLaunchKernelGGL(k_bser_segblocks, dim3(1), dim3(kSegThreads), 0, st, desc, ntraj, B * RR);
+  if (prebegun) {
+    hipLaunchKernelGGL(k_bser_init, dim3((ntraj + 255) / 256), dim3(256), 0, st, desc, ntraj);
+  } else {
 #define SB_LAUNCH(BB, QQ)                                                                      \
   if (mixed)                                                                                   \
     hipLaunchKernelGGL((k_bser_begin<float, BB, QQ>), dim3(gs), dim3(BB), 0, st, desc, ntraj); \
   else                                                                                         \
     hipLaunchKernelGGL((k_bser_begin<double, BB, QQ>), dim3(gs), dim3(BB), 0, st, desc, ntraj)
-  PACES_CFG_SWITCH(B, RR, SB_LAUNCH)
+    PACES_CFG_SWITCH(B, RR, SB_LAUNCH)
 #undef SB_LAUNCH
-  hipLaunchKernelGGL(k_bser_reduce, dim3(ntraj), dim3(64), 0, st, desc, ntraj, 1, 0, 0);
+    hipLaunchKernelGGL(k_bser_reduce, dim3(ntraj), dim3(64), 0, st, desc, ntraj, 1, 0, 0);
+  }
   for (int j = 0; j < nterms; ++j) {
 #define ST_LAUNCH_NS(BB, QQ, NS)                                                               \
   if (mixed)                                                                                   \

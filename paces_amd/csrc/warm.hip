@@ -331,7 +331,8 @@ __global__ void __launch_bounds__(kBlock) k_warm_emit(
     int64_t R, const uint8_t* __restrict__ flags, int be_mask, const int32_t* __restrict__ NP,
     const uint32_t* __restrict__ slots, int64_t stride_old, int nslots, uint32_t colmask,
     u64* __restrict__ out_rows, double2* __restrict__ out_psi, D* __restrict__ out_diag,
-    uint8_t* __restrict__ out_be, uint32_t* __restrict__ out_slots, int64_t stride_new) {
+    uint8_t* __restrict__ out_be, uint32_t* __restrict__ out_slots, int64_t stride_new,
+    float2* __restrict__ out_x0) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= R) return;
   const int f = flags[i] & 0x7f;
@@ -355,6 +356,7 @@ __global__ void __launch_bounds__(kBlock) k_warm_emit(
   }
   store_row<W>(out_rows, p, x);
   out_psi[p] = v;
+  if (out_x0) out_x0[p] = make_float2((float)v.x, (float)v.y);  // the series' term input
   out_diag[p] = d;
   out_be[p] = (f & be_mask) ? 1 : 0;
 #pragma unroll
@@ -375,12 +377,14 @@ __global__ void __launch_bounds__(kBlock) k_warm_emit_new(
     const u64* __restrict__ Y, int nY, const int32_t* __restrict__ ypos,
     const D* __restrict__ ydiag, const uint8_t* __restrict__ ybe, int nslots,
     u64* __restrict__ out_rows, double2* __restrict__ out_psi, D* __restrict__ out_diag,
-    uint8_t* __restrict__ out_be, uint32_t* __restrict__ out_slots, int64_t stride_new) {
+    uint8_t* __restrict__ out_be, uint32_t* __restrict__ out_slots, int64_t stride_new,
+    float2* __restrict__ out_x0) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nY) return;
   const int64_t p = ypos[j];
   store_row<W>(out_rows, p, load_row<W>(Y, j));
   out_psi[p] = make_double2(0.0, 0.0);
+  if (out_x0) out_x0[p] = make_float2(0.0f, 0.0f);
   out_diag[p] = ydiag[j];
   out_be[p] = ybe[j];
   for (int m = 0; m < nslots; ++m) out_slots[(int64_t)m * stride_new + p] = PACES_SLOT_EMPTY;
@@ -400,18 +404,20 @@ static int run_emit(const uint64_t* rows, const double* psi, const void* diag, i
                     int64_t R, const uint8_t* flags, int be_mask, const int32_t* NP,
                     const uint32_t* slots, int64_t stride_old, int nslots, uint32_t colmask,
                     uint64_t* out_rows, double* out_psi, void* out_diag, uint8_t* out_be,
-                    uint32_t* out_slots, int64_t stride_new, hipStream_t st) {
+                    uint32_t* out_slots, int64_t stride_new, void* out_x0, hipStream_t st) {
   if (R == 0) return PACES_OK;
   if (diag_f32)
     hipLaunchKernelGGL((k_warm_emit<W, float>), dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
                        (const u64*)rows, (const double2*)psi, (const float*)diag, R, flags,
                        be_mask, NP, slots, stride_old, nslots, colmask, (u64*)out_rows,
-                       (double2*)out_psi, (float*)out_diag, out_be, out_slots, stride_new);
+                       (double2*)out_psi, (float*)out_diag, out_be, out_slots, stride_new,
+                       (float2*)out_x0);
   else
     hipLaunchKernelGGL((k_warm_emit<W, double>), dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
                        (const u64*)rows, (const double2*)psi, (const double*)diag, R, flags,
                        be_mask, NP, slots, stride_old, nslots, colmask, (u64*)out_rows,
-                       (double2*)out_psi, (double*)out_diag, out_be, out_slots, stride_new);
+                       (double2*)out_psi, (double*)out_diag, out_be, out_slots, stride_new,
+                       (float2*)out_x0);
   return launch_status();
 }
 
@@ -419,18 +425,18 @@ template <int W>
 static int run_emit_new(const uint64_t* Y, int nY, const int32_t* ypos, const void* ydiag,
                         int diag_f32, const uint8_t* ybe, int nslots, uint64_t* out_rows,
                         double* out_psi, void* out_diag, uint8_t* out_be, uint32_t* out_slots,
-                        int64_t stride_new, hipStream_t st) {
+                        int64_t stride_new, void* out_x0, hipStream_t st) {
   if (nY == 0) return PACES_OK;
   if (diag_f32)
     hipLaunchKernelGGL((k_warm_emit_new<W, float>), dim3(grid_for(nY, kBlock)), dim3(kBlock), 0,
                        st, (const u64*)Y, nY, ypos, (const float*)ydiag, ybe, nslots,
                        (u64*)out_rows, (double2*)out_psi, (float*)out_diag, out_be, out_slots,
-                       stride_new);
+                       stride_new, (float2*)out_x0);
   else
     hipLaunchKernelGGL((k_warm_emit_new<W, double>), dim3(grid_for(nY, kBlock)), dim3(kBlock), 0,
                        st, (const u64*)Y, nY, ypos, (const double*)ydiag, ybe, nslots,
                        (u64*)out_rows, (double2*)out_psi, (double*)out_diag, out_be, out_slots,
-                       stride_new);
+                       stride_new, (float2*)out_x0);
   return launch_status();
 }
 
@@ -577,7 +583,8 @@ int paces_warm_emit(const uint64_t* rows, int32_t W, const double* psi, const vo
                     int32_t diag_f32, int64_t R, const uint8_t* flags, int32_t be_mask,
                     const int32_t* NP, const uint32_t* slots, int64_t stride_old, int32_t nslots,
                     int32_t col_bits, uint64_t* out_rows, double* out_psi, void* out_diag,
-                    uint8_t* out_be, uint32_t* out_slots, int64_t stride_new, void* stream) {
+                    uint8_t* out_be, uint32_t* out_slots, int64_t stride_new, void* out_x0,
+                    void* stream) {
   uint32_t cm;
   if (R < 0 || !colmask_of(col_bits, &cm) || nslots < 1 || nslots > PACES_MAX_SLOTS ||
       stride_old < R || stride_new < 0)
@@ -587,19 +594,19 @@ int paces_warm_emit(const uint64_t* rows, int32_t W, const double* psi, const vo
     return PACES_E_INVALID;
   PACES_DISPATCH_W(W, run_emit, rows, psi, diag, diag_f32, R, flags, (int)be_mask, NP, slots,
                    stride_old, (int)nslots, cm, out_rows, out_psi, out_diag, out_be, out_slots,
-                   stride_new, (hipStream_t)stream);
+                   stride_new, out_x0, (hipStream_t)stream);
 }
 
 int paces_warm_emit_new(const uint64_t* Y, int32_t W, int64_t nY, const int32_t* ypos,
                         const void* ydiag, int32_t diag_f32, const uint8_t* ybe, int32_t nslots,
                         uint64_t* out_rows, double* out_psi, void* out_diag, uint8_t* out_be,
-                        uint32_t* out_slots, int64_t stride_new, void* stream) {
+                        uint32_t* out_slots, int64_t stride_new, void* out_x0, void* stream) {
   if (nY < 0 || nY >= 2147483647LL || nslots < 1 || nslots > PACES_MAX_SLOTS) return PACES_E_INVALID;
   if (nY > 0 && (!Y || !ypos || !ydiag || !ybe || !out_rows || !out_psi || !out_diag ||
                  !out_be || !out_slots))
     return PACES_E_INVALID;
   PACES_DISPATCH_W(W, run_emit_new, Y, (int)nY, ypos, ydiag, diag_f32, ybe, (int)nslots,
-                   out_rows, out_psi, out_diag, out_be, out_slots, stride_new,
+                   out_rows, out_psi, out_diag, out_be, out_slots, stride_new, out_x0,
                    (hipStream_t)stream);
 }
 

@@ -1172,12 +1172,13 @@ class BatchedHamiltonian(Hamiltonian):
         return seg
 
     def site_observables_batched_async(self, rows, vector, seg_off, out=None, finish=None,
-                                       keys=None):
+                                       keys=None, vmax=None):
         """Launches the per-trajectory observables into out (device float64, ntraj * (2N + 1):
         per trajectory n_exc[N], n_pho[N], sum|psi|^2) and returns out; no host sync. finish =
         (term buffers, series states, mixed, k0): the series' finish is first done on vector in
         place (paces_holstein_observables_batched_finish: mixed, terms k0 .. j added to v; FP64,
-        the last even terms added); keys (int64, R) receives the rows' select keys."""
+        the last even terms added); keys (int64, R) receives the rows' select keys and vmax
+        (int64, ntraj) each trajectory's largest one."""
         R = rows.shape[0]
         N = self.nchain
         if out is None:
@@ -1190,7 +1191,8 @@ class BatchedHamiltonian(Hamiltonian):
             ptr(rows) if R else None, ptr(torch.view_as_real(vector)) if R else None, R,
             ptr(seg_off), C.byref(self._layout), C.byref(self._hs), tp, len(terms), k0,
             1 if mixed else 0, ptr(ts) if ts is not None else None,
-            ptr(keys) if keys is not None and R else None, ptr(out), wsp, wsn, stream()),
+            ptr(keys) if keys is not None and R else None,
+            ptr(vmax) if vmax is not None else None, ptr(out), wsp, wsn, stream()),
             "holstein_observables_batched_finish")
         return out
 
@@ -1250,6 +1252,14 @@ class BatchedTimeEvolution:
             raise ValueError("maxstates needs one positive value per trajectory.")
         self.maxstates = ms
         self._d_ms = torch.tensor(ms, dtype=torch.int64, device=self.device)
+        # the largest |psi|^2 of each trajectory (bits of a double), from the observables pass;
+        # it holds across an adaptation: the row attaining it is selected (every maxstates is at
+        # least 1) and keeps its amplitude. With the complex64 copy of the amplitudes that the
+        # incremental emit writes (_x0), the graph's series skips its begin pass over them.
+        self._vmax = torch.zeros(len(ms), dtype=torch.int64, device=self.device)
+        self._vmax_ok = False
+        self._x0 = None
+        self._prebegin = bool(numpy.min(ms) >= 1)
         self.enlarge_steps = int(enlarge_steps)
         if precision not in ("mixed", "fp64"):
             raise ValueError("precision must be 'mixed' or 'fp64'.")
@@ -1331,6 +1341,8 @@ class BatchedTimeEvolution:
         self.vector = torch.ones(h.ntraj, dtype=torch.complex128, device=self.device)
         self._keys = None
         self._finish = None
+        self._vmax_ok = False
+        self._x0 = None
         self._set_segments(h.segments(self.whoami))
 
     def _select(self):
@@ -1350,6 +1362,7 @@ class BatchedTimeEvolution:
         return out[:n]
 
     def adapt(self):
+        self._x0 = None
         if self.incremental and self._prev_be is not None and self.numstates > 0:
             done = self._adapt_incremental()
             if done is not None:
@@ -1556,16 +1569,20 @@ class BatchedTimeEvolution:
         diag1 = torch.empty(R1, dtype=diag0.dtype, device=dev)
         be1 = torch.empty(R1, dtype=torch.uint8, device=dev)
         slots1 = torch.empty((lay.nslots, R1), dtype=torch.int32, device=dev)
+        # the mixed series' complex64 term input, written with the amplitudes
+        x0 = torch.empty(R1, dtype=torch.complex64, device=dev) if f32 else None
+        x0p = ptr(torch.view_as_real(x0)) if x0 is not None and R1 else None
         check(_lib.paces_warm_emit(ptr(rows0), W, ptr(torch.view_as_real(psi0)), ptr(diag0), f32,
                                    R0, ptr(flags), 3, ptr(NP), slots0, R0, lay.nslots,
                                    lay.col_bits, ptr(rows1), ptr(torch.view_as_real(psi1)),
-                                   ptr(diag1), ptr(be1), ptr(slots1), R1, stream()), "warm_emit")
+                                   ptr(diag1), ptr(be1), ptr(slots1), R1, x0p, stream()),
+              "warm_emit")
         if nY:
             ydiag = h.generate_mel_diag(Y, dtype=diag0.dtype)
             check(_lib.paces_warm_emit_new(ptr(Y), W, nY, ptr(ypos), ptr(ydiag), f32, ptr(ybe),
                                            lay.nslots, ptr(rows1),
                                            ptr(torch.view_as_real(psi1)), ptr(diag1), ptr(be1),
-                                           ptr(slots1), R1, stream()), "warm_emit_new")
+                                           ptr(slots1), R1, x0p, stream()), "warm_emit_new")
         y1pos = None
         if ny1:
             # the rows of y1 in the new basis (they are the first pass's new rows)
@@ -1601,6 +1618,7 @@ class BatchedTimeEvolution:
         self._prev_be = be1
         self._keys = None
         self._obs_last = None
+        self._x0 = x0
         self._set_segments(h.segments(rows1))
         self.last_adapt = ("incremental", n1, n2, nY)
         return {}
@@ -1692,7 +1710,8 @@ class BatchedTimeEvolution:
             d.coded = 0
         return diag, keep, call, fill
 
-    def _propagate_graph(self, delta_t, fill, mixed, res, bufs, tbuf, ts, hdr, diag, cfg, st):
+    def _propagate_graph(self, delta_t, fill, mixed, res, bufs, tbuf, ts, hdr, diag, cfg, st,
+                         prebegun=False):
         """Terms 0 .. n-1 by replaying a captured graph; returns n (the eager loop continues
         from there if some trajectory has not stopped). bufs: x[0..2] of the descriptor; tbuf(k):
         the mixed form's buffer of term k."""
@@ -1721,6 +1740,8 @@ class BatchedTimeEvolution:
         if mixed:
             for k in range(nterms):
                 d.xt[k] = ptr(torch.view_as_real(tbuf(k)))
+        if prebegun:
+            d.vmax = ptr(self._vmax)
         d.seg_off = ptr(self.seg_off)
         d.ts = ptr(ts)
         d.hdr = ptr(hdr)
@@ -1735,14 +1756,15 @@ class BatchedTimeEvolution:
                         non_blocking=True)
         shape = (cfg.block, cfg.rows_per_thread) if cfg is not None else None
         nslots = d.nslots if d.slots else 0  # the term kernel instance: CSR (0) or slots
-        key = (cap, h.ntraj, nterms, mixed, shape, nslots)
+        key = (cap, h.ntraj, nterms, mixed, shape, nslots, prebegun)
         g = pool.graphs.get(key)
         if g is None:
             g = torch.cuda.CUDAGraph()
             cref = C.byref(cfg) if cfg is not None else None
             with torch.cuda.graph(g):
                 check(_lib.paces_batch_series_launch(ptr(pool.desc), cap, h.ntraj,
-                                                     1 if mixed else 0, nslots, nterms, cref,
+                                                     1 if mixed else 0, nslots, nterms,
+                                                     1 if prebegun else 0, cref,
                                                      stream()), "batch_series_launch")
             pool.graphs[key] = g
         g.replay()
@@ -1769,9 +1791,19 @@ class BatchedTimeEvolution:
                 tb.append(torch.empty(R, dtype=vdt, device=self.device))
             return tb[k]
 
+        # the graph's series starts from the emit's complex64 copy of v and the observables'
+        # per-trajectory maxima when both describe the current v (no begin pass)
+        prebegun = (self.graph and mixed and self._prebegin and self._vmax_ok and
+                    self._x0 is not None and self._x0.shape[0] == R)
+        # the series changes v: the maxima and the copy describe it again only after the
+        # observables pass (launched below with fuse_finish) and the next incremental emit
+        self._vmax_ok = False
+        x0_keep = self._x0
+        self._x0 = None
         if mixed:
             res = self.vector  # the finish forms v + t_0 + ... + t_j in place
-            bufs = [None, None, torch.empty(R, dtype=vdt, device=self.device)]
+            bufs = [None, None, x0_keep if prebegun else
+                    torch.empty(R, dtype=vdt, device=self.device)]
             m_star = min(self.m_star, _hip.SERIES_MAX_TERMS)
         else:
             res = torch.empty_like(self.vector)
@@ -1790,7 +1822,7 @@ class BatchedTimeEvolution:
         spec = 0  # terms the speculative observables pass finished
         if self.graph:
             j = self._propagate_graph(delta_t, fill, mixed, res, bufs, tbuf, ts, hdr, diag, cfg,
-                                      st)
+                                      st, prebegun)
             wsp, wsn = self._tws.get(0)  # the graph's workspace (same buffer, grown if needed)
             x_in = tb[j - 1] if mixed else bufs[(j - 1) % 2]
             if fuse_finish and mixed:
@@ -1916,7 +1948,8 @@ class BatchedTimeEvolution:
             self._obs_host = torch.empty(n, dtype=torch.float64, device="cpu", pin_memory=True)
         keys = torch.empty(self.numstates, dtype=torch.int64, device=self.device)
         h.site_observables_batched_async(self.whoami, self.vector, self.seg_off, self._obs_dev,
-                                         finish=self._finish, keys=keys)
+                                         finish=self._finish, keys=keys, vmax=self._vmax)
+        self._vmax_ok = True
         self._finish = None
         self._keys = keys
         self._obs_host.copy_(self._obs_dev, non_blocking=True)
