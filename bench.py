@@ -178,7 +178,7 @@ def btay_term_bytes(R, mats, mixed, form="explicit"):
 # WRITE_SIZE counter passes over the roofline loop below with the shape fixed (--term-shape),
 # each scaled by the counter's reading for a 1 GiB device copy
 # (profiles/k_btay_term_traj256_hbm_traffic.txt). Other forms and shapes report null.
-BTAY_TRAFFIC_BYTES_PER_ROW = {("mixed", (256, 4), "slots"): 53.13}
+BTAY_TRAFFIC_BYTES_PER_ROW = {("mixed", (256, 4), "slots"): 42.27}
 
 
 def measure_batched_term(bte, reps=20):
