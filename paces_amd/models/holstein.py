@@ -1425,7 +1425,7 @@ class BatchedTimeEvolution:
         U, cpos = self._union([(imgs, offs[k], offs[k + 1] - offs[k])
                                for k in range(len(offs) - 1)])
         nU = U.shape[0]
-        idx, hit, _ = engine.lower_bound(rows0, U, want_hit=True, q_sorted=True, ws=self._ws)
+        idx, hit, _ = engine.lower_bound(rows0, U, want_hit=True, q_sorted=False, ws=self._ws)
         miss = torch.empty(nU, dtype=torch.uint8, device=self.device)
         check(_lib.paces_mark_hits(ptr(idx), ptr(hit), nU, ptr(flags), bit, ptr(miss),
                                    stream()), "mark_hits")
