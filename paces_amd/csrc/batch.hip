@@ -213,13 +213,36 @@ __global__ void __launch_bounds__(kBlock) k_bsel_hist(const u64* __restrict__ ke
   }
 }
 
+// inclusive prefix sum of v over the 256 threads of a workgroup (wave scans, then the waves'
+// totals through LDS); *total receives the sum over all threads. Every thread must call it.
+__device__ __forceinline__ long long blk256_incl(long long v, long long* total) {
+  __shared__ long long wt[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  long long inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const long long u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += u;
+  }
+  if (lane == 63) wt[wave] = inc;
+  __syncthreads();
+  long long before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    before += w < wave ? wt[w] : 0;
+    all += wt[w];
+  }
+  __syncthreads();
+  *total = all;
+  return before + inc;
+}
+
 // the digit holding the krem-th largest remaining key: per thread 8 consecutive bins, a suffix
 // scan over the threads, then the thread whose bins cross krem finds the digit
 __global__ void __launch_bounds__(256) k_bsel_pick(SegSel* __restrict__ st,
                                                    unsigned* __restrict__ hist, int shift,
                                                    int width) {
   constexpr int kPer = kSelBins / 256;
-  __shared__ long long suf[257];
   const int t = blockIdx.x, d0 = threadIdx.x * kPer;
   unsigned* h = hist + (int64_t)t * kSelBins;
   const bool skip = st[t].all || st[t].krem == 0;
@@ -230,16 +253,12 @@ __global__ void __launch_bounds__(256) k_bsel_pick(SegSel* __restrict__ st,
     c[j] = skip ? 0u : h[d0 + j];
     tot += c[j];
   }
-  suf[threadIdx.x] = tot;
-  if (threadIdx.x == 0) suf[256] = 0;
-  __syncthreads();
-  if (threadIdx.x == 0)
-    for (int u = 255; u >= 0; --u) suf[u] += suf[u + 1];
-  __syncthreads();
+  long long all;
+  const long long incl = blk256_incl(tot, &all);
   if (!skip) {
     const long long krem = st[t].krem;
-    // keys in bins above this thread's: suf[tid + 1]; walk this thread's bins from the top
-    long long above = suf[threadIdx.x + 1];
+    // keys in bins above this thread's; walk this thread's bins from the top
+    long long above = all - incl;
     int pick = -1;
     long long above_pick = 0;
 #pragma unroll
@@ -310,7 +329,6 @@ __global__ void __launch_bounds__(kBlock) k_bsel_cand(const u64* __restrict__ ke
 __device__ __forceinline__ int blk_pick_bin(const unsigned* h, long long r, bool desc,
                                             long long* before) {
   constexpr int kPer = kSelBins / 256;
-  __shared__ long long acc[257];
   __shared__ int s_pick;
   __shared__ long long s_before;
   const int d0 = threadIdx.x * kPer;
@@ -321,27 +339,11 @@ __device__ __forceinline__ int blk_pick_bin(const unsigned* h, long long r, bool
     c[j] = h[d0 + j];
     tot += c[j];
   }
-  acc[threadIdx.x] = tot;
-  if (threadIdx.x == 0) {
-    acc[256] = 0;
-    s_pick = -1;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (desc) {  // acc[u] = keys in the bins of threads u .. 255
-      for (int u = 255; u >= 0; --u) acc[u] += acc[u + 1];
-    } else {  // acc[u] = keys in the bins of threads 0 .. u - 1
-      long long run = 0;
-      for (int u = 0; u < 256; ++u) {
-        const long long v = acc[u];
-        acc[u] = run;
-        run += v;
-      }
-    }
-  }
-  __syncthreads();
+  if (threadIdx.x == 0) s_pick = -1;
+  long long all;
+  const long long incl = blk256_incl(tot, &all);  // (its barriers publish s_pick's reset)
   if (desc) {
-    long long above = acc[threadIdx.x + 1];
+    long long above = all - incl;  // keys in the bins of the threads after this one
 #pragma unroll
     for (int j = kPer - 1; j >= 0; --j) {
       const long long with = above + c[j];
@@ -352,7 +354,7 @@ __device__ __forceinline__ int blk_pick_bin(const unsigned* h, long long r, bool
       above = with;
     }
   } else {
-    long long below = acc[threadIdx.x];
+    long long below = incl - tot;  // keys in the bins of the threads before this one
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
       const long long with = below + c[j];
