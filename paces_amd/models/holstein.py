@@ -1471,7 +1471,8 @@ class BatchedTimeEvolution:
         slots0 = ptr(sm0.slots)
 
         def mark(src, skip, bit, expl, counter):
-            check(_lib.paces_slots_mark(slots0, R0, lay.nslots, lay.col_bits, R0, ptr(flags), src,
+            check(_lib.paces_slots_mark(slots0, lay.stride, lay.nslots, lay.col_bits, R0,
+                                        ptr(flags), src,
                                         skip, bit, ptr(expl), counter, stream()), "slots_mark")
 
         # bit 0 selected; bit 1 the first expansion (neighbours of selected rows); bit 2 the
@@ -1559,30 +1560,34 @@ class BatchedTimeEvolution:
         check(_lib.paces_warm_positions(ptr(flags), R0, ptr(ins) if nY else None, nY, ptr(NP),
                                         ptr(ypos), ptr(tot), wsp, wsn, stream()),
               "warm_positions")
-        R1 = int(tot.item())
-        lay1 = h.slot_layout(R1)
-        if lay1 is None or lay1.col_bits != lay.col_bits or lay1.nslots != lay.nslots:
-            raise RuntimeError("the slot layout changed between steps")
+        # The new basis has at most Rb rows: the outputs are allocated for Rb, the emit and the
+        # explicit pairs are launched, and the new size is read from pinned memory while they
+        # run (the slot matrix keeps Rb as its stride).
+        Rb = R0 + nY
+        tot_h = torch.empty(1, dtype=torch.int32, device="cpu", pin_memory=True)
+        tot_h.copy_(tot, non_blocking=True)
+        tot_ev = torch.cuda.Event()
+        tot_ev.record()
         f32 = 1 if diag0.dtype == torch.float32 else 0
-        rows1 = torch.empty((R1, W), dtype=rows0.dtype, device=dev)
-        psi1 = torch.empty(R1, dtype=torch.complex128, device=dev)
-        diag1 = torch.empty(R1, dtype=diag0.dtype, device=dev)
-        be1 = torch.empty(R1, dtype=torch.uint8, device=dev)
-        slots1 = torch.empty((lay.nslots, R1), dtype=torch.int32, device=dev)
+        rows1 = torch.empty((max(Rb, 1), W), dtype=rows0.dtype, device=dev)
+        psi1 = torch.empty(max(Rb, 1), dtype=torch.complex128, device=dev)
+        diag1 = torch.empty(max(Rb, 1), dtype=diag0.dtype, device=dev)
+        be1 = torch.empty(max(Rb, 1), dtype=torch.uint8, device=dev)
+        slots1 = torch.empty((lay.nslots, max(Rb, 1)), dtype=torch.int32, device=dev)
         # the mixed series' complex64 term input, written with the amplitudes
-        x0 = torch.empty(R1, dtype=torch.complex64, device=dev) if f32 else None
-        x0p = ptr(torch.view_as_real(x0)) if x0 is not None and R1 else None
+        x0 = torch.empty(max(Rb, 1), dtype=torch.complex64, device=dev) if f32 else None
+        x0p = ptr(torch.view_as_real(x0)) if x0 is not None else None
         check(_lib.paces_warm_emit(ptr(rows0), W, ptr(torch.view_as_real(psi0)), ptr(diag0), f32,
-                                   R0, ptr(flags), 3, ptr(NP), slots0, R0, lay.nslots,
+                                   R0, ptr(flags), 3, ptr(NP), slots0, lay.stride, lay.nslots,
                                    lay.col_bits, ptr(rows1), ptr(torch.view_as_real(psi1)),
-                                   ptr(diag1), ptr(be1), ptr(slots1), R1, x0p, stream()),
+                                   ptr(diag1), ptr(be1), ptr(slots1), Rb, x0p, stream()),
               "warm_emit")
         if nY:
             ydiag = h.generate_mel_diag(Y, dtype=diag0.dtype)
             check(_lib.paces_warm_emit_new(ptr(Y), W, nY, ptr(ypos), ptr(ydiag), f32, ptr(ybe),
                                            lay.nslots, ptr(rows1),
                                            ptr(torch.view_as_real(psi1)), ptr(diag1), ptr(be1),
-                                           ptr(slots1), R1, x0p, stream()), "warm_emit_new")
+                                           ptr(slots1), Rb, x0p, stream()), "warm_emit_new")
         y1pos = None
         if ny1:
             # the rows of y1 in the new basis (they are the first pass's new rows)
@@ -1607,11 +1612,19 @@ class BatchedTimeEvolution:
                 yi = yidx[id(pr)]
                 self._move(up, ypos, yi, umiss, ym)
             run_off = (C.c_int64 * len(offs))(*offs)
-            check(_lib.paces_holstein_slots_place_explicit(R1, run_off, ptr(sp), ptr(up),
+            check(_lib.paces_holstein_slots_place_explicit(Rb, run_off, ptr(sp), ptr(up),
                                                            ptr(cpos), ptr(src), ptr(nup),
                                                            C.byref(h._hs), ptr(slots1), stream()),
                   "holstein_slots_place_explicit")
-        sm1 = engine.SlotMatrix(slots1, lay1, (R1, R1))
+        tot_ev.synchronize()
+        R1 = int(tot_h[0])
+        lay1 = h.slot_layout(R1)
+        if lay1 is None or lay1.col_bits != lay.col_bits or lay1.nslots != lay.nslots:
+            raise RuntimeError("the slot layout changed between steps")
+        lay1.stride = Rb
+        rows1, psi1, diag1, be1 = rows1[:R1], psi1[:R1], diag1[:R1], be1[:R1]
+        x0 = x0[:R1] if x0 is not None else None
+        sm1 = engine.SlotMatrix(slots1[:, :R1], lay1, (R1, R1))
         self.whoami, self.vector, self.diag_vals = rows1, psi1, diag1
         self.sparse_mats_dict = {name: sm1.term(t, lambda m, rows=rows1: h._coded_values(m, rows))
                                  for t, name in enumerate(h.offdiag_terms)}
