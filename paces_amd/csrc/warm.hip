@@ -33,6 +33,28 @@ constexpr int kWTile = kBlock * kWRpt;  // rows per workgroup
 // stored in both rows), so each row reads its own slots and marks itself when one names a source
 // in the old B_e: every row's flag byte is written by its own thread only, and the bits the
 // others read (src_mask, skip_mask, bit 7) are not among those it writes.
+// kWarmXcdRun > 0: workgroup b of the emit takes block (k / C) * 8C + (b % 8) * C + k % C of
+// rows, k = b / 8, C = kWarmXcdRun: each XCD (workgroup b runs on XCD b % 8) takes runs of C
+// consecutive blocks, so blocks whose rows' neighbours overlap share an L2 (the grid is padded
+// to a multiple of 8C; 4% faster than round robin, no gain for the marks)
+#ifndef PACES_WARM_XCD_RUN
+#define PACES_WARM_XCD_RUN 64
+#endif
+constexpr int kWarmXcdRun = PACES_WARM_XCD_RUN;
+
+__device__ __forceinline__ int64_t warm_block() {
+  const int64_t b = blockIdx.x;
+  if (kWarmXcdRun == 0) return b;
+  const int64_t k = b >> 3;
+  return (k / kWarmXcdRun) * (8 * kWarmXcdRun) + (b & 7) * kWarmXcdRun + k % kWarmXcdRun;
+}
+
+static unsigned warm_grid(int64_t g) {
+  if (kWarmXcdRun == 0) return (unsigned)g;
+  const int64_t q = 8 * kWarmXcdRun;
+  return (unsigned)((g + q - 1) / q * q);
+}
+
 // A thread takes kMarkRpt rows (t, t + kBlock, ...): every slot of its rows is loaded, then every
 // named row's flags, then the rows are decided, so the dependent loads of all rows overlap.
 constexpr int kMarkRpt = 8;
@@ -333,7 +355,7 @@ __global__ void __launch_bounds__(kBlock) k_warm_emit(
     u64* __restrict__ out_rows, double2* __restrict__ out_psi, D* __restrict__ out_diag,
     uint8_t* __restrict__ out_be, uint32_t* __restrict__ out_slots, int64_t stride_new,
     float2* __restrict__ out_x0) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = warm_block() * blockDim.x + threadIdx.x;
   if (i >= R) return;
   const int f = flags[i] & 0x7f;
   if (!f) return;
@@ -407,13 +429,15 @@ static int run_emit(const uint64_t* rows, const double* psi, const void* diag, i
                     uint32_t* out_slots, int64_t stride_new, void* out_x0, hipStream_t st) {
   if (R == 0) return PACES_OK;
   if (diag_f32)
-    hipLaunchKernelGGL((k_warm_emit<W, float>), dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
+    hipLaunchKernelGGL((k_warm_emit<W, float>), dim3(warm_grid(grid_for(R, kBlock))),
+                       dim3(kBlock), 0, st,
                        (const u64*)rows, (const double2*)psi, (const float*)diag, R, flags,
                        be_mask, NP, slots, stride_old, nslots, colmask, (u64*)out_rows,
                        (double2*)out_psi, (float*)out_diag, out_be, out_slots, stride_new,
                        (float2*)out_x0);
   else
-    hipLaunchKernelGGL((k_warm_emit<W, double>), dim3(grid_for(R, kBlock)), dim3(kBlock), 0, st,
+    hipLaunchKernelGGL((k_warm_emit<W, double>), dim3(warm_grid(grid_for(R, kBlock))),
+                       dim3(kBlock), 0, st,
                        (const u64*)rows, (const double2*)psi, (const double*)diag, R, flags,
                        be_mask, NP, slots, stride_old, nslots, colmask, (u64*)out_rows,
                        (double2*)out_psi, (double*)out_diag, out_be, out_slots, stride_new,
